@@ -1,0 +1,223 @@
+/*
+ * vxp.h — C ABI of the MI355X step pipeline for zig-voxel-physics.
+ *
+ * Every entry point below replaces one entry point of the reference's
+ * `VoxelPhysicsSystem` / `VoxelCollider` API (Zig).  Signatures use plain
+ * pointers and sizes only; no device pointers or torch types cross this
+ * boundary.  Every call returns an int status (VXP_OK or a negative code);
+ * nothing throws across the ABI.
+ *
+ * A `vxp_system` holds one or more independent worlds (a batch).  The
+ * reference API maps onto world 0; the `_w` variants take a world index and
+ * drive the batched path (many worlds stepped per launch).
+ */
+#ifndef VXP_H
+#define VXP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (src/system.zig:487 MaxBodiesReached; OOM) ---------- */
+#define VXP_OK 0
+#define VXP_ERR_INVALID_HANDLE (-1)
+#define VXP_ERR_CAPACITY (-2)      /* error.MaxBodiesReached */
+#define VXP_ERR_OOM (-3)           /* allocator OOM / HBM budget */
+#define VXP_ERR_DEVICE (-4)        /* HIP runtime or kernel fault */
+#define VXP_ERR_ARGUMENT (-5)
+
+/* ---- body types (src/body.zig:29-48, enum(u2)) -------------------------- */
+#define VXP_BODY_STATIC 0
+#define VXP_BODY_KINEMATIC 1
+#define VXP_BODY_DYNAMIC 2
+
+/* ---- voxel byte: u3 type | u5 normal index << 3 (src/collision_voxel.zig:133-191) */
+#define VXP_VOXEL_EMPTY 0
+#define VXP_VOXEL_CORNER 1
+#define VXP_VOXEL_EDGE 2
+#define VXP_VOXEL_FACE 3
+#define VXP_VOXEL_INSIDE 4
+#define VXP_VOXEL(type, normal) ((uint8_t)(((type) & 7u) | (((normal) & 31u) << 3)))
+
+/* Solve order inside one step.  The reference visits manifolds in Zig
+ * AutoHashMap order (src/system.zig:687-702), which cannot be reproduced
+ * outside Zig's stdlib; both orders below are deterministic restatements. */
+#define VXP_SOLVE_ORDER_COLOR 0     /* per-colour Gauss-Seidel (GPU order)  */
+#define VXP_SOLVE_ORDER_CANONICAL 1 /* ascending pair key                   */
+
+/*
+ * Configuration: PhysicsConfig (src/system.zig:47-88) + SolverConfig
+ * (src/solver.zig:75-119) + NarrowPhaseConfig (src/narrow_phase.zig:1325-1343)
+ * field for field, plus the documented deviation knobs at the end.
+ * Fill with vxp_config_default() and override.
+ */
+typedef struct vxp_config {
+  /* PhysicsConfig */
+  float gravity[3];            /* (0,-9.81,0) */
+  float fixed_timestep;        /* 1/60 */
+  uint32_t max_substeps;       /* 8: frame catch-up cap */
+  uint32_t max_bodies;         /* 1024 per world */
+  float default_friction;      /* 0.6 */
+  float default_restitution;   /* 0.2 */
+  float default_density;       /* 1.0 (never read by the reference) */
+  uint8_t enable_sleeping;     /* 1 */
+  uint8_t enable_warm_starting;/* 1 */
+  uint8_t enable_islands;      /* 1: island labels + island stats each step */
+  uint8_t enable_coloring;     /* 0: reference GraphColoring artifact (built
+                                  only with enable_threading in the reference) */
+  /* SolverConfig */
+  uint32_t substeps;           /* 4 */
+  uint32_t velocity_iterations;/* 1 */
+  uint32_t position_iterations;/* 2 */
+  float hertz;                 /* 30 */
+  float damping_ratio;         /* 1 */
+  float position_correction_rate; /* 0.2 */
+  float max_position_correction;  /* 0.04 */
+  float slop;                  /* 0.005 */
+  float warm_start_scale;      /* 0.8 */
+  uint8_t disable_rotation;    /* 0 */
+  uint8_t warm_starting;       /* 1 */
+  uint8_t enable_friction;     /* 1 */
+  uint8_t enable_restitution;  /* 1 */
+  /* NarrowPhaseConfig */
+  uint8_t use_brick_sat;       /* 1 */
+  uint8_t use_brick_pair_cache;/* 1 */
+  uint8_t pad0[2];
+  uint32_t brick_pair_cache_size; /* 1024 */
+  /* deviation knobs */
+  uint32_t pair_cap;           /* 0 = unlimited; 512 = the reference's
+                                  filtered-pair array (src/system.zig:594) */
+  uint32_t solve_order;        /* VXP_SOLVE_ORDER_* */
+  uint32_t cpu_threads;        /* CPU restatement only: narrow-phase workers */
+} vxp_config;
+
+/* Per-step statistics: PhysicsStats (src/system.zig:91-155) counters, for
+ * the last executed step of the last update() call, plus build counters. */
+typedef struct vxp_stats {
+  uint32_t total_bodies, dynamic_bodies, static_bodies, sleeping_bodies, active_bodies;
+  uint32_t broad_phase_pairs, narrow_phase_tested, narrow_phase_colliding;
+  uint32_t contacts_generated, voxels_checked, pairs_skipped_sleeping;
+  uint32_t cache_hits, cache_misses, manifolds_skipped_sleeping;
+  uint32_t island_count, largest_island_size, substeps_this_frame;
+  uint32_t manifolds, solver_manifolds, colors, position_iterations_run;
+  uint32_t steps_executed;
+  float max_penetration;
+  float accumulated_time;
+} vxp_stats;
+
+/* ContactPoint (src/narrow_phase.zig:336-393) */
+typedef struct vxp_contact {
+  float point[3];
+  float normal[3];
+  float penetration;
+  float local_anchor_a[3];
+  float local_anchor_b[3];
+  float normal_impulse;
+  float tangent_impulse[2];
+  float normal_mass;
+  float tangent_mass[2];
+  uint32_t pad;
+  uint64_t feature_id;
+} vxp_contact;
+
+/* ContactManifold (src/narrow_phase.zig:396-502) */
+typedef struct vxp_manifold {
+  uint32_t body_a_id, body_b_id;
+  float friction, restitution;
+  uint32_t contact_count;
+  uint32_t color;              /* solver colour this step (0xFFFFFFFF: not solved) */
+  vxp_contact contacts[4];
+} vxp_manifold;
+
+/* BodyHandle (src/system.zig:158-167) */
+typedef struct vxp_handle {
+  uint32_t index, generation;
+} vxp_handle;
+
+/* Body state, as the reference's VoxelBody hot fields (src/body.zig:177-203). */
+typedef struct vxp_body_state {
+  float position[3];
+  float rotation[4];           /* x, y, z, w */
+  float linear_velocity[3];
+  float angular_velocity[3];
+  float sleep_timer;
+  uint32_t id;
+  uint8_t body_type;
+  uint8_t is_sleeping;
+  uint8_t active;
+  uint8_t pad;
+} vxp_body_state;
+
+typedef struct vxp_system vxp_system;
+
+/* Defaults of PhysicsConfig{} / SolverConfig{} / NarrowPhaseConfig{}. */
+int vxp_config_default(vxp_config* cfg);
+
+/* VoxelPhysicsSystem.init / deinit (src/system.zig:213-291).
+ * n_worlds >= 1 independent worlds share the device and every launch. */
+int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out);
+int vxp_system_destroy(vxp_system* sys);
+
+/* VoxelCollider.init + setCollisionVoxel... + recomputeMetadata
+ * (src/collider.zig:168-179,267-273,283-333).  `brick_pos` holds 3*n_bricks
+ * brick coordinates, `voxels` 512*n_bricks voxel bytes indexed x+8y+64z.
+ * The collider is copied to the device; the caller may free its arrays.
+ * A collider may be shared by any number of bodies of any world. */
+int vxp_collider_create(vxp_system* sys, float voxel_size, uint32_t n_bricks,
+                        const int32_t* brick_pos, const uint8_t* voxels, uint32_t* out_collider);
+
+/* createDynamicBody / createStaticBody / createKinematicBody
+ * (src/system.zig:294-322,354-377,380-402) in world 0 and in world `w`. */
+int vxp_body_create(vxp_system* sys, uint32_t body_type, const float pos[3], const float rot[4],
+                    uint32_t collider, float density, vxp_handle* out);
+int vxp_body_create_w(vxp_system* sys, uint32_t world, uint32_t body_type, const float pos[3],
+                      const float rot[4], uint32_t collider, float density, vxp_handle* out);
+/* destroyBody (src/system.zig:405-433) */
+int vxp_body_destroy_w(vxp_system* sys, uint32_t world, vxp_handle h);
+
+/* update(dt) (src/system.zig:508-534) on every world; step(dt) (src/system.zig:537-555). */
+int vxp_system_update(vxp_system* sys, float dt);
+int vxp_system_step(vxp_system* sys, float dt);
+/* Batched: `frames` consecutive update(dt) calls on every world, one device
+ * launch sequence per phase for all worlds together. */
+int vxp_system_update_frames(vxp_system* sys, float dt, uint32_t frames);
+
+/* getBody / accessors (src/system.zig:436-445,885-1008): copy state out / in.
+ * `out` receives max_bodies entries (slot order) of world `w`. */
+int vxp_bodies_get_w(vxp_system* sys, uint32_t world, vxp_body_state* out, uint32_t capacity,
+                     uint32_t* count);
+/* setPosition/setRotation/setLinearVelocity/setAngularVelocity + wakeUp on one body. */
+int vxp_body_set_state_w(vxp_system* sys, uint32_t world, vxp_handle h, const float pos[3],
+                         const float rot[4], const float lin_vel[3], const float ang_vel[3]);
+
+/* getStats (src/system.zig:1024-1026) of world `w`. */
+int vxp_stats_get_w(vxp_system* sys, uint32_t world, vxp_stats* out);
+
+/* getContactManager (src/system.zig:1029-1031): the current manifolds of
+ * world `w` in ascending pair-key order. */
+int vxp_manifolds_get_w(vxp_system* sys, uint32_t world, vxp_manifold* out, uint32_t capacity,
+                        uint32_t* count);
+/* BroadPhase.findOverlappingPairs pair set of the last step, ascending
+ * (src/broad_phase.zig:606-619); each entry (a<<32)|b with a < b. */
+int vxp_pairs_get_w(vxp_system* sys, uint32_t world, uint64_t* out, uint32_t capacity,
+                    uint32_t* count);
+/* Island labels of the last step (src/islands.zig:318-441): for each
+ * manifold endpoint body id (ascending), the minimum body id of its island. */
+int vxp_islands_get_w(vxp_system* sys, uint32_t world, uint32_t* body_ids, uint32_t* labels,
+                      uint32_t capacity, uint32_t* count);
+/* GraphColoring.build colours (src/islands.zig:668-736) of the current
+ * manifolds in pair-key order (needs enable_coloring). */
+int vxp_coloring_get_w(vxp_system* sys, uint32_t world, uint8_t* colors, uint32_t capacity,
+                       uint32_t* count);
+
+/* Last error message of this thread (never NULL). */
+const char* vxp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VXP_H */

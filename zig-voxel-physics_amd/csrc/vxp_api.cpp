@@ -1,0 +1,1048 @@
+// vxp_api.cpp — host runtime and C ABI (include/vxp.h) of the MI355X step
+// pipeline.  The host keeps the reference's bookkeeping (body slots with
+// generations and a free list, monotonically increasing body ids, colliders
+// built from setCollisionVoxel writes, the fixed-step accumulator); all
+// per-step work runs in the gfx950 kernels of vxp_kernels.hip on one HIP
+// stream.  Body state lives on the device between steps; getters copy it out
+// and setters copy it in before the next step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/vxp.h"
+#include "vxp_dev.hpp"
+#include "vxp_kernels.hpp"
+
+using namespace vxm;
+using namespace vxd;
+
+static thread_local std::string g_err;
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      g_err = std::string(#expr) + ": " + hipGetErrorString(e_);                         \
+      return VXP_ERR_DEVICE;                                                             \
+    }                                                                                    \
+  } while (0)
+
+namespace {
+
+// Growable device buffer (contents are not preserved on growth unless asked).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need, bool keep = false, hipStream_t s = nullptr) {
+    if (need <= bytes && p) return hipSuccess;
+    size_t nb = std::max(need, bytes * 2);
+    if (nb < 256) nb = 256;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, nb);
+    if (e != hipSuccess) return e;
+    if (keep && p && bytes) {
+      e = hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+    }
+    if (p) (void)hipFree(p);
+    p = q;
+    bytes = nb;
+    return hipSuccess;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// ---- collider (src/collider.zig) -------------------------------------------
+struct HostCollider {
+  float vs = 0.1f;
+  std::vector<int32_t> pos;   // merged bricks, ascending (z, y, x)
+  std::vector<uint8_t> vox;   // 512 per merged brick
+  std::unordered_map<uint64_t, int32_t> by_hash;
+  int32_t minb[3] = {0, 0, 0}, maxb[3] = {0, 0, 0};
+  float lo[3], hi[3];
+  uint32_t solid = 0;
+  int32_t org[3], dim[3];
+  std::vector<int32_t> dir;
+  // mass properties cache key (density) -> (inv_mass, inv inertia)
+};
+
+// VoxelCollider.recomputeMetadata (src/collider.zig:90-133, 283-333)
+void colliderMetadata(HostCollider& c) {
+  c.minb[0] = c.minb[1] = c.minb[2] = INT32_MAX;
+  c.maxb[0] = c.maxb[1] = c.maxb[2] = INT32_MIN;
+  c.lo[0] = c.lo[1] = c.lo[2] = INFINITY;
+  c.hi[0] = c.hi[1] = c.hi[2] = -INFINITY;
+  c.solid = 0;
+  size_t nb = c.pos.size() / 3;
+  for (size_t b = 0; b < nb; b++) {
+    const int32_t* bp = &c.pos[3 * b];
+    const uint8_t* v = &c.vox[512 * b];
+    V3 mn{INFINITY, INFINITY, INFINITY}, mx{-INFINITY, -INFINITY, -INFINITY};
+    uint32_t cnt = 0;
+    for (int i = 0; i < 512; i++) {
+      if ((v[i] & 7u) == 0) continue;
+      cnt++;
+      int cx = i & 7, cy = (i >> 3) & 7, cz = i >> 6;
+      V3 lp = scale(V3{(float)bp[0] * 8.0f + (float)cx, (float)bp[1] * 8.0f + (float)cy,
+                       (float)bp[2] * 8.0f + (float)cz},
+                    c.vs);
+      mn = vmin(mn, lp);
+      mx = vmax(mx, add(lp, V3{c.vs, c.vs, c.vs}));
+      int32_t vx = bp[0] * 8 + cx, vy = bp[1] * 8 + cy, vz = bp[2] * 8 + cz;
+      c.minb[0] = std::min(c.minb[0], vx);
+      c.minb[1] = std::min(c.minb[1], vy);
+      c.minb[2] = std::min(c.minb[2], vz);
+      c.maxb[0] = std::max(c.maxb[0], vx + 1);
+      c.maxb[1] = std::max(c.maxb[1], vy + 1);
+      c.maxb[2] = std::max(c.maxb[2], vz + 1);
+    }
+    if (!cnt) continue;
+    c.solid += cnt;
+    c.lo[0] = fminf(c.lo[0], mn.x);
+    c.lo[1] = fminf(c.lo[1], mn.y);
+    c.lo[2] = fminf(c.lo[2], mn.z);
+    c.hi[0] = fmaxf(c.hi[0], mx.x);
+    c.hi[1] = fmaxf(c.hi[1], mx.y);
+    c.hi[2] = fmaxf(c.hi[2], mx.z);
+  }
+  if (c.solid == 0) {
+    for (int k = 0; k < 3; k++) c.minb[k] = c.maxb[k] = 0;
+  }
+  // dense directory over brick positions floor(min/8) .. floor(max/8)
+  size_t cells = 1;
+  for (int k = 0; k < 3; k++) {
+    c.org[k] = floorDiv8(c.minb[k]);
+    c.dim[k] = floorDiv8(c.maxb[k]) - c.org[k] + 1;
+    cells *= (size_t)c.dim[k];
+  }
+  c.dir.assign(cells, -1);
+  for (int32_t z = 0; z < c.dim[2]; z++)
+    for (int32_t y = 0; y < c.dim[1]; y++)
+      for (int32_t x = 0; x < c.dim[0]; x++) {
+        auto it = c.by_hash.find(hashBrickPos(c.org[0] + x, c.org[1] + y, c.org[2] + z));
+        if (it != c.by_hash.end())
+          c.dir[((size_t)z * c.dim[1] + y) * c.dim[0] + x] = it->second;
+      }
+}
+
+struct MassProps {
+  float inv_mass;
+  M3 inv_inertia;
+};
+
+// computeMassProperties (src/body.zig:462-519) + getCenterOfMass
+// (src/collider.zig:376-403), voxels summed brick by brick in ascending
+// (z, y, x) brick order.
+MassProps massProperties(const HostCollider& c, float density) {
+  if (c.solid == 0) return {0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}};
+  float vv = c.vs * c.vs * c.vs;
+  float vm = vv * density;
+  float total = vm * (float)c.solid;
+  float half = c.vs * 0.5f;
+  size_t nb = c.pos.size() / 3;
+  auto voxelPos = [&](size_t b, int i) {
+    int32_t vx = c.pos[3 * b] * 8 + (i & 7), vy = c.pos[3 * b + 1] * 8 + ((i >> 3) & 7),
+            vz = c.pos[3 * b + 2] * 8 + (i >> 6);
+    V3 l{(float)vx * c.vs + half, (float)vy * c.vs + half, (float)vz * c.vs + half};
+    return add(l, V3{c.vs * 0.5f, c.vs * 0.5f, c.vs * 0.5f});
+  };
+  V3 sum{0, 0, 0};
+  uint32_t count = 0;
+  for (size_t b = 0; b < nb; b++)
+    for (int i = 0; i < 512; i++) {
+      if ((c.vox[512 * b + i] & 7u) == 0) continue;
+      sum = add(sum, voxelPos(b, i));
+      count++;
+    }
+  V3 com = scale(sum, 1.0f / (float)count);
+  float I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t b = 0; b < nb; b++)
+    for (int i = 0; i < 512; i++) {
+      if ((c.vox[512 * b + i] & 7u) == 0) continue;
+      V3 r = sub(voxelPos(b, i), com);
+      float r2 = dot(r, r);
+      I[0] += vm * (r2 - r.x * r.x);
+      I[4] += vm * (r2 - r.y * r.y);
+      I[8] += vm * (r2 - r.z * r.z);
+      I[1] -= vm * r.x * r.y;
+      I[2] -= vm * r.x * r.z;
+      I[5] -= vm * r.y * r.z;
+    }
+  I[3] = I[1];
+  I[6] = I[2];
+  I[7] = I[5];
+  M3 m;
+  for (int i = 0; i < 9; i++) m.m[i] = I[i];
+  return {1.0f / total, minverse(m)};
+}
+
+}  // namespace
+
+struct vxp_system {
+  vxp_config cfg;
+  uint32_t nw = 1, MB = 0, id_cap = 0;
+  hipStream_t stream = nullptr;
+  float acc = 0.0f;
+
+  std::vector<HostCollider> colliders;
+  bool coll_dirty = true;
+  DevBuf d_hdr, d_dir, d_vox;
+
+  // host mirror of the body SoA
+  std::vector<float4> pos, rot, lv, av, fat_lo, fat_hi;
+  std::vector<float> inv_mass, iil, fric, rest, ldamp, adamp, sleep_timer;
+  std::vector<uint8_t> type, sleeping, active;
+  std::vector<int32_t> collider;
+  std::vector<uint32_t> id;
+  std::vector<int32_t> idmap;
+  std::vector<uint32_t> generation, next_id, body_count;
+  std::vector<std::vector<uint32_t>> free_list;
+  std::vector<uint32_t> n_total, n_dyn, n_static;
+  bool host_dirty = true, dev_newer = false;
+
+  DevBuf b_pos, b_rot, b_lv, b_av, b_fat_lo, b_fat_hi, b_inv_mass, b_iil, b_fric, b_rest,
+      b_ldamp, b_adamp, b_sleep_timer, b_type, b_sleeping, b_active, b_collider, b_id, b_idmap;
+
+  // per world / per step device state
+  DevBuf d_ws, d_stats, d_cache_keys, d_cache_vals;
+  DevBuf d_skeys, d_svals, d_skeys2, d_svals2, d_sort_tmp;
+  DevBuf d_pairs_raw, d_pairs, d_count, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_wi_vchk,
+      d_cand,
+      d_produce, d_pidx, d_staging, d_cstage, d_total;
+  DevBuf d_ml[2];
+  int cur = 0;
+  DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
+  DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_pt,
+      s_prio;
+  uint32_t n_pairs = 0, n_manifolds = 0;
+  uint32_t pair_cap = 0, cand_cap = 0;
+  std::vector<uint32_t> last_step_world_skip;
+};
+
+namespace {
+
+size_t slots(const vxp_system* s) { return (size_t)s->nw * s->MB; }
+
+int uploadColliders(vxp_system* s) {
+  if (!s->coll_dirty) return VXP_OK;
+  std::vector<DevCollider> hdr;
+  std::vector<int32_t> dir;
+  std::vector<uint8_t> vox;
+  uint32_t brick_base = 0;
+  for (const HostCollider& c : s->colliders) {
+    DevCollider h;
+    h.vs = c.vs;
+    for (int k = 0; k < 3; k++) {
+      h.minb[k] = c.minb[k];
+      h.maxb[k] = c.maxb[k];
+      h.lo[k] = c.lo[k];
+      h.hi[k] = c.hi[k];
+      h.org[k] = c.org[k];
+      h.dim[k] = c.dim[k];
+    }
+    h.dir_off = (uint32_t)dir.size();
+    h.solid = c.solid;
+    for (int32_t d : c.dir) dir.push_back(d < 0 ? -1 : d + (int32_t)brick_base);
+    vox.insert(vox.end(), c.vox.begin(), c.vox.end());
+    brick_base += (uint32_t)(c.vox.size() / 512);
+    hdr.push_back(h);
+  }
+  if (hdr.empty()) hdr.push_back(DevCollider{});
+  if (dir.empty()) dir.push_back(-1);
+  if (vox.empty()) vox.resize(512, 0);
+  HIP_TRY(s->d_hdr.ensure(hdr.size() * sizeof(DevCollider)));
+  HIP_TRY(s->d_dir.ensure(dir.size() * 4));
+  HIP_TRY(s->d_vox.ensure(vox.size()));
+  HIP_TRY(hipMemcpy(s->d_hdr.p, hdr.data(), hdr.size() * sizeof(DevCollider), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_dir.p, dir.data(), dir.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_vox.p, vox.data(), vox.size(), hipMemcpyHostToDevice));
+  s->coll_dirty = false;
+  return VXP_OK;
+}
+
+template <class T>
+int up(DevBuf& b, const std::vector<T>& v) {
+  HIP_TRY(b.ensure(std::max<size_t>(v.size() * sizeof(T), 16)));
+  if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return VXP_OK;
+}
+template <class T>
+int down(const DevBuf& b, std::vector<T>& v) {
+  if (!v.empty()) HIP_TRY(hipMemcpy(v.data(), b.p, v.size() * sizeof(T), hipMemcpyDeviceToHost));
+  return VXP_OK;
+}
+
+int ensureDevice(vxp_system* s) {
+  int rc = uploadColliders(s);
+  if (rc) return rc;
+  if (!s->host_dirty) return VXP_OK;
+  if ((rc = up(s->b_pos, s->pos)) || (rc = up(s->b_rot, s->rot)) || (rc = up(s->b_lv, s->lv)) ||
+      (rc = up(s->b_av, s->av)) || (rc = up(s->b_fat_lo, s->fat_lo)) ||
+      (rc = up(s->b_fat_hi, s->fat_hi)) || (rc = up(s->b_inv_mass, s->inv_mass)) ||
+      (rc = up(s->b_iil, s->iil)) || (rc = up(s->b_fric, s->fric)) || (rc = up(s->b_rest, s->rest)) ||
+      (rc = up(s->b_ldamp, s->ldamp)) || (rc = up(s->b_adamp, s->adamp)) ||
+      (rc = up(s->b_sleep_timer, s->sleep_timer)) || (rc = up(s->b_type, s->type)) ||
+      (rc = up(s->b_sleeping, s->sleeping)) || (rc = up(s->b_active, s->active)) ||
+      (rc = up(s->b_collider, s->collider)) || (rc = up(s->b_id, s->id)) ||
+      (rc = up(s->b_idmap, s->idmap)))
+    return rc;
+  s->host_dirty = false;
+  return VXP_OK;
+}
+
+int ensureHost(vxp_system* s) {
+  if (!s->dev_newer) return VXP_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  int rc;
+  if ((rc = down(s->b_pos, s->pos)) || (rc = down(s->b_rot, s->rot)) || (rc = down(s->b_lv, s->lv)) ||
+      (rc = down(s->b_av, s->av)) || (rc = down(s->b_fat_lo, s->fat_lo)) ||
+      (rc = down(s->b_fat_hi, s->fat_hi)) || (rc = down(s->b_sleep_timer, s->sleep_timer)) ||
+      (rc = down(s->b_sleeping, s->sleeping)))
+    return rc;
+  s->dev_newer = false;
+  return VXP_OK;
+}
+
+DevBodies devBodies(vxp_system* s) {
+  DevBodies B;
+  B.pos = s->b_pos.as<float4>();
+  B.rot = s->b_rot.as<float4>();
+  B.lv = s->b_lv.as<float4>();
+  B.av = s->b_av.as<float4>();
+  B.inv_mass = s->b_inv_mass.as<float>();
+  B.iil = s->b_iil.as<float>();
+  B.fric = s->b_fric.as<float>();
+  B.rest = s->b_rest.as<float>();
+  B.ldamp = s->b_ldamp.as<float>();
+  B.adamp = s->b_adamp.as<float>();
+  B.sleep_timer = s->b_sleep_timer.as<float>();
+  B.type = s->b_type.as<uint8_t>();
+  B.sleeping = s->b_sleeping.as<uint8_t>();
+  B.active = s->b_active.as<uint8_t>();
+  B.collider = s->b_collider.as<int32_t>();
+  B.id = s->b_id.as<uint32_t>();
+  B.fat_lo = s->b_fat_lo.as<float4>();
+  B.fat_hi = s->b_fat_hi.as<float4>();
+  B.idmap = s->b_idmap.as<int32_t>();
+  return B;
+}
+
+DevColliders devColliders(vxp_system* s) {
+  return DevColliders{s->d_hdr.as<DevCollider>(), s->d_dir.as<int32_t>(), s->d_vox.as<uint8_t>()};
+}
+
+DevCfg devCfg(vxp_system* s) {
+  const vxp_config& c = s->cfg;
+  DevCfg d{};
+  for (int k = 0; k < 3; k++) d.gravity[k] = c.gravity[k];
+  d.dt = c.fixed_timestep;
+  d.h = c.fixed_timestep / (float)c.substeps;
+  // computeSoftParams (src/solver.zig:57-72)
+  if (c.hertz <= 0.0f) {
+    d.bias_rate = 0.0f;
+    d.mass_coeff = 1.0f;
+    d.impulse_coeff = 1.0f;
+  } else {
+    float omega = 2.0f * kPi * c.hertz;
+    float a1 = 2.0f * c.damping_ratio + omega * d.h;
+    float a2 = d.h * omega * a1;
+    float a3 = 1.0f / (1.0f + a2);
+    d.bias_rate = omega / a1;
+    d.mass_coeff = a2 * a3;
+    d.impulse_coeff = a3;
+  }
+  d.slop = c.slop;
+  d.pcr = c.position_correction_rate;
+  d.mpc = c.max_position_correction;
+  d.wss = c.warm_start_scale;
+  d.substeps = c.substeps;
+  d.vel_iters = c.velocity_iterations;
+  d.pos_iters = c.position_iterations;
+  d.max_bodies = s->MB;
+  d.id_cap = s->id_cap;
+  d.cache_cap = c.brick_pair_cache_size ? c.brick_pair_cache_size : 1024;
+  d.enable_sleeping = c.enable_sleeping;
+  d.warm_starting = c.warm_starting;
+  d.enable_friction = c.enable_friction;
+  d.enable_restitution = c.enable_restitution;
+  d.disable_rotation = c.disable_rotation;
+  d.use_sat = c.use_brick_sat;
+  d.use_cache = c.use_brick_pair_cache;
+  d.enable_islands = c.enable_islands;
+  d.store_impulses = c.enable_warm_starting;
+  return d;
+}
+
+// Grow the id-indexed arrays (idmap and island scratch) to hold id `need`.
+void growIds(vxp_system* s, uint32_t need) {
+  if (need < s->id_cap) return;
+  uint32_t nc = std::max(need + 1, s->id_cap * 2);
+  std::vector<int32_t> m((size_t)s->nw * nc, -1);
+  for (uint32_t w = 0; w < s->nw; w++)
+    for (uint32_t i = 0; i < s->id_cap; i++) m[(size_t)w * nc + i] = s->idmap[(size_t)w * s->id_cap + i];
+  s->idmap.swap(m);
+  s->id_cap = nc;
+  s->host_dirty = true;
+}
+
+#define RC(x)            \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
+int stepAll(vxp_system* s) {
+  hipStream_t st = s->stream;
+  RC(ensureDevice(s));
+  DevCfg cfg = devCfg(s);
+  DevBodies B = devBodies(s);
+  DevColliders C = devColliders(s);
+  const uint32_t nw = s->nw;
+  const uint32_t ns = (uint32_t)slots(s);
+  WorldState* ws = s->d_ws.as<WorldState>();
+  vxp_stats* stats = s->d_stats.as<vxp_stats>();
+
+  // ContactManager.beginFrame: the current manifolds become the previous ones
+  s->cur ^= 1;
+  DevManifold* prev = s->d_ml[s->cur ^ 1].as<DevManifold>();
+  launch_step_begin(st, cfg, B, ws, stats, nw);
+  launch_fat_aabb(st, cfg, B, C, ws, ns);
+
+  // ---- pair set: sort-and-sweep --------------------------------------------
+  HIP_TRY(s->d_skeys.ensure((size_t)ns * 8));
+  HIP_TRY(s->d_svals.ensure((size_t)ns * 4));
+  HIP_TRY(s->d_skeys2.ensure((size_t)ns * 8));
+  HIP_TRY(s->d_svals2.ensure((size_t)ns * 4));
+  launch_sweep_keys(st, cfg, B, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns);
+  size_t tb = sort_pairs_u64_u32_bytes(ns);
+  HIP_TRY(s->d_sort_tmp.ensure(tb));
+  HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, tb, s->d_skeys.as<uint64_t>(), s->d_skeys2.as<uint64_t>(),
+                             s->d_svals.as<uint32_t>(), s->d_svals2.as<uint32_t>(), ns, st));
+  HIP_TRY(s->d_count.ensure(64));
+  uint32_t npairs = 0;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    if (s->pair_cap < 1024) s->pair_cap = std::max<uint32_t>(1024, ns * 8);
+    HIP_TRY(s->d_pairs_raw.ensure((size_t)s->pair_cap * 8));
+    HIP_TRY(hipMemsetAsync(s->d_count.p, 0, 4, st));
+    launch_sweep_pairs(st, cfg, B, s->d_skeys2.as<uint64_t>(), s->d_svals2.as<uint32_t>(), ns,
+                       s->d_pairs_raw.as<uint64_t>(), s->d_count.as<uint32_t>(), s->pair_cap);
+    HIP_TRY(hipMemcpyAsync(&npairs, s->d_count.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (npairs <= s->pair_cap) break;
+    s->pair_cap = npairs + npairs / 2;
+  }
+  s->n_pairs = npairs;
+  HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
+  if (npairs) {
+    size_t kb = sort_keys_u64_bytes(npairs);
+    HIP_TRY(s->d_sort_tmp.ensure(kb));
+    HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(), s->d_pairs.as<uint64_t>(),
+                          npairs, st));
+  }
+  const uint64_t* keys = s->d_pairs.as<uint64_t>();
+
+  // ---- narrow phase ----------------------------------------------------------
+  uint32_t np1 = std::max(npairs, 1u);
+  HIP_TRY(s->d_pr.ensure((size_t)np1 * sizeof(PairRec)));
+  HIP_TRY(s->d_counts.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_wi_off.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_produce.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_pidx.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_staging.ensure((size_t)np1 * sizeof(DevManifold)));
+  HIP_TRY(s->d_cstage.ensure((size_t)np1 * sizeof(CacheVal)));
+  PairRec* pr = s->d_pr.as<PairRec>();
+  launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev);
+  launch_cache_replay(st, cfg, ws, stats, pr, s->d_cache_keys.as<CacheKeyRec>(), nw);
+  uint32_t total_wi = 0;
+  if (npairs) {
+    launch_chunk_counts(st, pr, npairs, s->d_counts.as<uint32_t>());
+    size_t sb = exclusive_scan_u32_bytes(npairs);
+    HIP_TRY(s->d_sort_tmp.ensure(sb));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, sb, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
+                               npairs, st));
+    uint32_t last[2];
+    HIP_TRY(hipMemcpyAsync(&last[0], s->d_wi_off.as<uint32_t>() + npairs - 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&last[1], s->d_counts.as<uint32_t>() + npairs - 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    total_wi = last[0] + last[1];
+  }
+  uint32_t tw1 = std::max(total_wi, 1u);
+  HIP_TRY(s->d_wi_cbase.ensure((size_t)tw1 * 4));
+  HIP_TRY(s->d_wi_ccnt.ensure((size_t)tw1 * 4));
+  HIP_TRY(s->d_wi_vchk.ensure((size_t)tw1 * 4));
+  if (s->cand_cap < 4096) s->cand_cap = 1u << 16;
+  for (int attempt = 0; attempt < 16 && total_wi; attempt++) {
+    HIP_TRY(s->d_cand.ensure((size_t)s->cand_cap * sizeof(Cand)));
+    HIP_TRY(hipMemsetAsync(s->d_count.p, 0, 4, st));
+    // the probe is a pure function of the step's inputs, so it can be re-run
+    launch_probe(st, cfg, B, C, pr, npairs, s->d_wi_off.as<uint32_t>(), total_wi,
+                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
+                 s->d_wi_vchk.as<uint32_t>(), s->d_cand.as<Cand>(), s->d_count.as<uint32_t>(),
+                 s->cand_cap);
+    uint32_t nc = 0;
+    HIP_TRY(hipMemcpyAsync(&nc, s->d_count.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nc <= s->cand_cap) break;
+    s->cand_cap = nc + nc / 2;  // grow and re-run: the probe only writes fresh buffers
+  }
+  launch_reduce(st, cfg, B, ws, stats, pr, keys, npairs, s->d_wi_off.as<uint32_t>(),
+                s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
+                s->d_cand.as<Cand>(), prev,
+                s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>(),
+                s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>());
+  launch_cache_fill(st, cfg, ws, pr, npairs, s->d_cache_keys.as<CacheKeyRec>(),
+                    s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>());
+  if (npairs) {
+    size_t sb = exclusive_scan_u32_bytes(npairs);
+    HIP_TRY(s->d_sort_tmp.ensure(sb));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, sb, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(),
+                               npairs, st));
+  }
+  HIP_TRY(s->d_total.ensure(64));
+  launch_world_segments(st, ws, nw, s->d_total.as<uint32_t>());
+  uint32_t nman = 0;
+  HIP_TRY(hipMemcpyAsync(&nman, s->d_total.p, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  s->n_manifolds = nman;
+  uint32_t nm1 = std::max(nman, 1u);
+  HIP_TRY(s->d_ml[s->cur].ensure((size_t)nm1 * sizeof(DevManifold)));
+  DevManifold* ml = s->d_ml[s->cur].as<DevManifold>();
+  launch_scatter(st, ws, pr, npairs, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(),
+                 s->d_staging.as<DevManifold>(), ml);
+  launch_copy_skipped(st, ws, prev, ml, nw);
+
+  // ---- islands / colouring artefacts ------------------------------------------
+  if (s->cfg.enable_islands) {
+    size_t n = (size_t)nw * s->id_cap * 4;
+    HIP_TRY(s->d_isl_parent.ensure(n));
+    HIP_TRY(s->d_isl_end.ensure(n));
+    HIP_TRY(s->d_isl_size.ensure(n));
+    launch_islands(st, cfg, B, ws, stats, ml, s->d_isl_parent.as<uint32_t>(), s->d_isl_end.as<uint32_t>(),
+                   s->d_isl_size.as<uint32_t>(), nw);
+  }
+  if (s->cfg.enable_coloring) {
+    HIP_TRY(s->d_ref_masks.ensure((size_t)nw * s->id_cap * 32));
+    HIP_TRY(s->d_ref_colors.ensure((size_t)nm1));
+    launch_ref_coloring(st, cfg, ws, ml, s->d_ref_masks.as<uint64_t>(), s->d_ref_colors.as<uint8_t>(), nw);
+  }
+
+  // ---- solver ------------------------------------------------------------------
+  HIP_TRY(s->s_pos.ensure((size_t)ns * 16));
+  HIP_TRY(s->s_rot.ensure((size_t)ns * 16));
+  HIP_TRY(s->s_lv.ensure((size_t)ns * 16));
+  HIP_TRY(s->s_av.ensure((size_t)ns * 16));
+  HIP_TRY(s->s_ii.ensure((size_t)ns * 36));
+  HIP_TRY(s->s_adj_cnt.ensure((size_t)ns * 4));
+  HIP_TRY(s->s_adj_off.ensure((size_t)ns * 4));
+  HIP_TRY(s->s_adj.ensure((size_t)nm1 * 8));
+  HIP_TRY(s->s_color.ensure((size_t)nm1 * 4));
+  HIP_TRY(s->s_order.ensure((size_t)nm1 * 4));
+  HIP_TRY(s->s_prio.ensure((size_t)nm1 * 4));
+  HIP_TRY(s->s_pt.ensure((size_t)nm1 * 4 * kPtStride * 4));
+  SolverScratch S;
+  S.ws_pos = s->s_pos.as<float4>();
+  S.ws_rot = s->s_rot.as<float4>();
+  S.ws_lv = s->s_lv.as<float4>();
+  S.ws_av = s->s_av.as<float4>();
+  S.ws_ii = s->s_ii.as<float>();
+  S.adj_cnt = s->s_adj_cnt.as<uint32_t>();
+  S.adj_off = s->s_adj_off.as<uint32_t>();
+  S.adj = s->s_adj.as<uint32_t>();
+  S.color = s->s_color.as<int32_t>();
+  S.order = s->s_order.as<uint32_t>();
+  S.pt = s->s_pt.as<float>();
+  launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw);
+  HIP_TRY(hipGetLastError());
+  s->dev_newer = true;
+  return VXP_OK;
+}
+
+int checkErrors(vxp_system* s) {
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<WorldState> ws(s->nw);
+  HIP_TRY(hipMemcpy(ws.data(), s->d_ws.p, ws.size() * sizeof(WorldState), hipMemcpyDeviceToHost));
+  for (auto& w : ws)
+    if (w.error) {
+      g_err = "solver colouring exceeded 1024 colours";
+      return VXP_ERR_CAPACITY;
+    }
+  return VXP_OK;
+}
+
+int updateOnce(vxp_system* s, float dt) {
+  const vxp_config& c = s->cfg;
+  RC(ensureDevice(s));
+  s->acc += dt;
+  launch_update_begin(s->stream, s->d_stats.as<vxp_stats>(), s->nw, s->acc);
+  uint32_t n = 0;
+  while (s->acc >= c.fixed_timestep && n < c.max_substeps) {
+    RC(stepAll(s));
+    s->acc -= c.fixed_timestep;
+    n++;
+  }
+  if (s->acc > c.fixed_timestep * (float)c.max_substeps) s->acc = c.fixed_timestep;
+  launch_update_end(s->stream, devCfg(s), devBodies(s), s->d_stats.as<vxp_stats>(), s->nw, n, s->acc);
+  HIP_TRY(hipGetLastError());
+  return VXP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vxp_last_error(void) { return g_err.c_str(); }
+
+int vxp_config_default(vxp_config* c) {
+  if (!c) return VXP_ERR_ARGUMENT;
+  std::memset(c, 0, sizeof(*c));
+  c->gravity[0] = 0.0f;
+  c->gravity[1] = -9.81f;
+  c->gravity[2] = 0.0f;
+  c->fixed_timestep = 1.0f / 60.0f;
+  c->max_substeps = 8;
+  c->max_bodies = 1024;
+  c->default_friction = 0.6f;
+  c->default_restitution = 0.2f;
+  c->default_density = 1.0f;
+  c->enable_sleeping = 1;
+  c->enable_warm_starting = 1;
+  c->enable_islands = 1;
+  c->enable_coloring = 0;
+  c->substeps = 4;
+  c->velocity_iterations = 1;
+  c->position_iterations = 2;
+  c->hertz = 30.0f;
+  c->damping_ratio = 1.0f;
+  c->position_correction_rate = 0.2f;
+  c->max_position_correction = 0.04f;
+  c->slop = 0.005f;
+  c->warm_start_scale = 0.8f;
+  c->disable_rotation = 0;
+  c->warm_starting = 1;
+  c->enable_friction = 1;
+  c->enable_restitution = 1;
+  c->use_brick_sat = 1;
+  c->use_brick_pair_cache = 1;
+  c->brick_pair_cache_size = 1024;
+  c->pair_cap = 0;
+  c->solve_order = VXP_SOLVE_ORDER_COLOR;
+  c->cpu_threads = 1;
+  return VXP_OK;
+}
+
+int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out) {
+  if (!cfg || !out || n_worlds == 0 || cfg->max_bodies == 0 || cfg->substeps == 0) {
+    g_err = "invalid configuration";
+    return VXP_ERR_ARGUMENT;
+  }
+  if (cfg->pair_cap != 0) {
+    g_err = "pair_cap is only supported by the CPU restatement";
+    return VXP_ERR_ARGUMENT;
+  }
+  if (cfg->solve_order != VXP_SOLVE_ORDER_COLOR) {
+    g_err = "the GPU pipeline solves in colour order (VXP_SOLVE_ORDER_COLOR)";
+    return VXP_ERR_ARGUMENT;
+  }
+  uint32_t cap = cfg->brick_pair_cache_size ? cfg->brick_pair_cache_size : 1024;
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
+    g_err = "no HIP device";
+    return VXP_ERR_DEVICE;
+  }
+  auto* s = new vxp_system();
+  s->cfg = *cfg;
+  s->nw = n_worlds;
+  s->MB = cfg->max_bodies;
+  s->id_cap = cfg->max_bodies + 2;
+  size_t ns = slots(s);
+  s->pos.assign(ns, make_float4(0, 0, 0, 0));
+  s->rot.assign(ns, make_float4(0, 0, 0, 1));
+  s->lv.assign(ns, make_float4(0, 0, 0, 0));
+  s->av.assign(ns, make_float4(0, 0, 0, 0));
+  s->fat_lo.assign(ns, make_float4(0, 0, 0, 0));
+  s->fat_hi.assign(ns, make_float4(0, 0, 0, 0));
+  s->inv_mass.assign(ns, 0.0f);
+  s->iil.assign(ns * 9, 0.0f);
+  s->fric.assign(ns, 0.5f);
+  s->rest.assign(ns, 0.0f);
+  s->ldamp.assign(ns, 0.0f);
+  s->adamp.assign(ns, 0.0f);
+  s->sleep_timer.assign(ns, 0.0f);
+  s->type.assign(ns, 0);
+  s->sleeping.assign(ns, 0);
+  s->active.assign(ns, 0);
+  s->collider.assign(ns, -1);
+  s->id.assign(ns, 0);
+  s->idmap.assign((size_t)n_worlds * s->id_cap, -1);
+  s->generation.assign(ns, 0);
+  s->next_id.assign(n_worlds, 1);
+  s->body_count.assign(n_worlds, 0);
+  s->free_list.assign(n_worlds, {});
+  s->n_total.assign(n_worlds, 0);
+  s->n_dyn.assign(n_worlds, 0);
+  s->n_static.assign(n_worlds, 0);
+  int rc = VXP_OK;
+  auto fail = [&](int r) {
+    delete s;
+    return r;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    g_err = "hipStreamCreate failed";
+    return fail(VXP_ERR_DEVICE);
+  }
+  std::vector<WorldState> ws(n_worlds);
+  std::memset(ws.data(), 0, ws.size() * sizeof(WorldState));
+  std::vector<vxp_stats> st(n_worlds);
+  std::memset(st.data(), 0, st.size() * sizeof(vxp_stats));
+  std::vector<CacheKeyRec> ck((size_t)n_worlds * cap);
+  std::memset(ck.data(), 0, ck.size() * sizeof(CacheKeyRec));
+  if ((rc = up(s->d_ws, ws)) || (rc = up(s->d_stats, st)) || (rc = up(s->d_cache_keys, ck)))
+    return fail(rc);
+  if (s->d_cache_vals.ensure((size_t)n_worlds * cap * sizeof(CacheVal)) != hipSuccess ||
+      s->d_ml[0].ensure(sizeof(DevManifold)) != hipSuccess ||
+      s->d_ml[1].ensure(sizeof(DevManifold)) != hipSuccess) {
+    g_err = "device allocation failed";
+    return fail(VXP_ERR_OOM);
+  }
+  *out = s;
+  return VXP_OK;
+}
+
+int vxp_system_destroy(vxp_system* s) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  (void)hipStreamSynchronize(s->stream);
+  DevBuf* bufs[] = {&s->d_hdr, &s->d_dir, &s->d_vox, &s->b_pos, &s->b_rot, &s->b_lv, &s->b_av,
+                    &s->b_fat_lo, &s->b_fat_hi, &s->b_inv_mass, &s->b_iil, &s->b_fric, &s->b_rest,
+                    &s->b_ldamp, &s->b_adamp, &s->b_sleep_timer, &s->b_type, &s->b_sleeping,
+                    &s->b_active, &s->b_collider, &s->b_id, &s->b_idmap, &s->d_ws, &s->d_stats,
+                    &s->d_cache_keys, &s->d_cache_vals, &s->d_skeys, &s->d_svals, &s->d_skeys2,
+                    &s->d_svals2, &s->d_sort_tmp, &s->d_pairs_raw, &s->d_pairs, &s->d_count,
+                    &s->d_pr, &s->d_counts, &s->d_wi_off, &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_wi_vchk,
+                    &s->d_cand, &s->d_produce, &s->d_pidx, &s->d_staging, &s->d_cstage,
+                    &s->d_total, &s->d_ml[0], &s->d_ml[1], &s->d_isl_parent, &s->d_isl_end,
+                    &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors, &s->s_pos, &s->s_rot,
+                    &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,
+                    &s->s_color, &s->s_order, &s->s_pt, &s->s_prio};
+  for (DevBuf* b : bufs) b->release();
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+  return VXP_OK;
+}
+
+int vxp_collider_create(vxp_system* s, float voxel_size, uint32_t n, const int32_t* bp,
+                        const uint8_t* vox, uint32_t* out) {
+  if (!s || !out || (n && (!bp || !vox)) || !(voxel_size > 0.0f)) return VXP_ERR_ARGUMENT;
+  HostCollider c;
+  c.vs = voxel_size;
+  // setCollisionVoxel semantics (src/collider.zig:232-239,267-273): bricks keyed
+  // by hashBrickPos; colliding hashes share the first brick, later writes win.
+  struct Tmp {
+    int32_t p[3];
+    uint8_t v[512];
+  };
+  std::vector<Tmp> merged;
+  std::unordered_map<uint64_t, size_t> idx;
+  for (uint32_t i = 0; i < n; i++) {
+    uint64_t h = hashBrickPos(bp[3 * i], bp[3 * i + 1], bp[3 * i + 2]);
+    auto it = idx.find(h);
+    if (it == idx.end()) {
+      Tmp t;
+      t.p[0] = bp[3 * i];
+      t.p[1] = bp[3 * i + 1];
+      t.p[2] = bp[3 * i + 2];
+      std::memset(t.v, 0, 512);
+      it = idx.emplace(h, merged.size()).first;
+      merged.push_back(t);
+    }
+    Tmp& t = merged[it->second];
+    for (int j = 0; j < 512; j++)
+      if (vox[512ull * i + j] & 7u) t.v[j] = vox[512ull * i + j];
+  }
+  std::sort(merged.begin(), merged.end(), [](const Tmp& a, const Tmp& b) {
+    if (a.p[2] != b.p[2]) return a.p[2] < b.p[2];
+    if (a.p[1] != b.p[1]) return a.p[1] < b.p[1];
+    return a.p[0] < b.p[0];
+  });
+  for (size_t i = 0; i < merged.size(); i++) {
+    c.pos.insert(c.pos.end(), merged[i].p, merged[i].p + 3);
+    c.vox.insert(c.vox.end(), merged[i].v, merged[i].v + 512);
+    c.by_hash[hashBrickPos(merged[i].p[0], merged[i].p[1], merged[i].p[2])] = (int32_t)i;
+  }
+  colliderMetadata(c);
+  *out = (uint32_t)s->colliders.size();
+  s->colliders.push_back(std::move(c));
+  s->coll_dirty = true;
+  return VXP_OK;
+}
+
+int vxp_body_create_w(vxp_system* s, uint32_t w, uint32_t type, const float pos[3], const float rot[4],
+                      uint32_t col, float density, vxp_handle* out) {
+  if (!s || !out || !pos || !rot || w >= s->nw || type > VXP_BODY_DYNAMIC) return VXP_ERR_ARGUMENT;
+  if (col != 0xFFFFFFFFu && col >= s->colliders.size()) return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  // allocateBody (src/system.zig:475-499)
+  uint32_t index;
+  if (!s->free_list[w].empty()) {
+    index = s->free_list[w].back();
+    s->free_list[w].pop_back();
+  } else {
+    uint32_t i = 0;
+    for (; i < s->MB; i++)
+      if (!s->active[(size_t)w * s->MB + i]) break;
+    if (i == s->MB) {
+      g_err = "MaxBodiesReached";
+      return VXP_ERR_CAPACITY;
+    }
+    index = i;
+  }
+  size_t g = (size_t)w * s->MB + index;
+  uint32_t bid = s->next_id[w]++;
+  growIds(s, bid + 1);
+  s->active[g] = 1;
+  s->body_count[w]++;
+  s->id[g] = bid;
+  s->type[g] = (uint8_t)type;
+  s->pos[g] = make_float4(pos[0], pos[1], pos[2], 0.0f);
+  s->rot[g] = make_float4(rot[0], rot[1], rot[2], rot[3]);
+  s->lv[g] = make_float4(0, 0, 0, 0);
+  s->av[g] = make_float4(0, 0, 0, 0);
+  s->collider[g] = col == 0xFFFFFFFFu ? -1 : (int32_t)col;
+  s->sleep_timer[g] = 0.0f;
+  s->fric[g] = s->cfg.default_friction;
+  s->rest[g] = s->cfg.default_restitution;
+  if (type == VXP_BODY_DYNAMIC) {
+    MassProps mp = col == 0xFFFFFFFFu ? MassProps{0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}}
+                                      : massProperties(s->colliders[col], density);
+    s->inv_mass[g] = mp.inv_mass;
+    for (int i = 0; i < 9; i++) s->iil[g * 9 + i] = mp.inv_inertia.m[i];
+    s->ldamp[g] = 0.3f;
+    s->adamp[g] = 0.5f;
+    s->sleeping[g] = 0;
+  } else {
+    s->inv_mass[g] = 0.0f;
+    for (int i = 0; i < 9; i++) s->iil[g * 9 + i] = 0.0f;
+    s->ldamp[g] = 0.0f;
+    s->adamp[g] = 0.0f;
+    s->sleeping[g] = type == VXP_BODY_STATIC ? 1 : 0;
+  }
+  // getWorldAABB + insertBody inflate (src/body.zig:430-435, src/broad_phase.zig:150-163)
+  V3 p{pos[0], pos[1], pos[2]};
+  Box wb{p, p};
+  if (col != 0xFFFFFFFFu) {
+    const HostCollider& hc = s->colliders[col];
+    wb = boxTransform(Box{V3{hc.lo[0], hc.lo[1], hc.lo[2]}, V3{hc.hi[0], hc.hi[1], hc.hi[2]}}, p,
+                      Q4{rot[0], rot[1], rot[2], rot[3]});
+  }
+  Box fat = boxInflate(wb, 0.1f);
+  s->fat_lo[g] = make_float4(fat.lo.x, fat.lo.y, fat.lo.z, 0.0f);
+  s->fat_hi[g] = make_float4(fat.hi.x, fat.hi.y, fat.hi.z, 0.0f);
+  s->idmap[(size_t)w * s->id_cap + bid] = (int32_t)index;
+  s->n_total[w]++;
+  if (type == VXP_BODY_DYNAMIC) s->n_dyn[w]++;
+  if (type == VXP_BODY_STATIC) s->n_static[w]++;
+  s->host_dirty = true;
+  out->index = index;
+  out->generation = s->generation[g];
+  return VXP_OK;
+}
+
+int vxp_body_create(vxp_system* s, uint32_t type, const float pos[3], const float rot[4], uint32_t col,
+                    float density, vxp_handle* out) {
+  return vxp_body_create_w(s, 0, type, pos, rot, col, density, out);
+}
+
+static bool validHandle(vxp_system* s, uint32_t w, vxp_handle h) {
+  if (w >= s->nw || h.index >= s->MB) return false;
+  size_t g = (size_t)w * s->MB + h.index;
+  return s->active[g] && s->generation[g] == h.generation;
+}
+
+int vxp_body_destroy_w(vxp_system* s, uint32_t w, vxp_handle h) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
+  size_t g = (size_t)w * s->MB + h.index;
+  s->n_total[w]--;
+  if (s->type[g] == VXP_BODY_DYNAMIC) s->n_dyn[w]--;
+  if (s->type[g] == VXP_BODY_STATIC) s->n_static[w]--;
+  s->idmap[(size_t)w * s->id_cap + s->id[g]] = -1;
+  s->active[g] = 0;
+  s->generation[g]++;
+  s->body_count[w]--;
+  s->free_list[w].push_back(h.index);
+  s->host_dirty = true;
+  return VXP_OK;
+}
+
+int vxp_system_update(vxp_system* s, float dt) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  RC(updateOnce(s, dt));
+  return checkErrors(s);
+}
+
+int vxp_system_update_frames(vxp_system* s, float dt, uint32_t frames) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  for (uint32_t f = 0; f < frames; f++) RC(updateOnce(s, dt));
+  return checkErrors(s);
+}
+
+int vxp_system_step(vxp_system* s, float dt) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  if (dt != s->cfg.fixed_timestep) {
+    g_err = "step(dt) runs the fixed-timestep pipeline: dt must equal fixed_timestep";
+    return VXP_ERR_ARGUMENT;
+  }
+  RC(stepAll(s));
+  return checkErrors(s);
+}
+
+int vxp_bodies_get_w(vxp_system* s, uint32_t w, vxp_body_state* out, uint32_t cap, uint32_t* count) {
+  if (!s || !count || w >= s->nw) return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  uint32_t n = std::min(cap, s->MB);
+  for (uint32_t i = 0; i < n && out; i++) {
+    size_t g = (size_t)w * s->MB + i;
+    vxp_body_state& o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.active = s->active[g];
+    if (!o.active) continue;
+    o.position[0] = s->pos[g].x; o.position[1] = s->pos[g].y; o.position[2] = s->pos[g].z;
+    o.rotation[0] = s->rot[g].x; o.rotation[1] = s->rot[g].y; o.rotation[2] = s->rot[g].z;
+    o.rotation[3] = s->rot[g].w;
+    o.linear_velocity[0] = s->lv[g].x; o.linear_velocity[1] = s->lv[g].y; o.linear_velocity[2] = s->lv[g].z;
+    o.angular_velocity[0] = s->av[g].x; o.angular_velocity[1] = s->av[g].y; o.angular_velocity[2] = s->av[g].z;
+    o.sleep_timer = s->sleep_timer[g];
+    o.id = s->id[g];
+    o.body_type = s->type[g];
+    o.is_sleeping = s->sleeping[g];
+  }
+  *count = n;
+  return VXP_OK;
+}
+
+int vxp_body_set_state_w(vxp_system* s, uint32_t w, vxp_handle h, const float pos[3], const float rot[4],
+                         const float lv[3], const float av[3]) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
+  size_t g = (size_t)w * s->MB + h.index;
+  if (pos) s->pos[g] = make_float4(pos[0], pos[1], pos[2], 0.0f);
+  if (rot) s->rot[g] = make_float4(rot[0], rot[1], rot[2], rot[3]);
+  if (s->type[g] == VXP_BODY_DYNAMIC) {
+    if (lv) s->lv[g] = make_float4(lv[0], lv[1], lv[2], 0.0f);
+    if (av) s->av[g] = make_float4(av[0], av[1], av[2], 0.0f);
+  }
+  if (s->type[g] != VXP_BODY_STATIC) {  // wakeUp (src/body.zig:400-404)
+    s->sleeping[g] = 0;
+    s->sleep_timer[g] = 0.0f;
+  }
+  s->host_dirty = true;
+  return VXP_OK;
+}
+
+int vxp_stats_get_w(vxp_system* s, uint32_t w, vxp_stats* out) {
+  if (!s || !out || w >= s->nw) return VXP_ERR_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(out, s->d_stats.as<vxp_stats>() + w, sizeof(vxp_stats), hipMemcpyDeviceToHost));
+  out->total_bodies = s->n_total[w];
+  out->dynamic_bodies = s->n_dyn[w];
+  out->static_bodies = s->n_static[w];
+  return VXP_OK;
+}
+
+static int readWorlds(vxp_system* s, std::vector<WorldState>& ws) {
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  ws.resize(s->nw);
+  HIP_TRY(hipMemcpy(ws.data(), s->d_ws.p, ws.size() * sizeof(WorldState), hipMemcpyDeviceToHost));
+  return VXP_OK;
+}
+
+int vxp_manifolds_get_w(vxp_system* s, uint32_t w, vxp_manifold* out, uint32_t cap, uint32_t* count) {
+  if (!s || !count || w >= s->nw) return VXP_ERR_ARGUMENT;
+  std::vector<WorldState> ws;
+  RC(readWorlds(s, ws));
+  uint32_t n = ws[w].man_count;
+  *count = n;
+  if (!out || !cap || !n) return VXP_OK;
+  std::vector<DevManifold> tmp(std::min(n, cap));
+  HIP_TRY(hipMemcpy(tmp.data(), s->d_ml[s->cur].as<DevManifold>() + ws[w].man_start,
+                    tmp.size() * sizeof(DevManifold), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < tmp.size(); i++) out[i] = tmp[i].m;
+  return VXP_OK;
+}
+
+int vxp_pairs_get_w(vxp_system* s, uint32_t w, uint64_t* out, uint32_t cap, uint32_t* count) {
+  if (!s || !count || w >= s->nw) return VXP_ERR_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<uint64_t> all(s->n_pairs);
+  if (s->n_pairs)
+    HIP_TRY(hipMemcpy(all.data(), s->d_pairs.p, all.size() * 8, hipMemcpyDeviceToHost));
+  uint32_t n = 0;
+  for (uint64_t k : all) {
+    if (keyWorld(k) != w) continue;
+    if (out && n < cap) out[n] = exportKey(k);
+    n++;
+  }
+  *count = n;
+  return VXP_OK;
+}
+
+int vxp_islands_get_w(vxp_system* s, uint32_t w, uint32_t* ids, uint32_t* labels, uint32_t cap,
+                      uint32_t* count) {
+  if (!s || !count || w >= s->nw) return VXP_ERR_ARGUMENT;
+  if (!s->cfg.enable_islands || !s->d_isl_parent.p) {
+    *count = 0;
+    return VXP_OK;
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<uint32_t> par(s->id_cap), end(s->id_cap);
+  HIP_TRY(hipMemcpy(par.data(), s->d_isl_parent.as<uint32_t>() + (size_t)w * s->id_cap, s->id_cap * 4,
+                    hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(end.data(), s->d_isl_end.as<uint32_t>() + (size_t)w * s->id_cap, s->id_cap * 4,
+                    hipMemcpyDeviceToHost));
+  uint32_t n = 0;
+  for (uint32_t i = 0; i < s->id_cap; i++) {
+    if (!end[i]) continue;
+    if (ids && labels && n < cap) {
+      ids[n] = i;
+      labels[n] = par[i];
+    }
+    n++;
+  }
+  *count = n;
+  return VXP_OK;
+}
+
+int vxp_coloring_get_w(vxp_system* s, uint32_t w, uint8_t* colors, uint32_t cap, uint32_t* count) {
+  if (!s || !count || w >= s->nw) return VXP_ERR_ARGUMENT;
+  if (!s->cfg.enable_coloring || !s->d_ref_colors.p) {
+    *count = 0;
+    return VXP_OK;
+  }
+  std::vector<WorldState> ws;
+  RC(readWorlds(s, ws));
+  uint32_t n = ws[w].man_count;
+  *count = n;
+  if (colors && cap && n)
+    HIP_TRY(hipMemcpy(colors, s->d_ref_colors.as<uint8_t>() + ws[w].man_start, std::min(n, cap),
+                      hipMemcpyDeviceToHost));
+  return VXP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// vxp_dev.hpp — device data model of the step pipeline (structure-of-arrays
+// body state, collider brick pool + dense brick directories, pair / manifold /
+// candidate records) shared by the kernels and the host runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vxp.h"
+#include "vxp_math.hpp"
+
+namespace vxd {
+
+constexpr int kWave = 64;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Collider header.  The brick directory is a dense grid over brick positions
+// floor(min_bound/8) .. floor(max_bound/8) (inclusive: the overlap region loop
+// of src/narrow_phase.zig:759-764 reaches max_bound); each cell holds the pool
+// index of the brick that VoxelCollider.getBrick returns for that position
+// (hashBrickPos lookup, aliasing included), or -1.
+struct DevCollider {
+  float vs;
+  int32_t minb[3], maxb[3];
+  float lo[3], hi[3];  // local AABB
+  int32_t org[3], dim[3];
+  uint32_t dir_off;
+  uint32_t solid;
+};
+
+// Configuration values the kernels read (vxp_config subset, by value).
+struct DevCfg {
+  float gravity[3];
+  float dt;            // fixed step
+  float h;             // dt / substeps
+  float bias_rate, mass_coeff, impulse_coeff;
+  float slop, pcr, mpc, wss;
+  uint32_t substeps, vel_iters, pos_iters;
+  uint32_t max_bodies;   // slots per world
+  uint32_t id_cap;       // id-indexed arrays per world
+  uint32_t cache_cap;
+  uint8_t enable_sleeping, warm_starting, enable_friction, enable_restitution;
+  uint8_t disable_rotation, use_sat, use_cache, enable_islands;
+  uint8_t store_impulses, pad[3];
+};
+
+// Body state, global slot g = world * max_bodies + slot.
+struct DevBodies {
+  float4 *pos, *rot, *lv, *av;
+  float* inv_mass;
+  float* iil;  // 9 per body: local inverse inertia
+  float *fric, *rest, *ldamp, *adamp, *sleep_timer;
+  uint8_t *type, *sleeping, *active;
+  int32_t* collider;
+  uint32_t* id;
+  float4 *fat_lo, *fat_hi;
+  int32_t* idmap;  // [world * id_cap + id] -> slot or -1
+};
+
+struct DevColliders {
+  const DevCollider* hdr;
+  const int32_t* dir;
+  const uint8_t* vox;  // 512 bytes per brick
+};
+
+// Per-world step bookkeeping.
+struct WorldState {
+  uint32_t skip;           // all dynamic bodies asleep: step() returns early
+  uint32_t awake;
+  uint32_t pair_lo, pair_hi;
+  uint32_t man_start, man_count;      // current manifold segment
+  uint32_t prev_start, prev_count;    // previous frame's segment
+  uint32_t produced;
+  uint32_t frame;                     // BrickPairCache.current_frame
+  uint32_t error;
+  uint32_t pad;
+};
+
+// Pair flags
+enum : uint32_t {
+  PF_NONE = 0,     // no manifold this step
+  PF_CARRY = 1,    // both asleep: carry previous manifold
+  PF_DETECT = 2,   // voxel detection, no cache
+  PF_CACHE = 3,    // cache-eligible, before replay
+  PF_HIT = 4,      // cache hit
+  PF_MISS = 5,     // cache miss: detect then fill
+};
+
+struct PairRec {
+  uint32_t world, sa, sb;  // slots within the world
+  uint32_t flag;
+  int32_t mn[3];
+  uint32_t nx, ny, nz;
+  uint32_t nchunks;
+  uint32_t aux;       // PF_CARRY: prev manifold index; HIT/MISS: cache entry
+  uint32_t ida, idb;
+  uint32_t vchk;
+  uint64_t th;        // cache transform hash
+};
+
+// Manifold record (device): pair key + slots + the exported vxp_manifold.
+struct DevManifold {
+  uint64_t key;  // (world << 44) | (id_a << 22) | id_b
+  uint32_t sa, sb;
+  vxp_manifold m;
+};
+
+struct Cand {  // one contact candidate of generateContact
+  float point[3], normal[3], pen;
+  float la[3], lb[3];
+  uint32_t fid_lo, fid_hi;
+  uint32_t pad;
+};
+
+struct CacheKeyRec {
+  uint32_t a, b;
+  uint64_t th;
+  uint64_t last_used;
+  uint32_t valid;
+  uint32_t owner_frame;
+  uint32_t owner_pair;
+  uint32_t pad;
+};
+struct CacheVal {
+  vxp_contact c[4];
+  uint32_t count, has_collision, vchk, pad;
+};
+
+inline __host__ __device__ uint64_t makeKey(uint32_t w, uint32_t a, uint32_t b) {
+  return ((uint64_t)w << 44) | ((uint64_t)a << 22) | (uint64_t)b;
+}
+inline __host__ __device__ uint32_t keyWorld(uint64_t k) { return (uint32_t)(k >> 44); }
+inline __host__ __device__ uint32_t keyA(uint64_t k) { return (uint32_t)((k >> 22) & 0x3FFFFF); }
+inline __host__ __device__ uint32_t keyB(uint64_t k) { return (uint32_t)(k & 0x3FFFFF); }
+inline __host__ __device__ uint64_t exportKey(uint64_t k) {
+  return ((uint64_t)keyA(k) << 32) | keyB(k);
+}
+
+// Solver scratch: per constraint point (4 per manifold), structure of arrays.
+struct SolverScratch {
+  float4 *ws_pos, *ws_rot, *ws_lv, *ws_av;  // per body slot
+  float* ws_ii;                              // 9 per body slot
+  uint32_t *adj_cnt, *adj_off, *adj;         // CSR of awake-dynamic incidences
+  int32_t* color;                            // per manifold
+  uint32_t* order;                           // per manifold: grouped by colour
+  float* pt;                                 // kPtStride floats per point
+};
+constexpr int kPtStride = 40;
+
+}  // namespace vxd

@@ -1,0 +1,1675 @@
+// vxp_kernels.hip — gfx950 kernels of the step pipeline of zig-voxel-physics.
+//
+// One step (src/system.zig:537-555) runs as:
+//   k_step_begin      allDynamicBodiesSleeping per world (src/system.zig:538-567)
+//   k_fat_aabb        broadPhase fat-AABB rule (src/system.zig:570-583, src/broad_phase.zig:175-212)
+//   k_pairs_*         findOverlappingPairs pair set (src/broad_phase.zig:488-526)
+//   k_pair_process    narrowPhase filter + processPair up to detection (src/system.zig:586-655,
+//                     src/narrow_phase.zig:1414-1449, 644-698)
+//   k_cache_replay    BrickPairCache get/put in pair order (src/narrow_phase.zig:228-296,1451-1513)
+//   k_probe           detectVoxelCollisions/checkNeighborhood/generateContact (src/narrow_phase.zig:722-922)
+//   k_reduce          ContactManifold.addContact replay + warm start (src/narrow_phase.zig:451-481,1049-1059)
+//   k_cache_fill, k_scatter, k_copy_skipped   cache values, ContactManager.manifolds compaction
+//   k_islands         buildIslandsWithBodyTypes labels (src/islands.zig:318-441)
+//   k_ref_coloring    GraphColoring.build artifact (src/islands.zig:668-736)
+//   k_solve           ContactSolver.solve per world, per-colour Gauss-Seidel (src/solver.zig:494-958),
+//                     write-back and updateSleepStates (src/system.zig:786-792)
+#include <hip/hip_runtime.h>
+
+#include "vxp_dev.hpp"
+#include "vxp_kernels.hpp"
+
+using namespace vxm;
+namespace vxd {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ V3 f4v(float4 a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ Q4 f4q(float4 a) { return Q4{a.x, a.y, a.z, a.w}; }
+__device__ __forceinline__ float4 v4(V3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+__device__ __forceinline__ float4 q4(Q4 a) { return make_float4(a.x, a.y, a.z, a.w); }
+
+__device__ __forceinline__ uint8_t voxelAt(const DevColliders& C, const DevCollider& h, int32_t x,
+                                           int32_t y, int32_t z) {
+  int32_t bx = (x >> 3) - h.org[0], by = (y >> 3) - h.org[1], bz = (z >> 3) - h.org[2];
+  if ((uint32_t)bx >= (uint32_t)h.dim[0] || (uint32_t)by >= (uint32_t)h.dim[1] ||
+      (uint32_t)bz >= (uint32_t)h.dim[2])
+    return 0;
+  int32_t b = C.dir[h.dir_off + ((uint32_t)bz * h.dim[1] + by) * h.dim[0] + bx];
+  if (b < 0) return 0;
+  return C.vox[(size_t)b * 512 + (x & 7) + 8 * (y & 7) + 64 * (z & 7)];
+}
+
+__device__ __forceinline__ Box colliderBox(const DevCollider& h) {
+  return Box{V3{h.lo[0], h.lo[1], h.lo[2]}, V3{h.hi[0], h.hi[1], h.hi[2]}};
+}
+
+// src/collider.zig:213-220
+__device__ __forceinline__ V3 voxelToLocal(float vs, int32_t x, int32_t y, int32_t z) {
+  float half = vs * 0.5f;
+  return V3{(float)x * vs + half, (float)y * vs + half, (float)z * vs + half};
+}
+// src/collider.zig:223-229
+__device__ __forceinline__ void localToVoxel(float vs, V3 p, int32_t o[3]) {
+  o[0] = (int32_t)floorf(p.x / vs);
+  o[1] = (int32_t)floorf(p.y / vs);
+  o[2] = (int32_t)floorf(p.z / vs);
+}
+
+__device__ __forceinline__ bool shouldCheck(uint32_t ta, uint32_t tb) {
+  if (ta == 0 || tb == 0) return false;
+  if (ta == VXP_VOXEL_CORNER || tb == VXP_VOXEL_CORNER) return true;
+  return ta == VXP_VOXEL_EDGE && tb == VXP_VOXEL_EDGE;
+}
+
+// src/narrow_phase.zig:384-392
+__device__ __forceinline__ uint64_t featureId(int32_t ax, int32_t ay, int32_t az, int32_t bx,
+                                              int32_t by, int32_t bz) {
+  return (uint64_t)((int64_t)ax & 0xFFFF) | ((uint64_t)((int64_t)ay & 0xFFFF) << 16) |
+         ((uint64_t)((int64_t)az & 0xFFFF) << 32) | ((uint64_t)((int64_t)bx & 0xF) << 48) |
+         ((uint64_t)((int64_t)by & 0xF) << 52) | ((uint64_t)((int64_t)bz & 0xF) << 56);
+}
+
+__device__ __forceinline__ int waveInclusiveScan(int x, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    int t = __shfl_up(x, d, kWave);
+    if (lane >= d) x += t;
+  }
+  return x;
+}
+__device__ __forceinline__ uint32_t waveSum(uint32_t x) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// step begin: allDynamicBodiesSleeping, per-step stat reset
+// ---------------------------------------------------------------------------
+__global__ void k_step_begin(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st,
+                             uint32_t n_worlds) {
+  uint32_t w = blockIdx.x;
+  if (w >= n_worlds) return;
+  __shared__ uint32_t awake;
+  if (threadIdx.x == 0) awake = 0;
+  __syncthreads();
+  uint32_t g0 = w * cfg.max_bodies, mine = 0;
+  for (uint32_t s = threadIdx.x; s < cfg.max_bodies; s += blockDim.x) {
+    uint32_t g = g0 + s;
+    if (B.active[g] && B.type[g] == VXP_BODY_DYNAMIC && !B.sleeping[g]) mine++;
+  }
+  if (mine) atomicAdd(&awake, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ws[w].awake = awake;
+    ws[w].skip = (cfg.enable_sleeping && awake == 0) ? 1u : 0u;
+    ws[w].pair_lo = ws[w].pair_hi = 0;
+    ws[w].produced = 0;
+    // ContactManager.beginFrame: the current segment becomes the previous one
+    ws[w].prev_start = ws[w].man_start;
+    ws[w].prev_count = ws[w].man_count;
+    if (!ws[w].skip) {
+      vxp_stats& s = st[w];
+      s.steps_executed++;
+      s.broad_phase_pairs = s.narrow_phase_tested = s.narrow_phase_colliding = 0;
+      s.contacts_generated = s.voxels_checked = s.pairs_skipped_sleeping = 0;
+      s.cache_hits = s.cache_misses = s.manifolds_skipped_sleeping = 0;
+      s.manifolds = s.solver_manifolds = s.colors = s.position_iterations_run = 0;
+      s.max_penetration = 0.0f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// broad phase: fat AABB rule (src/broad_phase.zig:175-212) for awake dynamic bodies
+// ---------------------------------------------------------------------------
+__device__ Box worldBoxOf(const DevBodies& B, const DevColliders& C, uint32_t g) {
+  V3 p = f4v(B.pos[g]);
+  int32_t c = B.collider[g];
+  if (c < 0) return Box{p, p};
+  return boxTransform(colliderBox(C.hdr[c]), p, f4q(B.rot[g]));
+}
+
+__global__ void k_fat_aabb(DevCfg cfg, DevBodies B, DevColliders C, const WorldState* ws,
+                           uint32_t n_slots) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_slots) return;
+  if (ws[g / cfg.max_bodies].skip) return;
+  if (!B.active[g] || B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) return;
+  Box aabb = worldBoxOf(B, C, g);
+  Box fat = boxInflate(aabb, 0.1f);
+  Box old{f4v(B.fat_lo[g]), f4v(B.fat_hi[g])};
+  if (boxContains(old, aabb)) {
+    Box large = boxInflate(fat, 0.1f * 4.0f);
+    if (boxContains(large, old)) return;
+  }
+  V3 d = scale(f4v(B.lv[g]), 2.0f);
+  Box e = fat;
+  if (d.x < 0) e.lo.x += d.x; else e.hi.x += d.x;
+  if (d.y < 0) e.lo.y += d.y; else e.hi.y += d.y;
+  if (d.z < 0) e.lo.z += d.z; else e.hi.z += d.z;
+  B.fat_lo[g] = v4(e.lo);
+  B.fat_hi[g] = v4(e.hi);
+}
+
+// ---------------------------------------------------------------------------
+// pair set: every inclusive overlap of fat AABBs within a world, except
+// static-static; key (world, min id, max id).  Sort-and-sweep: bodies are
+// ordered by (world, fat min.x) and each body scans forward while the next
+// body's min.x <= its max.x.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t orderedFloat(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void k_sweep_keys(DevCfg cfg, DevBodies B, uint64_t* keys, uint32_t* vals,
+                             uint32_t n_slots) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_slots) return;
+  uint32_t w = g / cfg.max_bodies;
+  // inactive slots sort to the end of their world
+  uint32_t k = B.active[g] ? orderedFloat(B.fat_lo[g].x) : 0xFFFFFFFFu;
+  keys[g] = ((uint64_t)w << 32) | k;
+  vals[g] = g;
+}
+
+__global__ void k_sweep_pairs(DevCfg cfg, DevBodies B, const uint64_t* keys, const uint32_t* order,
+                              uint32_t n_slots, uint64_t* out, uint32_t* count, uint32_t cap) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_slots) return;
+  uint32_t gi = order[i];
+  if (!B.active[gi]) return;
+  uint32_t w = gi / cfg.max_bodies;
+  Box bi{f4v(B.fat_lo[gi]), f4v(B.fat_hi[gi])};
+  bool si = B.type[gi] == VXP_BODY_STATIC;
+  uint32_t idi = B.id[gi];
+  for (uint32_t j = i + 1; j < n_slots; j++) {
+    uint32_t gj = order[j];
+    if (gj / cfg.max_bodies != w || !B.active[gj]) break;
+    float x0 = B.fat_lo[gj].x;
+    if (x0 > bi.hi.x) break;
+    Box bj{f4v(B.fat_lo[gj]), f4v(B.fat_hi[gj])};
+    if (!boxOverlaps(bi, bj)) continue;
+    if (si && B.type[gj] == VXP_BODY_STATIC) continue;
+    uint32_t idj = B.id[gj];
+    uint32_t a = min(idi, idj), b = max(idi, idj);
+    uint32_t slot = atomicAdd(count, 1u);
+    if (slot < cap) out[slot] = makeKey(w, a, b);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// narrow phase part 1: filter, SAT, cache key, overlap region
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void projObb(const V3 v[8], V3 ax, float& mn, float& mx) {
+  mn = INFINITY;
+  mx = -INFINITY;
+  for (int i = 0; i < 8; i++) {
+    float p = dot(v[i], ax);
+    mn = fminf(mn, p);
+    mx = fmaxf(mx, p);
+  }
+}
+__device__ __forceinline__ bool sepAxis(const V3 va[8], const V3 vb[8], V3 ax) {
+  float a0, a1, b0, b1;
+  projObb(va, ax, a0, a1);
+  projObb(vb, ax, b0, b1);
+  return a1 < b0 || b1 < a0;
+}
+// satEarlyOut (src/narrow_phase.zig:532-569): true when separated
+__device__ bool satSeparated(const Obb& A, const Obb& Bo) {
+  V3 va[8], vb[8], aa[3], bb[3];
+  obbVertices(A, va);
+  obbVertices(Bo, vb);
+  obbAxes(A, aa);
+  obbAxes(Bo, bb);
+  for (int i = 0; i < 3; i++)
+    if (sepAxis(va, vb, aa[i])) return true;
+  for (int i = 0; i < 3; i++)
+    if (sepAxis(va, vb, bb[i])) return true;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      V3 cr = cross(aa[i], bb[j]);
+      float l = len(cr);
+      if (l < 0.001f) continue;
+      if (sepAxis(va, vb, scale(cr, 1.0f / l))) return true;
+    }
+  return false;
+}
+
+// src/narrow_phase.zig:148-169
+__device__ uint64_t transformHash(V3 rp, Q4 rq) {
+  int32_t qx = (int32_t)(rp.x * 100.0f), qy = (int32_t)(rp.y * 100.0f), qz = (int32_t)(rp.z * 100.0f);
+  int16_t rw = (int16_t)(int32_t)clampz(rq.w * 1000.0f, -32767.0f, 32767.0f);
+  int16_t rx = (int16_t)(int32_t)clampz(rq.x * 1000.0f, -32767.0f, 32767.0f);
+  int16_t ry = (int16_t)(int32_t)clampz(rq.y * 1000.0f, -32767.0f, 32767.0f);
+  int16_t rz = (int16_t)(int32_t)clampz(rq.z * 1000.0f, -32767.0f, 32767.0f);
+  uint64_t h = 0;
+  h = h * 31 + (uint64_t)(uint32_t)qx;
+  h = h * 31 + (uint64_t)(uint32_t)qy;
+  h = h * 31 + (uint64_t)(uint32_t)qz;
+  h = h * 31 + (uint64_t)(uint16_t)rw;
+  h = h * 31 + (uint64_t)(uint16_t)rx;
+  h = h * 31 + (uint64_t)(uint16_t)ry;
+  h = h * 31 + (uint64_t)(uint16_t)rz;
+  return h;
+}
+
+__device__ int32_t findPrev(const DevManifold* prev, uint32_t lo, uint32_t hi, uint64_t key) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint64_t k = prev[mid].key;
+    if (k == key) return (int32_t)mid;
+    if (k < key) lo = mid + 1; else hi = mid;
+  }
+  return -1;
+}
+
+__global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
+                               vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
+                               const DevManifold* prev) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  uint64_t key = keys[p];
+  uint32_t w = keyWorld(key), ida = keyA(key), idb = keyB(key);
+  if (p == 0 || keyWorld(keys[p - 1]) != w) ws[w].pair_lo = p;
+  if (p == n_pairs - 1 || keyWorld(keys[p + 1]) != w) ws[w].pair_hi = p + 1;
+  PairRec r;
+  r.world = w;
+  r.ida = ida;
+  r.idb = idb;
+  r.flag = PF_NONE;
+  r.nchunks = 0;
+  r.vchk = 0;
+  r.aux = kNone;
+  r.th = 0;
+  r.mn[0] = r.mn[1] = r.mn[2] = 0;
+  r.nx = r.ny = r.nz = 0;
+  int32_t sa = B.idmap[(size_t)w * cfg.id_cap + ida];
+  int32_t sb = B.idmap[(size_t)w * cfg.id_cap + idb];
+  r.sa = (uint32_t)sa;
+  r.sb = (uint32_t)sb;
+  if (ws[w].skip || sa < 0 || sb < 0) {
+    pr[p] = r;
+    return;
+  }
+  atomicAdd(&st[w].broad_phase_pairs, 1u);
+  uint32_t ga = w * cfg.max_bodies + sa, gb = w * cfg.max_bodies + sb;
+  bool asl = B.sleeping[ga], bsl = B.sleeping[gb];
+  if (asl && bsl) {
+    atomicAdd(&st[w].pairs_skipped_sleeping, 1u);
+    int32_t j = findPrev(prev, ws[w].prev_start, ws[w].prev_start + ws[w].prev_count, key);
+    if (j >= 0) {
+      r.flag = PF_CARRY;
+      r.aux = (uint32_t)j;
+    }
+    pr[p] = r;
+    return;
+  }
+  atomicAdd(&st[w].narrow_phase_tested, 1u);
+  int32_t ca = B.collider[ga], cb = B.collider[gb];
+  if (ca < 0 || cb < 0) {
+    pr[p] = r;
+    return;
+  }
+  const DevCollider& HA = C.hdr[ca];
+  const DevCollider& HB = C.hdr[cb];
+  V3 pa = f4v(B.pos[ga]), pb = f4v(B.pos[gb]);
+  Q4 qa = f4q(B.rot[ga]), qb = f4q(B.rot[gb]);
+  if (cfg.use_sat) {
+    Obb oa = obbFromBox(colliderBox(HA), pa, qa);
+    Obb ob = obbFromBox(colliderBox(HB), pb, qb);
+    if (satSeparated(oa, ob)) {
+      pr[p] = r;
+      return;
+    }
+  }
+  uint8_t ta = B.type[ga], tb = B.type[gb];
+  bool use_cache = cfg.use_cache && (ta == VXP_BODY_STATIC || tb == VXP_BODY_STATIC || asl || bsl);
+  if (use_cache) {
+    r.th = transformHash(sub(pb, pa), qmul(qconj(qa), qb));
+    r.flag = PF_CACHE;
+  } else {
+    r.flag = PF_DETECT;
+  }
+  // getOverlapRegion (src/narrow_phase.zig:644-698)
+  Obb obb = obbFromBox(colliderBox(HB), pb, qb);
+  V3 vb[8];
+  obbVertices(obb, vb);
+  Q4 inv = qconj(qa);
+  V3 tmin{INFINITY, INFINITY, INFINITY}, tmax{-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < 8; i++) {
+    V3 rel = qrotate(inv, sub(vb[i], pa));
+    tmin = vmin(tmin, rel);
+    tmax = vmax(tmax, rel);
+  }
+  float vs = HA.vs;
+  tmin = sub(tmin, V3{vs, vs, vs});
+  tmax = add(tmax, V3{vs, vs, vs});
+  V3 alo{HA.lo[0], HA.lo[1], HA.lo[2]}, ahi{HA.hi[0], HA.hi[1], HA.hi[2]};
+  V3 mnc = vclamp(tmin, alo, ahi), mxc = vclamp(tmax, alo, ahi);
+  if (!(mnc.x >= mxc.x || mnc.y >= mxc.y || mnc.z >= mxc.z)) {
+    int32_t a[3], b[3];
+    localToVoxel(vs, mnc, a);
+    localToVoxel(vs, mxc, b);
+    for (int k = 0; k < 3; k++) {
+      a[k] = max(a[k], HA.minb[k]);
+      b[k] = min(b[k], HA.maxb[k]);
+    }
+    if (b[0] >= a[0] && b[1] >= a[1] && b[2] >= a[2]) {
+      r.mn[0] = a[0];
+      r.mn[1] = a[1];
+      r.mn[2] = a[2];
+      r.nx = (uint32_t)(b[0] - a[0] + 1);
+      r.ny = (uint32_t)(b[1] - a[1] + 1);
+      r.nz = (uint32_t)(b[2] - a[2] + 1);
+      uint64_t nv = (uint64_t)r.nx * r.ny * r.nz;
+      r.nchunks = (uint32_t)((nv + kWave - 1) / kWave);
+    }
+  }
+  pr[p] = r;
+}
+
+// ---------------------------------------------------------------------------
+// BrickPairCache replay: one thread per world walks its cache-eligible pairs in
+// pair order (get, then put on a miss).  Puts reserve their entry at miss time
+// and record (frame, pair) as owner; values are written by k_cache_fill after
+// the hits have been read.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t cacheHash(uint32_t a, uint32_t b, uint64_t th) {
+  uint64_t h = 0;
+  h = h * 31 + a;
+  h = h * 31 + b;
+  for (int i = 0; i < 6; i++) h = h * 31;
+  return h * 31 + th;
+}
+
+__global__ void k_cache_replay(DevCfg cfg, WorldState* ws, vxp_stats* st, PairRec* pr,
+                               CacheKeyRec* keys, uint32_t n_worlds) {
+  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_worlds || ws[w].skip) return;
+  uint32_t frame = ++ws[w].frame;
+  if (!cfg.use_cache) return;
+  CacheKeyRec* E = keys + (size_t)w * cfg.cache_cap;
+  uint32_t cap = cfg.cache_cap, maxp = cap < 16 ? cap : 16;
+  uint32_t hits = 0, misses = 0;
+  for (uint32_t p = ws[w].pair_lo; p < ws[w].pair_hi; p++) {
+    PairRec& r = pr[p];
+    if (r.flag != PF_CACHE) continue;
+    uint64_t h = cacheHash(r.ida, r.idb, r.th);
+    uint32_t start = (uint32_t)(h % cap), idx = start;
+    int32_t hit = -1;
+    for (uint32_t k = 0; k < maxp; k++) {
+      CacheKeyRec& e = E[idx];
+      if (!e.valid) break;
+      if (e.a == r.ida && e.b == r.idb && e.th == r.th) {
+        hit = (int32_t)idx;
+        e.last_used = frame;
+        break;
+      }
+      idx = (idx + 1) % cap;
+    }
+    if (hit >= 0) {
+      r.flag = PF_HIT;
+      r.aux = (uint32_t)hit;
+      hits++;
+      continue;
+    }
+    misses++;
+    idx = start;
+    uint32_t oldest = start, put = kNone;
+    uint64_t oldest_frame = ~0ull;
+    for (uint32_t k = 0; k < maxp; k++) {
+      CacheKeyRec& e = E[idx];
+      if (!e.valid || (e.a == r.ida && e.b == r.idb && e.th == r.th)) {
+        put = idx;
+        break;
+      }
+      if (e.last_used < oldest_frame) {
+        oldest_frame = e.last_used;
+        oldest = idx;
+      }
+      idx = (idx + 1) % cap;
+    }
+    if (put == kNone) put = oldest;
+    CacheKeyRec& e = E[put];
+    e.a = r.ida;
+    e.b = r.idb;
+    e.th = r.th;
+    e.last_used = frame;
+    e.valid = 1;
+    e.owner_frame = frame;
+    e.owner_pair = p;
+    r.flag = PF_MISS;
+    r.aux = put;
+  }
+  st[w].cache_hits += hits;
+  st[w].cache_misses += misses;
+}
+
+__global__ void k_chunk_counts(const PairRec* pr, uint32_t n_pairs, uint32_t* counts) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  uint32_t f = pr[p].flag;
+  counts[p] = (f == PF_DETECT || f == PF_MISS) ? pr[p].nchunks : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// voxel probe: one wave per 64-voxel chunk of a pair's overlap region (region
+// order z, y, x as src/narrow_phase.zig:759-764; lane = position in the chunk).
+// Each lane runs the 27 probes of checkNeighborhood in (dz, dy, dx) order.
+// Candidates of a chunk are written contiguously in (lane, probe) order, so
+// chunk order + in-chunk order is the reference's candidate sequence.
+// ---------------------------------------------------------------------------
+struct ProbeCtx {
+  V3 pa, pb;
+  Q4 qa, qb;
+  float vsa, vsb, spec;
+  int32_t bminb[3], bmaxb[3];
+};
+
+template <bool kWrite>
+__device__ __forceinline__ uint32_t probeVoxel(const DevColliders& C, const DevCollider& HA,
+                                               const DevCollider& HB, const ProbeCtx& c, int32_t x,
+                                               int32_t y, int32_t z, uint8_t va, uint32_t* vchk,
+                                               Cand* out) {
+  V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
+  V3 lb = invXformPoint(c.pb, c.qb, wa);
+  int32_t cb[3];
+  localToVoxel(c.vsb, lb, cb);
+  uint32_t n = 0, vc = 0;
+  uint32_t ta = va & 7u;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        int32_t bx = cb[0] + dx, by = cb[1] + dy, bz = cb[2] + dz;
+        if (bx < c.bminb[0] || bx >= c.bmaxb[0] || by < c.bminb[1] || by >= c.bmaxb[1] ||
+            bz < c.bminb[2] || bz >= c.bmaxb[2])
+          continue;
+        uint8_t vb = voxelAt(C, HB, bx, by, bz);
+        uint32_t tb = vb & 7u;
+        if (tb == 0) continue;
+        vc++;
+        if (!shouldCheck(ta, tb)) continue;
+        V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
+        V3 diff = sub(wb, wa);
+        float dsq = dot(diff, diff);
+        if (!(dsq < c.spec)) continue;
+        if (kWrite) {
+          // generateContact (src/narrow_phase.zig:860-922)
+          float dist = sqrtf(dsq);
+          V3 cp = scale(add(wa, wb), 0.5f);
+          float pen = c.vsa - dist;
+          V3 na = qrotate(c.qa, lutNormal(va >> 3));
+          V3 nb = qrotate(c.qb, lutNormal(vb >> 3));
+          V3 nrm;
+          if (ta == VXP_VOXEL_FACE) {
+            nrm = na;
+          } else if (tb == VXP_VOXEL_FACE) {
+            nrm = neg(nb);
+          } else if (dist > 0.001f) {
+            V3 dn = scale(diff, 1.0f / dist);
+            if (dot(dn, na) > 0.3f) nrm = na;
+            else if (dot(dn, neg(nb)) > 0.3f) nrm = neg(nb);
+            else nrm = dn;
+          } else {
+            nrm = na;
+          }
+          if (ta == VXP_VOXEL_INSIDE || tb == VXP_VOXEL_INSIDE) pen = fmaxf(pen, c.vsa);
+          V3 la = invXformPoint(c.pa, c.qa, cp);
+          V3 lbb = invXformPoint(c.pb, c.qb, cp);
+          uint64_t fid = featureId(x, y, z, bx, by, bz);
+          Cand& o = out[n];
+          o.point[0] = cp.x; o.point[1] = cp.y; o.point[2] = cp.z;
+          o.normal[0] = nrm.x; o.normal[1] = nrm.y; o.normal[2] = nrm.z;
+          o.pen = pen;
+          o.la[0] = la.x; o.la[1] = la.y; o.la[2] = la.z;
+          o.lb[0] = lbb.x; o.lb[1] = lbb.y; o.lb[2] = lbb.z;
+          o.fid_lo = (uint32_t)fid;
+          o.fid_hi = (uint32_t)(fid >> 32);
+          o.pad = 0;
+        }
+        n++;
+      }
+  if (vchk) *vchk = vc;
+  return n;
+}
+
+__device__ __forceinline__ uint32_t upperBound(const uint32_t* a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColliders C,
+                                               PairRec* pr, uint32_t n_pairs,
+                                               const uint32_t* wi_off, uint32_t total_wi,
+                                               uint32_t* wi_cbase, uint32_t* wi_ccnt,
+                                               uint32_t* wi_vchk, Cand* cand,
+                                               uint32_t* cand_count, uint32_t cand_cap) {
+  uint32_t lane = threadIdx.x & (kWave - 1);
+  uint32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (wid >= total_wi) return;  // whole wave
+  uint32_t p = upperBound(wi_off, n_pairs, wid) - 1;
+  const PairRec& r = pr[p];
+  uint32_t chunk = wid - wi_off[p];
+  uint32_t w = r.world;
+  uint32_t ga = w * cfg.max_bodies + r.sa, gb = w * cfg.max_bodies + r.sb;
+  const DevCollider& HA = C.hdr[B.collider[ga]];
+  const DevCollider& HB = C.hdr[B.collider[gb]];
+  ProbeCtx c;
+  c.pa = f4v(B.pos[ga]);
+  c.pb = f4v(B.pos[gb]);
+  c.qa = f4q(B.rot[ga]);
+  c.qb = f4q(B.rot[gb]);
+  c.vsa = HA.vs;
+  c.vsb = HB.vs;
+  float cds = c.vsa * c.vsa * 1.0f * 1.0f;
+  c.spec = cds + 0.02f * 0.02f + 2.0f * sqrtf(cds) * 0.02f;
+  for (int k = 0; k < 3; k++) {
+    c.bminb[k] = HB.minb[k];
+    c.bmaxb[k] = HB.maxb[k];
+  }
+  uint32_t lin = chunk * kWave + lane;
+  uint32_t nxy = r.nx * r.ny;
+  bool valid = lin < nxy * r.nz;
+  int32_t x = 0, y = 0, z = 0;
+  uint8_t va = 0;
+  if (valid) {
+    x = r.mn[0] + (int32_t)(lin % r.nx);
+    y = r.mn[1] + (int32_t)((lin / r.nx) % r.ny);
+    z = r.mn[2] + (int32_t)(lin / nxy);
+    va = voxelAt(C, HA, x, y, z);
+  }
+  uint32_t vchk = 0, n = 0;
+  if ((va & 7u) != 0) n = probeVoxel<false>(C, HA, HB, c, x, y, z, va, &vchk, nullptr);
+  int incl = waveInclusiveScan((int)n, (int)lane);
+  uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
+  uint32_t vsum = waveSum(vchk);
+  uint32_t base = 0;
+  if (total > 0) {
+    if (lane == 0) base = atomicAdd(cand_count, total);
+    base = (uint32_t)__shfl((int)base, 0, kWave);
+    if (n > 0 && base + total <= cand_cap)
+      probeVoxel<true>(C, HA, HB, c, x, y, z, va, nullptr, cand + base + (uint32_t)incl - n);
+  }
+  if (lane == 0) {
+    wi_cbase[wid] = base;
+    wi_ccnt[wid] = total;
+    wi_vchk[wid] = vsum;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// manifold reduction: ContactManifold.addContact replay over the candidates
+// in sequence order, warm start against the previous frame, cache hit/carry.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void candToContact(const Cand& c, vxp_contact& o) {
+  o.point[0] = c.point[0]; o.point[1] = c.point[1]; o.point[2] = c.point[2];
+  o.normal[0] = c.normal[0]; o.normal[1] = c.normal[1]; o.normal[2] = c.normal[2];
+  o.penetration = c.pen;
+  o.local_anchor_a[0] = c.la[0]; o.local_anchor_a[1] = c.la[1]; o.local_anchor_a[2] = c.la[2];
+  o.local_anchor_b[0] = c.lb[0]; o.local_anchor_b[1] = c.lb[1]; o.local_anchor_b[2] = c.lb[2];
+  o.normal_impulse = 0.0f;
+  o.tangent_impulse[0] = o.tangent_impulse[1] = 0.0f;
+  o.normal_mass = 0.0f;
+  o.tangent_mass[0] = o.tangent_mass[1] = 0.0f;
+  o.pad = 0;
+  o.feature_id = ((uint64_t)c.fid_hi << 32) | c.fid_lo;
+}
+
+// ContactManager.addManifold warm start (src/narrow_phase.zig:1011-1021,1049-1059)
+__device__ void warmStartFrom(vxp_manifold& m, const DevManifold* prev, int32_t j) {
+  if (j < 0) return;
+  const vxp_manifold& o = prev[j].m;
+  for (uint32_t i = 0; i < m.contact_count; i++)
+    for (uint32_t k = 0; k < o.contact_count; k++)
+      if (m.contacts[i].feature_id == o.contacts[k].feature_id) {
+        m.contacts[i].normal_impulse = o.contacts[k].normal_impulse;
+        m.contacts[i].tangent_impulse[0] = o.contacts[k].tangent_impulse[0];
+        m.contacts[i].tangent_impulse[1] = o.contacts[k].tangent_impulse[1];
+        break;
+      }
+}
+
+__global__ void k_reduce(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st, PairRec* pr,
+                         const uint64_t* keys, uint32_t n_pairs, const uint32_t* wi_off,
+                         const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
+                         const uint32_t* wi_vchk, const Cand* cand,
+                         const DevManifold* prev, const CacheVal* cvals, CacheVal* cstage,
+                         DevManifold* staging, uint32_t* produce) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const PairRec& r = pr[p];
+  uint32_t f = r.flag;
+  produce[p] = 0;
+  if (f == PF_NONE || f == PF_CACHE) return;
+  uint32_t w = r.world;
+  uint64_t key = keys[p];
+  if (f == PF_CARRY) {
+    staging[p] = prev[r.aux];
+    produce[p] = 1;
+    atomicAdd(&ws[w].produced, 1u);
+    return;
+  }
+  uint32_t ga = w * cfg.max_bodies + r.sa, gb = w * cfg.max_bodies + r.sb;
+  DevManifold d;
+  d.key = key;
+  d.sa = r.sa;
+  d.sb = r.sb;
+  vxp_manifold& m = d.m;
+  m.body_a_id = r.ida;
+  m.body_b_id = r.idb;
+  m.friction = sqrtf(B.fric[ga] * B.fric[gb]);
+  m.restitution = fmaxf(B.rest[ga], B.rest[gb]);
+  m.contact_count = 0;
+  m.color = kNone;
+  uint32_t vchk;
+  if (f == PF_HIT) {
+    const CacheVal& v = cvals[(size_t)w * cfg.cache_cap + r.aux];
+    if (!v.has_collision) return;
+    for (uint32_t i = 0; i < v.count; i++) m.contacts[i] = v.c[i];
+    m.contact_count = v.count;
+    vchk = v.vchk;
+  } else {
+    // streaming top-4 (src/narrow_phase.zig:451-481) over candidate indices
+    uint32_t slot[4];
+    float spen[4];
+    uint64_t sfid[4];
+    uint32_t n = 0;
+    uint32_t w0 = wi_off[p];
+    uint32_t w1 = w0 + r.nchunks;
+    uint32_t vsum = 0;
+    for (uint32_t wi = w0; wi < w1; wi++) {
+      vsum += wi_vchk[wi];
+      uint32_t cnt = wi_ccnt[wi], base = wi_cbase[wi];
+      for (uint32_t k = 0; k < cnt; k++) {
+        const Cand& c = cand[base + k];
+        uint64_t fid = ((uint64_t)c.fid_hi << 32) | c.fid_lo;
+        float pen = c.pen;
+        bool done = false;
+        for (uint32_t i = 0; i < n; i++)
+          if (sfid[i] == fid) {
+            slot[i] = base + k;
+            spen[i] = pen;
+            done = true;
+            break;
+          }
+        if (done) continue;
+        if (n < 4) {
+          slot[n] = base + k;
+          spen[n] = pen;
+          sfid[n] = fid;
+          n++;
+          continue;
+        }
+        float minp = pen;
+        int32_t mi = -1;
+        for (uint32_t i = 0; i < 4; i++)
+          if (spen[i] < minp) {
+            minp = spen[i];
+            mi = (int32_t)i;
+          }
+        if (mi >= 0) {
+          slot[mi] = base + k;
+          spen[mi] = pen;
+          sfid[mi] = fid;
+        }
+      }
+    }
+    for (uint32_t i = 0; i < n; i++) candToContact(cand[slot[i]], m.contacts[i]);
+    m.contact_count = n;
+    vchk = vsum;
+    if (f == PF_MISS) {
+      CacheVal& cv = cstage[p];
+      cv.count = n;
+      cv.has_collision = n > 0;
+      cv.vchk = vchk;
+      for (uint32_t i = 0; i < n; i++) cv.c[i] = m.contacts[i];
+    }
+  }
+  atomicAdd(&st[w].voxels_checked, vchk);
+  if (m.contact_count == 0) return;
+  atomicAdd(&st[w].narrow_phase_colliding, 1u);
+  atomicAdd(&st[w].contacts_generated, m.contact_count);
+  int32_t j = findPrev(prev, ws[w].prev_start, ws[w].prev_start + ws[w].prev_count, key);
+  warmStartFrom(m, prev, j);
+  staging[p] = d;
+  produce[p] = 1;
+  atomicAdd(&ws[w].produced, 1u);
+}
+
+__global__ void k_cache_fill(DevCfg cfg, const WorldState* ws, const PairRec* pr, uint32_t n_pairs,
+                             const CacheKeyRec* keys, CacheVal* vals, const CacheVal* cstage) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const PairRec& r = pr[p];
+  if (r.flag != PF_MISS) return;
+  size_t e = (size_t)r.world * cfg.cache_cap + r.aux;
+  if (keys[e].owner_frame == ws[r.world].frame && keys[e].owner_pair == p) vals[e] = cstage[p];
+}
+
+// New manifold segments: skipped worlds keep their previous segment.
+__global__ void k_world_segments(WorldState* ws, uint32_t n_worlds, uint32_t* total) {
+  // single block: exclusive scan of per-world counts
+  __shared__ uint32_t carry;
+  __shared__ uint32_t buf[1024];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n_worlds; base += blockDim.x) {
+    uint32_t w = base + threadIdx.x;
+    uint32_t c = 0;
+    if (w < n_worlds) c = ws[w].skip ? ws[w].prev_count : ws[w].produced;
+    buf[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
+      uint32_t t = threadIdx.x >= d ? buf[threadIdx.x - d] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (w < n_worlds) {
+      ws[w].man_start = carry + buf[threadIdx.x] - c;
+      ws[w].man_count = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += buf[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scatter(const WorldState* ws, const PairRec* pr, uint32_t n_pairs,
+                          const uint32_t* produce, const uint32_t* pidx,
+                          const DevManifold* staging, DevManifold* out) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs || !produce[p]) return;
+  uint32_t w = pr[p].world;
+  uint32_t dst = ws[w].man_start + pidx[p] - pidx[ws[w].pair_lo];
+  out[dst] = staging[p];
+}
+
+__global__ void k_copy_skipped(const WorldState* ws, const DevManifold* prev, DevManifold* out) {
+  uint32_t w = blockIdx.x;
+  if (!ws[w].skip) return;
+  for (uint32_t i = threadIdx.x; i < ws[w].prev_count; i += blockDim.x)
+    out[ws[w].man_start + i] = prev[ws[w].prev_start + i];
+}
+
+// ---------------------------------------------------------------------------
+// islands: union of manifold endpoints across edges with no static body;
+// label = minimum body id of the component (src/islands.zig:318-441).
+// One block per world; hooking + pointer jumping on id-indexed parents.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const WorldState* ws,
+                                                  vxp_stats* st, const DevManifold* ml,
+                                                  uint32_t* parent, uint32_t* endpoint,
+                                                  uint32_t* size) {
+  uint32_t w = blockIdx.x;
+  if (ws[w].skip) return;
+  uint32_t* P = parent + (size_t)w * cfg.id_cap;
+  uint32_t* E = endpoint + (size_t)w * cfg.id_cap;
+  uint32_t* S = size + (size_t)w * cfg.id_cap;
+  uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
+  for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x) {
+    P[i] = i;
+    E[i] = 0;
+    S[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+    E[ml[m0 + i].m.body_a_id] = 1;
+    E[ml[m0 + i].m.body_b_id] = 1;
+  }
+  __shared__ uint32_t changed;
+  for (int round = 0; round < 100000; round++) {
+    __syncthreads();
+    if (threadIdx.x == 0) changed = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+      const DevManifold& d = ml[m0 + i];
+      uint32_t ga = w * cfg.max_bodies + d.sa, gb = w * cfg.max_bodies + d.sb;
+      if (B.type[ga] == VXP_BODY_STATIC || B.type[gb] == VXP_BODY_STATIC) continue;
+      uint32_t ra = d.m.body_a_id, rb = d.m.body_b_id;
+      while (true) {
+        uint32_t t = __atomic_load_n(&P[ra], __ATOMIC_RELAXED);
+        if (t == ra) break;
+        ra = t;
+      }
+      while (true) {
+        uint32_t t = __atomic_load_n(&P[rb], __ATOMIC_RELAXED);
+        if (t == rb) break;
+        rb = t;
+      }
+      if (ra != rb) {
+        atomicMin(&P[max(ra, rb)], min(ra, rb));
+        changed = 1;
+      }
+    }
+    __syncthreads();
+    // pointer jumping
+    for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x) {
+      uint32_t r = i;
+      while (true) {
+        uint32_t t = __atomic_load_n(&P[r], __ATOMIC_RELAXED);
+        if (t == r) break;
+        r = t;
+      }
+      P[i] = r;
+    }
+    __syncthreads();
+    if (!changed) break;
+  }
+  __syncthreads();
+  __shared__ uint32_t n_isl, largest;
+  if (threadIdx.x == 0) {
+    n_isl = 0;
+    largest = 0;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x)
+    if (E[i]) atomicAdd(&S[P[i]], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x)
+    if (S[i]) {
+      atomicAdd(&n_isl, 1u);
+      atomicMax(&largest, S[i]);
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st[w].island_count = n_isl;
+    st[w].largest_island_size = largest;
+  }
+}
+
+// GraphColoring.build (src/islands.zig:668-736): sequential greedy over the
+// manifolds in pair-key order; static bodies count; colours saturate at 255.
+__global__ void k_ref_coloring(DevCfg cfg, const WorldState* ws, const DevManifold* ml,
+                               uint64_t* masks, uint8_t* colors) {
+  uint32_t w = blockIdx.x;
+  if (ws[w].skip || threadIdx.x != 0) return;
+  uint64_t* Mk = masks + (size_t)w * cfg.id_cap * 4;
+  for (uint32_t i = 0; i < cfg.id_cap * 4; i++) Mk[i] = 0;
+  uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
+  for (uint32_t i = 0; i < M; i++) {
+    uint32_t a = ml[m0 + i].m.body_a_id, b = ml[m0 + i].m.body_b_id;
+    uint32_t c = 255;
+    for (uint32_t q = 0; q < 4; q++) {
+      uint64_t used = Mk[a * 4 + q] | Mk[b * 4 + q];
+      if (~used) {
+        c = q * 64 + (uint32_t)__builtin_ctzll(~used);
+        break;
+      }
+    }
+    if (c > 255) c = 255;
+    colors[m0 + i] = (uint8_t)c;
+    Mk[a * 4 + c / 64] |= 1ull << (c % 64);
+    Mk[b * 4 + c / 64] |= 1ull << (c % 64);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// solver: ContactSolver.solve for one world per block (src/solver.zig:494-561)
+// ---------------------------------------------------------------------------
+struct PtView {
+  float* f;
+  __device__ V3 g3(int o) const { return V3{f[o], f[o + 1], f[o + 2]}; }
+  __device__ void s3(int o, V3 v) const {
+    f[o] = v.x;
+    f[o + 1] = v.y;
+    f[o + 2] = v.z;
+  }
+};
+enum {
+  PT_POINT = 0, PT_NORMAL = 3, PT_LA = 6, PT_LB = 9, PT_RA = 12, PT_RB = 15, PT_BASE = 18,
+  PT_NM = 19, PT_TM0 = 20, PT_TM1 = 21, PT_T1 = 22, PT_T2 = 25, PT_BIAS = 28, PT_NI = 29,
+  PT_TI0 = 30, PT_TI1 = 31, PT_VN0 = 32, PT_PEN = 33
+};
+
+struct BodyRef {
+  float4 *pos, *rot, *lv, *av;
+  float* ii;
+  uint32_t g;
+  float im;
+  uint8_t type, sleeping;
+};
+
+__device__ __forceinline__ M3 loadM3(const float* p) {
+  M3 r;
+  for (int i = 0; i < 9; i++) r.m[i] = p[i];
+  return r;
+}
+__device__ __forceinline__ void storeM3(float* p, const M3& m) {
+  for (int i = 0; i < 9; i++) p[i] = m.m[i];
+}
+
+__device__ __forceinline__ V3 pointVel(const SolverScratch& S, uint32_t g, V3 pt) {
+  return add(f4v(S.ws_lv[g]), cross(f4v(S.ws_av[g]), sub(pt, f4v(S.ws_pos[g]))));
+}
+// BodyState.applyImpulseAtPoint (src/solver.zig:316-322)
+__device__ __forceinline__ void applyImpulse(const SolverScratch& S, const DevBodies& B, uint32_t g,
+                                             V3 imp, V3 r) {
+  if (B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) return;
+  float im = B.inv_mass[g];
+  S.ws_lv[g] = v4(add(f4v(S.ws_lv[g]), scale(imp, im)));
+  M3 ii = loadM3(S.ws_ii + (size_t)g * 9);
+  S.ws_av[g] = v4(add(f4v(S.ws_av[g]), mulVec(ii, cross(r, imp))));
+}
+
+// src/solver.zig:967-981
+__device__ __forceinline__ V3 computeOrthogonal(V3 v) {
+  if (fabsf(v.x) < fabsf(v.y)) {
+    if (fabsf(v.x) < fabsf(v.z)) return normalizeOrZero(V3{0.0f, -v.z, v.y});
+    return normalizeOrZero(V3{-v.y, v.x, 0.0f});
+  }
+  if (fabsf(v.y) < fabsf(v.z)) return normalizeOrZero(V3{-v.z, 0.0f, v.x});
+  return normalizeOrZero(V3{-v.y, v.x, 0.0f});
+}
+
+__device__ void blockScanExclusive(uint32_t* data, uint32_t n, uint32_t* tmp /*blockDim*/,
+                                   uint32_t* total) {
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < n ? data[i] : 0;
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
+      uint32_t t = threadIdx.x >= d ? tmp[threadIdx.x - d] : 0;
+      __syncthreads();
+      tmp[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) data[i] = carry + tmp[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += tmp[threadIdx.x];
+    __syncthreads();
+  }
+  if (total && threadIdx.x == 0) *total = carry;
+  __syncthreads();
+}
+
+constexpr uint32_t kMaxColors = 1024;
+
+__global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldState* ws,
+                                                vxp_stats* st, DevManifold* ml, SolverScratch S,
+                                                uint32_t* prio_buf) {
+  uint32_t w = blockIdx.x;
+  if (ws[w].skip) return;
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
+  const uint32_t MB = cfg.max_bodies, g0 = w * MB;
+  const uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
+  __shared__ uint32_t tmp[1024];
+  __shared__ uint32_t ccount[kMaxColors + 1];
+  __shared__ uint32_t ccursor[kMaxColors];
+  __shared__ uint32_t s_left, s_ncol, s_nsolve, s_skipped;
+  __shared__ float s_red[1024];
+
+  // body working state (BodyState.fromBody, src/solver.zig:231-243)
+  for (uint32_t s = tid; s < MB; s += bs) {
+    uint32_t g = g0 + s;
+    S.adj_cnt[g] = 0;
+    if (!B.active[g]) continue;
+    float4 rot = B.rot[g];
+    S.ws_pos[g] = B.pos[g];
+    S.ws_rot[g] = rot;
+    S.ws_lv[g] = B.lv[g];
+    S.ws_av[g] = B.av[g];
+    M3 ii;
+    if (B.inv_mass[g] == 0.0f) {
+      for (int i = 0; i < 9; i++) ii.m[i] = 0.0f;
+    } else {
+      ii = worldInvInertia(f4q(rot), loadM3(B.iil + (size_t)g * 9));
+    }
+    storeM3(S.ws_ii + (size_t)g * 9, ii);
+  }
+  if (tid == 0) {
+    s_nsolve = 0;
+    s_skipped = 0;
+    s_ncol = 0;
+  }
+  __syncthreads();
+  // solvable manifolds (src/system.zig:687-702) and awake-dynamic incidence counts
+  for (uint32_t i = tid; i < M; i += bs) {
+    DevManifold& d = ml[m0 + i];
+    uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+    bool asl = B.sleeping[ga], bsl = B.sleeping[gb];
+    d.m.color = kNone;
+    if (asl && bsl) {
+      S.color[m0 + i] = -2;
+      atomicAdd(&s_skipped, 1u);
+      continue;
+    }
+    atomicAdd(&s_nsolve, 1u);
+    S.color[m0 + i] = -1;
+    prio_buf[m0 + i] = colorPrio(exportKey(d.key));
+    if (B.type[ga] == VXP_BODY_DYNAMIC && !asl) atomicAdd(&S.adj_cnt[ga], 1u);
+    if (B.type[gb] == VXP_BODY_DYNAMIC && !bsl) atomicAdd(&S.adj_cnt[gb], 1u);
+  }
+  __syncthreads();
+  // CSR offsets (relative to 2 * m0)
+  for (uint32_t s = tid; s < MB; s += bs) S.adj_off[g0 + s] = S.adj_cnt[g0 + s];
+  __syncthreads();
+  blockScanExclusive(S.adj_off + g0, MB, tmp, nullptr);
+  for (uint32_t s = tid; s < MB; s += bs) S.adj_cnt[g0 + s] = 0;
+  __syncthreads();
+  uint32_t* adj = S.adj + 2 * (size_t)m0;
+  for (uint32_t i = tid; i < M; i += bs) {
+    if (S.color[m0 + i] != -1) continue;
+    const DevManifold& d = ml[m0 + i];
+    uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+    if (B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga])
+      adj[S.adj_off[ga] + atomicAdd(&S.adj_cnt[ga], 1u)] = i;
+    if (B.type[gb] == VXP_BODY_DYNAMIC && !B.sleeping[gb])
+      adj[S.adj_off[gb] + atomicAdd(&S.adj_cnt[gb], 1u)] = i;
+  }
+  __syncthreads();
+  // Jones-Plassmann rounds: a manifold takes the smallest colour unused by its
+  // lower-(prio, key) neighbours once they are all coloured (= greedy in
+  // ascending (prio, key) order).
+  for (int round = 0; round < 1000000; round++) {
+    if (tid == 0) s_left = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < M; i += bs) {
+      if (__atomic_load_n(&S.color[m0 + i], __ATOMIC_RELAXED) != -1) continue;
+      const DevManifold& d = ml[m0 + i];
+      uint32_t pm = prio_buf[m0 + i];
+      uint64_t km = d.key;
+      bool ready = true;
+      uint64_t used[kMaxColors / 64];
+      for (uint32_t q = 0; q < kMaxColors / 64; q++) used[q] = 0;
+      uint32_t ends[2] = {g0 + d.sa, g0 + d.sb};
+      for (int e = 0; e < 2 && ready; e++) {
+        uint32_t g = ends[e];
+        if (B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) continue;
+        uint32_t o = S.adj_off[g], c = S.adj_cnt[g];
+        for (uint32_t k = 0; k < c; k++) {
+          uint32_t n = adj[o + k];
+          if (n == i) continue;
+          uint32_t pn = prio_buf[m0 + n];
+          uint64_t kn = ml[m0 + n].key;
+          if (pn < pm || (pn == pm && kn < km)) {
+            int32_t cn = __atomic_load_n(&S.color[m0 + n], __ATOMIC_RELAXED);
+            if (cn < 0) {
+              ready = false;
+              break;
+            }
+            if (cn < (int32_t)kMaxColors) used[cn >> 6] |= 1ull << (cn & 63);
+          }
+        }
+      }
+      if (!ready) {
+        s_left = 1;
+        continue;
+      }
+      uint32_t col = kMaxColors;
+      for (uint32_t q = 0; q < kMaxColors / 64; q++)
+        if (~used[q]) {
+          col = q * 64 + (uint32_t)__builtin_ctzll(~used[q]);
+          break;
+        }
+      if (col >= kMaxColors) {
+        col = kMaxColors - 1;
+        atomicOr(&ws[w].error, 1u);
+      }
+      __atomic_store_n(&S.color[m0 + i], (int32_t)col, __ATOMIC_RELAXED);
+      atomicMax(&s_ncol, col + 1);
+    }
+    __syncthreads();
+    if (!s_left) break;
+    __syncthreads();
+  }
+  __syncthreads();
+  const uint32_t ncol = s_ncol;
+  // group manifolds by colour
+  for (uint32_t c = tid; c <= kMaxColors; c += bs) ccount[c] = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < M; i += bs) {
+    int32_t c = S.color[m0 + i];
+    if (c >= 0) {
+      atomicAdd(&ccount[c], 1u);
+      ml[m0 + i].m.color = (uint32_t)c;
+    }
+  }
+  __syncthreads();
+  blockScanExclusive(ccount, ncol + 1, tmp, nullptr);
+  for (uint32_t c = tid; c < ncol; c += bs) ccursor[c] = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < M; i += bs) {
+    int32_t c = S.color[m0 + i];
+    if (c >= 0) S.order[m0 + ccount[c] + atomicAdd(&ccursor[c], 1u)] = i;
+  }
+  // ContactSolver.addManifold (src/solver.zig:449-491)
+  for (uint32_t i = tid; i < M; i += bs) {
+    if (S.color[m0 + i] < 0) continue;
+    const DevManifold& d = ml[m0 + i];
+    uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+    V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
+    Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+    for (uint32_t k = 0; k < d.m.contact_count; k++) {
+      const vxp_contact& c = d.m.contacts[k];
+      PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+      V3 la{c.local_anchor_a[0], c.local_anchor_a[1], c.local_anchor_a[2]};
+      V3 lb{c.local_anchor_b[0], c.local_anchor_b[1], c.local_anchor_b[2]};
+      V3 n{c.normal[0], c.normal[1], c.normal[2]};
+      V3 wa = add(qrotate(qa, la), pa);
+      V3 wb = add(qrotate(qb, lb), pb);
+      V3 ad = sub(wb, wa);
+      P.s3(PT_POINT, V3{c.point[0], c.point[1], c.point[2]});
+      P.s3(PT_NORMAL, n);
+      P.s3(PT_LA, la);
+      P.s3(PT_LB, lb);
+      P.s3(PT_RA, sub(wa, pa));
+      P.s3(PT_RB, sub(wb, pb));
+      P.f[PT_BASE] = -c.penetration - dot(ad, n);
+      P.f[PT_NM] = 0.0f;
+      P.f[PT_TM0] = P.f[PT_TM1] = 0.0f;
+      P.s3(PT_T1, V3{0.f, 0.f, 0.f});
+      P.s3(PT_T2, V3{0.f, 0.f, 0.f});
+      P.f[PT_BIAS] = 0.0f;
+      P.f[PT_NI] = c.normal_impulse;
+      P.f[PT_TI0] = c.tangent_impulse[0];
+      P.f[PT_TI1] = c.tangent_impulse[1];
+      P.f[PT_VN0] = 0.0f;
+      P.f[PT_PEN] = c.penetration;
+    }
+  }
+  __syncthreads();
+  const bool has_c = s_nsolve > 0;
+  const float h = cfg.h;
+  const V3 grav{cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]};
+  const float dt_inv = h > kEps ? 1.0f / h : 0.0f;
+
+  for (uint32_t sub_i = 0; sub_i < cfg.substeps; sub_i++) {
+    // applyGravity (src/solver.zig:564-605)
+    for (uint32_t s = tid; s < MB; s += bs) {
+      uint32_t g = g0 + s;
+      if (!B.active[g] || B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) continue;
+      V3 lv = add(f4v(S.ws_lv[g]), scale(grav, h));
+      V3 av = f4v(S.ws_av[g]);
+      float ld = B.ldamp[g], ad = B.adamp[g];
+      float ls = len(lv), as = len(av);
+      if (ls < 0.5f) ld += 5.0f * (1.0f - (ls / 0.5f));
+      if (as < 0.5f) ad += 5.0f * (1.0f - (as / 0.5f));
+      float ldec = fmaxf(0.0f, 1.0f - ld * h), adec = fmaxf(0.0f, 1.0f - ad * h);
+      lv = scale(lv, ldec);
+      av = scale(av, adec);
+      if (dot(lv, lv) < 0.001f * 0.001f) lv = V3{0.f, 0.f, 0.f};
+      if (dot(av, av) < 0.001f * 0.001f) av = V3{0.f, 0.f, 0.f};
+      S.ws_lv[g] = v4(lv);
+      S.ws_av[g] = v4(av);
+    }
+    __syncthreads();
+    if (has_c) {
+      // prepareConstraints (src/solver.zig:608-699)
+      for (uint32_t i = tid; i < M; i += bs) {
+        if (S.color[m0 + i] < 0) continue;
+        const DevManifold& d = ml[m0 + i];
+        uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+        V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
+        Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+        float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
+        M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
+        for (uint32_t k = 0; k < d.m.contact_count; k++) {
+          PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+          V3 n = P.g3(PT_NORMAL);
+          V3 wa = add(qrotate(qa, P.g3(PT_LA)), pa);
+          V3 wb = add(qrotate(qb, P.g3(PT_LB)), pb);
+          V3 ra = sub(wa, pa), rb = sub(wb, pb);
+          P.s3(PT_RA, ra);
+          P.s3(PT_RB, rb);
+          V3 pt = scale(add(wa, wb), 0.5f);
+          P.s3(PT_POINT, pt);
+          V3 ad = sub(wb, wa);
+          float sep = P.f[PT_BASE] + dot(ad, n);
+          V3 rna = cross(ra, n), rnb = cross(rb, n);
+          float k_ = ima + imb;
+          k_ += dot(rna, mulVec(iia, rna));
+          k_ += dot(rnb, mulVec(iib, rnb));
+          P.f[PT_NM] = k_ > kEps ? 1.0f / k_ : 0.0f;
+          V3 rel = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+          float vn = dot(rel, n);
+          P.f[PT_VN0] = vn;
+          float bias;
+          if (sep > 0.0f) bias = sep * dt_inv;
+          else bias = fmaxf(cfg.bias_rate * sep, -0.2f);
+          if (cfg.enable_restitution && vn < -1.0f) bias = fmaxf(bias, -d.m.restitution * vn);
+          P.f[PT_BIAS] = bias;
+          if (cfg.enable_friction) {
+            V3 t1 = computeOrthogonal(n);
+            V3 t2 = cross(n, t1);
+            P.s3(PT_T1, t1);
+            P.s3(PT_T2, t2);
+            V3 a1 = cross(ra, t1), b1 = cross(rb, t1);
+            float k1 = ima + imb;
+            k1 += dot(a1, mulVec(iia, a1));
+            k1 += dot(b1, mulVec(iib, b1));
+            P.f[PT_TM0] = k1 > kEps ? 1.0f / k1 : 0.0f;
+            V3 a2 = cross(ra, t2), b2 = cross(rb, t2);
+            float k2 = ima + imb;
+            k2 += dot(a2, mulVec(iia, a2));
+            k2 += dot(b2, mulVec(iib, b2));
+            P.f[PT_TM1] = k2 > kEps ? 1.0f / k2 : 0.0f;
+          }
+        }
+      }
+      __syncthreads();
+      // warmStart (src/solver.zig:702-729), substep 0 only
+      if (cfg.warm_starting && sub_i == 0) {
+        for (uint32_t c = 0; c < ncol; c++) {
+          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
+            uint32_t i = S.order[m0 + j];
+            const DevManifold& d = ml[m0 + i];
+            uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+            for (uint32_t k = 0; k < d.m.contact_count; k++) {
+              PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+              float sn = P.f[PT_NI] * cfg.wss, s1 = P.f[PT_TI0] * cfg.wss, s2 = P.f[PT_TI1] * cfg.wss;
+              V3 imp = add(add(scale(P.g3(PT_NORMAL), sn), scale(P.g3(PT_T1), s1)), scale(P.g3(PT_T2), s2));
+              applyImpulse(S, B, ga, neg(imp), P.g3(PT_RA));
+              applyImpulse(S, B, gb, imp, P.g3(PT_RB));
+              P.f[PT_NI] = sn;
+              P.f[PT_TI0] = s1;
+              P.f[PT_TI1] = s2;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // solveVelocityConstraints with bias (src/solver.zig:732-787)
+      for (uint32_t it = 0; it < cfg.vel_iters; it++) {
+        for (uint32_t c = 0; c < ncol; c++) {
+          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
+            uint32_t i = S.order[m0 + j];
+            const DevManifold& d = ml[m0 + i];
+            uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+            float fr = d.m.friction;
+            for (uint32_t k = 0; k < d.m.contact_count; k++) {
+              PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+              V3 n = P.g3(PT_NORMAL), pt = P.g3(PT_POINT), ra = P.g3(PT_RA), rb = P.g3(PT_RB);
+              V3 rel = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+              float vn = dot(rel, n);
+              float ni = P.f[PT_NI];
+              float imp = -cfg.mass_coeff * P.f[PT_NM] * (vn + P.f[PT_BIAS]) - cfg.impulse_coeff * ni;
+              float nw = fmaxf(ni + imp, 0.0f);
+              imp = nw - ni;
+              P.f[PT_NI] = nw;
+              V3 Pn = scale(n, imp);
+              applyImpulse(S, B, ga, neg(Pn), ra);
+              applyImpulse(S, B, gb, Pn, rb);
+              if (cfg.enable_friction && fr > 0.0f) {
+                // solveFriction (src/solver.zig:790-838)
+                V3 rel1 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+                float maxf = fr * P.f[PT_NI];
+                V3 t1 = P.g3(PT_T1), t2 = P.g3(PT_T2);
+                float vt = dot(rel1, t1);
+                float fi = -P.f[PT_TM0] * vt;
+                float old = P.f[PT_TI0];
+                float nt = clampz(old + fi, -maxf, maxf);
+                P.f[PT_TI0] = nt;
+                fi = nt - old;
+                V3 i1 = scale(t1, fi);
+                applyImpulse(S, B, ga, neg(i1), ra);
+                applyImpulse(S, B, gb, i1, rb);
+                V3 rel2 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+                float vt2 = dot(rel2, t2);
+                float fi2 = -P.f[PT_TM1] * vt2;
+                float old2 = P.f[PT_TI1];
+                float nt2 = clampz(old2 + fi2, -maxf, maxf);
+                P.f[PT_TI1] = nt2;
+                fi2 = nt2 - old2;
+                V3 i2 = scale(t2, fi2);
+                applyImpulse(S, B, ga, neg(i2), ra);
+                applyImpulse(S, B, gb, i2, rb);
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+    // integratePositions (src/solver.zig:341-356,841-845)
+    for (uint32_t s = tid; s < MB; s += bs) {
+      uint32_t g = g0 + s;
+      if (!B.active[g] || B.type[g] == VXP_BODY_STATIC || B.sleeping[g]) continue;
+      V3 p = add(f4v(S.ws_pos[g]), scale(f4v(S.ws_lv[g]), h));
+      S.ws_pos[g] = v4(p);
+      if (!cfg.disable_rotation) {
+        Q4 q = integrateRotation(f4q(S.ws_rot[g]), f4v(S.ws_av[g]), h);
+        S.ws_rot[g] = q4(q);
+      }
+      if (B.type[g] == VXP_BODY_DYNAMIC)
+        storeM3(S.ws_ii + (size_t)g * 9,
+                worldInvInertia(f4q(S.ws_rot[g]), loadM3(B.iil + (size_t)g * 9)));
+    }
+    __syncthreads();
+    // relaxConstraints (src/solver.zig:848-856): hard, unbiased
+    if (has_c) {
+      for (uint32_t c = 0; c < ncol; c++) {
+        for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
+          uint32_t i = S.order[m0 + j];
+          const DevManifold& d = ml[m0 + i];
+          uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+          float fr = d.m.friction;
+          for (uint32_t k = 0; k < d.m.contact_count; k++) {
+            PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+            V3 n = P.g3(PT_NORMAL), pt = P.g3(PT_POINT), ra = P.g3(PT_RA), rb = P.g3(PT_RB);
+            V3 rel = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+            float vn = dot(rel, n);
+            float ni = P.f[PT_NI];
+            float imp = -P.f[PT_NM] * vn;
+            float nw = fmaxf(ni + imp, 0.0f);
+            imp = nw - ni;
+            P.f[PT_NI] = nw;
+            V3 Pn = scale(n, imp);
+            applyImpulse(S, B, ga, neg(Pn), ra);
+            applyImpulse(S, B, gb, Pn, rb);
+            if (cfg.enable_friction && fr > 0.0f) {
+              V3 rel1 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+              float maxf = fr * P.f[PT_NI];
+              V3 t1 = P.g3(PT_T1), t2 = P.g3(PT_T2);
+              float vt = dot(rel1, t1);
+              float fi = -P.f[PT_TM0] * vt;
+              float old = P.f[PT_TI0];
+              float nt = clampz(old + fi, -maxf, maxf);
+              P.f[PT_TI0] = nt;
+              fi = nt - old;
+              V3 i1 = scale(t1, fi);
+              applyImpulse(S, B, ga, neg(i1), ra);
+              applyImpulse(S, B, gb, i1, rb);
+              V3 rel2 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+              float vt2 = dot(rel2, t2);
+              float fi2 = -P.f[PT_TM1] * vt2;
+              float old2 = P.f[PT_TI1];
+              float nt2 = clampz(old2 + fi2, -maxf, maxf);
+              P.f[PT_TI1] = nt2;
+              fi2 = nt2 - old2;
+              V3 i2 = scale(t2, fi2);
+              applyImpulse(S, B, ga, neg(i2), ra);
+              applyImpulse(S, B, gb, i2, rb);
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // solvePositionConstraints (src/solver.zig:860-923) with the early exit
+  float maxpen = 0.0f;
+  uint32_t pos_run = 0;
+  if (has_c) {
+    for (uint32_t it = 0; it < cfg.pos_iters; it++) {
+      pos_run++;
+      float minsep = 0.0f;
+      for (uint32_t c = 0; c < ncol; c++) {
+        for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
+          uint32_t i = S.order[m0 + j];
+          const DevManifold& d = ml[m0 + i];
+          uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+          uint8_t ta = B.type[ga], tb = B.type[gb];
+          if (ta == VXP_BODY_STATIC && tb == VXP_BODY_STATIC) continue;
+          float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
+          bool awa = ta == VXP_BODY_DYNAMIC && !B.sleeping[ga];
+          bool awb = tb == VXP_BODY_DYNAMIC && !B.sleeping[gb];
+          for (uint32_t k = 0; k < d.m.contact_count; k++) {
+            PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+            V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
+            Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+            V3 n = P.g3(PT_NORMAL);
+            V3 wa = add(qrotate(qa, P.g3(PT_LA)), pa);
+            V3 wb = add(qrotate(qb, P.g3(PT_LB)), pb);
+            V3 ad = sub(wb, wa);
+            float sep = P.f[PT_BASE] + dot(ad, n);
+            minsep = fminf(minsep, sep);
+            maxpen = fmaxf(maxpen, -sep);
+            if (sep >= -cfg.slop) continue;
+            float corr = fmaxf(cfg.pcr * (sep + cfg.slop), -cfg.mpc);
+            V3 ra = sub(wa, pa), rb = sub(wb, pb);
+            V3 rna = cross(ra, n), rnb = cross(rb, n);
+            M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
+            float k_ = ima + imb;
+            k_ += dot(rna, mulVec(iia, rna));
+            k_ += dot(rnb, mulVec(iib, rnb));
+            if (k_ < kEps) continue;
+            V3 Pi = scale(n, -corr / k_);
+            if (awa) {
+              S.ws_pos[ga] = v4(sub(pa, scale(Pi, ima)));
+              S.ws_rot[ga] = q4(integrateRotationSmall(qa, mulVec(iia, cross(ra, neg(Pi)))));
+            }
+            if (awb) {
+              S.ws_pos[gb] = v4(add(pb, scale(Pi, imb)));
+              S.ws_rot[gb] = q4(integrateRotationSmall(qb, mulVec(iib, cross(rb, Pi))));
+            }
+          }
+        }
+        __syncthreads();
+      }
+      s_red[tid] = minsep;
+      __syncthreads();
+      for (uint32_t d = bs / 2; d > 0; d >>= 1) {
+        if (tid < d) s_red[tid] = fminf(s_red[tid], s_red[tid + d]);
+        __syncthreads();
+      }
+      bool done = s_red[0] >= -3.0f * cfg.slop;
+      __syncthreads();
+      if (done) break;
+    }
+  }
+  s_red[tid] = maxpen;
+  __syncthreads();
+  for (uint32_t d = bs / 2; d > 0; d >>= 1) {
+    if (tid < d) s_red[tid] = fmaxf(s_red[tid], s_red[tid + d]);
+    __syncthreads();
+  }
+  // writeBackBodies (src/solver.zig:246-296,926-930)
+  for (uint32_t s = tid; s < MB; s += bs) {
+    uint32_t g = g0 + s;
+    if (!B.active[g] || B.type[g] == VXP_BODY_STATIC) continue;
+    V3 p = f4v(S.ws_pos[g]);
+    Q4 q = f4q(S.ws_rot[g]);
+    V3 lv = f4v(S.ws_lv[g]), av = f4v(S.ws_av[g]);
+    bool fq = isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w);
+    if (!finite3(p) || !fq) {
+      V3 op = f4v(B.pos[g]);
+      p = finite3(op) ? op : V3{0.0f, 2.0f, 0.0f};
+      q = Q4{0.f, 0.f, 0.f, 1.f};
+      lv = V3{0.f, 0.f, 0.f};
+      av = V3{0.f, 0.f, 0.f};
+    }
+    if (!finite3(lv)) lv = V3{0.f, 0.f, 0.f};
+    if (!finite3(av)) av = V3{0.f, 0.f, 0.f};
+    float ls = dot(lv, lv), as = dot(av, av);
+    if (ls > 50.0f * 50.0f) lv = scale(lv, 50.0f / sqrtf(ls));
+    if (as > 30.0f * 30.0f) av = scale(av, 30.0f / sqrtf(as));
+    if (ls < 0.002f) lv = V3{0.f, 0.f, 0.f};
+    if (as < 0.002f) av = V3{0.f, 0.f, 0.f};
+    q = qnormalize(q);
+    B.pos[g] = v4(p);
+    B.rot[g] = q4(q);
+    B.lv[g] = v4(lv);
+    B.av[g] = v4(av);
+  }
+  // storeImpulses (src/solver.zig:933-958)
+  if (has_c && cfg.store_impulses) {
+    for (uint32_t i = tid; i < M; i += bs) {
+      if (S.color[m0 + i] < 0) continue;
+      DevManifold& d = ml[m0 + i];
+      for (uint32_t k = 0; k < d.m.contact_count; k++) {
+        PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+        vxp_contact& c = d.m.contacts[k];
+        c.normal_impulse = P.f[PT_NI];
+        c.tangent_impulse[0] = P.f[PT_TI0];
+        c.tangent_impulse[1] = P.f[PT_TI1];
+        c.normal_mass = P.f[PT_NM];
+        c.tangent_mass[0] = P.f[PT_TM0];
+        c.tangent_mass[1] = P.f[PT_TM1];
+      }
+    }
+  }
+  __syncthreads();
+  // updateSleepStates (src/system.zig:786-792, src/body.zig:415-427)
+  if (cfg.enable_sleeping) {
+    for (uint32_t s = tid; s < MB; s += bs) {
+      uint32_t g = g0 + s;
+      if (!B.active[g] || B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) continue;
+      V3 lv = f4v(B.lv[g]), av = f4v(B.av[g]);
+      if (len(lv) < 0.1f && len(av) < 0.1f) {
+        float t = B.sleep_timer[g] + cfg.dt;
+        B.sleep_timer[g] = t;
+        if (t >= 0.5f) {
+          B.sleeping[g] = 1;
+          B.lv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+          B.av[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      } else {
+        B.sleep_timer[g] = 0.0f;
+      }
+    }
+  }
+  if (tid == 0) {
+    vxp_stats& sx = st[w];
+    sx.max_penetration = s_red[0];
+    sx.position_iterations_run = pos_run;
+    sx.manifolds = M;
+    sx.solver_manifolds = s_nsolve;
+    sx.manifolds_skipped_sleeping = s_skipped;
+    sx.colors = ncol;
+  }
+}
+
+__global__ void k_update_end(DevCfg cfg, DevBodies B, vxp_stats* st, uint32_t n_worlds,
+                             uint32_t substeps_this_frame, float acc) {
+  uint32_t w = blockIdx.x;
+  if (w >= n_worlds) return;
+  __shared__ uint32_t sl, ac;
+  if (threadIdx.x == 0) sl = ac = 0;
+  __syncthreads();
+  uint32_t g0 = w * cfg.max_bodies, a = 0, s_ = 0;
+  for (uint32_t s = threadIdx.x; s < cfg.max_bodies; s += blockDim.x) {
+    uint32_t g = g0 + s;
+    if (!B.active[g] || B.type[g] != VXP_BODY_DYNAMIC) continue;
+    if (B.sleeping[g]) s_++; else a++;
+  }
+  if (a) atomicAdd(&ac, a);
+  if (s_) atomicAdd(&sl, s_);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st[w].sleeping_bodies = sl;
+    st[w].active_bodies = ac;
+    st[w].substeps_this_frame = substeps_this_frame;
+    st[w].accumulated_time = acc;
+  }
+}
+
+// PhysicsStats.reset (src/system.zig:129-154) at the start of update()
+__global__ void k_update_begin(vxp_stats* st, uint32_t n_worlds, float acc) {
+  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_worlds) return;
+  vxp_stats& s = st[w];
+  uint32_t tb = s.total_bodies, db = s.dynamic_bodies, sb = s.static_bodies;
+  vxp_stats z = {};
+  z.total_bodies = tb;
+  z.dynamic_bodies = db;
+  z.static_bodies = sb;
+  z.accumulated_time = acc;
+  s = z;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+void launch_update_begin(hipStream_t s, vxp_stats* st, uint32_t nw, float acc) {
+  k_update_begin<<<cdiv(nw, 256), 256, 0, s>>>(st, nw, acc);
+}
+void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       vxp_stats* st, uint32_t nw) {
+  k_step_begin<<<nw, 256, 0, s>>>(cfg, B, ws, st, nw);
+}
+void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
+                     const WorldState* ws, uint32_t n_slots) {
+  k_fat_aabb<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, C, ws, n_slots);
+}
+void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint64_t* keys,
+                       uint32_t* vals, uint32_t n_slots) {
+  k_sweep_keys<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, keys, vals, n_slots);
+}
+void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint64_t* keys,
+                        const uint32_t* order, uint32_t n_slots, uint64_t* out, uint32_t* count,
+                        uint32_t cap) {
+  k_sweep_pairs<<<cdiv(n_slots, 128), 128, 0, s>>>(cfg, B, keys, order, n_slots, out, count, cap);
+}
+void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                         const DevColliders& C, WorldState* ws, vxp_stats* st,
+                         const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev) {
+  if (n) k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev);
+}
+void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
+                         PairRec* pr, CacheKeyRec* keys, uint32_t nw) {
+  k_cache_replay<<<cdiv(nw, 64), 64, 0, s>>>(cfg, ws, st, pr, keys, nw);
+}
+void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts) {
+  if (n) k_chunk_counts<<<cdiv(n, 256), 256, 0, s>>>(pr, n, counts);
+}
+void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
+                  PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
+                  uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, Cand* cand,
+                  uint32_t* cand_count, uint32_t cand_cap) {
+  if (total_wi)
+    k_probe<<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, n_pairs, wi_off, total_wi, wi_cbase,
+                                              wi_ccnt, wi_vchk, cand, cand_count, cand_cap);
+}
+void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                   vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
+                   const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
+                   const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
+                   const CacheVal* cvals, CacheVal* cstage, DevManifold* staging,
+                   uint32_t* produce) {
+  if (n)
+    k_reduce<<<cdiv(n, 64), 64, 0, s>>>(cfg, B, ws, st, pr, keys, n, wi_off, wi_cbase, wi_ccnt,
+                                        wi_vchk, cand, prev, cvals, cstage, staging, produce);
+}
+void launch_cache_fill(hipStream_t s, const DevCfg& cfg, const WorldState* ws, const PairRec* pr,
+                       uint32_t n, const CacheKeyRec* keys, CacheVal* vals, const CacheVal* cstage) {
+  if (n) k_cache_fill<<<cdiv(n, 256), 256, 0, s>>>(cfg, ws, pr, n, keys, vals, cstage);
+}
+void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total) {
+  k_world_segments<<<1, 1024, 0, s>>>(ws, nw, total);
+}
+void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
+                    const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
+                    DevManifold* out) {
+  if (n) k_scatter<<<cdiv(n, 256), 256, 0, s>>>(ws, pr, n, produce, pidx, staging, out);
+}
+void launch_copy_skipped(hipStream_t s, const WorldState* ws, const DevManifold* prev,
+                         DevManifold* out, uint32_t nw) {
+  k_copy_skipped<<<nw, 256, 0, s>>>(ws, prev, out);
+}
+void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                    vxp_stats* st, const DevManifold* ml, uint32_t* parent, uint32_t* endpoint,
+                    uint32_t* size, uint32_t nw) {
+  k_islands<<<nw, 1024, 0, s>>>(cfg, B, ws, st, ml, parent, endpoint, size);
+}
+void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
+                         const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw) {
+  k_ref_coloring<<<nw, 64, 0, s>>>(cfg, ws, ml, masks, colors);
+}
+void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                  vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
+                  uint32_t nw) {
+  k_solve<<<nw, 1024, 0, s>>>(cfg, B, ws, st, ml, S, prio);
+}
+void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
+                       uint32_t nw, uint32_t substeps, float acc) {
+  k_update_end<<<nw, 256, 0, s>>>(cfg, B, st, nw, substeps, acc);
+}
+
+}  // namespace vxd

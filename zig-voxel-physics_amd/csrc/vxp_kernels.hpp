@@ -1,0 +1,65 @@
+// vxp_kernels.hpp — host-callable launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "vxp_dev.hpp"
+
+namespace vxd {
+
+void launch_update_begin(hipStream_t s, vxp_stats* st, uint32_t nw, float acc);
+void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       vxp_stats* st, uint32_t nw);
+void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
+                     const WorldState* ws, uint32_t n_slots);
+void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint64_t* keys,
+                       uint32_t* vals, uint32_t n_slots);
+void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint64_t* keys,
+                        const uint32_t* order, uint32_t n_slots, uint64_t* out, uint32_t* count,
+                        uint32_t cap);
+void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                         const DevColliders& C, WorldState* ws, vxp_stats* st,
+                         const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev);
+void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
+                         PairRec* pr, CacheKeyRec* keys, uint32_t nw);
+void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts);
+void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
+                  PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
+                  uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, Cand* cand,
+                  uint32_t* cand_count, uint32_t cand_cap);
+void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                   vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
+                   const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
+                   const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
+                   const CacheVal* cvals, CacheVal* cstage, DevManifold* staging,
+                   uint32_t* produce);
+void launch_cache_fill(hipStream_t s, const DevCfg& cfg, const WorldState* ws, const PairRec* pr,
+                       uint32_t n, const CacheKeyRec* keys, CacheVal* vals, const CacheVal* cstage);
+void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total);
+void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
+                    const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
+                    DevManifold* out);
+void launch_copy_skipped(hipStream_t s, const WorldState* ws, const DevManifold* prev,
+                         DevManifold* out, uint32_t nw);
+void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                    vxp_stats* st, const DevManifold* ml, uint32_t* parent, uint32_t* endpoint,
+                    uint32_t* size, uint32_t nw);
+void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
+                         const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw);
+void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                  vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
+                  uint32_t nw);
+void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
+                       uint32_t nw, uint32_t substeps, float acc);
+
+// sort / scan primitives (vxp_prims.hip)
+size_t sort_pairs_u64_u32_bytes(uint32_t n);
+hipError_t sort_pairs_u64_u32(void* tmp, size_t bytes, const uint64_t* kin, uint64_t* kout,
+                              const uint32_t* vin, uint32_t* vout, uint32_t n, hipStream_t s);
+size_t sort_keys_u64_bytes(uint32_t n);
+hipError_t sort_keys_u64(void* tmp, size_t bytes, const uint64_t* kin, uint64_t* kout, uint32_t n,
+                         hipStream_t s);
+size_t exclusive_scan_u32_bytes(uint32_t n);
+hipError_t exclusive_scan_u32(void* tmp, size_t bytes, const uint32_t* in, uint32_t* out,
+                              uint32_t n, hipStream_t s);
+
+}  // namespace vxd

@@ -1,0 +1,205 @@
+"""Python mirror of the reference's `VoxelPhysicsSystem` API over the C ABI.
+
+Method names and argument meaning follow src/system.zig (createDynamicBody,
+createStaticBody, update, step, getPosition, ...).  The class is the same over
+the product library (HIP pipeline) and, for the checker, over the CPU
+restatement; `VoxelPhysicsSystem()` binds the product and fails loudly when the
+HIP library is not built.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+class VoxelCollider:
+    """VoxelCollider built on the host (src/collider.zig:144-409), uploaded on
+    `VoxelPhysicsSystem.addCollider`.  Voxels are set per coordinate like
+    setCollisionVoxel; bricks are 8^3 and keyed by floor-div brick position."""
+
+    def __init__(self, voxel_size=0.1):
+        self.voxel_size = float(voxel_size)
+        self.bricks = {}
+
+    def setCollisionVoxel(self, x, y, z, voxel_byte):
+        bp = (x // 8, y // 8, z // 8)
+        b = self.bricks.get(bp)
+        if b is None:
+            b = np.zeros(512, dtype=np.uint8)
+            self.bricks[bp] = b
+        b[(x % 8) + 8 * (y % 8) + 64 * (z % 8)] = voxel_byte
+
+    def arrays(self):
+        keys = sorted(self.bricks, key=lambda p: (p[2], p[1], p[0]))
+        pos = np.array(keys, dtype=np.int32).reshape(-1, 3)
+        vox = np.stack([self.bricks[k] for k in keys]) if keys else np.zeros((0, 512), np.uint8)
+        return pos, vox
+
+
+class VoxelPhysicsSystem:
+    def __init__(self, config=None, n_worlds=1, lib=None, **over):
+        self.lib = lib if lib is not None else _abi.product_lib()
+        self.cfg = config if config is not None else _abi.default_config(**over)
+        self.n_worlds = n_worlds
+        self.ptr = C.c_void_p()
+        self.lib.check(self.lib.system_create(C.byref(self.cfg), n_worlds, C.byref(self.ptr)),
+                       "system_create")
+
+    def close(self):
+        if self.ptr:
+            self.lib.system_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- colliders / bodies (src/system.zig:294-433) -------------------------
+    def addCollider(self, collider=None, voxel_size=None, brick_pos=None, voxels=None):
+        if collider is not None:
+            brick_pos, voxels = collider.arrays()
+            voxel_size = collider.voxel_size
+        bp = np.ascontiguousarray(brick_pos, dtype=np.int32)
+        vx = np.ascontiguousarray(voxels, dtype=np.uint8)
+        out = C.c_uint32()
+        self.lib.check(self.lib.collider_create(self.ptr, voxel_size, len(bp),
+                                                bp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                vx.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                C.byref(out)), "collider_create")
+        return out.value
+
+    def _create(self, t, position, rotation, collider, density=1.0, world=0):
+        h = _abi.Handle()
+        pos = (C.c_float * 3)(*position)
+        rot = (C.c_float * 4)(*rotation)
+        self.lib.check(self.lib.body_create_w(self.ptr, world, t, pos, rot, collider, density,
+                                              C.byref(h)), "body_create")
+        return h
+
+    def createDynamicBody(self, position, rotation, collider, density, world=0):
+        return self._create(_abi.DYNAMIC, position, rotation, collider, density, world)
+
+    def createStaticBody(self, position, rotation, collider, world=0):
+        return self._create(_abi.STATIC, position, rotation, collider, 1.0, world)
+
+    def createKinematicBody(self, position, rotation, collider, world=0):
+        return self._create(_abi.KINEMATIC, position, rotation, collider, 1.0, world)
+
+    def destroyBody(self, handle, world=0):
+        return self.lib.body_destroy_w(self.ptr, world, handle)
+
+    # ---- stepping (src/system.zig:508-555) ------------------------------------
+    def update(self, dt):
+        self.lib.check(self.lib.system_update(self.ptr, dt), "system_update")
+
+    def step(self, dt):
+        self.lib.check(self.lib.system_step(self.ptr, dt), "system_step")
+
+    def update_frames(self, dt, frames):
+        self.lib.check(self.lib.system_update_frames(self.ptr, dt, frames), "system_update_frames")
+
+    # ---- read-back ------------------------------------------------------------
+    def bodies(self, world=0):
+        """Body states of `world` in slot order as a structured numpy array."""
+        n = self.cfg.max_bodies
+        arr = (_abi.BodyState * n)()
+        cnt = C.c_uint32()
+        self.lib.check(self.lib.bodies_get_w(self.ptr, world, arr, n, C.byref(cnt)), "bodies_get")
+        out = np.frombuffer(arr, dtype=np.dtype([
+            ("position", "<f4", 3), ("rotation", "<f4", 4), ("linear_velocity", "<f4", 3),
+            ("angular_velocity", "<f4", 3), ("sleep_timer", "<f4"), ("id", "<u4"),
+            ("body_type", "u1"), ("is_sleeping", "u1"), ("active", "u1"), ("pad", "u1")]))
+        return out[: cnt.value].copy()
+
+    def stats(self, world=0):
+        s = _abi.Stats()
+        self.lib.check(self.lib.stats_get_w(self.ptr, world, C.byref(s)), "stats_get")
+        return s.as_dict()
+
+    def manifolds(self, world=0):
+        cnt = C.c_uint32()
+        self.lib.check(self.lib.manifolds_get_w(self.ptr, world, None, 0, C.byref(cnt)), "manifolds")
+        n = cnt.value
+        arr = (_abi.Manifold * max(n, 1))()
+        self.lib.check(self.lib.manifolds_get_w(self.ptr, world, arr, n, C.byref(cnt)), "manifolds")
+        return [arr[i] for i in range(n)]
+
+    def manifold_array(self, world=0):
+        """Manifolds as a raw structured array (byte-exact view of vxp_manifold)."""
+        ms = self.manifolds(world)
+        if not ms:
+            return np.zeros(0, dtype=np.uint8).reshape(0, C.sizeof(_abi.Manifold))
+        buf = b"".join(bytes(m) for m in ms)
+        return np.frombuffer(buf, dtype=np.uint8).reshape(len(ms), C.sizeof(_abi.Manifold))
+
+    def pairs(self, world=0):
+        cnt = C.c_uint32()
+        self.lib.check(self.lib.pairs_get_w(self.ptr, world, None, 0, C.byref(cnt)), "pairs")
+        n = cnt.value
+        arr = np.zeros(max(n, 1), dtype=np.uint64)
+        self.lib.check(self.lib.pairs_get_w(self.ptr, world,
+                                            arr.ctypes.data_as(C.POINTER(C.c_uint64)), n,
+                                            C.byref(cnt)), "pairs")
+        return arr[:n]
+
+    def islands(self, world=0):
+        cnt = C.c_uint32()
+        self.lib.check(self.lib.islands_get_w(self.ptr, world, None, None, 0, C.byref(cnt)), "islands")
+        n = cnt.value
+        ids = np.zeros(max(n, 1), dtype=np.uint32)
+        lab = np.zeros(max(n, 1), dtype=np.uint32)
+        self.lib.check(self.lib.islands_get_w(self.ptr, world,
+                                              ids.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                              lab.ctypes.data_as(C.POINTER(C.c_uint32)), n,
+                                              C.byref(cnt)), "islands")
+        return ids[:n], lab[:n]
+
+    def coloring(self, world=0):
+        cnt = C.c_uint32()
+        self.lib.check(self.lib.coloring_get_w(self.ptr, world, None, 0, C.byref(cnt)), "coloring")
+        n = cnt.value
+        arr = np.zeros(max(n, 1), dtype=np.uint8)
+        self.lib.check(self.lib.coloring_get_w(self.ptr, world,
+                                               arr.ctypes.data_as(C.POINTER(C.c_uint8)), n,
+                                               C.byref(cnt)), "coloring")
+        return arr[:n]
+
+    def setState(self, handle, position=None, rotation=None, linear_velocity=None,
+                 angular_velocity=None, world=0):
+        def arr(v, n):
+            return None if v is None else (C.c_float * n)(*v)
+        self.lib.check(self.lib.body_set_state_w(self.ptr, world, handle, arr(position, 3),
+                                                 arr(rotation, 4), arr(linear_velocity, 3),
+                                                 arr(angular_velocity, 3)), "body_set_state")
+
+    # reference-named accessors (src/system.zig:967-1008)
+    def getPosition(self, handle, world=0):
+        return tuple(self.bodies(world)[handle.index]["position"])
+
+    def getLinearVelocity(self, handle, world=0):
+        return tuple(self.bodies(world)[handle.index]["linear_velocity"])
+
+    def isSleeping(self, handle, world=0):
+        return bool(self.bodies(world)[handle.index]["is_sleeping"])
+
+
+def from_scene(scene, lib=None, cfg_over=None, worlds=None):
+    """Build a VoxelPhysicsSystem holding a generated scene."""
+    from .scenes import build_system
+    lib = lib if lib is not None else _abi.product_lib()
+    cfg = dict(scene.config)
+    if cfg_over:
+        cfg.update(cfg_over)
+    wl = scene.worlds if worlds is None else [scene.worlds[i] for i in worlds]
+    cfg["max_bodies"] = max(cfg.get("max_bodies", 0), max(len(w) for w in wl))
+    sysobj = VoxelPhysicsSystem.__new__(VoxelPhysicsSystem)
+    sysobj.lib = lib
+    sysobj.cfg = _abi.default_config(**cfg)
+    sysobj.n_worlds = len(wl)
+    ptr, handles = build_system(lib, scene, cfg_over=cfg, worlds=worlds)
+    sysobj.ptr = ptr
+    sysobj.handles = handles
+    return sysobj
