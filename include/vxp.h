@@ -104,6 +104,8 @@ typedef struct vxp_stats {
   uint32_t island_count, largest_island_size, substeps_this_frame;
   uint32_t manifolds, solver_manifolds, colors, position_iterations_run;
   uint32_t steps_executed;
+  uint32_t awake_bodies_stepped;  /* sum over executed steps of awake dynamic bodies */
+  uint32_t solver_points;         /* contact points in solved manifolds (last step) */
   float max_penetration;
   float accumulated_time;
 } vxp_stats;
@@ -212,6 +214,24 @@ int vxp_islands_get_w(vxp_system* sys, uint32_t world, uint32_t* body_ids, uint3
  * manifolds in pair-key order (needs enable_coloring). */
 int vxp_coloring_get_w(vxp_system* sys, uint32_t world, uint8_t* colors, uint32_t capacity,
                        uint32_t* count);
+
+/* Kernel timing (HIP events on the pipeline's stream), for measurement.
+ * Kernel ids: */
+#define VXP_K_STEP_BEGIN 0
+#define VXP_K_FAT_AABB 1
+#define VXP_K_PAIRS 2
+#define VXP_K_PAIR_PROCESS 3
+#define VXP_K_CACHE_REPLAY 4
+#define VXP_K_PROBE 5
+#define VXP_K_REDUCE 6
+#define VXP_K_COMPACT 7
+#define VXP_K_ISLANDS 8
+#define VXP_K_SOLVE 9
+#define VXP_K_COUNT 10
+/* Enable (1) or disable (0) timing; enabling clears the accumulators. */
+int vxp_profile_enable(vxp_system* sys, int on);
+/* Accumulated device time (ms) and launch count of kernel `id`. */
+int vxp_profile_get(vxp_system* sys, uint32_t id, double* total_ms, uint64_t* launches);
 
 /* Last error message of this thread (never NULL). */
 const char* vxp_last_error(void);

@@ -913,6 +913,56 @@ struct Solver {
 };
 
 // ---------------------------------------------------------------------------
+// src/islands.zig — labels and the reference colouring as free functions
+// ---------------------------------------------------------------------------
+// UnionFind over manifold endpoints, uniting across edges where neither body is
+// static (src/islands.zig:25-138,369-380); the label of a body is the minimum
+// body id of its component.  Returns (body id, label) ascending by id.
+std::vector<std::pair<uint32_t, uint32_t>> islandLabels(
+    const std::vector<std::pair<uint32_t, uint32_t>>& edges,
+    const std::function<bool(uint32_t)>& isStatic) {
+  std::map<uint32_t, uint32_t> parent;
+  std::function<uint32_t(uint32_t)> findp = [&](uint32_t x) {
+    while (parent[x] != x) x = parent[x];
+    return x;
+  };
+  for (auto& e : edges) {
+    parent.emplace(e.first, e.first);
+    parent.emplace(e.second, e.second);
+  }
+  for (auto& e : edges) {
+    if (isStatic(e.first) || isStatic(e.second)) continue;
+    uint32_t ra = findp(e.first), rb = findp(e.second);
+    if (ra == rb) continue;
+    if (ra < rb) parent[rb] = ra; else parent[ra] = rb;  // root = min id
+  }
+  std::vector<std::pair<uint32_t, uint32_t>> out;
+  for (auto& kv : parent) out.push_back({kv.first, findp(kv.first)});
+  return out;
+}
+
+// GraphColoring.build (src/islands.zig:668-736) over manifolds in the given
+// order: static bodies count, colours saturate at 255.
+std::vector<uint8_t> refColoring(const std::vector<std::pair<uint32_t, uint32_t>>& ms) {
+  std::vector<uint8_t> colors;
+  std::unordered_map<uint32_t, std::vector<uint32_t>> body_m;
+  for (uint32_t i = 0; i < ms.size(); i++) {
+    body_m[ms[i].first].push_back(i);
+    body_m[ms[i].second].push_back(i);
+  }
+  for (uint32_t i = 0; i < ms.size(); i++) {
+    bool used[256] = {false};
+    for (uint32_t id : {ms[i].first, ms[i].second})
+      for (uint32_t j : body_m[id])
+        if (j < colors.size()) used[colors[j]] = true;
+    uint32_t c = 0;
+    while (c < 255 && used[c]) c++;
+    colors.push_back((uint8_t)c);
+  }
+  return colors;
+}
+
+// ---------------------------------------------------------------------------
 // src/system.zig — VoxelPhysicsSystem
 // ---------------------------------------------------------------------------
 struct Shared {
@@ -1327,27 +1377,11 @@ struct World {
       Body* b = find(id);
       return b == nullptr || b->type == VXP_BODY_STATIC;
     };
-    std::map<uint32_t, uint32_t> parent;
-    std::function<uint32_t(uint32_t)> findp = [&](uint32_t x) {
-      while (parent[x] != x) x = parent[x];
-      return x;
-    };
-    for (auto& kv : cur) {
-      parent.emplace(kv.second.a, kv.second.a);
-      parent.emplace(kv.second.b, kv.second.b);
-    }
-    for (auto& kv : cur) {
-      if (isStatic(kv.second.a) || isStatic(kv.second.b)) continue;
-      uint32_t ra = findp(kv.second.a), rb = findp(kv.second.b);
-      if (ra == rb) continue;
-      if (ra < rb) parent[rb] = ra; else parent[ra] = rb;  // root = min id
-    }
+    std::vector<std::pair<uint32_t, uint32_t>> edges;
+    for (auto& kv : cur) edges.push_back({kv.second.a, kv.second.b});
+    island_labels = islandLabels(edges, isStatic);
     std::map<uint32_t, uint32_t> sizes;
-    for (auto& kv : parent) {
-      uint32_t r = findp(kv.first);
-      island_labels.push_back({kv.first, r});
-      sizes[r]++;
-    }
+    for (auto& kv : island_labels) sizes[kv.second]++;
     mgr_islands = (uint32_t)sizes.size();
     for (auto& kv : sizes) mgr_largest = std::max(mgr_largest, kv.second);
   }
@@ -1355,23 +1389,9 @@ struct World {
   // GraphColoring.build (src/islands.zig:668-736) over the current manifolds
   // in pair-key order: static bodies count, colours saturate at 255.
   void buildRefColoring() {
-    ref_colors.clear();
-    std::unordered_map<uint32_t, std::vector<uint32_t>> body_m;
-    std::vector<const Manifold*> ms;
-    for (auto& kv : cur) ms.push_back(&kv.second);
-    for (uint32_t i = 0; i < ms.size(); i++) {
-      body_m[ms[i]->a].push_back(i);
-      body_m[ms[i]->b].push_back(i);
-    }
-    for (uint32_t i = 0; i < ms.size(); i++) {
-      bool used[256] = {false};
-      for (uint32_t id : {ms[i]->a, ms[i]->b})
-        for (uint32_t j : body_m[id])
-          if (j < ref_colors.size()) used[ref_colors[j]] = true;
-      uint32_t c = 0;
-      while (c < 255 && used[c]) c++;
-      ref_colors.push_back((uint8_t)c);
-    }
+    std::vector<std::pair<uint32_t, uint32_t>> ms;
+    for (auto& kv : cur) ms.push_back({kv.second.a, kv.second.b});
+    ref_colors = refColoring(ms);
   }
 
   // src/system.zig:670-783
@@ -1406,6 +1426,8 @@ struct World {
     for (size_t i : order) solver.addManifold(*ms[i]);
     stats.manifolds = (uint32_t)cur.size();
     stats.solver_manifolds = (uint32_t)solver.cms.size();
+    stats.solver_points = 0;
+    for (const CManifold& cm : solver.cms) stats.solver_points += (uint32_t)cm.n;
     stats.colors = ncol;
     if (cfg.enable_islands) {
       buildIslands();
@@ -1463,6 +1485,8 @@ struct World {
   void step(float dt) {
     if (cfg.enable_sleeping && allDynamicSleeping()) return;
     stats.steps_executed++;
+    for (Slot& s : slots)
+      if (s.active && s.body.type == VXP_BODY_DYNAMIC && !s.body.sleeping) stats.awake_bodies_stepped++;
     broadPhase();
     narrowPhase();
     solve(dt);
@@ -1801,6 +1825,26 @@ void vxo_cache_script(uint32_t cap, uint32_t n, const uint32_t* ops, const uint3
   }
 }
 uint32_t vxo_color_prio(uint64_t key) { return colorPrio(key); }
+// Island labels of an edge list (src/islands.zig:318-441); is_static[id] for
+// ids < n_ids.  Writes (id, label) ascending; returns the count.
+uint32_t vxo_island_labels(uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t n_ids,
+                           const uint8_t* is_static, uint32_t* ids, uint32_t* labels) {
+  std::vector<std::pair<uint32_t, uint32_t>> e;
+  for (uint32_t i = 0; i < n; i++) e.push_back({a[i], b[i]});
+  auto st = [&](uint32_t id) { return id < n_ids && is_static[id] != 0; };
+  auto out = islandLabels(e, st);
+  for (size_t i = 0; i < out.size(); i++) {
+    ids[i] = out[i].first;
+    labels[i] = out[i].second;
+  }
+  return (uint32_t)out.size();
+}
+void vxo_ref_coloring(uint32_t n, const uint32_t* a, const uint32_t* b, uint8_t* out) {
+  std::vector<std::pair<uint32_t, uint32_t>> ms;
+  for (uint32_t i = 0; i < n; i++) ms.push_back({a[i], b[i]});
+  auto c = refColoring(ms);
+  for (uint32_t i = 0; i < n; i++) out[i] = c[i];
+}
 // Mass properties of a collider: inv_mass, inv_inertia[9], com[3]
 int vxo_mass_properties(vxo_system* s, uint32_t collider, float density, float out[13]) {
   if (collider >= s->shared.colliders.size()) return VXP_ERR_ARGUMENT;

@@ -1,0 +1,95 @@
+"""Parity of the HIP pipeline (libvxp.so, called through the C ABI) with the
+CPU restatement on identical seeded scenes.
+
+The bar is exact agreement, frame after frame, of every artefact of the step:
+the broad-phase pair set, every manifold field (contacts, feature ids, solver
+colours, warm-start impulses and effective masses), every body's pose,
+velocities and sleep state, island labels and the step statistics.  Floats are
+compared with IEEE equality (NaN == NaN), i.e. exact up to the sign of zero,
+which the reference itself leaves unspecified (Zig @min/@max lower to
+minnum/maxnum).  The north-star tolerances (1e-5 contacts, 1e-4 state) are
+therefore met with margin 0.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import vxp
+from vxp import compare, scenes
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _lockstep(product, oracle, scene, frames, check_every=1, cfg_over=None, worlds=None):
+    over = dict(cfg_over or {})
+    print(f"lockstep {scene.name} frames={frames} over={over}", flush=True)
+    f, msg, _ = compare.lockstep(scene, product, oracle, frames, cfg_over=over, worlds=worlds,
+                                 check_every=check_every)
+    assert f is None, f"frame {f}: {msg}"
+
+
+def test_c0_every_frame(product, oracle):
+    _lockstep(product, oracle, scenes.c0_small(), 300)
+
+
+@pytest.mark.parametrize("over", [dict(use_brick_pair_cache=0), dict(enable_warm_starting=0, warm_starting=0),
+                                  dict(enable_sleeping=0), dict(enable_coloring=1),
+                                  dict(use_brick_sat=0, velocity_iterations=2, position_iterations=4),
+                                  dict(brick_pair_cache_size=64), dict(enable_friction=0)])
+def test_c0_config_knobs(product, oracle, over):
+    _lockstep(product, oracle, scenes.c0_small(), 90, check_every=3, cfg_over=over)
+
+
+def test_c1_4096_bodies(product, oracle):
+    _lockstep(product, oracle, scenes.c1_large(), 120, check_every=20,
+              cfg_over=dict(cpu_threads=THREADS))
+
+
+def test_c2_rocks(product, oracle):
+    _lockstep(product, oracle, scenes.c2_rocks(), 40, check_every=10,
+              cfg_over=dict(cpu_threads=THREADS))
+
+
+def test_batched_worlds_match_oracle_and_single(product, oracle):
+    sc = scenes.batched_worlds(n_worlds=6, lo=8, hi=96, blob_pool=8)
+    _lockstep(product, oracle, sc, 120, check_every=20, cfg_over=dict(cpu_threads=THREADS))
+    # every world stepped alone equals the same world inside the batch
+    batch = vxp.from_scene(sc, lib=product)
+    singles = [vxp.from_scene(sc, lib=product, worlds=[w]) for w in range(len(sc.worlds))]
+    for _ in range(60):
+        batch.update(sc.dt)
+        for s in singles:
+            s.update(sc.dt)
+    for w, s in enumerate(singles):
+        assert compare.diff_bodies(batch.bodies(w), s.bodies(0)) is None
+        assert compare.diff_manifolds(batch.manifolds(w), s.manifolds(0)) is None
+
+
+def test_destroy_create_and_wake(product, oracle):
+    sc = scenes.c0_small()
+    a = vxp.from_scene(sc, lib=product)
+    b = vxp.from_scene(sc, lib=oracle)
+    for fr in range(200):
+        for s in (a, b):
+            s.update(sc.dt)
+            if fr == 60:
+                s.destroyBody(s.handles[0][5])
+                s.destroyBody(s.handles[0][9])
+            if fr == 61:
+                s.handles[0][5] = s.createDynamicBody((0.3, 4.0, 0.2), (0, 0, 0, 1), 1, 1.0)
+            if fr == 150:  # wake a sleeping body with a kick
+                s.setState(s.handles[0][3], linear_velocity=(0.0, 3.0, 1.0))
+        if fr % 10 == 0 or fr in (60, 61, 150, 151):
+            assert compare.diff_bodies(a.bodies(), b.bodies()) is None, fr
+            assert compare.diff_manifolds(a.manifolds(), b.manifolds()) is None, fr
+            assert compare.diff_stats(a.stats(), b.stats()) is None, fr
+
+
+def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
+    from vxp import _abi
+    monkeypatch.setattr(_abi, "PRODUCT_SO", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_abi, "_cache", {})
+    with pytest.raises(vxp.VxpError):
+        vxp.VoxelPhysicsSystem()

@@ -232,7 +232,17 @@ struct vxp_system {
       s_prio;
   uint32_t n_pairs = 0, n_manifolds = 0;
   uint32_t pair_cap = 0, cand_cap = 0;
-  std::vector<uint32_t> last_step_world_skip;
+  size_t lds_limit = 0;  // dynamic LDS available to the solver block
+  // kernel timing (HIP events on `stream`)
+  bool prof = false;
+  double prof_ms[VXP_K_COUNT] = {};
+  uint64_t prof_n[VXP_K_COUNT] = {};
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
 };
 
 namespace {
@@ -401,6 +411,43 @@ void growIds(vxp_system* s, uint32_t need) {
   s->host_dirty = true;
 }
 
+hipEvent_t takeEvent(vxp_system* s) {
+  if (!s->ev_pool.empty()) {
+    hipEvent_t e = s->ev_pool.back();
+    s->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+// Brackets a group of launches of kernel `id` with events (when profiling).
+hipEvent_t profBegin(vxp_system* s) {
+  if (!s->prof) return nullptr;
+  hipEvent_t a = takeEvent(s);
+  (void)hipEventRecord(a, s->stream);
+  return a;
+}
+void profEnd(vxp_system* s, int id, hipEvent_t a) {
+  if (!s->prof || !a) return;
+  hipEvent_t b = takeEvent(s);
+  (void)hipEventRecord(b, s->stream);
+  s->pending.push_back({id, a, b});
+}
+// After a stream synchronization: fold the recorded intervals in.
+void profFlush(vxp_system* s) {
+  for (auto& p : s->pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      s->prof_ms[p.id] += ms;
+      s->prof_n[p.id]++;
+    }
+    s->ev_pool.push_back(p.a);
+    s->ev_pool.push_back(p.b);
+  }
+  s->pending.clear();
+}
+
 #define RC(x)            \
   do {                   \
     int rc_ = (x);       \
@@ -421,15 +468,20 @@ int stepAll(vxp_system* s) {
   // ContactManager.beginFrame: the current manifolds become the previous ones
   s->cur ^= 1;
   DevManifold* prev = s->d_ml[s->cur ^ 1].as<DevManifold>();
+  hipEvent_t pe = profBegin(s);
   launch_step_begin(st, cfg, B, ws, stats, nw);
+  profEnd(s, VXP_K_STEP_BEGIN, pe);
+  pe = profBegin(s);
   launch_fat_aabb(st, cfg, B, C, ws, ns);
+  profEnd(s, VXP_K_FAT_AABB, pe);
 
   // ---- pair set: sort-and-sweep --------------------------------------------
   HIP_TRY(s->d_skeys.ensure((size_t)ns * 8));
   HIP_TRY(s->d_svals.ensure((size_t)ns * 4));
   HIP_TRY(s->d_skeys2.ensure((size_t)ns * 8));
   HIP_TRY(s->d_svals2.ensure((size_t)ns * 4));
-  launch_sweep_keys(st, cfg, B, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns);
+  pe = profBegin(s);
+  launch_sweep_keys(st, cfg, B, ws, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns, nw);
   size_t tb = sort_pairs_u64_u32_bytes(ns);
   HIP_TRY(s->d_sort_tmp.ensure(tb));
   HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, tb, s->d_skeys.as<uint64_t>(), s->d_skeys2.as<uint64_t>(),
@@ -440,8 +492,10 @@ int stepAll(vxp_system* s) {
     if (s->pair_cap < 1024) s->pair_cap = std::max<uint32_t>(1024, ns * 8);
     HIP_TRY(s->d_pairs_raw.ensure((size_t)s->pair_cap * 8));
     HIP_TRY(hipMemsetAsync(s->d_count.p, 0, 4, st));
-    launch_sweep_pairs(st, cfg, B, s->d_skeys2.as<uint64_t>(), s->d_svals2.as<uint32_t>(), ns,
+    launch_sweep_pairs(st, cfg, B, ws, s->d_skeys2.as<uint64_t>(), s->d_svals2.as<uint32_t>(), ns,
                        s->d_pairs_raw.as<uint64_t>(), s->d_count.as<uint32_t>(), s->pair_cap);
+    profEnd(s, VXP_K_PAIRS, pe);
+    pe = nullptr;
     HIP_TRY(hipMemcpyAsync(&npairs, s->d_count.p, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (npairs <= s->pair_cap) break;
@@ -449,12 +503,14 @@ int stepAll(vxp_system* s) {
   }
   s->n_pairs = npairs;
   HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
+  pe = profBegin(s);
   if (npairs) {
     size_t kb = sort_keys_u64_bytes(npairs);
     HIP_TRY(s->d_sort_tmp.ensure(kb));
     HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(), s->d_pairs.as<uint64_t>(),
                           npairs, st));
   }
+  profEnd(s, VXP_K_PAIRS, pe);
   const uint64_t* keys = s->d_pairs.as<uint64_t>();
 
   // ---- narrow phase ----------------------------------------------------------
@@ -467,8 +523,12 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_staging.ensure((size_t)np1 * sizeof(DevManifold)));
   HIP_TRY(s->d_cstage.ensure((size_t)np1 * sizeof(CacheVal)));
   PairRec* pr = s->d_pr.as<PairRec>();
+  pe = profBegin(s);
   launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev);
+  profEnd(s, VXP_K_PAIR_PROCESS, pe);
+  pe = profBegin(s);
   launch_cache_replay(st, cfg, ws, stats, pr, s->d_cache_keys.as<CacheKeyRec>(), nw);
+  profEnd(s, VXP_K_CACHE_REPLAY, pe);
   uint32_t total_wi = 0;
   if (npairs) {
     launch_chunk_counts(st, pr, npairs, s->d_counts.as<uint32_t>());
@@ -491,16 +551,19 @@ int stepAll(vxp_system* s) {
     HIP_TRY(s->d_cand.ensure((size_t)s->cand_cap * sizeof(Cand)));
     HIP_TRY(hipMemsetAsync(s->d_count.p, 0, 4, st));
     // the probe is a pure function of the step's inputs, so it can be re-run
+    pe = profBegin(s);
     launch_probe(st, cfg, B, C, pr, npairs, s->d_wi_off.as<uint32_t>(), total_wi,
                  s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
                  s->d_wi_vchk.as<uint32_t>(), s->d_cand.as<Cand>(), s->d_count.as<uint32_t>(),
                  s->cand_cap);
+    profEnd(s, VXP_K_PROBE, pe);
     uint32_t nc = 0;
     HIP_TRY(hipMemcpyAsync(&nc, s->d_count.p, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (nc <= s->cand_cap) break;
     s->cand_cap = nc + nc / 2;  // grow and re-run: the probe only writes fresh buffers
   }
+  pe = profBegin(s);
   launch_reduce(st, cfg, B, ws, stats, pr, keys, npairs, s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
                 s->d_cand.as<Cand>(), prev,
@@ -508,6 +571,8 @@ int stepAll(vxp_system* s) {
                 s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>());
   launch_cache_fill(st, cfg, ws, pr, npairs, s->d_cache_keys.as<CacheKeyRec>(),
                     s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>());
+  profEnd(s, VXP_K_REDUCE, pe);
+  pe = profBegin(s);
   if (npairs) {
     size_t sb = exclusive_scan_u32_bytes(npairs);
     HIP_TRY(s->d_sort_tmp.ensure(sb));
@@ -516,6 +581,7 @@ int stepAll(vxp_system* s) {
   }
   HIP_TRY(s->d_total.ensure(64));
   launch_world_segments(st, ws, nw, s->d_total.as<uint32_t>());
+  profEnd(s, VXP_K_COMPACT, pe);
   uint32_t nman = 0;
   HIP_TRY(hipMemcpyAsync(&nman, s->d_total.p, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -523,9 +589,11 @@ int stepAll(vxp_system* s) {
   uint32_t nm1 = std::max(nman, 1u);
   HIP_TRY(s->d_ml[s->cur].ensure((size_t)nm1 * sizeof(DevManifold)));
   DevManifold* ml = s->d_ml[s->cur].as<DevManifold>();
+  pe = profBegin(s);
   launch_scatter(st, ws, pr, npairs, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(),
                  s->d_staging.as<DevManifold>(), ml);
   launch_copy_skipped(st, ws, prev, ml, nw);
+  profEnd(s, VXP_K_COMPACT, pe);
 
   // ---- islands / colouring artefacts ------------------------------------------
   if (s->cfg.enable_islands) {
@@ -533,8 +601,10 @@ int stepAll(vxp_system* s) {
     HIP_TRY(s->d_isl_parent.ensure(n));
     HIP_TRY(s->d_isl_end.ensure(n));
     HIP_TRY(s->d_isl_size.ensure(n));
+    pe = profBegin(s);
     launch_islands(st, cfg, B, ws, stats, ml, s->d_isl_parent.as<uint32_t>(), s->d_isl_end.as<uint32_t>(),
                    s->d_isl_size.as<uint32_t>(), nw);
+    profEnd(s, VXP_K_ISLANDS, pe);
   }
   if (s->cfg.enable_coloring) {
     HIP_TRY(s->d_ref_masks.ensure((size_t)nw * s->id_cap * 32));
@@ -567,7 +637,14 @@ int stepAll(vxp_system* s) {
   S.color = s->s_color.as<int32_t>();
   S.order = s->s_order.as<uint32_t>();
   S.pt = s->s_pt.as<float>();
-  launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw);
+  pe = profBegin(s);
+  launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw, s->lds_limit);
+  if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
+    (void)hipGetLastError();
+    s->lds_limit = 0;
+    launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw, 0);
+  }
+  profEnd(s, VXP_K_SOLVE, pe);
   HIP_TRY(hipGetLastError());
   s->dev_newer = true;
   return VXP_OK;
@@ -575,6 +652,7 @@ int stepAll(vxp_system* s) {
 
 int checkErrors(vxp_system* s) {
   HIP_TRY(hipStreamSynchronize(s->stream));
+  profFlush(s);
   std::vector<WorldState> ws(s->nw);
   HIP_TRY(hipMemcpy(ws.data(), s->d_ws.p, ws.size() * sizeof(WorldState), hipMemcpyDeviceToHost));
   for (auto& w : ws)
@@ -607,6 +685,28 @@ int updateOnce(vxp_system* s, float dt) {
 extern "C" {
 
 const char* vxp_last_error(void) { return g_err.c_str(); }
+
+int vxp_profile_enable(vxp_system* s, int on) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  profFlush(s);
+  s->prof = on != 0;
+  if (s->prof)
+    for (int i = 0; i < VXP_K_COUNT; i++) {
+      s->prof_ms[i] = 0.0;
+      s->prof_n[i] = 0;
+    }
+  return VXP_OK;
+}
+
+int vxp_profile_get(vxp_system* s, uint32_t id, double* total_ms, uint64_t* launches) {
+  if (!s || id >= VXP_K_COUNT || !total_ms || !launches) return VXP_ERR_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  profFlush(s);
+  *total_ms = s->prof_ms[id];
+  *launches = s->prof_n[id];
+  return VXP_OK;
+}
 
 int vxp_config_default(vxp_config* c) {
   if (!c) return VXP_ERR_ARGUMENT;
@@ -660,6 +760,10 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
     return VXP_ERR_ARGUMENT;
   }
   uint32_t cap = cfg->brick_pair_cache_size ? cfg->brick_pair_cache_size : 1024;
+  if (cap * sizeof(CacheKeyRec) > 60 * 1024) {
+    g_err = "brick_pair_cache_size above 1536 entries is not supported (LDS-resident replay)";
+    return VXP_ERR_ARGUMENT;
+  }
   int dev_count = 0;
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
     g_err = "no HIP device";
@@ -702,6 +806,13 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
     delete s;
     return r;
   };
+  {
+    int dev = 0, lds = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        lds > 24 * 1024)
+      s->lds_limit = (size_t)lds - 20 * 1024;  // the solver's static LDS
+  }
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     g_err = "hipStreamCreate failed";
     return fail(VXP_ERR_DEVICE);
@@ -740,6 +851,8 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,
                     &s->s_color, &s->s_order, &s->s_pt, &s->s_prio};
   for (DevBuf* b : bufs) b->release();
+  profFlush(s);
+  for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(s->stream);
   delete s;
   return VXP_OK;

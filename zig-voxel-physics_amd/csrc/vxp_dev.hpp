@@ -72,7 +72,7 @@ struct WorldState {
   uint32_t produced;
   uint32_t frame;                     // BrickPairCache.current_frame
   uint32_t error;
-  uint32_t pad;
+  uint32_t axis;                      // sort-and-sweep axis (longest extent)
 };
 
 // Pair flags

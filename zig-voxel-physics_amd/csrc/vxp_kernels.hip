@@ -85,6 +85,28 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t x) {
   return x;
 }
 
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, kWave);
+  uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// atomicAdd(addr, v) aggregated over the wave when every lane with v != 0
+// targets the same counter.  Call from converged code (all 64 lanes).
+__device__ __forceinline__ void aggAdd(uint32_t* addr, uint32_t v) {
+  uint64_t act = __ballot(v != 0);
+  if (!act) return;
+  int leader = __ffsll((long long)act) - 1;
+  uint64_t a = (uint64_t)(uintptr_t)addr;
+  uint64_t la = shfl64(a, leader);
+  if (__ballot(v != 0 && a != la) == 0) {
+    uint32_t sum = waveSum(v);
+    if ((int)(threadIdx.x & (kWave - 1)) == leader) atomicAdd(addr, sum);
+  } else if (v) {
+    atomicAdd(addr, v);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // step begin: allDynamicBodiesSleeping, per-step stat reset
 // ---------------------------------------------------------------------------
@@ -113,6 +135,7 @@ __global__ void k_step_begin(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats*
     if (!ws[w].skip) {
       vxp_stats& s = st[w];
       s.steps_executed++;
+      s.awake_bodies_stepped += awake;
       s.broad_phase_pairs = s.narrow_phase_tested = s.narrow_phase_colliding = 0;
       s.contacts_generated = s.voxels_checked = s.pairs_skipped_sleeping = 0;
       s.cache_hits = s.cache_misses = s.manifolds_skipped_sleeping = 0;
@@ -165,39 +188,98 @@ __device__ __forceinline__ uint32_t orderedFloat(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__global__ void k_sweep_keys(DevCfg cfg, DevBodies B, uint64_t* keys, uint32_t* vals,
-                             uint32_t n_slots) {
+__device__ __forceinline__ float comp(const float4& v, uint32_t a) {
+  return a == 0 ? v.x : (a == 1 ? v.y : v.z);
+}
+
+// Sweep axis of each world: the axis along which the fat-box centres spread
+// furthest (the pair set does not depend on the axis).
+__global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n_worlds) {
+  uint32_t w = blockIdx.x;
+  if (w >= n_worlds) return;
+  __shared__ float lo[3][256], hi[3][256];
+  float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t s = threadIdx.x; s < cfg.max_bodies; s += blockDim.x) {
+    uint32_t g = w * cfg.max_bodies + s;
+    if (!B.active[g]) continue;
+    float4 a = B.fat_lo[g], b = B.fat_hi[g];
+    float c[3] = {a.x + b.x, a.y + b.y, a.z + b.z};
+    for (int k = 0; k < 3; k++) {
+      l[k] = fminf(l[k], c[k]);
+      h[k] = fmaxf(h[k], c[k]);
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    lo[k][threadIdx.x] = l[k];
+    hi[k][threadIdx.x] = h[k];
+  }
+  __syncthreads();
+  for (uint32_t d = blockDim.x / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d)
+      for (int k = 0; k < 3; k++) {
+        lo[k][threadIdx.x] = fminf(lo[k][threadIdx.x], lo[k][threadIdx.x + d]);
+        hi[k][threadIdx.x] = fmaxf(hi[k][threadIdx.x], hi[k][threadIdx.x + d]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float e[3];
+    for (int k = 0; k < 3; k++) e[k] = hi[k][0] - lo[k][0];
+    uint32_t ax = 0;
+    if (e[1] > e[ax]) ax = 1;
+    if (e[2] > e[ax]) ax = 2;
+    ws[w].axis = ax;
+  }
+}
+
+__global__ void k_sweep_keys(DevCfg cfg, DevBodies B, const WorldState* ws, uint64_t* keys,
+                             uint32_t* vals, uint32_t n_slots) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_slots) return;
   uint32_t w = g / cfg.max_bodies;
   // inactive slots sort to the end of their world
-  uint32_t k = B.active[g] ? orderedFloat(B.fat_lo[g].x) : 0xFFFFFFFFu;
+  uint32_t k = B.active[g] ? orderedFloat(comp(B.fat_lo[g], ws[w].axis)) : 0xFFFFFFFFu;
   keys[g] = ((uint64_t)w << 32) | k;
   vals[g] = g;
 }
 
-__global__ void k_sweep_pairs(DevCfg cfg, DevBodies B, const uint64_t* keys, const uint32_t* order,
-                              uint32_t n_slots, uint64_t* out, uint32_t* count, uint32_t cap) {
+__global__ void k_sweep_pairs(DevCfg cfg, DevBodies B, const WorldState* ws, const uint64_t* keys,
+                              const uint32_t* order, uint32_t n_slots, uint64_t* out,
+                              uint32_t* count, uint32_t cap) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_slots) return;
   uint32_t gi = order[i];
   if (!B.active[gi]) return;
   uint32_t w = gi / cfg.max_bodies;
+  uint32_t ax = ws[w].axis;
   Box bi{f4v(B.fat_lo[gi]), f4v(B.fat_hi[gi])};
+  float hi_ax = comp(B.fat_hi[gi], ax);
   bool si = B.type[gi] == VXP_BODY_STATIC;
   uint32_t idi = B.id[gi];
+  // pairs are staged per thread and appended with one atomic per batch
+  constexpr int kBatch = 16;
+  uint64_t buf[kBatch];
+  int nb = 0;
   for (uint32_t j = i + 1; j < n_slots; j++) {
     uint32_t gj = order[j];
     if (gj / cfg.max_bodies != w || !B.active[gj]) break;
-    float x0 = B.fat_lo[gj].x;
-    if (x0 > bi.hi.x) break;
+    if (comp(B.fat_lo[gj], ax) > hi_ax) break;
     Box bj{f4v(B.fat_lo[gj]), f4v(B.fat_hi[gj])};
     if (!boxOverlaps(bi, bj)) continue;
     if (si && B.type[gj] == VXP_BODY_STATIC) continue;
     uint32_t idj = B.id[gj];
-    uint32_t a = min(idi, idj), b = max(idi, idj);
-    uint32_t slot = atomicAdd(count, 1u);
-    if (slot < cap) out[slot] = makeKey(w, a, b);
+    buf[nb++] = makeKey(w, min(idi, idj), max(idi, idj));
+    if (nb == kBatch) {
+      uint32_t slot = atomicAdd(count, (uint32_t)nb);
+      for (int k = 0; k < nb; k++)
+        if (slot + k < cap) out[slot + k] = buf[k];
+      nb = 0;
+    }
+  }
+  if (nb) {
+    uint32_t slot = atomicAdd(count, (uint32_t)nb);
+    for (int k = 0; k < nb; k++)
+      if (slot + k < cap) out[slot + k] = buf[k];
   }
 }
 
@@ -268,11 +350,14 @@ __device__ int32_t findPrev(const DevManifold* prev, uint32_t lo, uint32_t hi, u
   return -1;
 }
 
-__global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
-                               vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
-                               const DevManifold* prev) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
+struct PairCounts {
+  uint32_t bp, skipped, tested;
+};
+
+__device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
+                                  WorldState* ws, const uint64_t* keys, uint32_t n_pairs,
+                                  uint32_t p, PairRec* pr, const DevManifold* prev) {
+  PairCounts cnt{0, 0, 0};
   uint64_t key = keys[p];
   uint32_t w = keyWorld(key), ida = keyA(key), idb = keyB(key);
   if (p == 0 || keyWorld(keys[p - 1]) != w) ws[w].pair_lo = p;
@@ -294,26 +379,26 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
   r.sb = (uint32_t)sb;
   if (ws[w].skip || sa < 0 || sb < 0) {
     pr[p] = r;
-    return;
+    return cnt;
   }
-  atomicAdd(&st[w].broad_phase_pairs, 1u);
+  cnt.bp = 1;
   uint32_t ga = w * cfg.max_bodies + sa, gb = w * cfg.max_bodies + sb;
   bool asl = B.sleeping[ga], bsl = B.sleeping[gb];
   if (asl && bsl) {
-    atomicAdd(&st[w].pairs_skipped_sleeping, 1u);
+    cnt.skipped = 1;
     int32_t j = findPrev(prev, ws[w].prev_start, ws[w].prev_start + ws[w].prev_count, key);
     if (j >= 0) {
       r.flag = PF_CARRY;
       r.aux = (uint32_t)j;
     }
     pr[p] = r;
-    return;
+    return cnt;
   }
-  atomicAdd(&st[w].narrow_phase_tested, 1u);
+  cnt.tested = 1;
   int32_t ca = B.collider[ga], cb = B.collider[gb];
   if (ca < 0 || cb < 0) {
     pr[p] = r;
-    return;
+    return cnt;
   }
   const DevCollider& HA = C.hdr[ca];
   const DevCollider& HB = C.hdr[cb];
@@ -324,7 +409,7 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
     Obb ob = obbFromBox(colliderBox(HB), pb, qb);
     if (satSeparated(oa, ob)) {
       pr[p] = r;
-      return;
+      return cnt;
     }
   }
   uint8_t ta = B.type[ga], tb = B.type[gb];
@@ -371,7 +456,25 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
     }
   }
   pr[p] = r;
+  return cnt;
 }
+
+__global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
+                               vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
+                               const DevManifold* prev) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  PairCounts c{0, 0, 0};
+  uint32_t w = 0;
+  if (p < n_pairs) {
+    c = pairProcess(cfg, B, C, ws, keys, n_pairs, p, pr, prev);
+    w = keyWorld(keys[p]);
+  }
+  aggAdd(&st[w].broad_phase_pairs, c.bp);
+  aggAdd(&st[w].pairs_skipped_sleeping, c.skipped);
+  aggAdd(&st[w].narrow_phase_tested, c.tested);
+}
+
+
 
 // ---------------------------------------------------------------------------
 // BrickPairCache replay: one thread per world walks its cache-eligible pairs in
@@ -387,67 +490,104 @@ __device__ __forceinline__ uint64_t cacheHash(uint32_t a, uint32_t b, uint64_t t
   return h * 31 + th;
 }
 
-__global__ void k_cache_replay(DevCfg cfg, WorldState* ws, vxp_stats* st, PairRec* pr,
-                               CacheKeyRec* keys, uint32_t n_worlds) {
-  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per world; the world's key table is staged in LDS.  Lanes 0..15
+// probe the 16-entry window of one get/put in parallel (ballot picks the
+// first stopping entry); pairs are visited strictly in pair order.
+__global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws, vxp_stats* st,
+                                                     PairRec* pr, CacheKeyRec* keys,
+                                                     uint32_t n_worlds) {
+  extern __shared__ CacheKeyRec E[];
+  const uint32_t w = blockIdx.x, lane = threadIdx.x;
   if (w >= n_worlds || ws[w].skip) return;
-  uint32_t frame = ++ws[w].frame;
+  const uint32_t frame = ws[w].frame + 1;
+  __syncthreads();
+  if (lane == 0) ws[w].frame = frame;
   if (!cfg.use_cache) return;
-  CacheKeyRec* E = keys + (size_t)w * cfg.cache_cap;
-  uint32_t cap = cfg.cache_cap, maxp = cap < 16 ? cap : 16;
+  const uint32_t cap = cfg.cache_cap, maxp = cap < 16 ? cap : 16;
+  CacheKeyRec* G = keys + (size_t)w * cap;
+  for (uint32_t i = lane; i < cap; i += kWave) E[i] = G[i];
+  __syncthreads();
   uint32_t hits = 0, misses = 0;
-  for (uint32_t p = ws[w].pair_lo; p < ws[w].pair_hi; p++) {
-    PairRec& r = pr[p];
-    if (r.flag != PF_CACHE) continue;
-    uint64_t h = cacheHash(r.ida, r.idb, r.th);
-    uint32_t start = (uint32_t)(h % cap), idx = start;
-    int32_t hit = -1;
-    for (uint32_t k = 0; k < maxp; k++) {
-      CacheKeyRec& e = E[idx];
-      if (!e.valid) break;
-      if (e.a == r.ida && e.b == r.idb && e.th == r.th) {
-        hit = (int32_t)idx;
-        e.last_used = frame;
-        break;
+  const uint32_t lo = ws[w].pair_lo, hi = ws[w].pair_hi;
+  for (uint32_t base = lo; base < hi; base += kWave) {
+    uint32_t p = base + lane;
+    bool inb = p < hi;
+    uint32_t flag = inb ? pr[p].flag : PF_NONE;
+    uint32_t ida = inb ? pr[p].ida : 0, idb = inb ? pr[p].idb : 0;
+    uint64_t th = inb ? pr[p].th : 0;
+    uint32_t my_flag = flag, my_aux = kNone;
+    uint64_t elig = __ballot(flag == PF_CACHE);
+    while (elig) {
+      int j = __ffsll((long long)elig) - 1;
+      elig &= elig - 1;
+      uint32_t a = (uint32_t)__shfl((int)ida, j, kWave), b = (uint32_t)__shfl((int)idb, j, kWave);
+      uint64_t t = shfl64(th, j);
+      uint64_t h = cacheHash(a, b, t);
+      uint32_t start = (uint32_t)(h % cap);
+      bool probe = lane < maxp;
+      uint32_t idx = (start + lane) % cap;
+      bool valid = false, eq = false;
+      uint64_t lu = ~0ull;
+      if (probe) {
+        valid = E[idx].valid != 0;
+        eq = valid && E[idx].a == a && E[idx].b == b && E[idx].th == t;
+        lu = E[idx].last_used;
       }
-      idx = (idx + 1) % cap;
-    }
-    if (hit >= 0) {
-      r.flag = PF_HIT;
-      r.aux = (uint32_t)hit;
-      hits++;
-      continue;
-    }
-    misses++;
-    idx = start;
-    uint32_t oldest = start, put = kNone;
-    uint64_t oldest_frame = ~0ull;
-    for (uint32_t k = 0; k < maxp; k++) {
-      CacheKeyRec& e = E[idx];
-      if (!e.valid || (e.a == r.ida && e.b == r.idb && e.th == r.th)) {
-        put = idx;
-        break;
+      uint64_t stop = __ballot(probe && (!valid || eq));
+      uint32_t rflag, raux;
+      if (stop) {
+        int f = __ffsll((long long)stop) - 1;
+        bool feq = __shfl((int)eq, f, kWave) != 0;
+        raux = (start + (uint32_t)f) % cap;
+        rflag = feq ? PF_HIT : PF_MISS;
+      } else {
+        // window full: evict the first entry with the smallest last_used
+        uint64_t v = lu;
+        uint32_t l = lane;
+        for (int d = kWave / 2; d > 0; d >>= 1) {
+          uint64_t ov = shfl64(v, (int)(lane ^ d));
+          uint32_t ol = (uint32_t)__shfl((int)l, (int)(lane ^ d), kWave);
+          if (ov < v || (ov == v && ol < l)) {
+            v = ov;
+            l = ol;
+          }
+        }
+        raux = (start + (uint32_t)__shfl((int)l, 0, kWave)) % cap;
+        rflag = PF_MISS;
       }
-      if (e.last_used < oldest_frame) {
-        oldest_frame = e.last_used;
-        oldest = idx;
+      if (lane == 0) {
+        if (rflag == PF_HIT) {
+          E[raux].last_used = frame;
+          hits++;
+        } else {
+          CacheKeyRec& e = E[raux];
+          e.a = a;
+          e.b = b;
+          e.th = t;
+          e.last_used = frame;
+          e.valid = 1;
+          e.owner_frame = frame;
+          e.owner_pair = base + (uint32_t)j;
+          misses++;
+        }
       }
-      idx = (idx + 1) % cap;
+      __syncthreads();
+      if (lane == (uint32_t)j) {
+        my_flag = rflag;
+        my_aux = raux;
+      }
     }
-    if (put == kNone) put = oldest;
-    CacheKeyRec& e = E[put];
-    e.a = r.ida;
-    e.b = r.idb;
-    e.th = r.th;
-    e.last_used = frame;
-    e.valid = 1;
-    e.owner_frame = frame;
-    e.owner_pair = p;
-    r.flag = PF_MISS;
-    r.aux = put;
+    if (inb && flag == PF_CACHE) {
+      pr[p].flag = my_flag;
+      pr[p].aux = my_aux;
+    }
   }
-  st[w].cache_hits += hits;
-  st[w].cache_misses += misses;
+  __syncthreads();
+  for (uint32_t i = lane; i < cap; i += kWave) G[i] = E[i];
+  if (lane == 0) {
+    st[w].cache_hits += hits;
+    st[w].cache_misses += misses;
+  }
 }
 
 __global__ void k_chunk_counts(const PairRec* pr, uint32_t n_pairs, uint32_t* counts) {
@@ -638,25 +778,29 @@ __device__ void warmStartFrom(vxp_manifold& m, const DevManifold* prev, int32_t 
       }
 }
 
-__global__ void k_reduce(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st, PairRec* pr,
-                         const uint64_t* keys, uint32_t n_pairs, const uint32_t* wi_off,
-                         const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
-                         const uint32_t* wi_vchk, const Cand* cand,
-                         const DevManifold* prev, const CacheVal* cvals, CacheVal* cstage,
-                         DevManifold* staging, uint32_t* produce) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
+struct ReduceCounts {
+  uint32_t produced, vchk, colliding, contacts;
+};
+
+__device__ ReduceCounts reducePair(const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                                   const PairRec* pr, const uint64_t* keys, uint32_t p,
+                                   const uint32_t* wi_off, const uint32_t* wi_cbase,
+                                   const uint32_t* wi_ccnt, const uint32_t* wi_vchk,
+                                   const Cand* cand, const DevManifold* prev,
+                                   const CacheVal* cvals, CacheVal* cstage, DevManifold* staging,
+                                   uint32_t* produce) {
+  ReduceCounts rc{0, 0, 0, 0};
   const PairRec& r = pr[p];
   uint32_t f = r.flag;
   produce[p] = 0;
-  if (f == PF_NONE || f == PF_CACHE) return;
+  if (f == PF_NONE || f == PF_CACHE) return rc;
   uint32_t w = r.world;
   uint64_t key = keys[p];
   if (f == PF_CARRY) {
     staging[p] = prev[r.aux];
     produce[p] = 1;
-    atomicAdd(&ws[w].produced, 1u);
-    return;
+    rc.produced = 1;
+    return rc;
   }
   uint32_t ga = w * cfg.max_bodies + r.sa, gb = w * cfg.max_bodies + r.sb;
   DevManifold d;
@@ -673,7 +817,7 @@ __global__ void k_reduce(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st,
   uint32_t vchk;
   if (f == PF_HIT) {
     const CacheVal& v = cvals[(size_t)w * cfg.cache_cap + r.aux];
-    if (!v.has_collision) return;
+    if (!v.has_collision) return rc;
     for (uint32_t i = 0; i < v.count; i++) m.contacts[i] = v.c[i];
     m.contact_count = v.count;
     vchk = v.vchk;
@@ -734,15 +878,36 @@ __global__ void k_reduce(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st,
       for (uint32_t i = 0; i < n; i++) cv.c[i] = m.contacts[i];
     }
   }
-  atomicAdd(&st[w].voxels_checked, vchk);
-  if (m.contact_count == 0) return;
-  atomicAdd(&st[w].narrow_phase_colliding, 1u);
-  atomicAdd(&st[w].contacts_generated, m.contact_count);
+  rc.vchk = vchk;
+  if (m.contact_count == 0) return rc;
+  rc.colliding = 1;
+  rc.contacts = m.contact_count;
   int32_t j = findPrev(prev, ws[w].prev_start, ws[w].prev_start + ws[w].prev_count, key);
   warmStartFrom(m, prev, j);
   staging[p] = d;
   produce[p] = 1;
-  atomicAdd(&ws[w].produced, 1u);
+  rc.produced = 1;
+  return rc;
+}
+
+__global__ void k_reduce(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st, PairRec* pr,
+                         const uint64_t* keys, uint32_t n_pairs, const uint32_t* wi_off,
+                         const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
+                         const uint32_t* wi_vchk, const Cand* cand,
+                         const DevManifold* prev, const CacheVal* cvals, CacheVal* cstage,
+                         DevManifold* staging, uint32_t* produce) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  ReduceCounts c{0, 0, 0, 0};
+  uint32_t w = 0;
+  if (p < n_pairs) {
+    c = reducePair(cfg, B, ws, pr, keys, p, wi_off, wi_cbase, wi_ccnt, wi_vchk, cand, prev, cvals,
+                   cstage, staging, produce);
+    w = pr[p].world;
+  }
+  aggAdd(&ws[w].produced, c.produced);
+  aggAdd(&st[w].voxels_checked, c.vchk);
+  aggAdd(&st[w].narrow_phase_colliding, c.colliding);
+  aggAdd(&st[w].contacts_generated, c.contacts);
 }
 
 __global__ void k_cache_fill(DevCfg cfg, const WorldState* ws, const PairRec* pr, uint32_t n_pairs,
@@ -932,14 +1097,6 @@ enum {
   PT_TI0 = 30, PT_TI1 = 31, PT_VN0 = 32, PT_PEN = 33
 };
 
-struct BodyRef {
-  float4 *pos, *rot, *lv, *av;
-  float* ii;
-  uint32_t g;
-  float im;
-  uint8_t type, sleeping;
-};
-
 __device__ __forceinline__ M3 loadM3(const float* p) {
   M3 r;
   for (int i = 0; i < 9; i++) r.m[i] = p[i];
@@ -947,19 +1104,6 @@ __device__ __forceinline__ M3 loadM3(const float* p) {
 }
 __device__ __forceinline__ void storeM3(float* p, const M3& m) {
   for (int i = 0; i < 9; i++) p[i] = m.m[i];
-}
-
-__device__ __forceinline__ V3 pointVel(const SolverScratch& S, uint32_t g, V3 pt) {
-  return add(f4v(S.ws_lv[g]), cross(f4v(S.ws_av[g]), sub(pt, f4v(S.ws_pos[g]))));
-}
-// BodyState.applyImpulseAtPoint (src/solver.zig:316-322)
-__device__ __forceinline__ void applyImpulse(const SolverScratch& S, const DevBodies& B, uint32_t g,
-                                             V3 imp, V3 r) {
-  if (B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) return;
-  float im = B.inv_mass[g];
-  S.ws_lv[g] = v4(add(f4v(S.ws_lv[g]), scale(imp, im)));
-  M3 ii = loadM3(S.ws_ii + (size_t)g * 9);
-  S.ws_av[g] = v4(add(f4v(S.ws_av[g]), mulVec(ii, cross(r, imp))));
 }
 
 // src/solver.zig:967-981
@@ -998,31 +1142,141 @@ __device__ void blockScanExclusive(uint32_t* data, uint32_t n, uint32_t* tmp /*b
 }
 
 constexpr uint32_t kMaxColors = 1024;
+constexpr uint32_t kSolveThreads = 1024;
 
-__global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldState* ws,
-                                                vxp_stats* st, DevManifold* ml, SolverScratch S,
-                                                uint32_t* prio_buf) {
+// One body of a constraint during the velocity passes: velocities live in
+// VL/VA (LDS or global), everything else is read-only within a pass.
+struct SBody {
+  uint32_t s;       // slot within the world
+  V3 pos;
+  M3 ii;
+  float im;
+  bool awake;       // dynamic and not sleeping: receives impulses
+};
+
+template <class VelPtr>
+__device__ __forceinline__ V3 pointVel(VelPtr VL, VelPtr VA, const SBody& b, V3 pt) {
+  return add(f4v(VL[b.s]), cross(f4v(VA[b.s]), sub(pt, b.pos)));
+}
+// BodyState.applyImpulseAtPoint (src/solver.zig:316-322)
+template <class VelPtr>
+__device__ __forceinline__ void applyImpulse(VelPtr VL, VelPtr VA, const SBody& b, V3 imp, V3 r) {
+  if (!b.awake) return;
+  VL[b.s] = v4(add(f4v(VL[b.s]), scale(imp, b.im)));
+  VA[b.s] = v4(add(f4v(VA[b.s]), mulVec(b.ii, cross(r, imp))));
+}
+
+__device__ __forceinline__ SBody loadSBody(const DevBodies& B, const SolverScratch& S, uint32_t g0,
+                                           uint32_t s) {
+  uint32_t g = g0 + s;
+  SBody b;
+  b.s = s;
+  b.pos = f4v(S.ws_pos[g]);
+  b.ii = loadM3(S.ws_ii + (size_t)g * 9);
+  b.im = B.inv_mass[g];
+  b.awake = B.type[g] == VXP_BODY_DYNAMIC && !B.sleeping[g];
+  return b;
+}
+
+// solveVelocityConstraints for one manifold (src/solver.zig:732-787) with
+// solveFriction (src/solver.zig:790-838); kBias selects the soft biased pass
+// or the hard relax pass (src/solver.zig:848-856).
+template <bool kBias, class VelPtr>
+__device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, const DevBodies& B,
+                                                 const SolverScratch& S, VelPtr VL, VelPtr VA,
+                                                 const DevManifold& d, uint32_t gi, uint32_t g0) {
+  SBody A = loadSBody(B, S, g0, d.sa), Bb = loadSBody(B, S, g0, d.sb);
+  float fr = d.m.friction;
+  uint32_t n = d.m.contact_count;
+  for (uint32_t k = 0; k < n; k++) {
+    float* P = S.pt + ((size_t)gi * 4 + k) * kPtStride;
+    V3 nrm{P[PT_NORMAL], P[PT_NORMAL + 1], P[PT_NORMAL + 2]};
+    V3 pt{P[PT_POINT], P[PT_POINT + 1], P[PT_POINT + 2]};
+    V3 ra{P[PT_RA], P[PT_RA + 1], P[PT_RA + 2]};
+    V3 rb{P[PT_RB], P[PT_RB + 1], P[PT_RB + 2]};
+    V3 t1{P[PT_T1], P[PT_T1 + 1], P[PT_T1 + 2]};
+    V3 t2{P[PT_T2], P[PT_T2 + 1], P[PT_T2 + 2]};
+    float nm = P[PT_NM], tm0 = P[PT_TM0], tm1 = P[PT_TM1], bias = P[PT_BIAS];
+    float ni = P[PT_NI], ti0 = P[PT_TI0], ti1 = P[PT_TI1];
+    V3 rel = sub(pointVel(VL, VA, Bb, pt), pointVel(VL, VA, A, pt));
+    float vn = dot(rel, nrm);
+    float imp;
+    if (kBias) imp = -cfg.mass_coeff * nm * (vn + bias) - cfg.impulse_coeff * ni;
+    else imp = -nm * vn;
+    float nw = fmaxf(ni + imp, 0.0f);
+    imp = nw - ni;
+    ni = nw;
+    V3 Pn = scale(nrm, imp);
+    applyImpulse(VL, VA, A, neg(Pn), ra);
+    applyImpulse(VL, VA, Bb, Pn, rb);
+    if (cfg.enable_friction && fr > 0.0f) {
+      V3 rel1 = sub(pointVel(VL, VA, Bb, pt), pointVel(VL, VA, A, pt));
+      float maxf = fr * ni;
+      float vt = dot(rel1, t1);
+      float fi = -tm0 * vt;
+      float old = ti0;
+      ti0 = clampz(old + fi, -maxf, maxf);
+      fi = ti0 - old;
+      V3 i1 = scale(t1, fi);
+      applyImpulse(VL, VA, A, neg(i1), ra);
+      applyImpulse(VL, VA, Bb, i1, rb);
+      V3 rel2 = sub(pointVel(VL, VA, Bb, pt), pointVel(VL, VA, A, pt));
+      float vt2 = dot(rel2, t2);
+      float fi2 = -tm1 * vt2;
+      float old2 = ti1;
+      ti1 = clampz(old2 + fi2, -maxf, maxf);
+      fi2 = ti1 - old2;
+      V3 i2 = scale(t2, fi2);
+      applyImpulse(VL, VA, A, neg(i2), ra);
+      applyImpulse(VL, VA, Bb, i2, rb);
+      P[PT_TI0] = ti0;
+      P[PT_TI1] = ti1;
+    }
+    P[PT_NI] = ni;
+  }
+}
+
+// ContactSolver.solve for one world per block (src/solver.zig:494-561).  With
+// kLds the body velocities of the world are staged in LDS for the whole solve.
+template <bool kLds>
+__global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B, WorldState* ws,
+                                                         vxp_stats* st, DevManifold* ml,
+                                                         SolverScratch S, uint32_t* prio_buf) {
+  extern __shared__ float4 dyn_vel[];
   uint32_t w = blockIdx.x;
   if (ws[w].skip) return;
   const uint32_t tid = threadIdx.x, bs = blockDim.x;
   const uint32_t MB = cfg.max_bodies, g0 = w * MB;
   const uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
-  __shared__ uint32_t tmp[1024];
+  __shared__ uint32_t tmp[kSolveThreads];
   __shared__ uint32_t ccount[kMaxColors + 1];
   __shared__ uint32_t ccursor[kMaxColors];
-  __shared__ uint32_t s_left, s_ncol, s_nsolve, s_skipped;
-  __shared__ float s_red[1024];
+  __shared__ uint32_t s_left, s_ncol, s_nsolve, s_skipped, s_points;
+  __shared__ float s_red[kSolveThreads];
+  float4* VL;
+  float4* VA;
+  if (kLds) {
+    VL = dyn_vel;
+    VA = dyn_vel + MB;
+  } else {
+    VL = S.ws_lv + g0;
+    VA = S.ws_av + g0;
+  }
 
   // body working state (BodyState.fromBody, src/solver.zig:231-243)
   for (uint32_t s = tid; s < MB; s += bs) {
     uint32_t g = g0 + s;
     S.adj_cnt[g] = 0;
-    if (!B.active[g]) continue;
+    if (!B.active[g]) {
+      VL[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      VA[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
     float4 rot = B.rot[g];
     S.ws_pos[g] = B.pos[g];
     S.ws_rot[g] = rot;
-    S.ws_lv[g] = B.lv[g];
-    S.ws_av[g] = B.av[g];
+    VL[s] = B.lv[g];
+    VA[s] = B.av[g];
     M3 ii;
     if (B.inv_mass[g] == 0.0f) {
       for (int i = 0; i < 9; i++) ii.m[i] = 0.0f;
@@ -1035,6 +1289,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
     s_nsolve = 0;
     s_skipped = 0;
     s_ncol = 0;
+    s_points = 0;
   }
   __syncthreads();
   // solvable manifolds (src/system.zig:687-702) and awake-dynamic incidence counts
@@ -1049,6 +1304,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
       continue;
     }
     atomicAdd(&s_nsolve, 1u);
+    atomicAdd(&s_points, d.m.contact_count);
     S.color[m0 + i] = -1;
     prio_buf[m0 + i] = colorPrio(exportKey(d.key));
     if (B.type[ga] == VXP_BODY_DYNAMIC && !asl) atomicAdd(&S.adj_cnt[ga], 1u);
@@ -1129,7 +1385,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
   }
   __syncthreads();
   const uint32_t ncol = s_ncol;
-  // group manifolds by colour
+  // group manifolds by colour (order inside a colour is free: no shared body)
   for (uint32_t c = tid; c <= kMaxColors; c += bs) ccount[c] = 0;
   __syncthreads();
   for (uint32_t i = tid; i < M; i += bs) {
@@ -1193,8 +1449,8 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
     for (uint32_t s = tid; s < MB; s += bs) {
       uint32_t g = g0 + s;
       if (!B.active[g] || B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) continue;
-      V3 lv = add(f4v(S.ws_lv[g]), scale(grav, h));
-      V3 av = f4v(S.ws_av[g]);
+      V3 lv = add(f4v(VL[s]), scale(grav, h));
+      V3 av = f4v(VA[s]);
       float ld = B.ldamp[g], ad = B.adamp[g];
       float ls = len(lv), as = len(av);
       if (ls < 0.5f) ld += 5.0f * (1.0f - (ls / 0.5f));
@@ -1204,8 +1460,8 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
       av = scale(av, adec);
       if (dot(lv, lv) < 0.001f * 0.001f) lv = V3{0.f, 0.f, 0.f};
       if (dot(av, av) < 0.001f * 0.001f) av = V3{0.f, 0.f, 0.f};
-      S.ws_lv[g] = v4(lv);
-      S.ws_av[g] = v4(av);
+      VL[s] = v4(lv);
+      VA[s] = v4(av);
     }
     __syncthreads();
     if (has_c) {
@@ -1218,6 +1474,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
         Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
         float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
         M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
+        V3 lva = f4v(VL[d.sa]), ava = f4v(VA[d.sa]), lvb = f4v(VL[d.sb]), avb = f4v(VA[d.sb]);
         for (uint32_t k = 0; k < d.m.contact_count; k++) {
           PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
           V3 n = P.g3(PT_NORMAL);
@@ -1235,7 +1492,9 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
           k_ += dot(rna, mulVec(iia, rna));
           k_ += dot(rnb, mulVec(iib, rnb));
           P.f[PT_NM] = k_ > kEps ? 1.0f / k_ : 0.0f;
-          V3 rel = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
+          V3 va = add(lva, cross(ava, sub(pt, pa)));
+          V3 vb = add(lvb, cross(avb, sub(pt, pb)));
+          V3 rel = sub(vb, va);
           float vn = dot(rel, n);
           P.f[PT_VN0] = vn;
           float bias;
@@ -1268,13 +1527,13 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
           for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
             uint32_t i = S.order[m0 + j];
             const DevManifold& d = ml[m0 + i];
-            uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+            SBody A = loadSBody(B, S, g0, d.sa), Bb = loadSBody(B, S, g0, d.sb);
             for (uint32_t k = 0; k < d.m.contact_count; k++) {
               PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
               float sn = P.f[PT_NI] * cfg.wss, s1 = P.f[PT_TI0] * cfg.wss, s2 = P.f[PT_TI1] * cfg.wss;
               V3 imp = add(add(scale(P.g3(PT_NORMAL), sn), scale(P.g3(PT_T1), s1)), scale(P.g3(PT_T2), s2));
-              applyImpulse(S, B, ga, neg(imp), P.g3(PT_RA));
-              applyImpulse(S, B, gb, imp, P.g3(PT_RB));
+              applyImpulse(VL, VA, A, neg(imp), P.g3(PT_RA));
+              applyImpulse(VL, VA, Bb, imp, P.g3(PT_RB));
               P.f[PT_NI] = sn;
               P.f[PT_TI0] = s1;
               P.f[PT_TI1] = s2;
@@ -1283,53 +1542,12 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
           __syncthreads();
         }
       }
-      // solveVelocityConstraints with bias (src/solver.zig:732-787)
+      // solveVelocityConstraints with bias
       for (uint32_t it = 0; it < cfg.vel_iters; it++) {
         for (uint32_t c = 0; c < ncol; c++) {
           for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
             uint32_t i = S.order[m0 + j];
-            const DevManifold& d = ml[m0 + i];
-            uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
-            float fr = d.m.friction;
-            for (uint32_t k = 0; k < d.m.contact_count; k++) {
-              PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
-              V3 n = P.g3(PT_NORMAL), pt = P.g3(PT_POINT), ra = P.g3(PT_RA), rb = P.g3(PT_RB);
-              V3 rel = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
-              float vn = dot(rel, n);
-              float ni = P.f[PT_NI];
-              float imp = -cfg.mass_coeff * P.f[PT_NM] * (vn + P.f[PT_BIAS]) - cfg.impulse_coeff * ni;
-              float nw = fmaxf(ni + imp, 0.0f);
-              imp = nw - ni;
-              P.f[PT_NI] = nw;
-              V3 Pn = scale(n, imp);
-              applyImpulse(S, B, ga, neg(Pn), ra);
-              applyImpulse(S, B, gb, Pn, rb);
-              if (cfg.enable_friction && fr > 0.0f) {
-                // solveFriction (src/solver.zig:790-838)
-                V3 rel1 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
-                float maxf = fr * P.f[PT_NI];
-                V3 t1 = P.g3(PT_T1), t2 = P.g3(PT_T2);
-                float vt = dot(rel1, t1);
-                float fi = -P.f[PT_TM0] * vt;
-                float old = P.f[PT_TI0];
-                float nt = clampz(old + fi, -maxf, maxf);
-                P.f[PT_TI0] = nt;
-                fi = nt - old;
-                V3 i1 = scale(t1, fi);
-                applyImpulse(S, B, ga, neg(i1), ra);
-                applyImpulse(S, B, gb, i1, rb);
-                V3 rel2 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
-                float vt2 = dot(rel2, t2);
-                float fi2 = -P.f[PT_TM1] * vt2;
-                float old2 = P.f[PT_TI1];
-                float nt2 = clampz(old2 + fi2, -maxf, maxf);
-                P.f[PT_TI1] = nt2;
-                fi2 = nt2 - old2;
-                V3 i2 = scale(t2, fi2);
-                applyImpulse(S, B, ga, neg(i2), ra);
-                applyImpulse(S, B, gb, i2, rb);
-              }
-            }
+            solveManifoldVel<true>(cfg, B, S, VL, VA, ml[m0 + i], m0 + i, g0);
           }
           __syncthreads();
         }
@@ -1339,10 +1557,10 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
     for (uint32_t s = tid; s < MB; s += bs) {
       uint32_t g = g0 + s;
       if (!B.active[g] || B.type[g] == VXP_BODY_STATIC || B.sleeping[g]) continue;
-      V3 p = add(f4v(S.ws_pos[g]), scale(f4v(S.ws_lv[g]), h));
+      V3 p = add(f4v(S.ws_pos[g]), scale(f4v(VL[s]), h));
       S.ws_pos[g] = v4(p);
       if (!cfg.disable_rotation) {
-        Q4 q = integrateRotation(f4q(S.ws_rot[g]), f4v(S.ws_av[g]), h);
+        Q4 q = integrateRotation(f4q(S.ws_rot[g]), f4v(VA[s]), h);
         S.ws_rot[g] = q4(q);
       }
       if (B.type[g] == VXP_BODY_DYNAMIC)
@@ -1355,47 +1573,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
       for (uint32_t c = 0; c < ncol; c++) {
         for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
           uint32_t i = S.order[m0 + j];
-          const DevManifold& d = ml[m0 + i];
-          uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
-          float fr = d.m.friction;
-          for (uint32_t k = 0; k < d.m.contact_count; k++) {
-            PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
-            V3 n = P.g3(PT_NORMAL), pt = P.g3(PT_POINT), ra = P.g3(PT_RA), rb = P.g3(PT_RB);
-            V3 rel = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
-            float vn = dot(rel, n);
-            float ni = P.f[PT_NI];
-            float imp = -P.f[PT_NM] * vn;
-            float nw = fmaxf(ni + imp, 0.0f);
-            imp = nw - ni;
-            P.f[PT_NI] = nw;
-            V3 Pn = scale(n, imp);
-            applyImpulse(S, B, ga, neg(Pn), ra);
-            applyImpulse(S, B, gb, Pn, rb);
-            if (cfg.enable_friction && fr > 0.0f) {
-              V3 rel1 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
-              float maxf = fr * P.f[PT_NI];
-              V3 t1 = P.g3(PT_T1), t2 = P.g3(PT_T2);
-              float vt = dot(rel1, t1);
-              float fi = -P.f[PT_TM0] * vt;
-              float old = P.f[PT_TI0];
-              float nt = clampz(old + fi, -maxf, maxf);
-              P.f[PT_TI0] = nt;
-              fi = nt - old;
-              V3 i1 = scale(t1, fi);
-              applyImpulse(S, B, ga, neg(i1), ra);
-              applyImpulse(S, B, gb, i1, rb);
-              V3 rel2 = sub(pointVel(S, gb, pt), pointVel(S, ga, pt));
-              float vt2 = dot(rel2, t2);
-              float fi2 = -P.f[PT_TM1] * vt2;
-              float old2 = P.f[PT_TI1];
-              float nt2 = clampz(old2 + fi2, -maxf, maxf);
-              P.f[PT_TI1] = nt2;
-              fi2 = nt2 - old2;
-              V3 i2 = scale(t2, fi2);
-              applyImpulse(S, B, ga, neg(i2), ra);
-              applyImpulse(S, B, gb, i2, rb);
-            }
-          }
+          solveManifoldVel<false>(cfg, B, S, VL, VA, ml[m0 + i], m0 + i, g0);
         }
         __syncthreads();
       }
@@ -1418,6 +1596,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
           float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
           bool awa = ta == VXP_BODY_DYNAMIC && !B.sleeping[ga];
           bool awb = tb == VXP_BODY_DYNAMIC && !B.sleeping[gb];
+          M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
           for (uint32_t k = 0; k < d.m.contact_count; k++) {
             PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
             V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
@@ -1433,7 +1612,6 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
             float corr = fmaxf(cfg.pcr * (sep + cfg.slop), -cfg.mpc);
             V3 ra = sub(wa, pa), rb = sub(wb, pb);
             V3 rna = cross(ra, n), rnb = cross(rb, n);
-            M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
             float k_ = ima + imb;
             k_ += dot(rna, mulVec(iia, rna));
             k_ += dot(rnb, mulVec(iib, rnb));
@@ -1474,7 +1652,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
     if (!B.active[g] || B.type[g] == VXP_BODY_STATIC) continue;
     V3 p = f4v(S.ws_pos[g]);
     Q4 q = f4q(S.ws_rot[g]);
-    V3 lv = f4v(S.ws_lv[g]), av = f4v(S.ws_av[g]);
+    V3 lv = f4v(VL[s]), av = f4v(VA[s]);
     bool fq = isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w);
     if (!finite3(p) || !fq) {
       V3 op = f4v(B.pos[g]);
@@ -1539,6 +1717,7 @@ __global__ __launch_bounds__(1024) void k_solve(DevCfg cfg, DevBodies B, WorldSt
     sx.position_iterations_run = pos_run;
     sx.manifolds = M;
     sx.solver_manifolds = s_nsolve;
+    sx.solver_points = s_points;
     sx.manifolds_skipped_sleeping = s_skipped;
     sx.colors = ncol;
   }
@@ -1598,14 +1777,16 @@ void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const
                      const WorldState* ws, uint32_t n_slots) {
   k_fat_aabb<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, C, ws, n_slots);
 }
-void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint64_t* keys,
-                       uint32_t* vals, uint32_t n_slots) {
-  k_sweep_keys<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, keys, vals, n_slots);
+void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw) {
+  k_world_axis<<<nw, 256, 0, s>>>(cfg, B, ws, nw);
+  k_sweep_keys<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, ws, keys, vals, n_slots);
 }
-void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint64_t* keys,
-                        const uint32_t* order, uint32_t n_slots, uint64_t* out, uint32_t* count,
-                        uint32_t cap) {
-  k_sweep_pairs<<<cdiv(n_slots, 128), 128, 0, s>>>(cfg, B, keys, order, n_slots, out, count, cap);
+void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                        const uint64_t* keys, const uint32_t* order, uint32_t n_slots,
+                        uint64_t* out, uint32_t* count, uint32_t cap) {
+  k_sweep_pairs<<<cdiv(n_slots, 128), 128, 0, s>>>(cfg, B, ws, keys, order, n_slots, out, count,
+                                                   cap);
 }
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
@@ -1614,7 +1795,7 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, uint32_t nw) {
-  k_cache_replay<<<cdiv(nw, 64), 64, 0, s>>>(cfg, ws, st, pr, keys, nw);
+  k_cache_replay<<<nw, 64, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(cfg, ws, st, pr, keys, nw);
 }
 void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts) {
   if (n) k_chunk_counts<<<cdiv(n, 256), 256, 0, s>>>(pr, n, counts);
@@ -1664,8 +1845,12 @@ void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
 }
 void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                   vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
-                  uint32_t nw) {
-  k_solve<<<nw, 1024, 0, s>>>(cfg, B, ws, st, ml, S, prio);
+                  uint32_t nw, size_t lds_limit) {
+  size_t vel_bytes = (size_t)cfg.max_bodies * 2 * sizeof(float4);
+  if (vel_bytes <= lds_limit)
+    k_solve<true><<<nw, kSolveThreads, vel_bytes, s>>>(cfg, B, ws, st, ml, S, prio);
+  else
+    k_solve<false><<<nw, kSolveThreads, 0, s>>>(cfg, B, ws, st, ml, S, prio);
 }
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc) {

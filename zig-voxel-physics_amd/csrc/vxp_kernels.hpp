@@ -11,11 +11,11 @@ void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
                        vxp_stats* st, uint32_t nw);
 void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                      const WorldState* ws, uint32_t n_slots);
-void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint64_t* keys,
-                       uint32_t* vals, uint32_t n_slots);
-void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint64_t* keys,
-                        const uint32_t* order, uint32_t n_slots, uint64_t* out, uint32_t* count,
-                        uint32_t cap);
+void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw);
+void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                        const uint64_t* keys, const uint32_t* order, uint32_t n_slots,
+                        uint64_t* out, uint32_t* count, uint32_t cap);
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev);
@@ -47,7 +47,7 @@ void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
                          const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw);
 void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                   vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
-                  uint32_t nw);
+                  uint32_t nw, size_t lds_limit);
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc);
 

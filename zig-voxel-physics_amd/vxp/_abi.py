@@ -106,7 +106,8 @@ class Stats(C.Structure):
         "cache_hits", "cache_misses", "manifolds_skipped_sleeping",
         "island_count", "largest_island_size", "substeps_this_frame",
         "manifolds", "solver_manifolds", "colors", "position_iterations_run",
-        "steps_executed")] + [("max_penetration", C.c_float), ("accumulated_time", C.c_float)]
+        "steps_executed", "awake_bodies_stepped", "solver_points")] + [
+        ("max_penetration", C.c_float), ("accumulated_time", C.c_float)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -199,7 +200,11 @@ SIGNATURES = {
 # product-only exports (declared in include/vxp.h)
 PRODUCT_ONLY = {
     "config_default": (C.c_int, [C.POINTER(Config)]),
+    "profile_enable": (C.c_int, [_P, C.c_int]),
+    "profile_get": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
 }
+KERNEL_NAMES = ["step_begin", "fat_aabb", "pairs", "pair_process", "cache_replay", "probe",
+                "reduce", "compact", "islands", "solve"]
 
 
 class VxpError(RuntimeError):

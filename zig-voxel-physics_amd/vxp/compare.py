@@ -34,13 +34,19 @@ def manifold_fields(ms):
     return out
 
 
+def _neq_exact(x, y):
+    """IEEE inequality with NaN == NaN: exact up to the sign of zero, which the
+    reference leaves unspecified (Zig @min/@max lower to minnum/maxnum)."""
+    return ~((x == y) | (np.isnan(x) & np.isnan(y)))
+
+
 def diff_bodies(a, b, rtol=0.0, atol=0.0):
-    """First differing (slot, field) or None.  rtol/atol = 0 means bit-exact."""
+    """First differing (slot, field) or None.  rtol/atol = 0 means exact."""
     for f in BODY_FIELDS:
         x, y = a[f], b[f]
         if x.dtype.kind == "f":
             if rtol == 0 and atol == 0:
-                bad = np.nonzero((x.view(np.uint32) != y.view(np.uint32)).reshape(len(x), -1).any(axis=1))[0]
+                bad = np.nonzero(_neq_exact(x, y).reshape(len(x), -1).any(axis=1))[0]
             else:
                 bad = np.nonzero((~np.isclose(x, y, rtol=rtol, atol=atol)).reshape(len(x), -1).any(axis=1))[0]
         else:
@@ -67,12 +73,12 @@ def diff_manifolds(ma, mb, rtol=0.0, atol=0.0):
     for k in ("fr", "f"):
         x, y = a[k], b[k]
         if rtol == 0 and atol == 0:
-            bad = np.nonzero((x.view(np.uint32) != y.view(np.uint32)).reshape(len(ma), -1).any(axis=1))[0]
+            bad = np.nonzero(_neq_exact(x, y).reshape(len(ma), -1).any(axis=1))[0]
         else:
             bad = np.nonzero((~np.isclose(x, y, rtol=rtol, atol=atol)).reshape(len(ma), -1).any(axis=1))[0]
         if len(bad):
             i = int(bad[0])
-            j = np.nonzero((x[i] != y[i]).reshape(-1))[0][:4]
+            j = np.nonzero(_neq_exact(x[i], y[i]).reshape(-1))[0][:4]
             return (f"manifold {i} {tuple(a['ids'][i])} field {k} idx {j.tolist()}: "
                     f"{x[i].reshape(-1)[j]} vs {y[i].reshape(-1)[j]}")
     return None
@@ -82,7 +88,8 @@ STAT_KEYS = ("broad_phase_pairs", "narrow_phase_tested", "narrow_phase_colliding
              "contacts_generated", "voxels_checked", "pairs_skipped_sleeping", "cache_hits",
              "cache_misses", "manifolds_skipped_sleeping", "island_count", "largest_island_size",
              "substeps_this_frame", "manifolds", "solver_manifolds", "colors",
-             "position_iterations_run", "steps_executed", "sleeping_bodies", "active_bodies",
+             "position_iterations_run", "steps_executed", "awake_bodies_stepped",
+             "solver_points", "sleeping_bodies", "active_bodies",
              "total_bodies", "dynamic_bodies", "static_bodies")
 
 
@@ -90,7 +97,7 @@ def diff_stats(sa, sb):
     for k in STAT_KEYS:
         if sa[k] != sb[k]:
             return f"stat {k}: {sa[k]} vs {sb[k]}"
-    if np.float32(sa["max_penetration"]).view(np.uint32) != np.float32(sb["max_penetration"]).view(np.uint32):
+    if sa["max_penetration"] != sb["max_penetration"]:
         return f"stat max_penetration: {sa['max_penetration']} vs {sb['max_penetration']}"
     return None
 
@@ -114,6 +121,14 @@ def lockstep(scene, lib_a, lib_b, frames, cfg_over=None, worlds=None, check_ever
             msg = msg or diff_manifolds(sa.manifolds(w), sb.manifolds(w), rtol, atol)
             msg = msg or diff_bodies(sa.bodies(w), sb.bodies(w), rtol, atol)
             msg = msg or diff_stats(sa.stats(w), sb.stats(w))
+            if not msg and sa.cfg.enable_islands:
+                ia, ib = sa.islands(w), sb.islands(w)
+                if len(ia[0]) != len(ib[0]) or (ia[0] != ib[0]).any() or (ia[1] != ib[1]).any():
+                    msg = f"island labels differ ({len(ia[0])} vs {len(ib[0])} bodies)"
+            if not msg and sa.cfg.enable_coloring:
+                ca, cb = sa.coloring(w), sb.coloring(w)
+                if len(ca) != len(cb) or (ca != cb).any():
+                    msg = "reference colouring differs"
             if msg:
                 return f, f"world {w}: {msg}", (sa, sb)
         if log:

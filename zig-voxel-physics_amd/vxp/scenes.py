@@ -279,48 +279,56 @@ def _rot_aabb_half(half, q):
     return np.abs(m) @ np.asarray(half)
 
 
-class _Placer:
-    """Rejection sampler over a spatial hash: keeps initial world AABBs disjoint."""
+class _HeightMap:
+    """Drop placement at voxel resolution over a square footprint.  A body keeps
+    its sampled x, z and orientation; its rotated voxels are binned into
+    vertical columns of `cell` metres, and it rests at max(sampled y, the
+    lowest height where each of its columns clears the current column tops).
+    Initial bodies never share a column interval, so they start disjoint and
+    nest into one another as a pile would."""
 
-    def __init__(self, cell=4.0):
+    def __init__(self, half, cell=0.2, margin=12.0):
         self.cell = cell
-        self.grid = {}
-        self.boxes = []
+        self.off = half + margin
+        self.n = int(math.ceil(2 * self.off / cell)) + 1
+        self.top = np.full((self.n, self.n), -1e9)
 
-    def _cells(self, lo, hi):
+    def place(self, x, z, y, pts):
+        """pts: rotated voxel centres relative to the body origin (N, 3)."""
         c = self.cell
-        for i in range(int(math.floor(lo[0] / c)), int(math.floor(hi[0] / c)) + 1):
-            for j in range(int(math.floor(lo[1] / c)), int(math.floor(hi[1] / c)) + 1):
-                for k in range(int(math.floor(lo[2] / c)), int(math.floor(hi[2] / c)) + 1):
-                    yield (i, j, k)
-
-    def free(self, lo, hi):
-        for key in self._cells(lo, hi):
-            for b in self.grid.get(key, ()):
-                bl, bh = self.boxes[b]
-                if (lo < bh).all() and (hi > bl).all():
-                    return False
-        return True
-
-    def add(self, lo, hi):
-        idx = len(self.boxes)
-        self.boxes.append((lo, hi))
-        for key in self._cells(lo, hi):
-            self.grid.setdefault(key, []).append(idx)
+        ix = np.clip(np.floor((x + pts[:, 0] + self.off) / c).astype(np.int64), 0, self.n - 1)
+        iz = np.clip(np.floor((z + pts[:, 2] + self.off) / c).astype(np.int64), 0, self.n - 1)
+        key = ix * self.n + iz
+        order = np.argsort(key, kind="stable")
+        ks, ys = key[order], pts[order, 1]
+        uniq, start = np.unique(ks, return_index=True)
+        lo = np.minimum.reduceat(ys, start) - c * 0.5
+        hi = np.maximum.reduceat(ys, start) + c * 0.5
+        ui, uk = uniq // self.n, uniq % self.n
+        need = float(np.max(self.top[ui, uk] - lo)) + 0.02
+        oy = max(y, need)
+        self.top[ui, uk] = np.maximum(self.top[ui, uk], oy + hi)
+        return oy
 
 
-def _drop(rng, placer, half_local, center_off, xz, y_range, tries=48):
-    lo_y, hi_y = y_range
-    while True:
-        for _ in range(tries):
-            q = rng.quat()
-            h = _rot_aabb_half(half_local, q) + 0.05
-            p = np.array([rng.uniform(-xz, xz), rng.uniform(lo_y, hi_y), rng.uniform(-xz, xz)])
-            c = p + center_off
-            if placer.free(c - h, c + h):
-                placer.add(c - h, c + h)
-                return tuple(float(v) for v in p), q
-        hi_y += 5.0  # the stated drop column is full: extend it upward
+def _rot_matrix(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def _drop(rng, hmap, local_pts, xz, y_range):
+    q = rng.quat()
+    x, y, z = rng.uniform(-xz, xz), rng.uniform(*y_range), rng.uniform(-xz, xz)
+    pts = local_pts @ _rot_matrix(q).T
+    oy = hmap.place(x, z, y, pts)
+    return (float(x), float(oy), float(z)), q
+
+
+def _voxel_centres(vox, origin, vs):
+    ii = np.argwhere((vox & 7) != 0)
+    return (ii + np.asarray(origin)[None, :] + 0.5) * vs
 
 
 def _shape_half(vox, origin, vs):
@@ -358,7 +366,7 @@ def c0_small(seed=42, frames=300, n_side=8):
 
 
 def _mixed_world(rng, sc, n_bodies, ground, vs, xz, y_range, shape_pool=None, box_cache=None):
-    placer = _Placer()
+    placer = _HeightMap(xz)
     gspec = BodySpec(_abi.STATIC, (0.0, -2 * vs, 0.0), (0.0, 0.0, 0.0, 1.0), ground)
     bodies = [gspec]
     for k in range(n_bodies):
@@ -366,21 +374,21 @@ def _mixed_world(rng, sc, n_bodies, ground, vs, xz, y_range, shape_pool=None, bo
             sx, sy, sz = rng.randint(3, 16), rng.randint(3, 16), rng.randint(3, 16)
             key = (sx, sy, sz)
             if box_cache is not None and key in box_cache:
-                cid, half, off = box_cache[key]
+                cid, pts = box_cache[key]
             else:
                 vox, org = box_shape(sx, sy, sz)
                 cid = _add_collider(sc, vox, org, vs, f"box{sx}x{sy}x{sz}")
-                half, off = _shape_half(vox, org, vs)
+                pts = _voxel_centres(vox, org, vs)
                 if box_cache is not None:
-                    box_cache[key] = (cid, half, off)
+                    box_cache[key] = (cid, pts)
         else:
             if shape_pool is not None:
-                cid, half, off = shape_pool[rng.randint(0, len(shape_pool) - 1)]
+                cid, pts = shape_pool[rng.randint(0, len(shape_pool) - 1)]
             else:
                 vox, org = blob_shape(rng)
                 cid = _add_collider(sc, vox, org, vs, "blob16")
-                half, off = _shape_half(vox, org, vs)
-        pos, q = _drop(rng, placer, half, off, xz, y_range)
+                pts = _voxel_centres(vox, org, vs)
+        pos, q = _drop(rng, placer, pts, xz, y_range)
         bodies.append(BodySpec(_abi.DYNAMIC, pos, q, cid, 1.0))
     return bodies
 
@@ -388,7 +396,8 @@ def _mixed_world(rng, sc, n_bodies, ground, vs, xz, y_range, shape_pool=None, bo
 def c1_large(seed=43, frames=600, n_bodies=4096, ground_vox=256):
     """4096 bodies (half boxes with sides in {3..16} voxels, half 16^3 40% blobs),
     0.2 m voxels, dropped from 1-40 m into a 32x32 m footprint over a
-    256x2x256 ground.  The drop column is extended upward where it is full."""
+    256x2x256 ground.  Bodies that would overlap earlier ones are lifted onto
+    them (voxel-column heightmap drop), so the scene starts as a deep column."""
     rng = SplitMix64(seed)
     vs = 0.2
     sc = Scene("c1_large", scene_config("c1", 1 / 60, 4, 8, n_bodies + 1), 1 / 60, frames)
@@ -438,8 +447,7 @@ def batched_worlds(seed=45, n_worlds=512, lo=8, hi=1024, frames=300, blob_pool=2
     for _ in range(blob_pool):
         vox, org = blob_shape(rng)
         cid = _add_collider(sc, vox, org, vs, "blob16")
-        half, off = _shape_half(vox, org, vs)
-        pool.append((cid, half, off))
+        pool.append((cid, _voxel_centres(vox, org, vs)))
     box_cache = {}
     for n in counts:
         w = _mixed_world(rng, sc, n, g, vs, 6.0, (1.0, 20.0), shape_pool=pool, box_cache=box_cache)
