@@ -232,6 +232,10 @@ int vxp_coloring_get_w(vxp_system* sys, uint32_t world, uint8_t* colors, uint32_
 int vxp_profile_enable(vxp_system* sys, int on);
 /* Accumulated device time (ms) and launch count of kernel `id`. */
 int vxp_profile_get(vxp_system* sys, uint32_t id, double* total_ms, uint64_t* launches);
+/* Solver phase timestamps of the last profiled step of world `w` (device
+ * wall clock ticks): start, colouring start/end, records built, substeps
+ * done, position passes done, sleep update done (ts[0..6]). */
+int vxp_profile_solver_phases(vxp_system* sys, uint32_t world, uint64_t ts[8]);
 
 /* Last error message of this thread (never NULL). */
 const char* vxp_last_error(void);

@@ -228,8 +228,8 @@ struct vxp_system {
   DevBuf d_ml[2];
   int cur = 0;
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
-  DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_pt,
-      s_prio;
+  DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
+      s_pts, s_prio, s_tstamp;
   uint32_t n_pairs = 0, n_manifolds = 0;
   uint32_t pair_cap = 0, cand_cap = 0;
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
@@ -624,7 +624,9 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->s_color.ensure((size_t)nm1 * 4));
   HIP_TRY(s->s_order.ensure((size_t)nm1 * 4));
   HIP_TRY(s->s_prio.ensure((size_t)nm1 * 4));
-  HIP_TRY(s->s_pt.ensure((size_t)nm1 * 4 * kPtStride * 4));
+  HIP_TRY(s->s_rec.ensure((size_t)nm1 * 9 * sizeof(float4)));
+  HIP_TRY(s->s_pts.ensure((size_t)nm1 * 4 * 8 * sizeof(float4)));
+  HIP_TRY(s->s_tstamp.ensure((size_t)nw * 16 * 8));
   SolverScratch S;
   S.ws_pos = s->s_pos.as<float4>();
   S.ws_rot = s->s_rot.as<float4>();
@@ -636,7 +638,9 @@ int stepAll(vxp_system* s) {
   S.adj = s->s_adj.as<uint32_t>();
   S.color = s->s_color.as<int32_t>();
   S.order = s->s_order.as<uint32_t>();
-  S.pt = s->s_pt.as<float>();
+  S.rec = s->s_rec.as<float4>();
+  S.pts = s->s_pts.as<float4>();
+  S.tstamp = s->prof ? s->s_tstamp.as<uint64_t>() : nullptr;
   pe = profBegin(s);
   launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw, s->lds_limit);
   if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
@@ -696,6 +700,13 @@ int vxp_profile_enable(vxp_system* s, int on) {
       s->prof_ms[i] = 0.0;
       s->prof_n[i] = 0;
     }
+  return VXP_OK;
+}
+
+int vxp_profile_solver_phases(vxp_system* s, uint32_t w, uint64_t* ts) {
+  if (!s || !ts || w >= s->nw || !s->s_tstamp.p) return VXP_ERR_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(ts, s->s_tstamp.as<uint64_t>() + (size_t)w * 16, 8 * 8, hipMemcpyDeviceToHost));
   return VXP_OK;
 }
 
@@ -849,7 +860,7 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->d_total, &s->d_ml[0], &s->d_ml[1], &s->d_isl_parent, &s->d_isl_end,
                     &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors, &s->s_pos, &s->s_rot,
                     &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,
-                    &s->s_color, &s->s_order, &s->s_pt, &s->s_prio};
+                    &s->s_color, &s->s_order, &s->s_rec, &s->s_pts, &s->s_prio, &s->s_tstamp};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);

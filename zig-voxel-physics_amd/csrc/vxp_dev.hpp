@@ -142,8 +142,9 @@ struct SolverScratch {
   uint32_t *adj_cnt, *adj_off, *adj;         // CSR of awake-dynamic incidences
   int32_t* color;                            // per manifold
   uint32_t* order;                           // per manifold: grouped by colour
-  float* pt;                                 // kPtStride floats per point
+  float4* rec;                               // 9 per manifold (colour order)
+  float4* pts;                               // 4 x 8 per manifold (colour order)
+  uint64_t* tstamp;                          // optional phase timestamps (16 per world)
 };
-constexpr int kPtStride = 40;
 
 }  // namespace vxd

@@ -1082,20 +1082,154 @@ __global__ void k_ref_coloring(DevCfg cfg, const WorldState* ws, const DevManifo
 // ---------------------------------------------------------------------------
 // solver: ContactSolver.solve for one world per block (src/solver.zig:494-561)
 // ---------------------------------------------------------------------------
-struct PtView {
-  float* f;
-  __device__ V3 g3(int o) const { return V3{f[o], f[o + 1], f[o + 2]}; }
-  __device__ void s3(int o, V3 v) const {
-    f[o] = v.x;
-    f[o + 1] = v.y;
-    f[o + 2] = v.z;
+constexpr uint32_t kMaxColors = 1024;
+constexpr uint32_t kSolveThreads = 1024;
+
+// Solver records, indexed by position j of a solved manifold in colour order.
+// Point record (8 x float4):
+//   q0 normal.xyz nm | q1 point.xyz tm0 | q2 r_a.xyz tm1 | q3 r_b.xyz bias
+//   q4 t1.xyz ni     | q5 t2.xyz ti0    | q6 anchor_a.xyz ti1 | q7 anchor_b.xyz base_sep
+// Manifold record (9 x float4), refreshed by prepare every substep:
+//   m0 (slot a, slot b, friction, count) | m1 pos_a.xyz im_a | m2 pos_b.xyz im_b
+//   m3 (awake a, awake b, restitution, -) | m4..m8 inv_inertia_world a[9], b[9]
+constexpr int kPtQ = 8;
+constexpr int kMrQ = 9;
+
+__device__ __forceinline__ float4 f4(float x, float y, float z, float w) {
+  return make_float4(x, y, z, w);
+}
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+struct MRec {
+  uint32_t sa, sb, n;
+  float friction;
+  V3 pa, pb;
+  float ima, imb;
+  bool awa, awb;
+  M3 iia, iib;
+};
+
+__device__ __forceinline__ MRec loadMRec(const float4* R) {
+  float4 m0 = R[0], m1 = R[1], m2 = R[2], m3 = R[3], m4 = R[4], m5 = R[5], m6 = R[6], m7 = R[7],
+         m8 = R[8];
+  MRec r;
+  r.sa = __float_as_uint(m0.x);
+  r.sb = __float_as_uint(m0.y);
+  r.friction = m0.z;
+  r.n = __float_as_uint(m0.w);
+  r.pa = f4v(m1);
+  r.ima = m1.w;
+  r.pb = f4v(m2);
+  r.imb = m2.w;
+  r.awa = __float_as_uint(m3.x) != 0;
+  r.awb = __float_as_uint(m3.y) != 0;
+  r.iia = M3{{m4.x, m4.y, m4.z, m4.w, m5.x, m5.y, m5.z, m5.w, m6.x}};
+  r.iib = M3{{m6.y, m6.z, m6.w, m7.x, m7.y, m7.z, m7.w, m8.x, m8.y}};
+  return r;
+}
+
+// One constraint's two bodies held in registers for a whole manifold.
+struct VPair {
+  V3 la, aa, lb, ab;
+};
+
+__device__ __forceinline__ V3 velAt(V3 lv, V3 av, V3 pos, V3 pt) {
+  return add(lv, cross(av, sub(pt, pos)));
+}
+// BodyState.applyImpulseAtPoint (src/solver.zig:316-322) on register state
+__device__ __forceinline__ void impulseTo(bool awake, V3& lv, V3& av, float im, const M3& ii,
+                                          V3 imp, V3 r) {
+  if (!awake) return;
+  lv = add(lv, scale(imp, im));
+  av = add(av, mulVec(ii, cross(r, imp)));
+}
+
+// solveVelocityConstraints for one manifold (src/solver.zig:732-787) with
+// solveFriction (src/solver.zig:790-838); kBias selects the soft biased pass
+// or the hard relax pass (src/solver.zig:848-856).
+template <bool kBias, class VelPtr>
+__device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, VelPtr VA,
+                                                 const float4* R, float4* P) {
+  MRec m = loadMRec(R);
+  V3 la = f4v(VL[m.sa]), aa = f4v(VA[m.sa]), lb = f4v(VL[m.sb]), ab = f4v(VA[m.sb]);
+  for (uint32_t k = 0; k < m.n; k++) {
+    float4* Q = P + k * kPtQ;
+    float4 q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5], q6 = Q[6];
+    V3 nrm = f4v(q0), pt = f4v(q1), ra = f4v(q2), rb = f4v(q3), t1 = f4v(q4), t2 = f4v(q5);
+    float nm = q0.w, tm0 = q1.w, tm1 = q2.w, bias = q3.w;
+    float ni = q4.w, ti0 = q5.w, ti1 = q6.w;
+    V3 rel = sub(velAt(lb, ab, m.pb, pt), velAt(la, aa, m.pa, pt));
+    float vn = dot(rel, nrm);
+    float imp;
+    if (kBias) imp = -cfg.mass_coeff * nm * (vn + bias) - cfg.impulse_coeff * ni;
+    else imp = -nm * vn;
+    float nw = fmaxf(ni + imp, 0.0f);
+    imp = nw - ni;
+    ni = nw;
+    V3 Pn = scale(nrm, imp);
+    impulseTo(m.awa, la, aa, m.ima, m.iia, neg(Pn), ra);
+    impulseTo(m.awb, lb, ab, m.imb, m.iib, Pn, rb);
+    if (cfg.enable_friction && m.friction > 0.0f) {
+      V3 rel1 = sub(velAt(lb, ab, m.pb, pt), velAt(la, aa, m.pa, pt));
+      float maxf = m.friction * ni;
+      float vt = dot(rel1, t1);
+      float fi = -tm0 * vt;
+      float old = ti0;
+      ti0 = clampz(old + fi, -maxf, maxf);
+      fi = ti0 - old;
+      V3 i1 = scale(t1, fi);
+      impulseTo(m.awa, la, aa, m.ima, m.iia, neg(i1), ra);
+      impulseTo(m.awb, lb, ab, m.imb, m.iib, i1, rb);
+      V3 rel2 = sub(velAt(lb, ab, m.pb, pt), velAt(la, aa, m.pa, pt));
+      float vt2 = dot(rel2, t2);
+      float fi2 = -tm1 * vt2;
+      float old2 = ti1;
+      ti1 = clampz(old2 + fi2, -maxf, maxf);
+      fi2 = ti1 - old2;
+      V3 i2 = scale(t2, fi2);
+      impulseTo(m.awa, la, aa, m.ima, m.iia, neg(i2), ra);
+      impulseTo(m.awb, lb, ab, m.imb, m.iib, i2, rb);
+      Q[5].w = ti0;
+      Q[6].w = ti1;
+    }
+    Q[4].w = ni;
   }
-};
-enum {
-  PT_POINT = 0, PT_NORMAL = 3, PT_LA = 6, PT_LB = 9, PT_RA = 12, PT_RB = 15, PT_BASE = 18,
-  PT_NM = 19, PT_TM0 = 20, PT_TM1 = 21, PT_T1 = 22, PT_T2 = 25, PT_BIAS = 28, PT_NI = 29,
-  PT_TI0 = 30, PT_TI1 = 31, PT_VN0 = 32, PT_PEN = 33
-};
+  if (m.awa) {
+    VL[m.sa] = v4(la);
+    VA[m.sa] = v4(aa);
+  }
+  if (m.awb) {
+    VL[m.sb] = v4(lb);
+    VA[m.sb] = v4(ab);
+  }
+}
+
+// warmStart for one manifold (src/solver.zig:702-729)
+template <class VelPtr>
+__device__ __forceinline__ void warmManifold(float wss, VelPtr VL, VelPtr VA, const float4* R,
+                                             float4* P) {
+  MRec m = loadMRec(R);
+  V3 la = f4v(VL[m.sa]), aa = f4v(VA[m.sa]), lb = f4v(VL[m.sb]), ab = f4v(VA[m.sb]);
+  for (uint32_t k = 0; k < m.n; k++) {
+    float4* Q = P + k * kPtQ;
+    float4 q0 = Q[0], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5], q6 = Q[6];
+    float sn = q4.w * wss, s1 = q5.w * wss, s2 = q6.w * wss;
+    V3 imp = add(add(scale(f4v(q0), sn), scale(f4v(q4), s1)), scale(f4v(q5), s2));
+    impulseTo(m.awa, la, aa, m.ima, m.iia, neg(imp), f4v(q2));
+    impulseTo(m.awb, lb, ab, m.imb, m.iib, imp, f4v(q3));
+    Q[4].w = sn;
+    Q[5].w = s1;
+    Q[6].w = s2;
+  }
+  if (m.awa) {
+    VL[m.sa] = v4(la);
+    VA[m.sa] = v4(aa);
+  }
+  if (m.awb) {
+    VL[m.sb] = v4(lb);
+    VA[m.sb] = v4(ab);
+  }
+}
 
 __device__ __forceinline__ M3 loadM3(const float* p) {
   M3 r;
@@ -1141,99 +1275,8 @@ __device__ void blockScanExclusive(uint32_t* data, uint32_t n, uint32_t* tmp /*b
   __syncthreads();
 }
 
-constexpr uint32_t kMaxColors = 1024;
-constexpr uint32_t kSolveThreads = 1024;
-
-// One body of a constraint during the velocity passes: velocities live in
-// VL/VA (LDS or global), everything else is read-only within a pass.
-struct SBody {
-  uint32_t s;       // slot within the world
-  V3 pos;
-  M3 ii;
-  float im;
-  bool awake;       // dynamic and not sleeping: receives impulses
-};
-
-template <class VelPtr>
-__device__ __forceinline__ V3 pointVel(VelPtr VL, VelPtr VA, const SBody& b, V3 pt) {
-  return add(f4v(VL[b.s]), cross(f4v(VA[b.s]), sub(pt, b.pos)));
-}
-// BodyState.applyImpulseAtPoint (src/solver.zig:316-322)
-template <class VelPtr>
-__device__ __forceinline__ void applyImpulse(VelPtr VL, VelPtr VA, const SBody& b, V3 imp, V3 r) {
-  if (!b.awake) return;
-  VL[b.s] = v4(add(f4v(VL[b.s]), scale(imp, b.im)));
-  VA[b.s] = v4(add(f4v(VA[b.s]), mulVec(b.ii, cross(r, imp))));
-}
-
-__device__ __forceinline__ SBody loadSBody(const DevBodies& B, const SolverScratch& S, uint32_t g0,
-                                           uint32_t s) {
-  uint32_t g = g0 + s;
-  SBody b;
-  b.s = s;
-  b.pos = f4v(S.ws_pos[g]);
-  b.ii = loadM3(S.ws_ii + (size_t)g * 9);
-  b.im = B.inv_mass[g];
-  b.awake = B.type[g] == VXP_BODY_DYNAMIC && !B.sleeping[g];
-  return b;
-}
-
-// solveVelocityConstraints for one manifold (src/solver.zig:732-787) with
-// solveFriction (src/solver.zig:790-838); kBias selects the soft biased pass
-// or the hard relax pass (src/solver.zig:848-856).
-template <bool kBias, class VelPtr>
-__device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, const DevBodies& B,
-                                                 const SolverScratch& S, VelPtr VL, VelPtr VA,
-                                                 const DevManifold& d, uint32_t gi, uint32_t g0) {
-  SBody A = loadSBody(B, S, g0, d.sa), Bb = loadSBody(B, S, g0, d.sb);
-  float fr = d.m.friction;
-  uint32_t n = d.m.contact_count;
-  for (uint32_t k = 0; k < n; k++) {
-    float* P = S.pt + ((size_t)gi * 4 + k) * kPtStride;
-    V3 nrm{P[PT_NORMAL], P[PT_NORMAL + 1], P[PT_NORMAL + 2]};
-    V3 pt{P[PT_POINT], P[PT_POINT + 1], P[PT_POINT + 2]};
-    V3 ra{P[PT_RA], P[PT_RA + 1], P[PT_RA + 2]};
-    V3 rb{P[PT_RB], P[PT_RB + 1], P[PT_RB + 2]};
-    V3 t1{P[PT_T1], P[PT_T1 + 1], P[PT_T1 + 2]};
-    V3 t2{P[PT_T2], P[PT_T2 + 1], P[PT_T2 + 2]};
-    float nm = P[PT_NM], tm0 = P[PT_TM0], tm1 = P[PT_TM1], bias = P[PT_BIAS];
-    float ni = P[PT_NI], ti0 = P[PT_TI0], ti1 = P[PT_TI1];
-    V3 rel = sub(pointVel(VL, VA, Bb, pt), pointVel(VL, VA, A, pt));
-    float vn = dot(rel, nrm);
-    float imp;
-    if (kBias) imp = -cfg.mass_coeff * nm * (vn + bias) - cfg.impulse_coeff * ni;
-    else imp = -nm * vn;
-    float nw = fmaxf(ni + imp, 0.0f);
-    imp = nw - ni;
-    ni = nw;
-    V3 Pn = scale(nrm, imp);
-    applyImpulse(VL, VA, A, neg(Pn), ra);
-    applyImpulse(VL, VA, Bb, Pn, rb);
-    if (cfg.enable_friction && fr > 0.0f) {
-      V3 rel1 = sub(pointVel(VL, VA, Bb, pt), pointVel(VL, VA, A, pt));
-      float maxf = fr * ni;
-      float vt = dot(rel1, t1);
-      float fi = -tm0 * vt;
-      float old = ti0;
-      ti0 = clampz(old + fi, -maxf, maxf);
-      fi = ti0 - old;
-      V3 i1 = scale(t1, fi);
-      applyImpulse(VL, VA, A, neg(i1), ra);
-      applyImpulse(VL, VA, Bb, i1, rb);
-      V3 rel2 = sub(pointVel(VL, VA, Bb, pt), pointVel(VL, VA, A, pt));
-      float vt2 = dot(rel2, t2);
-      float fi2 = -tm1 * vt2;
-      float old2 = ti1;
-      ti1 = clampz(old2 + fi2, -maxf, maxf);
-      fi2 = ti1 - old2;
-      V3 i2 = scale(t2, fi2);
-      applyImpulse(VL, VA, A, neg(i2), ra);
-      applyImpulse(VL, VA, Bb, i2, rb);
-      P[PT_TI0] = ti0;
-      P[PT_TI1] = ti1;
-    }
-    P[PT_NI] = ni;
-  }
+__device__ __forceinline__ void stamp(uint64_t* t, int i) {
+  if (t && threadIdx.x == 0) t[blockIdx.x * 16 + i] = wall_clock64();
 }
 
 // ContactSolver.solve for one world per block (src/solver.zig:494-561).  With
@@ -1241,7 +1284,8 @@ __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, const DevBod
 template <bool kLds>
 __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B, WorldState* ws,
                                                          vxp_stats* st, DevManifold* ml,
-                                                         SolverScratch S, uint32_t* prio_buf) {
+                                                         SolverScratch S, uint32_t* prio_buf,
+                                                         uint64_t* tstamp) {
   extern __shared__ float4 dyn_vel[];
   uint32_t w = blockIdx.x;
   if (ws[w].skip) return;
@@ -1262,6 +1306,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     VL = S.ws_lv + g0;
     VA = S.ws_av + g0;
   }
+  float4* REC = S.rec + (size_t)m0 * kMrQ;
+  float4* PTS = S.pts + (size_t)m0 * 4 * kPtQ;
+  stamp(tstamp, 0);
 
   // body working state (BodyState.fromBody, src/solver.zig:231-243)
   for (uint32_t s = tid; s < MB; s += bs) {
@@ -1328,6 +1375,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       adj[S.adj_off[gb] + atomicAdd(&S.adj_cnt[gb], 1u)] = i;
   }
   __syncthreads();
+  stamp(tstamp, 1);
   // Jones-Plassmann rounds: a manifold takes the smallest colour unused by its
   // lower-(prio, key) neighbours once they are all coloured (= greedy in
   // ascending (prio, key) order).
@@ -1384,7 +1432,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     __syncthreads();
   }
   __syncthreads();
-  const uint32_t ncol = s_ncol;
+  stamp(tstamp, 2);
+  const uint32_t ncol = s_ncol, nsolve = s_nsolve;
   // group manifolds by colour (order inside a colour is free: no shared body)
   for (uint32_t c = tid; c <= kMaxColors; c += bs) ccount[c] = 0;
   __syncthreads();
@@ -1403,43 +1452,42 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     int32_t c = S.color[m0 + i];
     if (c >= 0) S.order[m0 + ccount[c] + atomicAdd(&ccursor[c], 1u)] = i;
   }
-  // ContactSolver.addManifold (src/solver.zig:449-491)
-  for (uint32_t i = tid; i < M; i += bs) {
-    if (S.color[m0 + i] < 0) continue;
+  __syncthreads();
+  // ContactSolver.addManifold (src/solver.zig:449-491), records in colour order
+  for (uint32_t j = tid; j < nsolve; j += bs) {
+    uint32_t i = S.order[m0 + j];
     const DevManifold& d = ml[m0 + i];
     uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
     V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
     Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+    float4* R = REC + (size_t)j * kMrQ;
+    R[0] = make_float4(__uint_as_float(d.sa), __uint_as_float(d.sb), d.m.friction,
+                       __uint_as_float(d.m.contact_count));
+    R[3] = make_float4(__uint_as_float(B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga]),
+                       __uint_as_float(B.type[gb] == VXP_BODY_DYNAMIC && !B.sleeping[gb]),
+                       d.m.restitution, 0.0f);
     for (uint32_t k = 0; k < d.m.contact_count; k++) {
       const vxp_contact& c = d.m.contacts[k];
-      PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+      float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
       V3 la{c.local_anchor_a[0], c.local_anchor_a[1], c.local_anchor_a[2]};
       V3 lb{c.local_anchor_b[0], c.local_anchor_b[1], c.local_anchor_b[2]};
       V3 n{c.normal[0], c.normal[1], c.normal[2]};
       V3 wa = add(qrotate(qa, la), pa);
       V3 wb = add(qrotate(qb, lb), pb);
       V3 ad = sub(wb, wa);
-      P.s3(PT_POINT, V3{c.point[0], c.point[1], c.point[2]});
-      P.s3(PT_NORMAL, n);
-      P.s3(PT_LA, la);
-      P.s3(PT_LB, lb);
-      P.s3(PT_RA, sub(wa, pa));
-      P.s3(PT_RB, sub(wb, pb));
-      P.f[PT_BASE] = -c.penetration - dot(ad, n);
-      P.f[PT_NM] = 0.0f;
-      P.f[PT_TM0] = P.f[PT_TM1] = 0.0f;
-      P.s3(PT_T1, V3{0.f, 0.f, 0.f});
-      P.s3(PT_T2, V3{0.f, 0.f, 0.f});
-      P.f[PT_BIAS] = 0.0f;
-      P.f[PT_NI] = c.normal_impulse;
-      P.f[PT_TI0] = c.tangent_impulse[0];
-      P.f[PT_TI1] = c.tangent_impulse[1];
-      P.f[PT_VN0] = 0.0f;
-      P.f[PT_PEN] = c.penetration;
+      Q[0] = f4(n, 0.0f);
+      Q[1] = f4(V3{c.point[0], c.point[1], c.point[2]}, 0.0f);
+      Q[2] = f4(sub(wa, pa), 0.0f);
+      Q[3] = f4(sub(wb, pb), 0.0f);
+      Q[4] = f4(V3{0.f, 0.f, 0.f}, c.normal_impulse);
+      Q[5] = f4(V3{0.f, 0.f, 0.f}, c.tangent_impulse[0]);
+      Q[6] = f4(la, c.tangent_impulse[1]);
+      Q[7] = f4(lb, -c.penetration - dot(ad, n));
     }
   }
   __syncthreads();
-  const bool has_c = s_nsolve > 0;
+  stamp(tstamp, 3);
+  const bool has_c = nsolve > 0;
   const float h = cfg.h;
   const V3 grav{cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]};
   const float dt_inv = h > kEps ? 1.0f / h : 0.0f;
@@ -1465,90 +1513,87 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     }
     __syncthreads();
     if (has_c) {
-      // prepareConstraints (src/solver.zig:608-699)
-      for (uint32_t i = tid; i < M; i += bs) {
-        if (S.color[m0 + i] < 0) continue;
-        const DevManifold& d = ml[m0 + i];
-        uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+      // prepareConstraints (src/solver.zig:608-699); also refreshes the
+      // per-manifold copies of body position, mass and world inertia
+      for (uint32_t j = tid; j < nsolve; j += bs) {
+        float4* R = REC + (size_t)j * kMrQ;
+        float4 r0 = R[0], r3 = R[3];
+        uint32_t sa = __float_as_uint(r0.x), sb = __float_as_uint(r0.y), n = __float_as_uint(r0.w);
+        float restitution = r3.z;
+        uint32_t ga = g0 + sa, gb = g0 + sb;
         V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
         Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
         float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
         M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
-        V3 lva = f4v(VL[d.sa]), ava = f4v(VA[d.sa]), lvb = f4v(VL[d.sb]), avb = f4v(VA[d.sb]);
-        for (uint32_t k = 0; k < d.m.contact_count; k++) {
-          PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
-          V3 n = P.g3(PT_NORMAL);
-          V3 wa = add(qrotate(qa, P.g3(PT_LA)), pa);
-          V3 wb = add(qrotate(qb, P.g3(PT_LB)), pb);
+        R[1] = f4(pa, ima);
+        R[2] = f4(pb, imb);
+        R[4] = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
+        R[5] = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
+        R[6] = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
+        R[7] = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
+        R[8] = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
+        V3 lva = f4v(VL[sa]), ava = f4v(VA[sa]), lvb = f4v(VL[sb]), avb = f4v(VA[sb]);
+        for (uint32_t k = 0; k < n; k++) {
+          float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
+          float4 q0 = Q[0], q6 = Q[6], q7 = Q[7];
+          V3 nrm = f4v(q0);
+          V3 wa = add(qrotate(qa, f4v(q6)), pa);
+          V3 wb = add(qrotate(qb, f4v(q7)), pb);
           V3 ra = sub(wa, pa), rb = sub(wb, pb);
-          P.s3(PT_RA, ra);
-          P.s3(PT_RB, rb);
           V3 pt = scale(add(wa, wb), 0.5f);
-          P.s3(PT_POINT, pt);
           V3 ad = sub(wb, wa);
-          float sep = P.f[PT_BASE] + dot(ad, n);
-          V3 rna = cross(ra, n), rnb = cross(rb, n);
+          float sep = q7.w + dot(ad, nrm);
+          V3 rna = cross(ra, nrm), rnb = cross(rb, nrm);
           float k_ = ima + imb;
           k_ += dot(rna, mulVec(iia, rna));
           k_ += dot(rnb, mulVec(iib, rnb));
-          P.f[PT_NM] = k_ > kEps ? 1.0f / k_ : 0.0f;
+          float nm = k_ > kEps ? 1.0f / k_ : 0.0f;
           V3 va = add(lva, cross(ava, sub(pt, pa)));
           V3 vb = add(lvb, cross(avb, sub(pt, pb)));
           V3 rel = sub(vb, va);
-          float vn = dot(rel, n);
-          P.f[PT_VN0] = vn;
+          float vn = dot(rel, nrm);
           float bias;
           if (sep > 0.0f) bias = sep * dt_inv;
           else bias = fmaxf(cfg.bias_rate * sep, -0.2f);
-          if (cfg.enable_restitution && vn < -1.0f) bias = fmaxf(bias, -d.m.restitution * vn);
-          P.f[PT_BIAS] = bias;
+          if (cfg.enable_restitution && vn < -1.0f) bias = fmaxf(bias, -restitution * vn);
+          float tm0 = Q[1].w, tm1 = Q[2].w;
+          V3 t1 = f4v(Q[4]), t2 = f4v(Q[5]);
           if (cfg.enable_friction) {
-            V3 t1 = computeOrthogonal(n);
-            V3 t2 = cross(n, t1);
-            P.s3(PT_T1, t1);
-            P.s3(PT_T2, t2);
+            t1 = computeOrthogonal(nrm);
+            t2 = cross(nrm, t1);
             V3 a1 = cross(ra, t1), b1 = cross(rb, t1);
             float k1 = ima + imb;
             k1 += dot(a1, mulVec(iia, a1));
             k1 += dot(b1, mulVec(iib, b1));
-            P.f[PT_TM0] = k1 > kEps ? 1.0f / k1 : 0.0f;
+            tm0 = k1 > kEps ? 1.0f / k1 : 0.0f;
             V3 a2 = cross(ra, t2), b2 = cross(rb, t2);
             float k2 = ima + imb;
             k2 += dot(a2, mulVec(iia, a2));
             k2 += dot(b2, mulVec(iib, b2));
-            P.f[PT_TM1] = k2 > kEps ? 1.0f / k2 : 0.0f;
+            tm1 = k2 > kEps ? 1.0f / k2 : 0.0f;
           }
+          Q[0] = f4(nrm, nm);
+          Q[1] = f4(pt, tm0);
+          Q[2] = f4(ra, tm1);
+          Q[3] = f4(rb, bias);
+          Q[4] = f4(t1, Q[4].w);
+          Q[5] = f4(t2, Q[5].w);
         }
       }
       __syncthreads();
       // warmStart (src/solver.zig:702-729), substep 0 only
       if (cfg.warm_starting && sub_i == 0) {
         for (uint32_t c = 0; c < ncol; c++) {
-          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
-            uint32_t i = S.order[m0 + j];
-            const DevManifold& d = ml[m0 + i];
-            SBody A = loadSBody(B, S, g0, d.sa), Bb = loadSBody(B, S, g0, d.sb);
-            for (uint32_t k = 0; k < d.m.contact_count; k++) {
-              PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
-              float sn = P.f[PT_NI] * cfg.wss, s1 = P.f[PT_TI0] * cfg.wss, s2 = P.f[PT_TI1] * cfg.wss;
-              V3 imp = add(add(scale(P.g3(PT_NORMAL), sn), scale(P.g3(PT_T1), s1)), scale(P.g3(PT_T2), s2));
-              applyImpulse(VL, VA, A, neg(imp), P.g3(PT_RA));
-              applyImpulse(VL, VA, Bb, imp, P.g3(PT_RB));
-              P.f[PT_NI] = sn;
-              P.f[PT_TI0] = s1;
-              P.f[PT_TI1] = s2;
-            }
-          }
+          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
+            warmManifold(cfg.wss, VL, VA, REC + (size_t)j * kMrQ, PTS + (size_t)j * 4 * kPtQ);
           __syncthreads();
         }
       }
       // solveVelocityConstraints with bias
       for (uint32_t it = 0; it < cfg.vel_iters; it++) {
         for (uint32_t c = 0; c < ncol; c++) {
-          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
-            uint32_t i = S.order[m0 + j];
-            solveManifoldVel<true>(cfg, B, S, VL, VA, ml[m0 + i], m0 + i, g0);
-          }
+          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
+            solveManifoldVel<true>(cfg, VL, VA, REC + (size_t)j * kMrQ, PTS + (size_t)j * 4 * kPtQ);
           __syncthreads();
         }
       }
@@ -1568,17 +1613,31 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
                 worldInvInertia(f4q(S.ws_rot[g]), loadM3(B.iil + (size_t)g * 9)));
     }
     __syncthreads();
-    // relaxConstraints (src/solver.zig:848-856): hard, unbiased
+    // relaxConstraints (src/solver.zig:848-856): hard, unbiased; the bodies'
+    // positions and inertia moved in integrate, the point data did not
     if (has_c) {
+      for (uint32_t j = tid; j < nsolve; j += bs) {
+        float4* R = REC + (size_t)j * kMrQ;
+        float4 r0 = R[0];
+        uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
+        M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
+        R[1] = f4(f4v(S.ws_pos[ga]), R[1].w);
+        R[2] = f4(f4v(S.ws_pos[gb]), R[2].w);
+        R[4] = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
+        R[5] = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
+        R[6] = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
+        R[7] = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
+        R[8] = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
+      }
+      __syncthreads();
       for (uint32_t c = 0; c < ncol; c++) {
-        for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
-          uint32_t i = S.order[m0 + j];
-          solveManifoldVel<false>(cfg, B, S, VL, VA, ml[m0 + i], m0 + i, g0);
-        }
+        for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
+          solveManifoldVel<false>(cfg, VL, VA, REC + (size_t)j * kMrQ, PTS + (size_t)j * 4 * kPtQ);
         __syncthreads();
       }
     }
   }
+  stamp(tstamp, 4);
   // solvePositionConstraints (src/solver.zig:860-923) with the early exit
   float maxpen = 0.0f;
   uint32_t pos_run = 0;
@@ -1588,43 +1647,53 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       float minsep = 0.0f;
       for (uint32_t c = 0; c < ncol; c++) {
         for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
-          uint32_t i = S.order[m0 + j];
-          const DevManifold& d = ml[m0 + i];
-          uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+          float4* R = REC + (size_t)j * kMrQ;
+          float4 r0 = R[0];
+          uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
+          uint32_t n = __float_as_uint(r0.w);
           uint8_t ta = B.type[ga], tb = B.type[gb];
           if (ta == VXP_BODY_STATIC && tb == VXP_BODY_STATIC) continue;
           float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
           bool awa = ta == VXP_BODY_DYNAMIC && !B.sleeping[ga];
           bool awb = tb == VXP_BODY_DYNAMIC && !B.sleeping[gb];
           M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
-          for (uint32_t k = 0; k < d.m.contact_count; k++) {
-            PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
-            V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
-            Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
-            V3 n = P.g3(PT_NORMAL);
-            V3 wa = add(qrotate(qa, P.g3(PT_LA)), pa);
-            V3 wb = add(qrotate(qb, P.g3(PT_LB)), pb);
+          V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
+          Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+          for (uint32_t k = 0; k < n; k++) {
+            float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
+            V3 nrm = f4v(Q[0]);
+            float4 q6 = Q[6], q7 = Q[7];
+            V3 wa = add(qrotate(qa, f4v(q6)), pa);
+            V3 wb = add(qrotate(qb, f4v(q7)), pb);
             V3 ad = sub(wb, wa);
-            float sep = P.f[PT_BASE] + dot(ad, n);
+            float sep = q7.w + dot(ad, nrm);
             minsep = fminf(minsep, sep);
             maxpen = fmaxf(maxpen, -sep);
             if (sep >= -cfg.slop) continue;
             float corr = fmaxf(cfg.pcr * (sep + cfg.slop), -cfg.mpc);
             V3 ra = sub(wa, pa), rb = sub(wb, pb);
-            V3 rna = cross(ra, n), rnb = cross(rb, n);
+            V3 rna = cross(ra, nrm), rnb = cross(rb, nrm);
             float k_ = ima + imb;
             k_ += dot(rna, mulVec(iia, rna));
             k_ += dot(rnb, mulVec(iib, rnb));
             if (k_ < kEps) continue;
-            V3 Pi = scale(n, -corr / k_);
+            V3 Pi = scale(nrm, -corr / k_);
             if (awa) {
-              S.ws_pos[ga] = v4(sub(pa, scale(Pi, ima)));
-              S.ws_rot[ga] = q4(integrateRotationSmall(qa, mulVec(iia, cross(ra, neg(Pi)))));
+              pa = sub(pa, scale(Pi, ima));
+              qa = integrateRotationSmall(qa, mulVec(iia, cross(ra, neg(Pi))));
             }
             if (awb) {
-              S.ws_pos[gb] = v4(add(pb, scale(Pi, imb)));
-              S.ws_rot[gb] = q4(integrateRotationSmall(qb, mulVec(iib, cross(rb, Pi))));
+              pb = add(pb, scale(Pi, imb));
+              qb = integrateRotationSmall(qb, mulVec(iib, cross(rb, Pi)));
             }
+          }
+          if (awa) {
+            S.ws_pos[ga] = v4(pa);
+            S.ws_rot[ga] = q4(qa);
+          }
+          if (awb) {
+            S.ws_pos[gb] = v4(pb);
+            S.ws_rot[gb] = q4(qb);
           }
         }
         __syncthreads();
@@ -1646,6 +1715,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     if (tid < d) s_red[tid] = fmaxf(s_red[tid], s_red[tid + d]);
     __syncthreads();
   }
+  stamp(tstamp, 5);
   // writeBackBodies (src/solver.zig:246-296,926-930)
   for (uint32_t s = tid; s < MB; s += bs) {
     uint32_t g = g0 + s;
@@ -1676,18 +1746,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
   }
   // storeImpulses (src/solver.zig:933-958)
   if (has_c && cfg.store_impulses) {
-    for (uint32_t i = tid; i < M; i += bs) {
-      if (S.color[m0 + i] < 0) continue;
+    for (uint32_t j = tid; j < nsolve; j += bs) {
+      uint32_t i = S.order[m0 + j];
       DevManifold& d = ml[m0 + i];
       for (uint32_t k = 0; k < d.m.contact_count; k++) {
-        PtView P{S.pt + ((size_t)(m0 + i) * 4 + k) * kPtStride};
+        const float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
         vxp_contact& c = d.m.contacts[k];
-        c.normal_impulse = P.f[PT_NI];
-        c.tangent_impulse[0] = P.f[PT_TI0];
-        c.tangent_impulse[1] = P.f[PT_TI1];
-        c.normal_mass = P.f[PT_NM];
-        c.tangent_mass[0] = P.f[PT_TM0];
-        c.tangent_mass[1] = P.f[PT_TM1];
+        c.normal_impulse = Q[4].w;
+        c.tangent_impulse[0] = Q[5].w;
+        c.tangent_impulse[1] = Q[6].w;
+        c.normal_mass = Q[0].w;
+        c.tangent_mass[0] = Q[1].w;
+        c.tangent_mass[1] = Q[2].w;
       }
     }
   }
@@ -1711,12 +1781,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       }
     }
   }
+  stamp(tstamp, 6);
   if (tid == 0) {
     vxp_stats& sx = st[w];
     sx.max_penetration = s_red[0];
     sx.position_iterations_run = pos_run;
     sx.manifolds = M;
-    sx.solver_manifolds = s_nsolve;
+    sx.solver_manifolds = nsolve;
     sx.solver_points = s_points;
     sx.manifolds_skipped_sleeping = s_skipped;
     sx.colors = ncol;
@@ -1848,9 +1919,9 @@ void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldSta
                   uint32_t nw, size_t lds_limit) {
   size_t vel_bytes = (size_t)cfg.max_bodies * 2 * sizeof(float4);
   if (vel_bytes <= lds_limit)
-    k_solve<true><<<nw, kSolveThreads, vel_bytes, s>>>(cfg, B, ws, st, ml, S, prio);
+    k_solve<true><<<nw, kSolveThreads, vel_bytes, s>>>(cfg, B, ws, st, ml, S, prio, S.tstamp);
   else
-    k_solve<false><<<nw, kSolveThreads, 0, s>>>(cfg, B, ws, st, ml, S, prio);
+    k_solve<false><<<nw, kSolveThreads, 0, s>>>(cfg, B, ws, st, ml, S, prio, S.tstamp);
 }
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc) {

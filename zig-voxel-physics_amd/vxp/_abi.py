@@ -202,6 +202,7 @@ PRODUCT_ONLY = {
     "config_default": (C.c_int, [C.POINTER(Config)]),
     "profile_enable": (C.c_int, [_P, C.c_int]),
     "profile_get": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "profile_solver_phases": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
 }
 KERNEL_NAMES = ["step_begin", "fat_aabb", "pairs", "pair_process", "cache_replay", "probe",
                 "reduce", "compact", "islands", "solve"]
