@@ -1,12 +1,15 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X step pipeline on the 4096-body scene.
+"""Benchmark of the MI355X step pipeline.
 
-One "step" is one update(dt) frame of the whole world (4 solver substeps).
-The metric is body-substeps/s: awake dynamic bodies at the start of every
-executed step x substeps, divided by wall time, with the world already resident
-in HBM.  Prints one JSON line (see the keys below).
+One "step" is one update(dt) frame of a world (4 solver substeps).  The metric
+is body-substeps/s: awake dynamic bodies at the start of every executed step x
+substeps, divided by the wall time of the update() calls, with the worlds
+already resident in HBM.  `value` is the 4096-body scene on one GPU; the
+"batched" object steps 512 independent worlds of uneven size together and
+compares them with the plain single-world path looped over the same worlds.
+Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps 590] [--warmup 10] [--config c1]
+  python bench.py [--gpus 1] [--steps 590] [--warmup 10] [--batched-steps 290]
 """
 import argparse
 import ctypes as C
@@ -22,19 +25,55 @@ sys.path.insert(0, os.path.join(REPO, "zig-voxel-physics_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def solver_bytes(st, cfg):
-    """Algorithmic HBM bytes of one solver launch (one world-step):
+def solver_bytes(st, substeps, iters):
+    """Algorithmic HBM bytes of one solver launch for one world-step:
     S*[N*208 + K*120 + M*184 + (I+1)*(K*112 + M*176)] + (K*72 + M*176)
     + J*(K*40 + M*192) + N*144 + K*20, with N awake bodies, K contact points and
     M manifolds in the solve, S substeps, I velocity iterations, J executed
     position iterations (FP32 SoA, each datum counted once per mandated pass)."""
-    N = st["awake_bodies_stepped"] // max(st["steps_executed"], 1)
-    K, M = st["solver_points"], st["solver_manifolds"]
-    S, I, J = cfg["substeps"], cfg["velocity_iterations"], st["position_iterations_run"]
+    if not st.steps_executed:
+        return 0
+    N = st.awake_bodies_stepped // st.steps_executed
+    K, M = st.solver_points, st.solver_manifolds
+    S, I, J = substeps, iters, st.position_iterations_run
     if M == 0:
         return S * N * 208 + N * 144
     return (S * (N * 208 + K * 120 + M * 184 + (I + 1) * (K * 112 + M * 176)) + (K * 72 + M * 176)
             + J * (K * 40 + M * 192) + N * 144 + K * 20)
+
+
+def run(sys_, sc, warmup, steps, profile=False):
+    """Step `steps` frames after `warmup`; per-frame wall time of update()."""
+    lib = sys_.lib
+    cfg = sc.config
+    for _ in range(warmup):
+        sys_.update(sc.dt)
+    if profile:
+        lib.profile_enable(sys_.ptr, 1)
+    frame_ms, body_substeps, alg = [], 0, 0
+    for _ in range(steps):
+        t = time.perf_counter()
+        sys_.update(sc.dt)  # synchronous: returns after the frame's kernels finish
+        frame_ms.append((time.perf_counter() - t) * 1e3)
+        for st in sys_.stats_all():
+            body_substeps += st.awake_bodies_stepped * cfg["substeps"]
+            alg += solver_bytes(st, cfg["substeps"], cfg["velocity_iterations"])
+    kern = {}
+    if profile:
+        import vxp
+        for i, name in enumerate(vxp._abi.KERNEL_NAMES):
+            ms, n = C.c_double(), C.c_uint64()
+            lib.profile_get(sys_.ptr, i, C.byref(ms), C.byref(n))
+            kern[name] = (ms.value, n.value)
+        lib.profile_enable(sys_.ptr, 0)
+    wall = sum(frame_ms) / 1e3
+    return dict(value=body_substeps / wall, wall=wall, frame_ms=frame_ms, kern=kern,
+                alg_solver_bytes=alg, body_substeps=body_substeps)
+
+
+def pct(xs, q):
+    s = sorted(xs)
+    return s[min(len(s) - 1, int(q * len(s)))]
 
 
 def main():
@@ -42,9 +81,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=590)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--batched-steps", type=int, default=290)
+    ap.add_argument("--looped-worlds", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -54,48 +95,69 @@ def main():
     import vxp
     from vxp import scenes
 
-    gen = {"c0": scenes.c0_small, "c1": scenes.c1_large, "c2": scenes.c2_rocks}[args.config]
-    sc = gen()
     lib = vxp.product_lib()
-    sys_ = vxp.from_scene(sc, lib=lib)
-    cfg = dict(sc.config)
-    substeps = cfg["substeps"]
-
-    for _ in range(args.warmup):
-        sys_.update(sc.dt)
-    lib.profile_enable(sys_.ptr, 1)
-    frame_ms, body_substeps, alg_solver = [], 0, 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        sys_.update(sc.dt)  # synchronous: returns after the frame's kernels finish
-        frame_ms.append((time.perf_counter() - t) * 1e3)
-        st = sys_.stats()
-        body_substeps += st["awake_bodies_stepped"] * substeps
-        alg_solver += solver_bytes(st, {"substeps": substeps,
-                                        "velocity_iterations": cfg["velocity_iterations"]}) \
-            if st["steps_executed"] else 0
-    wall = time.perf_counter() - t0
-    kern = {}
-    for i, name in enumerate(vxp._abi.KERNEL_NAMES):
-        ms, n = C.c_double(), C.c_uint64()
-        lib.profile_get(sys_.ptr, i, C.byref(ms), C.byref(n))
-        kern[name] = (ms.value, n.value)
-    lib.profile_enable(sys_.ptr, 0)
-    final = sys_.stats()
-    sys_.close()
-
+    # ---- plain single scene: 4096 bodies ----------------------------------------
+    sc = scenes.c1_large()
+    s1 = vxp.from_scene(sc, lib=lib)
+    plain = run(s1, sc, args.warmup, args.steps, profile=True)
+    final = s1.stats()
+    s1.close()
+    kern = plain["kern"]
     dom = max(kern, key=lambda k: kern[k][0])
     solve_ms, solve_n = kern["solve"]
     avg_solve_s = solve_ms / max(solve_n, 1) / 1e3
-    achieved = (alg_solver / max(solve_n, 1)) / avg_solve_s / 1e9 if solve_n else 0.0
+    alg_per_launch = plain["alg_solver_bytes"] / max(solve_n, 1)
+    achieved = alg_per_launch / avg_solve_s / 1e9 if solve_n else 0.0
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "k_solve_c1_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
 
+    # ---- batched: 512 worlds ------------------------------------------------------
+    batched = None
+    if not args.no_batched:
+        sb = scenes.batched_worlds()
+        sysb = vxp.from_scene(sb, lib=lib)
+        rb = run(sysb, sb, args.warmup, args.batched_steps)
+        sysb.close()
+        # the plain single-world path looped over a sample of the same worlds,
+        # over the same frames
+        nl = min(args.looped_worlds, len(sb.worlds))
+        idx = [int(i * len(sb.worlds) / nl) for i in range(nl)]
+        singles = [vxp.from_scene(sb, lib=lib, worlds=[w]) for w in idx]
+        lf = min(args.batched_steps, 60)
+        for s in singles:
+            for _ in range(args.warmup):
+                s.update(sb.dt)
+        loop_t, loop_bs = 0.0, 0
+        for _ in range(lf):
+            for s in singles:
+                t = time.perf_counter()
+                s.update(sb.dt)
+                loop_t += time.perf_counter() - t
+                loop_bs += s.stats_all()[0].awake_bodies_stepped * sb.config["substeps"]
+        for s in singles:
+            s.close()
+        batched = {
+            "value": rb["value"], "unit": "body-substeps/s",
+            "workload": f"{sb.name}: {len(sb.worlds)} worlds, {sb.n_dynamic} dynamic bodies "
+                        f"(log-uniform 8..1024 per world), frames {args.warmup}.."
+                        f"{args.warmup + args.batched_steps - 1}",
+            "ms_per_step": rb["wall"] * 1e3 / args.batched_steps,
+            "p50_step_ms": statistics.median(rb["frame_ms"]),
+            "p99_step_ms": pct(rb["frame_ms"], 0.99),
+            "plain_looped_value": loop_bs / loop_t if loop_t else None,
+            "plain_looped_sample": f"{nl} of the {len(sb.worlds)} worlds stepped one system at a "
+                                   f"time, frames {args.warmup}..{args.warmup + lf - 1}",
+        }
+
+    # ---- CPU baseline: the CPU restatement on the same scene, bounded sample ----
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         ora = vxp.from_scene(sc, lib=vxp.oracle_lib(), cfg_over=dict(cpu_threads=threads))
         gpu2 = vxp.from_scene(sc, lib=lib)
-        cb, frames, tc, tg = 0, 0, 0.0, 0.0
+        cb, gb, frames, tc, tg = 0, 0, 0, 0.0, 0.0
         while tc < args.cpu_seconds and frames < args.warmup + args.steps:
             t = time.perf_counter()
             ora.update(sc.dt)
@@ -103,44 +165,45 @@ def main():
             t = time.perf_counter()
             gpu2.update(sc.dt)
             tg += time.perf_counter() - t
-            cb += ora.stats()["awake_bodies_stepped"] * substeps
+            cb += ora.stats()["awake_bodies_stepped"] * sc.config["substeps"]
+            gb += gpu2.stats()["awake_bodies_stepped"] * sc.config["substeps"]
             frames += 1
         cpu = {"value": cb / tc, "unit": "body-substeps/s", "cores": threads, "kind": "port",
                "sample": f"CPU restatement (oracle/, {threads} narrow-phase threads) on frames "
-                         f"0..{frames - 1} of the same scene ({tc:.1f} s); the GPU on the same "
-                         f"frames: {cb / tg:.0f} body-substeps/s"}
+                         f"0..{frames - 1} of the 4096-body scene ({tc:.1f} s of CPU time); "
+                         f"the GPU on the same frames: {gb / tg:.0f} body-substeps/s"}
         ora.close()
         gpu2.close()
 
     out = {
-        "metric": "body-substeps/s (4096-body scene); p50/p99 step latency",
-        "value": body_substeps / wall,
+        "metric": "body-substeps/s (4096-body scene; 512 batched worlds); p50/p99 step latency",
+        "value": plain["value"],
         "unit": "body-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "p50_step_ms": statistics.median(frame_ms),
-        "p99_step_ms": sorted(frame_ms)[min(len(frame_ms) - 1, int(0.99 * len(frame_ms)))],
+        "ms_per_step": plain["wall"] * 1e3 / args.steps,
+        "p50_step_ms": statistics.median(plain["frame_ms"]),
+        "p99_step_ms": pct(plain["frame_ms"], 0.99),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scene generator, zig-voxel-physics_amd/vxp/scenes.py)",
         "config": {"workload": f"{sc.name}: {sc.n_dynamic} dynamic bodies + static ground, "
-                               f"dt {sc.dt:.5f}, {substeps} substeps, "
-                               f"{cfg['velocity_iterations']} iterations, frames "
+                               f"dt {sc.dt:.5f}, {sc.config['substeps']} substeps, "
+                               f"{sc.config['velocity_iterations']} iterations, frames "
                                f"{args.warmup}..{args.warmup + args.steps - 1}",
                    "worlds": 1},
         "roofline": {"bound": "hbm", "kernel": "k_solve", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": traffic,
                      "avg_launch_ms": avg_solve_s * 1e3, "launches": solve_n,
-                     "alg_bytes_per_launch": alg_solver / max(solve_n, 1),
-                     "dominant_kernel": dom},
+                     "alg_bytes_per_launch": alg_per_launch, "dominant_kernel": dom},
         "kernel_ms": {k: round(v[0], 3) for k, v in kern.items()},
         "final_stats": {k: final[k] for k in ("manifolds", "colors", "sleeping_bodies",
                                               "voxels_checked", "broad_phase_pairs")},
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -198,6 +198,9 @@ int vxp_body_set_state_w(vxp_system* sys, uint32_t world, vxp_handle h, const fl
 /* getStats (src/system.zig:1024-1026) of world `w`. */
 int vxp_stats_get_w(vxp_system* sys, uint32_t world, vxp_stats* out);
 
+/* getStats of every world at once: `out` receives n_worlds entries. */
+int vxp_stats_get_all(vxp_system* sys, vxp_stats* out, uint32_t capacity);
+
 /* getContactManager (src/system.zig:1029-1031): the current manifolds of
  * world `w` in ascending pair-key order. */
 int vxp_manifolds_get_w(vxp_system* sys, uint32_t world, vxp_manifold* out, uint32_t capacity,

@@ -1676,6 +1676,12 @@ int vxo_stats_get_w(vxo_system* s, uint32_t world, vxp_stats* out) {
   return VXP_OK;
 }
 
+int vxo_stats_get_all(vxo_system* s, vxp_stats* out, uint32_t cap) {
+  if (cap < s->worlds.size()) return VXP_ERR_ARGUMENT;
+  for (size_t w = 0; w < s->worlds.size(); w++) out[w] = s->worlds[w]->stats;
+  return VXP_OK;
+}
+
 static void exportContact(const Contact& c, vxp_contact& o) {
   std::memset(&o, 0, sizeof(o));
   o.point[0] = c.point.x; o.point[1] = c.point.y; o.point[2] = c.point.z;

@@ -1090,6 +1090,18 @@ int vxp_stats_get_w(vxp_system* s, uint32_t w, vxp_stats* out) {
   return VXP_OK;
 }
 
+int vxp_stats_get_all(vxp_system* s, vxp_stats* out, uint32_t cap) {
+  if (!s || !out || cap < s->nw) return VXP_ERR_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(out, s->d_stats.p, (size_t)s->nw * sizeof(vxp_stats), hipMemcpyDeviceToHost));
+  for (uint32_t w = 0; w < s->nw; w++) {
+    out[w].total_bodies = s->n_total[w];
+    out[w].dynamic_bodies = s->n_dyn[w];
+    out[w].static_bodies = s->n_static[w];
+  }
+  return VXP_OK;
+}
+
 static int readWorlds(vxp_system* s, std::vector<WorldState>& ws) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   ws.resize(s->nw);

@@ -186,6 +186,7 @@ SIGNATURES = {
                                C.POINTER(C.c_uint32)]),
     "body_set_state_w": (C.c_int, [_P, C.c_uint32, Handle, _f3, _f3, _f3, _f3]),
     "stats_get_w": (C.c_int, [_P, C.c_uint32, C.POINTER(Stats)]),
+    "stats_get_all": (C.c_int, [_P, C.POINTER(Stats), C.c_uint32]),
     "manifolds_get_w": (C.c_int, [_P, C.c_uint32, C.POINTER(Manifold), C.c_uint32,
                                   C.POINTER(C.c_uint32)]),
     "pairs_get_w": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64), C.c_uint32,

@@ -327,7 +327,11 @@ def _drop(rng, hmap, local_pts, xz, y_range):
 
 
 def _voxel_centres(vox, origin, vs):
-    ii = np.argwhere((vox & 7) != 0)
+    """Centres of the surface voxels (every voxel whose type is not .inside): the
+    lowest and highest voxel of any vertical column of a rotated body is a
+    surface voxel, so these give the same column profile as all voxels."""
+    t = vox & 7
+    ii = np.argwhere((t != 0) & (t != _abi.INSIDE))
     return (ii + np.asarray(origin)[None, :] + 0.5) * vs
 
 
@@ -365,8 +369,9 @@ def c0_small(seed=42, frames=300, n_side=8):
     return sc
 
 
-def _mixed_world(rng, sc, n_bodies, ground, vs, xz, y_range, shape_pool=None, box_cache=None):
-    placer = _HeightMap(xz)
+def _mixed_world(rng, sc, n_bodies, ground, vs, xz, y_range, shape_pool=None, box_cache=None,
+                 cell=0.2):
+    placer = _HeightMap(xz, cell=cell)
     gspec = BodySpec(_abi.STATIC, (0.0, -2 * vs, 0.0), (0.0, 0.0, 0.0, 1.0), ground)
     bodies = [gspec]
     for k in range(n_bodies):
@@ -450,7 +455,8 @@ def batched_worlds(seed=45, n_worlds=512, lo=8, hi=1024, frames=300, blob_pool=2
         pool.append((cid, _voxel_centres(vox, org, vs)))
     box_cache = {}
     for n in counts:
-        w = _mixed_world(rng, sc, n, g, vs, 6.0, (1.0, 20.0), shape_pool=pool, box_cache=box_cache)
+        w = _mixed_world(rng, sc, n, g, vs, 6.0, (1.0, 20.0), shape_pool=pool, box_cache=box_cache,
+                         cell=0.4)
         # the 64x1x64 ground is 1 voxel thick at y=0: centre it like c0
         w[0] = BodySpec(_abi.STATIC, (0.0, -vs * 0.5, 0.0), (0.0, 0.0, 0.0, 1.0), g)
         sc.worlds.append(w)

@@ -119,6 +119,12 @@ class VoxelPhysicsSystem:
         self.lib.check(self.lib.stats_get_w(self.ptr, world, C.byref(s)), "stats_get")
         return s.as_dict()
 
+    def stats_all(self):
+        """Stats of every world (one copy)."""
+        arr = (_abi.Stats * self.n_worlds)()
+        self.lib.check(self.lib.stats_get_all(self.ptr, arr, self.n_worlds), "stats_get_all")
+        return arr
+
     def manifolds(self, world=0):
         cnt = C.c_uint32()
         self.lib.check(self.lib.manifolds_get_w(self.ptr, world, None, 0, C.byref(cnt)), "manifolds")
