@@ -6,10 +6,10 @@ VoxelPhysicsSystem API and generates the seeded benchmark scenes.
 """
 from ._abi import (Config, Handle, Lib, VxpError, default_config, oracle_lib, product_lib,
                    STATIC, KINEMATIC, DYNAMIC, EMPTY, CORNER, EDGE, FACE, INSIDE,
-                   SOLVE_ORDER_COLOR, SOLVE_ORDER_CANONICAL)
+                   SOLVE_ORDER_COLOR, SOLVE_ORDER_CANONICAL, voxel_byte)
 from .system import VoxelCollider, VoxelPhysicsSystem, from_scene
 
 __all__ = ["Config", "Handle", "Lib", "VxpError", "default_config", "oracle_lib", "product_lib",
            "VoxelCollider", "VoxelPhysicsSystem", "from_scene", "STATIC", "KINEMATIC", "DYNAMIC",
            "EMPTY", "CORNER", "EDGE", "FACE", "INSIDE", "SOLVE_ORDER_COLOR",
-           "SOLVE_ORDER_CANONICAL"]
+           "SOLVE_ORDER_CANONICAL", "voxel_byte"]

@@ -20,6 +20,11 @@ SOLVE_ORDER_COLOR, SOLVE_ORDER_CANONICAL = 0, 1
 NO_COLLIDER = 0xFFFFFFFF
 
 
+def voxel_byte(vtype, normal=0):
+    """CollisionVoxel byte: u3 type | u5 normal index << 3 (src/collision_voxel.zig:133-191)."""
+    return (vtype & 7) | ((normal & 31) << 3)
+
+
 class Config(C.Structure):
     _fields_ = [
         ("gravity", C.c_float * 3),
