@@ -224,7 +224,7 @@ struct vxp_system {
   DevBuf d_skeys, d_svals, d_skeys2, d_svals2, d_sort_tmp;
   DevBuf d_pairs_raw, d_pairs, d_count, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_wi_vchk,
       d_cand,
-      d_produce, d_pidx, d_staging, d_cstage, d_total;
+      d_produce, d_pidx, d_staging, d_cstage, d_total, d_rstat;
   DevBuf d_ml[2];
   int cur = 0;
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
@@ -519,6 +519,7 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_counts.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_wi_off.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_produce.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_rstat.ensure((size_t)np1 * 16));
   HIP_TRY(s->d_pidx.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_staging.ensure((size_t)np1 * sizeof(DevManifold)));
   HIP_TRY(s->d_cstage.ensure((size_t)np1 * sizeof(CacheVal)));
@@ -566,11 +567,10 @@ int stepAll(vxp_system* s) {
   pe = profBegin(s);
   launch_reduce(st, cfg, B, ws, stats, pr, keys, npairs, s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
-                s->d_cand.as<Cand>(), prev,
+                s->d_cand.as<Cand>(), prev, s->d_cache_keys.as<CacheKeyRec>(),
                 s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>(),
-                s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>());
-  launch_cache_fill(st, cfg, ws, pr, npairs, s->d_cache_keys.as<CacheKeyRec>(),
-                    s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>());
+                s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>(),
+                s->d_rstat.as<uint4>());
   profEnd(s, VXP_K_REDUCE, pe);
   pe = profBegin(s);
   if (npairs) {
@@ -856,7 +856,7 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->d_cache_keys, &s->d_cache_vals, &s->d_skeys, &s->d_svals, &s->d_skeys2,
                     &s->d_svals2, &s->d_sort_tmp, &s->d_pairs_raw, &s->d_pairs, &s->d_count,
                     &s->d_pr, &s->d_counts, &s->d_wi_off, &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_wi_vchk,
-                    &s->d_cand, &s->d_produce, &s->d_pidx, &s->d_staging, &s->d_cstage,
+                    &s->d_cand, &s->d_produce, &s->d_pidx, &s->d_staging, &s->d_cstage, &s->d_rstat,
                     &s->d_total, &s->d_ml[0], &s->d_ml[1], &s->d_isl_parent, &s->d_isl_end,
                     &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors, &s->s_pos, &s->s_rot,
                     &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,

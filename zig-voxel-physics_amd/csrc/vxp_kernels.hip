@@ -9,7 +9,7 @@
 //   k_cache_replay    BrickPairCache get/put in pair order (src/narrow_phase.zig:228-296,1451-1513)
 //   k_probe           detectVoxelCollisions/checkNeighborhood/generateContact (src/narrow_phase.zig:722-922)
 //   k_reduce          ContactManifold.addContact replay + warm start (src/narrow_phase.zig:451-481,1049-1059)
-//   k_cache_fill, k_scatter, k_copy_skipped   cache values, ContactManager.manifolds compaction
+//   k_reduce_tail, k_scatter, k_copy_skipped  cache values + counters, ContactManager.manifolds compaction
 //   k_islands         buildIslandsWithBodyTypes labels (src/islands.zig:318-441)
 //   k_ref_coloring    GraphColoring.build artifact (src/islands.zig:668-736)
 //   k_solve           ContactSolver.solve per world, per-colour Gauss-Seidel (src/solver.zig:494-958),
@@ -479,7 +479,7 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
 // ---------------------------------------------------------------------------
 // BrickPairCache replay: one thread per world walks its cache-eligible pairs in
 // pair order (get, then put on a miss).  Puts reserve their entry at miss time
-// and record (frame, pair) as owner; values are written by k_cache_fill after
+// and record (frame, pair) as owner; values are written by k_reduce_tail after
 // the hits have been read.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t cacheHash(uint32_t a, uint32_t b, uint64_t th) {
@@ -778,146 +778,206 @@ __device__ void warmStartFrom(vxp_manifold& m, const DevManifold* prev, int32_t 
       }
 }
 
-struct ReduceCounts {
-  uint32_t produced, vchk, colliding, contacts;
-};
+__device__ __forceinline__ float rdlane(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+}
 
-__device__ ReduceCounts reducePair(const DevCfg& cfg, const DevBodies& B, WorldState* ws,
-                                   const PairRec* pr, const uint64_t* keys, uint32_t p,
-                                   const uint32_t* wi_off, const uint32_t* wi_cbase,
-                                   const uint32_t* wi_ccnt, const uint32_t* wi_vchk,
-                                   const Cand* cand, const DevManifold* prev,
-                                   const CacheVal* cvals, CacheVal* cstage, DevManifold* staging,
-                                   uint32_t* produce) {
-  ReduceCounts rc{0, 0, 0, 0};
-  const PairRec& r = pr[p];
-  uint32_t f = r.flag;
-  produce[p] = 0;
-  if (f == PF_NONE || f == PF_CACHE) return rc;
-  uint32_t w = r.world;
-  uint64_t key = keys[p];
-  if (f == PF_CARRY) {
-    staging[p] = prev[r.aux];
-    produce[p] = 1;
-    rc.produced = 1;
-    return rc;
+// One wave per pair.  The streaming top-4 of ContactManifold.addContact
+// (src/narrow_phase.zig:451-481) is replayed exactly in candidate order: the
+// wave loads 64 candidates at a time and a lane is visited only if it could
+// change the slots at its turn (fewer than 4 slots, its feature id is in a
+// slot, or some slot is strictly shallower); every other candidate leaves the
+// state unchanged in the sequential order too.  The slots are wave-uniform.
+// Per-pair counters go to rstat and are summed by k_reduce_tail.
+__global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, const WorldState* ws,
+                                                const PairRec* pr, const uint64_t* keys,
+                                                uint32_t n_pairs, const uint32_t* wi_off,
+                                                const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
+                                                const uint32_t* wi_vchk, const Cand* cand,
+                                                const DevManifold* prev, const CacheVal* cvals,
+                                                CacheVal* cstage, DevManifold* staging,
+                                                uint32_t* produce, uint4* rstat) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (p >= n_pairs) return;  // whole wave
+  const uint32_t f = pr[p].flag;
+  if (f == PF_NONE || f == PF_CACHE) {
+    if (lane == 0) {
+      produce[p] = 0;
+      rstat[p] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    return;
   }
-  uint32_t ga = w * cfg.max_bodies + r.sa, gb = w * cfg.max_bodies + r.sb;
-  DevManifold d;
-  d.key = key;
-  d.sa = r.sa;
-  d.sb = r.sb;
-  vxp_manifold& m = d.m;
-  m.body_a_id = r.ida;
-  m.body_b_id = r.idb;
-  m.friction = sqrtf(B.fric[ga] * B.fric[gb]);
-  m.restitution = fmaxf(B.rest[ga], B.rest[gb]);
-  m.contact_count = 0;
-  m.color = kNone;
-  uint32_t vchk;
+  const uint32_t aux = pr[p].aux;
+  if (f == PF_CARRY) {
+    const uint2* src = reinterpret_cast<const uint2*>(prev + aux);
+    uint2* dst = reinterpret_cast<uint2*>(staging + p);
+    for (uint32_t i = lane; i < sizeof(DevManifold) / 8; i += kWave) dst[i] = src[i];
+    if (lane == 0) {
+      produce[p] = 1;
+      rstat[p] = make_uint4(1u, 0u, 0u, 0u);
+    }
+    return;
+  }
+  const uint32_t w = pr[p].world, sa = pr[p].sa, sb = pr[p].sb;
+  const uint32_t ga = w * cfg.max_bodies + sa, gb = w * cfg.max_bodies + sb;
+  uint32_t ncont = 0, vchk = 0;
+  vxp_contact mine;  // contact `lane` of the manifold (lanes 0..3)
   if (f == PF_HIT) {
-    const CacheVal& v = cvals[(size_t)w * cfg.cache_cap + r.aux];
-    if (!v.has_collision) return rc;
-    for (uint32_t i = 0; i < v.count; i++) m.contacts[i] = v.c[i];
-    m.contact_count = v.count;
+    const CacheVal& v = cvals[(size_t)w * cfg.cache_cap + aux];
+    if (!v.has_collision) {
+      if (lane == 0) {
+        produce[p] = 0;
+        rstat[p] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      return;
+    }
+    ncont = v.count;
     vchk = v.vchk;
+    if (lane < ncont) mine = v.c[lane];
   } else {
-    // streaming top-4 (src/narrow_phase.zig:451-481) over candidate indices
-    uint32_t slot[4];
-    float spen[4];
-    uint64_t sfid[4];
-    uint32_t n = 0;
-    uint32_t w0 = wi_off[p];
-    uint32_t w1 = w0 + r.nchunks;
-    uint32_t vsum = 0;
+    const uint32_t w0 = wi_off[p], w1 = w0 + pr[p].nchunks;
+    uint32_t vs = 0;
+    for (uint32_t wi = w0 + lane; wi < w1; wi += kWave) vs += wi_vchk[wi];
+    vchk = waveSum(vs);
+    uint32_t sidx0 = 0, sidx1 = 0, sidx2 = 0, sidx3 = 0;
+    float spen0 = 0.f, spen1 = 0.f, spen2 = 0.f, spen3 = 0.f;
+    uint64_t sfid0 = 0, sfid1 = 0, sfid2 = 0, sfid3 = 0;
+    uint32_t ns = 0;
     for (uint32_t wi = w0; wi < w1; wi++) {
-      vsum += wi_vchk[wi];
-      uint32_t cnt = wi_ccnt[wi], base = wi_cbase[wi];
-      for (uint32_t k = 0; k < cnt; k++) {
-        const Cand& c = cand[base + k];
-        uint64_t fid = ((uint64_t)c.fid_hi << 32) | c.fid_lo;
-        float pen = c.pen;
-        bool done = false;
-        for (uint32_t i = 0; i < n; i++)
-          if (sfid[i] == fid) {
-            slot[i] = base + k;
-            spen[i] = pen;
-            done = true;
-            break;
-          }
-        if (done) continue;
-        if (n < 4) {
-          slot[n] = base + k;
-          spen[n] = pen;
-          sfid[n] = fid;
-          n++;
-          continue;
+      const uint32_t cnt = wi_ccnt[wi], base = wi_cbase[wi];
+      for (uint32_t off = 0; off < cnt; off += kWave) {
+        const bool valid = off + lane < cnt;
+        float pen = 0.0f;
+        uint64_t fid = 0;
+        if (valid) {
+          const Cand& c = cand[base + off + lane];
+          pen = c.pen;
+          fid = ((uint64_t)c.fid_hi << 32) | c.fid_lo;
         }
-        float minp = pen;
-        int32_t mi = -1;
-        for (uint32_t i = 0; i < 4; i++)
-          if (spen[i] < minp) {
-            minp = spen[i];
-            mi = (int32_t)i;
+        int32_t last = -1;
+        while (true) {
+          bool cand_lane = valid && (int32_t)lane > last;
+          if (cand_lane && ns == 4) {
+            bool same = fid == sfid0 || fid == sfid1 || fid == sfid2 || fid == sfid3;
+            bool deeper = spen0 < pen || spen1 < pen || spen2 < pen || spen3 < pen;
+            cand_lane = same || deeper;
           }
-        if (mi >= 0) {
-          slot[mi] = base + k;
-          spen[mi] = pen;
-          sfid[mi] = fid;
+          const uint64_t mask = __ballot(cand_lane);
+          if (!mask) break;
+          const int k = __ffsll((long long)mask) - 1;
+          const float pk = rdlane(pen, k);
+          const uint64_t fk = ((uint64_t)rdlane((uint32_t)(fid >> 32), k) << 32) |
+                              rdlane((uint32_t)fid, k);
+          const uint32_t ik = base + off + (uint32_t)k;
+          last = k;
+          // addContact on the uniform slots
+          if (ns > 0 && sfid0 == fk) { sidx0 = ik; spen0 = pk; continue; }
+          if (ns > 1 && sfid1 == fk) { sidx1 = ik; spen1 = pk; continue; }
+          if (ns > 2 && sfid2 == fk) { sidx2 = ik; spen2 = pk; continue; }
+          if (ns > 3 && sfid3 == fk) { sidx3 = ik; spen3 = pk; continue; }
+          if (ns < 4) {
+            if (ns == 0) { sidx0 = ik; spen0 = pk; sfid0 = fk; }
+            else if (ns == 1) { sidx1 = ik; spen1 = pk; sfid1 = fk; }
+            else if (ns == 2) { sidx2 = ik; spen2 = pk; sfid2 = fk; }
+            else { sidx3 = ik; spen3 = pk; sfid3 = fk; }
+            ns++;
+            continue;
+          }
+          float minp = pk;
+          int mi = -1;
+          if (spen0 < minp) { minp = spen0; mi = 0; }
+          if (spen1 < minp) { minp = spen1; mi = 1; }
+          if (spen2 < minp) { minp = spen2; mi = 2; }
+          if (spen3 < minp) { minp = spen3; mi = 3; }
+          if (mi == 0) { sidx0 = ik; spen0 = pk; sfid0 = fk; }
+          else if (mi == 1) { sidx1 = ik; spen1 = pk; sfid1 = fk; }
+          else if (mi == 2) { sidx2 = ik; spen2 = pk; sfid2 = fk; }
+          else if (mi == 3) { sidx3 = ik; spen3 = pk; sfid3 = fk; }
         }
       }
     }
-    for (uint32_t i = 0; i < n; i++) candToContact(cand[slot[i]], m.contacts[i]);
-    m.contact_count = n;
-    vchk = vsum;
+    ncont = ns;
+    if (lane < ncont) {
+      uint32_t my = lane == 0 ? sidx0 : lane == 1 ? sidx1 : lane == 2 ? sidx2 : sidx3;
+      candToContact(cand[my], mine);
+    }
     if (f == PF_MISS) {
+      // cached value: contacts before warm starting (src/narrow_phase.zig:1501-1509)
       CacheVal& cv = cstage[p];
-      cv.count = n;
-      cv.has_collision = n > 0;
-      cv.vchk = vchk;
-      for (uint32_t i = 0; i < n; i++) cv.c[i] = m.contacts[i];
+      if (lane < ncont) cv.c[lane] = mine;
+      if (lane == 0) {
+        cv.count = ncont;
+        cv.has_collision = ncont > 0;
+        cv.vchk = vchk;
+        cv.pad = 0;
+      }
     }
   }
-  rc.vchk = vchk;
-  if (m.contact_count == 0) return rc;
-  rc.colliding = 1;
-  rc.contacts = m.contact_count;
-  int32_t j = findPrev(prev, ws[w].prev_start, ws[w].prev_start + ws[w].prev_count, key);
-  warmStartFrom(m, prev, j);
-  staging[p] = d;
-  produce[p] = 1;
-  rc.produced = 1;
-  return rc;
+  if (ncont == 0) {
+    if (lane == 0) {
+      produce[p] = 0;
+      rstat[p] = make_uint4(0u, vchk, 0u, 0u);
+    }
+    return;
+  }
+  // ContactManager.addManifold warm start (src/narrow_phase.zig:1011-1021,1049-1059)
+  const uint64_t key = keys[p];
+  const int32_t j = findPrev(prev, ws[w].prev_start, ws[w].prev_start + ws[w].prev_count, key);
+  if (j >= 0 && lane < ncont) {
+    const vxp_manifold& o = prev[j].m;
+    for (uint32_t k = 0; k < o.contact_count; k++)
+      if (mine.feature_id == o.contacts[k].feature_id) {
+        mine.normal_impulse = o.contacts[k].normal_impulse;
+        mine.tangent_impulse[0] = o.contacts[k].tangent_impulse[0];
+        mine.tangent_impulse[1] = o.contacts[k].tangent_impulse[1];
+        break;
+      }
+  }
+  DevManifold& d = staging[p];
+  if (lane < 4) {
+    if (lane >= ncont) memset(&mine, 0, sizeof(mine));
+    d.m.contacts[lane] = mine;
+  }
+  if (lane == 0) {
+    d.key = key;
+    d.sa = sa;
+    d.sb = sb;
+    d.m.body_a_id = pr[p].ida;
+    d.m.body_b_id = pr[p].idb;
+    d.m.friction = sqrtf(B.fric[ga] * B.fric[gb]);
+    d.m.restitution = fmaxf(B.rest[ga], B.rest[gb]);
+    d.m.contact_count = ncont;
+    d.m.color = kNone;
+    produce[p] = 1;
+    rstat[p] = make_uint4(1u, vchk, 1u, ncont);
+  }
 }
 
-__global__ void k_reduce(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st, PairRec* pr,
-                         const uint64_t* keys, uint32_t n_pairs, const uint32_t* wi_off,
-                         const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
-                         const uint32_t* wi_vchk, const Cand* cand,
-                         const DevManifold* prev, const CacheVal* cvals, CacheVal* cstage,
-                         DevManifold* staging, uint32_t* produce) {
+// BrickPairCache.put of this frame's misses (entry owner = the pair that
+// reserved it during the replay), and the per-world narrow-phase counters.
+__global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const PairRec* pr,
+                              uint32_t n_pairs, const CacheKeyRec* keys, CacheVal* vals,
+                              const CacheVal* cstage, const uint4* rstat) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  ReduceCounts c{0, 0, 0, 0};
+  uint4 c = make_uint4(0u, 0u, 0u, 0u);
   uint32_t w = 0;
   if (p < n_pairs) {
-    c = reducePair(cfg, B, ws, pr, keys, p, wi_off, wi_cbase, wi_ccnt, wi_vchk, cand, prev, cvals,
-                   cstage, staging, produce);
-    w = pr[p].world;
+    const PairRec& r = pr[p];
+    w = r.world;
+    c = rstat[p];
+    if (r.flag == PF_MISS) {
+      size_t e = (size_t)w * cfg.cache_cap + r.aux;
+      if (keys[e].owner_frame == ws[w].frame && keys[e].owner_pair == p) vals[e] = cstage[p];
+    }
   }
-  aggAdd(&ws[w].produced, c.produced);
-  aggAdd(&st[w].voxels_checked, c.vchk);
-  aggAdd(&st[w].narrow_phase_colliding, c.colliding);
-  aggAdd(&st[w].contacts_generated, c.contacts);
-}
-
-__global__ void k_cache_fill(DevCfg cfg, const WorldState* ws, const PairRec* pr, uint32_t n_pairs,
-                             const CacheKeyRec* keys, CacheVal* vals, const CacheVal* cstage) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
-  const PairRec& r = pr[p];
-  if (r.flag != PF_MISS) return;
-  size_t e = (size_t)r.world * cfg.cache_cap + r.aux;
-  if (keys[e].owner_frame == ws[r.world].frame && keys[e].owner_pair == p) vals[e] = cstage[p];
+  aggAdd(&ws[w].produced, c.x);
+  aggAdd(&st[w].voxels_checked, c.y);
+  aggAdd(&st[w].narrow_phase_colliding, c.z);
+  aggAdd(&st[w].contacts_generated, c.w);
 }
 
 // New manifold segments: skipped worlds keep their previous segment.
@@ -1883,15 +1943,12 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldSt
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
-                   const CacheVal* cvals, CacheVal* cstage, DevManifold* staging,
-                   uint32_t* produce) {
-  if (n)
-    k_reduce<<<cdiv(n, 64), 64, 0, s>>>(cfg, B, ws, st, pr, keys, n, wi_off, wi_cbase, wi_ccnt,
-                                        wi_vchk, cand, prev, cvals, cstage, staging, produce);
-}
-void launch_cache_fill(hipStream_t s, const DevCfg& cfg, const WorldState* ws, const PairRec* pr,
-                       uint32_t n, const CacheKeyRec* keys, CacheVal* vals, const CacheVal* cstage) {
-  if (n) k_cache_fill<<<cdiv(n, 256), 256, 0, s>>>(cfg, ws, pr, n, keys, vals, cstage);
+                   const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
+                   DevManifold* staging, uint32_t* produce, uint4* rstat) {
+  if (!n) return;
+  k_reduce<<<cdiv(n, 4), 256, 0, s>>>(cfg, B, ws, pr, keys, n, wi_off, wi_cbase, wi_ccnt, wi_vchk,
+                                      cand, prev, cvals, cstage, staging, produce, rstat);
+  k_reduce_tail<<<cdiv(n, 256), 256, 0, s>>>(cfg, ws, st, pr, n, ckeys, cvals, cstage, rstat);
 }
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total) {
   k_world_segments<<<1, 1024, 0, s>>>(ws, nw, total);

@@ -30,10 +30,8 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldSt
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
-                   const CacheVal* cvals, CacheVal* cstage, DevManifold* staging,
-                   uint32_t* produce);
-void launch_cache_fill(hipStream_t s, const DevCfg& cfg, const WorldState* ws, const PairRec* pr,
-                       uint32_t n, const CacheKeyRec* keys, CacheVal* vals, const CacheVal* cstage);
+                   const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
+                   DevManifold* staging, uint32_t* produce, uint4* rstat);
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total);
 void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
                     const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
