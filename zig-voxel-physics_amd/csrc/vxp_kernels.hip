@@ -1145,7 +1145,10 @@ __global__ void k_ref_coloring(DevCfg cfg, const WorldState* ws, const DevManifo
 constexpr uint32_t kMaxColors = 1024;
 constexpr uint32_t kSolveThreads = 1024;
 
-// Solver records, indexed by position j of a solved manifold in colour order.
+// Solver records, indexed by position j of a solved manifold in colour order,
+// stored field-major inside the world's slice (M = the world's manifold count):
+// manifold field q of j at REC[q * M + j], point field q of (j, k) at
+// PTS[q * 4M + k * M + j], so a wave's loads of one field are contiguous.
 // Point record (8 x float4):
 //   q0 normal.xyz nm | q1 point.xyz tm0 | q2 r_a.xyz tm1 | q3 r_b.xyz bias
 //   q4 t1.xyz ni     | q5 t2.xyz ti0    | q6 anchor_a.xyz ti1 | q7 anchor_b.xyz base_sep
@@ -1160,6 +1163,16 @@ __device__ __forceinline__ float4 f4(float x, float y, float z, float w) {
 }
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
+struct SoaRec {
+  float4* rec;
+  float4* pts;
+  size_t M;
+  __device__ __forceinline__ float4& r(uint32_t j, int q) const { return rec[q * M + j]; }
+  __device__ __forceinline__ float4& p(uint32_t j, uint32_t k, int q) const {
+    return pts[q * 4 * M + k * M + j];
+  }
+};
+
 struct MRec {
   uint32_t sa, sb, n;
   float friction;
@@ -1169,9 +1182,9 @@ struct MRec {
   M3 iia, iib;
 };
 
-__device__ __forceinline__ MRec loadMRec(const float4* R) {
-  float4 m0 = R[0], m1 = R[1], m2 = R[2], m3 = R[3], m4 = R[4], m5 = R[5], m6 = R[6], m7 = R[7],
-         m8 = R[8];
+__device__ __forceinline__ MRec loadMRec(const SoaRec& S, uint32_t j) {
+  float4 m0 = S.r(j, 0), m1 = S.r(j, 1), m2 = S.r(j, 2), m3 = S.r(j, 3), m4 = S.r(j, 4),
+         m5 = S.r(j, 5), m6 = S.r(j, 6), m7 = S.r(j, 7), m8 = S.r(j, 8);
   MRec r;
   r.sa = __float_as_uint(m0.x);
   r.sb = __float_as_uint(m0.y);
@@ -1209,12 +1222,12 @@ __device__ __forceinline__ void impulseTo(bool awake, V3& lv, V3& av, float im, 
 // or the hard relax pass (src/solver.zig:848-856).
 template <bool kBias, class VelPtr>
 __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, VelPtr VA,
-                                                 const float4* R, float4* P) {
-  MRec m = loadMRec(R);
+                                                 const SoaRec& S, uint32_t j) {
+  MRec m = loadMRec(S, j);
   V3 la = f4v(VL[m.sa]), aa = f4v(VA[m.sa]), lb = f4v(VL[m.sb]), ab = f4v(VA[m.sb]);
   for (uint32_t k = 0; k < m.n; k++) {
-    float4* Q = P + k * kPtQ;
-    float4 q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5], q6 = Q[6];
+    float4 q0 = S.p(j, k, 0), q1 = S.p(j, k, 1), q2 = S.p(j, k, 2), q3 = S.p(j, k, 3),
+           q4 = S.p(j, k, 4), q5 = S.p(j, k, 5), q6 = S.p(j, k, 6);
     V3 nrm = f4v(q0), pt = f4v(q1), ra = f4v(q2), rb = f4v(q3), t1 = f4v(q4), t2 = f4v(q5);
     float nm = q0.w, tm0 = q1.w, tm1 = q2.w, bias = q3.w;
     float ni = q4.w, ti0 = q5.w, ti1 = q6.w;
@@ -1249,10 +1262,10 @@ __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, V
       V3 i2 = scale(t2, fi2);
       impulseTo(m.awa, la, aa, m.ima, m.iia, neg(i2), ra);
       impulseTo(m.awb, lb, ab, m.imb, m.iib, i2, rb);
-      Q[5].w = ti0;
-      Q[6].w = ti1;
+      S.p(j, k, 5).w = ti0;
+      S.p(j, k, 6).w = ti1;
     }
-    Q[4].w = ni;
+    S.p(j, k, 4).w = ni;
   }
   if (m.awa) {
     VL[m.sa] = v4(la);
@@ -1266,20 +1279,20 @@ __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, V
 
 // warmStart for one manifold (src/solver.zig:702-729)
 template <class VelPtr>
-__device__ __forceinline__ void warmManifold(float wss, VelPtr VL, VelPtr VA, const float4* R,
-                                             float4* P) {
-  MRec m = loadMRec(R);
+__device__ __forceinline__ void warmManifold(float wss, VelPtr VL, VelPtr VA, const SoaRec& S,
+                                             uint32_t j) {
+  MRec m = loadMRec(S, j);
   V3 la = f4v(VL[m.sa]), aa = f4v(VA[m.sa]), lb = f4v(VL[m.sb]), ab = f4v(VA[m.sb]);
   for (uint32_t k = 0; k < m.n; k++) {
-    float4* Q = P + k * kPtQ;
-    float4 q0 = Q[0], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5], q6 = Q[6];
+    float4 q0 = S.p(j, k, 0), q2 = S.p(j, k, 2), q3 = S.p(j, k, 3), q4 = S.p(j, k, 4),
+           q5 = S.p(j, k, 5), q6 = S.p(j, k, 6);
     float sn = q4.w * wss, s1 = q5.w * wss, s2 = q6.w * wss;
     V3 imp = add(add(scale(f4v(q0), sn), scale(f4v(q4), s1)), scale(f4v(q5), s2));
     impulseTo(m.awa, la, aa, m.ima, m.iia, neg(imp), f4v(q2));
     impulseTo(m.awb, lb, ab, m.imb, m.iib, imp, f4v(q3));
-    Q[4].w = sn;
-    Q[5].w = s1;
-    Q[6].w = s2;
+    S.p(j, k, 4).w = sn;
+    S.p(j, k, 5).w = s1;
+    S.p(j, k, 6).w = s2;
   }
   if (m.awa) {
     VL[m.sa] = v4(la);
@@ -1366,8 +1379,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     VL = S.ws_lv + g0;
     VA = S.ws_av + g0;
   }
-  float4* REC = S.rec + (size_t)m0 * kMrQ;
-  float4* PTS = S.pts + (size_t)m0 * 4 * kPtQ;
+  const SoaRec SR{S.rec + (size_t)m0 * kMrQ, S.pts + (size_t)m0 * 4 * kPtQ, M};
   stamp(tstamp, 0);
 
   // body working state (BodyState.fromBody, src/solver.zig:231-243)
@@ -1520,29 +1532,27 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
     V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
     Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
-    float4* R = REC + (size_t)j * kMrQ;
-    R[0] = make_float4(__uint_as_float(d.sa), __uint_as_float(d.sb), d.m.friction,
-                       __uint_as_float(d.m.contact_count));
-    R[3] = make_float4(__uint_as_float(B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga]),
+    SR.r(j, 0) = make_float4(__uint_as_float(d.sa), __uint_as_float(d.sb), d.m.friction,
+                             __uint_as_float(d.m.contact_count));
+    SR.r(j, 3) = make_float4(__uint_as_float(B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga]),
                        __uint_as_float(B.type[gb] == VXP_BODY_DYNAMIC && !B.sleeping[gb]),
                        d.m.restitution, 0.0f);
     for (uint32_t k = 0; k < d.m.contact_count; k++) {
       const vxp_contact& c = d.m.contacts[k];
-      float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
       V3 la{c.local_anchor_a[0], c.local_anchor_a[1], c.local_anchor_a[2]};
       V3 lb{c.local_anchor_b[0], c.local_anchor_b[1], c.local_anchor_b[2]};
       V3 n{c.normal[0], c.normal[1], c.normal[2]};
       V3 wa = add(qrotate(qa, la), pa);
       V3 wb = add(qrotate(qb, lb), pb);
       V3 ad = sub(wb, wa);
-      Q[0] = f4(n, 0.0f);
-      Q[1] = f4(V3{c.point[0], c.point[1], c.point[2]}, 0.0f);
-      Q[2] = f4(sub(wa, pa), 0.0f);
-      Q[3] = f4(sub(wb, pb), 0.0f);
-      Q[4] = f4(V3{0.f, 0.f, 0.f}, c.normal_impulse);
-      Q[5] = f4(V3{0.f, 0.f, 0.f}, c.tangent_impulse[0]);
-      Q[6] = f4(la, c.tangent_impulse[1]);
-      Q[7] = f4(lb, -c.penetration - dot(ad, n));
+      SR.p(j, k, 0) = f4(n, 0.0f);
+      SR.p(j, k, 1) = f4(V3{c.point[0], c.point[1], c.point[2]}, 0.0f);
+      SR.p(j, k, 2) = f4(sub(wa, pa), 0.0f);
+      SR.p(j, k, 3) = f4(sub(wb, pb), 0.0f);
+      SR.p(j, k, 4) = f4(V3{0.f, 0.f, 0.f}, c.normal_impulse);
+      SR.p(j, k, 5) = f4(V3{0.f, 0.f, 0.f}, c.tangent_impulse[0]);
+      SR.p(j, k, 6) = f4(la, c.tangent_impulse[1]);
+      SR.p(j, k, 7) = f4(lb, -c.penetration - dot(ad, n));
     }
   }
   __syncthreads();
@@ -1576,8 +1586,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       // prepareConstraints (src/solver.zig:608-699); also refreshes the
       // per-manifold copies of body position, mass and world inertia
       for (uint32_t j = tid; j < nsolve; j += bs) {
-        float4* R = REC + (size_t)j * kMrQ;
-        float4 r0 = R[0], r3 = R[3];
+        float4 r0 = SR.r(j, 0), r3 = SR.r(j, 3);
         uint32_t sa = __float_as_uint(r0.x), sb = __float_as_uint(r0.y), n = __float_as_uint(r0.w);
         float restitution = r3.z;
         uint32_t ga = g0 + sa, gb = g0 + sb;
@@ -1585,17 +1594,16 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
         Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
         float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
         M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
-        R[1] = f4(pa, ima);
-        R[2] = f4(pb, imb);
-        R[4] = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
-        R[5] = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
-        R[6] = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
-        R[7] = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
-        R[8] = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
+        SR.r(j, 1) = f4(pa, ima);
+        SR.r(j, 2) = f4(pb, imb);
+        SR.r(j, 4) = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
+        SR.r(j, 5) = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
+        SR.r(j, 6) = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
+        SR.r(j, 7) = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
+        SR.r(j, 8) = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
         V3 lva = f4v(VL[sa]), ava = f4v(VA[sa]), lvb = f4v(VL[sb]), avb = f4v(VA[sb]);
         for (uint32_t k = 0; k < n; k++) {
-          float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
-          float4 q0 = Q[0], q6 = Q[6], q7 = Q[7];
+          float4 q0 = SR.p(j, k, 0), q6 = SR.p(j, k, 6), q7 = SR.p(j, k, 7);
           V3 nrm = f4v(q0);
           V3 wa = add(qrotate(qa, f4v(q6)), pa);
           V3 wb = add(qrotate(qb, f4v(q7)), pb);
@@ -1616,8 +1624,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
           if (sep > 0.0f) bias = sep * dt_inv;
           else bias = fmaxf(cfg.bias_rate * sep, -0.2f);
           if (cfg.enable_restitution && vn < -1.0f) bias = fmaxf(bias, -restitution * vn);
-          float tm0 = Q[1].w, tm1 = Q[2].w;
-          V3 t1 = f4v(Q[4]), t2 = f4v(Q[5]);
+          float tm0 = SR.p(j, k, 1).w, tm1 = SR.p(j, k, 2).w;
+          float4 q4 = SR.p(j, k, 4), q5 = SR.p(j, k, 5);
+          V3 t1 = f4v(q4), t2 = f4v(q5);
           if (cfg.enable_friction) {
             t1 = computeOrthogonal(nrm);
             t2 = cross(nrm, t1);
@@ -1632,12 +1641,12 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
             k2 += dot(b2, mulVec(iib, b2));
             tm1 = k2 > kEps ? 1.0f / k2 : 0.0f;
           }
-          Q[0] = f4(nrm, nm);
-          Q[1] = f4(pt, tm0);
-          Q[2] = f4(ra, tm1);
-          Q[3] = f4(rb, bias);
-          Q[4] = f4(t1, Q[4].w);
-          Q[5] = f4(t2, Q[5].w);
+          SR.p(j, k, 0) = f4(nrm, nm);
+          SR.p(j, k, 1) = f4(pt, tm0);
+          SR.p(j, k, 2) = f4(ra, tm1);
+          SR.p(j, k, 3) = f4(rb, bias);
+          SR.p(j, k, 4) = f4(t1, q4.w);
+          SR.p(j, k, 5) = f4(t2, q5.w);
         }
       }
       __syncthreads();
@@ -1645,7 +1654,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       if (cfg.warm_starting && sub_i == 0) {
         for (uint32_t c = 0; c < ncol; c++) {
           for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
-            warmManifold(cfg.wss, VL, VA, REC + (size_t)j * kMrQ, PTS + (size_t)j * 4 * kPtQ);
+            warmManifold(cfg.wss, VL, VA, SR, j);
           __syncthreads();
         }
       }
@@ -1653,7 +1662,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       for (uint32_t it = 0; it < cfg.vel_iters; it++) {
         for (uint32_t c = 0; c < ncol; c++) {
           for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
-            solveManifoldVel<true>(cfg, VL, VA, REC + (size_t)j * kMrQ, PTS + (size_t)j * 4 * kPtQ);
+            solveManifoldVel<true>(cfg, VL, VA, SR, j);
           __syncthreads();
         }
       }
@@ -1677,22 +1686,21 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     // positions and inertia moved in integrate, the point data did not
     if (has_c) {
       for (uint32_t j = tid; j < nsolve; j += bs) {
-        float4* R = REC + (size_t)j * kMrQ;
-        float4 r0 = R[0];
+        float4 r0 = SR.r(j, 0);
         uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
         M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
-        R[1] = f4(f4v(S.ws_pos[ga]), R[1].w);
-        R[2] = f4(f4v(S.ws_pos[gb]), R[2].w);
-        R[4] = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
-        R[5] = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
-        R[6] = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
-        R[7] = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
-        R[8] = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
+        SR.r(j, 1) = f4(f4v(S.ws_pos[ga]), SR.r(j, 1).w);
+        SR.r(j, 2) = f4(f4v(S.ws_pos[gb]), SR.r(j, 2).w);
+        SR.r(j, 4) = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
+        SR.r(j, 5) = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
+        SR.r(j, 6) = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
+        SR.r(j, 7) = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
+        SR.r(j, 8) = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
       }
       __syncthreads();
       for (uint32_t c = 0; c < ncol; c++) {
         for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
-          solveManifoldVel<false>(cfg, VL, VA, REC + (size_t)j * kMrQ, PTS + (size_t)j * 4 * kPtQ);
+          solveManifoldVel<false>(cfg, VL, VA, SR, j);
         __syncthreads();
       }
     }
@@ -1707,8 +1715,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       float minsep = 0.0f;
       for (uint32_t c = 0; c < ncol; c++) {
         for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
-          float4* R = REC + (size_t)j * kMrQ;
-          float4 r0 = R[0];
+          float4 r0 = SR.r(j, 0);
           uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
           uint32_t n = __float_as_uint(r0.w);
           uint8_t ta = B.type[ga], tb = B.type[gb];
@@ -1720,9 +1727,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
           V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
           Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
           for (uint32_t k = 0; k < n; k++) {
-            float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
-            V3 nrm = f4v(Q[0]);
-            float4 q6 = Q[6], q7 = Q[7];
+            V3 nrm = f4v(SR.p(j, k, 0));
+            float4 q6 = SR.p(j, k, 6), q7 = SR.p(j, k, 7);
             V3 wa = add(qrotate(qa, f4v(q6)), pa);
             V3 wb = add(qrotate(qb, f4v(q7)), pb);
             V3 ad = sub(wb, wa);
@@ -1810,14 +1816,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       uint32_t i = S.order[m0 + j];
       DevManifold& d = ml[m0 + i];
       for (uint32_t k = 0; k < d.m.contact_count; k++) {
-        const float4* Q = PTS + ((size_t)j * 4 + k) * kPtQ;
         vxp_contact& c = d.m.contacts[k];
-        c.normal_impulse = Q[4].w;
-        c.tangent_impulse[0] = Q[5].w;
-        c.tangent_impulse[1] = Q[6].w;
-        c.normal_mass = Q[0].w;
-        c.tangent_mass[0] = Q[1].w;
-        c.tangent_mass[1] = Q[2].w;
+        c.normal_impulse = SR.p(j, k, 4).w;
+        c.tangent_impulse[0] = SR.p(j, k, 5).w;
+        c.tangent_impulse[1] = SR.p(j, k, 6).w;
+        c.normal_mass = SR.p(j, k, 0).w;
+        c.tangent_mass[0] = SR.p(j, k, 1).w;
+        c.tangent_mass[1] = SR.p(j, k, 2).w;
       }
     }
   }
