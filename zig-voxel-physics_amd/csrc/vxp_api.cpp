@@ -222,16 +222,15 @@ struct vxp_system {
   // per world / per step device state
   DevBuf d_ws, d_stats, d_cache_keys, d_cache_vals;
   DevBuf d_skeys, d_svals, d_skeys2, d_svals2, d_sort_tmp;
-  DevBuf d_pairs_raw, d_pairs, d_count, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_wi_vchk,
+  DevBuf d_slo, d_shi, d_send, d_wcnt, d_woff, d_swc, d_swo, d_pairs_raw, d_pairs, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_wi_vchk, d_lane_n,
       d_cand,
-      d_produce, d_pidx, d_staging, d_cstage, d_total, d_rstat;
+      d_produce, d_pidx, d_staging, d_cstage, d_total, d_rstat, d_clist, d_cflag, d_cidx, d_wlist, d_step_ctr;
   DevBuf d_ml[2];
   int cur = 0;
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
   DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
       s_pts, s_prio, s_tstamp;
   uint32_t n_pairs = 0, n_manifolds = 0;
-  uint32_t pair_cap = 0, cand_cap = 0;
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   // kernel timing (HIP events on `stream`)
   bool prof = false;
@@ -469,7 +468,8 @@ int stepAll(vxp_system* s) {
   s->cur ^= 1;
   DevManifold* prev = s->d_ml[s->cur ^ 1].as<DevManifold>();
   hipEvent_t pe = profBegin(s);
-  launch_step_begin(st, cfg, B, ws, stats, nw);
+  HIP_TRY(s->d_step_ctr.ensure(kStepCounters * 4));
+  launch_step_begin(st, cfg, B, ws, stats, nw, s->d_step_ctr.as<uint32_t>());
   profEnd(s, VXP_K_STEP_BEGIN, pe);
   pe = profBegin(s);
   launch_fat_aabb(st, cfg, B, C, ws, ns);
@@ -480,27 +480,47 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_svals.ensure((size_t)ns * 4));
   HIP_TRY(s->d_skeys2.ensure((size_t)ns * 8));
   HIP_TRY(s->d_svals2.ensure((size_t)ns * 4));
+  HIP_TRY(s->d_slo.ensure((size_t)ns * 16));
+  HIP_TRY(s->d_shi.ensure((size_t)ns * 16));
   pe = profBegin(s);
   launch_sweep_keys(st, cfg, B, ws, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns, nw);
   size_t tb = sort_pairs_u64_u32_bytes(ns);
   HIP_TRY(s->d_sort_tmp.ensure(tb));
   HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, tb, s->d_skeys.as<uint64_t>(), s->d_skeys2.as<uint64_t>(),
                              s->d_svals.as<uint32_t>(), s->d_svals2.as<uint32_t>(), ns, st));
-  HIP_TRY(s->d_count.ensure(64));
-  uint32_t npairs = 0;
-  for (int attempt = 0; attempt < 8; attempt++) {
-    if (s->pair_cap < 1024) s->pair_cap = std::max<uint32_t>(1024, ns * 8);
-    HIP_TRY(s->d_pairs_raw.ensure((size_t)s->pair_cap * 8));
-    HIP_TRY(hipMemsetAsync(s->d_count.p, 0, 4, st));
-    launch_sweep_pairs(st, cfg, B, ws, s->d_skeys2.as<uint64_t>(), s->d_svals2.as<uint32_t>(), ns,
-                       s->d_pairs_raw.as<uint64_t>(), s->d_count.as<uint32_t>(), s->pair_cap);
-    profEnd(s, VXP_K_PAIRS, pe);
-    pe = nullptr;
-    HIP_TRY(hipMemcpyAsync(&npairs, s->d_count.p, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (npairs <= s->pair_cap) break;
-    s->pair_cap = npairs + npairs / 2;
+  HIP_TRY(s->d_send.ensure((size_t)ns * 4));
+  HIP_TRY(s->d_wcnt.ensure((size_t)(ns + 1) * 4));
+  HIP_TRY(s->d_woff.ensure((size_t)(ns + 1) * 4));
+  launch_sweep_count(st, cfg, B, ws, s->d_svals2.as<uint32_t>(), ns, s->d_slo.as<float4>(),
+                     s->d_shi.as<float4>(), s->d_send.as<uint32_t>(), s->d_wcnt.as<uint32_t>());
+  {
+    size_t cb = exclusive_scan_u32_bytes(ns + 1);
+    HIP_TRY(s->d_sort_tmp.ensure(cb));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wcnt.as<uint32_t>(),
+                               s->d_woff.as<uint32_t>(), ns + 1, st));
   }
+  // two passes over the same work-item partition: count per wave, scan, write
+  const uint32_t swv = sweep_waves();
+  HIP_TRY(s->d_swc.ensure((size_t)(swv + 1) * 4));
+  HIP_TRY(s->d_swo.ensure((size_t)(swv + 1) * 4));
+  launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
+                     s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), s->d_swc.as<uint32_t>(),
+                     nullptr, nullptr, true);
+  {
+    size_t cb = exclusive_scan_u32_bytes(swv + 1);
+    HIP_TRY(s->d_sort_tmp.ensure(cb));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_swc.as<uint32_t>(),
+                               s->d_swo.as<uint32_t>(), swv + 1, st));
+  }
+  uint32_t npairs = 0;
+  HIP_TRY(hipMemcpyAsync(&npairs, s->d_swo.as<uint32_t>() + swv, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
+  if (npairs)
+    launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
+                       s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), nullptr,
+                       s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false);
+  profEnd(s, VXP_K_PAIRS, pe);
   s->n_pairs = npairs;
   HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
   pe = profBegin(s);
@@ -520,15 +540,29 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_wi_off.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_produce.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_rstat.ensure((size_t)np1 * 16));
+  HIP_TRY(s->d_clist.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_wlist.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_cflag.ensure((size_t)(npairs + 1) * 4));
+  HIP_TRY(s->d_cidx.ensure((size_t)(npairs + 1) * 4));
   HIP_TRY(s->d_pidx.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_staging.ensure((size_t)np1 * sizeof(DevManifold)));
   HIP_TRY(s->d_cstage.ensure((size_t)np1 * sizeof(CacheVal)));
   PairRec* pr = s->d_pr.as<PairRec>();
   pe = profBegin(s);
-  launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev);
+  launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, s->d_cflag.as<uint32_t>(),
+                      s->d_produce.as<uint32_t>(), s->d_rstat.as<uint4>(), s->d_wlist.as<uint32_t>(),
+                      s->d_step_ctr.as<uint32_t>());
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
   pe = profBegin(s);
-  launch_cache_replay(st, cfg, ws, stats, pr, s->d_cache_keys.as<CacheKeyRec>(), nw);
+  if (npairs) {
+    size_t cb = exclusive_scan_u32_bytes(npairs + 1);
+    HIP_TRY(s->d_sort_tmp.ensure(cb));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_cflag.as<uint32_t>(),
+                               s->d_cidx.as<uint32_t>(), npairs + 1, st));
+  }
+  launch_cache_replay(st, cfg, ws, stats, pr, s->d_cache_keys.as<CacheKeyRec>(),
+                      s->d_cflag.as<uint32_t>(), s->d_cidx.as<uint32_t>(),
+                      s->d_clist.as<uint32_t>(), npairs, nw);
   profEnd(s, VXP_K_CACHE_REPLAY, pe);
   uint32_t total_wi = 0;
   if (npairs) {
@@ -543,29 +577,35 @@ int stepAll(vxp_system* s) {
     HIP_TRY(hipStreamSynchronize(st));
     total_wi = last[0] + last[1];
   }
-  uint32_t tw1 = std::max(total_wi, 1u);
-  HIP_TRY(s->d_wi_cbase.ensure((size_t)tw1 * 4));
-  HIP_TRY(s->d_wi_ccnt.ensure((size_t)tw1 * 4));
-  HIP_TRY(s->d_wi_vchk.ensure((size_t)tw1 * 4));
-  if (s->cand_cap < 4096) s->cand_cap = 1u << 16;
-  for (int attempt = 0; attempt < 16 && total_wi; attempt++) {
-    HIP_TRY(s->d_cand.ensure((size_t)s->cand_cap * sizeof(Cand)));
-    HIP_TRY(hipMemsetAsync(s->d_count.p, 0, 4, st));
-    // the probe is a pure function of the step's inputs, so it can be re-run
-    pe = profBegin(s);
+  HIP_TRY(s->d_wi_cbase.ensure((size_t)(total_wi + 1) * 4));
+  HIP_TRY(s->d_wi_ccnt.ensure((size_t)(total_wi + 1) * 4));
+  HIP_TRY(s->d_wi_vchk.ensure((size_t)std::max(total_wi, 1u) * 4));
+  HIP_TRY(s->d_lane_n.ensure((size_t)std::max(total_wi, 1u) * 64));
+  // pass 1 counts, the scan places every work item's candidates, pass 2 writes
+  pe = profBegin(s);
+  uint32_t ncand = 0;
+  if (total_wi) {
+    launch_probe(st, cfg, B, C, pr, npairs, s->d_wi_off.as<uint32_t>(), total_wi, nullptr,
+                 s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
+                 nullptr, true);
+    size_t cb = exclusive_scan_u32_bytes(total_wi + 1);
+    HIP_TRY(s->d_sort_tmp.ensure(cb));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wi_ccnt.as<uint32_t>(),
+                               s->d_wi_cbase.as<uint32_t>(), total_wi + 1, st));
+    HIP_TRY(hipMemcpyAsync(&ncand, s->d_wi_cbase.as<uint32_t>() + total_wi, 4,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  HIP_TRY(s->d_cand.ensure((size_t)std::max(ncand, 1u) * sizeof(Cand)));
+  if (ncand)
     launch_probe(st, cfg, B, C, pr, npairs, s->d_wi_off.as<uint32_t>(), total_wi,
                  s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
-                 s->d_wi_vchk.as<uint32_t>(), s->d_cand.as<Cand>(), s->d_count.as<uint32_t>(),
-                 s->cand_cap);
-    profEnd(s, VXP_K_PROBE, pe);
-    uint32_t nc = 0;
-    HIP_TRY(hipMemcpyAsync(&nc, s->d_count.p, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (nc <= s->cand_cap) break;
-    s->cand_cap = nc + nc / 2;  // grow and re-run: the probe only writes fresh buffers
-  }
+                 s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(), s->d_cand.as<Cand>(),
+                 false);
+  profEnd(s, VXP_K_PROBE, pe);
   pe = profBegin(s);
-  launch_reduce(st, cfg, B, ws, stats, pr, keys, npairs, s->d_wi_off.as<uint32_t>(),
+  launch_reduce(st, cfg, B, ws, stats, pr, keys, npairs, s->d_wlist.as<uint32_t>(),
+                s->d_step_ctr.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
                 s->d_cand.as<Cand>(), prev, s->d_cache_keys.as<CacheKeyRec>(),
                 s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>(),
@@ -854,9 +894,9 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->b_ldamp, &s->b_adamp, &s->b_sleep_timer, &s->b_type, &s->b_sleeping,
                     &s->b_active, &s->b_collider, &s->b_id, &s->b_idmap, &s->d_ws, &s->d_stats,
                     &s->d_cache_keys, &s->d_cache_vals, &s->d_skeys, &s->d_svals, &s->d_skeys2,
-                    &s->d_svals2, &s->d_sort_tmp, &s->d_pairs_raw, &s->d_pairs, &s->d_count,
-                    &s->d_pr, &s->d_counts, &s->d_wi_off, &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_wi_vchk,
-                    &s->d_cand, &s->d_produce, &s->d_pidx, &s->d_staging, &s->d_cstage, &s->d_rstat,
+                    &s->d_svals2, &s->d_sort_tmp, &s->d_slo, &s->d_shi, &s->d_send, &s->d_wcnt, &s->d_woff, &s->d_swc, &s->d_swo, &s->d_pairs_raw, &s->d_pairs,
+                    &s->d_pr, &s->d_counts, &s->d_wi_off, &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_wi_vchk, &s->d_lane_n,
+                    &s->d_cand, &s->d_produce, &s->d_pidx, &s->d_staging, &s->d_cstage, &s->d_rstat, &s->d_clist, &s->d_cflag, &s->d_cidx, &s->d_wlist, &s->d_step_ctr,
                     &s->d_total, &s->d_ml[0], &s->d_ml[1], &s->d_isl_parent, &s->d_isl_end,
                     &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors, &s->s_pos, &s->s_rot,
                     &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,

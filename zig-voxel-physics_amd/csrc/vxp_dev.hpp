@@ -12,6 +12,9 @@ namespace vxd {
 
 constexpr int kWave = 64;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Device counters cleared at the start of every step (step_ctr[]).
+constexpr uint32_t kCtrReduceWork = 0;  // pairs on the reduction work list
+constexpr uint32_t kStepCounters = 4;
 
 // Collider header.  The brick directory is a dense grid over brick positions
 // floor(min_bound/8) .. floor(max_bound/8) (inclusive: the overlap region loop

@@ -91,6 +91,50 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ __forceinline__ float rdlane(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+}
+
+// First index in the sorted a[0..n) whose value exceeds v.
+__device__ __forceinline__ uint32_t upperBound(const uint32_t* a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Exclusive scan of data[0..n) by the whole block (tmp holds blockDim.x
+// entries); *total receives the sum when non-null.
+__device__ void blockScanExclusive(uint32_t* data, uint32_t n, uint32_t* tmp /*blockDim*/,
+                                   uint32_t* total) {
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < n ? data[i] : 0;
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
+      uint32_t t = threadIdx.x >= d ? tmp[threadIdx.x - d] : 0;
+      __syncthreads();
+      tmp[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) data[i] = carry + tmp[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += tmp[threadIdx.x];
+    __syncthreads();
+  }
+  if (total && threadIdx.x == 0) *total = carry;
+  __syncthreads();
+}
+
 // atomicAdd(addr, v) aggregated over the wave when every lane with v != 0
 // targets the same counter.  Call from converged code (all 64 lanes).
 __device__ __forceinline__ void aggAdd(uint32_t* addr, uint32_t v) {
@@ -107,13 +151,69 @@ __device__ __forceinline__ void aggAdd(uint32_t* addr, uint32_t v) {
   }
 }
 
+// Per-world counters summed over the block: thread t adds v[k] to
+// base[k][world * stride] (k < N).  Worlds must be non-decreasing in thread
+// order among contributing threads (pairs are sorted by world).  One atomic
+// per run of equal worlds per counter; every thread of the block must call.
+template <int N>
+__device__ void blockAggAdd(uint32_t world, const uint32_t (&v)[N], uint32_t* const (&base)[N],
+                            uint32_t stride) {
+  __shared__ uint32_t s_w[32], s_v[32][N];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < N; k++) any |= v[k] != 0;
+  const uint64_t act = __ballot(any);
+  uint32_t lw = kNone;
+  if (act) {
+    const int leader = __ffsll((long long)act) - 1;
+    lw = rdlane(world, leader);
+    if (__ballot(any && world != lw) == 0) {
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        uint32_t sum = waveSum(v[k]);
+        if (lane == 0) s_v[wave][k] = sum;
+      }
+    } else {
+      if (any)
+#pragma unroll
+        for (int k = 0; k < N; k++)
+          if (v[k]) atomicAdd(base[k] + (size_t)world * stride, v[k]);
+      lw = kNone;
+    }
+  }
+  if (lane == 0) s_w[wave] = lw;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t nwv = (blockDim.x + kWave - 1) / kWave;
+    uint32_t cw = kNone, acc[N];
+    for (uint32_t q = 0; q <= nwv; q++) {
+      uint32_t qw = q < nwv ? s_w[q] : kNone;
+      if (q < nwv && qw == kNone) continue;
+      if (qw != cw || q == nwv) {
+        if (cw != kNone)
+#pragma unroll
+          for (int k = 0; k < N; k++)
+            if (acc[k]) atomicAdd(base[k] + (size_t)cw * stride, acc[k]);
+        cw = qw;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc[k] = 0;
+      }
+      if (q < nwv)
+#pragma unroll
+        for (int k = 0; k < N; k++) acc[k] += s_v[q][k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // step begin: allDynamicBodiesSleeping, per-step stat reset
 // ---------------------------------------------------------------------------
 __global__ void k_step_begin(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats* st,
-                             uint32_t n_worlds) {
+                             uint32_t n_worlds, uint32_t* step_ctr) {
   uint32_t w = blockIdx.x;
   if (w >= n_worlds) return;
+  if (w == 0 && threadIdx.x < kStepCounters) step_ctr[threadIdx.x] = 0;
   __shared__ uint32_t awake;
   if (threadIdx.x == 0) awake = 0;
   __syncthreads();
@@ -192,26 +292,31 @@ __device__ __forceinline__ float comp(const float4& v, uint32_t a) {
   return a == 0 ? v.x : (a == 1 ? v.y : v.z);
 }
 
-// Sweep axis of each world: the axis along which the fat-box centres spread
-// furthest (the pair set does not depend on the axis).
+// Sweep axis of each world: the axis with the fewest expected overlaps, i.e.
+// the smallest mean fat-box extent relative to the spread of the box centres
+// (the pair set does not depend on the axis).
 __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n_worlds) {
   uint32_t w = blockIdx.x;
   if (w >= n_worlds) return;
-  __shared__ float lo[3][256], hi[3][256];
+  __shared__ float lo[3][256], hi[3][256], ext[3][256];
   float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float e[3] = {0.f, 0.f, 0.f};
   for (uint32_t s = threadIdx.x; s < cfg.max_bodies; s += blockDim.x) {
     uint32_t g = w * cfg.max_bodies + s;
     if (!B.active[g]) continue;
     float4 a = B.fat_lo[g], b = B.fat_hi[g];
     float c[3] = {a.x + b.x, a.y + b.y, a.z + b.z};
+    float x[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
     for (int k = 0; k < 3; k++) {
       l[k] = fminf(l[k], c[k]);
       h[k] = fmaxf(h[k], c[k]);
+      e[k] += fminf(x[k], 1e30f);
     }
   }
   for (int k = 0; k < 3; k++) {
     lo[k][threadIdx.x] = l[k];
     hi[k][threadIdx.x] = h[k];
+    ext[k][threadIdx.x] = e[k];
   }
   __syncthreads();
   for (uint32_t d = blockDim.x / 2; d > 0; d >>= 1) {
@@ -219,15 +324,20 @@ __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n
       for (int k = 0; k < 3; k++) {
         lo[k][threadIdx.x] = fminf(lo[k][threadIdx.x], lo[k][threadIdx.x + d]);
         hi[k][threadIdx.x] = fmaxf(hi[k][threadIdx.x], hi[k][threadIdx.x + d]);
+        ext[k][threadIdx.x] += ext[k][threadIdx.x + d];
       }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    float e[3];
-    for (int k = 0; k < 3; k++) e[k] = hi[k][0] - lo[k][0];
+    // overlap density ~ 2 * total extent / spread of centres (centres are doubled)
+    float r[3];
+    for (int k = 0; k < 3; k++) {
+      float spread = hi[k][0] - lo[k][0];
+      r[k] = spread > 0.0f ? ext[k][0] / spread : INFINITY;
+    }
     uint32_t ax = 0;
-    if (e[1] > e[ax]) ax = 1;
-    if (e[2] > e[ax]) ax = 2;
+    if (r[1] < r[ax]) ax = 1;
+    if (r[2] < r[ax]) ax = 2;
     ws[w].axis = ax;
   }
 }
@@ -243,43 +353,110 @@ __global__ void k_sweep_keys(DevCfg cfg, DevBodies B, const WorldState* ws, uint
   vals[g] = g;
 }
 
-__global__ void k_sweep_pairs(DevCfg cfg, DevBodies B, const WorldState* ws, const uint64_t* keys,
-                              const uint32_t* order, uint32_t n_slots, uint64_t* out,
-                              uint32_t* count, uint32_t cap) {
+// Fat boxes gathered into sweep order, field-major: lo.xyz + id, hi.xyz +
+// (world << 2 | active << 1 | static).
+__global__ void k_sweep_gather(DevCfg cfg, DevBodies B, const uint32_t* order, uint32_t n_slots,
+                               float4* slo, float4* shi) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_slots) return;
-  uint32_t gi = order[i];
-  if (!B.active[gi]) return;
-  uint32_t w = gi / cfg.max_bodies;
-  uint32_t ax = ws[w].axis;
-  Box bi{f4v(B.fat_lo[gi]), f4v(B.fat_hi[gi])};
-  float hi_ax = comp(B.fat_hi[gi], ax);
-  bool si = B.type[gi] == VXP_BODY_STATIC;
-  uint32_t idi = B.id[gi];
-  // pairs are staged per thread and appended with one atomic per batch
-  constexpr int kBatch = 16;
-  uint64_t buf[kBatch];
-  int nb = 0;
-  for (uint32_t j = i + 1; j < n_slots; j++) {
-    uint32_t gj = order[j];
-    if (gj / cfg.max_bodies != w || !B.active[gj]) break;
-    if (comp(B.fat_lo[gj], ax) > hi_ax) break;
-    Box bj{f4v(B.fat_lo[gj]), f4v(B.fat_hi[gj])};
-    if (!boxOverlaps(bi, bj)) continue;
-    if (si && B.type[gj] == VXP_BODY_STATIC) continue;
-    uint32_t idj = B.id[gj];
-    buf[nb++] = makeKey(w, min(idi, idj), max(idi, idj));
-    if (nb == kBatch) {
-      uint32_t slot = atomicAdd(count, (uint32_t)nb);
-      for (int k = 0; k < nb; k++)
-        if (slot + k < cap) out[slot + k] = buf[k];
-      nb = 0;
+  uint32_t g = order[i];
+  float4 lo = B.fat_lo[g], hi = B.fat_hi[g];
+  uint32_t meta = ((g / cfg.max_bodies) << 2) | (B.active[g] ? 2u : 0u) |
+                  (B.type[g] == VXP_BODY_STATIC ? 1u : 0u);
+  lo.w = __uint_as_float(B.id[g]);
+  hi.w = __uint_as_float(meta);
+  slo[i] = lo;
+  shi[i] = hi;
+}
+
+// Sweep length of body i: its candidates are the bodies after it in sweep
+// order, up to the first one that starts past i's upper bound on the sweep
+// axis, is inactive or belongs to the next world (a monotone predicate over
+// the world's sorted slots [w * max_bodies, (w + 1) * max_bodies)).  send[i]
+// = end of the range, wcnt[i] = its 64-candidate work items; wcnt[n] = 0.
+__global__ void k_sweep_count(DevCfg cfg, const WorldState* ws, const float4* slo,
+                              const float4* shi, uint32_t n_slots, uint32_t* send,
+                              uint32_t* wcnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_slots) return;
+  if (i == n_slots - 1) wcnt[n_slots] = 0;
+  const float4 hi4 = shi[i];
+  const uint32_t mi = __float_as_uint(hi4.w);
+  uint32_t e = i + 1, c = 0;
+  if (mi & 2u) {
+    const uint32_t w = mi >> 2, ax = ws[w].axis;
+    const float hax = comp(hi4, ax);
+    uint32_t lo = i + 1, hi = min(n_slots, (w + 1) * cfg.max_bodies);
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      bool past = !(__float_as_uint(shi[mid].w) & 2u) || comp(slo[mid], ax) > hax;
+      if (past) hi = mid; else lo = mid + 1;
+    }
+    e = lo;
+    c = (e - (i + 1) + kWave - 1) / kWave;
+  }
+  send[i] = e;
+  wcnt[i] = c;
+}
+
+// Work item = (body i, 64 consecutive candidates).  Wave gw of the fixed grid
+// owns the contiguous items [gw * q, (gw + 1) * q), q = ceil(total / waves).
+// kCount: wcount[gw] = its overlapping pairs (wcount[waves] = 0).  Otherwise
+// the pairs (inclusive overlap, src/broad_phase.zig:488-526; static-static
+// excluded) are written from woff[gw] on, in work-item order.
+constexpr uint32_t kSweepWaves = 8192;
+
+template <bool kCount>
+__global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const float4* slo,
+                                                     const float4* shi, uint32_t n_slots,
+                                                     const uint32_t* send, const uint32_t* woff_i,
+                                                     uint32_t* wcount, const uint32_t* woff_w,
+                                                     uint64_t* out) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint32_t total = woff_i[n_slots];
+  const uint32_t q = (total + kSweepWaves - 1) / kSweepWaves;
+  const uint32_t g0 = min(total, gw * q), g1 = min(total, g0 + q);
+  uint32_t n = 0;
+  uint32_t o = kCount ? 0 : woff_w[gw];
+  if (g0 < g1) {
+    uint32_t i = upperBound(woff_i, n_slots, g0) - 1, cur = kNone, end = 0, w = 0, idi = 0;
+    bool si = false;
+    Box bi;
+    for (uint32_t wid = g0; wid < g1; wid++) {
+      while (woff_i[i + 1] <= wid) i++;
+      if (i != cur) {
+        cur = i;
+        const float4 li = slo[i], hi4 = shi[i];
+        const uint32_t mi = __float_as_uint(hi4.w);
+        w = mi >> 2;
+        idi = __float_as_uint(li.w);
+        si = mi & 1u;
+        bi = Box{f4v(li), f4v(hi4)};
+        end = send[i];
+      }
+      const uint32_t j = i + 1 + (wid - woff_i[i]) * kWave + lane;
+      bool emit = false;
+      uint32_t idj = 0;
+      if (j < end) {
+        const float4 lj = slo[j], hj = shi[j];
+        emit = boxOverlaps(bi, Box{f4v(lj), f4v(hj)}) && !(si && (__float_as_uint(hj.w) & 1u));
+        idj = __float_as_uint(lj.w);
+      }
+      const uint64_t em = __ballot(emit);
+      if (kCount) {
+        n += (uint32_t)__popcll(em);
+      } else {
+        if (emit)
+          out[o + (uint32_t)__popcll(em & ((1ull << lane) - 1ull))] =
+              makeKey(w, min(idi, idj), max(idi, idj));
+        o += (uint32_t)__popcll(em);
+      }
     }
   }
-  if (nb) {
-    uint32_t slot = atomicAdd(count, (uint32_t)nb);
-    for (int k = 0; k < nb; k++)
-      if (slot + k < cap) out[slot + k] = buf[k];
+  if (kCount && lane == 0) {
+    wcount[gw] = n;
+    if (gw == kSweepWaves - 1) wcount[kSweepWaves] = 0;
   }
 }
 
@@ -459,19 +636,40 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
   return cnt;
 }
 
+// Also flags the cache-eligible pairs (cflag[0..n_pairs], last entry 0) for
+// their ordered compaction, clears the pair's reduction outputs and appends
+// the pairs that the reduction must visit to wlist (any order).
 __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
-                               const DevManifold* prev) {
+                               const DevManifold* prev, uint32_t* cflag, uint32_t* produce,
+                               uint4* rstat, uint32_t* wlist, uint32_t* step_ctr) {
+  __shared__ uint32_t s_n, s_base;
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   PairCounts c{0, 0, 0};
   uint32_t w = 0;
+  bool work = false;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  uint32_t pos = 0;
   if (p < n_pairs) {
     c = pairProcess(cfg, B, C, ws, keys, n_pairs, p, pr, prev);
     w = keyWorld(keys[p]);
+    const uint32_t f = pr[p].flag;
+    cflag[p] = f == PF_CACHE;
+    if (p == n_pairs - 1) cflag[n_pairs] = 0;
+    produce[p] = 0;
+    rstat[p] = make_uint4(0u, 0u, 0u, 0u);
+    work = f == PF_CARRY || f == PF_DETECT || f == PF_CACHE;
+    if (work) pos = atomicAdd(&s_n, 1u);
   }
-  aggAdd(&st[w].broad_phase_pairs, c.bp);
-  aggAdd(&st[w].pairs_skipped_sleeping, c.skipped);
-  aggAdd(&st[w].narrow_phase_tested, c.tested);
+  __syncthreads();
+  if (threadIdx.x == 0) s_base = s_n ? atomicAdd(&step_ctr[kCtrReduceWork], s_n) : 0;
+  __syncthreads();
+  if (work) wlist[s_base + pos] = p;
+  const uint32_t v[3] = {c.bp, c.skipped, c.tested};
+  uint32_t* const ctr[3] = {&st->broad_phase_pairs, &st->pairs_skipped_sleeping,
+                            &st->narrow_phase_tested};
+  blockAggAdd<3>(w, v, ctr, sizeof(vxp_stats) / 4);
 }
 
 
@@ -490,42 +688,55 @@ __device__ __forceinline__ uint64_t cacheHash(uint32_t a, uint32_t b, uint64_t t
   return h * 31 + th;
 }
 
-// One wave per world; the world's key table is staged in LDS.  Lanes 0..15
-// probe the 16-entry window of one get/put in parallel (ballot picks the
-// first stopping entry); pairs are visited strictly in pair order.
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Cache-eligible pairs in pair order: clist[cidx[p]] = p.
+__global__ void k_cache_scatter(const uint32_t* cflag, const uint32_t* cidx, uint32_t n_pairs,
+                                uint32_t* clist) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n_pairs && cflag[p]) clist[cidx[p]] = p;
+}
+
+// One wave per world replays the gets/puts of its cache-eligible pairs
+// (clist[cidx[pair_lo] .. cidx[pair_hi])) strictly in pair order, with the
+// key table staged in LDS: lanes 0..15 probe the 16-entry window of one
+// get/put in parallel and a ballot picks the first stopping entry.
 __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws, vxp_stats* st,
                                                      PairRec* pr, CacheKeyRec* keys,
+                                                     const uint32_t* cidx, const uint32_t* clist,
                                                      uint32_t n_worlds) {
   extern __shared__ CacheKeyRec E[];
   const uint32_t w = blockIdx.x, lane = threadIdx.x;
   if (w >= n_worlds || ws[w].skip) return;
   const uint32_t frame = ws[w].frame + 1;
-  __syncthreads();
+  waveSync();
   if (lane == 0) ws[w].frame = frame;
   if (!cfg.use_cache) return;
   const uint32_t cap = cfg.cache_cap, maxp = cap < 16 ? cap : 16;
   CacheKeyRec* G = keys + (size_t)w * cap;
   for (uint32_t i = lane; i < cap; i += kWave) E[i] = G[i];
-  __syncthreads();
-  uint32_t hits = 0, misses = 0;
   const uint32_t lo = ws[w].pair_lo, hi = ws[w].pair_hi;
-  for (uint32_t base = lo; base < hi; base += kWave) {
-    uint32_t p = base + lane;
-    bool inb = p < hi;
-    uint32_t flag = inb ? pr[p].flag : PF_NONE;
-    uint32_t ida = inb ? pr[p].ida : 0, idb = inb ? pr[p].idb : 0;
-    uint64_t th = inb ? pr[p].th : 0;
-    uint32_t my_flag = flag, my_aux = kNone;
-    uint64_t elig = __ballot(flag == PF_CACHE);
-    while (elig) {
-      int j = __ffsll((long long)elig) - 1;
-      elig &= elig - 1;
-      uint32_t a = (uint32_t)__shfl((int)ida, j, kWave), b = (uint32_t)__shfl((int)idb, j, kWave);
-      uint64_t t = shfl64(th, j);
-      uint64_t h = cacheHash(a, b, t);
-      uint32_t start = (uint32_t)(h % cap);
-      bool probe = lane < maxp;
-      uint32_t idx = (start + lane) % cap;
+  const uint32_t c0 = lo < hi ? cidx[lo] : 0, total = lo < hi ? cidx[hi] - c0 : 0;
+  waveSync();
+  uint32_t hits = 0, misses = 0;
+  for (uint32_t base = 0; base < total; base += kWave) {
+    const bool inb = base + lane < total;
+    const uint32_t p = inb ? clist[c0 + base + lane] : 0;
+    const uint32_t ida = inb ? pr[p].ida : 0, idb = inb ? pr[p].idb : 0;
+    const uint64_t th = inb ? pr[p].th : 0;
+    uint32_t my_flag = PF_CACHE, my_aux = kNone;
+    const uint32_t nb = min(total - base, (uint32_t)kWave);
+    for (uint32_t j = 0; j < nb; j++) {
+      const uint32_t a = rdlane(ida, (int)j), b = rdlane(idb, (int)j);
+      const uint64_t t = ((uint64_t)rdlane((uint32_t)(th >> 32), (int)j) << 32) |
+                         rdlane((uint32_t)th, (int)j);
+      const uint64_t h = cacheHash(a, b, t);
+      const uint32_t start = (uint32_t)(h % cap);
+      const bool probe = lane < maxp;
+      const uint32_t idx = (start + lane) % cap;
       bool valid = false, eq = false;
       uint64_t lu = ~0ull;
       if (probe) {
@@ -533,11 +744,11 @@ __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws,
         eq = valid && E[idx].a == a && E[idx].b == b && E[idx].th == t;
         lu = E[idx].last_used;
       }
-      uint64_t stop = __ballot(probe && (!valid || eq));
+      const uint64_t stop = __ballot(probe && (!valid || eq));
       uint32_t rflag, raux;
       if (stop) {
-        int f = __ffsll((long long)stop) - 1;
-        bool feq = __shfl((int)eq, f, kWave) != 0;
+        const int f = __ffsll((long long)stop) - 1;
+        const bool feq = rdlane((uint32_t)eq, f) != 0;
         raux = (start + (uint32_t)f) % cap;
         rflag = feq ? PF_HIT : PF_MISS;
       } else {
@@ -545,14 +756,14 @@ __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws,
         uint64_t v = lu;
         uint32_t l = lane;
         for (int d = kWave / 2; d > 0; d >>= 1) {
-          uint64_t ov = shfl64(v, (int)(lane ^ d));
-          uint32_t ol = (uint32_t)__shfl((int)l, (int)(lane ^ d), kWave);
+          const uint64_t ov = shfl64(v, (int)(lane ^ d));
+          const uint32_t ol = (uint32_t)__shfl((int)l, (int)(lane ^ d), kWave);
           if (ov < v || (ov == v && ol < l)) {
             v = ov;
             l = ol;
           }
         }
-        raux = (start + (uint32_t)__shfl((int)l, 0, kWave)) % cap;
+        raux = (start + rdlane(l, 0)) % cap;
         rflag = PF_MISS;
       }
       if (lane == 0) {
@@ -567,22 +778,22 @@ __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws,
           e.last_used = frame;
           e.valid = 1;
           e.owner_frame = frame;
-          e.owner_pair = base + (uint32_t)j;
+          e.owner_pair = rdlane(p, (int)j);
           misses++;
         }
       }
-      __syncthreads();
-      if (lane == (uint32_t)j) {
+      waveSync();
+      if (lane == j) {
         my_flag = rflag;
         my_aux = raux;
       }
     }
-    if (inb && flag == PF_CACHE) {
+    if (inb) {
       pr[p].flag = my_flag;
       pr[p].aux = my_aux;
     }
   }
-  __syncthreads();
+  waveSync();
   for (uint32_t i = lane; i < cap; i += kWave) G[i] = E[i];
   if (lane == 0) {
     st[w].cache_hits += hits;
@@ -678,24 +889,21 @@ __device__ __forceinline__ uint32_t probeVoxel(const DevColliders& C, const DevC
   return n;
 }
 
-__device__ __forceinline__ uint32_t upperBound(const uint32_t* a, uint32_t n, uint32_t v) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] <= v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
 
+// Two passes over the same work items.  kCount: per-lane candidate counts
+// (lane_n), per-work-item totals wi_ccnt (wi_ccnt[total_wi] = 0) and
+// voxels_checked sums.  Otherwise: candidates written from wi_cbase[wid] (the
+// exclusive scan of wi_ccnt) in (lane, probe) order.
+template <bool kCount>
 __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColliders C,
-                                               PairRec* pr, uint32_t n_pairs,
+                                               const PairRec* pr, uint32_t n_pairs,
                                                const uint32_t* wi_off, uint32_t total_wi,
-                                               uint32_t* wi_cbase, uint32_t* wi_ccnt,
-                                               uint32_t* wi_vchk, Cand* cand,
-                                               uint32_t* cand_count, uint32_t cand_cap) {
+                                               const uint32_t* wi_cbase, uint32_t* wi_ccnt,
+                                               uint32_t* wi_vchk, uint8_t* lane_n, Cand* cand) {
   uint32_t lane = threadIdx.x & (kWave - 1);
   uint32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (wid >= total_wi) return;  // whole wave
+  if (!kCount && wi_ccnt[wid] == 0) return;
   uint32_t p = upperBound(wi_off, n_pairs, wid) - 1;
   const PairRec& r = pr[p];
   uint32_t chunk = wid - wi_off[p];
@@ -727,22 +935,21 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
     z = r.mn[2] + (int32_t)(lin / nxy);
     va = voxelAt(C, HA, x, y, z);
   }
-  uint32_t vchk = 0, n = 0;
-  if ((va & 7u) != 0) n = probeVoxel<false>(C, HA, HB, c, x, y, z, va, &vchk, nullptr);
-  int incl = waveInclusiveScan((int)n, (int)lane);
-  uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
-  uint32_t vsum = waveSum(vchk);
-  uint32_t base = 0;
-  if (total > 0) {
-    if (lane == 0) base = atomicAdd(cand_count, total);
-    base = (uint32_t)__shfl((int)base, 0, kWave);
-    if (n > 0 && base + total <= cand_cap)
-      probeVoxel<true>(C, HA, HB, c, x, y, z, va, nullptr, cand + base + (uint32_t)incl - n);
-  }
-  if (lane == 0) {
-    wi_cbase[wid] = base;
-    wi_ccnt[wid] = total;
-    wi_vchk[wid] = vsum;
+  if (kCount) {
+    uint32_t vchk = 0, n = 0;
+    if ((va & 7u) != 0) n = probeVoxel<false>(C, HA, HB, c, x, y, z, va, &vchk, nullptr);
+    lane_n[(size_t)wid * kWave + lane] = (uint8_t)n;
+    uint32_t total = waveSum(n), vsum = waveSum(vchk);
+    if (lane == 0) {
+      wi_ccnt[wid] = total;
+      wi_vchk[wid] = vsum;
+      if (wid == total_wi - 1) wi_ccnt[total_wi] = 0;
+    }
+  } else {
+    uint32_t n = lane_n[(size_t)wid * kWave + lane];
+    int incl = waveInclusiveScan((int)n, (int)lane);
+    if (n > 0)
+      probeVoxel<true>(C, HA, HB, c, x, y, z, va, nullptr, cand + wi_cbase[wid] + (uint32_t)incl - n);
   }
 }
 
@@ -778,12 +985,6 @@ __device__ void warmStartFrom(vxp_manifold& m, const DevManifold* prev, int32_t 
       }
 }
 
-__device__ __forceinline__ float rdlane(float v, int k) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-}
-__device__ __forceinline__ uint32_t rdlane(uint32_t v, int k) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
-}
 
 // One wave per pair.  The streaming top-4 of ContactManifold.addContact
 // (src/narrow_phase.zig:451-481) is replayed exactly in candidate order: the
@@ -792,25 +993,16 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, int k) {
 // slot, or some slot is strictly shallower); every other candidate leaves the
 // state unchanged in the sequential order too.  The slots are wave-uniform.
 // Per-pair counters go to rstat and are summed by k_reduce_tail.
-__global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, const WorldState* ws,
-                                                const PairRec* pr, const uint64_t* keys,
-                                                uint32_t n_pairs, const uint32_t* wi_off,
-                                                const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
-                                                const uint32_t* wi_vchk, const Cand* cand,
-                                                const DevManifold* prev, const CacheVal* cvals,
-                                                CacheVal* cstage, DevManifold* staging,
-                                                uint32_t* produce, uint4* rstat) {
+__device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodies& B,
+                                               const WorldState* ws, const PairRec* pr,
+                                               const uint64_t* keys, uint32_t p, uint32_t f,
+                                               const uint32_t* wi_off, const uint32_t* wi_cbase,
+                                               const uint32_t* wi_ccnt, const uint32_t* wi_vchk,
+                                               const Cand* cand, const DevManifold* prev,
+                                               const CacheVal* cvals, CacheVal* cstage,
+                                               DevManifold* staging, uint32_t* produce,
+                                               uint4* rstat) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (p >= n_pairs) return;  // whole wave
-  const uint32_t f = pr[p].flag;
-  if (f == PF_NONE || f == PF_CACHE) {
-    if (lane == 0) {
-      produce[p] = 0;
-      rstat[p] = make_uint4(0u, 0u, 0u, 0u);
-    }
-    return;
-  }
   const uint32_t aux = pr[p].aux;
   if (f == PF_CARRY) {
     const uint2* src = reinterpret_cast<const uint2*>(prev + aux);
@@ -957,6 +1149,27 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, const W
   }
 }
 
+// Waves take the pairs of the work list (built by k_pair_process; pairs not
+// on it were cleared there) one at a time in a grid-stride loop.
+__global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, const WorldState* ws,
+                                                const PairRec* pr, const uint64_t* keys,
+                                                const uint32_t* wlist, const uint32_t* step_ctr,
+                                                const uint32_t* wi_off,
+                                                const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
+                                                const uint32_t* wi_vchk, const Cand* cand,
+                                                const DevManifold* prev, const CacheVal* cvals,
+                                                CacheVal* cstage, DevManifold* staging,
+                                                uint32_t* produce, uint4* rstat) {
+  const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t total = step_ctr[kCtrReduceWork];
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < total;
+       i += nwaves) {
+    const uint32_t p = wlist[i];
+    reducePairWave(cfg, B, ws, pr, keys, p, pr[p].flag, wi_off, wi_cbase, wi_ccnt, wi_vchk, cand,
+                   prev, cvals, cstage, staging, produce, rstat);
+  }
+}
+
 // BrickPairCache.put of this frame's misses (entry owner = the pair that
 // reserved it during the replay), and the per-world narrow-phase counters.
 __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const PairRec* pr,
@@ -974,10 +1187,14 @@ __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const P
       if (keys[e].owner_frame == ws[w].frame && keys[e].owner_pair == p) vals[e] = cstage[p];
     }
   }
-  aggAdd(&ws[w].produced, c.x);
-  aggAdd(&st[w].voxels_checked, c.y);
-  aggAdd(&st[w].narrow_phase_colliding, c.z);
-  aggAdd(&st[w].contacts_generated, c.w);
+  const uint32_t v0[1] = {c.x};
+  uint32_t* const c0[1] = {&ws->produced};
+  blockAggAdd<1>(w, v0, c0, sizeof(WorldState) / 4);
+  __syncthreads();  // blockAggAdd's LDS slots are reused
+  const uint32_t v1[3] = {c.y, c.z, c.w};
+  uint32_t* const c1[3] = {&st->voxels_checked, &st->narrow_phase_colliding,
+                           &st->contacts_generated};
+  blockAggAdd<3>(w, v1, c1, sizeof(vxp_stats) / 4);
 }
 
 // New manifold segments: skipped worlds keep their previous segment.
@@ -1323,30 +1540,6 @@ __device__ __forceinline__ V3 computeOrthogonal(V3 v) {
   return normalizeOrZero(V3{-v.y, v.x, 0.0f});
 }
 
-__device__ void blockScanExclusive(uint32_t* data, uint32_t n, uint32_t* tmp /*blockDim*/,
-                                   uint32_t* total) {
-  __shared__ uint32_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < n; base += blockDim.x) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t v = i < n ? data[i] : 0;
-    tmp[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
-      uint32_t t = threadIdx.x >= d ? tmp[threadIdx.x - d] : 0;
-      __syncthreads();
-      tmp[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < n) data[i] = carry + tmp[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) carry += tmp[threadIdx.x];
-    __syncthreads();
-  }
-  if (total && threadIdx.x == 0) *total = carry;
-  __syncthreads();
-}
 
 __device__ __forceinline__ void stamp(uint64_t* t, int i) {
   if (t && threadIdx.x == 0) t[blockIdx.x * 16 + i] = wall_clock64();
@@ -1906,8 +2099,8 @@ void launch_update_begin(hipStream_t s, vxp_stats* st, uint32_t nw, float acc) {
   k_update_begin<<<cdiv(nw, 256), 256, 0, s>>>(st, nw, acc);
 }
 void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
-                       vxp_stats* st, uint32_t nw) {
-  k_step_begin<<<nw, 256, 0, s>>>(cfg, B, ws, st, nw);
+                       vxp_stats* st, uint32_t nw, uint32_t* step_ctr) {
+  k_step_begin<<<nw, 256, 0, s>>>(cfg, B, ws, st, nw, step_ctr);
 }
 void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                      const WorldState* ws, uint32_t n_slots) {
@@ -1918,41 +2111,66 @@ void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
   k_world_axis<<<nw, 256, 0, s>>>(cfg, B, ws, nw);
   k_sweep_keys<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, ws, keys, vals, n_slots);
 }
-void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
-                        const uint64_t* keys, const uint32_t* order, uint32_t n_slots,
-                        uint64_t* out, uint32_t* count, uint32_t cap) {
-  k_sweep_pairs<<<cdiv(n_slots, 128), 128, 0, s>>>(cfg, B, ws, keys, order, n_slots, out, count,
-                                                   cap);
+void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                        const uint32_t* order, uint32_t n_slots, float4* slo, float4* shi,
+                        uint32_t* send, uint32_t* wcnt) {
+  k_sweep_gather<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, order, n_slots, slo, shi);
+  k_sweep_count<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, ws, slo, shi, n_slots, send, wcnt);
+}
+uint32_t sweep_waves() { return kSweepWaves; }
+void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
+                        uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
+                        uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count) {
+  constexpr uint32_t blocks = kSweepWaves / 4;
+  if (count)
+    k_sweep_pairs<true><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount, woff_w,
+                                               out);
+  else
+    k_sweep_pairs<false><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount,
+                                                woff_w, out);
 }
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
-                         const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev) {
-  if (n) k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev);
+                         const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
+                         uint32_t* cflag, uint32_t* produce, uint4* rstat, uint32_t* wlist,
+                         uint32_t* step_ctr) {
+  if (n)
+    k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, cflag,
+                                                produce, rstat, wlist, step_ctr);
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
-                         PairRec* pr, CacheKeyRec* keys, uint32_t nw) {
-  k_cache_replay<<<nw, 64, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(cfg, ws, st, pr, keys, nw);
+                         PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
+                         const uint32_t* cidx, uint32_t* clist, uint32_t n_pairs, uint32_t nw) {
+  if (n_pairs) k_cache_scatter<<<cdiv(n_pairs, 256), 256, 0, s>>>(cflag, cidx, n_pairs, clist);
+  k_cache_replay<<<nw, 64, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(cfg, ws, st, pr, keys, cidx,
+                                                                      clist, nw);
 }
 void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts) {
   if (n) k_chunk_counts<<<cdiv(n, 256), 256, 0, s>>>(pr, n, counts);
 }
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                  PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
-                  uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, Cand* cand,
-                  uint32_t* cand_count, uint32_t cand_cap) {
-  if (total_wi)
-    k_probe<<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, n_pairs, wi_off, total_wi, wi_cbase,
-                                              wi_ccnt, wi_vchk, cand, cand_count, cand_cap);
+                  const PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
+                  const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, uint8_t* lane_n,
+                  Cand* cand, bool count) {
+  if (!total_wi) return;
+  if (count)
+    k_probe<true><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, n_pairs, wi_off, total_wi,
+                                                    wi_cbase, wi_ccnt, wi_vchk, lane_n, cand);
+  else
+    k_probe<false><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, n_pairs, wi_off, total_wi,
+                                                     wi_cbase, wi_ccnt, wi_vchk, lane_n, cand);
 }
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
+                   const uint32_t* wlist, const uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
                    DevManifold* staging, uint32_t* produce, uint4* rstat) {
   if (!n) return;
-  k_reduce<<<cdiv(n, 4), 256, 0, s>>>(cfg, B, ws, pr, keys, n, wi_off, wi_cbase, wi_ccnt, wi_vchk,
-                                      cand, prev, cvals, cstage, staging, produce, rstat);
+  uint32_t blocks = std::min<uint32_t>(2048, cdiv(n, 4));
+  k_reduce<<<blocks, 256, 0, s>>>(cfg, B, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase, wi_ccnt,
+                                  wi_vchk, cand, prev, cvals, cstage, staging, produce, rstat);
   k_reduce_tail<<<cdiv(n, 256), 256, 0, s>>>(cfg, ws, st, pr, n, ckeys, cvals, cstage, rstat);
 }
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total) {

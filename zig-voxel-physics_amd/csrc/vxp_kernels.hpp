@@ -8,26 +8,34 @@ namespace vxd {
 
 void launch_update_begin(hipStream_t s, vxp_stats* st, uint32_t nw, float acc);
 void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
-                       vxp_stats* st, uint32_t nw);
+                       vxp_stats* st, uint32_t nw, uint32_t* step_ctr);
 void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                      const WorldState* ws, uint32_t n_slots);
 void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw);
-void launch_sweep_pairs(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
-                        const uint64_t* keys, const uint32_t* order, uint32_t n_slots,
-                        uint64_t* out, uint32_t* count, uint32_t cap);
+void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                        const uint32_t* order, uint32_t n_slots, float4* slo, float4* shi,
+                        uint32_t* send, uint32_t* wcnt);
+uint32_t sweep_waves();
+void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
+                        uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
+                        uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count);
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
-                         const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev);
+                         const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
+                         uint32_t* cflag, uint32_t* produce, uint4* rstat, uint32_t* wlist,
+                         uint32_t* step_ctr);
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
-                         PairRec* pr, CacheKeyRec* keys, uint32_t nw);
+                         PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
+                         const uint32_t* cidx, uint32_t* clist, uint32_t n_pairs, uint32_t nw);
 void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts);
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                  PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
-                  uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, Cand* cand,
-                  uint32_t* cand_count, uint32_t cand_cap);
+                  const PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
+                  const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, uint8_t* lane_n,
+                  Cand* cand, bool count);
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
+                   const uint32_t* wlist, const uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
