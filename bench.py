@@ -19,6 +19,8 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "zig-voxel-physics_amd"))
 
@@ -76,6 +78,80 @@ def pct(xs, q):
     return s[min(len(s) - 1, int(q * len(s)))]
 
 
+def run_serving(lib, args):
+    """Requests "advance world W by k frames" arrive as a Poisson process at each
+    offered rate for --serve-seconds (W uniform over the resident worlds, k
+    uniform in {1, 2, 4, 8}); the loop submits every request whose arrival time
+    has passed, then ticks.  Latency is completion minus arrival; throughput is
+    the body-substeps of the requests completed within the window per second."""
+    import random
+    from vxp import scenes, serve
+    t0 = time.perf_counter()
+    sc = scenes.serving_pool(n_worlds=args.serve_worlds)
+    t_gen = time.perf_counter() - t0
+    srv = serve.from_scene(sc, slots=args.serve_slots, budget_bytes=200 * 10**9, lib=lib)
+    t_build = time.perf_counter() - t0 - t_gen
+    nw = len(sc.worlds)
+    rng = random.Random(46)
+    # warm-up: one request per slot, drained
+    for i in range(args.serve_slots):
+        srv.submit(rng.randrange(nw), 1, 1 << 62 | i)
+    while srv.info()["completed"] < args.serve_slots:
+        srv.tick()
+    srv.poll()
+    rates, tag = [], 0
+    for lam in [float(x) for x in args.serve_rates.split(",") if x]:
+        arrivals, t = [], 0.0
+        while True:
+            t += rng.expovariate(lam)
+            if t >= args.serve_seconds:
+                break
+            arrivals.append((t, rng.randrange(nw), rng.choice((1, 2, 4, 8))))
+        start = time.monotonic()
+        i, ticks = 0, 0
+        while True:
+            now = time.monotonic()
+            if now - start >= args.serve_seconds:
+                break
+            while i < len(arrivals) and start + arrivals[i][0] <= now:
+                a, w, k = arrivals[i]
+                srv.submit(w, k, tag, start + a)
+                tag += 1
+                i += 1
+            if srv.info()["occupied"] or srv.info()["queued"]:
+                srv.tick()
+                ticks += 1
+            elif i < len(arrivals):
+                time.sleep(max(0.0, start + arrivals[i][0] - time.monotonic()))
+        wall = time.monotonic() - start
+        done = srv.poll()
+        done = done[done["tag"] < (1 << 62)]
+        lat = (done["done_s"] - done["submit_s"]) * 1e3
+        info1 = srv.info()
+        rates.append({
+            "offered_rps": lam, "submitted": i, "completed": int(len(done)),
+            "served_rps": len(done) / wall,
+            "body_substeps_per_s": float(done["body_substeps"].sum()) / wall,
+            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+            "ticks": ticks, "ms_per_tick": wall * 1e3 / max(ticks, 1),
+            "queued_at_end": info1["queued"], "occupied_at_end": info1["occupied"]})
+        # drain what is left before the next rate (not timed)
+        while srv.info()["occupied"] or srv.info()["queued"]:
+            srv.tick()
+        srv.poll()
+    info = srv.info()
+    srv.close()
+    return {
+        "workload": f"{sc.name}: {nw} resident worlds, {sc.n_dynamic} dynamic bodies (log-uniform "
+                    f"8..2048 per world), {args.serve_slots} batch slots, Poisson arrivals for "
+                    f"{args.serve_seconds:g} s per rate, k uniform in {{1,2,4,8}}",
+        "unit": "body-substeps/s", "rates": rates,
+        "resident_bytes": info["resident_bytes"], "budget_bytes": info["budget_bytes"],
+        "setup_s": {"generate": round(t_gen, 1), "load": round(t_build, 1)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +162,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-worlds", type=int, default=2048)
+    ap.add_argument("--serve-slots", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rates", default="2000,5000,10000,20000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -151,6 +232,11 @@ def main():
                                    f"time, frames {args.warmup}..{args.warmup + lf - 1}",
         }
 
+    # ---- serving: Poisson requests over 2048 resident worlds -------------------
+    serving = None
+    if not args.no_serving:
+        serving = run_serving(lib, args)
+
     # ---- CPU baseline: the CPU restatement on the same scene, bounded sample ----
     cpu = None
     if not args.no_cpu:
@@ -204,6 +290,7 @@ def main():
         "final_stats": {k: final[k] for k in ("manifolds", "colors", "sleeping_bodies",
                                               "voxels_checked", "broad_phase_pairs")},
         "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -240,6 +240,62 @@ int vxp_profile_get(vxp_system* sys, uint32_t id, double* total_ms, uint64_t* la
  * done, position passes done, sleep update done (ts[0..6]). */
 int vxp_profile_solver_phases(vxp_system* sys, uint32_t world, uint64_t ts[8]);
 
+/* ---- serving: continuous batching over resident worlds -------------------
+ * No counterpart in the reference, which steps one world per update() call;
+ * this is the request loop for callers that advance many worlds on demand.
+ * A server keeps worlds resident in device memory (a pool) and steps up to
+ * `batch_slots` of them together, one update(fixed_timestep) frame per tick.
+ * Requests "advance world W by k frames" queue in submission order; every tick
+ * admits queued requests into free slots (one request per world at a time,
+ * FIFO per world), advances every occupied slot by one frame with one launch
+ * per pipeline phase, and retires the requests whose frames are done.  Worlds
+ * are admitted by an HBM budget: the pool holds as many worlds as the budget
+ * covers and vxp_server_world_create fails with VXP_ERR_OOM beyond that.
+ * Times are seconds of CLOCK_MONOTONIC. */
+typedef struct vxp_server vxp_server;
+
+typedef struct vxp_request_done {
+  uint64_t tag;
+  uint32_t world, frames;
+  double submit_s, admit_s, done_s;
+  uint64_t body_substeps;  /* awake dynamic bodies x substeps over its frames */
+} vxp_request_done;
+
+typedef struct vxp_server_info {
+  uint32_t worlds, capacity, slots, occupied, queued, pad;
+  uint64_t ticks, completed, body_substeps;
+  uint64_t resident_bytes, budget_bytes;
+} vxp_server_info;
+
+int vxp_server_create(const vxp_config* cfg, uint32_t max_worlds, uint32_t batch_slots,
+                      uint64_t hbm_budget_bytes, vxp_server** out);
+int vxp_server_destroy(vxp_server* srv);
+/* VoxelCollider.init + setCollisionVoxel... (src/collider.zig:168-179,267-273), shared by all worlds. */
+int vxp_server_collider_create(vxp_server* srv, float voxel_size, uint32_t n_bricks,
+                               const int32_t* brick_pos, const uint8_t* voxels,
+                               uint32_t* out_collider);
+/* A new empty resident world (VoxelPhysicsSystem.init, src/system.zig:213-259). */
+int vxp_server_world_create(vxp_server* srv, uint32_t* out_world);
+/* create*Body (src/system.zig:294-402) in a resident world that is not in a slot. */
+int vxp_server_body_create(vxp_server* srv, uint32_t world, uint32_t body_type, const float pos[3],
+                           const float rot[4], uint32_t collider, float density, vxp_handle* out);
+/* Queue "advance `world` by `frames` frames"; submit_s < 0 stamps the call time. */
+int vxp_server_submit(vxp_server* srv, uint32_t world, uint32_t frames, uint64_t tag,
+                      double submit_s);
+/* One tick: admit, step every occupied slot one frame, retire.  *n_done
+ * receives the number of requests retired by this tick. */
+int vxp_server_tick(vxp_server* srv, uint32_t* n_done);
+/* Drain the retired requests (oldest first). */
+int vxp_server_poll(vxp_server* srv, vxp_request_done* out, uint32_t capacity, uint32_t* count);
+/* State of a resident world that is not in a slot (as vxp_bodies_get_w,
+ * vxp_stats_get_w, vxp_manifolds_get_w). */
+int vxp_server_bodies_get(vxp_server* srv, uint32_t world, vxp_body_state* out, uint32_t capacity,
+                          uint32_t* count);
+int vxp_server_stats_get(vxp_server* srv, uint32_t world, vxp_stats* out);
+int vxp_server_manifolds_get(vxp_server* srv, uint32_t world, vxp_manifold* out,
+                             uint32_t capacity, uint32_t* count);
+int vxp_server_info_get(vxp_server* srv, vxp_server_info* out);
+
 /* Last error message of this thread (never NULL). */
 const char* vxp_last_error(void);
 

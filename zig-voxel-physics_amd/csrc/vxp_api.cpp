@@ -5,82 +5,13 @@
 // per-step work runs in the gfx950 kernels of vxp_kernels.hip on one HIP
 // stream.  Body state lives on the device between steps; getters copy it out
 // and setters copy it in before the next step.
-#include <hip/hip_runtime.h>
+#include "vxp_host.hpp"
 
-#include <algorithm>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <unordered_map>
-#include <utility>
-#include <vector>
-
-#include "../../include/vxp.h"
-#include "vxp_dev.hpp"
-#include "vxp_kernels.hpp"
-
-using namespace vxm;
-using namespace vxd;
-
-static thread_local std::string g_err;
-
-#define HIP_TRY(expr)                                                                    \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess) {                                                              \
-      g_err = std::string(#expr) + ": " + hipGetErrorString(e_);                         \
-      return VXP_ERR_DEVICE;                                                             \
-    }                                                                                    \
-  } while (0)
-
-namespace {
-
-// Growable device buffer (contents are not preserved on growth unless asked).
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  hipError_t ensure(size_t need, bool keep = false, hipStream_t s = nullptr) {
-    if (need <= bytes && p) return hipSuccess;
-    size_t nb = std::max(need, bytes * 2);
-    if (nb < 256) nb = 256;
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, nb);
-    if (e != hipSuccess) return e;
-    if (keep && p && bytes) {
-      e = hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) return e;
-      e = hipStreamSynchronize(s);
-      if (e != hipSuccess) return e;
-    }
-    if (p) (void)hipFree(p);
-    p = q;
-    bytes = nb;
-    return hipSuccess;
-  }
-  template <class T>
-  T* as() const {
-    return (T*)p;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-};
-
-// ---- collider (src/collider.zig) -------------------------------------------
-struct HostCollider {
-  float vs = 0.1f;
-  std::vector<int32_t> pos;   // merged bricks, ascending (z, y, x)
-  std::vector<uint8_t> vox;   // 512 per merged brick
-  std::unordered_map<uint64_t, int32_t> by_hash;
-  int32_t minb[3] = {0, 0, 0}, maxb[3] = {0, 0, 0};
-  float lo[3], hi[3];
-  uint32_t solid = 0;
-  int32_t org[3], dim[3];
-  std::vector<int32_t> dir;
-  // mass properties cache key (density) -> (inv_mass, inv inertia)
-};
+namespace vxh {
+std::string& lastError() {
+  static thread_local std::string e;
+  return e;
+}
 
 // VoxelCollider.recomputeMetadata (src/collider.zig:90-133, 283-333)
 void colliderMetadata(HostCollider& c) {
@@ -141,11 +72,6 @@ void colliderMetadata(HostCollider& c) {
       }
 }
 
-struct MassProps {
-  float inv_mass;
-  M3 inv_inertia;
-};
-
 // computeMassProperties (src/body.zig:462-519) + getCenterOfMass
 // (src/collider.zig:376-403), voxels summed brick by brick in ascending
 // (z, y, x) brick order.
@@ -192,59 +118,9 @@ MassProps massProperties(const HostCollider& c, float density) {
   return {1.0f / total, minverse(m)};
 }
 
-}  // namespace
+}  // namespace vxh
 
-struct vxp_system {
-  vxp_config cfg;
-  uint32_t nw = 1, MB = 0, id_cap = 0;
-  hipStream_t stream = nullptr;
-  float acc = 0.0f;
-
-  std::vector<HostCollider> colliders;
-  bool coll_dirty = true;
-  DevBuf d_hdr, d_dir, d_vox;
-
-  // host mirror of the body SoA
-  std::vector<float4> pos, rot, lv, av, fat_lo, fat_hi;
-  std::vector<float> inv_mass, iil, fric, rest, ldamp, adamp, sleep_timer;
-  std::vector<uint8_t> type, sleeping, active;
-  std::vector<int32_t> collider;
-  std::vector<uint32_t> id;
-  std::vector<int32_t> idmap;
-  std::vector<uint32_t> generation, next_id, body_count;
-  std::vector<std::vector<uint32_t>> free_list;
-  std::vector<uint32_t> n_total, n_dyn, n_static;
-  bool host_dirty = true, dev_newer = false;
-
-  DevBuf b_pos, b_rot, b_lv, b_av, b_fat_lo, b_fat_hi, b_inv_mass, b_iil, b_fric, b_rest,
-      b_ldamp, b_adamp, b_sleep_timer, b_type, b_sleeping, b_active, b_collider, b_id, b_idmap;
-
-  // per world / per step device state
-  DevBuf d_ws, d_stats, d_cache_keys, d_cache_vals;
-  DevBuf d_skeys, d_svals, d_skeys2, d_svals2, d_sort_tmp;
-  DevBuf d_slo, d_shi, d_send, d_wcnt, d_woff, d_swc, d_swo, d_pairs_raw, d_pairs, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_wi_vchk, d_lane_n,
-      d_cand,
-      d_produce, d_pidx, d_staging, d_cstage, d_total, d_rstat, d_clist, d_cflag, d_cidx, d_wlist, d_step_ctr;
-  DevBuf d_ml[2];
-  int cur = 0;
-  DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
-  DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
-      s_pts, s_prio, s_tstamp;
-  uint32_t n_pairs = 0, n_manifolds = 0;
-  size_t lds_limit = 0;  // dynamic LDS available to the solver block
-  // kernel timing (HIP events on `stream`)
-  bool prof = false;
-  double prof_ms[VXP_K_COUNT] = {};
-  uint64_t prof_n[VXP_K_COUNT] = {};
-  std::vector<hipEvent_t> ev_pool;
-  struct Pending {
-    int id;
-    hipEvent_t a, b;
-  };
-  std::vector<Pending> pending;
-};
-
-namespace {
+namespace vxh {
 
 size_t slots(const vxp_system* s) { return (size_t)s->nw * s->MB; }
 
@@ -446,12 +322,6 @@ void profFlush(vxp_system* s) {
   }
   s->pending.clear();
 }
-
-#define RC(x)            \
-  do {                   \
-    int rc_ = (x);       \
-    if (rc_) return rc_; \
-  } while (0)
 
 int stepAll(vxp_system* s) {
   hipStream_t st = s->stream;
@@ -724,7 +594,7 @@ int updateOnce(vxp_system* s, float dt) {
   return VXP_OK;
 }
 
-}  // namespace
+}  // namespace vxh
 
 extern "C" {
 
@@ -991,8 +861,15 @@ int vxp_body_create_w(vxp_system* s, uint32_t w, uint32_t type, const float pos[
   s->fric[g] = s->cfg.default_friction;
   s->rest[g] = s->cfg.default_restitution;
   if (type == VXP_BODY_DYNAMIC) {
-    MassProps mp = col == 0xFFFFFFFFu ? MassProps{0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}}
-                                      : massProperties(s->colliders[col], density);
+    MassProps mp{0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}};
+    if (col != 0xFFFFFFFFu) {
+      HostCollider& hc = s->colliders[col];
+      uint32_t key;
+      std::memcpy(&key, &density, 4);
+      auto it = hc.mass_cache.find(key);
+      if (it == hc.mass_cache.end()) it = hc.mass_cache.emplace(key, massProperties(hc, density)).first;
+      mp = it->second;
+    }
     s->inv_mass[g] = mp.inv_mass;
     for (int i = 0; i < 9; i++) s->iil[g * 9 + i] = mp.inv_inertia.m[i];
     s->ldamp[g] = 0.3f;

@@ -128,6 +128,13 @@ struct CacheVal {
   uint32_t count, has_collision, vchk, pad;
 };
 
+// One device-to-device byte range of a batched copy.
+struct CopyDesc {
+  const void* src;
+  void* dst;
+  uint64_t bytes;
+};
+
 inline __host__ __device__ uint64_t makeKey(uint32_t w, uint32_t a, uint32_t b) {
   return ((uint64_t)w << 44) | ((uint64_t)a << 22) | (uint64_t)b;
 }

@@ -1357,6 +1357,45 @@ __global__ void k_ref_coloring(DevCfg cfg, const WorldState* ws, const DevManifo
 }
 
 // ---------------------------------------------------------------------------
+// serving: moving worlds between the resident pool and the batch slots
+// ---------------------------------------------------------------------------
+// Copies every descriptor's byte range; blockIdx.y selects the descriptor.
+__global__ void k_copy_many(const CopyDesc* d, uint32_t n) {
+  const uint32_t k = blockIdx.y;
+  if (k >= n) return;
+  const uint8_t* src = (const uint8_t*)d[k].src;
+  uint8_t* dst = (uint8_t*)d[k].dst;
+  const uint64_t bytes = d[k].bytes;
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  if (((uintptr_t)src | (uintptr_t)dst | bytes) % 16 == 0) {
+    const uint4* s4 = (const uint4*)src;
+    uint4* d4 = (uint4*)dst;
+    for (uint64_t i = t0; i < bytes / 16; i += stride) d4[i] = s4[i];
+  } else {
+    for (uint64_t i = t0; i < bytes; i += stride) dst[i] = src[i];
+  }
+}
+
+// Manifolds moved into a batch slot: pair keys carry the slot as world.
+__global__ void k_rekey_manifolds(DevManifold* ml, uint32_t begin, uint32_t count, uint32_t world) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  DevManifold& d = ml[begin + i];
+  d.key = makeKey(world, keyA(d.key), keyB(d.key));
+}
+
+// Empties batch slot w: no active bodies, no id mappings.
+__global__ void k_clear_slot(DevCfg cfg, DevBodies B, uint32_t w) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cfg.max_bodies;
+       i += gridDim.x * blockDim.x)
+    B.active[(size_t)w * cfg.max_bodies + i] = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cfg.id_cap;
+       i += gridDim.x * blockDim.x)
+    B.idmap[(size_t)w * cfg.id_cap + i] = -1;
+}
+
+// ---------------------------------------------------------------------------
 // solver: ContactSolver.solve for one world per block (src/solver.zig:494-561)
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMaxColors = 1024;
@@ -2206,6 +2245,17 @@ void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldSta
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc) {
   k_update_end<<<nw, 256, 0, s>>>(cfg, B, st, nw, substeps, acc);
+}
+
+void launch_copy_many(hipStream_t s, const CopyDesc* d, uint32_t n) {
+  if (n) k_copy_many<<<dim3(32, n), 256, 0, s>>>(d, n);
+}
+void launch_rekey_manifolds(hipStream_t s, DevManifold* ml, uint32_t begin, uint32_t count,
+                            uint32_t world) {
+  if (count) k_rekey_manifolds<<<cdiv(count, 256), 256, 0, s>>>(ml, begin, count, world);
+}
+void launch_clear_slot(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint32_t w) {
+  k_clear_slot<<<8, 256, 0, s>>>(cfg, B, w);
 }
 
 }  // namespace vxd

@@ -68,4 +68,9 @@ size_t exclusive_scan_u32_bytes(uint32_t n);
 hipError_t exclusive_scan_u32(void* tmp, size_t bytes, const uint32_t* in, uint32_t* out,
                               uint32_t n, hipStream_t s);
 
+void launch_copy_many(hipStream_t s, const CopyDesc* d, uint32_t n);
+void launch_rekey_manifolds(hipStream_t s, DevManifold* ml, uint32_t begin, uint32_t count,
+                            uint32_t world);
+void launch_clear_slot(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint32_t w);
+
 }  // namespace vxd

@@ -165,6 +165,22 @@ class BodyState(C.Structure):
     ]
 
 
+class RequestDone(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("world", C.c_uint32), ("frames", C.c_uint32),
+                ("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+                ("body_substeps", C.c_uint64)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("worlds", C.c_uint32), ("capacity", C.c_uint32), ("slots", C.c_uint32),
+                ("occupied", C.c_uint32), ("queued", C.c_uint32), ("pad", C.c_uint32),
+                ("ticks", C.c_uint64), ("completed", C.c_uint64), ("body_substeps", C.c_uint64),
+                ("resident_bytes", C.c_uint64), ("budget_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 assert C.sizeof(Config) == 100, C.sizeof(Config)
 assert C.sizeof(Contact) == 88, C.sizeof(Contact)
 assert C.sizeof(Manifold) == 376, C.sizeof(Manifold)
@@ -209,6 +225,24 @@ PRODUCT_ONLY = {
     "profile_enable": (C.c_int, [_P, C.c_int]),
     "profile_get": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "profile_solver_phases": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
+    # serving (continuous batching over resident worlds)
+    "server_create": (C.c_int, [C.POINTER(Config), C.c_uint32, C.c_uint32, C.c_uint64,
+                                C.POINTER(_P)]),
+    "server_destroy": (C.c_int, [_P]),
+    "server_collider_create": (C.c_int, [_P, C.c_float, C.c_uint32, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
+    "server_world_create": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "server_body_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, _f3, _f3, C.c_uint32, C.c_float,
+                                     C.POINTER(Handle)]),
+    "server_submit": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_double]),
+    "server_tick": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "server_poll": (C.c_int, [_P, C.POINTER(RequestDone), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "server_bodies_get": (C.c_int, [_P, C.c_uint32, C.POINTER(BodyState), C.c_uint32,
+                                    C.POINTER(C.c_uint32)]),
+    "server_stats_get": (C.c_int, [_P, C.c_uint32, C.POINTER(Stats)]),
+    "server_manifolds_get": (C.c_int, [_P, C.c_uint32, C.POINTER(Manifold), C.c_uint32,
+                                       C.POINTER(C.c_uint32)]),
+    "server_info_get": (C.c_int, [_P, C.POINTER(ServerInfo)]),
 }
 KERNEL_NAMES = ["step_begin", "fat_aabb", "pairs", "pair_process", "cache_replay", "probe",
                 "reduce", "compact", "islands", "solve"]

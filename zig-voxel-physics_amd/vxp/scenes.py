@@ -463,6 +463,56 @@ def batched_worlds(seed=45, n_worlds=512, lo=8, hi=1024, frames=300, blob_pool=2
     return sc
 
 
+_POOL_SHAPES = None  # (blob pool, box table) inherited by serving-pool workers
+
+
+def _pool_world(args):
+    seed, n, ground, vs = args
+    blobs, boxes = _POOL_SHAPES
+    w = _mixed_world(SplitMix64(seed), None, n, ground, vs, 6.0, (1.0, 20.0), shape_pool=blobs,
+                     box_cache=boxes, cell=0.4)
+    w[0] = BodySpec(_abi.STATIC, (0.0, -vs * 0.5, 0.0), (0.0, 0.0, 0.0, 1.0), ground)
+    return w
+
+
+def serving_pool(seed=46, n_worlds=2048, lo=8, hi=2048, blob_pool=256, name="c4_serving",
+                 workers=16):
+    """Resident worlds for the serving loop: body counts log-uniform in [lo, hi],
+    the c1 shape mix over a 64x1x64 ground (as batched_worlds).  Every box size
+    has its collider up front and each world draws from its own SplitMix64
+    stream (seeded from the scene stream), so worlds generate in parallel."""
+    global _POOL_SHAPES
+    rng = SplitMix64(seed)
+    vs = 0.2
+    counts = log_uniform_counts(rng, n_worlds, lo, hi)
+    sc = Scene(name, scene_config("c4", 1 / 60, 4, 8, max(counts) + 1), 1 / 60, 0)
+    g = _add_collider(sc, *hand_ground(64, 64), vs, "ground64")
+    blobs = []
+    for _ in range(blob_pool):
+        vox, org = blob_shape(rng)
+        blobs.append((_add_collider(sc, vox, org, vs, "blob16"), _voxel_centres(vox, org, vs)))
+    boxes = {}
+    for sx in range(3, 17):
+        for sy in range(3, 17):
+            for sz in range(3, 17):
+                vox, org = box_shape(sx, sy, sz)
+                boxes[(sx, sy, sz)] = (_add_collider(sc, vox, org, vs, f"box{sx}x{sy}x{sz}"),
+                                       _voxel_centres(vox, org, vs))
+    seeds = [rng.next_u64() for _ in range(n_worlds)]
+    jobs = [(seeds[i], counts[i], g, vs) for i in range(n_worlds)]
+    _POOL_SHAPES = (blobs, boxes)
+    try:
+        if workers and workers > 1 and n_worlds > 32:
+            import multiprocessing as mp
+            with mp.get_context("fork").Pool(workers) as pool:
+                sc.worlds = pool.map(_pool_world, jobs, chunksize=8)
+        else:
+            sc.worlds = [_pool_world(j) for j in jobs]
+    finally:
+        _POOL_SHAPES = None
+    return sc
+
+
 def build_system(lib, scene, cfg_over=None, worlds=None):
     """Create a system of `lib` (product or oracle) holding `scene`.
     Returns (system pointer, per-world handle lists)."""
