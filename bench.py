@@ -4,12 +4,17 @@
 One "step" is one update(dt) frame of a world (4 solver substeps).  The metric
 is body-substeps/s: awake dynamic bodies at the start of every executed step x
 substeps, divided by the wall time of the update() calls, with the worlds
-already resident in HBM.  `value` is the 4096-body scene on one GPU; the
-"batched" object steps 512 independent worlds of uneven size together and
-compares them with the plain single-world path looped over the same worlds.
-Prints one JSON line.
+already resident in HBM.  `value` is the 4096-body scene on one GPU.  Beside
+it: "batched" steps 512 independent worlds of uneven size together and
+compares them with the plain single-world path looped over the same worlds;
+"rocks" reports voxel-pair tests/s and contacts/s on the 256-rock scene;
+"serving" drives the continuous-batching loop over 2048 resident worlds with
+Poisson request arrivals and reports p50/p99 request latency per offered rate;
+"cpu_baseline" times the CPU restatement on the 4096-body scene.  Prints one
+JSON line.
 
   python bench.py [--gpus 1] [--steps 590] [--warmup 10] [--batched-steps 290]
+                  [--c2-steps 390] [--serve-rates 2000,5000,10000,20000]
 """
 import argparse
 import ctypes as C
@@ -25,6 +30,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "zig-voxel-physics_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def solver_bytes(st, substeps, iters):
@@ -128,6 +137,7 @@ def run_serving(lib, args):
         done = done[done["tag"] < (1 << 62)]
         lat = (done["done_s"] - done["submit_s"]) * 1e3
         info1 = srv.info()
+        log(f"serving: {lam:g} req/s offered, {len(done)} completed")
         rates.append({
             "offered_rps": lam, "submitted": i, "completed": int(len(done)),
             "served_rps": len(done) / wall,
@@ -163,6 +173,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--c2-steps", type=int, default=390)
     ap.add_argument("--serve-worlds", type=int, default=2048)
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -178,6 +189,7 @@ def main():
 
     lib = vxp.product_lib()
     # ---- plain single scene: 4096 bodies ----------------------------------------
+    log("4096-body scene")
     sc = scenes.c1_large()
     s1 = vxp.from_scene(sc, lib=lib)
     plain = run(s1, sc, args.warmup, args.steps, profile=True)
@@ -197,6 +209,7 @@ def main():
     # ---- batched: 512 worlds ------------------------------------------------------
     batched = None
     if not args.no_batched:
+        log("512 batched worlds")
         sb = scenes.batched_worlds()
         sysb = vxp.from_scene(sb, lib=lib)
         rb = run(sysb, sb, args.warmup, args.batched_steps)
@@ -232,14 +245,41 @@ def main():
                                    f"time, frames {args.warmup}..{args.warmup + lf - 1}",
         }
 
+    # ---- narrow-phase-heavy scene: 256 rocks -------------------------------------
+    rocks = None
+    if args.c2_steps:
+        log("256-rock scene")
+        sr = scenes.c2_rocks()
+        s2 = vxp.from_scene(sr, lib=lib)
+        for _ in range(args.warmup):
+            s2.update(sr.dt)
+        vchk = cont = bsub = 0
+        t_tot = 0.0
+        for _ in range(args.c2_steps):
+            t = time.perf_counter()
+            s2.update(sr.dt)
+            t_tot += time.perf_counter() - t
+            st = s2.stats()
+            vchk += st["voxels_checked"]
+            cont += st["contacts_generated"]
+            bsub += st["awake_bodies_stepped"] * sr.config["substeps"]
+        s2.close()
+        rocks = {"workload": f"{sr.name}: 256 rocks of 32^3 voxels (0.1 m), dt 1/120, "
+                             f"{sr.config['substeps']} substeps, {sr.config['velocity_iterations']} "
+                             f"iterations, frames {args.warmup}..{args.warmup + args.c2_steps - 1}",
+                 "voxel_pair_tests_per_s": vchk / t_tot, "contacts_per_s": cont / t_tot,
+                 "body_substeps_per_s": bsub / t_tot, "ms_per_step": t_tot * 1e3 / args.c2_steps}
+
     # ---- serving: Poisson requests over 2048 resident worlds -------------------
     serving = None
     if not args.no_serving:
+        log("serving")
         serving = run_serving(lib, args)
 
     # ---- CPU baseline: the CPU restatement on the same scene, bounded sample ----
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline")
         threads = min(16, os.cpu_count() or 1)
         ora = vxp.from_scene(sc, lib=vxp.oracle_lib(), cfg_over=dict(cpu_threads=threads))
         gpu2 = vxp.from_scene(sc, lib=lib)
@@ -290,6 +330,7 @@ def main():
         "final_stats": {k: final[k] for k in ("manifolds", "colors", "sleeping_bodies",
                                               "voxels_checked", "broad_phase_pairs")},
         "batched": batched,
+        "rocks": rocks,
         "serving": serving,
         "cpu_baseline": cpu,
     }

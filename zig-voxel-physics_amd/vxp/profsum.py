@@ -2,6 +2,8 @@
 
   python3 -m vxp.profsum trace  <results.db> [top]
   python3 -m vxp.profsum pmc    <results.db> <COUNTER> [name-filter]
+  python3 vxp/profsum.py copycal      (the calibration program: five 256 MiB
+                                       device copies, profiled like the bench)
 """
 import sqlite3
 import sys
@@ -34,8 +36,21 @@ def pmc(db, counter, filt=""):
     return "\n".join(out)
 
 
+def copycal():
+    import torch
+    n = 64 * 1024 * 1024  # floats: 256 MiB
+    a = torch.ones(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied", n * 4, "bytes x5", flush=True)
+
+
 if __name__ == "__main__":
-    if sys.argv[1] == "trace":
+    if sys.argv[1] == "copycal":
+        copycal()
+    elif sys.argv[1] == "trace":
         print(trace(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 15))
     else:
         print(pmc(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else ""))
