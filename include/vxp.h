@@ -177,6 +177,12 @@ int vxp_body_create(vxp_system* sys, uint32_t body_type, const float pos[3], con
                     uint32_t collider, float density, vxp_handle* out);
 int vxp_body_create_w(vxp_system* sys, uint32_t world, uint32_t body_type, const float pos[3],
                       const float rot[4], uint32_t collider, float density, vxp_handle* out);
+/* createDynamicBodyWithMass (src/system.zig:323-351): a dynamic body with the given
+ * inverse mass and local inverse inertia (row-major 3x3) instead of the
+ * collider's computeMassProperties. */
+int vxp_body_create_with_mass_w(vxp_system* sys, uint32_t world, const float pos[3],
+                                const float rot[4], uint32_t collider, float inv_mass,
+                                const float inv_inertia[9], vxp_handle* out);
 /* destroyBody (src/system.zig:405-433) */
 int vxp_body_destroy_w(vxp_system* sys, uint32_t world, vxp_handle h);
 
@@ -186,6 +192,13 @@ int vxp_system_step(vxp_system* sys, float dt);
 /* Batched: `frames` consecutive update(dt) calls on every world, one device
  * launch sequence per phase for all worlds together. */
 int vxp_system_update_frames(vxp_system* sys, float dt, uint32_t frames);
+
+/* setGravity / setSolverConfig (src/system.zig:1034-1052): gravity and the
+ * SolverConfig fields of `cfg` (substeps, iterations, soft-constraint and
+ * position-correction parameters, warm starting, friction, restitution,
+ * rotation) apply from the next step; every other field must equal the
+ * system's (VXP_ERR_ARGUMENT otherwise). */
+int vxp_system_set_config(vxp_system* sys, const vxp_config* cfg);
 
 /* getBody / accessors (src/system.zig:436-445,885-1008): copy state out / in.
  * `out` receives max_bodies entries (slot order) of world `w`. */

@@ -1028,7 +1028,10 @@ struct World {
   }
 
   // src/system.zig:294-402
-  int createBody(uint32_t type, Vec3 pos, Quat rot, int collider, float density, vxp_handle* h) {
+  // createDynamicBodyWithMass (src/system.zig:323-351) passes `given` instead
+  // of computing the mass properties from the collider.
+  int createBody(uint32_t type, Vec3 pos, Quat rot, int collider, float density, vxp_handle* h,
+                 const MassProps* given = nullptr) {
     int rc = allocate(h);
     if (rc) return rc;
     Slot& s = slots[h->index];
@@ -1039,8 +1042,9 @@ struct World {
     b.rot = rot;
     b.collider = collider;
     if (type == VXP_BODY_DYNAMIC) {
-      MassProps mp = collider >= 0 ? computeMassProperties(*shared->colliders[collider], density)
-                                   : MassProps{0.0f, Mat3::zero()};
+      MassProps mp = given ? *given
+                     : collider >= 0 ? computeMassProperties(*shared->colliders[collider], density)
+                                     : MassProps{0.0f, Mat3::zero()};
       b.inv_mass = mp.inv_mass;
       b.inv_inertia_local = mp.inv_inertia;
       b.sleeping = false;
@@ -1533,6 +1537,23 @@ struct vxo_system {
 
 static thread_local std::string g_err;
 
+// Fields that setGravity / setSolverConfig leave alone must not change.
+static bool configChangeAllowed(const vxp_config& a, const vxp_config& b) {
+  vxp_config x = a, y = b;
+  auto clear = [](vxp_config& c) {
+    for (float& g : c.gravity) g = 0.0f;
+    c.substeps = c.velocity_iterations = c.position_iterations = 0;
+    c.hertz = c.damping_ratio = c.position_correction_rate = c.max_position_correction = 0.0f;
+    c.slop = c.warm_start_scale = 0.0f;
+    c.disable_rotation = c.warm_starting = c.enable_friction = c.enable_restitution = 0;
+    c.pad0[0] = c.pad0[1] = 0;
+    c.cpu_threads = 0;
+  };
+  clear(x);
+  clear(y);
+  return std::memcmp(&x, &y, sizeof(vxp_config)) == 0 && b.substeps > 0;
+}
+
 extern "C" {
 
 const char* vxo_last_error(void) { return g_err.c_str(); }
@@ -1606,6 +1627,28 @@ int vxo_body_create_w(vxo_system* s, uint32_t world, uint32_t type, const float 
 int vxo_body_create(vxo_system* s, uint32_t type, const float pos[3], const float rot[4],
                     uint32_t collider, float density, vxp_handle* out) {
   return vxo_body_create_w(s, 0, type, pos, rot, collider, density, out);
+}
+int vxo_body_create_with_mass_w(vxo_system* s, uint32_t world, const float pos[3],
+                                const float rot[4], uint32_t collider, float inv_mass,
+                                const float inv_inertia[9], vxp_handle* out) {
+  if (!s || !out || !pos || !rot || !inv_inertia || world >= s->worlds.size()) return VXP_ERR_ARGUMENT;
+  if (collider != 0xFFFFFFFFu && collider >= s->shared.colliders.size()) return VXP_ERR_ARGUMENT;
+  MassProps mp;
+  mp.inv_mass = inv_mass;
+  for (int i = 0; i < 9; i++) mp.inv_inertia.m[i / 3][i % 3] = inv_inertia[i];
+  return s->worlds[world]->createBody(VXP_BODY_DYNAMIC, Vec3{pos[0], pos[1], pos[2]},
+                                      Quat{rot[0], rot[1], rot[2], rot[3]},
+                                      collider == 0xFFFFFFFFu ? -1 : (int)collider, 1.0f, out, &mp);
+}
+// setGravity / setSolverConfig (src/system.zig:1034-1052)
+int vxo_system_set_config(vxo_system* s, const vxp_config* cfg) {
+  if (!s || !cfg || s->worlds.empty()) return VXP_ERR_ARGUMENT;
+  if (!configChangeAllowed(s->worlds[0]->cfg, *cfg)) {
+    g_err = "only gravity and the SolverConfig fields can change after creation";
+    return VXP_ERR_ARGUMENT;
+  }
+  for (auto& w : s->worlds) w->cfg = *cfg;
+  return VXP_OK;
 }
 int vxo_body_destroy_w(vxo_system* s, uint32_t world, vxp_handle h) {
   if (world >= s->worlds.size()) return VXP_ERR_ARGUMENT;

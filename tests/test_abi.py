@@ -89,3 +89,17 @@ def test_header_functions_cite_the_reference():
                  "vxp_manifolds_get_w", "vxp_pairs_get_w", "vxp_islands_get_w"):
         i = text.index(name)
         assert "src/" in text[max(0, i - 600):i], name
+
+
+def test_set_config_accepts_only_gravity_and_solver_fields(oracle):
+    import vxp as _v
+    s = _v.VoxelPhysicsSystem(lib=oracle, max_bodies=8)
+    s.setGravity((0.0, -3.0, 0.0))
+    assert s.getGravity() == (0.0, -3.0, 0.0)
+    s.setSolverConfig(substeps=2, velocity_iterations=5, hertz=15.0, enable_friction=0)
+    with pytest.raises(_v.VxpError):
+        s.setSolverConfig(max_bodies=16)          # a PhysicsConfig field
+    with pytest.raises(_v.VxpError):
+        s.setSolverConfig(substeps=0)
+    assert s.cfg.max_bodies == 8 and s.cfg.substeps == 2
+    s.close()

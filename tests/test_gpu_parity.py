@@ -93,3 +93,29 @@ def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_abi, "_cache", {})
     with pytest.raises(vxp.VxpError):
         vxp.VoxelPhysicsSystem()
+
+
+def test_mass_gravity_solver_config_and_kinematic(product, oracle):
+    """createDynamicBodyWithMass, setGravity, setSolverConfig and a kinematic
+    body driven through the pile, in lockstep with the restatement."""
+    sc = scenes.c0_small()
+    a = vxp.from_scene(sc, lib=product, cfg_over=dict(max_bodies=68))
+    b = vxp.from_scene(sc, lib=oracle, cfg_over=dict(max_bodies=68))
+    inv_i = (3.0, 0.1, 0.0, 0.1, 2.5, 0.0, 0.0, 0.0, 4.0)
+    for fr in range(160):
+        for s in (a, b):
+            if fr == 20:
+                s.setGravity((0.0, -4.0, 1.5))
+            if fr == 35:
+                s.setSolverConfig(velocity_iterations=3, hertz=20.0, position_iterations=4)
+            if fr == 45:
+                s.extra = s.createDynamicBodyWithMass((0.4, 3.0, -0.3), (0.0, 0.0, 0.0, 1.0), 1,
+                                                      0.7, inv_i)
+            if fr == 60:
+                s.kin = s.createKinematicBody((-4.5, 0.2, 0.0), (0.0, 0.0, 0.0, 1.0), 1)
+                s.setState(s.kin, linear_velocity=(2.0, 0.0, 0.3))
+            s.update(sc.dt)
+        if fr % 5 == 0 or fr in (20, 35, 45, 60, 61):
+            assert compare.diff_bodies(a.bodies(), b.bodies()) is None, fr
+            assert compare.diff_manifolds(a.manifolds(), b.manifolds()) is None, fr
+            assert compare.diff_stats(a.stats(), b.stats()) is None, fr

@@ -825,8 +825,9 @@ int vxp_collider_create(vxp_system* s, float voxel_size, uint32_t n, const int32
   return VXP_OK;
 }
 
-int vxp_body_create_w(vxp_system* s, uint32_t w, uint32_t type, const float pos[3], const float rot[4],
-                      uint32_t col, float density, vxp_handle* out) {
+static int createBody(vxp_system* s, uint32_t w, uint32_t type, const float pos[3],
+                      const float rot[4], uint32_t col, float density, const MassProps* given,
+                      vxp_handle* out) {
   if (!s || !out || !pos || !rot || w >= s->nw || type > VXP_BODY_DYNAMIC) return VXP_ERR_ARGUMENT;
   if (col != 0xFFFFFFFFu && col >= s->colliders.size()) return VXP_ERR_ARGUMENT;
   RC(ensureHost(s));
@@ -862,7 +863,9 @@ int vxp_body_create_w(vxp_system* s, uint32_t w, uint32_t type, const float pos[
   s->rest[g] = s->cfg.default_restitution;
   if (type == VXP_BODY_DYNAMIC) {
     MassProps mp{0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}};
-    if (col != 0xFFFFFFFFu) {
+    if (given) {
+      mp = *given;
+    } else if (col != 0xFFFFFFFFu) {
       HostCollider& hc = s->colliders[col];
       uint32_t key;
       std::memcpy(&key, &density, 4);
@@ -903,9 +906,52 @@ int vxp_body_create_w(vxp_system* s, uint32_t w, uint32_t type, const float pos[
   return VXP_OK;
 }
 
+int vxp_body_create_w(vxp_system* s, uint32_t w, uint32_t type, const float pos[3], const float rot[4],
+                      uint32_t col, float density, vxp_handle* out) {
+  return createBody(s, w, type, pos, rot, col, density, nullptr, out);
+}
+
 int vxp_body_create(vxp_system* s, uint32_t type, const float pos[3], const float rot[4], uint32_t col,
                     float density, vxp_handle* out) {
   return vxp_body_create_w(s, 0, type, pos, rot, col, density, out);
+}
+
+int vxp_body_create_with_mass_w(vxp_system* s, uint32_t w, const float pos[3], const float rot[4],
+                                uint32_t col, float inv_mass, const float inv_inertia[9],
+                                vxp_handle* out) {
+  if (!inv_inertia) return VXP_ERR_ARGUMENT;
+  MassProps mp;
+  mp.inv_mass = inv_mass;
+  for (int i = 0; i < 9; i++) mp.inv_inertia.m[i] = inv_inertia[i];
+  return createBody(s, w, VXP_BODY_DYNAMIC, pos, rot, col, 1.0f, &mp, out);
+}
+
+// Fields that setGravity / setSolverConfig leave alone must not change.
+static bool configChangeAllowed(const vxp_config& a, const vxp_config& b) {
+  vxp_config x = a, y = b;
+  auto clear = [](vxp_config& c) {
+    for (float& g : c.gravity) g = 0.0f;
+    c.substeps = c.velocity_iterations = c.position_iterations = 0;
+    c.hertz = c.damping_ratio = c.position_correction_rate = c.max_position_correction = 0.0f;
+    c.slop = c.warm_start_scale = 0.0f;
+    c.disable_rotation = c.warm_starting = c.enable_friction = c.enable_restitution = 0;
+    c.pad0[0] = c.pad0[1] = 0;
+    c.cpu_threads = 0;
+  };
+  clear(x);
+  clear(y);
+  return std::memcmp(&x, &y, sizeof(vxp_config)) == 0 && b.substeps > 0;
+}
+
+// setGravity / setSolverConfig (src/system.zig:1034-1052)
+int vxp_system_set_config(vxp_system* s, const vxp_config* cfg) {
+  if (!s || !cfg) return VXP_ERR_ARGUMENT;
+  if (!configChangeAllowed(s->cfg, *cfg)) {
+    g_err = "only gravity and the SolverConfig fields can change after creation";
+    return VXP_ERR_ARGUMENT;
+  }
+  s->cfg = *cfg;
+  return VXP_OK;
 }
 
 static bool validHandle(vxp_system* s, uint32_t w, vxp_handle h) {

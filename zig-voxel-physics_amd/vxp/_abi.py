@@ -199,6 +199,9 @@ SIGNATURES = {
                               C.POINTER(Handle)]),
     "body_create_w": (C.c_int, [_P, C.c_uint32, C.c_uint32, _f3, _f3, C.c_uint32, C.c_float,
                                 C.POINTER(Handle)]),
+    "body_create_with_mass_w": (C.c_int, [_P, C.c_uint32, _f3, _f3, C.c_uint32, C.c_float, _f3,
+                                          C.POINTER(Handle)]),
+    "system_set_config": (C.c_int, [_P, C.POINTER(Config)]),
     "body_destroy_w": (C.c_int, [_P, C.c_uint32, Handle]),
     "system_update": (C.c_int, [_P, C.c_float]),
     "system_step": (C.c_int, [_P, C.c_float]),

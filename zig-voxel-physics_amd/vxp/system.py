@@ -88,6 +88,36 @@ class VoxelPhysicsSystem:
     def createKinematicBody(self, position, rotation, collider, world=0):
         return self._create(_abi.KINEMATIC, position, rotation, collider, 1.0, world)
 
+    def createDynamicBodyWithMass(self, position, rotation, collider, inv_mass, inv_inertia,
+                                  world=0):
+        """createDynamicBodyWithMass (src/system.zig:323-351); inv_inertia is the
+        local inverse inertia, 9 floats row-major."""
+        h = _abi.Handle()
+        self.lib.check(self.lib.body_create_with_mass_w(self.ptr, world, (C.c_float * 3)(*position),
+                                                        (C.c_float * 4)(*rotation), collider,
+                                                        inv_mass, (C.c_float * 9)(*inv_inertia),
+                                                        C.byref(h)), "body_create_with_mass_w")
+        return h
+
+    # ---- configuration (src/system.zig:1034-1052) ------------------------------
+    def setGravity(self, gravity):
+        self.cfg.gravity[:] = list(gravity)
+        self.lib.check(self.lib.system_set_config(self.ptr, C.byref(self.cfg)), "system_set_config")
+
+    def getGravity(self):
+        return tuple(self.cfg.gravity)
+
+    def setSolverConfig(self, **fields):
+        """SolverConfig fields by name (substeps, velocity_iterations, hertz, ...)."""
+        old = {k: getattr(self.cfg, k) for k in fields}
+        for k, v in fields.items():
+            setattr(self.cfg, k, v)
+        rc = self.lib.system_set_config(self.ptr, C.byref(self.cfg))
+        if rc != _abi.VXP_OK:
+            for k, v in old.items():
+                setattr(self.cfg, k, v)
+        self.lib.check(rc, "system_set_config")
+
     def destroyBody(self, handle, world=0):
         return self.lib.body_destroy_w(self.ptr, world, handle)
 
