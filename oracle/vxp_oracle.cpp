@@ -269,6 +269,20 @@ struct Manifold {  // src/narrow_phase.zig:396-502
   }
 };
 
+// ContactManager.addManifold warm start (src/narrow_phase.zig:1049-1059): each
+// new contact takes the impulses of the previous frame's contact with the same
+// feature id (the first one).
+inline void warmStartFrom(Manifold& m, const Manifold& old) {
+  for (int i = 0; i < m.n; i++)
+    for (int j = 0; j < old.n; j++)
+      if (m.c[i].feature_id == old.c[j].feature_id) {
+        m.c[i].normal_impulse = old.c[j].normal_impulse;
+        m.c[i].tangent_impulse[0] = old.c[j].tangent_impulse[0];
+        m.c[i].tangent_impulse[1] = old.c[j].tangent_impulse[1];
+        break;
+      }
+}
+
 inline uint64_t pairKey(uint32_t a, uint32_t b) {
   uint32_t lo = std::min(a, b), hi = std::max(a, b);
   return ((uint64_t)lo << 32) | hi;
@@ -1121,7 +1135,13 @@ struct World {
     std::vector<It> items;
     for (uint32_t i = 0; i < slots.size(); i++)
       if (slots[i].active) items.push_back({slots[i].body.fat.min.x, i});
+    // NaN boxes (a body whose collider is empty has an infinite local AABB, whose
+    // transform is NaN) overlap nothing; they sort last so the order stays a
+    // strict weak ordering and finite boxes keep their early exit.
     std::sort(items.begin(), items.end(), [](const It& a, const It& b) {
+      bool na = std::isnan(a.x0), nb = std::isnan(b.x0);
+      if (na != nb) return nb;
+      if (na) return a.slot < b.slot;
       return a.x0 < b.x0 || (a.x0 == b.x0 && a.slot < b.slot);
     });
     pairs.clear();
@@ -1174,17 +1194,7 @@ struct World {
   void addManifold(Manifold m) {
     uint64_t key = pairKey(m.a, m.b);
     auto it = prev.find(key);
-    if (it != prev.end()) {
-      const Manifold& old = it->second;
-      for (int i = 0; i < m.n; i++)
-        for (int j = 0; j < old.n; j++)
-          if (m.c[i].feature_id == old.c[j].feature_id) {
-            m.c[i].normal_impulse = old.c[j].normal_impulse;
-            m.c[i].tangent_impulse[0] = old.c[j].tangent_impulse[0];
-            m.c[i].tangent_impulse[1] = old.c[j].tangent_impulse[1];
-            break;
-          }
-    }
+    if (it != prev.end()) warmStartFrom(m, it->second);
     cur[key] = m;
   }
 
@@ -1874,6 +1884,33 @@ void vxo_cache_script(uint32_t cap, uint32_t n, const uint32_t* ops, const uint3
   }
 }
 uint32_t vxo_color_prio(uint64_t key) { return colorPrio(key); }
+// VoxelCollider.voxelToBrickPos / voxelToLocalCoord (src/collider.zig:195-210)
+void vxo_voxel_to_brick(const int32_t v[3], int32_t brick[3], int32_t local[3]) {
+  for (int k = 0; k < 3; k++) {
+    brick[k] = floorDiv8(v[k]);
+    local[k] = mod8(v[k]);
+  }
+}
+// warm start of one manifold from the previous frame's (impulses 3 per contact)
+void vxo_warm_start(uint32_t n_new, const uint64_t* new_fid, uint32_t n_old, const uint64_t* old_fid,
+                    const float* old_imp, float* out_imp) {
+  Manifold m, o;
+  m.n = (int)std::min(n_new, 4u);
+  o.n = (int)std::min(n_old, 4u);
+  for (int i = 0; i < m.n; i++) m.c[i].feature_id = new_fid[i];
+  for (int j = 0; j < o.n; j++) {
+    o.c[j].feature_id = old_fid[j];
+    o.c[j].normal_impulse = old_imp[3 * j];
+    o.c[j].tangent_impulse[0] = old_imp[3 * j + 1];
+    o.c[j].tangent_impulse[1] = old_imp[3 * j + 2];
+  }
+  warmStartFrom(m, o);
+  for (int i = 0; i < m.n; i++) {
+    out_imp[3 * i] = m.c[i].normal_impulse;
+    out_imp[3 * i + 1] = m.c[i].tangent_impulse[0];
+    out_imp[3 * i + 2] = m.c[i].tangent_impulse[1];
+  }
+}
 // Island labels of an edge list (src/islands.zig:318-441); is_static[id] for
 // ids < n_ids.  Writes (id, label) ascending; returns the count.
 uint32_t vxo_island_labels(uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t n_ids,

@@ -119,3 +119,26 @@ def test_mass_gravity_solver_config_and_kinematic(product, oracle):
             assert compare.diff_bodies(a.bodies(), b.bodies()) is None, fr
             assert compare.diff_manifolds(a.manifolds(), b.manifolds()) is None, fr
             assert compare.diff_stats(a.stats(), b.stats()) is None, fr
+
+
+def test_empty_collider_body_in_scene(product, oracle):
+    """A body whose collider has no voxels (src/system.zig:1139-1158) falls under
+    gravity; its broad-phase box is NaN (transform of the empty AABB) and must
+    not disturb the pair set of the other bodies."""
+    sc = scenes.c0_small()
+    runs = []
+    for lib in (product, oracle):
+        s = vxp.from_scene(sc, lib=lib, cfg_over=dict(max_bodies=67))
+        cid = s.addCollider(voxel_size=0.1, brick_pos=np.zeros((0, 3), np.int32),
+                            voxels=np.zeros((0, 512), np.uint8))
+        h = s.createDynamicBody((0.2, 5.0, 0.1), (0.0, 0.0, 0.0, 1.0), cid, 1.0)
+        runs.append((s, h))
+    (a, ha), (b, hb) = runs
+    for fr in range(90):
+        a.update(sc.dt)
+        b.update(sc.dt)
+        if fr % 10 == 0:
+            assert compare.diff_bodies(a.bodies(), b.bodies()) is None, fr
+            assert np.array_equal(a.pairs(), b.pairs()), fr
+            assert compare.diff_manifolds(a.manifolds(), b.manifolds()) is None, fr
+    assert a.getPosition(ha)[1] < 5.0

@@ -47,6 +47,9 @@ def k(oracle):
         refcolor=_bind(L, "vxo_ref_coloring", None, [C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint8)]),
         mass=_bind(L, "vxo_mass_properties", C.c_int, [C.c_void_p, C.c_uint32, C.c_float, P(C.c_float)]),
         info=_bind(L, "vxo_collider_info", C.c_int, [C.c_void_p, C.c_uint32, P(C.c_int32), P(C.c_float), P(C.c_uint32)]),
+        brick=_bind(L, "vxo_voxel_to_brick", None, [P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+        warm=_bind(L, "vxo_warm_start", None, [C.c_uint32, P(C.c_uint64), C.c_uint32, P(C.c_uint64),
+                                               P(C.c_float), P(C.c_float)]),
         detect=_bind(L, "vxo_detect_pair", C.c_int, [C.c_void_p, C.c_uint32, P(C.c_float), C.c_uint32, P(C.c_float), P(vxp._abi.Manifold), P(C.c_uint32), P(C.c_uint32)]),
     )
 
@@ -561,3 +564,47 @@ def test_solve_order_drift_is_small(oracle):
         b.update(sc.dt)
     d = np.abs(a.bodies()["position"] - b.bodies()["position"]).max()
     assert d < 0.05, d
+
+
+# ---- src/collider.zig:466-486: brick position and local coordinate -----------
+def test_voxel_to_brick_and_local(k):
+    def conv(v):
+        b, l = I3(), I3()
+        k["brick"](I3(*v), b, l)
+        return tuple(b), tuple(l)
+    assert conv((15, 20, 25))[0] == (1, 2, 3)
+    assert conv((-1, -10, -20))[0] == (-1, -2, -3)
+    assert conv((13, 18, 23))[1] == (5, 2, 7)
+    # floor division and modulo agree with Python's on a sweep around zero
+    for x in range(-40, 41):
+        b, l = conv((x, -x, x * 3))
+        assert b == (x // 8, (-x) // 8, (x * 3) // 8) and l == (x % 8, (-x) % 8, (x * 3) % 8)
+    # the host-side collider builder maps voxels the same way
+    col = vxp.VoxelCollider(0.1)
+    col.setCollisionVoxel(-1, -10, -20, vxp.voxel_byte(vxp.FACE, 2))
+    pos, vox = col.arrays()
+    assert tuple(pos[0]) == (-1, -2, -3)
+    assert vox[0][7 + 8 * 6 + 64 * 4] != 0
+
+
+# ---- src/narrow_phase.zig:1796-1825: ContactManager warm starting -------------
+def test_warm_start_carries_impulse(k):
+    out = (C.c_float * 12)()
+    k["warm"](1, (C.c_uint64 * 1)(12345), 1, (C.c_uint64 * 1)(12345),
+              (C.c_float * 3)(50.0, 1.5, -2.0), out)
+    assert list(out)[:3] == [50.0, 1.5, -2.0]
+    # unmatched feature ids start from zero; the first matching old contact wins
+    k["warm"](3, (C.c_uint64 * 3)(7, 12345, 9), 3, (C.c_uint64 * 3)(9, 12345, 9),
+              (C.c_float * 9)(1, 2, 3, 4, 5, 6, 7, 8, 9), out)
+    assert list(out)[:9] == [0, 0, 0, 4, 5, 6, 1, 2, 3]
+
+
+# ---- src/system.zig:1139-1158: one step of a body with an empty collider ------
+def test_system_step_empty_collider(oracle):
+    s = vxp.VoxelPhysicsSystem(lib=oracle)
+    cid = s.addCollider(voxel_size=0.1, brick_pos=np.zeros((0, 3), np.int32),
+                        voxels=np.zeros((0, 512), np.uint8))
+    h = s.createDynamicBody((0.0, 5.0, 0.0), (0.0, 0.0, 0.0, 1.0), cid, 1.0)
+    s.step(1.0 / 60.0)
+    assert s.getPosition(h)[1] < 5.0
+    s.close()
