@@ -61,13 +61,14 @@ def run(sys_, sc, warmup, steps, profile=False):
         sys_.update(sc.dt)
     if profile:
         lib.profile_enable(sys_.ptr, 1)
-    frame_ms, body_substeps, alg = [], 0, 0
+    frame_ms, body_substeps, alg, cands = [], 0, 0, 0
     for _ in range(steps):
         t = time.perf_counter()
         sys_.update(sc.dt)  # synchronous: returns after the frame's kernels finish
         frame_ms.append((time.perf_counter() - t) * 1e3)
         for st in sys_.stats_all():
             body_substeps += st.awake_bodies_stepped * cfg["substeps"]
+            cands += st.contact_candidates
             alg += solver_bytes(st, cfg["substeps"], cfg["velocity_iterations"])
     kern = {}
     if profile:
@@ -79,7 +80,7 @@ def run(sys_, sc, warmup, steps, profile=False):
         lib.profile_enable(sys_.ptr, 0)
     wall = sum(frame_ms) / 1e3
     return dict(value=body_substeps / wall, wall=wall, frame_ms=frame_ms, kern=kern,
-                alg_solver_bytes=alg, body_substeps=body_substeps)
+                alg_solver_bytes=alg, body_substeps=body_substeps, candidates=cands)
 
 
 def pct(xs, q):
@@ -238,6 +239,7 @@ def main():
                         f"(log-uniform 8..1024 per world), frames {args.warmup}.."
                         f"{args.warmup + args.batched_steps - 1}",
             "ms_per_step": rb["wall"] * 1e3 / args.batched_steps,
+            "contact_candidates_per_step": rb["candidates"] / args.batched_steps,
             "p50_step_ms": statistics.median(rb["frame_ms"]),
             "p99_step_ms": pct(rb["frame_ms"], 0.99),
             "plain_looped_value": loop_bs / loop_t if loop_t else None,
@@ -253,7 +255,7 @@ def main():
         s2 = vxp.from_scene(sr, lib=lib)
         for _ in range(args.warmup):
             s2.update(sr.dt)
-        vchk = cont = bsub = 0
+        vchk = cont = bsub = cand = 0
         t_tot = 0.0
         for _ in range(args.c2_steps):
             t = time.perf_counter()
@@ -262,12 +264,14 @@ def main():
             st = s2.stats()
             vchk += st["voxels_checked"]
             cont += st["contacts_generated"]
+            cand += st["contact_candidates"]
             bsub += st["awake_bodies_stepped"] * sr.config["substeps"]
         s2.close()
         rocks = {"workload": f"{sr.name}: 256 rocks of 32^3 voxels (0.1 m), dt 1/120, "
                              f"{sr.config['substeps']} substeps, {sr.config['velocity_iterations']} "
                              f"iterations, frames {args.warmup}..{args.warmup + args.c2_steps - 1}",
                  "voxel_pair_tests_per_s": vchk / t_tot, "contacts_per_s": cont / t_tot,
+                 "contact_candidates_per_s": cand / t_tot,
                  "body_substeps_per_s": bsub / t_tot, "ms_per_step": t_tot * 1e3 / args.c2_steps}
 
     # ---- serving: Poisson requests over 2048 resident worlds -------------------

@@ -108,6 +108,7 @@ typedef struct vxp_stats {
   uint32_t solver_points;         /* contact points in solved manifolds (last step) */
   float max_penetration;
   float accumulated_time;
+  uint32_t contact_candidates;    /* generateContact candidates before the top-4 (last step) */
 } vxp_stats;
 
 /* ContactPoint (src/narrow_phase.zig:336-393) */

@@ -1215,6 +1215,7 @@ struct World {
   }
   void finishDetect(PairWork& w) {
     stats.voxels_checked += w.det.voxels_checked;
+    stats.contact_candidates += w.det.candidates;
     if (w.det.has_collision) {
       stats.narrow_phase_colliding++;
       stats.contacts_generated += (uint32_t)w.det.m.n;
@@ -1239,6 +1240,7 @@ struct World {
     cache.frame++;
     stats.narrow_phase_tested = stats.narrow_phase_colliding = stats.contacts_generated = 0;
     stats.voxels_checked = stats.cache_hits = stats.cache_misses = 0;
+    stats.contact_candidates = 0;
     findPairs();
     stats.broad_phase_pairs = (uint32_t)pairs.size();
     std::vector<PairWork> work;

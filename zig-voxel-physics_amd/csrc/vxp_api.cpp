@@ -323,6 +323,30 @@ void profFlush(vxp_system* s) {
   s->pending.clear();
 }
 
+// Waits for the system's stream by polling: a blocking wait can sleep the host
+// thread, which then wakes late and leaves the GPU idle before the next launch.
+int streamWait(vxp_system* s) {
+  for (;;) {
+    hipError_t e = hipStreamQuery(s->stream);
+    if (e == hipSuccess) return VXP_OK;
+    if (e != hipErrorNotReady) {
+      g_err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+      return VXP_ERR_DEVICE;
+    }
+    __builtin_ia32_pause();
+  }
+}
+
+// Copies device words into the pinned read area and waits for them.
+int readSizes(vxp_system* s, std::initializer_list<const uint32_t*> src, uint32_t* out) {
+  uint32_t i = 0;
+  for (const uint32_t* p : src)
+    HIP_TRY(hipMemcpyAsync(s->h_read + i++, p, 4, hipMemcpyDeviceToHost, s->stream));
+  RC(streamWait(s));
+  for (uint32_t k = 0; k < i; k++) out[k] = s->h_read[k];
+  return VXP_OK;
+}
+
 int stepAll(vxp_system* s) {
   hipStream_t st = s->stream;
   RC(ensureDevice(s));
@@ -354,10 +378,14 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_shi.ensure((size_t)ns * 16));
   pe = profBegin(s);
   launch_sweep_keys(st, cfg, B, ws, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns, nw);
-  size_t tb = sort_pairs_u64_u32_bytes(ns);
+  // keys are (world << 32 | sweep coordinate): zero above bit 32 + log2(worlds)
+  uint32_t wbits = 0;
+  while ((1u << wbits) < nw) wbits++;
+  size_t tb = sort_pairs_u64_u32_bytes(ns, 32 + wbits);
   HIP_TRY(s->d_sort_tmp.ensure(tb));
   HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, tb, s->d_skeys.as<uint64_t>(), s->d_skeys2.as<uint64_t>(),
-                             s->d_svals.as<uint32_t>(), s->d_svals2.as<uint32_t>(), ns, st));
+                             s->d_svals.as<uint32_t>(), s->d_svals2.as<uint32_t>(), ns, 32 + wbits,
+                             st));
   HIP_TRY(s->d_send.ensure((size_t)ns * 4));
   HIP_TRY(s->d_wcnt.ensure((size_t)(ns + 1) * 4));
   HIP_TRY(s->d_woff.ensure((size_t)(ns + 1) * 4));
@@ -383,8 +411,7 @@ int stepAll(vxp_system* s) {
                                s->d_swo.as<uint32_t>(), swv + 1, st));
   }
   uint32_t npairs = 0;
-  HIP_TRY(hipMemcpyAsync(&npairs, s->d_swo.as<uint32_t>() + swv, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  RC(readSizes(s, {s->d_swo.as<uint32_t>() + swv}, &npairs));
   HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
   if (npairs)
     launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
@@ -395,10 +422,11 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
   pe = profBegin(s);
   if (npairs) {
-    size_t kb = sort_keys_u64_bytes(npairs);
+    // pair keys (world << 44 | a << 22 | b) are zero above bit 44 + log2(worlds)
+    size_t kb = sort_keys_u64_bytes(npairs, 44 + wbits);
     HIP_TRY(s->d_sort_tmp.ensure(kb));
     HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(), s->d_pairs.as<uint64_t>(),
-                          npairs, st));
+                          npairs, 44 + wbits, st));
   }
   profEnd(s, VXP_K_PAIRS, pe);
   const uint64_t* keys = s->d_pairs.as<uint64_t>();
@@ -409,7 +437,7 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_counts.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_wi_off.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_produce.ensure((size_t)np1 * 4));
-  HIP_TRY(s->d_rstat.ensure((size_t)np1 * 16));
+  HIP_TRY(s->d_rstat.ensure((size_t)np1 * sizeof(RStat)));
   HIP_TRY(s->d_clist.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_wlist.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_cflag.ensure((size_t)(npairs + 1) * 4));
@@ -420,7 +448,7 @@ int stepAll(vxp_system* s) {
   PairRec* pr = s->d_pr.as<PairRec>();
   pe = profBegin(s);
   launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, s->d_cflag.as<uint32_t>(),
-                      s->d_produce.as<uint32_t>(), s->d_rstat.as<uint4>(), s->d_wlist.as<uint32_t>(),
+                      s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(), s->d_wlist.as<uint32_t>(),
                       s->d_step_ctr.as<uint32_t>());
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
   pe = profBegin(s);
@@ -442,9 +470,8 @@ int stepAll(vxp_system* s) {
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, sb, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
                                npairs, st));
     uint32_t last[2];
-    HIP_TRY(hipMemcpyAsync(&last[0], s->d_wi_off.as<uint32_t>() + npairs - 1, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&last[1], s->d_counts.as<uint32_t>() + npairs - 1, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    RC(readSizes(s, {s->d_wi_off.as<uint32_t>() + npairs - 1, s->d_counts.as<uint32_t>() + npairs - 1},
+                 last));
     total_wi = last[0] + last[1];
   }
   HIP_TRY(s->d_wi_cbase.ensure((size_t)(total_wi + 1) * 4));
@@ -462,9 +489,7 @@ int stepAll(vxp_system* s) {
     HIP_TRY(s->d_sort_tmp.ensure(cb));
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wi_ccnt.as<uint32_t>(),
                                s->d_wi_cbase.as<uint32_t>(), total_wi + 1, st));
-    HIP_TRY(hipMemcpyAsync(&ncand, s->d_wi_cbase.as<uint32_t>() + total_wi, 4,
-                           hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    RC(readSizes(s, {s->d_wi_cbase.as<uint32_t>() + total_wi}, &ncand));
   }
   HIP_TRY(s->d_cand.ensure((size_t)std::max(ncand, 1u) * sizeof(Cand)));
   if (ncand)
@@ -474,13 +499,13 @@ int stepAll(vxp_system* s) {
                  false);
   profEnd(s, VXP_K_PROBE, pe);
   pe = profBegin(s);
-  launch_reduce(st, cfg, B, ws, stats, pr, keys, npairs, s->d_wlist.as<uint32_t>(),
+  launch_reduce(st, cfg, B, C, ws, stats, pr, keys, npairs, s->d_wlist.as<uint32_t>(),
                 s->d_step_ctr.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
                 s->d_cand.as<Cand>(), prev, s->d_cache_keys.as<CacheKeyRec>(),
                 s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>(),
                 s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>(),
-                s->d_rstat.as<uint4>());
+                s->d_rstat.as<RStat>());
   profEnd(s, VXP_K_REDUCE, pe);
   pe = profBegin(s);
   if (npairs) {
@@ -493,8 +518,7 @@ int stepAll(vxp_system* s) {
   launch_world_segments(st, ws, nw, s->d_total.as<uint32_t>());
   profEnd(s, VXP_K_COMPACT, pe);
   uint32_t nman = 0;
-  HIP_TRY(hipMemcpyAsync(&nman, s->d_total.p, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  RC(readSizes(s, {s->d_total.as<uint32_t>()}, &nman));
   s->n_manifolds = nman;
   uint32_t nm1 = std::max(nman, 1u);
   HIP_TRY(s->d_ml[s->cur].ensure((size_t)nm1 * sizeof(DevManifold)));
@@ -724,6 +748,8 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
   s->n_static.assign(n_worlds, 0);
   int rc = VXP_OK;
   auto fail = [&](int r) {
+    if (s->h_read) (void)hipHostFree(s->h_read);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return r;
   };
@@ -737,6 +763,10 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     g_err = "hipStreamCreate failed";
     return fail(VXP_ERR_DEVICE);
+  }
+  if (hipHostMalloc((void**)&s->h_read, 256, hipHostMallocDefault) != hipSuccess) {
+    g_err = "pinned host allocation failed";
+    return fail(VXP_ERR_OOM);
   }
   std::vector<WorldState> ws(n_worlds);
   std::memset(ws.data(), 0, ws.size() * sizeof(WorldState));
@@ -774,6 +804,7 @@ int vxp_system_destroy(vxp_system* s) {
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+  if (s->h_read) (void)hipHostFree(s->h_read);
   (void)hipStreamDestroy(s->stream);
   delete s;
   return VXP_OK;

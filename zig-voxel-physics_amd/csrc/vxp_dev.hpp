@@ -107,11 +107,10 @@ struct DevManifold {
   vxp_manifold m;
 };
 
-struct Cand {  // one contact candidate of generateContact
-  float point[3], normal[3], pen;
-  float la[3], lb[3];
+struct Cand {  // one contact candidate: what the top-4 replay reads, and where it came from
+  float pen;
   uint32_t fid_lo, fid_hi;
-  uint32_t pad;
+  uint32_t code;  // lane (A voxel in the work item) << 5 | probe index (B neighbour)
 };
 
 struct CacheKeyRec {
@@ -126,6 +125,11 @@ struct CacheKeyRec {
 struct CacheVal {
   vxp_contact c[4];
   uint32_t count, has_collision, vchk, pad;
+};
+
+// Narrow-phase counters of one pair, summed per world after the reduction.
+struct RStat {
+  uint32_t produced, vchk, colliding, contacts, cands, pad;
 };
 
 // One device-to-device byte range of a batched copy.

@@ -11,6 +11,7 @@
 #include <unordered_map>
 #include <utility>
 #include <vector>
+#include <initializer_list>
 
 #include "../../include/vxp.h"
 #include "vxp_dev.hpp"
@@ -135,6 +136,7 @@ struct vxp_system {
   DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
       s_pts, s_prio, s_tstamp;
   uint32_t n_pairs = 0, n_manifolds = 0;
+  uint32_t* h_read = nullptr;  // pinned host words for the per-step size reads
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   // kernel timing (HIP events on `stream`)
   bool prof = false;
@@ -159,6 +161,8 @@ DevColliders devColliders(vxp_system* s);
 DevCfg devCfg(vxp_system* s);
 int stepAll(vxp_system* s);
 int checkErrors(vxp_system* s);
+int streamWait(vxp_system* s);
+int readSizes(vxp_system* s, std::initializer_list<const uint32_t*> src, uint32_t* out);
 int updateOnce(vxp_system* s, float dt);
 }  // namespace vxh
 

@@ -41,6 +41,33 @@ __device__ __forceinline__ uint8_t voxelAt(const DevColliders& C, const DevColli
   return C.vox[(size_t)b * 512 + (x & 7) + 8 * (y & 7) + 64 * (z & 7)];
 }
 
+// One collider's brick directory geometry, held in registers for a probe loop.
+struct VoxMap {
+  int32_t org[3], dim[3];
+  uint32_t dir_off;
+};
+__device__ __forceinline__ VoxMap voxMap(const DevCollider& h) {
+  VoxMap m;
+  for (int k = 0; k < 3; k++) {
+    m.org[k] = h.org[k];
+    m.dim[k] = h.dim[k];
+  }
+  m.dir_off = h.dir_off;
+  return m;
+}
+// voxelAt with the directory and brick pool passed as non-aliasing pointers.
+__device__ __forceinline__ uint8_t voxelAt(const int32_t* __restrict__ dir,
+                                           const uint8_t* __restrict__ vox, const VoxMap& m,
+                                           int32_t x, int32_t y, int32_t z) {
+  int32_t bx = (x >> 3) - m.org[0], by = (y >> 3) - m.org[1], bz = (z >> 3) - m.org[2];
+  if ((uint32_t)bx >= (uint32_t)m.dim[0] || (uint32_t)by >= (uint32_t)m.dim[1] ||
+      (uint32_t)bz >= (uint32_t)m.dim[2])
+    return 0;
+  int32_t b = dir[m.dir_off + ((uint32_t)bz * m.dim[1] + by) * m.dim[0] + bx];
+  if (b < 0) return 0;
+  return vox[(size_t)b * 512 + (x & 7) + 8 * (y & 7) + 64 * (z & 7)];
+}
+
 __device__ __forceinline__ Box colliderBox(const DevCollider& h) {
   return Box{V3{h.lo[0], h.lo[1], h.lo[2]}, V3{h.hi[0], h.hi[1], h.hi[2]}};
 }
@@ -239,6 +266,7 @@ __global__ void k_step_begin(DevCfg cfg, DevBodies B, WorldState* ws, vxp_stats*
       s.broad_phase_pairs = s.narrow_phase_tested = s.narrow_phase_colliding = 0;
       s.contacts_generated = s.voxels_checked = s.pairs_skipped_sleeping = 0;
       s.cache_hits = s.cache_misses = s.manifolds_skipped_sleeping = 0;
+      s.contact_candidates = 0;
       s.manifolds = s.solver_manifolds = s.colors = s.position_iterations_run = 0;
       s.max_penetration = 0.0f;
     }
@@ -424,7 +452,12 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
     bool si = false;
     Box bi;
     for (uint32_t wid = g0; wid < g1; wid++) {
-      while (woff_i[i + 1] <= wid) i++;
+      // next body with work: usually adjacent; across a world's inactive slots
+      // (and bodies without candidates) search instead of walking
+      if (woff_i[i + 1] <= wid) {
+        if (woff_i[i + 2] > wid) i++;
+        else i = upperBound(woff_i, n_slots, wid) - 1;
+      }
       if (i != cur) {
         cur = i;
         const float4 li = slo[i], hi4 = shi[i];
@@ -642,7 +675,7 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
 __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
                                const DevManifold* prev, uint32_t* cflag, uint32_t* produce,
-                               uint4* rstat, uint32_t* wlist, uint32_t* step_ctr) {
+                               RStat* rstat, uint32_t* wlist, uint32_t* step_ctr) {
   __shared__ uint32_t s_n, s_base;
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   PairCounts c{0, 0, 0};
@@ -658,7 +691,7 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
     cflag[p] = f == PF_CACHE;
     if (p == n_pairs - 1) cflag[n_pairs] = 0;
     produce[p] = 0;
-    rstat[p] = make_uint4(0u, 0u, 0u, 0u);
+    rstat[p] = RStat{0u, 0u, 0u, 0u, 0u, 0u};
     work = f == PF_CARRY || f == PF_DETECT || f == PF_CACHE;
     if (work) pos = atomicAdd(&s_n, 1u);
   }
@@ -822,11 +855,15 @@ struct ProbeCtx {
   int32_t bminb[3], bmaxb[3];
 };
 
+// Probe order of a neighbour (dz, dy, dx in -1..1): (dz+1)*9 + (dy+1)*3 + (dx+1).
+// kWrite records each candidate as (penetration, feature id, lane << 5 | probe);
+// the full contact of the few that survive the top-4 is rebuilt by contactAt.
 template <bool kWrite>
-__device__ __forceinline__ uint32_t probeVoxel(const DevColliders& C, const DevCollider& HA,
-                                               const DevCollider& HB, const ProbeCtx& c, int32_t x,
-                                               int32_t y, int32_t z, uint8_t va, uint32_t* vchk,
-                                               Cand* out) {
+__device__ __forceinline__ uint32_t probeVoxel(const int32_t* __restrict__ dir,
+                                               const uint8_t* __restrict__ vox, const VoxMap& mb,
+                                               const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
+                                               uint8_t va, uint32_t lane, uint32_t* vchk,
+                                               Cand* __restrict__ out) {
   V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
   V3 lb = invXformPoint(c.pb, c.qb, wa);
   int32_t cb[3];
@@ -840,7 +877,7 @@ __device__ __forceinline__ uint32_t probeVoxel(const DevColliders& C, const DevC
         if (bx < c.bminb[0] || bx >= c.bmaxb[0] || by < c.bminb[1] || by >= c.bmaxb[1] ||
             bz < c.bminb[2] || bz >= c.bmaxb[2])
           continue;
-        uint8_t vb = voxelAt(C, HB, bx, by, bz);
+        uint8_t vb = voxelAt(dir, vox, mb, bx, by, bz);
         uint32_t tb = vb & 7u;
         if (tb == 0) continue;
         vc++;
@@ -850,38 +887,16 @@ __device__ __forceinline__ uint32_t probeVoxel(const DevColliders& C, const DevC
         float dsq = dot(diff, diff);
         if (!(dsq < c.spec)) continue;
         if (kWrite) {
-          // generateContact (src/narrow_phase.zig:860-922)
+          // the penetration of generateContact (src/narrow_phase.zig:879-906)
           float dist = sqrtf(dsq);
-          V3 cp = scale(add(wa, wb), 0.5f);
           float pen = c.vsa - dist;
-          V3 na = qrotate(c.qa, lutNormal(va >> 3));
-          V3 nb = qrotate(c.qb, lutNormal(vb >> 3));
-          V3 nrm;
-          if (ta == VXP_VOXEL_FACE) {
-            nrm = na;
-          } else if (tb == VXP_VOXEL_FACE) {
-            nrm = neg(nb);
-          } else if (dist > 0.001f) {
-            V3 dn = scale(diff, 1.0f / dist);
-            if (dot(dn, na) > 0.3f) nrm = na;
-            else if (dot(dn, neg(nb)) > 0.3f) nrm = neg(nb);
-            else nrm = dn;
-          } else {
-            nrm = na;
-          }
           if (ta == VXP_VOXEL_INSIDE || tb == VXP_VOXEL_INSIDE) pen = fmaxf(pen, c.vsa);
-          V3 la = invXformPoint(c.pa, c.qa, cp);
-          V3 lbb = invXformPoint(c.pb, c.qb, cp);
           uint64_t fid = featureId(x, y, z, bx, by, bz);
           Cand& o = out[n];
-          o.point[0] = cp.x; o.point[1] = cp.y; o.point[2] = cp.z;
-          o.normal[0] = nrm.x; o.normal[1] = nrm.y; o.normal[2] = nrm.z;
           o.pen = pen;
-          o.la[0] = la.x; o.la[1] = la.y; o.la[2] = la.z;
-          o.lb[0] = lbb.x; o.lb[1] = lbb.y; o.lb[2] = lbb.z;
           o.fid_lo = (uint32_t)fid;
           o.fid_hi = (uint32_t)(fid >> 32);
-          o.pad = 0;
+          o.code = (lane << 5) | (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
         }
         n++;
       }
@@ -889,6 +904,87 @@ __device__ __forceinline__ uint32_t probeVoxel(const DevColliders& C, const DevC
   return n;
 }
 
+// generateContact (src/narrow_phase.zig:860-922) for A voxel (x, y, z) against
+// the B voxel of probe `probe` around it, with the same arithmetic as the probe.
+__device__ vxp_contact contactAt(const DevColliders& C, const DevCollider& HA,
+                                 const DevCollider& HB, const ProbeCtx& c, int32_t x, int32_t y,
+                                 int32_t z, uint32_t probe) {
+  const uint8_t va = voxelAt(C, HA, x, y, z);
+  V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
+  V3 lb = invXformPoint(c.pb, c.qb, wa);
+  int32_t cb[3];
+  localToVoxel(c.vsb, lb, cb);
+  const int32_t dz = (int32_t)(probe / 9) - 1, dy = (int32_t)(probe / 3 % 3) - 1,
+                dx = (int32_t)(probe % 3) - 1;
+  const int32_t bx = cb[0] + dx, by = cb[1] + dy, bz = cb[2] + dz;
+  const uint8_t vb = voxelAt(C, HB, bx, by, bz);
+  const uint32_t ta = va & 7u, tb = vb & 7u;
+  V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
+  V3 diff = sub(wb, wa);
+  float dsq = dot(diff, diff);
+  float dist = sqrtf(dsq);
+  V3 cp = scale(add(wa, wb), 0.5f);
+  float pen = c.vsa - dist;
+  V3 na = qrotate(c.qa, lutNormal(va >> 3));
+  V3 nb = qrotate(c.qb, lutNormal(vb >> 3));
+  V3 nrm;
+  if (ta == VXP_VOXEL_FACE) {
+    nrm = na;
+  } else if (tb == VXP_VOXEL_FACE) {
+    nrm = neg(nb);
+  } else if (dist > 0.001f) {
+    V3 dn = scale(diff, 1.0f / dist);
+    if (dot(dn, na) > 0.3f) nrm = na;
+    else if (dot(dn, neg(nb)) > 0.3f) nrm = neg(nb);
+    else nrm = dn;
+  } else {
+    nrm = na;
+  }
+  if (ta == VXP_VOXEL_INSIDE || tb == VXP_VOXEL_INSIDE) pen = fmaxf(pen, c.vsa);
+  V3 la = invXformPoint(c.pa, c.qa, cp);
+  V3 lbb = invXformPoint(c.pb, c.qb, cp);
+  vxp_contact o;
+  o.point[0] = cp.x; o.point[1] = cp.y; o.point[2] = cp.z;
+  o.normal[0] = nrm.x; o.normal[1] = nrm.y; o.normal[2] = nrm.z;
+  o.penetration = pen;
+  o.local_anchor_a[0] = la.x; o.local_anchor_a[1] = la.y; o.local_anchor_a[2] = la.z;
+  o.local_anchor_b[0] = lbb.x; o.local_anchor_b[1] = lbb.y; o.local_anchor_b[2] = lbb.z;
+  o.normal_impulse = 0.0f;
+  o.tangent_impulse[0] = o.tangent_impulse[1] = 0.0f;
+  o.normal_mass = 0.0f;
+  o.tangent_mass[0] = o.tangent_mass[1] = 0.0f;
+  o.pad = 0;
+  o.feature_id = featureId(x, y, z, bx, by, bz);
+  return o;
+}
+
+// Probe context of pair r (body poses, voxel sizes, B's voxel bounds).
+__device__ __forceinline__ ProbeCtx probeCtx(const DevBodies& B, const DevCollider& HA,
+                                             const DevCollider& HB, uint32_t ga, uint32_t gb) {
+  ProbeCtx c;
+  c.pa = f4v(B.pos[ga]);
+  c.pb = f4v(B.pos[gb]);
+  c.qa = f4q(B.rot[ga]);
+  c.qb = f4q(B.rot[gb]);
+  c.vsa = HA.vs;
+  c.vsb = HB.vs;
+  float cds = c.vsa * c.vsa * 1.0f * 1.0f;
+  c.spec = cds + 0.02f * 0.02f + 2.0f * sqrtf(cds) * 0.02f;
+  for (int k = 0; k < 3; k++) {
+    c.bminb[k] = HB.minb[k];
+    c.bmaxb[k] = HB.maxb[k];
+  }
+  return c;
+}
+
+// Region voxel `lin` (z, y, x order) of pair r.
+__device__ __forceinline__ void regionVoxel(const PairRec& r, uint32_t lin, int32_t& x, int32_t& y,
+                                            int32_t& z) {
+  uint32_t nxy = r.nx * r.ny;
+  x = r.mn[0] + (int32_t)(lin % r.nx);
+  y = r.mn[1] + (int32_t)((lin / r.nx) % r.ny);
+  z = r.mn[2] + (int32_t)(lin / nxy);
+}
 
 // Two passes over the same work items.  kCount: per-lane candidate counts
 // (lane_n), per-work-item totals wi_ccnt (wi_ccnt[total_wi] = 0) and
@@ -911,33 +1007,19 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
   uint32_t ga = w * cfg.max_bodies + r.sa, gb = w * cfg.max_bodies + r.sb;
   const DevCollider& HA = C.hdr[B.collider[ga]];
   const DevCollider& HB = C.hdr[B.collider[gb]];
-  ProbeCtx c;
-  c.pa = f4v(B.pos[ga]);
-  c.pb = f4v(B.pos[gb]);
-  c.qa = f4q(B.rot[ga]);
-  c.qb = f4q(B.rot[gb]);
-  c.vsa = HA.vs;
-  c.vsb = HB.vs;
-  float cds = c.vsa * c.vsa * 1.0f * 1.0f;
-  c.spec = cds + 0.02f * 0.02f + 2.0f * sqrtf(cds) * 0.02f;
-  for (int k = 0; k < 3; k++) {
-    c.bminb[k] = HB.minb[k];
-    c.bmaxb[k] = HB.maxb[k];
-  }
+  const ProbeCtx c = probeCtx(B, HA, HB, ga, gb);
   uint32_t lin = chunk * kWave + lane;
-  uint32_t nxy = r.nx * r.ny;
-  bool valid = lin < nxy * r.nz;
+  bool valid = lin < r.nx * r.ny * r.nz;
   int32_t x = 0, y = 0, z = 0;
   uint8_t va = 0;
   if (valid) {
-    x = r.mn[0] + (int32_t)(lin % r.nx);
-    y = r.mn[1] + (int32_t)((lin / r.nx) % r.ny);
-    z = r.mn[2] + (int32_t)(lin / nxy);
+    regionVoxel(r, lin, x, y, z);
     va = voxelAt(C, HA, x, y, z);
   }
   if (kCount) {
     uint32_t vchk = 0, n = 0;
-    if ((va & 7u) != 0) n = probeVoxel<false>(C, HA, HB, c, x, y, z, va, &vchk, nullptr);
+    if ((va & 7u) != 0)
+      n = probeVoxel<false>(C.dir, C.vox, voxMap(HB), c, x, y, z, va, lane, &vchk, nullptr);
     lane_n[(size_t)wid * kWave + lane] = (uint8_t)n;
     uint32_t total = waveSum(n), vsum = waveSum(vchk);
     if (lane == 0) {
@@ -949,7 +1031,8 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
     uint32_t n = lane_n[(size_t)wid * kWave + lane];
     int incl = waveInclusiveScan((int)n, (int)lane);
     if (n > 0)
-      probeVoxel<true>(C, HA, HB, c, x, y, z, va, nullptr, cand + wi_cbase[wid] + (uint32_t)incl - n);
+      probeVoxel<true>(C.dir, C.vox, voxMap(HB), c, x, y, z, va, lane, nullptr,
+                       cand + wi_cbase[wid] + (uint32_t)incl - n);
   }
 }
 
@@ -957,19 +1040,6 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
 // manifold reduction: ContactManifold.addContact replay over the candidates
 // in sequence order, warm start against the previous frame, cache hit/carry.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void candToContact(const Cand& c, vxp_contact& o) {
-  o.point[0] = c.point[0]; o.point[1] = c.point[1]; o.point[2] = c.point[2];
-  o.normal[0] = c.normal[0]; o.normal[1] = c.normal[1]; o.normal[2] = c.normal[2];
-  o.penetration = c.pen;
-  o.local_anchor_a[0] = c.la[0]; o.local_anchor_a[1] = c.la[1]; o.local_anchor_a[2] = c.la[2];
-  o.local_anchor_b[0] = c.lb[0]; o.local_anchor_b[1] = c.lb[1]; o.local_anchor_b[2] = c.lb[2];
-  o.normal_impulse = 0.0f;
-  o.tangent_impulse[0] = o.tangent_impulse[1] = 0.0f;
-  o.normal_mass = 0.0f;
-  o.tangent_mass[0] = o.tangent_mass[1] = 0.0f;
-  o.pad = 0;
-  o.feature_id = ((uint64_t)c.fid_hi << 32) | c.fid_lo;
-}
 
 // ContactManager.addManifold warm start (src/narrow_phase.zig:1011-1021,1049-1059)
 __device__ void warmStartFrom(vxp_manifold& m, const DevManifold* prev, int32_t j) {
@@ -994,6 +1064,7 @@ __device__ void warmStartFrom(vxp_manifold& m, const DevManifold* prev, int32_t 
 // state unchanged in the sequential order too.  The slots are wave-uniform.
 // Per-pair counters go to rstat and are summed by k_reduce_tail.
 __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodies& B,
+                                               const DevColliders& C,
                                                const WorldState* ws, const PairRec* pr,
                                                const uint64_t* keys, uint32_t p, uint32_t f,
                                                const uint32_t* wi_off, const uint32_t* wi_cbase,
@@ -1001,7 +1072,7 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
                                                const Cand* cand, const DevManifold* prev,
                                                const CacheVal* cvals, CacheVal* cstage,
                                                DevManifold* staging, uint32_t* produce,
-                                               uint4* rstat) {
+                                               RStat* rstat) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t aux = pr[p].aux;
   if (f == PF_CARRY) {
@@ -1010,20 +1081,20 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
     for (uint32_t i = lane; i < sizeof(DevManifold) / 8; i += kWave) dst[i] = src[i];
     if (lane == 0) {
       produce[p] = 1;
-      rstat[p] = make_uint4(1u, 0u, 0u, 0u);
+      rstat[p] = RStat{1u, 0u, 0u, 0u, 0u, 0u};
     }
     return;
   }
   const uint32_t w = pr[p].world, sa = pr[p].sa, sb = pr[p].sb;
   const uint32_t ga = w * cfg.max_bodies + sa, gb = w * cfg.max_bodies + sb;
-  uint32_t ncont = 0, vchk = 0;
+  uint32_t ncont = 0, vchk = 0, ncand = 0;
   vxp_contact mine;  // contact `lane` of the manifold (lanes 0..3)
   if (f == PF_HIT) {
     const CacheVal& v = cvals[(size_t)w * cfg.cache_cap + aux];
     if (!v.has_collision) {
       if (lane == 0) {
         produce[p] = 0;
-        rstat[p] = make_uint4(0u, 0u, 0u, 0u);
+        rstat[p] = RStat{0u, 0u, 0u, 0u, 0u, 0u};
       }
       return;
     }
@@ -1036,11 +1107,13 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
     for (uint32_t wi = w0 + lane; wi < w1; wi += kWave) vs += wi_vchk[wi];
     vchk = waveSum(vs);
     uint32_t sidx0 = 0, sidx1 = 0, sidx2 = 0, sidx3 = 0;
+    uint32_t swi0 = 0, swi1 = 0, swi2 = 0, swi3 = 0;  // work item of each slot's candidate
     float spen0 = 0.f, spen1 = 0.f, spen2 = 0.f, spen3 = 0.f;
     uint64_t sfid0 = 0, sfid1 = 0, sfid2 = 0, sfid3 = 0;
     uint32_t ns = 0;
     for (uint32_t wi = w0; wi < w1; wi++) {
       const uint32_t cnt = wi_ccnt[wi], base = wi_cbase[wi];
+      ncand += cnt;
       for (uint32_t off = 0; off < cnt; off += kWave) {
         const bool valid = off + lane < cnt;
         float pen = 0.0f;
@@ -1067,15 +1140,15 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
           const uint32_t ik = base + off + (uint32_t)k;
           last = k;
           // addContact on the uniform slots
-          if (ns > 0 && sfid0 == fk) { sidx0 = ik; spen0 = pk; continue; }
-          if (ns > 1 && sfid1 == fk) { sidx1 = ik; spen1 = pk; continue; }
-          if (ns > 2 && sfid2 == fk) { sidx2 = ik; spen2 = pk; continue; }
-          if (ns > 3 && sfid3 == fk) { sidx3 = ik; spen3 = pk; continue; }
+          if (ns > 0 && sfid0 == fk) { sidx0 = ik; swi0 = wi; spen0 = pk; continue; }
+          if (ns > 1 && sfid1 == fk) { sidx1 = ik; swi1 = wi; spen1 = pk; continue; }
+          if (ns > 2 && sfid2 == fk) { sidx2 = ik; swi2 = wi; spen2 = pk; continue; }
+          if (ns > 3 && sfid3 == fk) { sidx3 = ik; swi3 = wi; spen3 = pk; continue; }
           if (ns < 4) {
-            if (ns == 0) { sidx0 = ik; spen0 = pk; sfid0 = fk; }
-            else if (ns == 1) { sidx1 = ik; spen1 = pk; sfid1 = fk; }
-            else if (ns == 2) { sidx2 = ik; spen2 = pk; sfid2 = fk; }
-            else { sidx3 = ik; spen3 = pk; sfid3 = fk; }
+            if (ns == 0) { sidx0 = ik; swi0 = wi; spen0 = pk; sfid0 = fk; }
+            else if (ns == 1) { sidx1 = ik; swi1 = wi; spen1 = pk; sfid1 = fk; }
+            else if (ns == 2) { sidx2 = ik; swi2 = wi; spen2 = pk; sfid2 = fk; }
+            else { sidx3 = ik; swi3 = wi; spen3 = pk; sfid3 = fk; }
             ns++;
             continue;
           }
@@ -1085,17 +1158,26 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
           if (spen1 < minp) { minp = spen1; mi = 1; }
           if (spen2 < minp) { minp = spen2; mi = 2; }
           if (spen3 < minp) { minp = spen3; mi = 3; }
-          if (mi == 0) { sidx0 = ik; spen0 = pk; sfid0 = fk; }
-          else if (mi == 1) { sidx1 = ik; spen1 = pk; sfid1 = fk; }
-          else if (mi == 2) { sidx2 = ik; spen2 = pk; sfid2 = fk; }
-          else if (mi == 3) { sidx3 = ik; spen3 = pk; sfid3 = fk; }
+          if (mi == 0) { sidx0 = ik; swi0 = wi; spen0 = pk; sfid0 = fk; }
+          else if (mi == 1) { sidx1 = ik; swi1 = wi; spen1 = pk; sfid1 = fk; }
+          else if (mi == 2) { sidx2 = ik; swi2 = wi; spen2 = pk; sfid2 = fk; }
+          else if (mi == 3) { sidx3 = ik; swi3 = wi; spen3 = pk; sfid3 = fk; }
         }
       }
     }
     ncont = ns;
     if (lane < ncont) {
-      uint32_t my = lane == 0 ? sidx0 : lane == 1 ? sidx1 : lane == 2 ? sidx2 : sidx3;
-      candToContact(cand[my], mine);
+      // rebuild the winner's contact from its A voxel (lane of its work item)
+      // and B neighbour (probe index)
+      const uint32_t my = lane == 0 ? sidx0 : lane == 1 ? sidx1 : lane == 2 ? sidx2 : sidx3;
+      const uint32_t mwi = lane == 0 ? swi0 : lane == 1 ? swi1 : lane == 2 ? swi2 : swi3;
+      const uint32_t code = cand[my].code;
+      const DevCollider& HA = C.hdr[B.collider[ga]];
+      const DevCollider& HB = C.hdr[B.collider[gb]];
+      const ProbeCtx c = probeCtx(B, HA, HB, ga, gb);
+      int32_t x, y, z;
+      regionVoxel(pr[p], (mwi - w0) * kWave + (code >> 5), x, y, z);
+      mine = contactAt(C, HA, HB, c, x, y, z, code & 31u);
     }
     if (f == PF_MISS) {
       // cached value: contacts before warm starting (src/narrow_phase.zig:1501-1509)
@@ -1112,7 +1194,7 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
   if (ncont == 0) {
     if (lane == 0) {
       produce[p] = 0;
-      rstat[p] = make_uint4(0u, vchk, 0u, 0u);
+      rstat[p] = RStat{0u, vchk, 0u, 0u, ncand, 0u};
     }
     return;
   }
@@ -1145,13 +1227,14 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
     d.m.contact_count = ncont;
     d.m.color = kNone;
     produce[p] = 1;
-    rstat[p] = make_uint4(1u, vchk, 1u, ncont);
+    rstat[p] = RStat{1u, vchk, 1u, ncont, ncand, 0u};
   }
 }
 
 // Waves take the pairs of the work list (built by k_pair_process; pairs not
 // on it were cleared there) one at a time in a grid-stride loop.
-__global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, const WorldState* ws,
+__global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, DevColliders C,
+                                                const WorldState* ws,
                                                 const PairRec* pr, const uint64_t* keys,
                                                 const uint32_t* wlist, const uint32_t* step_ctr,
                                                 const uint32_t* wi_off,
@@ -1159,13 +1242,13 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, const W
                                                 const uint32_t* wi_vchk, const Cand* cand,
                                                 const DevManifold* prev, const CacheVal* cvals,
                                                 CacheVal* cstage, DevManifold* staging,
-                                                uint32_t* produce, uint4* rstat) {
+                                                uint32_t* produce, RStat* rstat) {
   const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
   const uint32_t total = step_ctr[kCtrReduceWork];
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < total;
        i += nwaves) {
     const uint32_t p = wlist[i];
-    reducePairWave(cfg, B, ws, pr, keys, p, pr[p].flag, wi_off, wi_cbase, wi_ccnt, wi_vchk, cand,
+    reducePairWave(cfg, B, C, ws, pr, keys, p, pr[p].flag, wi_off, wi_cbase, wi_ccnt, wi_vchk, cand,
                    prev, cvals, cstage, staging, produce, rstat);
   }
 }
@@ -1174,9 +1257,9 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, const W
 // reserved it during the replay), and the per-world narrow-phase counters.
 __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const PairRec* pr,
                               uint32_t n_pairs, const CacheKeyRec* keys, CacheVal* vals,
-                              const CacheVal* cstage, const uint4* rstat) {
+                              const CacheVal* cstage, const RStat* rstat) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  uint4 c = make_uint4(0u, 0u, 0u, 0u);
+  RStat c{0u, 0u, 0u, 0u, 0u, 0u};
   uint32_t w = 0;
   if (p < n_pairs) {
     const PairRec& r = pr[p];
@@ -1187,14 +1270,14 @@ __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const P
       if (keys[e].owner_frame == ws[w].frame && keys[e].owner_pair == p) vals[e] = cstage[p];
     }
   }
-  const uint32_t v0[1] = {c.x};
+  const uint32_t v0[1] = {c.produced};
   uint32_t* const c0[1] = {&ws->produced};
   blockAggAdd<1>(w, v0, c0, sizeof(WorldState) / 4);
   __syncthreads();  // blockAggAdd's LDS slots are reused
-  const uint32_t v1[3] = {c.y, c.z, c.w};
-  uint32_t* const c1[3] = {&st->voxels_checked, &st->narrow_phase_colliding,
-                           &st->contacts_generated};
-  blockAggAdd<3>(w, v1, c1, sizeof(vxp_stats) / 4);
+  const uint32_t v1[4] = {c.vchk, c.colliding, c.contacts, c.cands};
+  uint32_t* const c1[4] = {&st->voxels_checked, &st->narrow_phase_colliding,
+                           &st->contacts_generated, &st->contact_candidates};
+  blockAggAdd<4>(w, v1, c1, sizeof(vxp_stats) / 4);
 }
 
 // New manifold segments: skipped worlds keep their previous segment.
@@ -1377,22 +1460,29 @@ __global__ void k_copy_many(const CopyDesc* d, uint32_t n) {
   }
 }
 
-// Manifolds moved into a batch slot: pair keys carry the slot as world.
-__global__ void k_rekey_manifolds(DevManifold* ml, uint32_t begin, uint32_t count, uint32_t world) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  DevManifold& d = ml[begin + i];
-  d.key = makeKey(world, keyA(d.key), keyB(d.key));
+// Manifolds moved into batch slots: pair keys carry the slot as world.
+// ranges[3 * r ..] = (begin, count, slot); blockIdx.y = range.
+__global__ void k_rekey_manifolds(DevManifold* ml, const uint32_t* ranges, uint32_t n_ranges) {
+  const uint32_t r = blockIdx.y;
+  if (r >= n_ranges) return;
+  const uint32_t begin = ranges[3 * r], count = ranges[3 * r + 1], world = ranges[3 * r + 2];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    DevManifold& d = ml[begin + i];
+    d.key = makeKey(world, keyA(d.key), keyB(d.key));
+  }
 }
 
-// Empties batch slot w: no active bodies, no id mappings.
-__global__ void k_clear_slot(DevCfg cfg, DevBodies B, uint32_t w) {
+// Empties the listed batch slots: no active bodies, no id mappings.
+__global__ void k_clear_slots(DevCfg cfg, DevBodies B, const uint32_t* slots, uint32_t n) {
+  const uint32_t k = blockIdx.y;
+  if (k >= n) return;
+  const size_t w = slots[k];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cfg.max_bodies;
        i += gridDim.x * blockDim.x)
-    B.active[(size_t)w * cfg.max_bodies + i] = 0;
+    B.active[w * cfg.max_bodies + i] = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cfg.id_cap;
        i += gridDim.x * blockDim.x)
-    B.idmap[(size_t)w * cfg.id_cap + i] = -1;
+    B.idmap[w * cfg.id_cap + i] = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2171,7 +2261,7 @@ void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, 
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
-                         uint32_t* cflag, uint32_t* produce, uint4* rstat, uint32_t* wlist,
+                         uint32_t* cflag, uint32_t* produce, RStat* rstat, uint32_t* wlist,
                          uint32_t* step_ctr) {
   if (n)
     k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, cflag,
@@ -2199,16 +2289,17 @@ void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const De
     k_probe<false><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, n_pairs, wi_off, total_wi,
                                                      wi_cbase, wi_ccnt, wi_vchk, lane_n, cand);
 }
-void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
+                   WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
                    const uint32_t* wlist, const uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, uint4* rstat) {
+                   DevManifold* staging, uint32_t* produce, RStat* rstat) {
   if (!n) return;
   uint32_t blocks = std::min<uint32_t>(2048, cdiv(n, 4));
-  k_reduce<<<blocks, 256, 0, s>>>(cfg, B, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase, wi_ccnt,
+  k_reduce<<<blocks, 256, 0, s>>>(cfg, B, C, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase, wi_ccnt,
                                   wi_vchk, cand, prev, cvals, cstage, staging, produce, rstat);
   k_reduce_tail<<<cdiv(n, 256), 256, 0, s>>>(cfg, ws, st, pr, n, ckeys, cvals, cstage, rstat);
 }
@@ -2250,12 +2341,12 @@ void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp
 void launch_copy_many(hipStream_t s, const CopyDesc* d, uint32_t n) {
   if (n) k_copy_many<<<dim3(32, n), 256, 0, s>>>(d, n);
 }
-void launch_rekey_manifolds(hipStream_t s, DevManifold* ml, uint32_t begin, uint32_t count,
-                            uint32_t world) {
-  if (count) k_rekey_manifolds<<<cdiv(count, 256), 256, 0, s>>>(ml, begin, count, world);
+void launch_rekey_manifolds(hipStream_t s, DevManifold* ml, const uint32_t* ranges, uint32_t n) {
+  if (n) k_rekey_manifolds<<<dim3(4, n), 256, 0, s>>>(ml, ranges, n);
 }
-void launch_clear_slot(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint32_t w) {
-  k_clear_slot<<<8, 256, 0, s>>>(cfg, B, w);
+void launch_clear_slots(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint32_t* slots,
+                        uint32_t n) {
+  if (n) k_clear_slots<<<dim3(4, n), 256, 0, s>>>(cfg, B, slots, n);
 }
 
 }  // namespace vxd

@@ -23,7 +23,7 @@ void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, 
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
-                         uint32_t* cflag, uint32_t* produce, uint4* rstat, uint32_t* wlist,
+                         uint32_t* cflag, uint32_t* produce, RStat* rstat, uint32_t* wlist,
                          uint32_t* step_ctr);
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
@@ -33,13 +33,14 @@ void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const De
                   const PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
                   const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, uint8_t* lane_n,
                   Cand* cand, bool count);
-void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
+                   WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
                    const uint32_t* wlist, const uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, uint4* rstat);
+                   DevManifold* staging, uint32_t* produce, RStat* rstat);
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total);
 void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
                     const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
@@ -58,19 +59,20 @@ void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp
                        uint32_t nw, uint32_t substeps, float acc);
 
 // sort / scan primitives (vxp_prims.hip)
-size_t sort_pairs_u64_u32_bytes(uint32_t n);
+size_t sort_pairs_u64_u32_bytes(uint32_t n, uint32_t end_bit);
 hipError_t sort_pairs_u64_u32(void* tmp, size_t bytes, const uint64_t* kin, uint64_t* kout,
-                              const uint32_t* vin, uint32_t* vout, uint32_t n, hipStream_t s);
-size_t sort_keys_u64_bytes(uint32_t n);
+                              const uint32_t* vin, uint32_t* vout, uint32_t n, uint32_t end_bit,
+                              hipStream_t s);
+size_t sort_keys_u64_bytes(uint32_t n, uint32_t end_bit);
 hipError_t sort_keys_u64(void* tmp, size_t bytes, const uint64_t* kin, uint64_t* kout, uint32_t n,
-                         hipStream_t s);
+                         uint32_t end_bit, hipStream_t s);
 size_t exclusive_scan_u32_bytes(uint32_t n);
 hipError_t exclusive_scan_u32(void* tmp, size_t bytes, const uint32_t* in, uint32_t* out,
                               uint32_t n, hipStream_t s);
 
 void launch_copy_many(hipStream_t s, const CopyDesc* d, uint32_t n);
-void launch_rekey_manifolds(hipStream_t s, DevManifold* ml, uint32_t begin, uint32_t count,
-                            uint32_t world);
-void launch_clear_slot(hipStream_t s, const DevCfg& cfg, const DevBodies& B, uint32_t w);
+void launch_rekey_manifolds(hipStream_t s, DevManifold* ml, const uint32_t* ranges, uint32_t n);
+void launch_clear_slots(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint32_t* slots,
+                        uint32_t n);
 
 }  // namespace vxd

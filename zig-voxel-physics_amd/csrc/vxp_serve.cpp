@@ -81,8 +81,8 @@ struct vxp_server {
   uint8_t* h_stage = nullptr;
   size_t h_cap = 0;
   DevBuf d_stage;
-  std::vector<WorldState> h_ws;
-  std::vector<vxp_stats> h_st;
+  WorldState* h_ws = nullptr;  // pinned per-slot reads after each tick
+  vxp_stats* h_st = nullptr;
 };
 
 namespace {
@@ -116,7 +116,7 @@ int runCopies(vxp_server* v, std::vector<CopyDesc>& d, const std::vector<WorldSt
   const size_t off_ws = nd * sizeof(CopyDesc);
   const size_t off_w = off_ws + wsrec.size() * sizeof(WorldState);
   const size_t bytes = off_w + words.size() * 4 + 16;
-  HIP_TRY(hipStreamSynchronize(b->stream));  // the staging buffer is reused
+  RC(streamWait(b));  // the staging buffer is reused
   if (bytes > v->h_cap) {
     if (v->h_stage) (void)hipHostFree(v->h_stage);
     v->h_stage = nullptr;
@@ -139,10 +139,7 @@ int runCopies(vxp_server* v, std::vector<CopyDesc>& d, const std::vector<WorldSt
   return VXP_OK;
 }
 
-int syncBatch(vxp_server* v) {
-  HIP_TRY(hipStreamSynchronize(v->batch->stream));
-  return VXP_OK;
-}
+int syncBatch(vxp_server* v) { return streamWait(v->batch); }
 
 // Pool -> slots for the admitted (world, slot) pairs.
 int admit(vxp_server* v, const std::vector<std::pair<uint32_t, uint32_t>>& moves) {
@@ -175,9 +172,9 @@ int admit(vxp_server* v, const std::vector<std::pair<uint32_t, uint32_t>>& moves
     r.in_slot = true;
   }
   b->n_manifolds = total;
-  RC(runCopies(v, d, wsrec, wsdst, {}, nullptr));
-  for (size_t i = 0; i < rekey.size(); i += 3)
-    launch_rekey_manifolds(b->stream, ml, rekey[i], rekey[i + 1], rekey[i + 2]);
+  const uint32_t* dwords = nullptr;
+  RC(runCopies(v, d, wsrec, wsdst, rekey, &dwords));
+  launch_rekey_manifolds(b->stream, ml, dwords, (uint32_t)(rekey.size() / 3));
   HIP_TRY(hipGetLastError());
   return VXP_OK;
 }
@@ -206,10 +203,11 @@ int retire(vxp_server* v, const std::vector<std::pair<uint32_t, uint32_t>>& move
     wsdst.push_back(b->d_ws.as<WorldState>() + m.second);
     r.in_slot = false;
   }
-  RC(runCopies(v, d, wsrec, wsdst, {}, nullptr));
-  DevCfg cfg = devCfg(b);
-  DevBodies B = devBodies(b);
-  for (auto& m : moves) launch_clear_slot(b->stream, cfg, B, m.second);
+  std::vector<uint32_t> slots;
+  for (auto& m : moves) slots.push_back(m.second);
+  const uint32_t* dslots = nullptr;
+  RC(runCopies(v, d, wsrec, wsdst, slots, &dslots));
+  launch_clear_slots(b->stream, devCfg(b), devBodies(b), dslots, (uint32_t)slots.size());
   HIP_TRY(hipGetLastError());
   v->pool->dev_newer = true;
   return VXP_OK;
@@ -256,6 +254,14 @@ int vxp_server_create(const vxp_config* cfg, uint32_t max_worlds, uint32_t batch
     delete v;
     return rc;
   }
+  if (hipHostMalloc((void**)&v->h_ws, batch_slots * sizeof(WorldState), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&v->h_st, batch_slots * sizeof(vxp_stats), hipHostMallocDefault) !=
+          hipSuccess) {
+    g_err = "pinned host allocation failed";
+    vxp_server_destroy(v);
+    return VXP_ERR_OOM;
+  }
   v->res.resize(v->capacity);
   v->slot.resize(batch_slots);
   v->busy.assign(batch_slots, 0);
@@ -269,6 +275,8 @@ int vxp_server_destroy(vxp_server* v) {
   for (Resident& r : v->res) r.ml.release();
   v->d_stage.release();
   if (v->h_stage) (void)hipHostFree(v->h_stage);
+  if (v->h_ws) (void)hipHostFree(v->h_ws);
+  if (v->h_st) (void)hipHostFree(v->h_st);
   vxp_system_destroy(v->pool);
   vxp_system_destroy(v->batch);
   delete v;
@@ -356,15 +364,13 @@ int vxp_server_tick(vxp_server* v, uint32_t* n_done) {
   // ---- one frame for every slot -------------------------------------------------
   RC(updateOnce(b, b->cfg.fixed_timestep));
   v->ticks++;
-  v->h_ws.resize(v->S);
-  v->h_st.resize(v->S);
-  HIP_TRY(hipMemcpyAsync(v->h_ws.data(), b->d_ws.p, v->S * sizeof(WorldState),
-                         hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipMemcpyAsync(v->h_st.data(), b->d_stats.p, v->S * sizeof(vxp_stats),
-                         hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  for (const WorldState& w : v->h_ws)
-    if (w.error) {
+  HIP_TRY(hipMemcpyAsync(v->h_ws, b->d_ws.p, v->S * sizeof(WorldState), hipMemcpyDeviceToHost,
+                         b->stream));
+  HIP_TRY(hipMemcpyAsync(v->h_st, b->d_stats.p, v->S * sizeof(vxp_stats), hipMemcpyDeviceToHost,
+                         b->stream));
+  RC(streamWait(b));
+  for (uint32_t s = 0; s < v->S; s++)
+    if (v->h_ws[s].error) {
       g_err = "solver colouring exceeded its colour limit";
       return VXP_ERR_CAPACITY;
     }

@@ -112,7 +112,8 @@ class Stats(C.Structure):
         "island_count", "largest_island_size", "substeps_this_frame",
         "manifolds", "solver_manifolds", "colors", "position_iterations_run",
         "steps_executed", "awake_bodies_stepped", "solver_points")] + [
-        ("max_penetration", C.c_float), ("accumulated_time", C.c_float)]
+        ("max_penetration", C.c_float), ("accumulated_time", C.c_float),
+        ("contact_candidates", C.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

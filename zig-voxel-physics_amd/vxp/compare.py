@@ -89,6 +89,7 @@ STAT_KEYS = ("broad_phase_pairs", "narrow_phase_tested", "narrow_phase_colliding
              "cache_misses", "manifolds_skipped_sleeping", "island_count", "largest_island_size",
              "substeps_this_frame", "manifolds", "solver_manifolds", "colors",
              "position_iterations_run", "steps_executed", "awake_bodies_stepped",
+             "contact_candidates",
              "solver_points", "sleeping_bodies", "active_bodies",
              "total_bodies", "dynamic_bodies", "static_bodies")
 
