@@ -377,15 +377,18 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_slo.ensure((size_t)ns * 16));
   HIP_TRY(s->d_shi.ensure((size_t)ns * 16));
   pe = profBegin(s);
-  launch_sweep_keys(st, cfg, B, ws, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns, nw);
-  // keys are (world << 32 | sweep coordinate): zero above bit 32 + log2(worlds)
   uint32_t wbits = 0;
   while ((1u << wbits) < nw) wbits++;
-  size_t tb = sort_pairs_u64_u32_bytes(ns, 32 + wbits);
-  HIP_TRY(s->d_sort_tmp.ensure(tb));
-  HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, tb, s->d_skeys.as<uint64_t>(), s->d_skeys2.as<uint64_t>(),
-                             s->d_svals.as<uint32_t>(), s->d_svals2.as<uint32_t>(), ns, 32 + wbits,
-                             st));
+  if (!launch_sweep_sort(st, cfg, B, ws, s->d_svals2.as<uint32_t>(), nw)) {
+    // worlds above the block sort's size: device-wide sort of (world << 32 | coordinate)
+    // keys, which are zero above bit 32 + log2(worlds)
+    launch_sweep_keys(st, cfg, B, ws, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns, nw);
+    size_t tb = sort_pairs_u64_u32_bytes(ns, 32 + wbits);
+    HIP_TRY(s->d_sort_tmp.ensure(tb));
+    HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, tb, s->d_skeys.as<uint64_t>(),
+                               s->d_skeys2.as<uint64_t>(), s->d_svals.as<uint32_t>(),
+                               s->d_svals2.as<uint32_t>(), ns, 32 + wbits, st));
+  }
   HIP_TRY(s->d_send.ensure((size_t)ns * 4));
   HIP_TRY(s->d_wcnt.ensure((size_t)(ns + 1) * 4));
   HIP_TRY(s->d_woff.ensure((size_t)(ns + 1) * 4));
@@ -401,9 +404,11 @@ int stepAll(vxp_system* s) {
   const uint32_t swv = sweep_waves();
   HIP_TRY(s->d_swc.ensure((size_t)(swv + 1) * 4));
   HIP_TRY(s->d_swo.ensure((size_t)(swv + 1) * 4));
+  uint32_t id_bits = 1;  // body ids are below id_cap
+  while ((1u << id_bits) < s->id_cap) id_bits++;
   launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
                      s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), s->d_swc.as<uint32_t>(),
-                     nullptr, nullptr, true);
+                     nullptr, nullptr, true, id_bits);
   {
     size_t cb = exclusive_scan_u32_bytes(swv + 1);
     HIP_TRY(s->d_sort_tmp.ensure(cb));
@@ -416,17 +421,19 @@ int stepAll(vxp_system* s) {
   if (npairs)
     launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
                        s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), nullptr,
-                       s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false);
+                       s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false, id_bits);
   profEnd(s, VXP_K_PAIRS, pe);
   s->n_pairs = npairs;
   HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
   pe = profBegin(s);
   if (npairs) {
-    // pair keys (world << 44 | a << 22 | b) are zero above bit 44 + log2(worlds)
-    size_t kb = sort_keys_u64_bytes(npairs, 44 + wbits);
+    // packed pair keys (world, a, b) are zero above bit 2 * id_bits + log2(worlds)
+    const uint32_t kbits = 2 * id_bits + wbits;
+    size_t kb = sort_keys_u64_bytes(npairs, kbits);
     HIP_TRY(s->d_sort_tmp.ensure(kb));
     HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(), s->d_pairs.as<uint64_t>(),
-                          npairs, 44 + wbits, st));
+                          npairs, kbits, st));
+    launch_expand_keys(st, s->d_pairs.as<uint64_t>(), npairs, id_bits);
   }
   profEnd(s, VXP_K_PAIRS, pe);
   const uint64_t* keys = s->d_pairs.as<uint64_t>();

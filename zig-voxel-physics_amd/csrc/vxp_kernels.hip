@@ -16,6 +16,9 @@
 //                     write-back and updateSleepStates (src/system.zig:786-792)
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <rocprim/block/block_radix_sort.hpp>
+
 #include "vxp_dev.hpp"
 #include "vxp_kernels.hpp"
 
@@ -370,6 +373,32 @@ __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n
   }
 }
 
+// Sweep order of each world in one block: its slots sorted by the sweep
+// coordinate of the fat box's lower corner (inactive slots and padding last,
+// ties in slot order, as the device-wide sort of (world, coordinate) keys).
+template <uint32_t kBS, uint32_t kIPT>
+__global__ __launch_bounds__(kBS) void k_sweep_sort(DevCfg cfg, DevBodies B, const WorldState* ws,
+                                                    uint32_t* order) {
+  using Sort = rocprim::block_radix_sort<uint32_t, kBS, kIPT, uint32_t>;
+  __shared__ typename Sort::storage_type storage;
+  const uint32_t w = blockIdx.x, MB = cfg.max_bodies, ax = ws[w].axis;
+  uint32_t keys[kIPT], vals[kIPT];
+#pragma unroll
+  for (uint32_t k = 0; k < kIPT; k++) {
+    const uint32_t sl = threadIdx.x * kIPT + k;
+    const uint32_t g = w * MB + sl;
+    keys[k] = 0xFFFFFFFFu;
+    vals[k] = g;
+    if (sl < MB && B.active[g]) keys[k] = orderedFloat(comp(B.fat_lo[g], ax));
+  }
+  Sort().sort(keys, vals, storage);
+#pragma unroll
+  for (uint32_t k = 0; k < kIPT; k++) {
+    const uint32_t r = threadIdx.x * kIPT + k;
+    if (r < MB) order[w * MB + r] = vals[k];
+  }
+}
+
 __global__ void k_sweep_keys(DevCfg cfg, DevBodies B, const WorldState* ws, uint64_t* keys,
                              uint32_t* vals, uint32_t n_slots) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -439,7 +468,7 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
                                                      const float4* shi, uint32_t n_slots,
                                                      const uint32_t* send, const uint32_t* woff_i,
                                                      uint32_t* wcount, const uint32_t* woff_w,
-                                                     uint64_t* out) {
+                                                     uint64_t* out, uint32_t id_bits) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const uint32_t total = woff_i[n_slots];
@@ -480,9 +509,10 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
       if (kCount) {
         n += (uint32_t)__popcll(em);
       } else {
+        // packed key (world, a, b) with ids of id_bits bits, expanded after sorting
         if (emit)
           out[o + (uint32_t)__popcll(em & ((1ull << lane) - 1ull))] =
-              makeKey(w, min(idi, idj), max(idi, idj));
+              ((uint64_t)w << (2 * id_bits)) | ((uint64_t)min(idi, idj) << id_bits) | max(idi, idj);
         o += (uint32_t)__popcll(em);
       }
     }
@@ -2240,6 +2270,17 @@ void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
   k_world_axis<<<nw, 256, 0, s>>>(cfg, B, ws, nw);
   k_sweep_keys<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, ws, keys, vals, n_slots);
 }
+bool launch_sweep_sort(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       uint32_t* order, uint32_t nw) {
+  const uint32_t MB = cfg.max_bodies;
+  if (MB > 8192) return false;
+  k_world_axis<<<nw, 256, 0, s>>>(cfg, B, ws, nw);
+  if (MB <= 1024) k_sweep_sort<256, 4><<<nw, 256, 0, s>>>(cfg, B, ws, order);
+  else if (MB <= 2048) k_sweep_sort<256, 8><<<nw, 256, 0, s>>>(cfg, B, ws, order);
+  else if (MB <= 4096) k_sweep_sort<512, 8><<<nw, 512, 0, s>>>(cfg, B, ws, order);
+  else k_sweep_sort<1024, 8><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
+  return true;
+}
 void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
                         const uint32_t* order, uint32_t n_slots, float4* slo, float4* shi,
                         uint32_t* send, uint32_t* wcnt) {
@@ -2247,16 +2288,28 @@ void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, co
   k_sweep_count<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, ws, slo, shi, n_slots, send, wcnt);
 }
 uint32_t sweep_waves() { return kSweepWaves; }
+
+// Packed (world, a, b) keys -> the canonical (world << 44 | a << 22 | b).
+__global__ void k_expand_keys(uint64_t* keys, uint32_t n, uint32_t id_bits) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i], m = (1ull << id_bits) - 1ull;
+  keys[i] = makeKey((uint32_t)(k >> (2 * id_bits)), (uint32_t)((k >> id_bits) & m), (uint32_t)(k & m));
+}
+void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits) {
+  if (n) k_expand_keys<<<cdiv(n, 256), 256, 0, s>>>(keys, n, id_bits);
+}
 void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
-                        uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count) {
+                        uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
+                        uint32_t id_bits) {
   constexpr uint32_t blocks = kSweepWaves / 4;
   if (count)
     k_sweep_pairs<true><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount, woff_w,
-                                               out);
+                                               out, id_bits);
   else
     k_sweep_pairs<false><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount,
-                                                woff_w, out);
+                                                woff_w, out, id_bits);
 }
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,

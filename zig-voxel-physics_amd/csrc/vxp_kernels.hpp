@@ -13,13 +13,19 @@ void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const
                      const WorldState* ws, uint32_t n_slots);
 void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw);
+// Per-world block sort of the sweep order (k_world_axis included); false when
+// max_bodies is above the largest block sort and the device-wide sort is needed.
+bool launch_sweep_sort(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       uint32_t* order, uint32_t nw);
 void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
                         const uint32_t* order, uint32_t n_slots, float4* slo, float4* shi,
                         uint32_t* send, uint32_t* wcnt);
 uint32_t sweep_waves();
 void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
-                        uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count);
+                        uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
+                        uint32_t id_bits);
+void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits);
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
