@@ -1653,6 +1653,92 @@ __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, V
   }
 }
 
+// The partner lane of a lane pair (2k, 2k + 1): DPP quad_perm(1, 0, 3, 2).
+__device__ __forceinline__ float pairSwap(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ V3 pairSwap(V3 v) { return V3{pairSwap(v.x), pairSwap(v.y), pairSwap(v.z)}; }
+
+// solveManifoldVel on a lane pair: lane `side` 0 holds body A, side 1 body B.
+// Each lane evaluates the contact velocity of its own body and applies its own
+// body's impulses; the relative velocity is assembled from the partner's value
+// and every shared scalar (vn, impulse, clamp) is computed identically in both
+// lanes, so the result equals the one-lane pass bit for bit.  Both lanes of a
+// pair follow the same control flow (same manifold, count and friction).
+template <bool kBias, class VelPtr>
+__device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, VelPtr VA,
+                                                  const SoaRec& S, uint32_t j, uint32_t side) {
+  const float4 r0 = S.r(j, 0), r3 = S.r(j, 3);
+  const uint32_t slot = __float_as_uint(side ? r0.y : r0.x);
+  const uint32_t n = __float_as_uint(r0.w);
+  const float friction = r0.z;
+  const float4 rp = S.r(j, side ? 2 : 1);  // position, inverse mass
+  const V3 pos = f4v(rp);
+  const float im = rp.w;
+  const bool aw = __float_as_uint(side ? r3.y : r3.x) != 0;
+  const float4 m4 = S.r(j, 4), m5 = S.r(j, 5), m6 = S.r(j, 6), m7 = S.r(j, 7), m8 = S.r(j, 8);
+  const M3 ii = side ? M3{{m6.y, m6.z, m6.w, m7.x, m7.y, m7.z, m7.w, m8.x, m8.y}}
+                     : M3{{m4.x, m4.y, m4.z, m4.w, m5.x, m5.y, m5.z, m5.w, m6.x}};
+  V3 lv = f4v(VL[slot]), av = f4v(VA[slot]);
+  for (uint32_t k = 0; k < n; k++) {
+    float4 q0 = S.p(j, k, 0), q1 = S.p(j, k, 1), q2 = S.p(j, k, 2), q3 = S.p(j, k, 3),
+           q4 = S.p(j, k, 4), q5 = S.p(j, k, 5), q6 = S.p(j, k, 6);
+    V3 nrm = f4v(q0), pt = f4v(q1), t1 = f4v(q4), t2 = f4v(q5);
+    V3 r = side ? f4v(q3) : f4v(q2);
+    float nm = q0.w, tm0 = q1.w, tm1 = q2.w, bias = q3.w;
+    float ni = q4.w, ti0 = q5.w, ti1 = q6.w;
+    const float sgn = side ? 1.0f : -1.0f;  // A receives -P, B receives +P (exact negation)
+    V3 mine = velAt(lv, av, pos, pt);
+    V3 other = pairSwap(mine);
+    V3 rel = side ? sub(mine, other) : sub(other, mine);
+    float vn = dot(rel, nrm);
+    float imp;
+    if (kBias) imp = -cfg.mass_coeff * nm * (vn + bias) - cfg.impulse_coeff * ni;
+    else imp = -nm * vn;
+    float nw = fmaxf(ni + imp, 0.0f);
+    imp = nw - ni;
+    ni = nw;
+    V3 Pn = scale(nrm, imp);
+    impulseTo(aw, lv, av, im, ii, side ? Pn : neg(Pn), r);
+    if (cfg.enable_friction && friction > 0.0f) {
+      mine = velAt(lv, av, pos, pt);
+      other = pairSwap(mine);
+      V3 rel1 = side ? sub(mine, other) : sub(other, mine);
+      float maxf = friction * ni;
+      float vt = dot(rel1, t1);
+      float fi = -tm0 * vt;
+      float old = ti0;
+      ti0 = clampz(old + fi, -maxf, maxf);
+      fi = ti0 - old;
+      V3 i1 = scale(t1, fi);
+      impulseTo(aw, lv, av, im, ii, side ? i1 : neg(i1), r);
+      mine = velAt(lv, av, pos, pt);
+      other = pairSwap(mine);
+      V3 rel2 = side ? sub(mine, other) : sub(other, mine);
+      float vt2 = dot(rel2, t2);
+      float fi2 = -tm1 * vt2;
+      float old2 = ti1;
+      ti1 = clampz(old2 + fi2, -maxf, maxf);
+      fi2 = ti1 - old2;
+      V3 i2 = scale(t2, fi2);
+      impulseTo(aw, lv, av, im, ii, side ? i2 : neg(i2), r);
+      if (side == 0) {
+        S.p(j, k, 5).w = ti0;
+        S.p(j, k, 6).w = ti1;
+      }
+    }
+    if (side == 0) S.p(j, k, 4).w = ni;
+    (void)sgn;
+  }
+  if (aw) {
+    VL[slot] = v4(lv);
+    VA[slot] = v4(av);
+  }
+}
+
+// Colours with at most this many manifolds run on lane pairs.
+constexpr uint32_t kPairColorMax = 128;
+
 // warmStart for one manifold (src/solver.zig:702-729)
 template <class VelPtr>
 __device__ __forceinline__ void warmManifold(float wss, VelPtr VL, VelPtr VA, const SoaRec& S,
@@ -2013,8 +2099,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       // solveVelocityConstraints with bias
       for (uint32_t it = 0; it < cfg.vel_iters; it++) {
         for (uint32_t c = 0; c < ncol; c++) {
-          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
-            solveManifoldVel<true>(cfg, VL, VA, SR, j);
+          const uint32_t c0 = ccount[c], cn = ccount[c + 1] - c0;
+          if (cn <= kPairColorMax) {
+            if (tid < 2 * cn) solveManifoldVel2<true>(cfg, VL, VA, SR, c0 + (tid >> 1), tid & 1u);
+          } else {
+            for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
+              solveManifoldVel<true>(cfg, VL, VA, SR, j);
+          }
           __syncthreads();
         }
       }
@@ -2051,8 +2142,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       }
       __syncthreads();
       for (uint32_t c = 0; c < ncol; c++) {
-        for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
-          solveManifoldVel<false>(cfg, VL, VA, SR, j);
+        const uint32_t c0 = ccount[c], cn = ccount[c + 1] - c0;
+        if (cn <= kPairColorMax) {
+          if (tid < 2 * cn) solveManifoldVel2<false>(cfg, VL, VA, SR, c0 + (tid >> 1), tid & 1u);
+        } else {
+          for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
+            solveManifoldVel<false>(cfg, VL, VA, SR, j);
+        }
         __syncthreads();
       }
     }
