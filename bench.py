@@ -80,7 +80,8 @@ def run(sys_, sc, warmup, steps, profile=False):
         lib.profile_enable(sys_.ptr, 0)
     wall = sum(frame_ms) / 1e3
     return dict(value=body_substeps / wall, wall=wall, frame_ms=frame_ms, kern=kern,
-                alg_solver_bytes=alg, body_substeps=body_substeps, candidates=cands)
+                alg_solver_bytes=alg, body_substeps=body_substeps, candidates=cands,
+                tuning=sys_.tuning())
 
 
 def pct(xs, q):
@@ -242,6 +243,7 @@ def main():
             "contact_candidates_per_step": rb["candidates"] / args.batched_steps,
             "p50_step_ms": statistics.median(rb["frame_ms"]),
             "p99_step_ms": pct(rb["frame_ms"], 0.99),
+            "tuning": rb["tuning"],
             "plain_looped_value": loop_bs / loop_t if loop_t else None,
             "plain_looped_sample": f"{nl} of the {len(sb.worlds)} worlds stepped one system at a "
                                    f"time, frames {args.warmup}..{args.warmup + lf - 1}",
@@ -331,6 +333,7 @@ def main():
                      "avg_launch_ms": avg_solve_s * 1e3, "launches": solve_n,
                      "alg_bytes_per_launch": alg_per_launch, "dominant_kernel": dom},
         "kernel_ms": {k: round(v[0], 3) for k, v in kern.items()},
+        "tuning": plain["tuning"],
         "final_stats": {k: final[k] for k in ("manifolds", "colors", "sleeping_bodies",
                                               "voxels_checked", "broad_phase_pairs")},
         "batched": batched,

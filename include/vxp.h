@@ -310,6 +310,21 @@ int vxp_server_manifolds_get(vxp_server* srv, uint32_t world, vxp_manifold* out,
                              uint32_t capacity, uint32_t* count);
 int vxp_server_info_get(vxp_server* srv, vxp_server_info* out);
 
+/* Launch-shape tuning.  Every choice computes bit-identical results; only the
+ * speed differs.  An online tuner times the candidates on live frames (a
+ * window of 15 frames at creation, again every 600 frames) and keeps the
+ * fastest.  pair_color_max: colours of at most this many manifolds are solved
+ * on lane pairs (one body per lane) instead of one lane per manifold. */
+typedef struct vxp_tuning {
+  uint32_t pair_color_max;
+  uint32_t windows;  /* tuning windows completed */
+  uint32_t fixed;    /* set by vxp_tuning_set: the tuner is off */
+  uint32_t pad;
+} vxp_tuning;
+int vxp_tuning_get(vxp_system* sys, vxp_tuning* out);
+/* Fix the tuning (the tuner stops); pair_color_max = 0xFFFFFFFF restarts it. */
+int vxp_tuning_set(vxp_system* sys, const vxp_tuning* tuning);
+
 /* Last error message of this thread (never NULL). */
 const char* vxp_last_error(void);
 

@@ -142,3 +142,33 @@ def test_empty_collider_body_in_scene(product, oracle):
             assert np.array_equal(a.pairs(), b.pairs()), fr
             assert compare.diff_manifolds(a.manifolds(), b.manifolds()) is None, fr
     assert a.getPosition(ha)[1] < 5.0
+
+
+def test_tuning_choices_are_bit_identical(product):
+    # The lane-pair colour limit changes only the launch shape of the solver:
+    # forced extremes (every colour on single lanes / every colour of up to
+    # 1024 manifolds on lane pairs) and the online tuner give the same bits.
+    sc = scenes.batched_worlds(n_worlds=6, lo=8, hi=1024, blob_pool=8)
+    runs = []
+    for forced in (0, 1024, None):
+        s = vxp.from_scene(sc, lib=product)
+        if forced is not None:
+            s.setTuning(forced)
+        print(f"tuning forced={forced}", flush=True)
+        for _ in range(40):
+            s.update(sc.dt)
+        t = s.tuning()
+        assert t["fixed"] == (forced is not None)
+        if forced is not None:
+            assert t["pair_color_max"] == forced and t["windows"] == 0
+        else:
+            assert t["windows"] >= 1
+        runs.append(([s.bodies(w) for w in range(len(sc.worlds))],
+                     [s.manifolds(w) for w in range(len(sc.worlds))],
+                     [s.stats(w) for w in range(len(sc.worlds))]))
+        s.close()
+    for other in runs[1:]:
+        for w in range(len(sc.worlds)):
+            assert compare.diff_bodies(other[0][w], runs[0][0][w]) is None, w
+            assert compare.diff_manifolds(other[1][w], runs[0][1][w]) is None, w
+            assert compare.diff_stats(other[2][w], runs[0][2][w]) is None, w

@@ -347,6 +347,59 @@ int readSizes(vxp_system* s, std::initializer_list<const uint32_t*> src, uint32_
   return VXP_OK;
 }
 
+// Folds in the solver time of the previous tuned frame (its events completed
+// before this step's first size read), then chooses this frame's candidate.
+int tuneBegin(vxp_system* s, DevCfg& cfg) {
+  auto& t = s->tune;
+  if (t.ms.size() != t.cand.size()) {
+    t.ms.assign(t.cand.size(), 0.0);
+    t.n.assign(t.cand.size(), 0);
+  }
+  if (t.pending) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventSynchronize(t.b));
+    HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+    t.ms[t.pending_idx] += ms;
+    t.n[t.pending_idx]++;
+    t.pending = false;
+    if (t.window_left == 0) {  // window over: keep the fastest
+      double best = 1e300;
+      for (size_t i = 0; i < t.cand.size(); i++)
+        if (t.n[i] && t.ms[i] / t.n[i] < best) {
+          best = t.ms[i] / t.n[i];
+          t.pick = t.cand[i];
+        }
+      t.windows++;
+      t.next_window = t.frame + 600;
+    }
+  }
+  cfg.pair_color_max = t.pick;
+  if (t.fixed) return VXP_OK;
+  if (t.window_left == 0 && t.frame >= t.next_window) {
+    t.window_left = 3 * (uint32_t)t.cand.size();
+    std::fill(t.ms.begin(), t.ms.end(), 0.0);
+    std::fill(t.n.begin(), t.n.end(), 0u);
+  }
+  if (t.window_left) {
+    if (!t.a) {
+      HIP_TRY(hipEventCreate(&t.a));
+      HIP_TRY(hipEventCreate(&t.b));
+    }
+    t.pending_idx = t.window_left % (uint32_t)t.cand.size();
+    cfg.pair_color_max = t.cand[t.pending_idx];
+    HIP_TRY(hipEventRecord(t.a, s->stream));
+    t.pending = true;
+    t.window_left--;
+  }
+  return VXP_OK;
+}
+int tuneEnd(vxp_system* s) {
+  auto& t = s->tune;
+  if (t.pending) HIP_TRY(hipEventRecord(t.b, s->stream));
+  t.frame++;
+  return VXP_OK;
+}
+
 int stepAll(vxp_system* s) {
   hipStream_t st = s->stream;
   RC(ensureDevice(s));
@@ -583,6 +636,7 @@ int stepAll(vxp_system* s) {
   S.pts = s->s_pts.as<float4>();
   S.tstamp = s->prof ? s->s_tstamp.as<uint64_t>() : nullptr;
   pe = profBegin(s);
+  RC(tuneBegin(s, cfg));
   launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw, s->lds_limit);
   if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
     (void)hipGetLastError();
@@ -590,6 +644,7 @@ int stepAll(vxp_system* s) {
     launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw, 0);
   }
   profEnd(s, VXP_K_SOLVE, pe);
+  RC(tuneEnd(s));
   HIP_TRY(hipGetLastError());
   s->dev_newer = true;
   return VXP_OK;
@@ -651,6 +706,26 @@ int vxp_profile_solver_phases(vxp_system* s, uint32_t w, uint64_t* ts) {
   return VXP_OK;
 }
 
+int vxp_tuning_get(vxp_system* s, vxp_tuning* out) {
+  if (!s || !out) return VXP_ERR_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  out->pair_color_max = s->tune.pick;
+  out->windows = s->tune.windows;
+  out->fixed = s->tune.fixed;
+  return VXP_OK;
+}
+int vxp_tuning_set(vxp_system* s, const vxp_tuning* t) {
+  if (!s || !t) return VXP_ERR_ARGUMENT;
+  if (t->pair_color_max == 0xFFFFFFFFu) {
+    s->tune.fixed = false;
+    s->tune.next_window = s->tune.frame;
+  } else {
+    s->tune.fixed = true;
+    s->tune.pick = t->pair_color_max;
+    s->tune.window_left = 0;
+  }
+  return VXP_OK;
+}
 int vxp_profile_get(vxp_system* s, uint32_t id, double* total_ms, uint64_t* launches) {
   if (!s || id >= VXP_K_COUNT || !total_ms || !launches) return VXP_ERR_ARGUMENT;
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -812,6 +887,8 @@ int vxp_system_destroy(vxp_system* s) {
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
   if (s->h_read) (void)hipHostFree(s->h_read);
+  if (s->tune.a) (void)hipEventDestroy(s->tune.a);
+  if (s->tune.b) (void)hipEventDestroy(s->tune.b);
   (void)hipStreamDestroy(s->stream);
   delete s;
   return VXP_OK;

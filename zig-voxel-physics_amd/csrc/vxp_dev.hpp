@@ -44,6 +44,7 @@ struct DevCfg {
   uint8_t enable_sleeping, warm_starting, enable_friction, enable_restitution;
   uint8_t disable_rotation, use_sat, use_cache, enable_islands;
   uint8_t store_impulses, pad[3];
+  uint32_t pair_color_max;  // colours of at most this many manifolds run on lane pairs
 };
 
 // Body state, global slot g = world * max_bodies + slot.

@@ -137,6 +137,18 @@ struct vxp_system {
       s_pts, s_prio, s_tstamp;
   uint32_t n_pairs = 0, n_manifolds = 0;
   uint32_t* h_read = nullptr;  // pinned host words for the per-step size reads
+  // Online tuning of launch shapes that do not change results: during a window
+  // of frames the candidates take turns (one per frame, timed with events
+  // around the solver); the fastest on average is kept until the next window.
+  struct Tuner {
+    std::vector<uint32_t> cand{0, 64, 128, 256, 512};  // lane-pair colour limits
+    std::vector<double> ms;
+    std::vector<uint32_t> n;
+    uint32_t pick = 128, frame = 0, window_left = 0, next_window = 0, windows = 0;
+    bool fixed = false, pending = false;
+    uint32_t pending_idx = 0;
+    hipEvent_t a = nullptr, b = nullptr;
+  } tune;
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   // kernel timing (HIP events on `stream`)
   bool prof = false;

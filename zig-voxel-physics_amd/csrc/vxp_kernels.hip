@@ -1736,9 +1736,6 @@ __device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, 
   }
 }
 
-// Colours with at most this many manifolds run on lane pairs.
-constexpr uint32_t kPairColorMax = 128;
-
 // warmStart for one manifold (src/solver.zig:702-729)
 template <class VelPtr>
 __device__ __forceinline__ void warmManifold(float wss, VelPtr VL, VelPtr VA, const SoaRec& S,
@@ -2100,7 +2097,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       for (uint32_t it = 0; it < cfg.vel_iters; it++) {
         for (uint32_t c = 0; c < ncol; c++) {
           const uint32_t c0 = ccount[c], cn = ccount[c + 1] - c0;
-          if (cn <= kPairColorMax) {
+          if (cn <= cfg.pair_color_max && 2 * cn <= bs) {
             if (tid < 2 * cn) solveManifoldVel2<true>(cfg, VL, VA, SR, c0 + (tid >> 1), tid & 1u);
           } else {
             for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
@@ -2143,7 +2140,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       __syncthreads();
       for (uint32_t c = 0; c < ncol; c++) {
         const uint32_t c0 = ccount[c], cn = ccount[c + 1] - c0;
-        if (cn <= kPairColorMax) {
+        if (cn <= cfg.pair_color_max && 2 * cn <= bs) {
           if (tid < 2 * cn) solveManifoldVel2<false>(cfg, VL, VA, SR, c0 + (tid >> 1), tid & 1u);
         } else {
           for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)

@@ -182,6 +182,13 @@ class ServerInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("pair_color_max", C.c_uint32), ("windows", C.c_uint32), ("fixed", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+TUNE_AUTO = 0xFFFFFFFF
+
 assert C.sizeof(Config) == 100, C.sizeof(Config)
 assert C.sizeof(Contact) == 88, C.sizeof(Contact)
 assert C.sizeof(Manifold) == 376, C.sizeof(Manifold)
@@ -229,6 +236,8 @@ PRODUCT_ONLY = {
     "profile_enable": (C.c_int, [_P, C.c_int]),
     "profile_get": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "profile_solver_phases": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "tuning_get": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "tuning_set": (C.c_int, [_P, C.POINTER(Tuning)]),
     # serving (continuous batching over resident worlds)
     "server_create": (C.c_int, [C.POINTER(Config), C.c_uint32, C.c_uint32, C.c_uint64,
                                 C.POINTER(_P)]),

@@ -57,6 +57,17 @@ class VoxelPhysicsSystem:
         except Exception:
             pass
 
+    # ---- launch tuning (product only; results never depend on it) -------------
+    def tuning(self):
+        t = _abi.Tuning()
+        self.lib.check(self.lib.tuning_get(self.ptr, C.byref(t)), "tuning_get")
+        return {"pair_color_max": t.pair_color_max, "windows": t.windows, "fixed": bool(t.fixed)}
+
+    def setTuning(self, pair_color_max=None):
+        """Fix the solver's lane-pair colour limit; None restarts the online tuner."""
+        t = _abi.Tuning(pair_color_max=_abi.TUNE_AUTO if pair_color_max is None else pair_color_max)
+        self.lib.check(self.lib.tuning_set(self.ptr, C.byref(t)), "tuning_set")
+
     # ---- colliders / bodies (src/system.zig:294-433) -------------------------
     def addCollider(self, collider=None, voxel_size=None, brick_pos=None, voxels=None):
         if collider is not None:
