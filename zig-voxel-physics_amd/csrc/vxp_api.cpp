@@ -577,8 +577,9 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_total.ensure(64));
   launch_world_segments(st, ws, nw, s->d_total.as<uint32_t>());
   profEnd(s, VXP_K_COMPACT, pe);
-  uint32_t nman = 0;
-  RC(readSizes(s, {s->d_total.as<uint32_t>()}, &nman));
+  uint32_t sizes[2] = {0, 0};
+  RC(readSizes(s, {s->d_total.as<uint32_t>(), s->d_total.as<uint32_t>() + 1}, sizes));
+  const uint32_t nman = sizes[0], max_man = sizes[1];
   s->n_manifolds = nman;
   uint32_t nm1 = std::max(nman, 1u);
   HIP_TRY(s->d_ml[s->cur].ensure((size_t)nm1 * sizeof(DevManifold)));
@@ -621,6 +622,16 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->s_rec.ensure((size_t)nm1 * 9 * sizeof(float4)));
   HIP_TRY(s->s_pts.ensure((size_t)nm1 * 4 * 8 * sizeof(float4)));
   HIP_TRY(s->s_tstamp.ensure((size_t)nw * 16 * 8));
+  HIP_TRY(s->s_upar.ensure((size_t)ns * 4));
+  HIP_TRY(s->s_ucnt.ensure((size_t)ns * 4));
+  HIP_TRY(s->s_uoff.ensure((size_t)ns * 4));
+  HIP_TRY(s->s_ubody.ensure((size_t)ns * 4));
+  HIP_TRY(s->s_uorder.ensure((size_t)nm1 * 4));
+  HIP_TRY(s->s_units.ensure((size_t)nw * kUnitsPerWorld * sizeof(uint4)));
+  HIP_TRY(s->s_uqueue.ensure((size_t)nw * kUnitsPerWorld * 4));
+  HIP_TRY(s->s_pcnt.ensure((size_t)nm1 * 4));
+  HIP_TRY(s->s_fq.ensure((size_t)nm1 * 4));
+  HIP_TRY(s->s_cend.ensure((size_t)nm1 * 16));
   SolverScratch S;
   S.ws_pos = s->s_pos.as<float4>();
   S.ws_rot = s->s_rot.as<float4>();
@@ -635,14 +646,31 @@ int stepAll(vxp_system* s) {
   S.rec = s->s_rec.as<float4>();
   S.pts = s->s_pts.as<float4>();
   S.tstamp = s->prof ? s->s_tstamp.as<uint64_t>() : nullptr;
+  S.upar = s->s_upar.as<uint32_t>();
+  S.ucnt = s->s_ucnt.as<uint32_t>();
+  S.uoff = s->s_uoff.as<uint32_t>();
+  S.ubody = s->s_ubody.as<uint32_t>();
+  S.uorder = s->s_uorder.as<uint32_t>();
+  S.units = s->s_units.as<uint4>();
+  S.uqueue = s->s_uqueue.as<uint32_t>();
+  S.pcnt = s->s_pcnt.as<uint32_t>();
+  S.fq = s->s_fq.as<uint32_t>();
+  S.cend = s->s_cend.as<uint4>();
+  // solve units of at least this many manifolds: about two per CU over the
+  // whole batch (results do not depend on it)
+  cfg.unit_min = std::max(64u, nman / (2 * std::max(s->n_cu, 1u)));
+  uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
   pe = profBegin(s);
   RC(tuneBegin(s, cfg));
-  launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw, s->lds_limit);
+  launch_solve_prep(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), ctr, nw, max_man,
+                    s->lds_limit);
+  launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit);
   if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
     (void)hipGetLastError();
     s->lds_limit = 0;
-    launch_solve(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), nw, 0);
+    launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, 0);
   }
+  launch_solve_tail(st, cfg, B, ws, stats, S, ctr, nw, s->n_cu);
   profEnd(s, VXP_K_SOLVE, pe);
   RC(tuneEnd(s));
   HIP_TRY(hipGetLastError());
@@ -841,6 +869,9 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
         lds > 24 * 1024)
       s->lds_limit = (size_t)lds - 20 * 1024;  // the solver's static LDS
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
+      s->n_cu = (uint32_t)cu;
   }
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     g_err = "hipStreamCreate failed";
@@ -882,7 +913,9 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->d_total, &s->d_ml[0], &s->d_ml[1], &s->d_isl_parent, &s->d_isl_end,
                     &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors, &s->s_pos, &s->s_rot,
                     &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,
-                    &s->s_color, &s->s_order, &s->s_rec, &s->s_pts, &s->s_prio, &s->s_tstamp};
+                    &s->s_color, &s->s_order, &s->s_rec, &s->s_pts, &s->s_prio, &s->s_tstamp,
+                    &s->s_upar, &s->s_ucnt, &s->s_uoff, &s->s_ubody, &s->s_uorder, &s->s_units,
+                    &s->s_uqueue, &s->s_pcnt, &s->s_fq, &s->s_cend};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);

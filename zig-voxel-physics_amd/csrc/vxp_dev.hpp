@@ -14,6 +14,9 @@ constexpr int kWave = 64;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 // Device counters cleared at the start of every step (step_ctr[]).
 constexpr uint32_t kCtrReduceWork = 0;  // pairs on the reduction work list
+constexpr uint32_t kUnitsPerWorld = 64;  // solve units per world, at most
+constexpr uint32_t kCtrSolveUnits = 1;  // solve units queued by the solver's prep
+constexpr uint32_t kCtrSolveUnitPull = 2;  // solve units taken by workgroups
 constexpr uint32_t kStepCounters = 4;
 
 // Collider header.  The brick directory is a dense grid over brick positions
@@ -45,6 +48,7 @@ struct DevCfg {
   uint8_t disable_rotation, use_sat, use_cache, enable_islands;
   uint8_t store_impulses, pad[3];
   uint32_t pair_color_max;  // colours of at most this many manifolds run on lane pairs
+  uint32_t unit_min;        // solve units hold at least this many manifolds (when split)
 };
 
 // Body state, global slot g = world * max_bodies + slot.
@@ -77,6 +81,11 @@ struct WorldState {
   uint32_t frame;                     // BrickPairCache.current_frame
   uint32_t error;
   uint32_t axis;                      // sort-and-sweep axis (longest extent)
+  // solver (written by its prep each step)
+  uint32_t n_units, nsolve;           // solve units, solved manifolds
+  uint32_t pos_run, pos_done;         // position passes run; early exit taken
+  uint32_t maxpen;                    // float bits of the maximum penetration
+  uint32_t pos_min[3];                // float bits of each pass's minimum separation
 };
 
 // Pair flags
@@ -160,6 +169,14 @@ struct SolverScratch {
   float4* rec;                               // 9 per manifold (colour order)
   float4* pts;                               // 4 x 8 per manifold (colour order)
   uint64_t* tstamp;                          // optional phase timestamps (16 per world)
+  // solve units: components of the graph of mobile bodies, cut into units
+  uint32_t *upar, *ucnt, *uoff;              // per body slot: parent, manifolds, offsets
+  uint32_t* ubody;                           // per body slot: mobile bodies by unit
+  uint32_t* uorder;                          // per manifold: solved manifolds by unit
+  uint4* units;                              // per world x 64: (man off, n, body off, n)
+  uint32_t* uqueue;                          // world * 64 + unit, in queue order
+  uint32_t *pcnt, *fq;                       // per manifold: uncoloured lower neighbours, queue
+  uint4* cend;                               // per manifold: endpoints' incidence ranges
 };
 
 }  // namespace vxd

@@ -134,7 +134,8 @@ struct vxp_system {
   int cur = 0;
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
   DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
-      s_pts, s_prio, s_tstamp;
+      s_pts, s_prio, s_tstamp, s_upar, s_ucnt, s_uoff, s_ubody, s_uorder, s_units, s_uqueue,
+      s_pcnt, s_fq, s_cend;
   uint32_t n_pairs = 0, n_manifolds = 0;
   uint32_t* h_read = nullptr;  // pinned host words for the per-step size reads
   // Online tuning of launch shapes that do not change results: during a window
@@ -150,6 +151,7 @@ struct vxp_system {
     hipEvent_t a = nullptr, b = nullptr;
   } tune;
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
+  uint32_t n_cu = 256;   // compute units of the device
   // kernel timing (HIP events on `stream`)
   bool prof = false;
   double prof_ms[VXP_K_COUNT] = {};

@@ -1312,10 +1312,10 @@ __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const P
 
 // New manifold segments: skipped worlds keep their previous segment.
 __global__ void k_world_segments(WorldState* ws, uint32_t n_worlds, uint32_t* total) {
-  // single block: exclusive scan of per-world counts
-  __shared__ uint32_t carry;
+  // single block: exclusive scan of per-world counts; total[1] = largest count
+  __shared__ uint32_t carry, largest;
   __shared__ uint32_t buf[1024];
-  if (threadIdx.x == 0) carry = 0;
+  if (threadIdx.x == 0) carry = largest = 0;
   __syncthreads();
   for (uint32_t base = 0; base < n_worlds; base += blockDim.x) {
     uint32_t w = base + threadIdx.x;
@@ -1332,12 +1332,16 @@ __global__ void k_world_segments(WorldState* ws, uint32_t n_worlds, uint32_t* to
     if (w < n_worlds) {
       ws[w].man_start = carry + buf[threadIdx.x] - c;
       ws[w].man_count = c;
+      if (c) atomicMax(&largest, c);
     }
     __syncthreads();
     if (threadIdx.x == blockDim.x - 1) carry += buf[threadIdx.x];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) {
+    total[0] = carry;
+    total[1] = largest;
+  }
 }
 
 __global__ void k_scatter(const WorldState* ws, const PairRec* pr, uint32_t n_pairs,
@@ -1596,14 +1600,24 @@ __device__ __forceinline__ void impulseTo(bool awake, V3& lv, V3& av, float im, 
 // solveVelocityConstraints for one manifold (src/solver.zig:732-787) with
 // solveFriction (src/solver.zig:790-838); kBias selects the soft biased pass
 // or the hard relax pass (src/solver.zig:848-856).
+// Every point record of the four contact slots is loaded before the first
+// constraint row (slots past the count hold stale data inside the world's
+// slice and are not used), so a pass waits for memory once per manifold.
 template <bool kBias, class VelPtr>
 __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, VelPtr VA,
                                                  const SoaRec& S, uint32_t j) {
   MRec m = loadMRec(S, j);
+  float4 Q[4][7];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int q = 0; q < 7; q++) Q[k][q] = S.p(j, k, q);
   V3 la = f4v(VL[m.sa]), aa = f4v(VA[m.sa]), lb = f4v(VL[m.sb]), ab = f4v(VA[m.sb]);
-  for (uint32_t k = 0; k < m.n; k++) {
-    float4 q0 = S.p(j, k, 0), q1 = S.p(j, k, 1), q2 = S.p(j, k, 2), q3 = S.p(j, k, 3),
-           q4 = S.p(j, k, 4), q5 = S.p(j, k, 5), q6 = S.p(j, k, 6);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) {
+    if (k >= m.n) break;
+    const float4 q0 = Q[k][0], q1 = Q[k][1], q2 = Q[k][2], q3 = Q[k][3], q4 = Q[k][4],
+                 q5 = Q[k][5], q6 = Q[k][6];
     V3 nrm = f4v(q0), pt = f4v(q1), ra = f4v(q2), rb = f4v(q3), t1 = f4v(q4), t2 = f4v(q5);
     float nm = q0.w, tm0 = q1.w, tm1 = q2.w, bias = q3.w;
     float ni = q4.w, ti0 = q5.w, ti1 = q6.w;
@@ -1653,6 +1667,16 @@ __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, V
   }
 }
 
+// impulseTo without a branch: the update is computed and kept only for an
+// awake body (the same arithmetic, so the same bits).
+__device__ __forceinline__ void impulseSel(bool awake, V3& lv, V3& av, float im, const M3& ii,
+                                           V3 imp, V3 r) {
+  V3 nl = add(lv, scale(imp, im));
+  V3 na = add(av, mulVec(ii, cross(r, imp)));
+  lv = V3{awake ? nl.x : lv.x, awake ? nl.y : lv.y, awake ? nl.z : lv.z};
+  av = V3{awake ? na.x : av.x, awake ? na.y : av.y, awake ? na.z : av.z};
+}
+
 // The partner lane of a lane pair (2k, 2k + 1): DPP quad_perm(1, 0, 3, 2).
 __device__ __forceinline__ float pairSwap(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
@@ -1679,10 +1703,17 @@ __device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, 
   const float4 m4 = S.r(j, 4), m5 = S.r(j, 5), m6 = S.r(j, 6), m7 = S.r(j, 7), m8 = S.r(j, 8);
   const M3 ii = side ? M3{{m6.y, m6.z, m6.w, m7.x, m7.y, m7.z, m7.w, m8.x, m8.y}}
                      : M3{{m4.x, m4.y, m4.z, m4.w, m5.x, m5.y, m5.z, m5.w, m6.x}};
+  float4 Q[4][7];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int q = 0; q < 7; q++) Q[k][q] = S.p(j, k, q);
   V3 lv = f4v(VL[slot]), av = f4v(VA[slot]);
-  for (uint32_t k = 0; k < n; k++) {
-    float4 q0 = S.p(j, k, 0), q1 = S.p(j, k, 1), q2 = S.p(j, k, 2), q3 = S.p(j, k, 3),
-           q4 = S.p(j, k, 4), q5 = S.p(j, k, 5), q6 = S.p(j, k, 6);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) {
+    if (k >= n) break;
+    const float4 q0 = Q[k][0], q1 = Q[k][1], q2 = Q[k][2], q3 = Q[k][3], q4 = Q[k][4],
+                 q5 = Q[k][5], q6 = Q[k][6];
     V3 nrm = f4v(q0), pt = f4v(q1), t1 = f4v(q4), t2 = f4v(q5);
     V3 r = side ? f4v(q3) : f4v(q2);
     float nm = q0.w, tm0 = q1.w, tm1 = q2.w, bias = q3.w;
@@ -1699,7 +1730,7 @@ __device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, 
     imp = nw - ni;
     ni = nw;
     V3 Pn = scale(nrm, imp);
-    impulseTo(aw, lv, av, im, ii, side ? Pn : neg(Pn), r);
+    impulseSel(aw, lv, av, im, ii, side ? Pn : neg(Pn), r);
     if (cfg.enable_friction && friction > 0.0f) {
       mine = velAt(lv, av, pos, pt);
       other = pairSwap(mine);
@@ -1711,7 +1742,7 @@ __device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, 
       ti0 = clampz(old + fi, -maxf, maxf);
       fi = ti0 - old;
       V3 i1 = scale(t1, fi);
-      impulseTo(aw, lv, av, im, ii, side ? i1 : neg(i1), r);
+      impulseSel(aw, lv, av, im, ii, side ? i1 : neg(i1), r);
       mine = velAt(lv, av, pos, pt);
       other = pairSwap(mine);
       V3 rel2 = side ? sub(mine, other) : sub(other, mine);
@@ -1721,13 +1752,11 @@ __device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, 
       ti1 = clampz(old2 + fi2, -maxf, maxf);
       fi2 = ti1 - old2;
       V3 i2 = scale(t2, fi2);
-      impulseTo(aw, lv, av, im, ii, side ? i2 : neg(i2), r);
-      if (side == 0) {
-        S.p(j, k, 5).w = ti0;
-        S.p(j, k, 6).w = ti1;
-      }
+      impulseSel(aw, lv, av, im, ii, side ? i2 : neg(i2), r);
+      S.p(j, k, 5).w = ti0;  // both lanes hold the same value
+      S.p(j, k, 6).w = ti1;
     }
-    if (side == 0) S.p(j, k, 4).w = ni;
+    S.p(j, k, 4).w = ni;
     (void)sgn;
   }
   if (aw) {
@@ -1787,50 +1816,244 @@ __device__ __forceinline__ void stamp(uint64_t* t, int i) {
   if (t && threadIdx.x == 0) t[blockIdx.x * 16 + i] = wall_clock64();
 }
 
-// ContactSolver.solve for one world per block (src/solver.zig:494-561).  With
-// kLds the body velocities of the world are staged in LDS for the whole solve.
-template <bool kLds>
-__global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B, WorldState* ws,
-                                                         vxp_stats* st, DevManifold* ml,
-                                                         SolverScratch S, uint32_t* prio_buf,
-                                                         uint64_t* tstamp) {
-  extern __shared__ float4 dyn_vel[];
-  uint32_t w = blockIdx.x;
+// ContactSolver.solve (src/solver.zig:494-561), split into solve units.
+//
+// A unit is a set of connected components of the graph whose nodes are the
+// bodies the step moves (dynamic awake and kinematic) and whose edges are the
+// solved manifolds between two such bodies.  Every body a unit writes belongs
+// to that unit only, and every other body its manifolds touch (static,
+// sleeping) is only read during the solve, so units are independent: solving
+// each one in its own workgroup, with the colour order restricted to its
+// manifolds, performs exactly the operations of the world-wide colour-ordered
+// pass on every body and manifold.  The only world-wide quantities are the
+// position-pass early exit (minimum separation over all manifolds, evaluated
+// between launches) and the maximum penetration (a maximum, combined with
+// atomics on the float bits of non-negative values).
+//
+//   k_solve_prep   block / world: body working state, colouring, units
+//   k_solve_units  persistent grid: one unit at a time, all substeps and the
+//                  first position pass, body velocities in LDS
+//   k_solve_pos    position passes 2.. (one launch per pass), early exit
+//   k_solve_final  block / world: writeBack, updateSleepStates, stats
+constexpr uint32_t kUnitsMax = kUnitsPerWorld;
+
+__device__ __forceinline__ bool mobileBody(const DevBodies& B, uint32_t g) {
+  return B.active[g] && B.type[g] != VXP_BODY_STATIC && !B.sleeping[g];
+}
+__device__ __forceinline__ uint32_t findRoot(uint32_t* P, uint32_t r) {
+  while (true) {
+    uint32_t t = __atomic_load_n(&P[r], __ATOMIC_RELAXED);
+    if (t == r) return r;
+    r = t;
+  }
+}
+
+// Greedy colouring of a world's solved manifolds in ascending (prio, key)
+// order, run as a dataflow: a manifold takes the smallest colour unused by its
+// lower neighbours once they are all coloured.  Each manifold counts its lower
+// neighbours; rounds colour the ready ones and release their higher
+// neighbours into the next round (one queue; each manifold enters once).
+// Same colours as the sequential greedy.  A world's manifolds are stored in
+// ascending key order, so the index breaks prio ties exactly as the key does.
+// With kL the per-manifold state and the incidence lists live in LDS
+// (kColourLdsPerManifold bytes per manifold).
+constexpr uint32_t kColourLdsPerManifold = 40;
+__device__ __forceinline__ bool colourBefore(uint32_t pn, uint32_t n, uint32_t pm, uint32_t m) {
+  return pn < pm || (pn == pm && n < m);
+}
+template <bool kL>
+__device__ void colourManifolds(uint32_t M, uint32_t nadj, uint32_t m0, uint32_t g0,
+                                const DevBodies& B, const DevManifold* ml, const SolverScratch& S,
+                                const uint32_t* prio_g, const uint32_t* adj_g, uint4* kdyn,
+                                WorldState& x, uint32_t& s_tail, uint32_t& s_ncol) {
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
+  uint4* ce;          // per manifold: (offset, count) of each endpoint's incidence list
+  uint32_t* cp;       // prio
+  int32_t* cc;        // colour (-1 uncoloured, -2 not solved)
+  uint32_t* cn;       // uncoloured lower neighbours
+  uint32_t* cq;       // queue
+  const uint32_t* cl; // incidence lists (manifold indices)
+  if (kL) {
+    ce = kdyn;
+    cp = (uint32_t*)(kdyn + M);
+    cc = (int32_t*)(cp + M);
+    cn = (uint32_t*)(cc + M);
+    cq = cn + M;
+    uint32_t* l = cq + M;
+    for (uint32_t k = tid; k < nadj; k += bs) l[k] = adj_g[k];
+    cl = l;
+  } else {
+    ce = S.cend + m0;
+    cp = (uint32_t*)prio_g + m0;
+    cc = S.color + m0;
+    cn = S.pcnt + m0;
+    cq = S.fq + m0;
+    cl = adj_g;
+  }
+  for (uint32_t i = tid; i < M; i += bs) {
+    const DevManifold& d = ml[m0 + i];
+    uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+    bool la = B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga];
+    bool lb = B.type[gb] == VXP_BODY_DYNAMIC && !B.sleeping[gb];
+    ce[i] = make_uint4(la ? S.adj_off[ga] : 0u, la ? S.adj_cnt[ga] : 0u, lb ? S.adj_off[gb] : 0u,
+                       lb ? S.adj_cnt[gb] : 0u);
+    if (kL) {
+      cp[i] = prio_g[m0 + i];
+      cc[i] = S.color[m0 + i];
+    }
+  }
+  if (tid == 0) s_tail = 0;
+  __syncthreads();
+  // Neighbours are visited eight at a time so that each level of the
+  // dependent loads (incidence entry, then its prio, then its colour or count)
+  // is issued together; a padding entry names the manifold itself.
+  constexpr uint32_t kB = 8;
+  for (uint32_t i = tid; i < M; i += bs) {
+    if (cc[i] != -1) continue;
+    const uint4 e = ce[i];
+    const uint32_t pm = cp[i];
+    uint32_t lower = 0;
+    for (int side = 0; side < 2; side++) {
+      const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
+      for (uint32_t k0 = 0; k0 < c; k0 += kB) {
+        uint32_t nn[kB], pp[kB];
+#pragma unroll
+        for (uint32_t t = 0; t < kB; t++) nn[t] = k0 + t < c ? cl[o + k0 + t] : i;
+#pragma unroll
+        for (uint32_t t = 0; t < kB; t++) pp[t] = cp[nn[t]];
+#pragma unroll
+        for (uint32_t t = 0; t < kB; t++) lower += colourBefore(pp[t], nn[t], pm, i) ? 1u : 0u;
+      }
+    }
+    cn[i] = lower;
+    if (!lower) cq[atomicAdd(&s_tail, 1u)] = i;
+  }
+  __syncthreads();
+  stamp(S.tstamp, 7);
+  uint32_t head = 0, tail = s_tail;
+  __syncthreads();
+  while (head < tail) {
+    for (uint32_t q = head + tid; q < tail; q += bs) {
+      const uint32_t i = cq[q];
+      const uint4 e = ce[i];
+      const uint32_t pm = cp[i];
+      uint64_t used0 = 0;  // colours below 64 taken by lower neighbours
+      for (int side = 0; side < 2; side++) {
+        const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
+        for (uint32_t k0 = 0; k0 < c; k0 += kB) {
+          uint32_t nn[kB], pp[kB];
+          int32_t col8[kB];
+#pragma unroll
+          for (uint32_t t = 0; t < kB; t++) nn[t] = k0 + t < c ? cl[o + k0 + t] : i;
+#pragma unroll
+          for (uint32_t t = 0; t < kB; t++) {
+            pp[t] = cp[nn[t]];
+            col8[t] = __atomic_load_n(&cc[nn[t]], __ATOMIC_RELAXED);
+          }
+#pragma unroll
+          for (uint32_t t = 0; t < kB; t++) {
+            const int32_t cv = colourBefore(pp[t], nn[t], pm, i) ? col8[t] : -1;
+            used0 |= (cv >= 0 && cv < 64) ? 1ull << cv : 0ull;
+          }
+        }
+      }
+      uint32_t col;
+      if (~used0) {
+        col = (uint32_t)__builtin_ctzll(~used0);
+      } else {  // all of 0..63 taken: the full mask
+        uint64_t used[kMaxColors / 64];
+        for (uint32_t u = 0; u < kMaxColors / 64; u++) used[u] = 0;
+        for (int side = 0; side < 2; side++) {
+          const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
+          for (uint32_t k = 0; k < c; k++) {
+            uint32_t n = cl[o + k];
+            if (n == i || !colourBefore(cp[n], n, pm, i)) continue;
+            int32_t cv = __atomic_load_n(&cc[n], __ATOMIC_RELAXED);
+            if (cv >= 0 && cv < (int32_t)kMaxColors) used[cv >> 6] |= 1ull << (cv & 63);
+          }
+        }
+        col = kMaxColors;
+        for (uint32_t u = 0; u < kMaxColors / 64; u++)
+          if (~used[u]) {
+            col = u * 64 + (uint32_t)__builtin_ctzll(~used[u]);
+            break;
+          }
+      }
+      if (col >= kMaxColors) {
+        col = kMaxColors - 1;
+        atomicOr(&x.error, 1u);
+      }
+      __atomic_store_n(&cc[i], (int32_t)col, __ATOMIC_RELAXED);
+      atomicMax(&s_ncol, col + 1);
+      for (int side = 0; side < 2; side++) {  // release the higher neighbours
+        const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
+        for (uint32_t k0 = 0; k0 < c; k0 += kB) {
+          uint32_t nn[kB], rel = 0;
+#pragma unroll
+          for (uint32_t t = 0; t < kB; t++) nn[t] = k0 + t < c ? cl[o + k0 + t] : i;
+#pragma unroll
+          for (uint32_t t = 0; t < kB; t++) {
+            // subtracting 0 from the manifold's own count (padding, lower
+            // neighbours) changes nothing
+            const bool hi = colourBefore(pm, i, cp[nn[t]], nn[t]);
+            const uint32_t left = atomicSub(&cn[nn[t]], hi ? 1u : 0u);
+            rel |= (hi && left == 1u) ? 1u << t : 0u;
+          }
+          if (rel) {
+            uint32_t at = atomicAdd(&s_tail, (uint32_t)__builtin_popcount(rel));
+#pragma unroll
+            for (uint32_t t = 0; t < kB; t++)
+              if (rel & (1u << t)) cq[at++] = nn[t];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    head = tail;
+    tail = s_tail;
+    __syncthreads();
+  }
+  if (kL)
+    for (uint32_t i = tid; i < M; i += bs) S.color[m0 + i] = cc[i];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBodies B,
+                                                              WorldState* ws, vxp_stats* st,
+                                                              DevManifold* ml, SolverScratch S,
+                                                              uint32_t* prio_buf,
+                                                              uint32_t* step_ctr,
+                                                              size_t lds_bytes) {
+  const uint32_t w = blockIdx.x;
   if (ws[w].skip) return;
   const uint32_t tid = threadIdx.x, bs = blockDim.x;
   const uint32_t MB = cfg.max_bodies, g0 = w * MB;
   const uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
   __shared__ uint32_t tmp[kSolveThreads];
-  __shared__ uint32_t ccount[kMaxColors + 1];
-  __shared__ uint32_t ccursor[kMaxColors];
-  __shared__ uint32_t s_left, s_ncol, s_nsolve, s_skipped, s_points;
-  __shared__ float s_red[kSolveThreads];
-  float4* VL;
-  float4* VA;
-  if (kLds) {
-    VL = dyn_vel;
-    VA = dyn_vel + MB;
-  } else {
-    VL = S.ws_lv + g0;
-    VA = S.ws_av + g0;
-  }
-  const SoaRec SR{S.rec + (size_t)m0 * kMrQ, S.pts + (size_t)m0 * 4 * kPtQ, M};
-  stamp(tstamp, 0);
+  __shared__ uint32_t s_left, s_ncol, s_nsolve, s_skipped, s_points, s_nunits, s_changed,
+      s_points_tot;
+  extern __shared__ uint4 kdyn[];
+  __shared__ uint32_t cnt_m[kUnitsMax], cnt_b[kUnitsMax], off_m[kUnitsMax], off_b[kUnitsMax];
+  __shared__ uint32_t uid[kUnitsMax];
+  stamp(S.tstamp, 0);
 
   // body working state (BodyState.fromBody, src/solver.zig:231-243)
+  uint32_t* P = S.upar + g0;
   for (uint32_t s = tid; s < MB; s += bs) {
     uint32_t g = g0 + s;
     S.adj_cnt[g] = 0;
+    P[s] = s;
+    S.ucnt[g] = 0;
     if (!B.active[g]) {
-      VL[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-      VA[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      S.ws_lv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+      S.ws_av[g] = make_float4(0.f, 0.f, 0.f, 0.f);
       continue;
     }
     float4 rot = B.rot[g];
     S.ws_pos[g] = B.pos[g];
     S.ws_rot[g] = rot;
-    VL[s] = B.lv[g];
-    VA[s] = B.av[g];
+    S.ws_lv[g] = B.lv[g];
+    S.ws_av[g] = B.av[g];
     M3 ii;
     if (B.inv_mass[g] == 0.0f) {
       for (int i = 0; i < 9; i++) ii.m[i] = 0.0f;
@@ -1844,6 +2067,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     s_skipped = 0;
     s_ncol = 0;
     s_points = 0;
+    s_nunits = 1;
   }
   __syncthreads();
   // solvable manifolds (src/system.zig:687-702) and awake-dynamic incidence counts
@@ -1868,7 +2092,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
   // CSR offsets (relative to 2 * m0)
   for (uint32_t s = tid; s < MB; s += bs) S.adj_off[g0 + s] = S.adj_cnt[g0 + s];
   __syncthreads();
-  blockScanExclusive(S.adj_off + g0, MB, tmp, nullptr);
+  blockScanExclusive(S.adj_off + g0, MB, tmp, &s_points_tot);
+  const uint32_t nadj = s_points_tot;
   for (uint32_t s = tid; s < MB; s += bs) S.adj_cnt[g0 + s] = 0;
   __syncthreads();
   uint32_t* adj = S.adj + 2 * (size_t)m0;
@@ -1882,358 +2107,590 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       adj[S.adj_off[gb] + atomicAdd(&S.adj_cnt[gb], 1u)] = i;
   }
   __syncthreads();
-  stamp(tstamp, 1);
-  // Jones-Plassmann rounds: a manifold takes the smallest colour unused by its
-  // lower-(prio, key) neighbours once they are all coloured (= greedy in
-  // ascending (prio, key) order).
-  for (int round = 0; round < 1000000; round++) {
-    if (tid == 0) s_left = 0;
+  stamp(S.tstamp, 1);
+  if ((size_t)M * kColourLdsPerManifold <= lds_bytes)
+    colourManifolds<true>(M, nadj, m0, g0, B, ml, S, prio_buf, adj, kdyn, ws[w], s_left, s_ncol);
+  else
+    colourManifolds<false>(M, nadj, m0, g0, B, ml, S, prio_buf, adj, kdyn, ws[w], s_left, s_ncol);
+  __syncthreads();
+  stamp(S.tstamp, 2);
+  const uint32_t nsolve = s_nsolve;
+  for (uint32_t i = tid; i < M; i += bs) {
+    int32_t c = S.color[m0 + i];
+    if (c >= 0) ml[m0 + i].m.color = (uint32_t)c;
+  }
+  // components over manifolds between two mobile bodies: hooking to the
+  // smaller root, then pointer jumping, until nothing changes
+  for (int round = 0; round < 100000; round++) {
+    if (tid == 0) s_changed = 0;
     __syncthreads();
     for (uint32_t i = tid; i < M; i += bs) {
-      if (__atomic_load_n(&S.color[m0 + i], __ATOMIC_RELAXED) != -1) continue;
+      if (S.color[m0 + i] < 0) continue;
       const DevManifold& d = ml[m0 + i];
-      uint32_t pm = prio_buf[m0 + i];
-      uint64_t km = d.key;
-      bool ready = true;
-      uint64_t used[kMaxColors / 64];
-      for (uint32_t q = 0; q < kMaxColors / 64; q++) used[q] = 0;
-      uint32_t ends[2] = {g0 + d.sa, g0 + d.sb};
-      for (int e = 0; e < 2 && ready; e++) {
-        uint32_t g = ends[e];
-        if (B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) continue;
-        uint32_t o = S.adj_off[g], c = S.adj_cnt[g];
-        for (uint32_t k = 0; k < c; k++) {
-          uint32_t n = adj[o + k];
-          if (n == i) continue;
-          uint32_t pn = prio_buf[m0 + n];
-          uint64_t kn = ml[m0 + n].key;
-          if (pn < pm || (pn == pm && kn < km)) {
-            int32_t cn = __atomic_load_n(&S.color[m0 + n], __ATOMIC_RELAXED);
-            if (cn < 0) {
-              ready = false;
-              break;
-            }
-            if (cn < (int32_t)kMaxColors) used[cn >> 6] |= 1ull << (cn & 63);
-          }
-        }
+      if (!mobileBody(B, g0 + d.sa) || !mobileBody(B, g0 + d.sb)) continue;
+      uint32_t ra = findRoot(P, d.sa), rb = findRoot(P, d.sb);
+      if (ra != rb) {
+        atomicMin(&P[max(ra, rb)], min(ra, rb));
+        s_changed = 1;
       }
-      if (!ready) {
-        s_left = 1;
-        continue;
-      }
-      uint32_t col = kMaxColors;
-      for (uint32_t q = 0; q < kMaxColors / 64; q++)
-        if (~used[q]) {
-          col = q * 64 + (uint32_t)__builtin_ctzll(~used[q]);
-          break;
-        }
-      if (col >= kMaxColors) {
-        col = kMaxColors - 1;
-        atomicOr(&ws[w].error, 1u);
-      }
-      __atomic_store_n(&S.color[m0 + i], (int32_t)col, __ATOMIC_RELAXED);
-      atomicMax(&s_ncol, col + 1);
     }
     __syncthreads();
-    if (!s_left) break;
+    for (uint32_t s = tid; s < MB; s += bs) P[s] = findRoot(P, s);
     __syncthreads();
+    const uint32_t more = s_changed;
+    __syncthreads();
+    if (!more) break;
   }
-  __syncthreads();
-  stamp(tstamp, 2);
-  const uint32_t ncol = s_ncol, nsolve = s_nsolve;
-  // group manifolds by colour (order inside a colour is free: no shared body)
-  for (uint32_t c = tid; c <= kMaxColors; c += bs) ccount[c] = 0;
-  __syncthreads();
+  // solved manifolds per component (indexed by the root slot), then units:
+  // components in root order, cut every T manifolds (a component goes to the
+  // unit where it starts)
   for (uint32_t i = tid; i < M; i += bs) {
-    int32_t c = S.color[m0 + i];
-    if (c >= 0) {
-      atomicAdd(&ccount[c], 1u);
-      ml[m0 + i].m.color = (uint32_t)c;
-    }
-  }
-  __syncthreads();
-  blockScanExclusive(ccount, ncol + 1, tmp, nullptr);
-  for (uint32_t c = tid; c < ncol; c += bs) ccursor[c] = 0;
-  __syncthreads();
-  for (uint32_t i = tid; i < M; i += bs) {
-    int32_t c = S.color[m0 + i];
-    if (c >= 0) S.order[m0 + ccount[c] + atomicAdd(&ccursor[c], 1u)] = i;
-  }
-  __syncthreads();
-  // ContactSolver.addManifold (src/solver.zig:449-491), records in colour order
-  for (uint32_t j = tid; j < nsolve; j += bs) {
-    uint32_t i = S.order[m0 + j];
+    if (S.color[m0 + i] < 0) continue;
     const DevManifold& d = ml[m0 + i];
-    uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
-    V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
-    Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
-    SR.r(j, 0) = make_float4(__uint_as_float(d.sa), __uint_as_float(d.sb), d.m.friction,
-                             __uint_as_float(d.m.contact_count));
-    SR.r(j, 3) = make_float4(__uint_as_float(B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga]),
-                       __uint_as_float(B.type[gb] == VXP_BODY_DYNAMIC && !B.sleeping[gb]),
-                       d.m.restitution, 0.0f);
-    for (uint32_t k = 0; k < d.m.contact_count; k++) {
-      const vxp_contact& c = d.m.contacts[k];
-      V3 la{c.local_anchor_a[0], c.local_anchor_a[1], c.local_anchor_a[2]};
-      V3 lb{c.local_anchor_b[0], c.local_anchor_b[1], c.local_anchor_b[2]};
-      V3 n{c.normal[0], c.normal[1], c.normal[2]};
-      V3 wa = add(qrotate(qa, la), pa);
-      V3 wb = add(qrotate(qb, lb), pb);
-      V3 ad = sub(wb, wa);
-      SR.p(j, k, 0) = f4(n, 0.0f);
-      SR.p(j, k, 1) = f4(V3{c.point[0], c.point[1], c.point[2]}, 0.0f);
-      SR.p(j, k, 2) = f4(sub(wa, pa), 0.0f);
-      SR.p(j, k, 3) = f4(sub(wb, pb), 0.0f);
-      SR.p(j, k, 4) = f4(V3{0.f, 0.f, 0.f}, c.normal_impulse);
-      SR.p(j, k, 5) = f4(V3{0.f, 0.f, 0.f}, c.tangent_impulse[0]);
-      SR.p(j, k, 6) = f4(la, c.tangent_impulse[1]);
-      SR.p(j, k, 7) = f4(lb, -c.penetration - dot(ad, n));
-    }
+    uint32_t sm = mobileBody(B, g0 + d.sa) ? d.sa : d.sb;
+    if (mobileBody(B, g0 + sm)) atomicAdd(&S.ucnt[g0 + P[sm]], 1u);
   }
   __syncthreads();
-  stamp(tstamp, 3);
-  const bool has_c = nsolve > 0;
-  const float h = cfg.h;
-  const V3 grav{cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]};
-  const float dt_inv = h > kEps ? 1.0f / h : 0.0f;
+  for (uint32_t s = tid; s < MB; s += bs) S.uoff[g0 + s] = S.ucnt[g0 + s];
+  __syncthreads();
+  blockScanExclusive(S.uoff + g0, MB, tmp, nullptr);
+  const uint32_t T = max(max(cfg.unit_min, 1u), (nsolve + kUnitsMax - 1) / kUnitsMax);
+  for (uint32_t u = tid; u < kUnitsMax; u += bs) cnt_m[u] = cnt_b[u] = uid[u] = 0;
+  __syncthreads();
+  for (uint32_t s = tid; s < MB; s += bs)
+    if (S.ucnt[g0 + s]) uid[S.uoff[g0 + s] / T] = 1;
+  __syncthreads();
+  if (tid == 0) {  // cut positions that start a component, numbered densely
+    uint32_t n = 0;
+    for (uint32_t u = 0; u < kUnitsMax; u++) {
+      uint32_t used = uid[u];
+      uid[u] = n;
+      n += used;
+    }
+    s_nunits = max(n, 1u);
+  }
+  __syncthreads();
+  const uint32_t nu = s_nunits;
+  // unit of each mobile body (bodies without solved manifolds: by slot) and
+  // of each solved manifold (one with no mobile body: unit 0)
+  for (uint32_t s = tid; s < MB; s += bs) {
+    uint32_t g = g0 + s;
+    if (!mobileBody(B, g)) continue;
+    uint32_t r = P[s];
+    uint32_t u = S.ucnt[g0 + r] ? uid[S.uoff[g0 + r] / T] : s % nu;
+    S.adj_cnt[g] = u;
+    atomicAdd(&cnt_b[u], 1u);
+  }
+  for (uint32_t i = tid; i < M; i += bs) {
+    if (S.color[m0 + i] < 0) continue;
+    const DevManifold& d = ml[m0 + i];
+    uint32_t sm = mobileBody(B, g0 + d.sa) ? d.sa : d.sb;
+    uint32_t u = mobileBody(B, g0 + sm) ? uid[S.uoff[g0 + P[sm]] / T] : 0;
+    prio_buf[m0 + i] = u;
+    atomicAdd(&cnt_m[u], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t am = 0, ab = 0;
+    for (uint32_t u = 0; u < nu; u++) {
+      off_m[u] = am;
+      off_b[u] = ab;
+      S.units[(size_t)w * kUnitsMax + u] = make_uint4(am, cnt_m[u], ab, cnt_b[u]);
+      am += cnt_m[u];
+      ab += cnt_b[u];
+    }
+    uint32_t base = atomicAdd(&step_ctr[kCtrSolveUnits], nu);
+    for (uint32_t u = 0; u < nu; u++) S.uqueue[base + u] = w * kUnitsMax + u;
+    WorldState& x = ws[w];
+    x.n_units = nu;
+    x.nsolve = nsolve;
+    x.pos_run = (nsolve > 0 && cfg.pos_iters > 0) ? 1u : 0u;
+    x.pos_done = 0;
+    x.maxpen = 0;
+    x.pos_min[0] = x.pos_min[1] = x.pos_min[2] = 0;
+    vxp_stats& sx = st[w];
+    sx.manifolds = M;
+    sx.solver_manifolds = nsolve;
+    sx.solver_points = s_points;
+    sx.manifolds_skipped_sleeping = s_skipped;
+    sx.colors = s_ncol;
+  }
+  __syncthreads();
+  for (uint32_t u = tid; u < nu; u += bs) cnt_m[u] = cnt_b[u] = 0;
+  __syncthreads();
+  for (uint32_t s = tid; s < MB; s += bs) {
+    uint32_t g = g0 + s;
+    if (!mobileBody(B, g)) continue;
+    uint32_t u = S.adj_cnt[g];
+    S.ubody[g0 + off_b[u] + atomicAdd(&cnt_b[u], 1u)] = s;
+  }
+  for (uint32_t i = tid; i < M; i += bs) {
+    if (S.color[m0 + i] < 0) continue;
+    uint32_t u = prio_buf[m0 + i];
+    S.uorder[m0 + off_m[u] + atomicAdd(&cnt_m[u], 1u)] = i;
+  }
+  stamp(S.tstamp, 3);
+}
 
-  for (uint32_t sub_i = 0; sub_i < cfg.substeps; sub_i++) {
-    // applyGravity (src/solver.zig:564-605)
-    for (uint32_t s = tid; s < MB; s += bs) {
-      uint32_t g = g0 + s;
-      if (!B.active[g] || B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) continue;
-      V3 lv = add(f4v(VL[s]), scale(grav, h));
-      V3 av = f4v(VA[s]);
-      float ld = B.ldamp[g], ad = B.adamp[g];
-      float ls = len(lv), as = len(av);
-      if (ls < 0.5f) ld += 5.0f * (1.0f - (ls / 0.5f));
-      if (as < 0.5f) ad += 5.0f * (1.0f - (as / 0.5f));
-      float ldec = fmaxf(0.0f, 1.0f - ld * h), adec = fmaxf(0.0f, 1.0f - ad * h);
-      lv = scale(lv, ldec);
-      av = scale(av, adec);
-      if (dot(lv, lv) < 0.001f * 0.001f) lv = V3{0.f, 0.f, 0.f};
-      if (dot(av, av) < 0.001f * 0.001f) av = V3{0.f, 0.f, 0.f};
-      VL[s] = v4(lv);
-      VA[s] = v4(av);
+// The unit's manifolds grouped by colour: S.order[m0 + mb + j] for j in
+// [ccount[c], ccount[c + 1]) has colour c.  Returns the unit's colour count.
+__device__ uint32_t unitColours(const SolverScratch& S, uint32_t m0, uint32_t mb, uint32_t mn,
+                                bool write_order, uint32_t* ccount, uint32_t* ccursor,
+                                uint32_t* tmp, uint32_t* s_ncol) {
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
+  for (uint32_t c = tid; c <= kMaxColors; c += bs) ccount[c] = 0;
+  if (tid == 0) *s_ncol = 0;
+  __syncthreads();
+  for (uint32_t k = tid; k < mn; k += bs) {
+    uint32_t c = (uint32_t)S.color[m0 + S.uorder[m0 + mb + k]];
+    atomicAdd(&ccount[c], 1u);
+    atomicMax(s_ncol, c + 1);
+  }
+  __syncthreads();
+  const uint32_t ncol = *s_ncol;
+  blockScanExclusive(ccount, ncol + 1, tmp, nullptr);
+  if (write_order) {
+    for (uint32_t c = tid; c < ncol; c += bs) ccursor[c] = 0;
+    __syncthreads();
+    for (uint32_t k = tid; k < mn; k += bs) {
+      uint32_t i = S.uorder[m0 + mb + k];
+      uint32_t c = (uint32_t)S.color[m0 + i];
+      S.order[m0 + mb + ccount[c] + atomicAdd(&ccursor[c], 1u)] = i;
     }
     __syncthreads();
-    if (has_c) {
-      // prepareConstraints (src/solver.zig:608-699); also refreshes the
-      // per-manifold copies of body position, mass and world inertia
-      for (uint32_t j = tid; j < nsolve; j += bs) {
-        float4 r0 = SR.r(j, 0), r3 = SR.r(j, 3);
-        uint32_t sa = __float_as_uint(r0.x), sb = __float_as_uint(r0.y), n = __float_as_uint(r0.w);
-        float restitution = r3.z;
-        uint32_t ga = g0 + sa, gb = g0 + sb;
-        V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
-        Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
-        float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
-        M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
-        SR.r(j, 1) = f4(pa, ima);
-        SR.r(j, 2) = f4(pb, imb);
-        SR.r(j, 4) = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
-        SR.r(j, 5) = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
-        SR.r(j, 6) = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
-        SR.r(j, 7) = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
-        SR.r(j, 8) = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
-        V3 lva = f4v(VL[sa]), ava = f4v(VA[sa]), lvb = f4v(VL[sb]), avb = f4v(VA[sb]);
-        for (uint32_t k = 0; k < n; k++) {
-          float4 q0 = SR.p(j, k, 0), q6 = SR.p(j, k, 6), q7 = SR.p(j, k, 7);
-          V3 nrm = f4v(q0);
-          V3 wa = add(qrotate(qa, f4v(q6)), pa);
-          V3 wb = add(qrotate(qb, f4v(q7)), pb);
-          V3 ra = sub(wa, pa), rb = sub(wb, pb);
-          V3 pt = scale(add(wa, wb), 0.5f);
-          V3 ad = sub(wb, wa);
-          float sep = q7.w + dot(ad, nrm);
-          V3 rna = cross(ra, nrm), rnb = cross(rb, nrm);
-          float k_ = ima + imb;
-          k_ += dot(rna, mulVec(iia, rna));
-          k_ += dot(rnb, mulVec(iib, rnb));
-          float nm = k_ > kEps ? 1.0f / k_ : 0.0f;
-          V3 va = add(lva, cross(ava, sub(pt, pa)));
-          V3 vb = add(lvb, cross(avb, sub(pt, pb)));
-          V3 rel = sub(vb, va);
-          float vn = dot(rel, nrm);
-          float bias;
-          if (sep > 0.0f) bias = sep * dt_inv;
-          else bias = fmaxf(cfg.bias_rate * sep, -0.2f);
-          if (cfg.enable_restitution && vn < -1.0f) bias = fmaxf(bias, -restitution * vn);
-          float tm0 = SR.p(j, k, 1).w, tm1 = SR.p(j, k, 2).w;
-          float4 q4 = SR.p(j, k, 4), q5 = SR.p(j, k, 5);
-          V3 t1 = f4v(q4), t2 = f4v(q5);
-          if (cfg.enable_friction) {
-            t1 = computeOrthogonal(nrm);
-            t2 = cross(nrm, t1);
-            V3 a1 = cross(ra, t1), b1 = cross(rb, t1);
-            float k1 = ima + imb;
-            k1 += dot(a1, mulVec(iia, a1));
-            k1 += dot(b1, mulVec(iib, b1));
-            tm0 = k1 > kEps ? 1.0f / k1 : 0.0f;
-            V3 a2 = cross(ra, t2), b2 = cross(rb, t2);
-            float k2 = ima + imb;
-            k2 += dot(a2, mulVec(iia, a2));
-            k2 += dot(b2, mulVec(iib, b2));
-            tm1 = k2 > kEps ? 1.0f / k2 : 0.0f;
-          }
-          SR.p(j, k, 0) = f4(nrm, nm);
-          SR.p(j, k, 1) = f4(pt, tm0);
-          SR.p(j, k, 2) = f4(ra, tm1);
-          SR.p(j, k, 3) = f4(rb, bias);
-          SR.p(j, k, 4) = f4(t1, q4.w);
-          SR.p(j, k, 5) = f4(t2, q5.w);
+  }
+  return ncol;
+}
+
+// solvePositionConstraints over one unit's manifolds (src/solver.zig:860-923);
+// returns the unit's minimum separation (block-reduced) and raises maxpen.
+__device__ float unitPositionPass(const DevCfg& cfg, const DevBodies& B, const SolverScratch& S,
+                                  const SoaRec& SR, uint32_t g0, uint32_t mb, uint32_t ncol,
+                                  const uint32_t* ccount, float* s_red, float& maxpen) {
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
+  float minsep = 0.0f;
+  for (uint32_t c = 0; c < ncol; c++) {
+    for (uint32_t j = mb + ccount[c] + tid; j < mb + ccount[c + 1]; j += bs) {
+      float4 r0 = SR.r(j, 0);
+      uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
+      uint32_t n = __float_as_uint(r0.w);
+      uint8_t ta = B.type[ga], tb = B.type[gb];
+      if (ta == VXP_BODY_STATIC && tb == VXP_BODY_STATIC) continue;
+      float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
+      bool awa = ta == VXP_BODY_DYNAMIC && !B.sleeping[ga];
+      bool awb = tb == VXP_BODY_DYNAMIC && !B.sleeping[gb];
+      M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
+      V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
+      Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+      for (uint32_t k = 0; k < n; k++) {
+        V3 nrm = f4v(SR.p(j, k, 0));
+        float4 q6 = SR.p(j, k, 6), q7 = SR.p(j, k, 7);
+        V3 wa = add(qrotate(qa, f4v(q6)), pa);
+        V3 wb = add(qrotate(qb, f4v(q7)), pb);
+        V3 ad = sub(wb, wa);
+        float sep = q7.w + dot(ad, nrm);
+        minsep = fminf(minsep, sep);
+        maxpen = fmaxf(maxpen, -sep);
+        if (sep >= -cfg.slop) continue;
+        float corr = fmaxf(cfg.pcr * (sep + cfg.slop), -cfg.mpc);
+        V3 ra = sub(wa, pa), rb = sub(wb, pb);
+        V3 rna = cross(ra, nrm), rnb = cross(rb, nrm);
+        float k_ = ima + imb;
+        k_ += dot(rna, mulVec(iia, rna));
+        k_ += dot(rnb, mulVec(iib, rnb));
+        if (k_ < kEps) continue;
+        V3 Pi = scale(nrm, -corr / k_);
+        if (awa) {
+          pa = sub(pa, scale(Pi, ima));
+          qa = integrateRotationSmall(qa, mulVec(iia, cross(ra, neg(Pi))));
+        }
+        if (awb) {
+          pb = add(pb, scale(Pi, imb));
+          qb = integrateRotationSmall(qb, mulVec(iib, cross(rb, Pi)));
         }
       }
-      __syncthreads();
-      // warmStart (src/solver.zig:702-729), substep 0 only
-      if (cfg.warm_starting && sub_i == 0) {
-        for (uint32_t c = 0; c < ncol; c++) {
-          for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs)
-            warmManifold(cfg.wss, VL, VA, SR, j);
-          __syncthreads();
-        }
+      if (awa) {
+        S.ws_pos[ga] = v4(pa);
+        S.ws_rot[ga] = q4(qa);
       }
-      // solveVelocityConstraints with bias
-      for (uint32_t it = 0; it < cfg.vel_iters; it++) {
-        for (uint32_t c = 0; c < ncol; c++) {
-          const uint32_t c0 = ccount[c], cn = ccount[c + 1] - c0;
-          if (cn <= cfg.pair_color_max && 2 * cn <= bs) {
-            if (tid < 2 * cn) solveManifoldVel2<true>(cfg, VL, VA, SR, c0 + (tid >> 1), tid & 1u);
-          } else {
-            for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
-              solveManifoldVel<true>(cfg, VL, VA, SR, j);
-          }
-          __syncthreads();
-        }
+      if (awb) {
+        S.ws_pos[gb] = v4(pb);
+        S.ws_rot[gb] = q4(qb);
       }
-    }
-    // integratePositions (src/solver.zig:341-356,841-845)
-    for (uint32_t s = tid; s < MB; s += bs) {
-      uint32_t g = g0 + s;
-      if (!B.active[g] || B.type[g] == VXP_BODY_STATIC || B.sleeping[g]) continue;
-      V3 p = add(f4v(S.ws_pos[g]), scale(f4v(VL[s]), h));
-      S.ws_pos[g] = v4(p);
-      if (!cfg.disable_rotation) {
-        Q4 q = integrateRotation(f4q(S.ws_rot[g]), f4v(VA[s]), h);
-        S.ws_rot[g] = q4(q);
-      }
-      if (B.type[g] == VXP_BODY_DYNAMIC)
-        storeM3(S.ws_ii + (size_t)g * 9,
-                worldInvInertia(f4q(S.ws_rot[g]), loadM3(B.iil + (size_t)g * 9)));
     }
     __syncthreads();
-    // relaxConstraints (src/solver.zig:848-856): hard, unbiased; the bodies'
-    // positions and inertia moved in integrate, the point data did not
-    if (has_c) {
-      for (uint32_t j = tid; j < nsolve; j += bs) {
-        float4 r0 = SR.r(j, 0);
-        uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
-        M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
-        SR.r(j, 1) = f4(f4v(S.ws_pos[ga]), SR.r(j, 1).w);
-        SR.r(j, 2) = f4(f4v(S.ws_pos[gb]), SR.r(j, 2).w);
-        SR.r(j, 4) = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
-        SR.r(j, 5) = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
-        SR.r(j, 6) = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
-        SR.r(j, 7) = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
-        SR.r(j, 8) = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
-      }
-      __syncthreads();
-      for (uint32_t c = 0; c < ncol; c++) {
-        const uint32_t c0 = ccount[c], cn = ccount[c + 1] - c0;
-        if (cn <= cfg.pair_color_max && 2 * cn <= bs) {
-          if (tid < 2 * cn) solveManifoldVel2<false>(cfg, VL, VA, SR, c0 + (tid >> 1), tid & 1u);
-        } else {
-          for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
-            solveManifoldVel<false>(cfg, VL, VA, SR, j);
-        }
-        __syncthreads();
-      }
-    }
   }
-  stamp(tstamp, 4);
-  // solvePositionConstraints (src/solver.zig:860-923) with the early exit
-  float maxpen = 0.0f;
-  uint32_t pos_run = 0;
-  if (has_c) {
-    for (uint32_t it = 0; it < cfg.pos_iters; it++) {
-      pos_run++;
-      float minsep = 0.0f;
-      for (uint32_t c = 0; c < ncol; c++) {
-        for (uint32_t j = ccount[c] + tid; j < ccount[c + 1]; j += bs) {
-          float4 r0 = SR.r(j, 0);
-          uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
-          uint32_t n = __float_as_uint(r0.w);
-          uint8_t ta = B.type[ga], tb = B.type[gb];
-          if (ta == VXP_BODY_STATIC && tb == VXP_BODY_STATIC) continue;
-          float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
-          bool awa = ta == VXP_BODY_DYNAMIC && !B.sleeping[ga];
-          bool awb = tb == VXP_BODY_DYNAMIC && !B.sleeping[gb];
-          M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
-          V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
-          Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
-          for (uint32_t k = 0; k < n; k++) {
-            V3 nrm = f4v(SR.p(j, k, 0));
-            float4 q6 = SR.p(j, k, 6), q7 = SR.p(j, k, 7);
-            V3 wa = add(qrotate(qa, f4v(q6)), pa);
-            V3 wb = add(qrotate(qb, f4v(q7)), pb);
-            V3 ad = sub(wb, wa);
-            float sep = q7.w + dot(ad, nrm);
-            minsep = fminf(minsep, sep);
-            maxpen = fmaxf(maxpen, -sep);
-            if (sep >= -cfg.slop) continue;
-            float corr = fmaxf(cfg.pcr * (sep + cfg.slop), -cfg.mpc);
-            V3 ra = sub(wa, pa), rb = sub(wb, pb);
-            V3 rna = cross(ra, nrm), rnb = cross(rb, nrm);
-            float k_ = ima + imb;
-            k_ += dot(rna, mulVec(iia, rna));
-            k_ += dot(rnb, mulVec(iib, rnb));
-            if (k_ < kEps) continue;
-            V3 Pi = scale(nrm, -corr / k_);
-            if (awa) {
-              pa = sub(pa, scale(Pi, ima));
-              qa = integrateRotationSmall(qa, mulVec(iia, cross(ra, neg(Pi))));
-            }
-            if (awb) {
-              pb = add(pb, scale(Pi, imb));
-              qb = integrateRotationSmall(qb, mulVec(iib, cross(rb, Pi)));
-            }
-          }
-          if (awa) {
-            S.ws_pos[ga] = v4(pa);
-            S.ws_rot[ga] = q4(qa);
-          }
-          if (awb) {
-            S.ws_pos[gb] = v4(pb);
-            S.ws_rot[gb] = q4(qb);
-          }
-        }
-        __syncthreads();
-      }
-      s_red[tid] = minsep;
-      __syncthreads();
-      for (uint32_t d = bs / 2; d > 0; d >>= 1) {
-        if (tid < d) s_red[tid] = fminf(s_red[tid], s_red[tid + d]);
-        __syncthreads();
-      }
-      bool done = s_red[0] >= -3.0f * cfg.slop;
-      __syncthreads();
-      if (done) break;
-    }
+  s_red[tid] = minsep;
+  __syncthreads();
+  for (uint32_t d = bs / 2; d > 0; d >>= 1) {
+    if (tid < d) s_red[tid] = fminf(s_red[tid], s_red[tid + d]);
+    __syncthreads();
   }
+  float r = s_red[0];
+  __syncthreads();
+  return r;
+}
+
+// Combine a unit's position-pass results into its world: the minimum
+// separation of pass `slot` (float bits of values <= 0 order as unsigned
+// integers in reverse) and the maximum penetration (bits of values >= +0).
+__device__ void unitPositionCombine(WorldState& x, uint32_t slot, float minsep, float maxpen,
+                                    float* s_red) {
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
   s_red[tid] = maxpen;
   __syncthreads();
   for (uint32_t d = bs / 2; d > 0; d >>= 1) {
     if (tid < d) s_red[tid] = fmaxf(s_red[tid], s_red[tid + d]);
     __syncthreads();
   }
-  stamp(tstamp, 5);
-  // writeBackBodies (src/solver.zig:246-296,926-930)
+  if (tid == 0) {
+    float mp = s_red[0];
+    if (mp == 0.0f) mp = 0.0f;
+    float ms = minsep;
+    if (ms == 0.0f) ms = 0.0f;
+    atomicMax(&x.pos_min[slot], __float_as_uint(ms));
+    atomicMax(&x.maxpen, __float_as_uint(mp));
+  }
+  __syncthreads();
+}
+
+constexpr uint32_t kUnitThreads = 512;  // 256 VGPRs: the passes keep a manifold's records in registers
+template <bool kLds>
+__global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBodies B,
+                                                              WorldState* ws, DevManifold* ml,
+                                                              SolverScratch S,
+                                                              uint32_t* step_ctr) {
+  extern __shared__ float4 dyn_vel[];
+  __shared__ uint32_t tmp[kUnitThreads];
+  __shared__ uint32_t ccount[kMaxColors + 1];
+  __shared__ uint32_t ccursor[kMaxColors];
+  __shared__ float s_red[kUnitThreads];
+  __shared__ uint32_t s_item, s_ncol;
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
+  const uint32_t total = step_ctr[kCtrSolveUnits];
+  const uint32_t MB = cfg.max_bodies;
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd(&step_ctr[kCtrSolveUnitPull], 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    __syncthreads();
+    if (item >= total) break;
+    const uint32_t q = S.uqueue[item];
+    const uint32_t w = q / kUnitsMax;
+    const uint32_t g0 = w * MB;
+    const uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
+    const uint4 ud = S.units[q];
+    const uint32_t mb = ud.x, mn = ud.y, bn = ud.w;
+    const uint32_t* ub = S.ubody + g0 + ud.z;
+    float4* VL;
+    float4* VA;
+    if (kLds) {
+      VL = dyn_vel;
+      VA = dyn_vel + MB;
+    } else {
+      VL = S.ws_lv + g0;
+      VA = S.ws_av + g0;
+    }
+    const SoaRec SR{S.rec + (size_t)m0 * kMrQ, S.pts + (size_t)m0 * 4 * kPtQ, M};
+    const uint32_t ncol = unitColours(S, m0, mb, mn, true, ccount, ccursor, tmp, &s_ncol);
+    // ContactSolver.addManifold (src/solver.zig:449-491), records in colour order
+    for (uint32_t j = mb + tid; j < mb + mn; j += bs) {
+      uint32_t i = S.order[m0 + j];
+      const DevManifold& d = ml[m0 + i];
+      uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
+      V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
+      Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+      SR.r(j, 0) = make_float4(__uint_as_float(d.sa), __uint_as_float(d.sb), d.m.friction,
+                               __uint_as_float(d.m.contact_count));
+      SR.r(j, 3) = make_float4(__uint_as_float(B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga]),
+                               __uint_as_float(B.type[gb] == VXP_BODY_DYNAMIC && !B.sleeping[gb]),
+                               d.m.restitution, 0.0f);
+      for (uint32_t k = 0; k < d.m.contact_count; k++) {
+        const vxp_contact& c = d.m.contacts[k];
+        V3 la{c.local_anchor_a[0], c.local_anchor_a[1], c.local_anchor_a[2]};
+        V3 lb{c.local_anchor_b[0], c.local_anchor_b[1], c.local_anchor_b[2]};
+        V3 n{c.normal[0], c.normal[1], c.normal[2]};
+        V3 wa = add(qrotate(qa, la), pa);
+        V3 wb = add(qrotate(qb, lb), pb);
+        V3 ad = sub(wb, wa);
+        SR.p(j, k, 0) = f4(n, 0.0f);
+        SR.p(j, k, 1) = f4(V3{c.point[0], c.point[1], c.point[2]}, 0.0f);
+        SR.p(j, k, 2) = f4(sub(wa, pa), 0.0f);
+        SR.p(j, k, 3) = f4(sub(wb, pb), 0.0f);
+        SR.p(j, k, 4) = f4(V3{0.f, 0.f, 0.f}, c.normal_impulse);
+        SR.p(j, k, 5) = f4(V3{0.f, 0.f, 0.f}, c.tangent_impulse[0]);
+        SR.p(j, k, 6) = f4(la, c.tangent_impulse[1]);
+        SR.p(j, k, 7) = f4(lb, -c.penetration - dot(ad, n));
+      }
+      if (kLds) {
+        VL[d.sa] = S.ws_lv[ga];
+        VA[d.sa] = S.ws_av[ga];
+        VL[d.sb] = S.ws_lv[gb];
+        VA[d.sb] = S.ws_av[gb];
+      }
+    }
+    if (kLds)
+      for (uint32_t k = tid; k < bn; k += bs) {
+        uint32_t s = ub[k];
+        VL[s] = S.ws_lv[g0 + s];
+        VA[s] = S.ws_av[g0 + s];
+      }
+    __syncthreads();
+    const float h = cfg.h;
+    const V3 grav{cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]};
+    const float dt_inv = h > kEps ? 1.0f / h : 0.0f;
+    for (uint32_t sub_i = 0; sub_i < cfg.substeps; sub_i++) {
+      // applyGravity (src/solver.zig:564-605)
+      for (uint32_t k = tid; k < bn; k += bs) {
+        uint32_t s = ub[k], g = g0 + s;
+        if (B.type[g] != VXP_BODY_DYNAMIC) continue;
+        V3 lv = add(f4v(VL[s]), scale(grav, h));
+        V3 av = f4v(VA[s]);
+        float ld = B.ldamp[g], ad = B.adamp[g];
+        float ls = len(lv), as = len(av);
+        if (ls < 0.5f) ld += 5.0f * (1.0f - (ls / 0.5f));
+        if (as < 0.5f) ad += 5.0f * (1.0f - (as / 0.5f));
+        float ldec = fmaxf(0.0f, 1.0f - ld * h), adec = fmaxf(0.0f, 1.0f - ad * h);
+        lv = scale(lv, ldec);
+        av = scale(av, adec);
+        if (dot(lv, lv) < 0.001f * 0.001f) lv = V3{0.f, 0.f, 0.f};
+        if (dot(av, av) < 0.001f * 0.001f) av = V3{0.f, 0.f, 0.f};
+        VL[s] = v4(lv);
+        VA[s] = v4(av);
+      }
+      __syncthreads();
+      if (mn) {
+        // prepareConstraints (src/solver.zig:608-699); also refreshes the
+        // per-manifold copies of body position, mass and world inertia
+        for (uint32_t j = mb + tid; j < mb + mn; j += bs) {
+          float4 r0 = SR.r(j, 0), r3 = SR.r(j, 3);
+          uint32_t sa = __float_as_uint(r0.x), sb = __float_as_uint(r0.y), n = __float_as_uint(r0.w);
+          float restitution = r3.z;
+          uint32_t ga = g0 + sa, gb = g0 + sb;
+          V3 pa = f4v(S.ws_pos[ga]), pb = f4v(S.ws_pos[gb]);
+          Q4 qa = f4q(S.ws_rot[ga]), qb = f4q(S.ws_rot[gb]);
+          float ima = B.inv_mass[ga], imb = B.inv_mass[gb];
+          M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
+          SR.r(j, 1) = f4(pa, ima);
+          SR.r(j, 2) = f4(pb, imb);
+          SR.r(j, 4) = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
+          SR.r(j, 5) = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
+          SR.r(j, 6) = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
+          SR.r(j, 7) = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
+          SR.r(j, 8) = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
+          V3 lva = f4v(VL[sa]), ava = f4v(VA[sa]), lvb = f4v(VL[sb]), avb = f4v(VA[sb]);
+          for (uint32_t k = 0; k < n; k++) {
+            float4 q0 = SR.p(j, k, 0), q6 = SR.p(j, k, 6), q7 = SR.p(j, k, 7);
+            V3 nrm = f4v(q0);
+            V3 wa = add(qrotate(qa, f4v(q6)), pa);
+            V3 wb = add(qrotate(qb, f4v(q7)), pb);
+            V3 ra = sub(wa, pa), rb = sub(wb, pb);
+            V3 pt = scale(add(wa, wb), 0.5f);
+            V3 ad = sub(wb, wa);
+            float sep = q7.w + dot(ad, nrm);
+            V3 rna = cross(ra, nrm), rnb = cross(rb, nrm);
+            float k_ = ima + imb;
+            k_ += dot(rna, mulVec(iia, rna));
+            k_ += dot(rnb, mulVec(iib, rnb));
+            float nm = k_ > kEps ? 1.0f / k_ : 0.0f;
+            V3 va = add(lva, cross(ava, sub(pt, pa)));
+            V3 vb = add(lvb, cross(avb, sub(pt, pb)));
+            V3 rel = sub(vb, va);
+            float vn = dot(rel, nrm);
+            float bias;
+            if (sep > 0.0f) bias = sep * dt_inv;
+            else bias = fmaxf(cfg.bias_rate * sep, -0.2f);
+            if (cfg.enable_restitution && vn < -1.0f) bias = fmaxf(bias, -restitution * vn);
+            float tm0 = SR.p(j, k, 1).w, tm1 = SR.p(j, k, 2).w;
+            float4 q4 = SR.p(j, k, 4), q5 = SR.p(j, k, 5);
+            V3 t1 = f4v(q4), t2 = f4v(q5);
+            if (cfg.enable_friction) {
+              t1 = computeOrthogonal(nrm);
+              t2 = cross(nrm, t1);
+              V3 a1 = cross(ra, t1), b1 = cross(rb, t1);
+              float k1 = ima + imb;
+              k1 += dot(a1, mulVec(iia, a1));
+              k1 += dot(b1, mulVec(iib, b1));
+              tm0 = k1 > kEps ? 1.0f / k1 : 0.0f;
+              V3 a2 = cross(ra, t2), b2 = cross(rb, t2);
+              float k2 = ima + imb;
+              k2 += dot(a2, mulVec(iia, a2));
+              k2 += dot(b2, mulVec(iib, b2));
+              tm1 = k2 > kEps ? 1.0f / k2 : 0.0f;
+            }
+            SR.p(j, k, 0) = f4(nrm, nm);
+            SR.p(j, k, 1) = f4(pt, tm0);
+            SR.p(j, k, 2) = f4(ra, tm1);
+            SR.p(j, k, 3) = f4(rb, bias);
+            SR.p(j, k, 4) = f4(t1, q4.w);
+            SR.p(j, k, 5) = f4(t2, q5.w);
+          }
+        }
+        __syncthreads();
+        // warmStart (src/solver.zig:702-729), substep 0 only
+        if (cfg.warm_starting && sub_i == 0) {
+          for (uint32_t c = 0; c < ncol; c++) {
+            for (uint32_t j = mb + ccount[c] + tid; j < mb + ccount[c + 1]; j += bs)
+              warmManifold(cfg.wss, VL, VA, SR, j);
+            __syncthreads();
+          }
+        }
+        // solveVelocityConstraints with bias
+        for (uint32_t it = 0; it < cfg.vel_iters; it++) {
+          for (uint32_t c = 0; c < ncol; c++) {
+            const uint32_t c0 = mb + ccount[c], cn = ccount[c + 1] - ccount[c];
+            if (cn <= cfg.pair_color_max) {  // lane pairs, bs / 2 manifolds at a time
+              for (uint32_t t = tid >> 1; t < cn; t += bs >> 1)
+                solveManifoldVel2<true>(cfg, VL, VA, SR, c0 + t, tid & 1u);
+            } else {
+              for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
+                solveManifoldVel<true>(cfg, VL, VA, SR, j);
+            }
+            __syncthreads();
+          }
+        }
+      }
+      // integratePositions (src/solver.zig:341-356,841-845): the unit's
+      // mobile bodies (active, not static, awake)
+      for (uint32_t k = tid; k < bn; k += bs) {
+        uint32_t s = ub[k], g = g0 + s;
+        V3 p = add(f4v(S.ws_pos[g]), scale(f4v(VL[s]), h));
+        S.ws_pos[g] = v4(p);
+        if (!cfg.disable_rotation) {
+          Q4 qq = integrateRotation(f4q(S.ws_rot[g]), f4v(VA[s]), h);
+          S.ws_rot[g] = q4(qq);
+        }
+        if (B.type[g] == VXP_BODY_DYNAMIC)
+          storeM3(S.ws_ii + (size_t)g * 9,
+                  worldInvInertia(f4q(S.ws_rot[g]), loadM3(B.iil + (size_t)g * 9)));
+      }
+      __syncthreads();
+      // relaxConstraints (src/solver.zig:848-856): hard, unbiased; the bodies'
+      // positions and inertia moved in integrate, the point data did not
+      if (mn) {
+        for (uint32_t j = mb + tid; j < mb + mn; j += bs) {
+          float4 r0 = SR.r(j, 0);
+          uint32_t ga = g0 + __float_as_uint(r0.x), gb = g0 + __float_as_uint(r0.y);
+          M3 iia = loadM3(S.ws_ii + (size_t)ga * 9), iib = loadM3(S.ws_ii + (size_t)gb * 9);
+          SR.r(j, 1) = f4(f4v(S.ws_pos[ga]), SR.r(j, 1).w);
+          SR.r(j, 2) = f4(f4v(S.ws_pos[gb]), SR.r(j, 2).w);
+          SR.r(j, 4) = f4(iia.m[0], iia.m[1], iia.m[2], iia.m[3]);
+          SR.r(j, 5) = f4(iia.m[4], iia.m[5], iia.m[6], iia.m[7]);
+          SR.r(j, 6) = f4(iia.m[8], iib.m[0], iib.m[1], iib.m[2]);
+          SR.r(j, 7) = f4(iib.m[3], iib.m[4], iib.m[5], iib.m[6]);
+          SR.r(j, 8) = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
+        }
+        __syncthreads();
+        for (uint32_t c = 0; c < ncol; c++) {
+          const uint32_t c0 = mb + ccount[c], cn = ccount[c + 1] - ccount[c];
+          if (cn <= cfg.pair_color_max) {
+            for (uint32_t t = tid >> 1; t < cn; t += bs >> 1)
+              solveManifoldVel2<false>(cfg, VL, VA, SR, c0 + t, tid & 1u);
+          } else {
+            for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
+              solveManifoldVel<false>(cfg, VL, VA, SR, j);
+          }
+          __syncthreads();
+        }
+      }
+    }
+    // first position pass (src/solver.zig:860-923); later passes run in
+    // k_solve_pos once every unit of the world has finished this one
+    if (mn && cfg.pos_iters > 0) {
+      float maxpen = 0.0f;
+      float minsep = unitPositionPass(cfg, B, S, SR, g0, mb, ncol, ccount, s_red, maxpen);
+      unitPositionCombine(ws[w], 0, minsep, maxpen, s_red);
+    }
+    // storeImpulses (src/solver.zig:933-958)
+    if (cfg.store_impulses) {
+      for (uint32_t j = mb + tid; j < mb + mn; j += bs) {
+        uint32_t i = S.order[m0 + j];
+        DevManifold& d = ml[m0 + i];
+        for (uint32_t k = 0; k < d.m.contact_count; k++) {
+          vxp_contact& c = d.m.contacts[k];
+          c.normal_impulse = SR.p(j, k, 4).w;
+          c.tangent_impulse[0] = SR.p(j, k, 5).w;
+          c.tangent_impulse[1] = SR.p(j, k, 6).w;
+          c.normal_mass = SR.p(j, k, 0).w;
+          c.tangent_mass[0] = SR.p(j, k, 1).w;
+          c.tangent_mass[1] = SR.p(j, k, 2).w;
+        }
+      }
+    }
+    if (kLds)
+      for (uint32_t k = tid; k < bn; k += bs) {
+        uint32_t s = ub[k];
+        S.ws_lv[g0 + s] = VL[s];
+        S.ws_av[g0 + s] = VA[s];
+      }
+    __syncthreads();
+  }
+}
+
+// Position pass `it` >= 1 for every unit whose world has not exited early:
+// the world-wide minimum separation of pass it - 1 is complete when this
+// launch starts.  Three rotating slots: pass it reads slot (it - 1) % 3,
+// combines into slot it % 3 and clears slot (it + 1) % 3 for pass it + 1.
+__global__ __launch_bounds__(256) void k_solve_pos(DevCfg cfg, DevBodies B, WorldState* ws,
+                                                   SolverScratch S, const uint32_t* step_ctr,
+                                                   uint32_t it) {
+  __shared__ uint32_t tmp[256];
+  __shared__ uint32_t ccount[kMaxColors + 1];
+  __shared__ float s_red[256];
+  __shared__ uint32_t s_ncol, s_run;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t total = step_ctr[kCtrSolveUnits];
+  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+    const uint32_t q = S.uqueue[item];
+    const uint32_t w = q / kUnitsMax;
+    WorldState& x = ws[w];
+    // decided by one thread: pos_done may be set by another unit of the same
+    // world during this launch, and then the slot test gives the same answer
+    if (tid == 0) {
+      uint32_t run = 0;
+      if (x.nsolve != 0 && !x.pos_done) {
+        if (__uint_as_float(x.pos_min[(it - 1) % 3]) >= -3.0f * cfg.slop) {
+          x.pos_done = 1;
+        } else {
+          run = 1;
+          x.pos_run = it + 1;
+          x.pos_min[(it + 1) % 3] = 0;
+        }
+      }
+      s_run = run;
+    }
+    __syncthreads();
+    const uint32_t run = s_run;
+    __syncthreads();
+    if (!run) continue;
+    const uint4 ud = S.units[q];
+    const uint32_t mb = ud.x, mn = ud.y;
+    if (!mn) continue;
+    const uint32_t g0 = w * cfg.max_bodies, m0 = x.man_start;
+    const SoaRec SR{S.rec + (size_t)m0 * kMrQ, S.pts + (size_t)m0 * 4 * kPtQ, x.man_count};
+    const uint32_t ncol = unitColours(S, m0, mb, mn, false, ccount, nullptr, tmp, &s_ncol);
+    float maxpen = 0.0f;
+    float minsep = unitPositionPass(cfg, B, S, SR, g0, mb, ncol, ccount, s_red, maxpen);
+    unitPositionCombine(x, it % 3, minsep, maxpen, s_red);
+  }
+}
+
+// writeBackBodies (src/solver.zig:246-296,926-930), updateSleepStates
+// (src/system.zig:786-792, src/body.zig:415-427) and the solver stats.
+__global__ __launch_bounds__(kSolveThreads) void k_solve_final(DevCfg cfg, DevBodies B,
+                                                               WorldState* ws, vxp_stats* st,
+                                                               SolverScratch S) {
+  const uint32_t w = blockIdx.x;
+  if (ws[w].skip) return;
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
+  const uint32_t MB = cfg.max_bodies, g0 = w * MB;
+  stamp(S.tstamp, 4);
   for (uint32_t s = tid; s < MB; s += bs) {
     uint32_t g = g0 + s;
     if (!B.active[g] || B.type[g] == VXP_BODY_STATIC) continue;
     V3 p = f4v(S.ws_pos[g]);
     Q4 q = f4q(S.ws_rot[g]);
-    V3 lv = f4v(VL[s]), av = f4v(VA[s]);
+    V3 lv = f4v(S.ws_lv[g]), av = f4v(S.ws_av[g]);
     bool fq = isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w);
     if (!finite3(p) || !fq) {
       V3 op = f4v(B.pos[g]);
@@ -2255,24 +2712,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
     B.lv[g] = v4(lv);
     B.av[g] = v4(av);
   }
-  // storeImpulses (src/solver.zig:933-958)
-  if (has_c && cfg.store_impulses) {
-    for (uint32_t j = tid; j < nsolve; j += bs) {
-      uint32_t i = S.order[m0 + j];
-      DevManifold& d = ml[m0 + i];
-      for (uint32_t k = 0; k < d.m.contact_count; k++) {
-        vxp_contact& c = d.m.contacts[k];
-        c.normal_impulse = SR.p(j, k, 4).w;
-        c.tangent_impulse[0] = SR.p(j, k, 5).w;
-        c.tangent_impulse[1] = SR.p(j, k, 6).w;
-        c.normal_mass = SR.p(j, k, 0).w;
-        c.tangent_mass[0] = SR.p(j, k, 1).w;
-        c.tangent_mass[1] = SR.p(j, k, 2).w;
-      }
-    }
-  }
   __syncthreads();
-  // updateSleepStates (src/system.zig:786-792, src/body.zig:415-427)
+  stamp(S.tstamp, 5);
   if (cfg.enable_sleeping) {
     for (uint32_t s = tid; s < MB; s += bs) {
       uint32_t g = g0 + s;
@@ -2291,16 +2732,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve(DevCfg cfg, DevBodies B
       }
     }
   }
-  stamp(tstamp, 6);
+  stamp(S.tstamp, 6);
   if (tid == 0) {
-    vxp_stats& sx = st[w];
-    sx.max_penetration = s_red[0];
-    sx.position_iterations_run = pos_run;
-    sx.manifolds = M;
-    sx.solver_manifolds = nsolve;
-    sx.solver_points = s_points;
-    sx.manifolds_skipped_sleeping = s_skipped;
-    sx.colors = ncol;
+    st[w].max_penetration = __uint_as_float(ws[w].maxpen);
+    st[w].position_iterations_run = ws[w].pos_run;
   }
 }
 
@@ -2470,14 +2905,30 @@ void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
                          const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw) {
   k_ref_coloring<<<nw, 64, 0, s>>>(cfg, ws, ml, masks, colors);
 }
-void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
-                  vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
-                  uint32_t nw, size_t lds_limit) {
+void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
+                       uint32_t* step_ctr, uint32_t nw, uint32_t max_manifolds,
+                       size_t lds_limit) {
+  size_t lds = std::min(lds_limit, (size_t)max_manifolds * kColourLdsPerManifold);
+  k_solve_prep<<<nw, kSolveThreads, lds, s>>>(cfg, B, ws, st, ml, S, prio, step_ctr, lds);
+}
+void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                        DevManifold* ml, const SolverScratch& S, uint32_t* step_ctr, uint32_t nw,
+                        uint32_t n_cu, size_t lds_limit) {
   size_t vel_bytes = (size_t)cfg.max_bodies * 2 * sizeof(float4);
+  uint32_t grid = std::min<uint32_t>(nw * kUnitsMax, 2 * std::max(n_cu, 1u));
   if (vel_bytes <= lds_limit)
-    k_solve<true><<<nw, kSolveThreads, vel_bytes, s>>>(cfg, B, ws, st, ml, S, prio, S.tstamp);
+    k_solve_units<true><<<grid, kUnitThreads, vel_bytes, s>>>(cfg, B, ws, ml, S, step_ctr);
   else
-    k_solve<false><<<nw, kSolveThreads, 0, s>>>(cfg, B, ws, st, ml, S, prio, S.tstamp);
+    k_solve_units<false><<<grid, kUnitThreads, 0, s>>>(cfg, B, ws, ml, S, step_ctr);
+}
+void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       vxp_stats* st, const SolverScratch& S, const uint32_t* step_ctr,
+                       uint32_t nw, uint32_t n_cu) {
+  uint32_t grid = std::min<uint32_t>(nw * kUnitsMax, 4 * std::max(n_cu, 1u));
+  for (uint32_t it = 1; it < cfg.pos_iters; it++)
+    k_solve_pos<<<grid, 256, 0, s>>>(cfg, B, ws, S, step_ctr, it);
+  k_solve_final<<<nw, kSolveThreads, 0, s>>>(cfg, B, ws, st, S);
 }
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc) {

@@ -58,9 +58,19 @@ void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const 
                     uint32_t* size, uint32_t nw);
 void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
                          const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw);
-void launch_solve(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
-                  vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
-                  uint32_t nw, size_t lds_limit);
+// The solver in three steps: prep (block / world), units (persistent grid over
+// the solve units, body velocities in LDS when they fit in lds_limit), tail
+// (position passes 2.., then writeback / sleep / stats per world).
+void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
+                       uint32_t* step_ctr, uint32_t nw, uint32_t max_manifolds,
+                       size_t lds_limit);
+void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                        DevManifold* ml, const SolverScratch& S, uint32_t* step_ctr, uint32_t nw,
+                        uint32_t n_cu, size_t lds_limit);
+void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                       vxp_stats* st, const SolverScratch& S, const uint32_t* step_ctr,
+                       uint32_t nw, uint32_t n_cu);
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc);
 
