@@ -37,20 +37,24 @@ def log(msg):
 
 
 def solver_bytes(st, substeps, iters):
-    """Algorithmic HBM bytes of one solver launch for one world-step:
+    """Algorithmic HBM bytes of the solve-units kernel (k_solve_units) for one
+    world-step: the substeps, the warm start, the first position pass and the
+    impulse store of ContactSolver.solve,
     S*[N*208 + K*120 + M*184 + (I+1)*(K*112 + M*176)] + (K*72 + M*176)
-    + J*(K*40 + M*192) + N*144 + K*20, with N awake bodies, K contact points and
+    + min(J, 1)*(K*40 + M*192) + K*20, with N awake bodies, K contact points and
     M manifolds in the solve, S substeps, I velocity iterations, J executed
-    position iterations (FP32 SoA, each datum counted once per mandated pass)."""
+    position iterations (FP32 SoA, each datum counted once per mandated pass).
+    The rest of the solver's bytes, (J - 1)*(K*40 + M*192) + N*144, belong to
+    k_solve_pos and k_solve_final."""
     if not st.steps_executed:
         return 0
     N = st.awake_bodies_stepped // st.steps_executed
     K, M = st.solver_points, st.solver_manifolds
     S, I, J = substeps, iters, st.position_iterations_run
     if M == 0:
-        return S * N * 208 + N * 144
+        return S * N * 208
     return (S * (N * 208 + K * 120 + M * 184 + (I + 1) * (K * 112 + M * 176)) + (K * 72 + M * 176)
-            + J * (K * 40 + M * 192) + N * 144 + K * 20)
+            + min(J, 1) * (K * 40 + M * 192) + K * 20)
 
 
 def run(sys_, sc, warmup, steps, profile=False):
@@ -199,12 +203,12 @@ def main():
     s1.close()
     kern = plain["kern"]
     dom = max(kern, key=lambda k: kern[k][0])
-    solve_ms, solve_n = kern["solve"]
+    solve_ms, solve_n = kern["solve_units"]
     avg_solve_s = solve_ms / max(solve_n, 1) / 1e3
     alg_per_launch = plain["alg_solver_bytes"] / max(solve_n, 1)
     achieved = alg_per_launch / avg_solve_s / 1e9 if solve_n else 0.0
     traffic = None
-    tf = os.path.join(REPO, "profiles", "k_solve_c1_traffic.json")
+    tf = os.path.join(REPO, "profiles", "k_solve_units_c1_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
 
@@ -327,7 +331,7 @@ def main():
                                f"{sc.config['velocity_iterations']} iterations, frames "
                                f"{args.warmup}..{args.warmup + args.steps - 1}",
                    "worlds": 1},
-        "roofline": {"bound": "hbm", "kernel": "k_solve", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "k_solve_units", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "avg_launch_ms": avg_solve_s * 1e3, "launches": solve_n,

@@ -243,16 +243,21 @@ int vxp_coloring_get_w(vxp_system* sys, uint32_t world, uint8_t* colors, uint32_
 #define VXP_K_REDUCE 6
 #define VXP_K_COMPACT 7
 #define VXP_K_ISLANDS 8
-#define VXP_K_SOLVE 9
-#define VXP_K_COUNT 10
+#define VXP_K_SOLVE 9        /* solver prep: body state, colouring, solve units */
+#define VXP_K_SOLVE_UNITS 10 /* the solve units: substeps, first position pass */
+#define VXP_K_SOLVE_TAIL 11  /* further position passes, writeback, sleep */
+#define VXP_K_COUNT 12
 /* Enable (1) or disable (0) timing; enabling clears the accumulators. */
 int vxp_profile_enable(vxp_system* sys, int on);
 /* Accumulated device time (ms) and launch count of kernel `id`. */
 int vxp_profile_get(vxp_system* sys, uint32_t id, double* total_ms, uint64_t* launches);
 /* Solver phase timestamps of the last profiled step of world `w` (device
- * wall clock ticks): start, colouring start/end, records built, substeps
- * done, position passes done, sleep update done (ts[0..6]). */
-int vxp_profile_solver_phases(vxp_system* sys, uint32_t world, uint64_t ts[8]);
+ * wall clock ticks, 100 MHz): prep start, colouring start, colouring end,
+ * units formed, writeback start, writeback end, sleep update done, colouring
+ * rounds start (ts[0..7]); the world's largest solve unit: start, records
+ * built, substep 0 prepared, substep 0 velocity iterations done, substep 0
+ * relaxed, substeps done, first position pass done, end (ts[8..15]). */
+int vxp_profile_solver_phases(vxp_system* sys, uint32_t world, uint64_t ts[16]);
 
 /* ---- serving: continuous batching over resident worlds -------------------
  * No counterpart in the reference, which steps one world per update() call;

@@ -660,18 +660,22 @@ int stepAll(vxp_system* s) {
   // whole batch (results do not depend on it)
   cfg.unit_min = std::max(64u, nman / (2 * std::max(s->n_cu, 1u)));
   uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
-  pe = profBegin(s);
   RC(tuneBegin(s, cfg));
+  pe = profBegin(s);
   launch_solve_prep(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), ctr, nw, max_man,
                     s->lds_limit);
+  profEnd(s, VXP_K_SOLVE, pe);
+  pe = profBegin(s);
   launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit);
   if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
     (void)hipGetLastError();
     s->lds_limit = 0;
     launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, 0);
   }
+  profEnd(s, VXP_K_SOLVE_UNITS, pe);
+  pe = profBegin(s);
   launch_solve_tail(st, cfg, B, ws, stats, S, ctr, nw, s->n_cu);
-  profEnd(s, VXP_K_SOLVE, pe);
+  profEnd(s, VXP_K_SOLVE_TAIL, pe);
   RC(tuneEnd(s));
   HIP_TRY(hipGetLastError());
   s->dev_newer = true;
@@ -730,7 +734,7 @@ int vxp_profile_enable(vxp_system* s, int on) {
 int vxp_profile_solver_phases(vxp_system* s, uint32_t w, uint64_t* ts) {
   if (!s || !ts || w >= s->nw || !s->s_tstamp.p) return VXP_ERR_ARGUMENT;
   HIP_TRY(hipStreamSynchronize(s->stream));
-  HIP_TRY(hipMemcpy(ts, s->s_tstamp.as<uint64_t>() + (size_t)w * 16, 8 * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ts, s->s_tstamp.as<uint64_t>() + (size_t)w * 16, 16 * 8, hipMemcpyDeviceToHost));
   return VXP_OK;
 }
 

@@ -83,6 +83,7 @@ struct WorldState {
   uint32_t axis;                      // sort-and-sweep axis (longest extent)
   // solver (written by its prep each step)
   uint32_t n_units, nsolve;           // solve units, solved manifolds
+  uint32_t max_unit_m;                // manifolds of the largest unit
   uint32_t pos_run, pos_done;         // position passes run; early exit taken
   uint32_t maxpen;                    // float bits of the maximum penetration
   uint32_t pos_min[3];                // float bits of each pass's minimum separation

@@ -756,6 +756,20 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The fields of a cache entry that a lookup reads.
+struct CacheWin {
+  uint32_t a, b, valid;
+  uint64_t th, lu;
+};
+__device__ __forceinline__ CacheWin cacheWin(const CacheKeyRec& e) {
+  return CacheWin{e.a, e.b, e.valid, e.th, e.last_used};
+}
+// Entry of probe lane `lane` (< 16 <= cap) of the window that starts at `start`.
+__device__ __forceinline__ uint32_t cacheSlot(uint32_t start, uint32_t lane, uint32_t cap) {
+  uint32_t i = start + lane;
+  return i >= cap ? i - cap : i;
+}
+
 // Cache-eligible pairs in pair order: clist[cidx[p]] = p.
 __global__ void k_cache_scatter(const uint32_t* cflag, const uint32_t* cidx, uint32_t n_pairs,
                                 uint32_t* clist) {
@@ -780,53 +794,75 @@ __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws,
   if (!cfg.use_cache) return;
   const uint32_t cap = cfg.cache_cap, maxp = cap < 16 ? cap : 16;
   CacheKeyRec* G = keys + (size_t)w * cap;
-  for (uint32_t i = lane; i < cap; i += kWave) E[i] = G[i];
+  // stage the key table as 16-byte words (an entry is 40 bytes)
+  static_assert(sizeof(CacheKeyRec) % 8 == 0, "cache key record");
+  const uint32_t nq = cap * (uint32_t)sizeof(CacheKeyRec) / 8;
+  {
+    const uint2* g8 = reinterpret_cast<const uint2*>(G);
+    uint2* e8 = reinterpret_cast<uint2*>(E);
+    for (uint32_t i = lane; i < nq; i += kWave) e8[i] = g8[i];
+  }
   const uint32_t lo = ws[w].pair_lo, hi = ws[w].pair_hi;
   const uint32_t c0 = lo < hi ? cidx[lo] : 0, total = lo < hi ? cidx[hi] - c0 : 0;
   waveSync();
   uint32_t hits = 0, misses = 0;
+  const bool probe = lane < maxp;
+  // pair data of a batch of 64 pairs; the next batch's is loaded (and each
+  // lane hashes its own pair) while the current batch is walked
+  auto fetch = [&](uint32_t base, uint32_t& p, uint32_t& ida, uint32_t& idb, uint64_t& th) {
+    const bool inb = base + lane < total;
+    p = inb ? clist[c0 + base + lane] : 0;
+    ida = inb ? pr[p].ida : 0;
+    idb = inb ? pr[p].idb : 0;
+    th = inb ? pr[p].th : 0;
+  };
+  uint32_t np = 0, nida = 0, nidb = 0;
+  uint64_t nth = 0;
+  if (total) fetch(0, np, nida, nidb, nth);
   for (uint32_t base = 0; base < total; base += kWave) {
     const bool inb = base + lane < total;
-    const uint32_t p = inb ? clist[c0 + base + lane] : 0;
-    const uint32_t ida = inb ? pr[p].ida : 0, idb = inb ? pr[p].idb : 0;
-    const uint64_t th = inb ? pr[p].th : 0;
+    const uint32_t p = np, ida = nida, idb = nidb;
+    const uint64_t th = nth;
+    if (base + kWave < total) fetch(base + kWave, np, nida, nidb, nth);
+    const uint32_t my_start = (uint32_t)(cacheHash(ida, idb, th) % cap);
     uint32_t my_flag = PF_CACHE, my_aux = kNone;
     const uint32_t nb = min(total - base, (uint32_t)kWave);
+    // The window of the pair being resolved is held in registers (lane l <
+    // maxp: entry start + l).  The next pair's window is loaded before this
+    // pair's update is stored; LDS operations of a wave complete in order, so
+    // that load sees every earlier update and only this pair's is patched in.
+    uint32_t idx = cacheSlot(rdlane(my_start, 0), lane, cap);
+    CacheWin cur = probe ? cacheWin(E[idx]) : CacheWin{};
     for (uint32_t j = 0; j < nb; j++) {
       const uint32_t a = rdlane(ida, (int)j), b = rdlane(idb, (int)j);
       const uint64_t t = ((uint64_t)rdlane((uint32_t)(th >> 32), (int)j) << 32) |
                          rdlane((uint32_t)th, (int)j);
-      const uint64_t h = cacheHash(a, b, t);
-      const uint32_t start = (uint32_t)(h % cap);
-      const bool probe = lane < maxp;
-      const uint32_t idx = (start + lane) % cap;
-      bool valid = false, eq = false;
-      uint64_t lu = ~0ull;
-      if (probe) {
-        valid = E[idx].valid != 0;
-        eq = valid && E[idx].a == a && E[idx].b == b && E[idx].th == t;
-        lu = E[idx].last_used;
+      const bool more = j + 1 < nb;
+      uint32_t nidx = 0;
+      CacheWin nxt{};
+      if (more) {
+        nidx = cacheSlot(rdlane(my_start, (int)j + 1), lane, cap);
+        if (probe) nxt = cacheWin(E[nidx]);
       }
-      const uint64_t stop = __ballot(probe && (!valid || eq));
+      const bool eq = cur.valid && cur.a == a && cur.b == b && cur.th == t;
+      const uint64_t stop = __ballot(probe && (!cur.valid || eq));
       uint32_t rflag, raux;
       if (stop) {
         const int f = __ffsll((long long)stop) - 1;
-        const bool feq = rdlane((uint32_t)eq, f) != 0;
-        raux = (start + (uint32_t)f) % cap;
-        rflag = feq ? PF_HIT : PF_MISS;
+        raux = rdlane(idx, f);
+        rflag = rdlane((uint32_t)eq, f) ? PF_HIT : PF_MISS;
       } else {
         // window full: evict the first entry with the smallest last_used
-        uint64_t v = lu;
-        uint32_t l = lane;
-        for (int d = kWave / 2; d > 0; d >>= 1) {
-          const uint64_t ov = shfl64(v, (int)(lane ^ d));
-          const uint32_t ol = (uint32_t)__shfl((int)l, (int)(lane ^ d), kWave);
-          if (ov < v || (ov == v && ol < l)) {
-            v = ov;
-            l = ol;
+        // (last_used only ever holds a 32-bit frame number)
+        uint32_t bl = 0, bv = rdlane((uint32_t)cur.lu, 0);
+        for (uint32_t l = 1; l < maxp; l++) {
+          const uint32_t v = rdlane((uint32_t)cur.lu, (int)l);
+          if (v < bv) {
+            bv = v;
+            bl = l;
           }
         }
-        raux = (start + rdlane(l, 0)) % cap;
+        raux = rdlane(idx, (int)bl);
         rflag = PF_MISS;
       }
       if (lane == 0) {
@@ -845,11 +881,21 @@ __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws,
           misses++;
         }
       }
-      waveSync();
+      if (more && probe && nidx == raux) {
+        nxt.lu = frame;
+        if (rflag == PF_MISS) {
+          nxt.a = a;
+          nxt.b = b;
+          nxt.th = t;
+          nxt.valid = 1;
+        }
+      }
       if (lane == j) {
         my_flag = rflag;
         my_aux = raux;
       }
+      cur = nxt;
+      idx = nidx;
     }
     if (inb) {
       pr[p].flag = my_flag;
@@ -857,7 +903,11 @@ __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws,
     }
   }
   waveSync();
-  for (uint32_t i = lane; i < cap; i += kWave) G[i] = E[i];
+  {
+    const uint2* e8 = reinterpret_cast<const uint2*>(E);
+    uint2* g8 = reinterpret_cast<uint2*>(G);
+    for (uint32_t i = lane; i < nq; i += kWave) g8[i] = e8[i];
+  }
   if (lane == 0) {
     st[w].cache_hits += hits;
     st[w].cache_misses += misses;
@@ -1815,6 +1865,10 @@ __device__ __forceinline__ V3 computeOrthogonal(V3 v) {
 __device__ __forceinline__ void stamp(uint64_t* t, int i) {
   if (t && threadIdx.x == 0) t[blockIdx.x * 16 + i] = wall_clock64();
 }
+// Timestamp i of world w, written by a block that is not the world's own.
+__device__ __forceinline__ void stampW(uint64_t* t, uint32_t w, int i) {
+  if (t && threadIdx.x == 0) t[w * 16 + i] = wall_clock64();
+}
 
 // ContactSolver.solve (src/solver.zig:494-561), split into solve units.
 //
@@ -2038,12 +2092,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
   stamp(S.tstamp, 0);
 
   // body working state (BodyState.fromBody, src/solver.zig:231-243)
-  uint32_t* P = S.upar + g0;
   for (uint32_t s = tid; s < MB; s += bs) {
     uint32_t g = g0 + s;
     S.adj_cnt[g] = 0;
-    P[s] = s;
-    S.ucnt[g] = 0;
     if (!B.active[g]) {
       S.ws_lv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
       S.ws_av[g] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -2119,16 +2170,36 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
     int32_t c = S.color[m0 + i];
     if (c >= 0) ml[m0 + i].m.color = (uint32_t)c;
   }
-  // components over manifolds between two mobile bodies: hooking to the
-  // smaller root, then pointer jumping, until nothing changes
+  // Solve units.  Working arrays: UE per manifold (slot a, slot b, bits:
+  // 1 solved, 2 a mobile, 4 b mobile) and, per body slot, the union-find
+  // parent P, solved manifolds per component root UC and their prefix UO;
+  // in LDS (the colouring's space, free now) when they fit.
+  const bool ul = (size_t)M * 16 + (size_t)MB * 12 <= lds_bytes;
+  uint4* UE = ul ? kdyn : S.cend + m0;
+  uint32_t* P = ul ? (uint32_t*)(kdyn + M) : S.upar + g0;
+  uint32_t* UC = ul ? P + MB : S.ucnt + g0;
+  uint32_t* UO = ul ? UC + MB : S.uoff + g0;
+  __syncthreads();
+  for (uint32_t s = tid; s < MB; s += bs) {
+    P[s] = s;
+    UC[s] = 0;
+  }
+  for (uint32_t i = tid; i < M; i += bs) {
+    const DevManifold& d = ml[m0 + i];
+    const uint32_t bits = (S.color[m0 + i] >= 0 ? 1u : 0u) | (mobileBody(B, g0 + d.sa) ? 2u : 0u) |
+                          (mobileBody(B, g0 + d.sb) ? 4u : 0u);
+    UE[i] = make_uint4(d.sa, d.sb, bits, 0u);
+  }
+  __syncthreads();
+  // components over solved manifolds between two mobile bodies: hooking to
+  // the smaller root, then pointer jumping, until nothing changes
   for (int round = 0; round < 100000; round++) {
     if (tid == 0) s_changed = 0;
     __syncthreads();
     for (uint32_t i = tid; i < M; i += bs) {
-      if (S.color[m0 + i] < 0) continue;
-      const DevManifold& d = ml[m0 + i];
-      if (!mobileBody(B, g0 + d.sa) || !mobileBody(B, g0 + d.sb)) continue;
-      uint32_t ra = findRoot(P, d.sa), rb = findRoot(P, d.sb);
+      const uint4 e = UE[i];
+      if (e.z != 7u) continue;
+      uint32_t ra = findRoot(P, e.x), rb = findRoot(P, e.y);
       if (ra != rb) {
         atomicMin(&P[max(ra, rb)], min(ra, rb));
         s_changed = 1;
@@ -2145,20 +2216,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
   // components in root order, cut every T manifolds (a component goes to the
   // unit where it starts)
   for (uint32_t i = tid; i < M; i += bs) {
-    if (S.color[m0 + i] < 0) continue;
-    const DevManifold& d = ml[m0 + i];
-    uint32_t sm = mobileBody(B, g0 + d.sa) ? d.sa : d.sb;
-    if (mobileBody(B, g0 + sm)) atomicAdd(&S.ucnt[g0 + P[sm]], 1u);
+    const uint4 e = UE[i];
+    if ((e.z & 1u) && (e.z & 6u)) atomicAdd(&UC[P[(e.z & 2u) ? e.x : e.y]], 1u);
   }
   __syncthreads();
-  for (uint32_t s = tid; s < MB; s += bs) S.uoff[g0 + s] = S.ucnt[g0 + s];
+  for (uint32_t s = tid; s < MB; s += bs) UO[s] = UC[s];
   __syncthreads();
-  blockScanExclusive(S.uoff + g0, MB, tmp, nullptr);
+  blockScanExclusive(UO, MB, tmp, nullptr);
   const uint32_t T = max(max(cfg.unit_min, 1u), (nsolve + kUnitsMax - 1) / kUnitsMax);
   for (uint32_t u = tid; u < kUnitsMax; u += bs) cnt_m[u] = cnt_b[u] = uid[u] = 0;
   __syncthreads();
   for (uint32_t s = tid; s < MB; s += bs)
-    if (S.ucnt[g0 + s]) uid[S.uoff[g0 + s] / T] = 1;
+    if (UC[s]) uid[UO[s] / T] = 1;
   __syncthreads();
   if (tid == 0) {  // cut positions that start a component, numbered densely
     uint32_t n = 0;
@@ -2177,22 +2246,22 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
     uint32_t g = g0 + s;
     if (!mobileBody(B, g)) continue;
     uint32_t r = P[s];
-    uint32_t u = S.ucnt[g0 + r] ? uid[S.uoff[g0 + r] / T] : s % nu;
+    uint32_t u = UC[r] ? uid[UO[r] / T] : s % nu;
     S.adj_cnt[g] = u;
     atomicAdd(&cnt_b[u], 1u);
   }
   for (uint32_t i = tid; i < M; i += bs) {
-    if (S.color[m0 + i] < 0) continue;
-    const DevManifold& d = ml[m0 + i];
-    uint32_t sm = mobileBody(B, g0 + d.sa) ? d.sa : d.sb;
-    uint32_t u = mobileBody(B, g0 + sm) ? uid[S.uoff[g0 + P[sm]] / T] : 0;
+    const uint4 e = UE[i];
+    if (!(e.z & 1u)) continue;
+    uint32_t u = (e.z & 6u) ? uid[UO[P[(e.z & 2u) ? e.x : e.y]] / T] : 0;
     prio_buf[m0 + i] = u;
     atomicAdd(&cnt_m[u], 1u);
   }
   __syncthreads();
   if (tid == 0) {
-    uint32_t am = 0, ab = 0;
+    uint32_t am = 0, ab = 0, big = 0;
     for (uint32_t u = 0; u < nu; u++) {
+      big = max(big, cnt_m[u]);
       off_m[u] = am;
       off_b[u] = ab;
       S.units[(size_t)w * kUnitsMax + u] = make_uint4(am, cnt_m[u], ab, cnt_b[u]);
@@ -2204,6 +2273,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
     WorldState& x = ws[w];
     x.n_units = nu;
     x.nsolve = nsolve;
+    x.max_unit_m = big;
     x.pos_run = (nsolve > 0 && cfg.pos_iters > 0) ? 1u : 0u;
     x.pos_done = 0;
     x.maxpen = 0;
@@ -2382,6 +2452,8 @@ __global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBod
     const uint4 ud = S.units[q];
     const uint32_t mb = ud.x, mn = ud.y, bn = ud.w;
     const uint32_t* ub = S.ubody + g0 + ud.z;
+    uint64_t* const ts = mn == ws[w].max_unit_m ? S.tstamp : nullptr;  // the largest unit
+    stampW(ts, w, 8);
     float4* VL;
     float4* VA;
     if (kLds) {
@@ -2436,6 +2508,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBod
         VA[s] = S.ws_av[g0 + s];
       }
     __syncthreads();
+    stampW(ts, w, 9);
     const float h = cfg.h;
     const V3 grav{cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]};
     const float dt_inv = h > kEps ? 1.0f / h : 0.0f;
@@ -2527,6 +2600,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBod
           }
         }
         __syncthreads();
+        if (sub_i == 0) stampW(ts, w, 10);
         // warmStart (src/solver.zig:702-729), substep 0 only
         if (cfg.warm_starting && sub_i == 0) {
           for (uint32_t c = 0; c < ncol; c++) {
@@ -2550,6 +2624,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBod
           }
         }
       }
+      if (sub_i == 0) stampW(ts, w, 11);
       // integratePositions (src/solver.zig:341-356,841-845): the unit's
       // mobile bodies (active, not static, awake)
       for (uint32_t k = tid; k < bn; k += bs) {
@@ -2593,7 +2668,9 @@ __global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBod
           __syncthreads();
         }
       }
+      if (sub_i == 0) stampW(ts, w, 12);
     }
+    stampW(ts, w, 13);
     // first position pass (src/solver.zig:860-923); later passes run in
     // k_solve_pos once every unit of the world has finished this one
     if (mn && cfg.pos_iters > 0) {
@@ -2601,6 +2678,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBod
       float minsep = unitPositionPass(cfg, B, S, SR, g0, mb, ncol, ccount, s_red, maxpen);
       unitPositionCombine(ws[w], 0, minsep, maxpen, s_red);
     }
+    stampW(ts, w, 14);
     // storeImpulses (src/solver.zig:933-958)
     if (cfg.store_impulses) {
       for (uint32_t j = mb + tid; j < mb + mn; j += bs) {
@@ -2624,6 +2702,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBod
         S.ws_av[g0 + s] = VA[s];
       }
     __syncthreads();
+    stampW(ts, w, 15);
   }
 }
 

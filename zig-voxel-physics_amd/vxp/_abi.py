@@ -258,7 +258,7 @@ PRODUCT_ONLY = {
     "server_info_get": (C.c_int, [_P, C.POINTER(ServerInfo)]),
 }
 KERNEL_NAMES = ["step_begin", "fat_aabb", "pairs", "pair_process", "cache_replay", "probe",
-                "reduce", "compact", "islands", "solve"]
+                "reduce", "compact", "islands", "solve_prep", "solve_units", "solve_tail"]
 
 
 class VxpError(RuntimeError):
