@@ -205,6 +205,43 @@ int vxp_system_set_config(vxp_system* sys, const vxp_config* cfg);
  * `out` receives max_bodies entries (slot order) of world `w`. */
 int vxp_bodies_get_w(vxp_system* sys, uint32_t world, vxp_body_state* out, uint32_t capacity,
                      uint32_t* count);
+/* Body mutators between frames, one per reference method (VoxelPhysicsSystem
+ * forwards to VoxelBody: src/system.zig:886-961, src/body.zig:266-414), with
+ * the same type guards, arithmetic and wake-ups:
+ *   VXP_OP_APPLY_FORCE            applyForce(v)            lv += v * inv_mass
+ *   VXP_OP_APPLY_FORCE_AT_POINT   applyForceAtPoint(v, p)  + applyTorque((p - pos) x v)
+ *   VXP_OP_APPLY_IMPULSE          applyImpulse(v)
+ *   VXP_OP_APPLY_IMPULSE_AT_POINT applyImpulseAtPoint(v, p)
+ *   VXP_OP_APPLY_TORQUE           applyTorque(v)           av += I_world^-1 v
+ *   VXP_OP_APPLY_ANGULAR_IMPULSE  applyAngularImpulse(v)
+ *   VXP_OP_SET_LINEAR_VELOCITY    setLinearVelocity(v)     (dynamic bodies)
+ *   VXP_OP_SET_ANGULAR_VELOCITY   setAngularVelocity(v)    (dynamic bodies)
+ *   VXP_OP_SET_POSITION           setPosition(v)
+ *   VXP_OP_SET_ROTATION           setRotation(v[0..3] = x, y, z, w)
+ *   VXP_OP_WAKE_UP                wakeUp()
+ *   VXP_OP_SLEEP                  sleep()                  zeroes both velocities
+ * `v` (3 floats, 4 for a rotation) is unused by WAKE_UP and SLEEP, `point` is
+ * used by the *_AT_POINT ops only. */
+#define VXP_OP_APPLY_FORCE 0
+#define VXP_OP_APPLY_FORCE_AT_POINT 1
+#define VXP_OP_APPLY_IMPULSE 2
+#define VXP_OP_APPLY_IMPULSE_AT_POINT 3
+#define VXP_OP_APPLY_TORQUE 4
+#define VXP_OP_APPLY_ANGULAR_IMPULSE 5
+#define VXP_OP_SET_LINEAR_VELOCITY 6
+#define VXP_OP_SET_ANGULAR_VELOCITY 7
+#define VXP_OP_SET_POSITION 8
+#define VXP_OP_SET_ROTATION 9
+#define VXP_OP_WAKE_UP 10
+#define VXP_OP_SLEEP 11
+int vxp_body_op_w(vxp_system* sys, uint32_t world, vxp_handle h, uint32_t op, const float* v,
+                  const float point[3]);
+/* getWorldAABB (src/system.zig:1003-1008, src/body.zig:430-435): the
+ * collider's local AABB through the body's transform; out = min xyz, max xyz. */
+int vxp_body_world_aabb_w(vxp_system* sys, uint32_t world, vxp_handle h, float out[6]);
+/* findBodyById (src/system.zig:462-473): VXP_ERR_INVALID_HANDLE if no active
+ * body of world `w` has this id. */
+int vxp_body_find_w(vxp_system* sys, uint32_t world, uint32_t id, vxp_handle* out);
 /* setPosition/setRotation/setLinearVelocity/setAngularVelocity + wakeUp on one body. */
 int vxp_body_set_state_w(vxp_system* sys, uint32_t world, vxp_handle h, const float pos[3],
                          const float rot[4], const float lin_vel[3], const float ang_vel[3]);

@@ -1725,6 +1725,108 @@ int vxo_body_set_state_w(vxo_system* s, uint32_t world, vxp_handle h, const floa
   return VXP_OK;
 }
 
+// VoxelPhysicsSystem.applyForce ... sleep (src/system.zig:886-961) forwarding
+// to the VoxelBody methods of src/body.zig:266-414.
+int vxo_body_op_w(vxo_system* s, uint32_t world, vxp_handle h, uint32_t op, const float* v,
+                  const float point[3]) {
+  if (world >= s->worlds.size() || op > VXP_OP_SLEEP) return VXP_ERR_ARGUMENT;
+  if (op != VXP_OP_WAKE_UP && op != VXP_OP_SLEEP && !v) return VXP_ERR_ARGUMENT;
+  if ((op == VXP_OP_APPLY_FORCE_AT_POINT || op == VXP_OP_APPLY_IMPULSE_AT_POINT) && !point)
+    return VXP_ERR_ARGUMENT;
+  World& w = *s->worlds[world];
+  if (!w.valid(h)) return VXP_ERR_INVALID_HANDLE;
+  Body& b = w.slots[h.index].body;
+  const bool dyn = b.type == VXP_BODY_DYNAMIC;
+  // wakeUp (src/body.zig:400-404)
+  auto wakeUp = [&] {
+    if (b.type == VXP_BODY_STATIC) return;
+    b.sleeping = false;
+    b.sleep_timer = 0.0f;
+  };
+  // applyForce / applyImpulse (src/body.zig:327-333, 355-361)
+  auto applyLinear = [&](Vec3 f) {
+    if (!dyn || b.inv_mass == 0.0f) return;
+    b.lv = b.lv.add(f.scale(b.inv_mass));
+    wakeUp();
+  };
+  // applyTorque / applyAngularImpulse (src/body.zig:346-353, 374-381)
+  auto applyAngular = [&](Vec3 t) {
+    if (!dyn) return;
+    b.av = b.av.add(b.worldInvInertia().mulVec(t));
+    wakeUp();
+  };
+  Vec3 a{0, 0, 0};
+  if (v) a = {v[0], v[1], v[2]};
+  switch (op) {
+    case VXP_OP_APPLY_FORCE:
+    case VXP_OP_APPLY_IMPULSE:
+      applyLinear(a);
+      break;
+    case VXP_OP_APPLY_FORCE_AT_POINT:  // src/body.zig:336-344
+    case VXP_OP_APPLY_IMPULSE_AT_POINT:  // src/body.zig:364-372
+      if (!dyn) break;
+      applyLinear(a);
+      applyAngular(Vec3{point[0], point[1], point[2]}.sub(b.pos).cross(a));
+      break;
+    case VXP_OP_APPLY_TORQUE:
+    case VXP_OP_APPLY_ANGULAR_IMPULSE:
+      applyAngular(a);
+      break;
+    case VXP_OP_SET_LINEAR_VELOCITY:  // src/body.zig:304-308
+      if (!dyn) break;
+      b.lv = a;
+      wakeUp();
+      break;
+    case VXP_OP_SET_ANGULAR_VELOCITY:  // src/body.zig:316-320
+      if (!dyn) break;
+      b.av = a;
+      wakeUp();
+      break;
+    case VXP_OP_SET_POSITION:  // src/body.zig:281-284
+      b.pos = a;
+      wakeUp();
+      break;
+    case VXP_OP_SET_ROTATION:  // src/body.zig:292-295
+      b.rot = {v[0], v[1], v[2], v[3]};
+      wakeUp();
+      break;
+    case VXP_OP_WAKE_UP:
+      wakeUp();
+      break;
+    case VXP_OP_SLEEP:  // src/body.zig:407-412
+      if (b.type == VXP_BODY_STATIC) break;
+      b.sleeping = true;
+      b.lv = {0, 0, 0};
+      b.av = {0, 0, 0};
+      break;
+  }
+  return VXP_OK;
+}
+
+// VoxelPhysicsSystem.getWorldAABB (src/system.zig:1003-1008)
+int vxo_body_world_aabb_w(vxo_system* s, uint32_t world, vxp_handle h, float out[6]) {
+  if (world >= s->worlds.size() || !out) return VXP_ERR_ARGUMENT;
+  World& w = *s->worlds[world];
+  if (!w.valid(h)) return VXP_ERR_INVALID_HANDLE;
+  AABB a = w.worldAABB(w.slots[h.index].body);
+  const float r[6] = {a.min.x, a.min.y, a.min.z, a.max.x, a.max.y, a.max.z};
+  std::memcpy(out, r, sizeof(r));
+  return VXP_OK;
+}
+
+// VoxelPhysicsSystem.findBodyById (src/system.zig:462-473): a linear scan
+int vxo_body_find_w(vxo_system* s, uint32_t world, uint32_t id, vxp_handle* out) {
+  if (world >= s->worlds.size() || !out) return VXP_ERR_ARGUMENT;
+  World& w = *s->worlds[world];
+  for (size_t i = 0; i < w.slots.size(); i++)
+    if (w.slots[i].active && w.slots[i].body.id == id) {
+      out->index = (uint32_t)i;
+      out->generation = w.slots[i].generation;
+      return VXP_OK;
+    }
+  return VXP_ERR_INVALID_HANDLE;
+}
+
 int vxo_stats_get_w(vxo_system* s, uint32_t world, vxp_stats* out) {
   if (world >= s->worlds.size()) return VXP_ERR_ARGUMENT;
   *out = s->worlds[world]->stats;

@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 import vxp
-from vxp import compare, scenes
+from vxp import _abi, compare, scenes
+from vxp.scenes import SplitMix64
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
@@ -88,7 +89,6 @@ def test_destroy_create_and_wake(product, oracle):
 
 
 def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
-    from vxp import _abi
     monkeypatch.setattr(_abi, "PRODUCT_SO", str(tmp_path / "missing.so"))
     monkeypatch.setattr(_abi, "_cache", {})
     with pytest.raises(vxp.VxpError):
@@ -172,3 +172,55 @@ def test_tuning_choices_are_bit_identical(product):
             assert compare.diff_bodies(other[0][w], runs[0][0][w]) is None, w
             assert compare.diff_manifolds(other[1][w], runs[0][1][w]) is None, w
             assert compare.diff_stats(other[2][w], runs[0][2][w]) is None, w
+
+
+def test_body_mutators_between_frames(product, oracle):
+    # every reference mutator (applyForce ... sleep) applied between frames to
+    # both libraries; bodies, manifolds and stats equal after every frame, and
+    # getWorldAABB / findBodyById agree
+    sc = scenes.c0_small()
+    a = vxp.from_scene(sc, lib=product)
+    b = vxp.from_scene(sc, lib=oracle)
+    rng = SplitMix64(11)
+
+    def v3(lo=-2.0, hi=2.0):
+        return tuple(rng.uniform(lo, hi) for _ in range(3))
+
+    schedule = {}
+    for f in range(5, 150, 10):
+        ops = []
+        for _ in range(6):
+            h = 1 + rng.randint(0, 63)
+            op = rng.randint(0, 11)
+            pos = v3(-4, 4)
+            if op == _abi.OP_SET_ROTATION:
+                q = np.array(rng.quat(), dtype=np.float32)
+                ops.append((h, op, tuple(float(x) for x in q), None))
+            elif op in (_abi.OP_APPLY_FORCE_AT_POINT, _abi.OP_APPLY_IMPULSE_AT_POINT):
+                ops.append((h, op, v3(), pos))
+            elif op == _abi.OP_SET_POSITION:
+                ops.append((h, op, (pos[0], rng.uniform(1.0, 4.0), pos[2]), None))
+            else:
+                ops.append((h, op, v3(), None))
+        schedule[f] = ops
+    ids = a.bodies()["id"]  # slot order
+    print(f"mutators: {sum(len(v) for v in schedule.values())} ops over 150 frames", flush=True)
+    for f in range(150):
+        for h, op, v, p in schedule.get(f, []):
+            for s in (a, b):
+                hh = s.findBodyById(int(ids[h]))
+                assert hh is not None
+                s._op(hh, op, v, p)
+        a.update(sc.dt)
+        b.update(sc.dt)
+        msg = (compare.diff_bodies(a.bodies(), b.bodies()) or
+               compare.diff_manifolds(a.manifolds(), b.manifolds()) or
+               compare.diff_stats(a.stats(), b.stats()))
+        assert msg is None, f"frame {f}: {msg}"
+    for h in range(1, 65):
+        ha = a.findBodyById(int(ids[h]))
+        hb = b.findBodyById(int(ids[h]))
+        assert (ha.index, ha.generation) == (hb.index, hb.generation)
+        assert a.getWorldAABB(ha) == b.getWorldAABB(hb)
+    a.close()
+    b.close()

@@ -1195,6 +1195,127 @@ int vxp_body_set_state_w(vxp_system* s, uint32_t w, vxp_handle h, const float po
   return VXP_OK;
 }
 
+// VoxelBody mutators (src/body.zig:266-414) on the host mirror, uploaded
+// before the next step.  Same arithmetic as the reference: Vec3 add/scale/
+// cross, Mat3.mulVec, getWorldInvInertia = (R * I^-1) * R^T.
+int vxp_body_op_w(vxp_system* s, uint32_t w, vxp_handle h, uint32_t op, const float* v,
+                  const float point[3]) {
+  if (!s || op > VXP_OP_SLEEP) return VXP_ERR_ARGUMENT;
+  if (op != VXP_OP_WAKE_UP && op != VXP_OP_SLEEP && !v) return VXP_ERR_ARGUMENT;
+  if ((op == VXP_OP_APPLY_FORCE_AT_POINT || op == VXP_OP_APPLY_IMPULSE_AT_POINT) && !point)
+    return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
+  const size_t g = (size_t)w * s->MB + h.index;
+  const uint8_t type = s->type[g];
+  const bool dyn = type == VXP_BODY_DYNAMIC;
+  V3 lv{s->lv[g].x, s->lv[g].y, s->lv[g].z}, av{s->av[g].x, s->av[g].y, s->av[g].z};
+  V3 pos{s->pos[g].x, s->pos[g].y, s->pos[g].z};
+  Q4 rot{s->rot[g].x, s->rot[g].y, s->rot[g].z, s->rot[g].w};
+  const float im = s->inv_mass[g];
+  auto wake = [&] {  // wakeUp (src/body.zig:400-404)
+    if (type == VXP_BODY_STATIC) return;
+    s->sleeping[g] = 0;
+    s->sleep_timer[g] = 0.0f;
+  };
+  auto linear = [&](V3 f) {  // applyForce / applyImpulse (src/body.zig:327-333,355-361)
+    if (!dyn || im == 0.0f) return;
+    lv = add(lv, scale(f, im));
+    wake();
+  };
+  auto angular = [&](V3 t) {  // applyTorque / applyAngularImpulse (src/body.zig:346-353,374-381)
+    if (!dyn) return;
+    M3 ii{};
+    if (im != 0.0f) {  // getWorldInvInertia (src/body.zig:384-390)
+      M3 l;
+      for (int i = 0; i < 9; i++) l.m[i] = s->iil[g * 9 + i];
+      ii = worldInvInertia(rot, l);
+    }
+    av = add(av, mulVec(ii, t));
+    wake();
+  };
+  const V3 a = v ? V3{v[0], v[1], v[2]} : V3{0.0f, 0.0f, 0.0f};
+  switch (op) {
+    case VXP_OP_APPLY_FORCE:
+    case VXP_OP_APPLY_IMPULSE:
+      linear(a);
+      break;
+    case VXP_OP_APPLY_FORCE_AT_POINT:
+    case VXP_OP_APPLY_IMPULSE_AT_POINT:
+      if (!dyn) break;
+      linear(a);
+      angular(cross(sub(V3{point[0], point[1], point[2]}, pos), a));
+      break;
+    case VXP_OP_APPLY_TORQUE:
+    case VXP_OP_APPLY_ANGULAR_IMPULSE:
+      angular(a);
+      break;
+    case VXP_OP_SET_LINEAR_VELOCITY:
+      if (!dyn) break;
+      lv = a;
+      wake();
+      break;
+    case VXP_OP_SET_ANGULAR_VELOCITY:
+      if (!dyn) break;
+      av = a;
+      wake();
+      break;
+    case VXP_OP_SET_POSITION:
+      pos = a;
+      wake();
+      break;
+    case VXP_OP_SET_ROTATION:
+      rot = Q4{v[0], v[1], v[2], v[3]};
+      wake();
+      break;
+    case VXP_OP_WAKE_UP:
+      wake();
+      break;
+    case VXP_OP_SLEEP:  // sleep (src/body.zig:407-412)
+      if (type == VXP_BODY_STATIC) break;
+      s->sleeping[g] = 1;
+      lv = V3{0.0f, 0.0f, 0.0f};
+      av = V3{0.0f, 0.0f, 0.0f};
+      break;
+  }
+  s->lv[g] = make_float4(lv.x, lv.y, lv.z, 0.0f);
+  s->av[g] = make_float4(av.x, av.y, av.z, 0.0f);
+  s->pos[g] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+  s->rot[g] = make_float4(rot.x, rot.y, rot.z, rot.w);
+  s->host_dirty = true;
+  return VXP_OK;
+}
+
+int vxp_body_world_aabb_w(vxp_system* s, uint32_t w, vxp_handle h, float out[6]) {
+  if (!s || !out) return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
+  const size_t g = (size_t)w * s->MB + h.index;
+  V3 p{s->pos[g].x, s->pos[g].y, s->pos[g].z};
+  Box b{p, p};
+  if (s->collider[g] >= 0) {
+    const HostCollider& hc = s->colliders[s->collider[g]];
+    b = boxTransform(Box{V3{hc.lo[0], hc.lo[1], hc.lo[2]}, V3{hc.hi[0], hc.hi[1], hc.hi[2]}}, p,
+                     Q4{s->rot[g].x, s->rot[g].y, s->rot[g].z, s->rot[g].w});
+  }
+  const float r[6] = {b.lo.x, b.lo.y, b.lo.z, b.hi.x, b.hi.y, b.hi.z};
+  std::memcpy(out, r, sizeof(r));
+  return VXP_OK;
+}
+
+int vxp_body_find_w(vxp_system* s, uint32_t w, uint32_t id, vxp_handle* out) {
+  if (!s || !out || w >= s->nw) return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  if (id >= s->id_cap) return VXP_ERR_INVALID_HANDLE;
+  const int32_t slot = s->idmap[(size_t)w * s->id_cap + id];
+  if (slot < 0) return VXP_ERR_INVALID_HANDLE;
+  const size_t g = (size_t)w * s->MB + (uint32_t)slot;
+  if (!s->active[g] || s->id[g] != id) return VXP_ERR_INVALID_HANDLE;
+  out->index = (uint32_t)slot;
+  out->generation = s->generation[g];
+  return VXP_OK;
+}
+
 int vxp_stats_get_w(vxp_system* s, uint32_t w, vxp_stats* out) {
   if (!s || !out || w >= s->nw) return VXP_ERR_ARGUMENT;
   HIP_TRY(hipStreamSynchronize(s->stream));

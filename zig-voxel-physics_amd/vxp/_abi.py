@@ -14,6 +14,7 @@ PRODUCT_SO = os.path.join(PKG_DIR, "libvxp.so")
 ORACLE_SO = os.path.join(REPO_DIR, "oracle", "_build", "libvxp_oracle.so")
 
 VXP_OK = 0
+VXP_ERR_INVALID_HANDLE, VXP_ERR_CAPACITY, VXP_ERR_OOM, VXP_ERR_DEVICE, VXP_ERR_ARGUMENT = -1, -2, -3, -4, -5
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
 EMPTY, CORNER, EDGE, FACE, INSIDE = 0, 1, 2, 3, 4
 SOLVE_ORDER_COLOR, SOLVE_ORDER_CANONICAL = 0, 1
@@ -189,6 +190,11 @@ class Tuning(C.Structure):
 
 TUNE_AUTO = 0xFFFFFFFF
 
+# body mutators of vxp_body_op_w (include/vxp.h)
+OP_APPLY_FORCE, OP_APPLY_FORCE_AT_POINT, OP_APPLY_IMPULSE, OP_APPLY_IMPULSE_AT_POINT, \
+    OP_APPLY_TORQUE, OP_APPLY_ANGULAR_IMPULSE, OP_SET_LINEAR_VELOCITY, OP_SET_ANGULAR_VELOCITY, \
+    OP_SET_POSITION, OP_SET_ROTATION, OP_WAKE_UP, OP_SLEEP = range(12)
+
 assert C.sizeof(Config) == 100, C.sizeof(Config)
 assert C.sizeof(Contact) == 88, C.sizeof(Contact)
 assert C.sizeof(Manifold) == 376, C.sizeof(Manifold)
@@ -211,6 +217,10 @@ SIGNATURES = {
                                           C.POINTER(Handle)]),
     "system_set_config": (C.c_int, [_P, C.POINTER(Config)]),
     "body_destroy_w": (C.c_int, [_P, C.c_uint32, Handle]),
+    "body_op_w": (C.c_int, [_P, C.c_uint32, Handle, C.c_uint32, C.POINTER(C.c_float),
+                            C.POINTER(C.c_float)]),
+    "body_world_aabb_w": (C.c_int, [_P, C.c_uint32, Handle, C.POINTER(C.c_float)]),
+    "body_find_w": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(Handle)]),
     "system_update": (C.c_int, [_P, C.c_float]),
     "system_step": (C.c_int, [_P, C.c_float]),
     "system_update_frames": (C.c_int, [_P, C.c_float, C.c_uint32]),

@@ -222,15 +222,84 @@ class VoxelPhysicsSystem:
                                                  arr(rotation, 4), arr(linear_velocity, 3),
                                                  arr(angular_velocity, 3)), "body_set_state")
 
-    # reference-named accessors (src/system.zig:967-1008)
+    # ---- reference-named mutators (src/system.zig:886-961) ---------------------
+    def _op(self, handle, op, v=None, point=None, world=0):
+        va = None if v is None else (C.c_float * len(v))(*v)
+        pa = None if point is None else (C.c_float * 3)(*point)
+        self.lib.check(self.lib.body_op_w(self.ptr, world, handle, op, va, pa), "body_op")
+
+    def applyForce(self, handle, force, world=0):
+        self._op(handle, _abi.OP_APPLY_FORCE, force, world=world)
+
+    def applyForceAtPoint(self, handle, force, point, world=0):
+        self._op(handle, _abi.OP_APPLY_FORCE_AT_POINT, force, point, world)
+
+    def applyImpulse(self, handle, impulse, world=0):
+        self._op(handle, _abi.OP_APPLY_IMPULSE, impulse, world=world)
+
+    def applyImpulseAtPoint(self, handle, impulse, point, world=0):
+        self._op(handle, _abi.OP_APPLY_IMPULSE_AT_POINT, impulse, point, world)
+
+    def applyTorque(self, handle, torque, world=0):
+        self._op(handle, _abi.OP_APPLY_TORQUE, torque, world=world)
+
+    def applyAngularImpulse(self, handle, impulse, world=0):
+        self._op(handle, _abi.OP_APPLY_ANGULAR_IMPULSE, impulse, world=world)
+
+    def setLinearVelocity(self, handle, velocity, world=0):
+        self._op(handle, _abi.OP_SET_LINEAR_VELOCITY, velocity, world=world)
+
+    def setAngularVelocity(self, handle, velocity, world=0):
+        self._op(handle, _abi.OP_SET_ANGULAR_VELOCITY, velocity, world=world)
+
+    def setPosition(self, handle, position, world=0):
+        self._op(handle, _abi.OP_SET_POSITION, position, world=world)
+
+    def setRotation(self, handle, rotation, world=0):
+        self._op(handle, _abi.OP_SET_ROTATION, rotation, world=world)
+
+    def wakeUp(self, handle, world=0):
+        self._op(handle, _abi.OP_WAKE_UP, world=world)
+
+    def sleep(self, handle, world=0):
+        self._op(handle, _abi.OP_SLEEP, world=world)
+
+    # ---- reference-named accessors (src/system.zig:462-473,963-1023) -------------
     def getPosition(self, handle, world=0):
         return tuple(self.bodies(world)[handle.index]["position"])
+
+    def getRotation(self, handle, world=0):
+        return tuple(self.bodies(world)[handle.index]["rotation"])
+
+    def getTransform(self, handle, world=0):
+        b = self.bodies(world)[handle.index]
+        return tuple(b["position"]), tuple(b["rotation"])
 
     def getLinearVelocity(self, handle, world=0):
         return tuple(self.bodies(world)[handle.index]["linear_velocity"])
 
+    def getAngularVelocity(self, handle, world=0):
+        return tuple(self.bodies(world)[handle.index]["angular_velocity"])
+
     def isSleeping(self, handle, world=0):
         return bool(self.bodies(world)[handle.index]["is_sleeping"])
+
+    def getWorldAABB(self, handle, world=0):
+        out = (C.c_float * 6)()
+        self.lib.check(self.lib.body_world_aabb_w(self.ptr, world, handle, out), "body_world_aabb")
+        return tuple(out[:3]), tuple(out[3:])
+
+    def findBodyById(self, body_id, world=0):
+        """The handle of the active body with this id, or None."""
+        h = _abi.Handle()
+        rc = self.lib.body_find_w(self.ptr, world, body_id, C.byref(h))
+        if rc == _abi.VXP_ERR_INVALID_HANDLE:
+            return None
+        self.lib.check(rc, "body_find")
+        return h
+
+    def getBodyCount(self, world=0):
+        return self.stats(world)["total_bodies"]
 
 
 def from_scene(scene, lib=None, cfg_over=None, worlds=None):
