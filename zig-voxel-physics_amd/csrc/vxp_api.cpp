@@ -598,7 +598,7 @@ int stepAll(vxp_system* s) {
     HIP_TRY(s->d_isl_size.ensure(n));
     pe = profBegin(s);
     launch_islands(st, cfg, B, ws, stats, ml, s->d_isl_parent.as<uint32_t>(), s->d_isl_end.as<uint32_t>(),
-                   s->d_isl_size.as<uint32_t>(), nw);
+                   s->d_isl_size.as<uint32_t>(), nw, max_man, s->lds_limit);
     profEnd(s, VXP_K_ISLANDS, pe);
   }
   if (s->cfg.enable_coloring) {

@@ -1416,25 +1416,48 @@ __global__ void k_copy_skipped(const WorldState* ws, const DevManifold* prev, De
 // label = minimum body id of the component (src/islands.zig:318-441).
 // One block per world; hooking + pointer jumping on id-indexed parents.
 // ---------------------------------------------------------------------------
+// Root of r in a union-find forest whose parents only ever point to smaller
+// indices (hooking to the smaller root), read while other threads hook.
+__device__ __forceinline__ uint32_t findRoot(uint32_t* P, uint32_t r) {
+  while (true) {
+    uint32_t t = __atomic_load_n(&P[r], __ATOMIC_RELAXED);
+    if (t == r) return r;
+    r = t;
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const WorldState* ws,
                                                   vxp_stats* st, const DevManifold* ml,
                                                   uint32_t* parent, uint32_t* endpoint,
-                                                  uint32_t* size) {
+                                                  uint32_t* size, size_t lds_bytes) {
+  extern __shared__ uint32_t il[];
   uint32_t w = blockIdx.x;
   if (ws[w].skip) return;
-  uint32_t* P = parent + (size_t)w * cfg.id_cap;
-  uint32_t* E = endpoint + (size_t)w * cfg.id_cap;
-  uint32_t* S = size + (size_t)w * cfg.id_cap;
-  uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
-  for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x) {
+  const uint32_t cap = cfg.id_cap, m0 = ws[w].man_start, M = ws[w].man_count;
+  // parent, endpoint flag and component size per body id, and each manifold's
+  // union edge (kNone: a static endpoint), in LDS when they fit
+  const bool L = ((size_t)3 * cap + 2 * (size_t)M) * 4 <= lds_bytes;
+  uint32_t* P = L ? il : parent + (size_t)w * cap;
+  uint32_t* E = L ? il + cap : endpoint + (size_t)w * cap;
+  uint32_t* S = L ? il + 2 * cap : size + (size_t)w * cap;
+  uint32_t* ea = L ? il + 3 * cap : nullptr;
+  uint32_t* eb = L ? ea + M : nullptr;
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
     P[i] = i;
     E[i] = 0;
     S[i] = 0;
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
-    E[ml[m0 + i].m.body_a_id] = 1;
-    E[ml[m0 + i].m.body_b_id] = 1;
+    const DevManifold& d = ml[m0 + i];
+    E[d.m.body_a_id] = 1;
+    E[d.m.body_b_id] = 1;
+    if (L) {
+      const uint32_t ga = w * cfg.max_bodies + d.sa, gb = w * cfg.max_bodies + d.sb;
+      const bool st_ = B.type[ga] == VXP_BODY_STATIC || B.type[gb] == VXP_BODY_STATIC;
+      ea[i] = d.m.body_a_id;
+      eb[i] = st_ ? kNone : d.m.body_b_id;
+    }
   }
   __shared__ uint32_t changed;
   for (int round = 0; round < 100000; round++) {
@@ -1442,20 +1465,20 @@ __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const
     if (threadIdx.x == 0) changed = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
-      const DevManifold& d = ml[m0 + i];
-      uint32_t ga = w * cfg.max_bodies + d.sa, gb = w * cfg.max_bodies + d.sb;
-      if (B.type[ga] == VXP_BODY_STATIC || B.type[gb] == VXP_BODY_STATIC) continue;
-      uint32_t ra = d.m.body_a_id, rb = d.m.body_b_id;
-      while (true) {
-        uint32_t t = __atomic_load_n(&P[ra], __ATOMIC_RELAXED);
-        if (t == ra) break;
-        ra = t;
+      uint32_t ra, rb;
+      if (L) {
+        if (eb[i] == kNone) continue;
+        ra = ea[i];
+        rb = eb[i];
+      } else {
+        const DevManifold& d = ml[m0 + i];
+        uint32_t ga = w * cfg.max_bodies + d.sa, gb = w * cfg.max_bodies + d.sb;
+        if (B.type[ga] == VXP_BODY_STATIC || B.type[gb] == VXP_BODY_STATIC) continue;
+        ra = d.m.body_a_id;
+        rb = d.m.body_b_id;
       }
-      while (true) {
-        uint32_t t = __atomic_load_n(&P[rb], __ATOMIC_RELAXED);
-        if (t == rb) break;
-        rb = t;
-      }
+      ra = findRoot(P, ra);
+      rb = findRoot(P, rb);
       if (ra != rb) {
         atomicMin(&P[max(ra, rb)], min(ra, rb));
         changed = 1;
@@ -1463,15 +1486,7 @@ __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const
     }
     __syncthreads();
     // pointer jumping
-    for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x) {
-      uint32_t r = i;
-      while (true) {
-        uint32_t t = __atomic_load_n(&P[r], __ATOMIC_RELAXED);
-        if (t == r) break;
-        r = t;
-      }
-      P[i] = r;
-    }
+    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) P[i] = findRoot(P, i);
     __syncthreads();
     if (!changed) break;
   }
@@ -1482,13 +1497,18 @@ __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const
     largest = 0;
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x)
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x)
     if (E[i]) atomicAdd(&S[P[i]], 1u);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < cfg.id_cap; i += blockDim.x)
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x)
     if (S[i]) {
       atomicAdd(&n_isl, 1u);
       atomicMax(&largest, S[i]);
+    }
+  if (L)  // labels and endpoint flags for the export (vxp_islands_get_w)
+    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+      parent[(size_t)w * cap + i] = P[i];
+      endpoint[(size_t)w * cap + i] = E[i];
     }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1893,13 +1913,6 @@ constexpr uint32_t kUnitsMax = kUnitsPerWorld;
 
 __device__ __forceinline__ bool mobileBody(const DevBodies& B, uint32_t g) {
   return B.active[g] && B.type[g] != VXP_BODY_STATIC && !B.sleeping[g];
-}
-__device__ __forceinline__ uint32_t findRoot(uint32_t* P, uint32_t r) {
-  while (true) {
-    uint32_t t = __atomic_load_n(&P[r], __ATOMIC_RELAXED);
-    if (t == r) return r;
-    r = t;
-  }
 }
 
 // Greedy colouring of a world's solved manifolds in ascending (prio, key)
@@ -2977,8 +2990,9 @@ void launch_copy_skipped(hipStream_t s, const WorldState* ws, const DevManifold*
 }
 void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
                     vxp_stats* st, const DevManifold* ml, uint32_t* parent, uint32_t* endpoint,
-                    uint32_t* size, uint32_t nw) {
-  k_islands<<<nw, 1024, 0, s>>>(cfg, B, ws, st, ml, parent, endpoint, size);
+                    uint32_t* size, uint32_t nw, uint32_t max_manifolds, size_t lds_limit) {
+  const size_t lds = std::min(lds_limit, ((size_t)3 * cfg.id_cap + 2 * (size_t)max_manifolds) * 4);
+  k_islands<<<nw, 1024, lds, s>>>(cfg, B, ws, st, ml, parent, endpoint, size, lds);
 }
 void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
                          const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw) {
