@@ -171,6 +171,29 @@ int vxp_system_destroy(vxp_system* sys);
  * A collider may be shared by any number of bodies of any world. */
 int vxp_collider_create(vxp_system* sys, float voxel_size, uint32_t n_bricks,
                         const int32_t* brick_pos, const uint8_t* voxels, uint32_t* out_collider);
+/* The same from solid occupancy: every solid voxel is classified with
+ * classifyFromNeighbors over its six face neighbours (src/collision_voxel.zig:
+ * 194-235; a neighbour in a brick that is not given is empty) on the device,
+ * then stored as by vxp_collider_create.  `solid` holds 8 words per brick,
+ * bit x + 8y of word z; brick positions must be distinct and within +-2^20. */
+int vxp_collider_create_solid(vxp_system* sys, float voxel_size, uint32_t n_bricks,
+                              const int32_t* brick_pos, const uint64_t* solid,
+                              uint32_t* out_collider);
+/* VoxelCollider metadata after recomputeMetadata (src/collider.zig:283-333):
+ * bounds = min_bound xyz, max_bound xyz (voxels, max exclusive); aabb =
+ * local AABB min xyz, max xyz; solid = solid_count. */
+int vxp_collider_info(vxp_system* sys, uint32_t collider, int32_t bounds[6], float aabb[6],
+                      uint32_t* solid);
+/* computeMassProperties (src/body.zig:462-519) of a collider at a density:
+ * out = inv_mass, inverse inertia (row-major 3x3), getCenterOfMass xyz
+ * (src/collider.zig:376-403). */
+int vxp_collider_mass_properties(vxp_system* sys, uint32_t collider, float density,
+                                 float out[13]);
+/* The collider's stored bricks (after hash merging, src/collider.zig:232-239),
+ * ascending (z, y, x) position: 3 ints and 512 voxel bytes each; `count`
+ * receives the brick count (call with capacity 0 to size the arrays). */
+int vxp_collider_bricks(vxp_system* sys, uint32_t collider, int32_t* brick_pos, uint8_t* voxels,
+                        uint32_t capacity, uint32_t* count);
 
 /* createDynamicBody / createStaticBody / createKinematicBody
  * (src/system.zig:294-322,354-377,380-402) in world 0 and in world `w`. */

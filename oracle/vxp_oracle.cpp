@@ -21,6 +21,7 @@
 //
 // Parity status of each part and what pins it: see tests/test_oracle_*.py.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstring>
 #include <functional>
@@ -1628,6 +1629,43 @@ int vxo_collider_create(vxo_system* s, float voxel_size, uint32_t n, const int32
   return VXP_OK;
 }
 
+// classifyFromNeighbors (src/collision_voxel.zig:213-235) for every voxel of
+// the given bricks (occupancy: 8 words per brick, bit x + 8y of word z; a
+// neighbour in a brick that is not given is empty), then the same collider
+// build as vxo_collider_create.
+int vxo_collider_create_solid(vxo_system* s, float voxel_size, uint32_t n, const int32_t* bp,
+                              const uint64_t* solid, uint32_t* out) {
+  if (!(voxel_size > 0.0f) || (n && (!bp || !solid))) return VXP_ERR_ARGUMENT;
+  std::map<std::array<int32_t, 3>, uint32_t> at;
+  for (uint32_t i = 0; i < n; i++) {
+    for (int k = 0; k < 3; k++)
+      if (bp[3 * i + k] < -(1 << 20) || bp[3 * i + k] >= (1 << 20)) return VXP_ERR_ARGUMENT;
+    if (!at.emplace(std::array<int32_t, 3>{bp[3 * i], bp[3 * i + 1], bp[3 * i + 2]}, i).second)
+      return VXP_ERR_ARGUMENT;
+  }
+  // solid voxel at global voxel coordinates
+  auto isSolid = [&](int32_t x, int32_t y, int32_t z) {
+    auto it = at.find({floorDiv8(x), floorDiv8(y), floorDiv8(z)});
+    if (it == at.end()) return false;
+    const int lx = x - floorDiv8(x) * 8, ly = y - floorDiv8(y) * 8, lz = z - floorDiv8(z) * 8;
+    return ((solid[(size_t)it->second * 8 + lz] >> (lx + 8 * ly)) & 1ull) != 0;
+  };
+  std::vector<uint8_t> vox((size_t)n * 512, 0);
+  for (uint32_t i = 0; i < n; i++)
+    for (int v = 0; v < 512; v++) {
+      const int32_t x = bp[3 * i] * 8 + (v & 7), y = bp[3 * i + 1] * 8 + ((v >> 3) & 7),
+                    z = bp[3 * i + 2] * 8 + (v >> 6);
+      if (!isSolid(x, y, z)) continue;
+      const bool nb[6] = {isSolid(x - 1, y, z), isSolid(x + 1, y, z), isSolid(x, y - 1, z),
+                          isSolid(x, y + 1, z), isSolid(x, y, z - 1), isSolid(x, y, z + 1)};
+      uint32_t empty = 0;
+      for (int k = 0; k < 6; k++)
+        if (!nb[k]) empty |= 1u << k;
+      vox[(size_t)i * 512 + v] = classifyVoxel(true, empty);
+    }
+  return vxo_collider_create(s, voxel_size, n, bp, vox.data(), out);
+}
+
 int vxo_body_create_w(vxo_system* s, uint32_t world, uint32_t type, const float pos[3],
                       const float rot[4], uint32_t collider, float density, vxp_handle* out) {
   if (world >= s->worlds.size()) return VXP_ERR_ARGUMENT;
@@ -2045,6 +2083,23 @@ int vxo_mass_properties(vxo_system* s, uint32_t collider, float density, float o
   Vec3 com = c.getCenterOfMass();
   out[10] = com.x; out[11] = com.y; out[12] = com.z;
   return VXP_OK;
+}
+int vxo_collider_bricks(vxo_system* s, uint32_t collider, int32_t* bp, uint8_t* vox, uint32_t cap,
+                        uint32_t* count) {
+  if (collider >= s->shared.colliders.size() || !count) return VXP_ERR_ARGUMENT;
+  const Collider& c = *s->shared.colliders[collider];
+  *count = (uint32_t)c.bricks.size();
+  if (cap && bp && vox)
+    for (uint32_t i = 0; i < std::min<uint32_t>(cap, (uint32_t)c.bricks.size()); i++) {
+      bp[3 * i] = c.bricks[i].pos.x;
+      bp[3 * i + 1] = c.bricks[i].pos.y;
+      bp[3 * i + 2] = c.bricks[i].pos.z;
+      std::memcpy(vox + (size_t)i * 512, c.bricks[i].v, 512);
+    }
+  return VXP_OK;
+}
+int vxo_collider_mass_properties(vxo_system* s, uint32_t collider, float density, float out[13]) {
+  return vxo_mass_properties(s, collider, density, out);
 }
 // Collider metadata: min_bound[3], max_bound[3], local_aabb[6] (as float bits in 6..11)
 int vxo_collider_info(vxo_system* s, uint32_t collider, int32_t bounds[6], float aabb[6],

@@ -13,44 +13,55 @@ std::string& lastError() {
   return e;
 }
 
-// VoxelCollider.recomputeMetadata (src/collider.zig:90-133, 283-333)
-void colliderMetadata(HostCollider& c) {
+// The merged bricks of a collider, uploaded for the collider-build kernels.
+int stageCollider(vxp_system* s, const HostCollider& c) {
+  const size_t nb = c.pos.size() / 3;
+  HIP_TRY(s->d_cb_pos.ensure(std::max<size_t>(nb * 12, 16)));
+  HIP_TRY(s->d_cb_vox.ensure(std::max<size_t>(nb * 512, 512)));
+  if (nb) {
+    HIP_TRY(hipMemcpyAsync(s->d_cb_pos.p, c.pos.data(), nb * 12, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_cb_vox.p, c.vox.data(), nb * 512, hipMemcpyHostToDevice, s->stream));
+  }
+  return VXP_OK;
+}
+
+// VoxelCollider.recomputeMetadata (src/collider.zig:90-133, 283-333): the
+// per-voxel work (solid counts, voxel bounds, brick AABBs) runs on the device
+// per brick; the per-brick results are combined here (sums, minima, maxima:
+// exact in any order) and the dense brick directory is built over the bounds.
+int colliderMetadata(vxp_system* s, HostCollider& c) {
   c.minb[0] = c.minb[1] = c.minb[2] = INT32_MAX;
   c.maxb[0] = c.maxb[1] = c.maxb[2] = INT32_MIN;
   c.lo[0] = c.lo[1] = c.lo[2] = INFINITY;
   c.hi[0] = c.hi[1] = c.hi[2] = -INFINITY;
   c.solid = 0;
-  size_t nb = c.pos.size() / 3;
-  for (size_t b = 0; b < nb; b++) {
-    const int32_t* bp = &c.pos[3 * b];
-    const uint8_t* v = &c.vox[512 * b];
-    V3 mn{INFINITY, INFINITY, INFINITY}, mx{-INFINITY, -INFINITY, -INFINITY};
-    uint32_t cnt = 0;
-    for (int i = 0; i < 512; i++) {
-      if ((v[i] & 7u) == 0) continue;
-      cnt++;
-      int cx = i & 7, cy = (i >> 3) & 7, cz = i >> 6;
-      V3 lp = scale(V3{(float)bp[0] * 8.0f + (float)cx, (float)bp[1] * 8.0f + (float)cy,
-                       (float)bp[2] * 8.0f + (float)cz},
-                    c.vs);
-      mn = vmin(mn, lp);
-      mx = vmax(mx, add(lp, V3{c.vs, c.vs, c.vs}));
-      int32_t vx = bp[0] * 8 + cx, vy = bp[1] * 8 + cy, vz = bp[2] * 8 + cz;
-      c.minb[0] = std::min(c.minb[0], vx);
-      c.minb[1] = std::min(c.minb[1], vy);
-      c.minb[2] = std::min(c.minb[2], vz);
-      c.maxb[0] = std::max(c.maxb[0], vx + 1);
-      c.maxb[1] = std::max(c.maxb[1], vy + 1);
-      c.maxb[2] = std::max(c.maxb[2], vz + 1);
+  const size_t nb = c.pos.size() / 3;
+  if (nb) {
+    RC(stageCollider(s, c));
+    HIP_TRY(s->d_cb_cnt.ensure(nb * 4));
+    HIP_TRY(s->d_cb_bounds.ensure(nb * 24));
+    HIP_TRY(s->d_cb_box.ensure(nb * 24));
+    launch_collider_meta(s->stream, s->d_cb_pos.as<int32_t>(), s->d_cb_vox.as<uint8_t>(),
+                         (uint32_t)nb, c.vs, s->d_cb_cnt.as<uint32_t>(),
+                         s->d_cb_bounds.as<int32_t>(), s->d_cb_box.as<float>());
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> cnt(nb);
+    std::vector<int32_t> bounds(nb * 6);
+    std::vector<float> box(nb * 6);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), s->d_cb_cnt.p, nb * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(bounds.data(), s->d_cb_bounds.p, nb * 24, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(box.data(), s->d_cb_box.p, nb * 24, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t b = 0; b < nb; b++) {
+      if (!cnt[b]) continue;
+      c.solid += cnt[b];
+      for (int k = 0; k < 3; k++) {
+        c.minb[k] = std::min(c.minb[k], bounds[6 * b + k]);
+        c.maxb[k] = std::max(c.maxb[k], bounds[6 * b + 3 + k]);
+        c.lo[k] = fminf(c.lo[k], box[6 * b + k]);
+        c.hi[k] = fmaxf(c.hi[k], box[6 * b + 3 + k]);
+      }
     }
-    if (!cnt) continue;
-    c.solid += cnt;
-    c.lo[0] = fminf(c.lo[0], mn.x);
-    c.lo[1] = fminf(c.lo[1], mn.y);
-    c.lo[2] = fminf(c.lo[2], mn.z);
-    c.hi[0] = fmaxf(c.hi[0], mx.x);
-    c.hi[1] = fmaxf(c.hi[1], mx.y);
-    c.hi[2] = fmaxf(c.hi[2], mx.z);
   }
   if (c.solid == 0) {
     for (int k = 0; k < 3; k++) c.minb[k] = c.maxb[k] = 0;
@@ -70,52 +81,30 @@ void colliderMetadata(HostCollider& c) {
         if (it != c.by_hash.end())
           c.dir[((size_t)z * c.dim[1] + y) * c.dim[0] + x] = it->second;
       }
+  return VXP_OK;
 }
 
-// computeMassProperties (src/body.zig:462-519) + getCenterOfMass
-// (src/collider.zig:376-403), voxels summed brick by brick in ascending
-// (z, y, x) brick order.
-MassProps massProperties(const HostCollider& c, float density) {
-  if (c.solid == 0) return {0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}};
-  float vv = c.vs * c.vs * c.vs;
-  float vm = vv * density;
-  float total = vm * (float)c.solid;
-  float half = c.vs * 0.5f;
-  size_t nb = c.pos.size() / 3;
-  auto voxelPos = [&](size_t b, int i) {
-    int32_t vx = c.pos[3 * b] * 8 + (i & 7), vy = c.pos[3 * b + 1] * 8 + ((i >> 3) & 7),
-            vz = c.pos[3 * b + 2] * 8 + (i >> 6);
-    V3 l{(float)vx * c.vs + half, (float)vy * c.vs + half, (float)vz * c.vs + half};
-    return add(l, V3{c.vs * 0.5f, c.vs * 0.5f, c.vs * 0.5f});
-  };
-  V3 sum{0, 0, 0};
-  uint32_t count = 0;
-  for (size_t b = 0; b < nb; b++)
-    for (int i = 0; i < 512; i++) {
-      if ((c.vox[512 * b + i] & 7u) == 0) continue;
-      sum = add(sum, voxelPos(b, i));
-      count++;
-    }
-  V3 com = scale(sum, 1.0f / (float)count);
-  float I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (size_t b = 0; b < nb; b++)
-    for (int i = 0; i < 512; i++) {
-      if ((c.vox[512 * b + i] & 7u) == 0) continue;
-      V3 r = sub(voxelPos(b, i), com);
-      float r2 = dot(r, r);
-      I[0] += vm * (r2 - r.x * r.x);
-      I[4] += vm * (r2 - r.y * r.y);
-      I[8] += vm * (r2 - r.z * r.z);
-      I[1] -= vm * r.x * r.y;
-      I[2] -= vm * r.x * r.z;
-      I[5] -= vm * r.y * r.z;
-    }
-  I[3] = I[1];
-  I[6] = I[2];
-  I[7] = I[5];
-  M3 m;
-  for (int i = 0; i < 9; i++) m.m[i] = I[i];
-  return {1.0f / total, minverse(m)};
+// computeMassProperties (src/body.zig:462-519) on the device (k_collider_mass:
+// voxels summed brick by brick in ascending (z, y, x) brick order).  com may be
+// null.
+int massProperties(vxp_system* s, const HostCollider& c, float density, MassProps* out, V3* com) {
+  if (c.solid == 0) {
+    *out = MassProps{0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}};
+    if (com) *com = V3{0.0f, 0.0f, 0.0f};
+    return VXP_OK;
+  }
+  RC(stageCollider(s, c));
+  HIP_TRY(s->d_cb_out.ensure(16 * 4));
+  launch_collider_mass(s->stream, s->d_cb_pos.as<int32_t>(), s->d_cb_vox.as<uint8_t>(),
+                       (uint32_t)(c.pos.size() / 3), c.solid, c.vs, density, s->d_cb_out.as<float>());
+  HIP_TRY(hipGetLastError());
+  float r[13];
+  HIP_TRY(hipMemcpyAsync(r, s->d_cb_out.p, sizeof(r), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  out->inv_mass = r[0];
+  for (int i = 0; i < 9; i++) out->inv_inertia.m[i] = r[1 + i];
+  if (com) *com = V3{r[10], r[11], r[12]};
+  return VXP_OK;
 }
 
 }  // namespace vxh
@@ -919,7 +908,9 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,
                     &s->s_color, &s->s_order, &s->s_rec, &s->s_pts, &s->s_prio, &s->s_tstamp,
                     &s->s_upar, &s->s_ucnt, &s->s_uoff, &s->s_ubody, &s->s_uorder, &s->s_units,
-                    &s->s_uqueue, &s->s_pcnt, &s->s_fq, &s->s_cend};
+                    &s->s_uqueue, &s->s_pcnt, &s->s_fq, &s->s_cend, &s->d_cb_pos, &s->d_cb_vox,
+                    &s->d_cb_cnt, &s->d_cb_bounds, &s->d_cb_box, &s->d_cb_out, &s->d_cb_solid,
+                    &s->d_cb_nbr};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -928,6 +919,88 @@ int vxp_system_destroy(vxp_system* s) {
   if (s->tune.b) (void)hipEventDestroy(s->tune.b);
   (void)hipStreamDestroy(s->stream);
   delete s;
+  return VXP_OK;
+}
+
+// Voxel bytes from solid occupancy on the device (k_collider_classify: the
+// classifyFromNeighbors rule of src/collision_voxel.zig:213-235 over the six
+// face neighbours, across bricks by exact position), then vxp_collider_create.
+int vxp_collider_create_solid(vxp_system* s, float voxel_size, uint32_t n, const int32_t* bp,
+                              const uint64_t* solid, uint32_t* out) {
+  if (!s || !out || (n && (!bp || !solid)) || !(voxel_size > 0.0f)) return VXP_ERR_ARGUMENT;
+  std::unordered_map<uint64_t, int32_t> at;  // exact position -> input brick
+  auto key = [](int32_t x, int32_t y, int32_t z) {
+    return ((uint64_t)(uint32_t)x << 42) ^ ((uint64_t)(uint32_t)y << 21) ^ (uint64_t)(uint32_t)z;
+  };
+  for (uint32_t i = 0; i < n; i++) {
+    const int32_t* p = bp + 3 * i;
+    if (p[0] < -(1 << 20) || p[0] >= (1 << 20) || p[1] < -(1 << 20) || p[1] >= (1 << 20) ||
+        p[2] < -(1 << 20) || p[2] >= (1 << 20) || !at.emplace(key(p[0], p[1], p[2]), (int32_t)i).second) {
+      g_err = "collider bricks must have distinct positions within +-2^20";
+      return VXP_ERR_ARGUMENT;
+    }
+  }
+  std::vector<int32_t> nbr((size_t)n * 6, -1);
+  const int32_t d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (uint32_t i = 0; i < n; i++)
+    for (int k = 0; k < 6; k++) {
+      auto it = at.find(key(bp[3 * i] + d[k][0], bp[3 * i + 1] + d[k][1], bp[3 * i + 2] + d[k][2]));
+      if (it != at.end()) nbr[(size_t)i * 6 + k] = it->second;
+    }
+  std::vector<uint8_t> vox((size_t)n * 512);
+  if (n) {
+    HIP_TRY(s->d_cb_solid.ensure((size_t)n * 64));
+    HIP_TRY(s->d_cb_nbr.ensure((size_t)n * 24));
+    HIP_TRY(s->d_cb_vox.ensure((size_t)n * 512));
+    HIP_TRY(hipMemcpyAsync(s->d_cb_solid.p, solid, (size_t)n * 64, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_cb_nbr.p, nbr.data(), (size_t)n * 24, hipMemcpyHostToDevice, s->stream));
+    launch_collider_classify(s->stream, s->d_cb_solid.as<uint64_t>(), s->d_cb_nbr.as<int32_t>(), n,
+                             s->d_cb_vox.as<uint8_t>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(vox.data(), s->d_cb_vox.p, (size_t)n * 512, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+  return vxp_collider_create(s, voxel_size, n, bp, vox.data(), out);
+}
+
+int vxp_collider_info(vxp_system* s, uint32_t collider, int32_t bounds[6], float aabb[6],
+                      uint32_t* solid) {
+  if (!s || !bounds || !aabb || !solid || collider >= s->colliders.size()) return VXP_ERR_ARGUMENT;
+  const HostCollider& c = s->colliders[collider];
+  for (int k = 0; k < 3; k++) {
+    bounds[k] = c.minb[k];
+    bounds[3 + k] = c.maxb[k];
+    aabb[k] = c.lo[k];
+    aabb[3 + k] = c.hi[k];
+  }
+  *solid = c.solid;
+  return VXP_OK;
+}
+
+int vxp_collider_bricks(vxp_system* s, uint32_t collider, int32_t* bp, uint8_t* vox, uint32_t cap,
+                        uint32_t* count) {
+  if (!s || !count || collider >= s->colliders.size()) return VXP_ERR_ARGUMENT;
+  const HostCollider& c = s->colliders[collider];
+  const uint32_t n = (uint32_t)(c.pos.size() / 3);
+  *count = n;
+  if (cap && bp && vox) {
+    const uint32_t k = std::min(n, cap);
+    std::memcpy(bp, c.pos.data(), (size_t)k * 12);
+    std::memcpy(vox, c.vox.data(), (size_t)k * 512);
+  }
+  return VXP_OK;
+}
+
+int vxp_collider_mass_properties(vxp_system* s, uint32_t collider, float density, float out[13]) {
+  if (!s || !out || collider >= s->colliders.size()) return VXP_ERR_ARGUMENT;
+  MassProps m;
+  V3 com;
+  RC(massProperties(s, s->colliders[collider], density, &m, &com));
+  out[0] = m.inv_mass;
+  for (int i = 0; i < 9; i++) out[1 + i] = m.inv_inertia.m[i];
+  out[10] = com.x;
+  out[11] = com.y;
+  out[12] = com.z;
   return VXP_OK;
 }
 
@@ -970,7 +1043,7 @@ int vxp_collider_create(vxp_system* s, float voxel_size, uint32_t n, const int32
     c.vox.insert(c.vox.end(), merged[i].v, merged[i].v + 512);
     c.by_hash[hashBrickPos(merged[i].p[0], merged[i].p[1], merged[i].p[2])] = (int32_t)i;
   }
-  colliderMetadata(c);
+  RC(colliderMetadata(s, c));
   *out = (uint32_t)s->colliders.size();
   s->colliders.push_back(std::move(c));
   s->coll_dirty = true;
@@ -1022,7 +1095,11 @@ static int createBody(vxp_system* s, uint32_t w, uint32_t type, const float pos[
       uint32_t key;
       std::memcpy(&key, &density, 4);
       auto it = hc.mass_cache.find(key);
-      if (it == hc.mass_cache.end()) it = hc.mass_cache.emplace(key, massProperties(hc, density)).first;
+      if (it == hc.mass_cache.end()) {
+        MassProps m;
+        RC(massProperties(s, hc, density, &m, nullptr));
+        it = hc.mass_cache.emplace(key, m).first;
+      }
       mp = it->second;
     }
     s->inv_mass[g] = mp.inv_mass;

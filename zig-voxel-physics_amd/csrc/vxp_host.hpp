@@ -92,8 +92,9 @@ struct HostCollider {
 
 
 // defined in vxp_api.cpp
-void colliderMetadata(HostCollider& c);
-MassProps massProperties(const HostCollider& c, float density);
+int stageCollider(vxp_system* s, const HostCollider& c);
+int colliderMetadata(vxp_system* s, HostCollider& c);
+int massProperties(vxp_system* s, const HostCollider& c, float density, MassProps* out, V3* com);
 
 }  // namespace vxh
 
@@ -152,6 +153,8 @@ struct vxp_system {
   } tune;
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   uint32_t n_cu = 256;   // compute units of the device
+  // collider build scratch (classification, metadata, mass properties)
+  DevBuf d_cb_pos, d_cb_vox, d_cb_cnt, d_cb_bounds, d_cb_box, d_cb_out, d_cb_solid, d_cb_nbr;
   // kernel timing (HIP events on `stream`)
   bool prof = false;
   double prof_ms[VXP_K_COUNT] = {};

@@ -2345,6 +2345,26 @@ __device__ uint32_t unitColours(const SolverScratch& S, uint32_t m0, uint32_t mb
   return ncol;
 }
 
+// Minimum (kMax false) or maximum of v over the block, for any block size:
+// the block is first folded onto the largest power of two below it.
+template <bool kMax>
+__device__ float blockReduce(float v, float* red) {
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
+  uint32_t p2 = 1;
+  while (p2 * 2 <= bs) p2 *= 2;
+  red[tid] = v;
+  __syncthreads();
+  if (tid < bs - p2) red[tid] = kMax ? fmaxf(red[tid], red[tid + p2]) : fminf(red[tid], red[tid + p2]);
+  __syncthreads();
+  for (uint32_t d = p2 / 2; d > 0; d >>= 1) {
+    if (tid < d) red[tid] = kMax ? fmaxf(red[tid], red[tid + d]) : fminf(red[tid], red[tid + d]);
+    __syncthreads();
+  }
+  const float r = red[0];
+  __syncthreads();
+  return r;
+}
+
 // solvePositionConstraints over one unit's manifolds (src/solver.zig:860-923);
 // returns the unit's minimum separation (block-reduced) and raises maxpen.
 __device__ float unitPositionPass(const DevCfg& cfg, const DevBodies& B, const SolverScratch& S,
@@ -2403,15 +2423,7 @@ __device__ float unitPositionPass(const DevCfg& cfg, const DevBodies& B, const S
     }
     __syncthreads();
   }
-  s_red[tid] = minsep;
-  __syncthreads();
-  for (uint32_t d = bs / 2; d > 0; d >>= 1) {
-    if (tid < d) s_red[tid] = fminf(s_red[tid], s_red[tid + d]);
-    __syncthreads();
-  }
-  float r = s_red[0];
-  __syncthreads();
-  return r;
+  return blockReduce<false>(minsep, s_red);
 }
 
 // Combine a unit's position-pass results into its world: the minimum
@@ -2419,15 +2431,10 @@ __device__ float unitPositionPass(const DevCfg& cfg, const DevBodies& B, const S
 // integers in reverse) and the maximum penetration (bits of values >= +0).
 __device__ void unitPositionCombine(WorldState& x, uint32_t slot, float minsep, float maxpen,
                                     float* s_red) {
-  const uint32_t tid = threadIdx.x, bs = blockDim.x;
-  s_red[tid] = maxpen;
-  __syncthreads();
-  for (uint32_t d = bs / 2; d > 0; d >>= 1) {
-    if (tid < d) s_red[tid] = fmaxf(s_red[tid], s_red[tid + d]);
-    __syncthreads();
-  }
+  const uint32_t tid = threadIdx.x;
+  const float red = blockReduce<true>(maxpen, s_red);
   if (tid == 0) {
-    float mp = s_red[0];
+    float mp = red;
     if (mp == 0.0f) mp = 0.0f;
     float ms = minsep;
     if (ms == 0.0f) ms = 0.0f;
@@ -2831,6 +2838,153 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_final(DevCfg cfg, DevBo
   }
 }
 
+// ---------------------------------------------------------------------------
+// collider build: classification, metadata and mass properties
+// ---------------------------------------------------------------------------
+
+// classifyFromNeighbors (src/collision_voxel.zig:213-235) for every voxel of
+// the input bricks: block per brick, thread per voxel.  solid: 8 words per
+// brick, bit x + 8y of word z; nbr: the -x, +x, -y, +y, -z, +z neighbour
+// brick of each brick (-1: none, all empty).
+__global__ __launch_bounds__(512) void k_collider_classify(const uint64_t* solid,
+                                                           const int32_t* nbr, uint8_t* out) {
+  const uint32_t b = blockIdx.x, i = threadIdx.x;
+  const int x = i & 7, y = (i >> 3) & 7, z = i >> 6;
+  auto at = [&](int32_t bb, int xx, int yy, int zz) -> bool {
+    return bb >= 0 && ((solid[(size_t)bb * 8 + zz] >> (xx + 8 * yy)) & 1ull) != 0;
+  };
+  uint8_t v = 0;
+  if (at((int32_t)b, x, y, z)) {
+    const int32_t* nb = nbr + (size_t)b * 6;
+    const bool side[6] = {x > 0 ? at((int32_t)b, x - 1, y, z) : at(nb[0], 7, y, z),
+                          x < 7 ? at((int32_t)b, x + 1, y, z) : at(nb[1], 0, y, z),
+                          y > 0 ? at((int32_t)b, x, y - 1, z) : at(nb[2], x, 7, z),
+                          y < 7 ? at((int32_t)b, x, y + 1, z) : at(nb[3], x, 0, z),
+                          z > 0 ? at((int32_t)b, x, y, z - 1) : at(nb[4], x, y, 7),
+                          z < 7 ? at((int32_t)b, x, y, z + 1) : at(nb[5], x, y, 0)};
+    uint32_t empty = 0;
+    for (int k = 0; k < 6; k++) empty |= side[k] ? 0u : 1u << k;
+    v = classifyByte(empty);
+  }
+  out[(size_t)b * 512 + i] = v;
+}
+
+// CollisionBrick.recomputeMetadata (src/collider.zig:90-133) per brick: solid
+// count, voxel bounds and the local AABB of its non-empty voxels (minima and
+// maxima, exact in any order).  Block per brick, thread per voxel.
+__global__ __launch_bounds__(512) void k_collider_meta(const int32_t* pos, const uint8_t* vox,
+                                                       float vs, uint32_t* cnt, int32_t* bounds,
+                                                       float* box) {
+  __shared__ int32_t ib[6][512];
+  __shared__ float fb[6][512];
+  __shared__ uint32_t n;
+  const uint32_t b = blockIdx.x, i = threadIdx.x;
+  if (i == 0) n = 0;
+  int32_t v[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+  float f[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  __syncthreads();
+  if (vox[(size_t)b * 512 + i] & 7u) {
+    atomicAdd(&n, 1u);
+    const int c[3] = {(int)(i & 7), (int)((i >> 3) & 7), (int)(i >> 6)};
+    for (int k = 0; k < 3; k++) {
+      const int32_t vx = pos[3 * b + k] * 8 + c[k];
+      v[k] = vx;
+      v[3 + k] = vx + 1;
+      const float lp = ((float)pos[3 * b + k] * 8.0f + (float)c[k]) * vs;
+      f[k] = lp;
+      f[3 + k] = lp + vs;
+    }
+  }
+  for (int k = 0; k < 6; k++) {
+    ib[k][i] = v[k];
+    fb[k][i] = f[k];
+  }
+  __syncthreads();
+  for (uint32_t d = 256; d > 0; d >>= 1) {
+    if (i < d)
+      for (int k = 0; k < 3; k++) {
+        ib[k][i] = min(ib[k][i], ib[k][i + d]);
+        ib[3 + k][i] = max(ib[3 + k][i], ib[3 + k][i + d]);
+        fb[k][i] = fminf(fb[k][i], fb[k][i + d]);
+        fb[3 + k][i] = fmaxf(fb[3 + k][i], fb[3 + k][i + d]);
+      }
+    __syncthreads();
+  }
+  if (i < 6) {
+    bounds[(size_t)b * 6 + i] = ib[i][0];
+    box[(size_t)b * 6 + i] = fb[i][0];
+  }
+  if (i == 0) cnt[b] = n;
+}
+
+// computeMassProperties (src/body.zig:462-519) with getCenterOfMass
+// (src/collider.zig:376-403): the voxels are summed one by one, bricks in the
+// order given (ascending (z, y, x) position), so the float sums follow that
+// order exactly; one lane accumulates while the block stages each brick's
+// voxel bytes in LDS.  out: inv_mass, inverse inertia (9), centre of mass (3).
+__global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const uint8_t* vox,
+                                                      uint32_t nb, uint32_t solid, float vs,
+                                                      float density, float* out) {
+  __shared__ uint8_t bv[512];
+  const uint32_t lane = threadIdx.x;
+  const float vv = vs * vs * vs;
+  const float vm = vv * density;
+  const float total = vm * (float)solid;
+  const float half = vs * 0.5f;
+  auto voxelPos = [&](uint32_t b, uint32_t i) {
+    const int32_t vx = pos[3 * b] * 8 + (int32_t)(i & 7), vy = pos[3 * b + 1] * 8 + (int32_t)((i >> 3) & 7),
+                  vz = pos[3 * b + 2] * 8 + (int32_t)(i >> 6);
+    V3 l{(float)vx * vs + half, (float)vy * vs + half, (float)vz * vs + half};
+    return add(l, V3{vs * 0.5f, vs * 0.5f, vs * 0.5f});
+  };
+  auto stage = [&](uint32_t b) {
+    __syncthreads();
+    reinterpret_cast<uint64_t*>(bv)[lane] = reinterpret_cast<const uint64_t*>(vox + (size_t)b * 512)[lane];
+    __syncthreads();
+  };
+  V3 sum{0.f, 0.f, 0.f};
+  uint32_t count = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    stage(b);
+    if (lane == 0)
+      for (uint32_t i = 0; i < 512; i++) {
+        if ((bv[i] & 7u) == 0) continue;
+        sum = add(sum, voxelPos(b, i));
+        count++;
+      }
+  }
+  const V3 com = scale(sum, 1.0f / (float)count);
+  float I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = 0; b < nb; b++) {
+    stage(b);
+    if (lane == 0)
+      for (uint32_t i = 0; i < 512; i++) {
+        if ((bv[i] & 7u) == 0) continue;
+        const V3 r = sub(voxelPos(b, i), com);
+        const float r2 = dot(r, r);
+        I[0] += vm * (r2 - r.x * r.x);
+        I[4] += vm * (r2 - r.y * r.y);
+        I[8] += vm * (r2 - r.z * r.z);
+        I[1] -= vm * r.x * r.y;
+        I[2] -= vm * r.x * r.z;
+        I[5] -= vm * r.y * r.z;
+      }
+  }
+  if (lane == 0) {
+    I[3] = I[1];
+    I[6] = I[2];
+    I[7] = I[5];
+    M3 m;
+    for (int k = 0; k < 9; k++) m.m[k] = I[k];
+    const M3 inv = minverse(m);
+    out[0] = 1.0f / total;
+    for (int k = 0; k < 9; k++) out[1 + k] = inv.m[k];
+    out[10] = com.x;
+    out[11] = com.y;
+    out[12] = com.z;
+  }
+}
+
 __global__ void k_update_end(DevCfg cfg, DevBodies B, vxp_stats* st, uint32_t n_worlds,
                              uint32_t substeps_this_frame, float acc) {
   uint32_t w = blockIdx.x;
@@ -3022,6 +3176,18 @@ void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
   for (uint32_t it = 1; it < cfg.pos_iters; it++)
     k_solve_pos<<<grid, 256, 0, s>>>(cfg, B, ws, S, step_ctr, it);
   k_solve_final<<<nw, kSolveThreads, 0, s>>>(cfg, B, ws, st, S);
+}
+void launch_collider_classify(hipStream_t s, const uint64_t* solid, const int32_t* nbr,
+                              uint32_t nb, uint8_t* out) {
+  if (nb) k_collider_classify<<<nb, 512, 0, s>>>(solid, nbr, out);
+}
+void launch_collider_meta(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
+                          float vs, uint32_t* cnt, int32_t* bounds, float* box) {
+  if (nb) k_collider_meta<<<nb, 512, 0, s>>>(pos, vox, vs, cnt, bounds, box);
+}
+void launch_collider_mass(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
+                          uint32_t solid, float vs, float density, float* out) {
+  k_collider_mass<<<1, 64, 0, s>>>(pos, vox, nb, solid, vs, density, out);
 }
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc) {

@@ -71,6 +71,13 @@ void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wo
 void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, const SolverScratch& S, const uint32_t* step_ctr,
                        uint32_t nw, uint32_t n_cu);
+// collider build (per voxel on the device; the brick bookkeeping is the host's)
+void launch_collider_classify(hipStream_t s, const uint64_t* solid, const int32_t* nbr,
+                              uint32_t nb, uint8_t* out);
+void launch_collider_meta(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
+                          float vs, uint32_t* cnt, int32_t* bounds, float* box);
+void launch_collider_mass(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
+                          uint32_t solid, float vs, float density, float* out);
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc);
 

@@ -226,6 +226,32 @@ VXD V3 lutNormal(uint32_t idx) {
   return normalize(V3{(float)L[idx][0], (float)L[idx][1], (float)L[idx][2]});
 }
 
+// computeNormalIndex (src/collision_voxel.zig:103-130) of a 6-bit empty-side
+// mask (bits -x, +x, -y, +y, -z, +z): the table entry equal to the summed
+// directions of the empty sides.  Every nonzero sum of unit steps is in the
+// table; a zero sum (opposite sides empty) goes to findClosestNormalIndex
+// (src/collision_voxel.zig:85-99), where every dot product is zero: entry 0.
+VXD uint32_t normalIndexOf(uint32_t empty) {
+  const signed char L[26][3] = {
+      {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},  {0, 0, 1},   {0, 0, -1},  {1, 1, 0},
+      {1, -1, 0},  {-1, 1, 0}, {-1, -1, 0}, {1, 0, 1},   {1, 0, -1},  {-1, 0, 1},  {-1, 0, -1},
+      {0, 1, 1},   {0, 1, -1}, {0, -1, 1}, {0, -1, -1}, {1, 1, 1},   {1, 1, -1},  {1, -1, 1},
+      {1, -1, -1}, {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+  const int d[3] = {(int)((empty >> 1) & 1u) - (int)(empty & 1u),
+                    (int)((empty >> 3) & 1u) - (int)((empty >> 2) & 1u),
+                    (int)((empty >> 5) & 1u) - (int)((empty >> 4) & 1u)};
+  for (uint32_t i = 0; i < 26; i++)
+    if (L[i][0] == d[0] && L[i][1] == d[1] && L[i][2] == d[2]) return i;
+  return 0;
+}
+// classifyFromNeighbors -> classifyVoxel (src/collision_voxel.zig:194-235):
+// the voxel byte of a solid voxel, type by the number of empty sides.
+VXD uint8_t classifyByte(uint32_t empty) {
+  const uint32_t c = (uint32_t)__builtin_popcount(empty & 63u);
+  const uint32_t t = c == 0 ? 4u : c == 1 ? 3u : c == 2 ? 2u : 1u;  // inside, face, edge, corner
+  return (uint8_t)(t | (normalIndexOf(empty & 63u) << 3));
+}
+
 // src/collider.zig:187-192
 VXD uint64_t hashBrickPos(int32_t x, int32_t y, int32_t z) {
   return (uint64_t)(int64_t)x ^ ((uint64_t)(int64_t)y * 0x517cc1b727220a95ull) ^

@@ -217,6 +217,13 @@ SIGNATURES = {
                                           C.POINTER(Handle)]),
     "system_set_config": (C.c_int, [_P, C.POINTER(Config)]),
     "body_destroy_w": (C.c_int, [_P, C.c_uint32, Handle]),
+    "collider_create_solid": (C.c_int, [_P, C.c_float, C.c_uint32, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "collider_info": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                C.POINTER(C.c_uint32)]),
+    "collider_mass_properties": (C.c_int, [_P, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
+    "collider_bricks": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
+                                  C.c_uint32, C.POINTER(C.c_uint32)]),
     "body_op_w": (C.c_int, [_P, C.c_uint32, Handle, C.c_uint32, C.POINTER(C.c_float),
                             C.POINTER(C.c_float)]),
     "body_world_aabb_w": (C.c_int, [_P, C.c_uint32, Handle, C.POINTER(C.c_float)]),

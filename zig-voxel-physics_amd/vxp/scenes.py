@@ -215,6 +215,25 @@ def box_shape(sx, sy, sz):
     return classify_grid(solid), (-(sx // 2), -(sy // 2), -(sz // 2))
 
 
+def grid_to_solid_bricks(solid, origin):
+    """Solid occupancy of a dense grid ([x, y, z], voxel (0,0,0) at `origin`) as
+    bricks: positions (n, 3) and 8 uint64 words per brick (bit x + 8y of word
+    z), for bricks holding at least one solid voxel."""
+    solid = np.asarray(solid, dtype=bool)
+    org = np.asarray(origin, dtype=np.int64)
+    idx = np.argwhere(solid) + org
+    if len(idx) == 0:
+        return np.zeros((0, 3), np.int32), np.zeros((0, 8), np.uint64)
+    bpos = np.floor_divide(idx, 8)
+    keys, inv = np.unique(bpos, axis=0, return_inverse=True)
+    inv = np.asarray(inv).reshape(-1)
+    local = idx - keys[inv] * 8
+    words = np.zeros((len(keys), 8), dtype=np.uint64)
+    bits = (np.uint64(1) << (local[:, 0] + 8 * local[:, 1]).astype(np.uint64))
+    np.bitwise_or.at(words, (inv, local[:, 2]), bits)
+    return keys.astype(np.int32), words
+
+
 def _largest_component(solid):
     from scipy import ndimage
     lab, n = ndimage.label(solid)  # default structure: 6-connectivity

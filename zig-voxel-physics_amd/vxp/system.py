@@ -82,6 +82,44 @@ class VoxelPhysicsSystem:
                                                 C.byref(out)), "collider_create")
         return out.value
 
+    def addColliderSolid(self, voxel_size, brick_pos, solid):
+        """A collider from solid occupancy (8 uint64 words per brick, bit x + 8y of
+        word z), classified with classifyFromNeighbors by the library."""
+        bp = np.ascontiguousarray(brick_pos, dtype=np.int32)
+        sm = np.ascontiguousarray(solid, dtype=np.uint64)
+        out = C.c_uint32()
+        self.lib.check(self.lib.collider_create_solid(self.ptr, voxel_size, len(bp),
+                                                      bp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                      sm.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                      C.byref(out)), "collider_create_solid")
+        return out.value
+
+    def colliderInfo(self, collider):
+        """(min_bound, max_bound, local AABB min, local AABB max, solid_count)."""
+        b, a, n = (C.c_int32 * 6)(), (C.c_float * 6)(), C.c_uint32()
+        self.lib.check(self.lib.collider_info(self.ptr, collider, b, a, C.byref(n)), "collider_info")
+        return tuple(b[:3]), tuple(b[3:]), tuple(a[:3]), tuple(a[3:]), n.value
+
+    def colliderMassProperties(self, collider, density):
+        """(inv_mass, inverse inertia 3x3, centre of mass)."""
+        out = (C.c_float * 13)()
+        self.lib.check(self.lib.collider_mass_properties(self.ptr, collider, density, out),
+                       "collider_mass_properties")
+        return out[0], np.array(out[1:10], dtype=np.float32).reshape(3, 3), tuple(out[10:13])
+
+    def colliderBricks(self, collider):
+        """Stored bricks: positions (n, 3) int32 and voxel bytes (n, 512) uint8."""
+        n = C.c_uint32()
+        self.lib.check(self.lib.collider_bricks(self.ptr, collider, None, None, 0, C.byref(n)),
+                       "collider_bricks")
+        bp = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+        vx = np.zeros((max(n.value, 1), 512), dtype=np.uint8)
+        self.lib.check(self.lib.collider_bricks(self.ptr, collider,
+                                                bp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                vx.ctypes.data_as(C.POINTER(C.c_uint8)), n.value,
+                                                C.byref(n)), "collider_bricks")
+        return bp[: n.value], vx[: n.value]
+
     def _create(self, t, position, rotation, collider, density=1.0, world=0):
         h = _abi.Handle()
         pos = (C.c_float * 3)(*position)
