@@ -146,6 +146,11 @@ int uploadColliders(vxp_system* s) {
   HIP_TRY(hipMemcpy(s->d_hdr.p, hdr.data(), hdr.size() * sizeof(DevCollider), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d_dir.p, dir.data(), dir.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d_vox.p, vox.data(), vox.size(), hipMemcpyHostToDevice));
+  const uint32_t nb = (uint32_t)(vox.size() / 512);
+  HIP_TRY(s->d_mask.ensure((size_t)nb * 24 * 8));
+  launch_brick_masks(s->stream, s->d_vox.as<uint8_t>(), nb, s->d_mask.as<uint64_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
   s->coll_dirty = false;
   return VXP_OK;
 }
@@ -218,7 +223,8 @@ DevBodies devBodies(vxp_system* s) {
 }
 
 DevColliders devColliders(vxp_system* s) {
-  return DevColliders{s->d_hdr.as<DevCollider>(), s->d_dir.as<int32_t>(), s->d_vox.as<uint8_t>()};
+  return DevColliders{s->d_hdr.as<DevCollider>(), s->d_dir.as<int32_t>(), s->d_vox.as<uint8_t>(),
+                      s->d_mask.as<uint64_t>()};
 }
 
 DevCfg devCfg(vxp_system* s) {
@@ -895,7 +901,7 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
 int vxp_system_destroy(vxp_system* s) {
   if (!s) return VXP_ERR_ARGUMENT;
   (void)hipStreamSynchronize(s->stream);
-  DevBuf* bufs[] = {&s->d_hdr, &s->d_dir, &s->d_vox, &s->b_pos, &s->b_rot, &s->b_lv, &s->b_av,
+  DevBuf* bufs[] = {&s->d_hdr, &s->d_dir, &s->d_vox, &s->d_mask, &s->b_pos, &s->b_rot, &s->b_lv, &s->b_av,
                     &s->b_fat_lo, &s->b_fat_hi, &s->b_inv_mass, &s->b_iil, &s->b_fric, &s->b_rest,
                     &s->b_ldamp, &s->b_adamp, &s->b_sleep_timer, &s->b_type, &s->b_sleeping,
                     &s->b_active, &s->b_collider, &s->b_id, &s->b_idmap, &s->d_ws, &s->d_stats,

@@ -67,7 +67,8 @@ struct DevBodies {
 struct DevColliders {
   const DevCollider* hdr;
   const int32_t* dir;
-  const uint8_t* vox;  // 512 bytes per brick
+  const uint8_t* vox;     // 512 bytes per brick
+  const uint64_t* mask;   // 24 words per brick: non-empty, corner, edge; bit x + 8y of word z
 };
 
 // Per-world step bookkeeping.

@@ -108,7 +108,7 @@ struct vxp_system {
 
   std::vector<HostCollider> colliders;
   bool coll_dirty = true;
-  DevBuf d_hdr, d_dir, d_vox;
+  DevBuf d_hdr, d_dir, d_vox, d_mask;
 
   // host mirror of the body SoA
   std::vector<float4> pos, rot, lv, av, fat_lo, fat_hi;

@@ -943,7 +943,7 @@ __device__ __forceinline__ uint32_t probeVoxel(const int32_t* __restrict__ dir,
                                                const uint8_t* __restrict__ vox, const VoxMap& mb,
                                                const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
                                                uint8_t va, uint32_t lane, uint32_t* vchk,
-                                               Cand* __restrict__ out) {
+                                               Cand* __restrict__ out, uint32_t limit = 0) {
   V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
   V3 lb = invXformPoint(c.pb, c.qb, wa);
   int32_t cb[3];
@@ -966,7 +966,7 @@ __device__ __forceinline__ uint32_t probeVoxel(const int32_t* __restrict__ dir,
         V3 diff = sub(wb, wa);
         float dsq = dot(diff, diff);
         if (!(dsq < c.spec)) continue;
-        if (kWrite) {
+        if (kWrite && n < limit) {  // never beyond the slots the count pass sized
           // the penetration of generateContact (src/narrow_phase.zig:879-906)
           float dist = sqrtf(dsq);
           float pen = c.vsa - dist;
@@ -981,6 +981,68 @@ __device__ __forceinline__ uint32_t probeVoxel(const int32_t* __restrict__ dir,
         n++;
       }
   if (vchk) *vchk = vc;
+  return n;
+}
+
+// The counts of probeVoxel<false> (candidates n, non-empty probed B voxels
+// vchk) from the brick masks: per (dz, dy) row of the 3x3x3 neighbourhood,
+// the in-bounds x window of the non-empty mask gives vchk by popcount, and
+// only the B voxels whose type passes shouldCheck with the A voxel's (any
+// non-empty one for an A corner, corner or edge for an A edge, corner
+// otherwise) take the distance test, with the same arithmetic.
+__device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
+                                               const uint64_t* __restrict__ mask,
+                                               const VoxMap& mb, const ProbeCtx& c, int32_t x,
+                                               int32_t y, int32_t z, uint8_t va, uint32_t* vchk) {
+  V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
+  V3 lb = invXformPoint(c.pb, c.qb, wa);
+  int32_t cb[3];
+  localToVoxel(c.vsb, lb, cb);
+  const uint32_t ta = va & 7u;
+  // in-bounds window of the neighbourhood per axis
+  int32_t lo[3], hi[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = max(cb[k] - 1, c.bminb[k]);
+    hi[k] = min(cb[k] + 1, c.bmaxb[k] - 1);
+  }
+  uint32_t n = 0, vc = 0;
+  if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) {
+    *vchk = 0;
+    return 0;
+  }
+  const int32_t bx0 = lo[0] >> 3, bx1 = hi[0] >> 3;
+  for (int32_t bz = lo[2]; bz <= hi[2]; bz++)
+    for (int32_t by = lo[1]; by <= hi[1]; by++)
+      for (int32_t bxb = bx0; bxb <= bx1; bxb++) {
+        // one brick's part of the row
+        const int32_t gx = bxb - mb.org[0], gy = (by >> 3) - mb.org[1], gz = (bz >> 3) - mb.org[2];
+        if ((uint32_t)gx >= (uint32_t)mb.dim[0] || (uint32_t)gy >= (uint32_t)mb.dim[1] ||
+            (uint32_t)gz >= (uint32_t)mb.dim[2])
+          continue;
+        const int32_t bi = dir[mb.dir_off + ((uint32_t)gz * mb.dim[1] + gy) * mb.dim[0] + gx];
+        if (bi < 0) continue;
+        const int32_t xa = max(lo[0], bxb * 8), xb = min(hi[0], bxb * 8 + 7);
+        const uint32_t sh = 8u * (uint32_t)(by & 7) + (uint32_t)(xa & 7);
+        const uint32_t wmask = (1u << (xb - xa + 1)) - 1u;
+        const uint64_t* m = mask + (size_t)bi * 24 + (bz & 7);
+        const uint32_t occ = (uint32_t)(m[0] >> sh) & wmask;
+        vc += (uint32_t)__builtin_popcount(occ);
+        uint32_t cand = occ;
+        if (ta != VXP_VOXEL_CORNER) {
+          uint32_t t = (uint32_t)(m[8] >> sh);
+          if (ta == VXP_VOXEL_EDGE) t |= (uint32_t)(m[16] >> sh);
+          cand &= t;
+        }
+        while (cand) {
+          const int32_t bx = xa + __builtin_ctz(cand);
+          cand &= cand - 1;
+          V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
+          V3 diff = sub(wb, wa);
+          float dsq = dot(diff, diff);
+          if (dsq < c.spec) n++;
+        }
+      }
+  *vchk = vc;
   return n;
 }
 
@@ -1098,8 +1160,7 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
   }
   if (kCount) {
     uint32_t vchk = 0, n = 0;
-    if ((va & 7u) != 0)
-      n = probeVoxel<false>(C.dir, C.vox, voxMap(HB), c, x, y, z, va, lane, &vchk, nullptr);
+    if ((va & 7u) != 0) n = probeCount(C.dir, C.mask, voxMap(HB), c, x, y, z, va, &vchk);
     lane_n[(size_t)wid * kWave + lane] = (uint8_t)n;
     uint32_t total = waveSum(n), vsum = waveSum(vchk);
     if (lane == 0) {
@@ -1112,7 +1173,7 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
     int incl = waveInclusiveScan((int)n, (int)lane);
     if (n > 0)
       probeVoxel<true>(C.dir, C.vox, voxMap(HB), c, x, y, z, va, lane, nullptr,
-                       cand + wi_cbase[wid] + (uint32_t)incl - n);
+                       cand + wi_cbase[wid] + (uint32_t)incl - n, n);
   }
 }
 
@@ -2842,6 +2903,32 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_final(DevCfg cfg, DevBo
 // collider build: classification, metadata and mass properties
 // ---------------------------------------------------------------------------
 
+// Per-brick voxel masks of the brick pool: words 0..7 non-empty, 8..15 corner,
+// 16..23 edge voxels; bit x + 8y of word z.  Block per brick, thread per
+// (y, z) row of 8 voxels.
+__global__ __launch_bounds__(64) void k_brick_masks(const uint8_t* vox, uint64_t* mask) {
+  __shared__ uint8_t rows[3][64];
+  const uint32_t b = blockIdx.x, t = threadIdx.x;  // t = y + 8z
+  const uint64_t w = reinterpret_cast<const uint64_t*>(vox + (size_t)b * 512)[t];
+  uint8_t occ = 0, cor = 0, edg = 0;
+  for (int x = 0; x < 8; x++) {
+    const uint32_t ty = (uint32_t)(w >> (8 * x)) & 7u;
+    occ |= ty != 0 ? 1u << x : 0u;
+    cor |= ty == VXP_VOXEL_CORNER ? 1u << x : 0u;
+    edg |= ty == VXP_VOXEL_EDGE ? 1u << x : 0u;
+  }
+  rows[0][t] = occ;
+  rows[1][t] = cor;
+  rows[2][t] = edg;
+  __syncthreads();
+  if (t < 24) {
+    const uint32_t k = t / 8, z = t % 8;
+    uint64_t word = 0;
+    for (int y = 0; y < 8; y++) word |= (uint64_t)rows[k][y + 8 * z] << (8 * y);
+    mask[(size_t)b * 24 + t] = word;
+  }
+}
+
 // classifyFromNeighbors (src/collision_voxel.zig:213-235) for every voxel of
 // the input bricks: block per brick, thread per voxel.  solid: 8 words per
 // brick, bit x + 8y of word z; nbr: the -x, +x, -y, +y, -z, +z neighbour
@@ -3176,6 +3263,9 @@ void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
   for (uint32_t it = 1; it < cfg.pos_iters; it++)
     k_solve_pos<<<grid, 256, 0, s>>>(cfg, B, ws, S, step_ctr, it);
   k_solve_final<<<nw, kSolveThreads, 0, s>>>(cfg, B, ws, st, S);
+}
+void launch_brick_masks(hipStream_t s, const uint8_t* vox, uint32_t nb, uint64_t* mask) {
+  if (nb) k_brick_masks<<<nb, 64, 0, s>>>(vox, mask);
 }
 void launch_collider_classify(hipStream_t s, const uint64_t* solid, const int32_t* nbr,
                               uint32_t nb, uint8_t* out) {

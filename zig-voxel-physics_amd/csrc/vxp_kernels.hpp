@@ -71,6 +71,8 @@ void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wo
 void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, const SolverScratch& S, const uint32_t* step_ctr,
                        uint32_t nw, uint32_t n_cu);
+// per-brick voxel masks of the brick pool (non-empty, corner, edge)
+void launch_brick_masks(hipStream_t s, const uint8_t* vox, uint32_t nb, uint64_t* mask);
 // collider build (per voxel on the device; the brick bookkeeping is the host's)
 void launch_collider_classify(hipStream_t s, const uint64_t* solid, const int32_t* nbr,
                               uint32_t nb, uint8_t* out);
