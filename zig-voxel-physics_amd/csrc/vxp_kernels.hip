@@ -3012,7 +3012,7 @@ __global__ __launch_bounds__(512) void k_collider_meta(const int32_t* pos, const
 __global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const uint8_t* vox,
                                                       uint32_t nb, uint32_t solid, float vs,
                                                       float density, float* out) {
-  __shared__ uint8_t bv[512];
+  __shared__ __align__(16) uint8_t bv[512];
   const uint32_t lane = threadIdx.x;
   const float vv = vs * vs * vs;
   const float vm = vv * density;
@@ -3029,15 +3029,21 @@ __global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const 
     reinterpret_cast<uint64_t*>(bv)[lane] = reinterpret_cast<const uint64_t*>(vox + (size_t)b * 512)[lane];
     __syncthreads();
   };
+  // lane 0 reads the staged bytes 16 at a time
   V3 sum{0.f, 0.f, 0.f};
   uint32_t count = 0;
   for (uint32_t b = 0; b < nb; b++) {
     stage(b);
     if (lane == 0)
-      for (uint32_t i = 0; i < 512; i++) {
-        if ((bv[i] & 7u) == 0) continue;
-        sum = add(sum, voxelPos(b, i));
-        count++;
+      for (uint32_t i0 = 0; i0 < 512; i0 += 16) {
+        const uint4 q = reinterpret_cast<const uint4*>(bv)[i0 / 16];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+          if (((w[k / 4] >> (8 * (k % 4))) & 7u) == 0) continue;
+          sum = add(sum, voxelPos(b, i0 + k));
+          count++;
+        }
       }
   }
   const V3 com = scale(sum, 1.0f / (float)count);
@@ -3045,16 +3051,21 @@ __global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const 
   for (uint32_t b = 0; b < nb; b++) {
     stage(b);
     if (lane == 0)
-      for (uint32_t i = 0; i < 512; i++) {
-        if ((bv[i] & 7u) == 0) continue;
-        const V3 r = sub(voxelPos(b, i), com);
-        const float r2 = dot(r, r);
-        I[0] += vm * (r2 - r.x * r.x);
-        I[4] += vm * (r2 - r.y * r.y);
-        I[8] += vm * (r2 - r.z * r.z);
-        I[1] -= vm * r.x * r.y;
-        I[2] -= vm * r.x * r.z;
-        I[5] -= vm * r.y * r.z;
+      for (uint32_t i0 = 0; i0 < 512; i0 += 16) {
+        const uint4 q = reinterpret_cast<const uint4*>(bv)[i0 / 16];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+          if (((w[k / 4] >> (8 * (k % 4))) & 7u) == 0) continue;
+          const V3 r = sub(voxelPos(b, i0 + k), com);
+          const float r2 = dot(r, r);
+          I[0] += vm * (r2 - r.x * r.x);
+          I[4] += vm * (r2 - r.y * r.y);
+          I[8] += vm * (r2 - r.z * r.z);
+          I[1] -= vm * r.x * r.y;
+          I[2] -= vm * r.x * r.z;
+          I[5] -= vm * r.y * r.z;
+        }
       }
   }
   if (lane == 0) {
