@@ -109,7 +109,14 @@ typedef struct vxp_stats {
   float max_penetration;
   float accumulated_time;
   uint32_t contact_candidates;    /* generateContact candidates before the top-4 (last step) */
+  uint32_t active_islands, sleeping_islands;  /* IslandManager stats as the island build left them */
 } vxp_stats;
+
+/* IslandManager.IslandStats (src/islands.zig:222-243) of one world. */
+typedef struct vxp_island_stats {
+  uint32_t total_islands, active_islands, sleeping_islands, largest_island_size;
+  uint32_t total_bodies, total_contacts, rebuilds_skipped, rebuilds_performed;
+} vxp_island_stats;
 
 /* ContactPoint (src/narrow_phase.zig:336-393) */
 typedef struct vxp_contact {
@@ -287,6 +294,20 @@ int vxp_pairs_get_w(vxp_system* sys, uint32_t world, uint64_t* out, uint32_t cap
  * manifold endpoint body id (ascending), the minimum body id of its island. */
 int vxp_islands_get_w(vxp_system* sys, uint32_t world, uint32_t* body_ids, uint32_t* labels,
                       uint32_t capacity, uint32_t* count);
+/* getIslandStats (src/system.zig:868-873): the island manager's statistics
+ * after the last step's sleep update.  VXP_ERR_ARGUMENT when islands are
+ * disabled (the reference returns null). */
+int vxp_island_stats_get_w(vxp_system* sys, uint32_t world, vxp_island_stats* out);
+/* isBodyInSleepingIsland (src/system.zig:845-853, src/islands.zig:554-562):
+ * *out = 1 if the body's island (as last built) is asleep, else 0 (a body in
+ * no island, islands disabled); an invalid handle sets 0 and returns
+ * VXP_ERR_INVALID_HANDLE. */
+int vxp_body_in_sleeping_island_w(vxp_system* sys, uint32_t world, vxp_handle h, int* out);
+/* wakeBodyIsland (src/system.zig:855-866, src/islands.zig:497-516): clears the
+ * sleep flag and timer of the body's island, forces an island rebuild on the
+ * next step, and wakes the body.  An invalid handle changes nothing and
+ * returns VXP_ERR_INVALID_HANDLE. */
+int vxp_body_wake_island_w(vxp_system* sys, uint32_t world, vxp_handle h);
 /* GraphColoring.build colours (src/islands.zig:668-736) of the current
  * manifolds in pair-key order (needs enable_coloring). */
 int vxp_coloring_get_w(vxp_system* sys, uint32_t world, uint8_t* colors, uint32_t capacity,

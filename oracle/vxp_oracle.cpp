@@ -999,10 +999,18 @@ struct World {
   // narrow phase
   std::map<uint64_t, Manifold> cur, prev;  // ContactManager maps; std::map iterates by key
   BrickPairCache cache;
-  // islands (src/islands.zig:196-441)
+  // IslandManager (src/islands.zig:196-270): islands.items, body_to_island,
+  // stats and the rebuild bookkeeping persist across steps
+  struct Island {
+    std::vector<uint32_t> bodies;  // body ids, ascending
+    bool sleeping = false;
+    float sleep_timer = 0.0f;
+  };
   uint64_t prev_graph_hash = 0;
   bool needs_rebuild = true;
-  uint32_t mgr_islands = 0, mgr_largest = 0;  // IslandManager.stats (persist across skips)
+  std::vector<Island> islands;
+  std::map<uint32_t, uint32_t> body_island;  // body id -> island index
+  vxp_island_stats mgr{};
   std::vector<std::pair<uint32_t, uint32_t>> island_labels;  // (body id, min id of island)
   // reference colouring artifact, pair-key order
   std::vector<uint8_t> ref_colors;
@@ -1368,8 +1376,10 @@ struct World {
     return ncol;
   }
 
-  // IslandManager.buildIslandsWithBodyTypes (src/islands.zig:318-441):
-  // labels are the minimum body id of each union-find component.
+  // IslandManager.buildIslandsWithBodyTypes (src/islands.zig:318-441). The
+  // graph hash runs over the manifolds in pair-key order (the order this
+  // restatement keeps them in); islands are numbered by ascending label (the
+  // minimum body id of the component), which no query observes.
   void buildIslands() {
     uint64_t h = 0;
     const uint64_t prime = 0x100000001b3ull;
@@ -1379,11 +1389,18 @@ struct World {
     }
     h ^= (uint64_t)cur.size();
     h *= prime;
-    if (!needs_rebuild && h == prev_graph_hash && mgr_islands > 0) return;
+    if (!needs_rebuild && h == prev_graph_hash && mgr.total_islands > 0) {
+      mgr.rebuilds_skipped++;
+      return;
+    }
     prev_graph_hash = h;
     needs_rebuild = false;
-    mgr_islands = 0;
-    mgr_largest = 0;
+    const uint32_t skipped = mgr.rebuilds_skipped, performed = mgr.rebuilds_performed + 1;
+    for (Island& is : islands) is = Island{};  // Island.clear (src/islands.zig:176-184)
+    body_island.clear();
+    mgr = vxp_island_stats{};
+    mgr.rebuilds_skipped = skipped;
+    mgr.rebuilds_performed = performed;
     island_labels.clear();
     if (cur.empty()) return;
     uint32_t max_id = 0;
@@ -1397,10 +1414,76 @@ struct World {
     std::vector<std::pair<uint32_t, uint32_t>> edges;
     for (auto& kv : cur) edges.push_back({kv.second.a, kv.second.b});
     island_labels = islandLabels(edges, isStatic);
-    std::map<uint32_t, uint32_t> sizes;
-    for (auto& kv : island_labels) sizes[kv.second]++;
-    mgr_islands = (uint32_t)sizes.size();
-    for (auto& kv : sizes) mgr_largest = std::max(mgr_largest, kv.second);
+    mgr.total_contacts = (uint32_t)cur.size();
+    std::map<uint32_t, uint32_t> index;  // label -> island index
+    for (auto& kv : island_labels) index.emplace(kv.second, (uint32_t)index.size());
+    if (islands.size() < index.size()) islands.resize(index.size());
+    for (auto& kv : island_labels) {
+      const uint32_t i = index[kv.second];
+      body_island[kv.first] = i;
+      islands[i].bodies.push_back(kv.first);
+    }
+    mgr.total_islands = (uint32_t)index.size();
+    for (uint32_t i = 0; i < mgr.total_islands; i++) {
+      const uint32_t n = (uint32_t)islands[i].bodies.size();
+      mgr.total_bodies += n;
+      mgr.largest_island_size = std::max(mgr.largest_island_size, n);
+    }
+    mgr.active_islands = mgr.total_islands;  // every island was just cleared
+  }
+
+  // IslandManager.updateSleepStates (src/islands.zig:444-495), called with the
+  // active bodies copied in slot order (src/system.zig:747-756): an island's
+  // body id indexes that list, so id k reads the k-th active slot's body and
+  // ids at or past the number of active bodies are passed over.
+  void updateIslandSleep(float dt) {
+    std::vector<const Body*> act;
+    for (Slot& s : slots)
+      if (s.active) act.push_back(&s.body);
+    const float lt = 0.1f * 0.1f, at = 0.1f * 0.1f;  // ISLAND_SLEEP_*_THRESHOLD squared
+    for (Island& is : islands) {
+      if (is.bodies.empty()) continue;
+      bool still = true;
+      for (uint32_t id : is.bodies) {
+        if (id >= act.size()) continue;
+        const Body& b = *act[id];
+        if (b.type != VXP_BODY_DYNAMIC) continue;
+        if (b.lv.lengthSquared() > lt || b.av.lengthSquared() > at) {
+          still = false;
+          break;
+        }
+      }
+      if (still) {
+        is.sleep_timer += dt;
+        if (is.sleep_timer >= 0.5f) is.sleeping = true;  // ISLAND_SLEEP_TIME_THRESHOLD
+      } else {
+        is.sleep_timer = 0.0f;
+        is.sleeping = false;
+      }
+    }
+    mgr.sleeping_islands = mgr.active_islands = 0;
+    for (uint32_t i = 0; i < mgr.total_islands; i++) {
+      if (islands[i].sleeping)
+        mgr.sleeping_islands++;
+      else
+        mgr.active_islands++;
+    }
+  }
+
+  // IslandManager.isBodyInSleepingIsland (src/islands.zig:554-562)
+  bool inSleepingIsland(uint32_t id) const {
+    auto it = body_island.find(id);
+    return it != body_island.end() && it->second < islands.size() && islands[it->second].sleeping;
+  }
+
+  // IslandManager.wakeBodyIsland / wakeIsland (src/islands.zig:497-516)
+  void wakeIsland(uint32_t id) {
+    auto it = body_island.find(id);
+    if (it != body_island.end() && it->second < islands.size()) {
+      islands[it->second].sleeping = false;
+      islands[it->second].sleep_timer = 0.0f;
+    }
+    needs_rebuild = true;
   }
 
   // GraphColoring.build (src/islands.zig:668-736) over the current manifolds
@@ -1446,10 +1529,13 @@ struct World {
     stats.solver_points = 0;
     for (const CManifold& cm : solver.cms) stats.solver_points += (uint32_t)cm.n;
     stats.colors = ncol;
-    if (cfg.enable_islands) {
+    if (cfg.enable_islands) {  // src/system.zig:705-757
       buildIslands();
-      stats.island_count = mgr_islands;
-      stats.largest_island_size = mgr_largest;
+      stats.island_count = mgr.total_islands;
+      stats.active_islands = mgr.active_islands;
+      stats.sleeping_islands = mgr.sleeping_islands;
+      stats.largest_island_size = mgr.largest_island_size;
+      updateIslandSleep(dt);
     }
     if (cfg.enable_coloring) buildRefColoring();
     solver.solve(dt);
@@ -1863,6 +1949,29 @@ int vxo_body_find_w(vxo_system* s, uint32_t world, uint32_t id, vxp_handle* out)
       return VXP_OK;
     }
   return VXP_ERR_INVALID_HANDLE;
+}
+
+int vxo_island_stats_get_w(vxo_system* s, uint32_t world, vxp_island_stats* out) {
+  if (world >= s->worlds.size() || !out || !s->worlds[world]->cfg.enable_islands) return VXP_ERR_ARGUMENT;
+  *out = s->worlds[world]->mgr;
+  return VXP_OK;
+}
+
+int vxo_body_in_sleeping_island_w(vxo_system* s, uint32_t world, vxp_handle h, int* out) {
+  if (world >= s->worlds.size() || !out) return VXP_ERR_ARGUMENT;
+  *out = 0;
+  World& w = *s->worlds[world];
+  if (!w.valid(h)) return VXP_ERR_INVALID_HANDLE;
+  if (w.cfg.enable_islands) *out = w.inSleepingIsland(w.slots[h.index].body.id) ? 1 : 0;
+  return VXP_OK;
+}
+
+int vxo_body_wake_island_w(vxo_system* s, uint32_t world, vxp_handle h) {
+  if (world >= s->worlds.size()) return VXP_ERR_ARGUMENT;
+  World& w = *s->worlds[world];
+  if (!w.valid(h)) return VXP_ERR_INVALID_HANDLE;
+  if (w.cfg.enable_islands) w.wakeIsland(w.slots[h.index].body.id);
+  return vxo_body_op_w(s, world, h, VXP_OP_WAKE_UP, nullptr, nullptr);
 }
 
 int vxo_stats_get_w(vxo_system* s, uint32_t world, vxp_stats* out) {

@@ -281,6 +281,34 @@ void growIds(vxp_system* s, uint32_t need) {
   s->host_dirty = true;
 }
 
+// The island arrays at the current id_cap, zeroed when first allocated; when
+// ids have grown since, each world's rows move to the wider layout.
+int ensureIslands(vxp_system* s) {
+  if (s->isl_cap == s->id_cap && s->d_isl_timer.p) return VXP_OK;
+  const size_t n = (size_t)s->nw * s->id_cap * 4;
+  DevBuf* bufs[4] = {&s->d_isl_parent, &s->d_isl_end, &s->d_isl_timer, &s->d_isl_sleep};
+  for (DevBuf* b : bufs) {
+    DevBuf nb;
+    HIP_TRY(nb.ensure(n));
+    HIP_TRY(hipMemsetAsync(nb.p, 0, n, s->stream));
+    if (b->p && s->isl_cap)
+      HIP_TRY(hipMemcpy2DAsync(nb.p, (size_t)s->id_cap * 4, b->p, (size_t)s->isl_cap * 4,
+                               (size_t)s->isl_cap * 4, s->nw, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    b->release();
+    *b = nb;
+  }
+  HIP_TRY(s->d_isl_size.ensure(n));
+  s->isl_cap = s->id_cap;
+  return VXP_OK;
+}
+
+IslandArrays islandArrays(vxp_system* s) {
+  return IslandArrays{s->d_isl_parent.as<uint32_t>(), s->d_isl_end.as<uint32_t>(),
+                      s->d_isl_size.as<uint32_t>(), s->d_isl_timer.as<float>(),
+                      s->d_isl_sleep.as<uint32_t>()};
+}
+
 hipEvent_t takeEvent(vxp_system* s) {
   if (!s->ev_pool.empty()) {
     hipEvent_t e = s->ev_pool.back();
@@ -587,13 +615,9 @@ int stepAll(vxp_system* s) {
 
   // ---- islands / colouring artefacts ------------------------------------------
   if (s->cfg.enable_islands) {
-    size_t n = (size_t)nw * s->id_cap * 4;
-    HIP_TRY(s->d_isl_parent.ensure(n));
-    HIP_TRY(s->d_isl_end.ensure(n));
-    HIP_TRY(s->d_isl_size.ensure(n));
+    RC(ensureIslands(s));
     pe = profBegin(s);
-    launch_islands(st, cfg, B, ws, stats, ml, s->d_isl_parent.as<uint32_t>(), s->d_isl_end.as<uint32_t>(),
-                   s->d_isl_size.as<uint32_t>(), nw, max_man, s->lds_limit);
+    launch_islands(st, cfg, B, ws, stats, ml, prev, islandArrays(s), nw, max_man, s->lds_limit);
     profEnd(s, VXP_K_ISLANDS, pe);
   }
   if (s->cfg.enable_coloring) {
@@ -882,6 +906,7 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
   }
   std::vector<WorldState> ws(n_worlds);
   std::memset(ws.data(), 0, ws.size() * sizeof(WorldState));
+  for (WorldState& x : ws) x.isl_need = 1;  // IslandManager.init (src/islands.zig:248-270)
   std::vector<vxp_stats> st(n_worlds);
   std::memset(st.data(), 0, st.size() * sizeof(vxp_stats));
   std::vector<CacheKeyRec> ck((size_t)n_worlds * cap);
@@ -916,7 +941,7 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->s_upar, &s->s_ucnt, &s->s_uoff, &s->s_ubody, &s->s_uorder, &s->s_units,
                     &s->s_uqueue, &s->s_pcnt, &s->s_fq, &s->s_cend, &s->d_cb_pos, &s->d_cb_vox,
                     &s->d_cb_cnt, &s->d_cb_bounds, &s->d_cb_box, &s->d_cb_out, &s->d_cb_solid,
-                    &s->d_cb_nbr};
+                    &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -1461,18 +1486,19 @@ int vxp_pairs_get_w(vxp_system* s, uint32_t w, uint64_t* out, uint32_t cap, uint
 int vxp_islands_get_w(vxp_system* s, uint32_t w, uint32_t* ids, uint32_t* labels, uint32_t cap,
                       uint32_t* count) {
   if (!s || !count || w >= s->nw) return VXP_ERR_ARGUMENT;
-  if (!s->cfg.enable_islands || !s->d_isl_parent.p) {
+  if (!s->cfg.enable_islands || !s->d_isl_timer.p) {
     *count = 0;
     return VXP_OK;
   }
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<uint32_t> par(s->id_cap), end(s->id_cap);
-  HIP_TRY(hipMemcpy(par.data(), s->d_isl_parent.as<uint32_t>() + (size_t)w * s->id_cap, s->id_cap * 4,
+  const uint32_t ic = s->isl_cap;
+  std::vector<uint32_t> par(ic), end(ic);
+  HIP_TRY(hipMemcpy(par.data(), s->d_isl_parent.as<uint32_t>() + (size_t)w * ic, ic * 4,
                     hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(end.data(), s->d_isl_end.as<uint32_t>() + (size_t)w * s->id_cap, s->id_cap * 4,
+  HIP_TRY(hipMemcpy(end.data(), s->d_isl_end.as<uint32_t>() + (size_t)w * ic, ic * 4,
                     hipMemcpyDeviceToHost));
   uint32_t n = 0;
-  for (uint32_t i = 0; i < s->id_cap; i++) {
+  for (uint32_t i = 0; i < ic; i++) {
     if (!end[i]) continue;
     if (ids && labels && n < cap) {
       ids[n] = i;
@@ -1482,6 +1508,68 @@ int vxp_islands_get_w(vxp_system* s, uint32_t w, uint32_t* ids, uint32_t* labels
   }
   *count = n;
   return VXP_OK;
+}
+
+int vxp_island_stats_get_w(vxp_system* s, uint32_t w, vxp_island_stats* out) {
+  if (!s || !out || w >= s->nw || !s->cfg.enable_islands) return VXP_ERR_ARGUMENT;
+  std::vector<WorldState> ws;
+  RC(readWorlds(s, ws));
+  *out = ws[w].isl;
+  return VXP_OK;
+}
+
+// Label of body `id` of world `w` in the last island build, or kNone when the
+// body was no manifold endpoint (body_to_island, src/islands.zig:204-205).
+static int islandOf(vxp_system* s, uint32_t w, uint32_t id, uint32_t* label) {
+  *label = 0xFFFFFFFFu;
+  if (!s->d_isl_timer.p || id >= s->isl_cap) return VXP_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const size_t i = (size_t)w * s->isl_cap + id;
+  uint32_t end = 0, par = 0;
+  HIP_TRY(hipMemcpy(&end, s->d_isl_end.as<uint32_t>() + i, 4, hipMemcpyDeviceToHost));
+  if (!end) return VXP_OK;
+  HIP_TRY(hipMemcpy(&par, s->d_isl_parent.as<uint32_t>() + i, 4, hipMemcpyDeviceToHost));
+  *label = par;
+  return VXP_OK;
+}
+
+int vxp_body_in_sleeping_island_w(vxp_system* s, uint32_t w, vxp_handle h, int* out) {
+  if (!s || !out || w >= s->nw) return VXP_ERR_ARGUMENT;
+  *out = 0;
+  RC(ensureHost(s));
+  if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
+  if (!s->cfg.enable_islands) return VXP_OK;
+  uint32_t label;
+  RC(islandOf(s, w, s->id[(size_t)w * s->MB + h.index], &label));
+  if (label == 0xFFFFFFFFu) return VXP_OK;
+  uint32_t z = 0;
+  HIP_TRY(hipMemcpy(&z, s->d_isl_sleep.as<uint32_t>() + (size_t)w * s->isl_cap + label, 4,
+                    hipMemcpyDeviceToHost));
+  *out = z ? 1 : 0;
+  return VXP_OK;
+}
+
+int vxp_body_wake_island_w(vxp_system* s, uint32_t w, vxp_handle h) {
+  if (!s || w >= s->nw) return VXP_ERR_ARGUMENT;
+  RC(ensureHost(s));
+  if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
+  if (s->cfg.enable_islands) {
+    // wakeIsland (src/islands.zig:497-507): the island's flag and timer
+    uint32_t label;
+    RC(islandOf(s, w, s->id[(size_t)w * s->MB + h.index], &label));
+    if (label != 0xFFFFFFFFu) {
+      const uint32_t zero = 0;
+      const size_t i = (size_t)w * s->isl_cap + label;
+      HIP_TRY(hipMemcpy(s->d_isl_sleep.as<uint32_t>() + i, &zero, 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s->d_isl_timer.as<float>() + i, &zero, 4, hipMemcpyHostToDevice));
+    }
+    // needs_rebuild, set on both paths (src/islands.zig:506,510-516)
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const uint32_t one = 1;
+    HIP_TRY(hipMemcpy((uint8_t*)(s->d_ws.as<WorldState>() + w) + offsetof(WorldState, isl_need), &one,
+                      4, hipMemcpyHostToDevice));
+  }
+  return vxp_body_op_w(s, w, h, VXP_OP_WAKE_UP, nullptr, nullptr);
 }
 
 int vxp_coloring_get_w(vxp_system* s, uint32_t w, uint8_t* colors, uint32_t cap, uint32_t* count) {

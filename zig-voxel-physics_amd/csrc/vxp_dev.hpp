@@ -88,6 +88,18 @@ struct WorldState {
   uint32_t pos_run, pos_done;         // position passes run; early exit taken
   uint32_t maxpen;                    // float bits of the maximum penetration
   uint32_t pos_min[3];                // float bits of each pass's minimum separation
+  // IslandManager (src/islands.zig:196-270)
+  uint32_t isl_need;                  // needs_rebuild
+  vxp_island_stats isl;               // stats after the last sleep update
+};
+
+// Id-indexed island arrays, [world * id_cap + body id]: union-find parents
+// (labels after a step), endpoint flags, component sizes, and the sleep timer
+// and flag of each island, kept under its label.
+struct IslandArrays {
+  uint32_t *parent, *endpoint, *size;
+  float* timer;
+  uint32_t* sleep;
 };
 
 // Pair flags

@@ -134,6 +134,8 @@ struct vxp_system {
   DevBuf d_ml[2];
   int cur = 0;
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
+  DevBuf d_isl_timer, d_isl_sleep;  // island sleep state (persistent)
+  uint32_t isl_cap = 0;             // id_cap the island arrays are laid out with
   DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
       s_pts, s_prio, s_tstamp, s_upar, s_ucnt, s_uoff, s_ubody, s_uorder, s_units, s_uqueue,
       s_pcnt, s_fq, s_cend;
@@ -172,6 +174,7 @@ namespace vxh {
 size_t slots(const vxp_system* s);
 int uploadColliders(vxp_system* s);
 int ensureDevice(vxp_system* s);
+int ensureIslands(vxp_system* s);
 int ensureHost(vxp_system* s);
 DevBodies devBodies(vxp_system* s);
 DevColliders devColliders(vxp_system* s);

@@ -1487,29 +1487,32 @@ __device__ __forceinline__ uint32_t findRoot(uint32_t* P, uint32_t r) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const WorldState* ws,
+__global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, WorldState* ws,
                                                   vxp_stats* st, const DevManifold* ml,
-                                                  uint32_t* parent, uint32_t* endpoint,
-                                                  uint32_t* size, size_t lds_bytes) {
+                                                  const DevManifold* prev, IslandArrays I,
+                                                  size_t lds_bytes) {
   extern __shared__ uint32_t il[];
   uint32_t w = blockIdx.x;
   if (ws[w].skip) return;
   const uint32_t cap = cfg.id_cap, m0 = ws[w].man_start, M = ws[w].man_count;
+  const uint32_t tid = threadIdx.x, bs = blockDim.x;
   // parent, endpoint flag and component size per body id, and each manifold's
   // union edge (kNone: a static endpoint), in LDS when they fit
   const bool L = ((size_t)3 * cap + 2 * (size_t)M) * 4 <= lds_bytes;
-  uint32_t* P = L ? il : parent + (size_t)w * cap;
-  uint32_t* E = L ? il + cap : endpoint + (size_t)w * cap;
-  uint32_t* S = L ? il + 2 * cap : size + (size_t)w * cap;
+  uint32_t* P = L ? il : I.parent + (size_t)w * cap;
+  uint32_t* E = L ? il + cap : I.endpoint + (size_t)w * cap;
+  uint32_t* S = L ? il + 2 * cap : I.size + (size_t)w * cap;
   uint32_t* ea = L ? il + 3 * cap : nullptr;
   uint32_t* eb = L ? ea + M : nullptr;
-  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+  for (uint32_t i = tid; i < cap; i += bs) {
     P[i] = i;
     E[i] = 0;
     S[i] = 0;
   }
+  __shared__ uint32_t differ;
+  if (tid == 0) differ = M != ws[w].prev_count;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+  for (uint32_t i = tid; i < M; i += bs) {
     const DevManifold& d = ml[m0 + i];
     E[d.m.body_a_id] = 1;
     E[d.m.body_b_id] = 1;
@@ -1519,13 +1522,15 @@ __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const
       ea[i] = d.m.body_a_id;
       eb[i] = st_ ? kNone : d.m.body_b_id;
     }
+    // the key sequence against the previous step's (same length only)
+    if (i < ws[w].prev_count && d.key != prev[ws[w].prev_start + i].key) atomicOr(&differ, 1u);
   }
   __shared__ uint32_t changed;
   for (int round = 0; round < 100000; round++) {
     __syncthreads();
-    if (threadIdx.x == 0) changed = 0;
+    if (tid == 0) changed = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+    for (uint32_t i = tid; i < M; i += bs) {
       uint32_t ra, rb;
       if (L) {
         if (eb[i] == kNone) continue;
@@ -1547,34 +1552,110 @@ __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, const
     }
     __syncthreads();
     // pointer jumping
-    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) P[i] = findRoot(P, i);
+    for (uint32_t i = tid; i < cap; i += bs) P[i] = findRoot(P, i);
     __syncthreads();
     if (!changed) break;
   }
   __syncthreads();
-  __shared__ uint32_t n_isl, largest;
-  if (threadIdx.x == 0) {
-    n_isl = 0;
-    largest = 0;
+  __shared__ uint32_t n_isl, largest, n_end, n_sleep, wsum[16], rbase;
+  if (tid == 0) {
+    n_isl = largest = n_end = n_sleep = rbase = 0;
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x)
+  for (uint32_t i = tid; i < cap; i += bs)
     if (E[i]) atomicAdd(&S[P[i]], 1u);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x)
+  for (uint32_t i = tid; i < cap; i += bs)
     if (S[i]) {
       atomicAdd(&n_isl, 1u);
+      atomicAdd(&n_end, S[i]);
       atomicMax(&largest, S[i]);
     }
-  if (L)  // labels and endpoint flags for the export (vxp_islands_get_w)
-    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
-      parent[(size_t)w * cap + i] = P[i];
-      endpoint[(size_t)w * cap + i] = E[i];
+  // IslandManager rebuild (src/islands.zig:318-441): the graph hash over the
+  // manifolds in key order changes exactly when the key sequence differs from
+  // the previous step's, which (skips keeping it) is the one last built from
+  WorldState& W = ws[w];
+  const bool rebuild = W.isl_need || differ || W.isl.total_islands == 0;
+  float* T = I.timer + (size_t)w * cap;
+  uint32_t* Z = I.sleep + (size_t)w * cap;
+  if (rebuild)  // Island.clear of every island (src/islands.zig:176-184)
+    for (uint32_t i = tid; i < cap; i += bs) {
+      T[i] = 0.0f;
+      Z[i] = 0;
     }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    st[w].island_count = n_isl;
-    st[w].largest_island_size = largest;
+  if (tid == 0) {
+    if (rebuild) {
+      const uint32_t skipped = W.isl.rebuilds_skipped, performed = W.isl.rebuilds_performed + 1;
+      W.isl = vxp_island_stats{};
+      W.isl.rebuilds_skipped = skipped;
+      W.isl.rebuilds_performed = performed;
+      if (M) {
+        W.isl.total_islands = W.isl.active_islands = n_isl;
+        W.isl.largest_island_size = largest;
+        W.isl.total_bodies = n_end;
+        W.isl.total_contacts = M;
+      }
+      W.isl_need = 0;
+    } else {
+      W.isl.rebuilds_skipped++;
+    }
+    // the system's copy is taken before the sleep update (src/system.zig:739-743)
+    st[w].island_count = W.isl.total_islands;
+    st[w].active_islands = W.isl.active_islands;
+    st[w].sleeping_islands = W.isl.sleeping_islands;
+    st[w].largest_island_size = W.isl.largest_island_size;
+  }
+  // IslandManager.updateSleepStates (src/islands.zig:444-495) over the active
+  // bodies copied in slot order (src/system.zig:747-756): island member id k
+  // reads the k-th active slot's body.  A moving dynamic body at rank k marks
+  // the island of body id k (bit 31 of its size word).
+  const float lt = 0.1f * 0.1f, at = 0.1f * 0.1f;
+  const uint32_t lane = tid & (kWave - 1), wave = tid / kWave;
+  for (uint32_t b0 = 0; b0 < cfg.max_bodies; b0 += bs) {
+    const uint32_t slot = b0 + tid;
+    const size_t g = (size_t)w * cfg.max_bodies + slot;
+    const bool act = slot < cfg.max_bodies && B.active[g];
+    const uint64_t bal = __ballot(act);
+    if (lane == 0) wsum[wave] = __popcll(bal);
+    __syncthreads();
+    uint32_t r = rbase + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    for (uint32_t k = 0; k < wave; k++) r += wsum[k];
+    if (act && r < cap && E[r] && B.type[g] == VXP_BODY_DYNAMIC) {
+      const float4 v = B.lv[g], a = B.av[g];
+      const float lsq = v.x * v.x + v.y * v.y + v.z * v.z;  // Vec3.lengthSquared
+      const float asq = a.x * a.x + a.y * a.y + a.z * a.z;
+      if (lsq > lt || asq > at) atomicOr(&S[P[r]], 0x80000000u);
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (uint32_t k = 0; k < bs / kWave; k++) rbase += wsum[k];
+    __syncthreads();
+  }
+  for (uint32_t i = tid; i < cap; i += bs) {
+    const uint32_t sz = S[i];
+    if (!(sz & 0x7FFFFFFFu)) continue;
+    uint32_t z;
+    if (!(sz >> 31)) {
+      const float t = T[i] + cfg.dt;
+      T[i] = t;
+      z = t >= 0.5f ? 1u : Z[i];  // ISLAND_SLEEP_TIME_THRESHOLD
+    } else {
+      T[i] = 0.0f;
+      z = 0;
+    }
+    Z[i] = z;
+    if (z) atomicAdd(&n_sleep, 1u);
+  }
+  if (L)  // labels and endpoint flags for the queries and the export
+    for (uint32_t i = tid; i < cap; i += bs) {
+      I.parent[(size_t)w * cap + i] = P[i];
+      I.endpoint[(size_t)w * cap + i] = E[i];
+    }
+  __syncthreads();
+  if (tid == 0) {
+    W.isl.sleeping_islands = n_sleep;
+    W.isl.active_islands = W.isl.total_islands - n_sleep;
   }
 }
 
@@ -3240,11 +3321,11 @@ void launch_copy_skipped(hipStream_t s, const WorldState* ws, const DevManifold*
                          DevManifold* out, uint32_t nw) {
   k_copy_skipped<<<nw, 256, 0, s>>>(ws, prev, out);
 }
-void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
-                    vxp_stats* st, const DevManifold* ml, uint32_t* parent, uint32_t* endpoint,
-                    uint32_t* size, uint32_t nw, uint32_t max_manifolds, size_t lds_limit) {
+void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                    vxp_stats* st, const DevManifold* ml, const DevManifold* prev,
+                    const IslandArrays& I, uint32_t nw, uint32_t max_manifolds, size_t lds_limit) {
   const size_t lds = std::min(lds_limit, ((size_t)3 * cfg.id_cap + 2 * (size_t)max_manifolds) * 4);
-  k_islands<<<nw, 1024, lds, s>>>(cfg, B, ws, st, ml, parent, endpoint, size, lds);
+  k_islands<<<nw, 1024, lds, s>>>(cfg, B, ws, st, ml, prev, I, lds);
 }
 void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
                          const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw) {

@@ -53,9 +53,9 @@ void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint
                     DevManifold* out);
 void launch_copy_skipped(hipStream_t s, const WorldState* ws, const DevManifold* prev,
                          DevManifold* out, uint32_t nw);
-void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
-                    vxp_stats* st, const DevManifold* ml, uint32_t* parent, uint32_t* endpoint,
-                    uint32_t* size, uint32_t nw, uint32_t max_manifolds, size_t lds_limit);
+void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                    vxp_stats* st, const DevManifold* ml, const DevManifold* prev,
+                    const IslandArrays& I, uint32_t nw, uint32_t max_manifolds, size_t lds_limit);
 void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
                          const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw);
 // The solver in three steps: prep (block / world), units (persistent grid over

@@ -38,6 +38,8 @@ struct Resident {
   uint32_t mcount = 0;  // parked manifolds
   uint32_t hw = 0;      // body slots in use (high-water mark)
   DevBuf ml;            // parked manifolds (DevManifold)
+  uint32_t isl_need = 1;  // island manager bookkeeping while parked
+  vxp_island_stats isl{};
 };
 
 // Per-body-slot device arrays of a system and their bytes per slot.
@@ -103,6 +105,11 @@ void worldCopies(std::vector<CopyDesc>& d, vxp_system* from, uint32_t wf, vxp_sy
                to->d_cache_keys.as<CacheKeyRec>() + wt * cap, cap * sizeof(CacheKeyRec)});
   d.push_back({from->d_cache_vals.as<CacheVal>() + wf * cap,
                to->d_cache_vals.as<CacheVal>() + wt * cap, cap * sizeof(CacheVal)});
+  if (from->cfg.enable_islands)  // labels, endpoint flags and island sleep state
+    for (DevBuf vxp_system::*a : {&vxp_system::d_isl_parent, &vxp_system::d_isl_end,
+                                  &vxp_system::d_isl_timer, &vxp_system::d_isl_sleep})
+      d.push_back({(from->*a).as<uint8_t>() + wf * ic * 4, (to->*a).as<uint8_t>() + wt * ic * 4,
+                   ic * 4});
 }
 
 // Uploads descriptors + world-state records + extra u32 words through the
@@ -166,6 +173,8 @@ int admit(vxp_server* v, const std::vector<std::pair<uint32_t, uint32_t>>& moves
     w.man_start = total;
     w.man_count = r.mcount;
     w.frame = r.frame;
+    w.isl_need = r.isl_need;
+    w.isl = r.isl;
     wsrec.push_back(w);
     wsdst.push_back(b->d_ws.as<WorldState>() + m.second);
     total += r.mcount;
@@ -193,6 +202,8 @@ int retire(vxp_server* v, const std::vector<std::pair<uint32_t, uint32_t>>& move
     worldCopies(d, b, m.second, v->pool, m.first, r.hw);
     r.frame = w.frame;
     r.mcount = w.man_count;
+    r.isl_need = w.isl_need;
+    r.isl = w.isl;
     if (r.mcount) {
       HIP_TRY(r.ml.ensure((size_t)r.mcount * sizeof(DevManifold)));
       d.push_back({ml + w.man_start, r.ml.p, (uint64_t)r.mcount * sizeof(DevManifold)});
@@ -332,6 +343,10 @@ int vxp_server_tick(vxp_server* v, uint32_t* n_done) {
   if (n_done) *n_done = 0;
   vxp_system* b = v->batch;
   RC(ensureDevice(v->pool));
+  if (b->cfg.enable_islands) {
+    RC(ensureIslands(v->pool));
+    RC(ensureIslands(b));
+  }
   // ---- admission: FIFO, one request per world in flight ----------------------
   std::vector<std::pair<uint32_t, uint32_t>> in;
   {

@@ -114,7 +114,18 @@ class Stats(C.Structure):
         "manifolds", "solver_manifolds", "colors", "position_iterations_run",
         "steps_executed", "awake_bodies_stepped", "solver_points")] + [
         ("max_penetration", C.c_float), ("accumulated_time", C.c_float),
-        ("contact_candidates", C.c_uint32)]
+        ("contact_candidates", C.c_uint32), ("active_islands", C.c_uint32),
+        ("sleeping_islands", C.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class IslandStats(C.Structure):
+    """IslandManager.IslandStats (src/islands.zig:222-243)."""
+    _fields_ = [(n, C.c_uint32) for n in (
+        "total_islands", "active_islands", "sleeping_islands", "largest_island_size",
+        "total_bodies", "total_contacts", "rebuilds_skipped", "rebuilds_performed")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -244,6 +255,9 @@ SIGNATURES = {
                                 C.c_uint32, C.POINTER(C.c_uint32)]),
     "coloring_get_w": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint8), C.c_uint32,
                                  C.POINTER(C.c_uint32)]),
+    "island_stats_get_w": (C.c_int, [_P, C.c_uint32, C.POINTER(IslandStats)]),
+    "body_in_sleeping_island_w": (C.c_int, [_P, C.c_uint32, Handle, C.POINTER(C.c_int)]),
+    "body_wake_island_w": (C.c_int, [_P, C.c_uint32, Handle]),
     "last_error": (C.c_char_p, []),
 }
 

@@ -87,6 +87,7 @@ def diff_manifolds(ma, mb, rtol=0.0, atol=0.0):
 STAT_KEYS = ("broad_phase_pairs", "narrow_phase_tested", "narrow_phase_colliding",
              "contacts_generated", "voxels_checked", "pairs_skipped_sleeping", "cache_hits",
              "cache_misses", "manifolds_skipped_sleeping", "island_count", "largest_island_size",
+             "active_islands", "sleeping_islands",
              "substeps_this_frame", "manifolds", "solver_manifolds", "colors",
              "position_iterations_run", "steps_executed", "awake_bodies_stepped",
              "contact_candidates",

@@ -336,6 +336,30 @@ class VoxelPhysicsSystem:
         self.lib.check(rc, "body_find")
         return h
 
+    def isBodyInSleepingIsland(self, handle, world=0):
+        """isBodyInSleepingIsland (src/system.zig:845-853); False for an invalid handle."""
+        out = C.c_int()
+        rc = self.lib.body_in_sleeping_island_w(self.ptr, world, handle, C.byref(out))
+        if rc == _abi.VXP_ERR_INVALID_HANDLE:
+            return False
+        self.lib.check(rc, "body_in_sleeping_island")
+        return bool(out.value)
+
+    def wakeBodyIsland(self, handle, world=0):
+        """wakeBodyIsland (src/system.zig:855-866); an invalid handle is ignored."""
+        rc = self.lib.body_wake_island_w(self.ptr, world, handle)
+        if rc != _abi.VXP_ERR_INVALID_HANDLE:
+            self.lib.check(rc, "body_wake_island")
+
+    def getIslandStats(self, world=0):
+        """getIslandStats (src/system.zig:868-873): a dict, or None when islands are off."""
+        st = _abi.IslandStats()
+        rc = self.lib.island_stats_get_w(self.ptr, world, C.byref(st))
+        if rc == _abi.VXP_ERR_ARGUMENT and not self.cfg.enable_islands:
+            return None
+        self.lib.check(rc, "island_stats_get")
+        return st.as_dict()
+
     def getBodyCount(self, world=0):
         return self.stats(world)["total_bodies"]
 
