@@ -48,6 +48,12 @@ extern "C" {
 #define VXP_SOLVE_ORDER_COLOR 0     /* per-colour Gauss-Seidel (GPU order)  */
 #define VXP_SOLVE_ORDER_CANONICAL 1 /* ascending pair key                   */
 
+/* Order in which the narrow phase takes the broad-phase pairs: the pair cap
+ * and the brick-pair cache follow it (detection results do not depend on it). */
+#define VXP_PAIR_ORDER_KEY 0   /* ascending pair key */
+#define VXP_PAIR_ORDER_TREE 1  /* the reference's AABB-tree order (src/broad_phase.zig:488-526):
+                                  with pair_cap = 512, the cap-faithful mode */
+
 /*
  * Configuration: PhysicsConfig (src/system.zig:47-88) + SolverConfig
  * (src/solver.zig:75-119) + NarrowPhaseConfig (src/narrow_phase.zig:1325-1343)
@@ -92,6 +98,7 @@ typedef struct vxp_config {
                                   filtered-pair array (src/system.zig:594) */
   uint32_t solve_order;        /* VXP_SOLVE_ORDER_* */
   uint32_t cpu_threads;        /* CPU restatement only: narrow-phase workers */
+  uint32_t pair_order;         /* VXP_PAIR_ORDER_* */
 } vxp_config;
 
 /* Per-step statistics: PhysicsStats (src/system.zig:91-155) counters, for
@@ -286,8 +293,9 @@ int vxp_stats_get_all(vxp_system* sys, vxp_stats* out, uint32_t capacity);
  * world `w` in ascending pair-key order. */
 int vxp_manifolds_get_w(vxp_system* sys, uint32_t world, vxp_manifold* out, uint32_t capacity,
                         uint32_t* count);
-/* BroadPhase.findOverlappingPairs pair set of the last step, ascending
- * (src/broad_phase.zig:606-619); each entry (a<<32)|b with a < b. */
+/* BroadPhase.findOverlappingPairs pairs of the last step (src/broad_phase.zig:606-619)
+ * in the narrow phase's order: ascending, or the trees' emission order under
+ * VXP_PAIR_ORDER_TREE; each entry (a<<32)|b with a < b. */
 int vxp_pairs_get_w(vxp_system* sys, uint32_t world, uint64_t* out, uint32_t capacity,
                     uint32_t* count);
 /* Island labels of the last step (src/islands.zig:318-441): for each
@@ -308,6 +316,20 @@ int vxp_body_in_sleeping_island_w(vxp_system* sys, uint32_t world, vxp_handle h,
  * next step, and wakes the body.  An invalid handle changes nothing and
  * returns VXP_ERR_INVALID_HANDLE. */
 int vxp_body_wake_island_w(vxp_system* sys, uint32_t world, vxp_handle h);
+/* One node of a broad-phase tree (TreeNode, src/broad_phase.zig:40-62). */
+typedef struct vxp_tree_node {
+  float min[3], max[3];
+  uint32_t body_id;                    /* leaves */
+  uint32_t parent, child1, child2;     /* 0xFFFFFFFF: none; free nodes: parent = next free */
+  int32_t height;                      /* -1: on the free list */
+  uint32_t pad;
+} vxp_tree_node;
+/* The static (which = 0) or dynamic (which = 1) AABBTree of world `w`
+ * (src/broad_phase.zig:64-529) as the last step left it: nodes.items in index
+ * order and the root.  Needs pair_order = VXP_PAIR_ORDER_TREE (VXP_ERR_ARGUMENT
+ * otherwise). */
+int vxp_tree_get_w(vxp_system* sys, uint32_t world, uint32_t which, vxp_tree_node* out,
+                   uint32_t capacity, uint32_t* count, uint32_t* root);
 /* GraphColoring.build colours (src/islands.zig:668-736) of the current
  * manifolds in pair-key order (needs enable_coloring). */
 int vxp_coloring_get_w(vxp_system* sys, uint32_t world, uint8_t* colors, uint32_t capacity,

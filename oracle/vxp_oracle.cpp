@@ -978,6 +978,230 @@ std::vector<uint8_t> refColoring(const std::vector<std::pair<uint32_t, uint32_t>
 }
 
 // ---------------------------------------------------------------------------
+// src/broad_phase.zig:40-529 — AABBTree, restated node for node so that the
+// pair order of findOverlappingPairs (and with it the 512-pair filter and the
+// brick-pair cache order) can be reproduced
+// ---------------------------------------------------------------------------
+struct AabbTree {
+  static constexpr uint32_t kNull = 0xFFFFFFFFu;  // NULL_NODE
+  struct Node {
+    AABB box;
+    uint32_t user = 0, parent = kNull, child1 = kNull, child2 = kNull;
+    int32_t height = 0;
+    bool isLeaf() const { return child1 == kNull; }
+  };
+  std::vector<Node> nodes;
+  uint32_t root = kNull, free_list = kNull, node_count = 0;
+  std::map<uint32_t, uint32_t> body_to_node;
+
+  // AABB.surfaceArea (src/math.zig:628-631)
+  static float area(const AABB& b) {
+    Vec3 s = b.max.sub(b.min);
+    return 2.0f * (s.x * s.y + s.y * s.z + s.z * s.x);
+  }
+  uint32_t allocateNode() {  // src/broad_phase.zig:112-140
+    uint32_t id;
+    if (free_list != kNull) {
+      id = free_list;
+      free_list = nodes[id].parent;
+    } else {
+      id = (uint32_t)nodes.size();
+      nodes.emplace_back();
+    }
+    nodes[id] = Node{};
+    nodes[id].box = AABB::empty();
+    node_count++;
+    return id;
+  }
+  void freeNode(uint32_t id) {  // src/broad_phase.zig:142-147
+    nodes[id].parent = free_list;
+    nodes[id].height = -1;
+    free_list = id;
+    node_count--;
+  }
+  void insertBody(uint32_t body, const AABB& fat) {  // src/broad_phase.zig:150-163
+    uint32_t id = allocateNode();
+    nodes[id].box = fat;
+    nodes[id].user = body;
+    nodes[id].height = 0;
+    body_to_node[body] = id;
+    insertLeaf(id);
+  }
+  void removeBody(uint32_t body) {  // src/broad_phase.zig:166-172
+    auto it = body_to_node.find(body);
+    if (it == body_to_node.end()) return;
+    removeLeaf(it->second);
+    freeNode(it->second);
+    body_to_node.erase(it);
+  }
+  void moveBody(uint32_t body, const AABB& fat) {  // src/broad_phase.zig:190-212
+    uint32_t id = body_to_node.at(body);
+    removeLeaf(id);
+    nodes[id].box = fat;
+    insertLeaf(id);
+  }
+  int32_t h(uint32_t i) const { return nodes[i].height; }
+  void insertLeaf(uint32_t leaf) {  // src/broad_phase.zig:215-314
+    if (root == kNull) {
+      root = leaf;
+      nodes[leaf].parent = kNull;
+      return;
+    }
+    const AABB lb = nodes[leaf].box;
+    uint32_t index = root;
+    while (!nodes[index].isLeaf()) {
+      const Node& n = nodes[index];
+      const float a = area(n.box);
+      const float ca = area(n.box.merge(lb));
+      const float cost = 2.0f * ca;
+      const float inh = 2.0f * (ca - a);
+      auto childCost = [&](uint32_t c) {
+        const float na = area(lb.merge(nodes[c].box));
+        return nodes[c].isLeaf() ? na + inh : (na - area(nodes[c].box)) + inh;
+      };
+      const float c1 = childCost(n.child1), c2 = childCost(n.child2);
+      if (cost < c1 && cost < c2) break;
+      index = c1 < c2 ? n.child1 : n.child2;
+    }
+    const uint32_t sibling = index, old_parent = nodes[sibling].parent;
+    const uint32_t np = allocateNode();
+    nodes[np].parent = old_parent;
+    nodes[np].user = 0;
+    nodes[np].box = lb.merge(nodes[sibling].box);
+    nodes[np].height = nodes[sibling].height + 1;
+    if (old_parent != kNull) {
+      if (nodes[old_parent].child1 == sibling)
+        nodes[old_parent].child1 = np;
+      else
+        nodes[old_parent].child2 = np;
+    } else {
+      root = np;
+    }
+    nodes[np].child1 = sibling;
+    nodes[np].child2 = leaf;
+    nodes[sibling].parent = np;
+    nodes[leaf].parent = np;
+    for (uint32_t w = np; w != kNull; w = nodes[w].parent) {
+      w = balance(w);
+      const uint32_t c1 = nodes[w].child1, c2 = nodes[w].child2;
+      nodes[w].height = 1 + std::max(h(c1), h(c2));
+      nodes[w].box = nodes[c1].box.merge(nodes[c2].box);
+    }
+  }
+  void removeLeaf(uint32_t leaf) {  // src/broad_phase.zig:317-362
+    if (leaf == root) {
+      root = kNull;
+      return;
+    }
+    const uint32_t parent = nodes[leaf].parent, grand = nodes[parent].parent;
+    const uint32_t sibling = nodes[parent].child1 == leaf ? nodes[parent].child2 : nodes[parent].child1;
+    if (grand == kNull) {
+      root = sibling;
+      nodes[sibling].parent = kNull;
+      freeNode(parent);
+      return;
+    }
+    if (nodes[grand].child1 == parent)
+      nodes[grand].child1 = sibling;
+    else
+      nodes[grand].child2 = sibling;
+    nodes[sibling].parent = grand;
+    freeNode(parent);
+    for (uint32_t i = grand; i != kNull; i = nodes[i].parent) {
+      i = balance(i);
+      const uint32_t c1 = nodes[i].child1, c2 = nodes[i].child2;
+      nodes[i].box = nodes[c1].box.merge(nodes[c2].box);
+      nodes[i].height = 1 + std::max(h(c1), h(c2));
+    }
+  }
+  uint32_t balance(uint32_t A) {  // src/broad_phase.zig:365-458
+    if (nodes[A].isLeaf() || nodes[A].height < 2) return A;
+    const uint32_t B = nodes[A].child1, C = nodes[A].child2;
+    const int32_t bal = h(C) - h(B);
+    if (bal > 1) {  // C moves up
+      const uint32_t F = nodes[C].child1, G = nodes[C].child2;
+      nodes[C].child1 = A;
+      nodes[C].parent = nodes[A].parent;
+      nodes[A].parent = C;
+      relink(C, A);
+      if (h(F) > h(G)) {
+        nodes[C].child2 = F;
+        nodes[A].child2 = G;
+        nodes[G].parent = A;
+        nodes[A].box = nodes[B].box.merge(nodes[G].box);
+        nodes[C].box = nodes[A].box.merge(nodes[F].box);
+        nodes[A].height = 1 + std::max(h(B), h(G));
+        nodes[C].height = 1 + std::max(h(A), h(F));
+      } else {
+        nodes[C].child2 = G;
+        nodes[A].child2 = F;
+        nodes[F].parent = A;
+        nodes[A].box = nodes[B].box.merge(nodes[F].box);
+        nodes[C].box = nodes[A].box.merge(nodes[G].box);
+        nodes[A].height = 1 + std::max(h(B), h(F));
+        nodes[C].height = 1 + std::max(h(A), h(G));
+      }
+      return C;
+    }
+    if (bal < -1) {  // B moves up
+      const uint32_t D = nodes[B].child1, E = nodes[B].child2;
+      nodes[B].child1 = A;
+      nodes[B].parent = nodes[A].parent;
+      nodes[A].parent = B;
+      relink(B, A);
+      if (h(D) > h(E)) {
+        nodes[B].child2 = D;
+        nodes[A].child1 = E;
+        nodes[E].parent = A;
+        nodes[A].box = nodes[C].box.merge(nodes[E].box);
+        nodes[B].box = nodes[A].box.merge(nodes[D].box);
+        nodes[A].height = 1 + std::max(h(C), h(E));
+        nodes[B].height = 1 + std::max(h(A), h(D));
+      } else {
+        nodes[B].child2 = E;
+        nodes[A].child1 = D;
+        nodes[D].parent = A;
+        nodes[A].box = nodes[C].box.merge(nodes[D].box);
+        nodes[B].box = nodes[A].box.merge(nodes[E].box);
+        nodes[A].height = 1 + std::max(h(C), h(D));
+        nodes[B].height = 1 + std::max(h(A), h(E));
+      }
+      return B;
+    }
+    return A;
+  }
+  // the parent of `up` (which now stands where `old` stood) points to `up`
+  void relink(uint32_t up, uint32_t old) {
+    const uint32_t p = nodes[up].parent;
+    if (p == kNull) {
+      root = up;
+    } else if (nodes[p].child1 == old) {
+      nodes[p].child1 = up;
+    } else {
+      nodes[p].child2 = up;
+    }
+  }
+  // query (src/broad_phase.zig:461-485): depth first, child2 popped first
+  void query(const AABB& q, std::vector<uint32_t>& out) const {
+    if (root == kNull) return;
+    std::vector<uint32_t> stack{root};
+    while (!stack.empty()) {
+      const uint32_t id = stack.back();
+      stack.pop_back();
+      if (id == kNull) continue;
+      const Node& n = nodes[id];
+      if (!n.box.overlaps(q)) continue;
+      if (n.isLeaf()) {
+        out.push_back(n.user);
+      } else {
+        stack.push_back(n.child1);
+        stack.push_back(n.child2);
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
 // src/system.zig — VoxelPhysicsSystem
 // ---------------------------------------------------------------------------
 struct Shared {
@@ -996,6 +1220,7 @@ struct World {
   // broad phase
   bool bp_dirty = true;
   std::vector<uint64_t> pairs;
+  AabbTree static_tree, dynamic_tree;  // maintained when pair_order is the tree order
   // narrow phase
   std::map<uint64_t, Manifold> cur, prev;  // ContactManager maps; std::map iterates by key
   BrickPairCache cache;
@@ -1082,6 +1307,8 @@ struct World {
     b.restitution = cfg.default_restitution;
     next_id++;
     b.fat = worldAABB(b).inflate(0.1f);  // insertBody (src/broad_phase.zig:150-163)
+    if (cfg.pair_order == VXP_PAIR_ORDER_TREE)  // addStaticBody / addDynamicBody
+      (type == VXP_BODY_STATIC ? static_tree : dynamic_tree).insertBody(b.id, b.fat);
     s.body = b;
     if (id_to_slot.size() <= b.id) id_to_slot.resize(b.id + 1, -1);
     id_to_slot[b.id] = (int32_t)h->index;
@@ -1104,6 +1331,10 @@ struct World {
     if (s.body.type == VXP_BODY_STATIC) stats.static_bodies--;
     id_to_slot[s.body.id] = -1;
     bp_dirty = true;
+    if (cfg.pair_order == VXP_PAIR_ORDER_TREE) {  // BroadPhase.removeBody
+      static_tree.removeBody(s.body.id);
+      dynamic_tree.removeBody(s.body.id);
+    }
     s.active = false;
     s.generation++;
     body_count--;
@@ -1128,7 +1359,27 @@ struct World {
       if (d.y < 0) ext.min.y += d.y; else ext.max.y += d.y;
       if (d.z < 0) ext.min.z += d.z; else ext.max.z += d.z;
       b.fat = ext;
+      if (cfg.pair_order == VXP_PAIR_ORDER_TREE) dynamic_tree.moveBody(b.id, ext);
       bp_dirty = true;
+    }
+  }
+
+  // findOverlappingPairs (src/broad_phase.zig:488-526,606-619) in the trees'
+  // order: every dynamic-tree leaf in node-index order queries the dynamic
+  // tree (keeping partners of higher id), then every dynamic-tree leaf queries
+  // the static tree.
+  void findPairsTree() {
+    pairs.clear();
+    std::vector<uint32_t> hits;
+    for (int pass = 0; pass < 2; pass++) {
+      const AabbTree& other = pass == 0 ? dynamic_tree : static_tree;
+      for (const AabbTree::Node& n : dynamic_tree.nodes) {
+        if (n.height < 0 || !n.isLeaf()) continue;
+        hits.clear();
+        other.query(n.box, hits);
+        for (uint32_t o : hits)
+          if (pass == 1 || o > n.user) pairs.push_back(pairKey(n.user, o));
+      }
     }
   }
 
@@ -1137,6 +1388,11 @@ struct World {
   // emitted in ascending pair key.
   void findPairs() {
     if (!bp_dirty) return;
+    if (cfg.pair_order == VXP_PAIR_ORDER_TREE) {
+      findPairsTree();
+      bp_dirty = false;
+      return;
+    }
     struct It {
       float x0;
       uint32_t slot;
@@ -1658,7 +1914,7 @@ extern "C" {
 const char* vxo_last_error(void) { return g_err.c_str(); }
 
 int vxo_system_create(const vxp_config* cfg, uint32_t n_worlds, vxo_system** out) {
-  if (!cfg || !out || n_worlds == 0) return VXP_ERR_ARGUMENT;
+  if (!cfg || !out || n_worlds == 0 || cfg->pair_order > VXP_PAIR_ORDER_TREE) return VXP_ERR_ARGUMENT;
   auto* s = new vxo_system();
   for (uint32_t i = 0; i < n_worlds; i++) {
     auto w = std::make_unique<World>();
@@ -1949,6 +2205,33 @@ int vxo_body_find_w(vxo_system* s, uint32_t world, uint32_t id, vxp_handle* out)
       return VXP_OK;
     }
   return VXP_ERR_INVALID_HANDLE;
+}
+
+int vxo_tree_get_w(vxo_system* s, uint32_t world, uint32_t which, vxp_tree_node* out, uint32_t cap,
+                   uint32_t* count, uint32_t* root) {
+  if (world >= s->worlds.size() || which > 1 || !count || !root) return VXP_ERR_ARGUMENT;
+  World& w = *s->worlds[world];
+  if (w.cfg.pair_order != VXP_PAIR_ORDER_TREE) return VXP_ERR_ARGUMENT;
+  const AabbTree& t = which ? w.dynamic_tree : w.static_tree;
+  *count = (uint32_t)t.nodes.size();
+  *root = t.root;
+  for (uint32_t i = 0; out && i < *count && i < cap; i++) {
+    const AabbTree::Node& n = t.nodes[i];
+    vxp_tree_node& o = out[i];
+    o.min[0] = n.box.min.x;
+    o.min[1] = n.box.min.y;
+    o.min[2] = n.box.min.z;
+    o.max[0] = n.box.max.x;
+    o.max[1] = n.box.max.y;
+    o.max[2] = n.box.max.z;
+    o.body_id = n.user;
+    o.parent = n.parent;
+    o.child1 = n.child1;
+    o.child2 = n.child2;
+    o.height = n.height;
+    o.pad = 0;
+  }
+  return VXP_OK;
 }
 
 int vxo_island_stats_get_w(vxo_system* s, uint32_t world, vxp_island_stats* out) {

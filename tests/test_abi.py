@@ -78,8 +78,8 @@ def test_invalid_arguments_rejected_without_device(lib):
     c = _abi.default_config()
     p = C.c_void_p()
     assert fn(C.byref(c), 0, C.byref(p)) == -5          # zero worlds
-    c.pair_cap = 512
-    assert fn(C.byref(c), 1, C.byref(p)) == -5          # CPU-restatement-only knob
+    c.pair_order = 7
+    assert fn(C.byref(c), 1, C.byref(p)) == -5          # unknown pair order
     c = _abi.default_config(brick_pair_cache_size=4096)
     assert fn(C.byref(c), 1, C.byref(p)) == -5
 

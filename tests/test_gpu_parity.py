@@ -121,14 +121,16 @@ def test_mass_gravity_solver_config_and_kinematic(product, oracle):
             assert compare.diff_stats(a.stats(), b.stats()) is None, fr
 
 
-def test_empty_collider_body_in_scene(product, oracle):
+@pytest.mark.parametrize("order", [_abi.PAIR_ORDER_KEY, _abi.PAIR_ORDER_TREE])
+def test_empty_collider_body_in_scene(product, oracle, order):
     """A body whose collider has no voxels (src/system.zig:1139-1158) falls under
     gravity; its broad-phase box is NaN (transform of the empty AABB) and must
-    not disturb the pair set of the other bodies."""
+    not disturb the pair set of the other bodies (nor, in the tree order, the
+    tree's descent and queries)."""
     sc = scenes.c0_small()
     runs = []
     for lib in (product, oracle):
-        s = vxp.from_scene(sc, lib=lib, cfg_over=dict(max_bodies=67))
+        s = vxp.from_scene(sc, lib=lib, cfg_over=dict(max_bodies=67, pair_order=order))
         cid = s.addCollider(voxel_size=0.1, brick_pos=np.zeros((0, 3), np.int32),
                             voxels=np.zeros((0, 512), np.uint8))
         h = s.createDynamicBody((0.2, 5.0, 0.1), (0.0, 0.0, 0.0, 1.0), cid, 1.0)

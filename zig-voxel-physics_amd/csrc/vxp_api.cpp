@@ -170,6 +170,12 @@ int down(const DevBuf& b, std::vector<T>& v) {
 int ensureDevice(vxp_system* s) {
   int rc = uploadColliders(s);
   if (rc) return rc;
+  if (s->t_host_dirty) {
+    if ((rc = up(s->d_t_nodes, s->t_nodes)) || (rc = up(s->d_t_heads, s->t_heads)) ||
+        (rc = up(s->d_t_leaf, s->t_leaf)))
+      return rc;
+    s->t_host_dirty = false;
+  }
   if (!s->host_dirty) return VXP_OK;
   if ((rc = up(s->b_pos, s->pos)) || (rc = up(s->b_rot, s->rot)) || (rc = up(s->b_lv, s->lv)) ||
       (rc = up(s->b_av, s->av)) || (rc = up(s->b_fat_lo, s->fat_lo)) ||
@@ -196,6 +202,32 @@ int ensureHost(vxp_system* s) {
     return rc;
   s->dev_newer = false;
   return VXP_OK;
+}
+
+// ---- broad-phase trees (pair order VXP_PAIR_ORDER_TREE) ----------------------
+// The host mirror is changed by body creation and destruction, the device copy
+// by the steps; each side fetches the other's changes before it edits.
+static bool treeMode(const vxp_system* s) { return s->cfg.pair_order == VXP_PAIR_ORDER_TREE; }
+
+int treesToHost(vxp_system* s) {
+  if (!s->t_dev_newer) return VXP_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(s->t_nodes.data(), s->d_t_nodes.p, s->t_nodes.size() * sizeof(TreeNode),
+                    hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(s->t_heads.data(), s->d_t_heads.p, s->t_heads.size() * sizeof(TreeHead),
+                    hipMemcpyDeviceToHost));
+  s->t_dev_newer = false;
+  return VXP_OK;
+}
+
+Tree hostTree(vxp_system* s, uint32_t w, uint32_t which) {
+  return Tree{s->t_nodes.data() + ((size_t)w * 2 + which) * s->t_nc,
+              s->t_heads.data() + (size_t)w * 2 + which, s->t_nc};
+}
+
+TreeSet treeSet(vxp_system* s) {
+  return TreeSet{s->d_t_nodes.as<TreeNode>(), s->d_t_heads.as<TreeHead>(), s->d_t_leaf.as<uint32_t>(),
+                 s->t_nc};
 }
 
 DevBodies devBodies(vxp_system* s) {
@@ -441,10 +473,47 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_step_ctr.ensure(kStepCounters * 4));
   launch_step_begin(st, cfg, B, ws, stats, nw, s->d_step_ctr.as<uint32_t>());
   profEnd(s, VXP_K_STEP_BEGIN, pe);
+  const bool tree = treeMode(s);
+  uint8_t* moved = nullptr;
+  if (tree) {
+    HIP_TRY(s->d_t_moved.ensure(ns));
+    moved = s->d_t_moved.as<uint8_t>();
+  }
   pe = profBegin(s);
-  launch_fat_aabb(st, cfg, B, C, ws, ns);
+  launch_fat_aabb(st, cfg, B, C, ws, ns, moved);
   profEnd(s, VXP_K_FAT_AABB, pe);
 
+  uint32_t wbits = 0;
+  while ((1u << wbits) < nw) wbits++;
+  uint32_t id_bits = 1;  // body ids are below id_cap
+  while ((1u << id_bits) < s->id_cap) id_bits++;
+  uint32_t npairs = 0;
+  if (tree) {
+    // ---- pair set in tree order: re-insert the moved leaves, then every
+    // dynamic leaf queries both trees (count, scan, write) --------------------
+    const TreeSet T = treeSet(s);
+    const uint32_t nq = nw * 2 * s->t_nc;
+    HIP_TRY(s->d_t_cnt.ensure((size_t)(nq + 1) * 4));
+    HIP_TRY(s->d_t_off.ensure((size_t)(nq + 1) * 4));
+    pe = profBegin(s);
+    launch_tree_update(st, cfg, B, ws, T, moved, nw, s->lds_limit);
+    launch_tree_pairs(st, ws, T, nw, s->d_t_cnt.as<uint32_t>(), nullptr, nullptr, nullptr, id_bits);
+    HIP_TRY(hipMemsetAsync(s->d_t_cnt.as<uint32_t>() + nq, 0, 4, st));
+    {
+      size_t cb = exclusive_scan_u32_bytes(nq + 1);
+      HIP_TRY(s->d_sort_tmp.ensure(cb));
+      HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_t_cnt.as<uint32_t>(),
+                                 s->d_t_off.as<uint32_t>(), nq + 1, st));
+    }
+    RC(readSizes(s, {s->d_t_off.as<uint32_t>() + nq}, &npairs));
+    HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
+    HIP_TRY(s->d_t_pos.ensure((size_t)std::max(npairs, 1u) * 4));
+    if (npairs)
+      launch_tree_pairs(st, ws, T, nw, nullptr, s->d_t_off.as<uint32_t>(),
+                        s->d_pairs_raw.as<uint64_t>(), s->d_t_pos.as<uint32_t>(), id_bits);
+    profEnd(s, VXP_K_PAIRS, pe);
+    s->t_dev_newer = true;
+  } else {
   // ---- pair set: sort-and-sweep --------------------------------------------
   HIP_TRY(s->d_skeys.ensure((size_t)ns * 8));
   HIP_TRY(s->d_svals.ensure((size_t)ns * 4));
@@ -453,8 +522,6 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_slo.ensure((size_t)ns * 16));
   HIP_TRY(s->d_shi.ensure((size_t)ns * 16));
   pe = profBegin(s);
-  uint32_t wbits = 0;
-  while ((1u << wbits) < nw) wbits++;
   if (!launch_sweep_sort(st, cfg, B, ws, s->d_svals2.as<uint32_t>(), nw)) {
     // worlds above the block sort's size: device-wide sort of (world << 32 | coordinate)
     // keys, which are zero above bit 32 + log2(worlds)
@@ -480,8 +547,6 @@ int stepAll(vxp_system* s) {
   const uint32_t swv = sweep_waves();
   HIP_TRY(s->d_swc.ensure((size_t)(swv + 1) * 4));
   HIP_TRY(s->d_swo.ensure((size_t)(swv + 1) * 4));
-  uint32_t id_bits = 1;  // body ids are below id_cap
-  while ((1u << id_bits) < s->id_cap) id_bits++;
   launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
                      s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), s->d_swc.as<uint32_t>(),
                      nullptr, nullptr, true, id_bits);
@@ -491,7 +556,6 @@ int stepAll(vxp_system* s) {
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_swc.as<uint32_t>(),
                                s->d_swo.as<uint32_t>(), swv + 1, st));
   }
-  uint32_t npairs = 0;
   RC(readSizes(s, {s->d_swo.as<uint32_t>() + swv}, &npairs));
   HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
   if (npairs)
@@ -499,16 +563,32 @@ int stepAll(vxp_system* s) {
                        s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), nullptr,
                        s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false, id_bits);
   profEnd(s, VXP_K_PAIRS, pe);
+  }
   s->n_pairs = npairs;
   HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
+  const uint32_t* perm = nullptr;  // key position -> tree position
   pe = profBegin(s);
   if (npairs) {
     // packed pair keys (world, a, b) are zero above bit 2 * id_bits + log2(worlds)
     const uint32_t kbits = 2 * id_bits + wbits;
-    size_t kb = sort_keys_u64_bytes(npairs, kbits);
-    HIP_TRY(s->d_sort_tmp.ensure(kb));
-    HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(), s->d_pairs.as<uint64_t>(),
-                          npairs, kbits, st));
+    if (tree) {
+      HIP_TRY(s->d_perm.ensure((size_t)npairs * 4));
+      HIP_TRY(s->d_pairs_tree.ensure((size_t)npairs * 8));
+      HIP_TRY(hipMemcpyAsync(s->d_pairs_tree.p, s->d_pairs_raw.p, (size_t)npairs * 8,
+                             hipMemcpyDeviceToDevice, st));
+      launch_expand_keys(st, s->d_pairs_tree.as<uint64_t>(), npairs, id_bits);
+      size_t kb = sort_pairs_u64_u32_bytes(npairs, kbits);
+      HIP_TRY(s->d_sort_tmp.ensure(kb));
+      HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(),
+                                 s->d_pairs.as<uint64_t>(), s->d_t_pos.as<uint32_t>(),
+                                 s->d_perm.as<uint32_t>(), npairs, kbits, st));
+      perm = s->d_perm.as<uint32_t>();
+    } else {
+      size_t kb = sort_keys_u64_bytes(npairs, kbits);
+      HIP_TRY(s->d_sort_tmp.ensure(kb));
+      HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(),
+                            s->d_pairs.as<uint64_t>(), npairs, kbits, st));
+    }
     launch_expand_keys(st, s->d_pairs.as<uint64_t>(), npairs, id_bits);
   }
   profEnd(s, VXP_K_PAIRS, pe);
@@ -530,20 +610,49 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_cstage.ensure((size_t)np1 * sizeof(CacheVal)));
   PairRec* pr = s->d_pr.as<PairRec>();
   pe = profBegin(s);
-  launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, s->d_cflag.as<uint32_t>(),
-                      s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(), s->d_wlist.as<uint32_t>(),
-                      s->d_step_ctr.as<uint32_t>());
+  // the filtered-pair cap (src/system.zig:594-616), counted in the narrow
+  // phase's order over the pairs that are not both asleep
+  const uint8_t* drop = nullptr;
+  if (s->cfg.pair_cap && npairs) {
+    HIP_TRY(s->d_awake.ensure((size_t)(npairs + 1) * 4));
+    HIP_TRY(s->d_arank.ensure((size_t)(npairs + 1) * 4));
+    HIP_TRY(s->d_drop.ensure((size_t)npairs));
+    launch_pair_awake(st, cfg, B, keys, perm, npairs, s->d_awake.as<uint32_t>());
+    size_t cb = exclusive_scan_u32_bytes(npairs + 1);
+    HIP_TRY(s->d_sort_tmp.ensure(cb));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_awake.as<uint32_t>(),
+                               s->d_arank.as<uint32_t>(), npairs + 1, st));
+    launch_pair_drop(st, keys, perm, npairs, s->d_awake.as<uint32_t>(), s->d_arank.as<uint32_t>(),
+                     s->cfg.pair_cap, s->d_drop.as<uint8_t>());
+    drop = s->d_drop.as<uint8_t>();
+  }
+  launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, drop,
+                      s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(),
+                      s->d_wlist.as<uint32_t>(), s->d_step_ctr.as<uint32_t>());
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
   pe = profBegin(s);
+  // cache-eligible flags in the narrow phase's order, and their positions
+  uint32_t* cflag_r = s->d_cflag.as<uint32_t>();
+  uint32_t* cidx_r = s->d_cidx.as<uint32_t>();
+  const uint32_t* order = nullptr;
+  if (tree && npairs) {
+    HIP_TRY(s->d_inv.ensure((size_t)npairs * 4));
+    HIP_TRY(s->d_cflag_r.ensure((size_t)(npairs + 1) * 4));
+    HIP_TRY(s->d_cidx_r.ensure((size_t)(npairs + 1) * 4));
+    launch_invert(st, perm, npairs, s->d_inv.as<uint32_t>());
+    launch_gather_flags(st, s->d_cflag.as<uint32_t>(), s->d_inv.as<uint32_t>(), npairs,
+                        s->d_cflag_r.as<uint32_t>());
+    cflag_r = s->d_cflag_r.as<uint32_t>();
+    cidx_r = s->d_cidx_r.as<uint32_t>();
+    order = s->d_inv.as<uint32_t>();
+  }
   if (npairs) {
     size_t cb = exclusive_scan_u32_bytes(npairs + 1);
     HIP_TRY(s->d_sort_tmp.ensure(cb));
-    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_cflag.as<uint32_t>(),
-                               s->d_cidx.as<uint32_t>(), npairs + 1, st));
+    HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, cflag_r, cidx_r, npairs + 1, st));
   }
-  launch_cache_replay(st, cfg, ws, stats, pr, s->d_cache_keys.as<CacheKeyRec>(),
-                      s->d_cflag.as<uint32_t>(), s->d_cidx.as<uint32_t>(),
-                      s->d_clist.as<uint32_t>(), npairs, nw);
+  launch_cache_replay(st, cfg, ws, stats, pr, s->d_cache_keys.as<CacheKeyRec>(), cflag_r, cidx_r,
+                      order, s->d_clist.as<uint32_t>(), npairs, nw);
   profEnd(s, VXP_K_CACHE_REPLAY, pe);
   uint32_t total_wi = 0;
   if (npairs) {
@@ -706,11 +815,16 @@ int checkErrors(vxp_system* s) {
   profFlush(s);
   std::vector<WorldState> ws(s->nw);
   HIP_TRY(hipMemcpy(ws.data(), s->d_ws.p, ws.size() * sizeof(WorldState), hipMemcpyDeviceToHost));
-  for (auto& w : ws)
-    if (w.error) {
+  for (auto& w : ws) {
+    if (w.error & kErrColours) {
       g_err = "solver colouring exceeded 1024 colours";
       return VXP_ERR_CAPACITY;
     }
+    if (w.error & kErrTreeStack) {
+      g_err = "a broad-phase tree query exceeded its stack (tree deeper than 63 levels)";
+      return VXP_ERR_CAPACITY;
+    }
+  }
   return VXP_OK;
 }
 
@@ -819,6 +933,7 @@ int vxp_config_default(vxp_config* c) {
   c->use_brick_pair_cache = 1;
   c->brick_pair_cache_size = 1024;
   c->pair_cap = 0;
+  c->pair_order = VXP_PAIR_ORDER_KEY;
   c->solve_order = VXP_SOLVE_ORDER_COLOR;
   c->cpu_threads = 1;
   return VXP_OK;
@@ -829,8 +944,8 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
     g_err = "invalid configuration";
     return VXP_ERR_ARGUMENT;
   }
-  if (cfg->pair_cap != 0) {
-    g_err = "pair_cap is only supported by the CPU restatement";
+  if (cfg->pair_order > VXP_PAIR_ORDER_TREE) {
+    g_err = "unknown pair_order";
     return VXP_ERR_ARGUMENT;
   }
   if (cfg->solve_order != VXP_SOLVE_ORDER_COLOR) {
@@ -853,6 +968,13 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
   s->MB = cfg->max_bodies;
   s->id_cap = cfg->max_bodies + 2;
   size_t ns = slots(s);
+  if (treeMode(s)) {  // leaves <= max_bodies, internal nodes one fewer
+    s->t_nc = 2 * s->MB + 2;
+    s->t_nodes.assign((size_t)n_worlds * 2 * s->t_nc, TreeNode{});
+    s->t_heads.assign((size_t)n_worlds * 2, TreeHead{kTreeNull, kTreeNull, 0, 0});
+    s->t_leaf.assign(ns, kTreeNull);
+    s->t_host_dirty = true;
+  }
   s->pos.assign(ns, make_float4(0, 0, 0, 0));
   s->rot.assign(ns, make_float4(0, 0, 0, 1));
   s->lv.assign(ns, make_float4(0, 0, 0, 0));
@@ -941,7 +1063,10 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->s_upar, &s->s_ucnt, &s->s_uoff, &s->s_ubody, &s->s_uorder, &s->s_units,
                     &s->s_uqueue, &s->s_pcnt, &s->s_fq, &s->s_cend, &s->d_cb_pos, &s->d_cb_vox,
                     &s->d_cb_cnt, &s->d_cb_bounds, &s->d_cb_box, &s->d_cb_out, &s->d_cb_solid,
-                    &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep};
+                    &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
+                    &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off,
+                    &s->d_t_pos, &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake,
+                    &s->d_arank, &s->d_drop, &s->d_cflag_r, &s->d_cidx_r};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
@@ -1156,6 +1281,12 @@ static int createBody(vxp_system* s, uint32_t w, uint32_t type, const float pos[
   Box fat = boxInflate(wb, 0.1f);
   s->fat_lo[g] = make_float4(fat.lo.x, fat.lo.y, fat.lo.z, 0.0f);
   s->fat_hi[g] = make_float4(fat.hi.x, fat.hi.y, fat.hi.z, 0.0f);
+  if (treeMode(s)) {  // addStaticBody / addDynamicBody (src/broad_phase.zig:580-590)
+    RC(treesToHost(s));
+    const float lo[3] = {fat.lo.x, fat.lo.y, fat.lo.z}, hi[3] = {fat.hi.x, fat.hi.y, fat.hi.z};
+    s->t_leaf[g] = treeInsertBody(hostTree(s, w, type == VXP_BODY_STATIC ? 0 : 1), bid, lo, hi);
+    s->t_host_dirty = true;
+  }
   s->idmap[(size_t)w * s->id_cap + bid] = (int32_t)index;
   s->n_total[w]++;
   if (type == VXP_BODY_DYNAMIC) s->n_dyn[w]++;
@@ -1229,6 +1360,12 @@ int vxp_body_destroy_w(vxp_system* s, uint32_t w, vxp_handle h) {
   if (s->type[g] == VXP_BODY_DYNAMIC) s->n_dyn[w]--;
   if (s->type[g] == VXP_BODY_STATIC) s->n_static[w]--;
   s->idmap[(size_t)w * s->id_cap + s->id[g]] = -1;
+  if (treeMode(s)) {  // BroadPhase.removeBody (src/broad_phase.zig:592-597)
+    RC(treesToHost(s));
+    treeRemoveBody(hostTree(s, w, s->type[g] == VXP_BODY_STATIC ? 0 : 1), s->t_leaf[g]);
+    s->t_leaf[g] = kTreeNull;
+    s->t_host_dirty = true;
+  }
   s->active[g] = 0;
   s->generation[g]++;
   s->body_count[w]--;
@@ -1471,8 +1608,9 @@ int vxp_pairs_get_w(vxp_system* s, uint32_t w, uint64_t* out, uint32_t cap, uint
   if (!s || !count || w >= s->nw) return VXP_ERR_ARGUMENT;
   HIP_TRY(hipStreamSynchronize(s->stream));
   std::vector<uint64_t> all(s->n_pairs);
+  const DevBuf& src = treeMode(s) ? s->d_pairs_tree : s->d_pairs;  // the narrow phase's order
   if (s->n_pairs)
-    HIP_TRY(hipMemcpy(all.data(), s->d_pairs.p, all.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(all.data(), src.p, all.size() * 8, hipMemcpyDeviceToHost));
   uint32_t n = 0;
   for (uint64_t k : all) {
     if (keyWorld(k) != w) continue;
@@ -1570,6 +1708,18 @@ int vxp_body_wake_island_w(vxp_system* s, uint32_t w, vxp_handle h) {
                       4, hipMemcpyHostToDevice));
   }
   return vxp_body_op_w(s, w, h, VXP_OP_WAKE_UP, nullptr, nullptr);
+}
+
+int vxp_tree_get_w(vxp_system* s, uint32_t w, uint32_t which, vxp_tree_node* out, uint32_t cap,
+                   uint32_t* count, uint32_t* root) {
+  if (!s || !count || !root || w >= s->nw || which > 1 || !treeMode(s)) return VXP_ERR_ARGUMENT;
+  RC(treesToHost(s));
+  const Tree t = hostTree(s, w, which);
+  static_assert(sizeof(vxp_tree_node) == sizeof(TreeNode), "tree node export");
+  *count = t.h->len;
+  *root = t.h->root;
+  if (out) std::memcpy(out, t.n, (size_t)std::min(cap, t.h->len) * sizeof(TreeNode));
+  return VXP_OK;
 }
 
 int vxp_coloring_get_w(vxp_system* s, uint32_t w, uint8_t* colors, uint32_t cap, uint32_t* count) {

@@ -7,6 +7,7 @@
 
 #include "../../include/vxp.h"
 #include "vxp_math.hpp"
+#include "vxp_tree.hpp"
 
 namespace vxd {
 
@@ -92,6 +93,23 @@ struct WorldState {
   uint32_t isl_need;                  // needs_rebuild
   vxp_island_stats isl;               // stats after the last sleep update
 };
+
+// The broad-phase trees of every world: tree (w, which) is nodes[(2w + which)
+// * nc ..], head heads[2w + which]; which 0 = static, 1 = dynamic.  leaf[g] is
+// the node of body slot g in its tree.
+struct TreeSet {
+  TreeNode* nodes;
+  TreeHead* heads;
+  uint32_t* leaf;
+  uint32_t nc;
+  __host__ __device__ Tree tree(uint32_t w, uint32_t which) const {
+    return Tree{nodes + ((size_t)w * 2 + which) * nc, heads + (size_t)w * 2 + which, nc};
+  }
+};
+
+// WorldState.error bits
+constexpr uint32_t kErrColours = 1;     // solver colouring past its colour limit
+constexpr uint32_t kErrTreeStack = 2;   // a tree query deeper than its stack
 
 // Id-indexed island arrays, [world * id_cap + body id]: union-find parents
 // (labels after a step), endpoint flags, component sizes, and the sleep timer

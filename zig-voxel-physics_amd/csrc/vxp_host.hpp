@@ -136,6 +136,15 @@ struct vxp_system {
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
   DevBuf d_isl_timer, d_isl_sleep;  // island sleep state (persistent)
   uint32_t isl_cap = 0;             // id_cap the island arrays are laid out with
+  // broad-phase trees (pair order VXP_PAIR_ORDER_TREE): host mirror, device
+  // copy, and the per-step buffers of the tree order and the pair cap
+  std::vector<TreeNode> t_nodes;
+  std::vector<TreeHead> t_heads;
+  std::vector<uint32_t> t_leaf;
+  uint32_t t_nc = 0;
+  bool t_host_dirty = false, t_dev_newer = false;
+  DevBuf d_t_nodes, d_t_heads, d_t_leaf, d_t_moved, d_t_cnt, d_t_off, d_t_pos, d_perm, d_inv,
+      d_pairs_tree, d_awake, d_arank, d_drop, d_cflag_r, d_cidx_r;
   DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
       s_pts, s_prio, s_tstamp, s_upar, s_ucnt, s_uoff, s_ubody, s_uorder, s_units, s_uqueue,
       s_pcnt, s_fq, s_cend;

@@ -286,10 +286,13 @@ __device__ Box worldBoxOf(const DevBodies& B, const DevColliders& C, uint32_t g)
   return boxTransform(colliderBox(C.hdr[c]), p, f4q(B.rot[g]));
 }
 
+// moved (tree order only): 1 where the fat box changed, the bodies the tree
+// re-inserts (BroadPhase.updateBody returning true).
 __global__ void k_fat_aabb(DevCfg cfg, DevBodies B, DevColliders C, const WorldState* ws,
-                           uint32_t n_slots) {
+                           uint32_t n_slots, uint8_t* moved) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_slots) return;
+  if (moved) moved[g] = 0;
   if (ws[g / cfg.max_bodies].skip) return;
   if (!B.active[g] || B.type[g] != VXP_BODY_DYNAMIC || B.sleeping[g]) return;
   Box aabb = worldBoxOf(B, C, g);
@@ -306,6 +309,7 @@ __global__ void k_fat_aabb(DevCfg cfg, DevBodies B, DevColliders C, const WorldS
   if (d.z < 0) e.lo.z += d.z; else e.hi.z += d.z;
   B.fat_lo[g] = v4(e.lo);
   B.fat_hi[g] = v4(e.hi);
+  if (moved) moved[g] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -596,7 +600,8 @@ struct PairCounts {
 
 __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                                   WorldState* ws, const uint64_t* keys, uint32_t n_pairs,
-                                  uint32_t p, PairRec* pr, const DevManifold* prev) {
+                                  uint32_t p, PairRec* pr, const DevManifold* prev,
+                                  const uint8_t* drop) {
   PairCounts cnt{0, 0, 0};
   uint64_t key = keys[p];
   uint32_t w = keyWorld(key), ida = keyA(key), idb = keyB(key);
@@ -631,6 +636,10 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
       r.flag = PF_CARRY;
       r.aux = (uint32_t)j;
     }
+    pr[p] = r;
+    return cnt;
+  }
+  if (drop && drop[p]) {  // past the filtered-pair array (src/system.zig:594,611-614)
     pr[p] = r;
     return cnt;
   }
@@ -704,8 +713,9 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
 // the pairs that the reduction must visit to wlist (any order).
 __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
-                               const DevManifold* prev, uint32_t* cflag, uint32_t* produce,
-                               RStat* rstat, uint32_t* wlist, uint32_t* step_ctr) {
+                               const DevManifold* prev, const uint8_t* drop, uint32_t* cflag,
+                               uint32_t* produce, RStat* rstat, uint32_t* wlist,
+                               uint32_t* step_ctr) {
   __shared__ uint32_t s_n, s_base;
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   PairCounts c{0, 0, 0};
@@ -715,7 +725,7 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
   __syncthreads();
   uint32_t pos = 0;
   if (p < n_pairs) {
-    c = pairProcess(cfg, B, C, ws, keys, n_pairs, p, pr, prev);
+    c = pairProcess(cfg, B, C, ws, keys, n_pairs, p, pr, prev, drop);
     w = keyWorld(keys[p]);
     const uint32_t f = pr[p].flag;
     cflag[p] = f == PF_CACHE;
@@ -770,11 +780,13 @@ __device__ __forceinline__ uint32_t cacheSlot(uint32_t start, uint32_t lane, uin
   return i >= cap ? i - cap : i;
 }
 
-// Cache-eligible pairs in pair order: clist[cidx[p]] = p.
-__global__ void k_cache_scatter(const uint32_t* cflag, const uint32_t* cidx, uint32_t n_pairs,
-                                uint32_t* clist) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n_pairs && cflag[p]) clist[cidx[p]] = p;
+// Cache-eligible pairs in the narrow phase's order: position i of that order
+// holds pair order[i] (i itself in key order, order == nullptr); cflag and
+// cidx are indexed by position: clist[cidx[i]] = pair.
+__global__ void k_cache_scatter(const uint32_t* cflag, const uint32_t* cidx, const uint32_t* order,
+                                uint32_t n_pairs, uint32_t* clist) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_pairs && cflag[i]) clist[cidx[i]] = order ? order[i] : i;
 }
 
 // One wave per world replays the gets/puts of its cache-eligible pairs
@@ -2190,7 +2202,7 @@ __device__ void colourManifolds(uint32_t M, uint32_t nadj, uint32_t m0, uint32_t
       }
       if (col >= kMaxColors) {
         col = kMaxColors - 1;
-        atomicOr(&x.error, 1u);
+        atomicOr(&x.error, kErrColours);
       }
       __atomic_store_n(&cc[i], (int32_t)col, __ATOMIC_RELAXED);
       atomicMax(&s_ncol, col + 1);
@@ -3214,9 +3226,174 @@ void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
                        vxp_stats* st, uint32_t nw, uint32_t* step_ctr) {
   k_step_begin<<<nw, 256, 0, s>>>(cfg, B, ws, st, nw, step_ctr);
 }
+// ---------------------------------------------------------------------------
+// Broad-phase trees (pair order VXP_PAIR_ORDER_TREE)
+// ---------------------------------------------------------------------------
+// BroadPhase.updateBody for the moved bodies of one world, in slot order
+// (src/system.zig:570-583): the block compacts the moved slots 256 at a time
+// and thread 0 re-inserts them; the dynamic tree is staged in LDS when it fits.
+__global__ __launch_bounds__(256) void k_tree_update(DevCfg cfg, DevBodies B, const WorldState* ws,
+                                                     TreeSet T, const uint8_t* moved,
+                                                     size_t lds_bytes) {
+  extern __shared__ TreeNode tn[];
+  __shared__ uint32_t list[256], wcnt[4], n_list;
+  __shared__ TreeHead head;
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  if (ws[w].skip) return;
+  Tree g = T.tree(w, 1);
+  const bool L = (size_t)T.nc * sizeof(TreeNode) <= lds_bytes;
+  if (tid == 0) head = *g.h;
+  __syncthreads();
+  const uint32_t len = head.len;
+  if (L)
+    for (uint32_t i = tid; i < len; i += blockDim.x) tn[i] = g.n[i];
+  Tree t{L ? tn : g.n, &head, T.nc};
+  const uint32_t lane = tid & (kWave - 1), wave = tid / kWave;
+  for (uint32_t b0 = 0; b0 < cfg.max_bodies; b0 += 256) {
+    const uint32_t slot = b0 + tid;
+    const size_t gs = (size_t)w * cfg.max_bodies + slot;
+    const bool m = slot < cfg.max_bodies && moved[gs];
+    const uint64_t bal = __ballot(m);
+    if (lane == 0) wcnt[wave] = __popcll(bal);
+    __syncthreads();
+    if (m) {
+      uint32_t at = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+      for (uint32_t k = 0; k < wave; k++) at += wcnt[k];
+      list[at] = slot;
+    }
+    if (tid == 0) n_list = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    __syncthreads();
+    if (tid == 0)
+      for (uint32_t k = 0; k < n_list; k++) {
+        const size_t gb = (size_t)w * cfg.max_bodies + list[k];
+        const float4 lo = B.fat_lo[gb], hi = B.fat_hi[gb];
+        const float flo[3] = {lo.x, lo.y, lo.z}, fhi[3] = {hi.x, hi.y, hi.z};
+        treeMoveLeaf(t, T.leaf[gb], flo, fhi);
+      }
+    __syncthreads();
+  }
+  if (L)  // re-insertions free a node before they allocate one, but copy what head.len says
+    for (uint32_t i = tid; i < head.len; i += blockDim.x) g.n[i] = tn[i];
+  if (tid == 0) *g.h = head;
+}
+
+// findOverlappingPairs + queryAgainstTree (src/broad_phase.zig:488-526,606-619)
+// for one dynamic-tree node: position (w, q, i), q = 0 self pairs (partners
+// of higher id), q = 1 against the static tree.  Offsets come from an
+// exclusive scan of the counts in (world, q, node) order, which is the
+// reference's emission order.
+template <bool kWrite>
+__global__ __launch_bounds__(256) void k_tree_pairs(WorldState* ws, TreeSet T, uint32_t nw,
+                                                    uint32_t* counts, const uint32_t* offs,
+                                                    uint64_t* out, uint32_t* pos,
+                                                    uint32_t id_bits) {
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per_w = 2 * (size_t)T.nc;
+  if (gid >= nw * per_w) return;
+  const uint32_t w = (uint32_t)(gid / per_w), q = (uint32_t)((gid % per_w) / T.nc),
+                 i = (uint32_t)(gid % T.nc);
+  const Tree d = T.tree(w, 1);
+  uint32_t n = 0;
+  if (i < d.h->len) {
+    const TreeNode me = d.n[i];
+    if (me.height >= 0 && treeLeaf(me)) {
+      const Tree o = T.tree(w, q == 0 ? 1 : 0);
+      const size_t base = kWrite ? offs[gid] : 0;
+      const bool ok = treeQuery(o.n, o.h->root, me.lo, me.hi, [&](uint32_t other) {
+        if (q == 1 || other > me.user) {
+          if (kWrite) {
+            const uint64_t a = min(me.user, other), b = max(me.user, other);
+            out[base + n] = ((uint64_t)w << (2 * id_bits)) | (a << id_bits) | b;
+            pos[base + n] = (uint32_t)(base + n);
+          }
+          n++;
+        }
+      });
+      if (!ok) atomicOr(&ws[w].error, kErrTreeStack);
+    }
+  }
+  if (!kWrite) counts[gid] = n;
+}
+
+__global__ void k_pair_awake(DevCfg cfg, DevBodies B, const uint64_t* keys, const uint32_t* perm,
+                             uint32_t n, uint32_t* awake) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > n) return;
+  if (p == n) {
+    awake[n] = 0;
+    return;
+  }
+  const uint64_t k = keys[p];
+  const uint32_t w = keyWorld(k);
+  const int32_t sa = B.idmap[(size_t)w * cfg.id_cap + keyA(k)];
+  const int32_t sb = B.idmap[(size_t)w * cfg.id_cap + keyB(k)];
+  const bool asl = sa >= 0 && B.sleeping[(size_t)w * cfg.max_bodies + sa];
+  const bool bsl = sb >= 0 && B.sleeping[(size_t)w * cfg.max_bodies + sb];
+  awake[perm ? perm[p] : p] = !(asl && bsl);
+}
+
+__global__ void k_pair_drop(const uint64_t* keys, const uint32_t* perm, uint32_t n,
+                            const uint32_t* awake, const uint32_t* rank, uint32_t cap,
+                            uint8_t* drop) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  // first pair of this world: its segment starts at the same index in either order
+  const uint32_t w = keyWorld(keys[p]);
+  uint32_t lo = 0, hi = p;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (keyWorld(keys[mid]) < w) lo = mid + 1; else hi = mid;
+  }
+  const uint32_t t = perm ? perm[p] : p;
+  drop[p] = awake[t] && rank[t] - rank[lo] >= cap;
+}
+
+__global__ void k_gather_flags(const uint32_t* in, const uint32_t* idx, uint32_t n, uint32_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+  if (i == n) out[n] = 0;
+}
+
+__global__ void k_invert(const uint32_t* perm, uint32_t n, uint32_t* inv) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) inv[perm[p]] = p;
+}
+
+void launch_tree_update(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                        const TreeSet& T, const uint8_t* moved, uint32_t nw, size_t lds_limit) {
+  const size_t need = (size_t)T.nc * sizeof(TreeNode);
+  const size_t lds = need <= lds_limit ? need : 0;
+  k_tree_update<<<nw, 256, lds, s>>>(cfg, B, ws, T, moved, lds);
+}
+void launch_tree_pairs(hipStream_t s, WorldState* ws, const TreeSet& T, uint32_t nw,
+                       uint32_t* counts, const uint32_t* offs, uint64_t* out, uint32_t* pos,
+                       uint32_t id_bits) {
+  const size_t n = (size_t)nw * 2 * T.nc;
+  const uint32_t blocks = (uint32_t)((n + 255) / 256);
+  if (out)
+    k_tree_pairs<true><<<blocks, 256, 0, s>>>(ws, T, nw, counts, offs, out, pos, id_bits);
+  else
+    k_tree_pairs<false><<<blocks, 256, 0, s>>>(ws, T, nw, counts, offs, out, pos, id_bits);
+}
+void launch_pair_awake(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint64_t* keys,
+                       const uint32_t* perm, uint32_t n, uint32_t* awake) {
+  k_pair_awake<<<cdiv(n + 1, 256), 256, 0, s>>>(cfg, B, keys, perm, n, awake);
+}
+void launch_pair_drop(hipStream_t s, const uint64_t* keys, const uint32_t* perm, uint32_t n,
+                      const uint32_t* awake, const uint32_t* rank, uint32_t cap, uint8_t* drop) {
+  if (n) k_pair_drop<<<cdiv(n, 256), 256, 0, s>>>(keys, perm, n, awake, rank, cap, drop);
+}
+void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t n,
+                         uint32_t* out) {
+  k_gather_flags<<<cdiv(n + 1, 256), 256, 0, s>>>(in, idx, n, out);
+}
+void launch_invert(hipStream_t s, const uint32_t* perm, uint32_t n, uint32_t* inv) {
+  if (n) k_invert<<<cdiv(n, 256), 256, 0, s>>>(perm, n, inv);
+}
+
 void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                     const WorldState* ws, uint32_t n_slots) {
-  k_fat_aabb<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, C, ws, n_slots);
+                     const WorldState* ws, uint32_t n_slots, uint8_t* moved) {
+  k_fat_aabb<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, C, ws, n_slots, moved);
 }
 void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw) {
@@ -3267,16 +3444,18 @@ void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, 
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
-                         uint32_t* cflag, uint32_t* produce, RStat* rstat, uint32_t* wlist,
-                         uint32_t* step_ctr) {
+                         const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
+                         uint32_t* wlist, uint32_t* step_ctr) {
   if (n)
-    k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, cflag,
+    k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag,
                                                 produce, rstat, wlist, step_ctr);
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
-                         const uint32_t* cidx, uint32_t* clist, uint32_t n_pairs, uint32_t nw) {
-  if (n_pairs) k_cache_scatter<<<cdiv(n_pairs, 256), 256, 0, s>>>(cflag, cidx, n_pairs, clist);
+                         const uint32_t* cidx, const uint32_t* order, uint32_t* clist,
+                         uint32_t n_pairs, uint32_t nw) {
+  if (n_pairs)
+    k_cache_scatter<<<cdiv(n_pairs, 256), 256, 0, s>>>(cflag, cidx, order, n_pairs, clist);
   k_cache_replay<<<nw, 64, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(cfg, ws, st, pr, keys, cidx,
                                                                       clist, nw);
 }

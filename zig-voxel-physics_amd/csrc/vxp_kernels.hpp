@@ -10,7 +10,7 @@ void launch_update_begin(hipStream_t s, vxp_stats* st, uint32_t nw, float acc);
 void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, uint32_t nw, uint32_t* step_ctr);
 void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                     const WorldState* ws, uint32_t n_slots);
+                     const WorldState* ws, uint32_t n_slots, uint8_t* moved);
 void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw);
 // Per-world block sort of the sweep order (k_world_axis included); false when
@@ -29,11 +29,32 @@ void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_b
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
-                         uint32_t* cflag, uint32_t* produce, RStat* rstat, uint32_t* wlist,
-                         uint32_t* step_ctr);
+                         const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
+                         uint32_t* wlist, uint32_t* step_ctr);
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
-                         const uint32_t* cidx, uint32_t* clist, uint32_t n_pairs, uint32_t nw);
+                         const uint32_t* cidx, const uint32_t* order, uint32_t* clist,
+                         uint32_t n_pairs, uint32_t nw);
+// Broad-phase trees (pair order VXP_PAIR_ORDER_TREE): per world, re-insert the
+// leaves of the dynamic tree whose fat box moved, in slot order.
+void launch_tree_update(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                        const TreeSet& T, const uint8_t* moved, uint32_t nw, size_t lds_limit);
+// Pairs in tree order: count (out == nullptr) or write packed keys
+// (world, a, b) and their positions; counts/offsets per (world, query, node).
+void launch_tree_pairs(hipStream_t s, WorldState* ws, const TreeSet& T, uint32_t nw,
+                       uint32_t* counts, const uint32_t* offs, uint64_t* out, uint32_t* pos,
+                       uint32_t id_bits);
+// The pair cap in the narrow phase's order: awake[order position] of each
+// pair (not both bodies asleep), then drop[p] for awake pairs at or past the
+// cap within their world (rank = scanned awake flags).
+void launch_pair_awake(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const uint64_t* keys,
+                       const uint32_t* perm, uint32_t n, uint32_t* awake);
+void launch_pair_drop(hipStream_t s, const uint64_t* keys, const uint32_t* perm, uint32_t n,
+                      const uint32_t* awake, const uint32_t* rank, uint32_t cap, uint8_t* drop);
+// out[i] = in[idx[i]] for i < n, out[n] = 0; inv[perm[p]] = p
+void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t n,
+                         uint32_t* out);
+void launch_invert(hipStream_t s, const uint32_t* perm, uint32_t n, uint32_t* inv);
 void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts);
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                   const PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,

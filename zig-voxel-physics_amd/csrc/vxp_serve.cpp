@@ -231,6 +231,10 @@ extern "C" {
 int vxp_server_create(const vxp_config* cfg, uint32_t max_worlds, uint32_t batch_slots,
                       uint64_t hbm_budget_bytes, vxp_server** out) {
   if (!cfg || !out || !max_worlds || !batch_slots) return VXP_ERR_ARGUMENT;
+  if (cfg->pair_order != VXP_PAIR_ORDER_KEY) {
+    g_err = "the server keeps no broad-phase trees: pair_order must be VXP_PAIR_ORDER_KEY";
+    return VXP_ERR_ARGUMENT;
+  }
   const uint64_t MB = cfg->max_bodies, ic = MB + 2, cap = cacheCap(*cfg);
   // resident bytes of one world: slot arrays, id map, counters, cache tables
   // and a reserve of two manifolds per body slot

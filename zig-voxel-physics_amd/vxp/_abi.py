@@ -18,6 +18,7 @@ VXP_ERR_INVALID_HANDLE, VXP_ERR_CAPACITY, VXP_ERR_OOM, VXP_ERR_DEVICE, VXP_ERR_A
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
 EMPTY, CORNER, EDGE, FACE, INSIDE = 0, 1, 2, 3, 4
 SOLVE_ORDER_COLOR, SOLVE_ORDER_CANONICAL = 0, 1
+PAIR_ORDER_KEY, PAIR_ORDER_TREE = 0, 1
 NO_COLLIDER = 0xFFFFFFFF
 
 
@@ -59,6 +60,7 @@ class Config(C.Structure):
         ("pair_cap", C.c_uint32),
         ("solve_order", C.c_uint32),
         ("cpu_threads", C.c_uint32),
+        ("pair_order", C.c_uint32),
     ]
 
 
@@ -96,6 +98,7 @@ def default_config(**over):
     c.pair_cap = 0
     c.solve_order = SOLVE_ORDER_COLOR
     c.cpu_threads = 1
+    c.pair_order = PAIR_ORDER_KEY
     for k, v in over.items():
         if k == "gravity":
             c.gravity[:] = v
@@ -129,6 +132,13 @@ class IslandStats(C.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TreeNode(C.Structure):
+    """TreeNode (src/broad_phase.zig:40-62) as vxp_tree_get_w exports it."""
+    _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3), ("body_id", C.c_uint32),
+                ("parent", C.c_uint32), ("child1", C.c_uint32), ("child2", C.c_uint32),
+                ("height", C.c_int32), ("pad", C.c_uint32)]
 
 
 class Contact(C.Structure):
@@ -206,7 +216,7 @@ OP_APPLY_FORCE, OP_APPLY_FORCE_AT_POINT, OP_APPLY_IMPULSE, OP_APPLY_IMPULSE_AT_P
     OP_APPLY_TORQUE, OP_APPLY_ANGULAR_IMPULSE, OP_SET_LINEAR_VELOCITY, OP_SET_ANGULAR_VELOCITY, \
     OP_SET_POSITION, OP_SET_ROTATION, OP_WAKE_UP, OP_SLEEP = range(12)
 
-assert C.sizeof(Config) == 100, C.sizeof(Config)
+assert C.sizeof(Config) == 104, C.sizeof(Config)
 assert C.sizeof(Contact) == 88, C.sizeof(Contact)
 assert C.sizeof(Manifold) == 376, C.sizeof(Manifold)
 assert C.sizeof(BodyState) == 64, C.sizeof(BodyState)
@@ -256,6 +266,8 @@ SIGNATURES = {
     "coloring_get_w": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint8), C.c_uint32,
                                  C.POINTER(C.c_uint32)]),
     "island_stats_get_w": (C.c_int, [_P, C.c_uint32, C.POINTER(IslandStats)]),
+    "tree_get_w": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(TreeNode), C.c_uint32,
+                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "body_in_sleeping_island_w": (C.c_int, [_P, C.c_uint32, Handle, C.POINTER(C.c_int)]),
     "body_wake_island_w": (C.c_int, [_P, C.c_uint32, Handle]),
     "last_error": (C.c_char_p, []),

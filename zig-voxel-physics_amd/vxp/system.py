@@ -242,6 +242,20 @@ class VoxelPhysicsSystem:
                                               C.byref(cnt)), "islands")
         return ids[:n], lab[:n]
 
+    def tree(self, which, world=0):
+        """The static (which=0) or dynamic (1) broad-phase tree under the tree
+        pair order: (nodes as a structured array in index order, root)."""
+        cnt, root = C.c_uint32(), C.c_uint32()
+        self.lib.check(self.lib.tree_get_w(self.ptr, world, which, None, 0, C.byref(cnt),
+                                           C.byref(root)), "tree_get")
+        arr = (_abi.TreeNode * max(cnt.value, 1))()
+        self.lib.check(self.lib.tree_get_w(self.ptr, world, which, arr, cnt.value, C.byref(cnt),
+                                           C.byref(root)), "tree_get")
+        out = np.frombuffer(arr, dtype=np.dtype([
+            ("min", "<f4", 3), ("max", "<f4", 3), ("body_id", "<u4"), ("parent", "<u4"),
+            ("child1", "<u4"), ("child2", "<u4"), ("height", "<i4"), ("pad", "<u4")]))
+        return out[: cnt.value].copy(), root.value
+
     def coloring(self, world=0):
         cnt = C.c_uint32()
         self.lib.check(self.lib.coloring_get_w(self.ptr, world, None, 0, C.byref(cnt)), "coloring")
