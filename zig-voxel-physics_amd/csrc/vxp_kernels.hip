@@ -774,6 +774,16 @@ struct CacheWin {
 __device__ __forceinline__ CacheWin cacheWin(const CacheKeyRec& e) {
   return CacheWin{e.a, e.b, e.valid, e.th, e.last_used};
 }
+// Minimum of v over each row of 16 lanes, in every lane of the row (DPP: quad
+// swaps, then rotations by 4 and 8 within the row).  The whole wave must be active.
+__device__ __forceinline__ uint32_t rowMin16(uint32_t v) {
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));   // [1,0,3,2]
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));   // [2,3,0,1]
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, false));  // row_ror:4
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false));  // row_ror:8
+  return v;
+}
+
 // Entry of probe lane `lane` (< 16 <= cap) of the window that starts at `start`.
 __device__ __forceinline__ uint32_t cacheSlot(uint32_t start, uint32_t lane, uint32_t cap) {
   uint32_t i = start + lane;
@@ -865,16 +875,12 @@ __global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws,
         rflag = rdlane((uint32_t)eq, f) ? PF_HIT : PF_MISS;
       } else {
         // window full: evict the first entry with the smallest last_used
-        // (last_used only ever holds a 32-bit frame number)
-        uint32_t bl = 0, bv = rdlane((uint32_t)cur.lu, 0);
-        for (uint32_t l = 1; l < maxp; l++) {
-          const uint32_t v = rdlane((uint32_t)cur.lu, (int)l);
-          if (v < bv) {
-            bv = v;
-            bl = l;
-          }
-        }
-        raux = rdlane(idx, (int)bl);
+        // (last_used only ever holds a 32-bit frame number; the probe lanes
+        // are lanes 0..maxp-1, all in the first row of 16)
+        const uint32_t lu = probe ? (uint32_t)cur.lu : 0xFFFFFFFFu;
+        const uint32_t lo = rdlane(rowMin16(lu), 0);
+        const uint64_t at = __ballot(probe && lu == lo);
+        raux = rdlane(idx, __ffsll((long long)at) - 1);
         rflag = PF_MISS;
       }
       if (lane == 0) {
