@@ -3108,10 +3108,17 @@ __global__ __launch_bounds__(512) void k_collider_meta(const int32_t* pos, const
 // order given (ascending (z, y, x) position), so the float sums follow that
 // order exactly; one lane accumulates while the block stages each brick's
 // voxel bytes in LDS.  out: inv_mass, inverse inertia (9), centre of mass (3).
+// computeMassProperties (src/body.zig:462-519) of one collider.  Every float
+// sum runs in the canonical order (bricks as given, voxels by index inside a
+// brick), one accumulator per lane: the 64 lanes compute the per-voxel terms of
+// a brick into LDS, then lane k adds term k of each solid voxel in order.  The
+// terms are the same expressions as a sequential loop's, so the sums are too.
 __global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const uint8_t* vox,
                                                       uint32_t nb, uint32_t solid, float vs,
                                                       float density, float* out) {
-  __shared__ __align__(16) uint8_t bv[512];
+  __shared__ float term[6][512];
+  __shared__ uint64_t occ[8];  // solid voxels of the staged brick, bit i % 64 of word i / 64
+  __shared__ float s_com[3];
   const uint32_t lane = threadIdx.x;
   const float vv = vs * vs * vs;
   const float vm = vv * density;
@@ -3123,56 +3130,74 @@ __global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const 
     V3 l{(float)vx * vs + half, (float)vy * vs + half, (float)vz * vs + half};
     return add(l, V3{vs * 0.5f, vs * 0.5f, vs * 0.5f});
   };
-  auto stage = [&](uint32_t b) {
+  // stage brick b: the occupancy mask, and pass 0 (positions) or 1 (inertia terms)
+  auto stage = [&](uint32_t b, int pass, V3 com) {
     __syncthreads();
-    reinterpret_cast<uint64_t*>(bv)[lane] = reinterpret_cast<const uint64_t*>(vox + (size_t)b * 512)[lane];
+    const uint8_t* v = vox + (size_t)b * 512;
+    for (uint32_t w = 0; w < 8; w++) {
+      const uint32_t i = w * 64 + lane;
+      const bool on = (v[i] & 7u) != 0;
+      const uint64_t m = __ballot(on);
+      if (lane == 0) occ[w] = m;
+      if (!on) continue;
+      const V3 p = voxelPos(b, i);
+      if (pass == 0) {
+        term[0][i] = p.x;
+        term[1][i] = p.y;
+        term[2][i] = p.z;
+      } else {
+        const V3 r = sub(p, com);
+        const float r2 = dot(r, r);
+        term[0][i] = vm * (r2 - r.x * r.x);
+        term[1][i] = vm * (r2 - r.y * r.y);
+        term[2][i] = vm * (r2 - r.z * r.z);
+        term[3][i] = vm * r.x * r.y;
+        term[4][i] = vm * r.x * r.z;
+        term[5][i] = vm * r.y * r.z;
+      }
+    }
     __syncthreads();
   };
-  // lane 0 reads the staged bytes 16 at a time
-  V3 sum{0.f, 0.f, 0.f};
+  // pass 0: sum of positions (lanes 0..2) and the count
+  float acc = 0.0f;
   uint32_t count = 0;
   for (uint32_t b = 0; b < nb; b++) {
-    stage(b);
-    if (lane == 0)
-      for (uint32_t i0 = 0; i0 < 512; i0 += 16) {
-        const uint4 q = reinterpret_cast<const uint4*>(bv)[i0 / 16];
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-        for (uint32_t k = 0; k < 16; k++) {
-          if (((w[k / 4] >> (8 * (k % 4))) & 7u) == 0) continue;
-          sum = add(sum, voxelPos(b, i0 + k));
-          count++;
+    stage(b, 0, V3{0.f, 0.f, 0.f});
+    for (uint32_t w = 0; w < 8; w++) {
+      uint64_t m = occ[w];
+      count += (uint32_t)__popcll(m);
+      if (lane < 3)
+        while (m) {
+          const uint32_t i = w * 64 + (uint32_t)__builtin_ctzll(m);
+          m &= m - 1;
+          acc = acc + term[lane][i];
         }
-      }
+    }
   }
-  const V3 com = scale(sum, 1.0f / (float)count);
-  float I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < 3) s_com[lane] = acc * (1.0f / (float)count);  // Vec3.scale by 1/count
+  __syncthreads();
+  const V3 com{s_com[0], s_com[1], s_com[2]};
+  // pass 1: inertia, I00 I11 I22 added and I01 I02 I12 subtracted (lanes 0..5)
+  acc = 0.0f;
   for (uint32_t b = 0; b < nb; b++) {
-    stage(b);
-    if (lane == 0)
-      for (uint32_t i0 = 0; i0 < 512; i0 += 16) {
-        const uint4 q = reinterpret_cast<const uint4*>(bv)[i0 / 16];
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-        for (uint32_t k = 0; k < 16; k++) {
-          if (((w[k / 4] >> (8 * (k % 4))) & 7u) == 0) continue;
-          const V3 r = sub(voxelPos(b, i0 + k), com);
-          const float r2 = dot(r, r);
-          I[0] += vm * (r2 - r.x * r.x);
-          I[4] += vm * (r2 - r.y * r.y);
-          I[8] += vm * (r2 - r.z * r.z);
-          I[1] -= vm * r.x * r.y;
-          I[2] -= vm * r.x * r.z;
-          I[5] -= vm * r.y * r.z;
+    stage(b, 1, com);
+    for (uint32_t w = 0; w < 8; w++) {
+      uint64_t m = occ[w];
+      if (lane < 6)
+        while (m) {
+          const uint32_t i = w * 64 + (uint32_t)__builtin_ctzll(m);
+          m &= m - 1;
+          acc = lane < 3 ? acc + term[lane][i] : acc - term[lane][i];
         }
-      }
+    }
   }
+  __syncthreads();
+  if (lane < 6) term[lane][0] = acc;
+  __syncthreads();
   if (lane == 0) {
-    I[3] = I[1];
-    I[6] = I[2];
-    I[7] = I[5];
-    M3 m;
-    for (int k = 0; k < 9; k++) m.m[k] = I[k];
+    const float d0 = term[0][0], d1 = term[1][0], d2 = term[2][0];
+    const float o01 = term[3][0], o02 = term[4][0], o12 = term[5][0];
+    M3 m{{d0, o01, o02, o01, d1, o12, o02, o12, d2}};
     const M3 inv = minverse(m);
     out[0] = 1.0f / total;
     for (int k = 0; k < 9; k++) out[1 + k] = inv.m[k];
