@@ -115,6 +115,12 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t x) {
   return x;
 }
 
+__device__ __forceinline__ uint64_t shflXor64(uint64_t v, uint32_t d) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, (int)d, kWave);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), (int)d, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
   uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, kWave);
   uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, kWave);
@@ -2157,80 +2163,76 @@ __device__ void colourManifolds(uint32_t M, uint32_t nadj, uint32_t m0, uint32_t
   }
   __syncthreads();
   stamp(S.tstamp, 7);
+  // Rounds: a group of kG lanes takes one ready manifold; its lanes split the
+  // incidence lists, OR their lower neighbours' colours, and release the higher
+  // neighbours whose last lower neighbour this was (enqueued per wave).
+  constexpr uint32_t kG = 8;
+  const uint32_t gl = tid % kG, ngroups = bs / kG;
+  const uint32_t lane = tid & (kWave - 1);
   uint32_t head = 0, tail = s_tail;
   __syncthreads();
   while (head < tail) {
-    for (uint32_t q = head + tid; q < tail; q += bs) {
+    for (uint32_t q = head + tid / kG; q < tail; q += ngroups) {
       const uint32_t i = cq[q];
       const uint4 e = ce[i];
       const uint32_t pm = cp[i];
       uint64_t used0 = 0;  // colours below 64 taken by lower neighbours
       for (int side = 0; side < 2; side++) {
         const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
-        for (uint32_t k0 = 0; k0 < c; k0 += kB) {
-          uint32_t nn[kB], pp[kB];
-          int32_t col8[kB];
-#pragma unroll
-          for (uint32_t t = 0; t < kB; t++) nn[t] = k0 + t < c ? cl[o + k0 + t] : i;
-#pragma unroll
-          for (uint32_t t = 0; t < kB; t++) {
-            pp[t] = cp[nn[t]];
-            col8[t] = __atomic_load_n(&cc[nn[t]], __ATOMIC_RELAXED);
-          }
-#pragma unroll
-          for (uint32_t t = 0; t < kB; t++) {
-            const int32_t cv = colourBefore(pp[t], nn[t], pm, i) ? col8[t] : -1;
-            used0 |= (cv >= 0 && cv < 64) ? 1ull << cv : 0ull;
-          }
+        for (uint32_t k = gl; k < c; k += kG) {
+          const uint32_t n = cl[o + k];
+          const uint32_t pn = cp[n];
+          const int32_t cv = __atomic_load_n(&cc[n], __ATOMIC_RELAXED);
+          if (colourBefore(pn, n, pm, i) && cv >= 0 && cv < 64) used0 |= 1ull << cv;
         }
       }
-      uint32_t col;
-      if (~used0) {
-        col = (uint32_t)__builtin_ctzll(~used0);
-      } else {  // all of 0..63 taken: the full mask
-        uint64_t used[kMaxColors / 64];
-        for (uint32_t u = 0; u < kMaxColors / 64; u++) used[u] = 0;
-        for (int side = 0; side < 2; side++) {
-          const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
-          for (uint32_t k = 0; k < c; k++) {
-            uint32_t n = cl[o + k];
-            if (n == i || !colourBefore(cp[n], n, pm, i)) continue;
-            int32_t cv = __atomic_load_n(&cc[n], __ATOMIC_RELAXED);
-            if (cv >= 0 && cv < (int32_t)kMaxColors) used[cv >> 6] |= 1ull << (cv & 63);
+#pragma unroll
+      for (uint32_t d = 1; d < kG; d <<= 1) used0 |= shflXor64(used0, d);
+      if (gl == 0) {
+        uint32_t col;
+        if (~used0) {
+          col = (uint32_t)__builtin_ctzll(~used0);
+        } else {  // all of 0..63 taken: the full mask, on the group's first lane
+          uint64_t used[kMaxColors / 64];
+          for (uint32_t u = 0; u < kMaxColors / 64; u++) used[u] = 0;
+          for (int side = 0; side < 2; side++) {
+            const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
+            for (uint32_t k = 0; k < c; k++) {
+              uint32_t n = cl[o + k];
+              if (n == i || !colourBefore(cp[n], n, pm, i)) continue;
+              int32_t cv = __atomic_load_n(&cc[n], __ATOMIC_RELAXED);
+              if (cv >= 0 && cv < (int32_t)kMaxColors) used[cv >> 6] |= 1ull << (cv & 63);
+            }
           }
+          col = kMaxColors;
+          for (uint32_t u = 0; u < kMaxColors / 64; u++)
+            if (~used[u]) {
+              col = u * 64 + (uint32_t)__builtin_ctzll(~used[u]);
+              break;
+            }
         }
-        col = kMaxColors;
-        for (uint32_t u = 0; u < kMaxColors / 64; u++)
-          if (~used[u]) {
-            col = u * 64 + (uint32_t)__builtin_ctzll(~used[u]);
-            break;
-          }
+        if (col >= kMaxColors) {
+          col = kMaxColors - 1;
+          atomicOr(&x.error, kErrColours);
+        }
+        __atomic_store_n(&cc[i], (int32_t)col, __ATOMIC_RELAXED);
+        atomicMax(&s_ncol, col + 1);
       }
-      if (col >= kMaxColors) {
-        col = kMaxColors - 1;
-        atomicOr(&x.error, kErrColours);
-      }
-      __atomic_store_n(&cc[i], (int32_t)col, __ATOMIC_RELAXED);
-      atomicMax(&s_ncol, col + 1);
-      for (int side = 0; side < 2; side++) {  // release the higher neighbours
+      // release the higher neighbours (they run in a later round, after the
+      // barrier that makes this colour visible)
+      for (int side = 0; side < 2; side++) {
         const uint32_t o = side ? e.z : e.x, c = side ? e.w : e.y;
-        for (uint32_t k0 = 0; k0 < c; k0 += kB) {
-          uint32_t nn[kB], rel = 0;
-#pragma unroll
-          for (uint32_t t = 0; t < kB; t++) nn[t] = k0 + t < c ? cl[o + k0 + t] : i;
-#pragma unroll
-          for (uint32_t t = 0; t < kB; t++) {
-            // subtracting 0 from the manifold's own count (padding, lower
-            // neighbours) changes nothing
-            const bool hi = colourBefore(pm, i, cp[nn[t]], nn[t]);
-            const uint32_t left = atomicSub(&cn[nn[t]], hi ? 1u : 0u);
-            rel |= (hi && left == 1u) ? 1u << t : 0u;
-          }
-          if (rel) {
-            uint32_t at = atomicAdd(&s_tail, (uint32_t)__builtin_popcount(rel));
-#pragma unroll
-            for (uint32_t t = 0; t < kB; t++)
-              if (rel & (1u << t)) cq[at++] = nn[t];
+        for (uint32_t k = gl; k < c; k += kG) {
+          const uint32_t n = cl[o + k];
+          const bool hi = colourBefore(pm, i, cp[n], n);
+          const bool rel = hi && atomicSub(&cn[n], 1u) == 1u;
+          const uint64_t m = __ballot(rel);
+          if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&s_tail, (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, leader, kWave);
+            if (rel) cq[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = n;
           }
         }
       }
