@@ -244,6 +244,7 @@ def main():
                         f"(log-uniform 8..1024 per world), frames {args.warmup}.."
                         f"{args.warmup + args.batched_steps - 1}",
             "ms_per_step": rb["wall"] * 1e3 / args.batched_steps,
+            "nominal_value": sb.n_dynamic * sb.config["substeps"] * args.batched_steps / rb["wall"],
             "contact_candidates_per_step": rb["candidates"] / args.batched_steps,
             "p50_step_ms": statistics.median(rb["frame_ms"]),
             "p99_step_ms": pct(rb["frame_ms"], 0.99),
@@ -319,6 +320,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": plain["wall"] * 1e3 / args.steps,
+        # every dynamic body counted every frame, asleep or not
+        "nominal_value": sc.n_dynamic * sc.config["substeps"] * args.steps / plain["wall"],
         "p50_step_ms": statistics.median(plain["frame_ms"]),
         "p99_step_ms": pct(plain["frame_ms"], 0.99),
         "higher_is_better": True,
