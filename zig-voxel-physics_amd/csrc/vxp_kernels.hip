@@ -1156,6 +1156,19 @@ __device__ __forceinline__ void regionVoxel(const PairRec& r, uint32_t lin, int3
 // (lane_n), per-work-item totals wi_ccnt (wi_ccnt[total_wi] = 0) and
 // voxels_checked sums.  Otherwise: candidates written from wi_cbase[wid] (the
 // exclusive scan of wi_ccnt) in (lane, probe) order.
+// Workgroups go to the 8 XCDs round-robin; this maps them so that runs of
+// kXcdRun consecutive logical blocks execute on one XCD (consecutive work
+// items of a pair then share that XCD's L2) while the runs still rotate over
+// the XCDs.  A bijection on [0, gridDim.x): blocks past the last whole round
+// of runs keep their index.
+constexpr uint32_t kXcds = 8, kXcdRun = 64;
+__device__ __forceinline__ uint32_t xcdBlock() {
+  const uint32_t b = blockIdx.x, full = gridDim.x / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+  if (b >= full) return b;
+  const uint32_t x = b % kXcds, r = b / kXcds;
+  return (r / kXcdRun) * (kXcds * kXcdRun) + x * kXcdRun + r % kXcdRun;
+}
+
 template <bool kCount>
 __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColliders C,
                                                const PairRec* pr, uint32_t n_pairs,
@@ -1163,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
                                                const uint32_t* wi_cbase, uint32_t* wi_ccnt,
                                                uint32_t* wi_vchk, uint8_t* lane_n, Cand* cand) {
   uint32_t lane = threadIdx.x & (kWave - 1);
-  uint32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  uint32_t wid = xcdBlock() * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (wid >= total_wi) return;  // whole wave
   if (!kCount && wi_ccnt[wid] == 0) return;
   uint32_t p = upperBound(wi_off, n_pairs, wid) - 1;
