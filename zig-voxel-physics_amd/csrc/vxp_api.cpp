@@ -670,11 +670,14 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_wi_ccnt.ensure((size_t)(total_wi + 1) * 4));
   HIP_TRY(s->d_wi_vchk.ensure((size_t)std::max(total_wi, 1u) * 4));
   HIP_TRY(s->d_lane_n.ensure((size_t)std::max(total_wi, 1u) * 64));
+  HIP_TRY(s->d_wi_pair.ensure((size_t)std::max(total_wi, 1u) * 4));
   // pass 1 counts, the scan places every work item's candidates, pass 2 writes
   pe = profBegin(s);
   uint32_t ncand = 0;
   if (total_wi) {
-    launch_probe(st, cfg, B, C, pr, npairs, s->d_wi_off.as<uint32_t>(), total_wi, nullptr,
+    launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), npairs,
+                   s->d_wi_pair.as<uint32_t>());
+    launch_probe(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), total_wi, nullptr,
                  s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
                  nullptr, true);
     size_t cb = exclusive_scan_u32_bytes(total_wi + 1);
@@ -685,7 +688,7 @@ int stepAll(vxp_system* s) {
   }
   HIP_TRY(s->d_cand.ensure((size_t)std::max(ncand, 1u) * sizeof(Cand)));
   if (ncand)
-    launch_probe(st, cfg, B, C, pr, npairs, s->d_wi_off.as<uint32_t>(), total_wi,
+    launch_probe(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), total_wi,
                  s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
                  s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(), s->d_cand.as<Cand>(),
                  false);
@@ -1066,7 +1069,7 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
                     &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off,
                     &s->d_t_pos, &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake,
-                    &s->d_arank, &s->d_drop, &s->d_cflag_r, &s->d_cidx_r};
+                    &s->d_arank, &s->d_drop, &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);

@@ -945,6 +945,15 @@ __global__ void k_chunk_counts(const PairRec* pr, uint32_t n_pairs, uint32_t* co
   counts[p] = (f == PF_DETECT || f == PF_MISS) ? pr[p].nchunks : 0u;
 }
 
+// The pair of every work item: wi_pair[wi_off[p] + c] = p for c < counts[p].
+__global__ void k_wi_fill(const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
+                          uint32_t* wi_pair) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const uint32_t c = counts[p], b = wi_off[p];
+  for (uint32_t k = 0; k < c; k++) wi_pair[b + k] = p;
+}
+
 // ---------------------------------------------------------------------------
 // voxel probe: one wave per 64-voxel chunk of a pair's overlap region (region
 // order z, y, x as src/narrow_phase.zig:759-764; lane = position in the chunk).
@@ -1171,7 +1180,7 @@ __device__ __forceinline__ uint32_t xcdBlock() {
 
 template <bool kCount>
 __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColliders C,
-                                               const PairRec* pr, uint32_t n_pairs,
+                                               const PairRec* pr, const uint32_t* wi_pair,
                                                const uint32_t* wi_off, uint32_t total_wi,
                                                const uint32_t* wi_cbase, uint32_t* wi_ccnt,
                                                uint32_t* wi_vchk, uint8_t* lane_n, Cand* cand) {
@@ -1179,7 +1188,7 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
   uint32_t wid = xcdBlock() * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (wid >= total_wi) return;  // whole wave
   if (!kCount && wi_ccnt[wid] == 0) return;
-  uint32_t p = upperBound(wi_off, n_pairs, wid) - 1;
+  uint32_t p = wi_pair[wid];
   const PairRec& r = pr[p];
   uint32_t chunk = wid - wi_off[p];
   uint32_t w = r.world;
@@ -3508,16 +3517,20 @@ void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_s
 void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts) {
   if (n) k_chunk_counts<<<cdiv(n, 256), 256, 0, s>>>(pr, n, counts);
 }
+void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
+                    uint32_t* wi_pair) {
+  if (n_pairs) k_wi_fill<<<cdiv(n_pairs, 256), 256, 0, s>>>(counts, wi_off, n_pairs, wi_pair);
+}
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                  const PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
-                  const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, uint8_t* lane_n,
-                  Cand* cand, bool count) {
+                  const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
+                  uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
+                  uint8_t* lane_n, Cand* cand, bool count) {
   if (!total_wi) return;
   if (count)
-    k_probe<true><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, n_pairs, wi_off, total_wi,
+    k_probe<true><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi,
                                                     wi_cbase, wi_ccnt, wi_vchk, lane_n, cand);
   else
-    k_probe<false><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, n_pairs, wi_off, total_wi,
+    k_probe<false><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi,
                                                      wi_cbase, wi_ccnt, wi_vchk, lane_n, cand);
 }
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,

@@ -56,10 +56,13 @@ void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx,
                          uint32_t* out);
 void launch_invert(hipStream_t s, const uint32_t* perm, uint32_t n, uint32_t* inv);
 void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts);
+// wi_pair[w] = the pair of work item w (after the scan of the chunk counts)
+void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
+                    uint32_t* wi_pair);
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                  const PairRec* pr, uint32_t n_pairs, const uint32_t* wi_off, uint32_t total_wi,
-                  const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk, uint8_t* lane_n,
-                  Cand* cand, bool count);
+                  const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
+                  uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
+                  uint8_t* lane_n, Cand* cand, bool count);
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
