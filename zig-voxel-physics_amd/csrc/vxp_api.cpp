@@ -760,6 +760,7 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->s_uorder.ensure((size_t)nm1 * 4));
   HIP_TRY(s->s_units.ensure((size_t)nw * kUnitsPerWorld * sizeof(uint4)));
   HIP_TRY(s->s_uqueue.ensure((size_t)nw * kUnitsPerWorld * 4));
+  HIP_TRY(s->s_uqueue2.ensure((size_t)nw * kUnitsPerWorld * 4));
   HIP_TRY(s->s_pcnt.ensure((size_t)nm1 * 4));
   HIP_TRY(s->s_fq.ensure((size_t)nm1 * 4));
   HIP_TRY(s->s_cend.ensure((size_t)nm1 * 16));
@@ -797,6 +798,7 @@ int stepAll(vxp_system* s) {
                     s->lds_limit);
   profEnd(s, VXP_K_SOLVE, pe);
   pe = profBegin(s);
+  launch_unit_order(st, S, ctr, s->s_uqueue2.as<uint32_t>());
   launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit);
   if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
     (void)hipGetLastError();
@@ -1064,7 +1066,7 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,
                     &s->s_color, &s->s_order, &s->s_rec, &s->s_pts, &s->s_prio, &s->s_tstamp,
                     &s->s_upar, &s->s_ucnt, &s->s_uoff, &s->s_ubody, &s->s_uorder, &s->s_units,
-                    &s->s_uqueue, &s->s_pcnt, &s->s_fq, &s->s_cend, &s->d_cb_pos, &s->d_cb_vox,
+                    &s->s_uqueue, &s->s_uqueue2, &s->s_pcnt, &s->s_fq, &s->s_cend, &s->d_cb_pos, &s->d_cb_vox,
                     &s->d_cb_cnt, &s->d_cb_bounds, &s->d_cb_box, &s->d_cb_out, &s->d_cb_solid,
                     &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
                     &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off,

@@ -147,7 +147,7 @@ struct vxp_system {
       d_pairs_tree, d_awake, d_arank, d_drop, d_cflag_r, d_cidx_r;
   DevBuf d_wi_pair;  // the pair of each probe work item
   DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,
-      s_pts, s_prio, s_tstamp, s_upar, s_ucnt, s_uoff, s_ubody, s_uorder, s_units, s_uqueue,
+      s_pts, s_prio, s_tstamp, s_upar, s_ucnt, s_uoff, s_ubody, s_uorder, s_units, s_uqueue, s_uqueue2,
       s_pcnt, s_fq, s_cend;
   uint32_t n_pairs = 0, n_manifolds = 0;
   uint32_t* h_read = nullptr;  // pinned host words for the per-step size reads

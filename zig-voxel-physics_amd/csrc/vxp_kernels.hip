@@ -3576,6 +3576,44 @@ void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
   size_t lds = std::min(lds_limit, (size_t)max_manifolds * kColourLdsPerManifold);
   k_solve_prep<<<nw, kSolveThreads, lds, s>>>(cfg, B, ws, st, ml, S, prio, step_ctr, lds);
 }
+// Longest units first: the queue is reordered by descending power-of-two
+// bucket of manifold count (order inside a bucket is free), so that the
+// persistent grid starts the units that bound the launch before the small ones.
+// Units are independent: any order gives the same results.
+__global__ __launch_bounds__(1024) void k_unit_order(SolverScratch S, const uint32_t* step_ctr,
+                                                     uint32_t* tmp) {
+  __shared__ uint32_t hist[33], cur[33];
+  const uint32_t tid = threadIdx.x, n = step_ctr[kCtrSolveUnits];
+  if (tid < 33) hist[tid] = 0;
+  __syncthreads();
+  auto bucket = [&](uint32_t q) {  // 0 for the largest units
+    const uint32_t mn = S.units[q].y;
+    return (uint32_t)__builtin_clz(mn | 1u);  // 32 - bit length: larger counts first
+  };
+  for (uint32_t i = tid; i < n; i += blockDim.x) {
+    const uint32_t q = S.uqueue[i];
+    tmp[i] = q;
+    atomicAdd(&hist[bucket(q)], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t a = 0;
+    for (uint32_t b = 0; b < 33; b++) {
+      cur[b] = a;
+      a += hist[b];
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += blockDim.x) {
+    const uint32_t q = tmp[i];
+    S.uqueue[atomicAdd(&cur[bucket(q)], 1u)] = q;
+  }
+}
+void launch_unit_order(hipStream_t s, const SolverScratch& S, const uint32_t* step_ctr,
+                       uint32_t* tmp) {
+  k_unit_order<<<1, 1024, 0, s>>>(S, step_ctr, tmp);
+}
+
 void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                         DevManifold* ml, const SolverScratch& S, uint32_t* step_ctr, uint32_t nw,
                         uint32_t n_cu, size_t lds_limit) {

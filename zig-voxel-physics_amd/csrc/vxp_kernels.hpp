@@ -89,6 +89,9 @@ void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
                        vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
                        uint32_t* step_ctr, uint32_t nw, uint32_t max_manifolds,
                        size_t lds_limit);
+// Reorders the solve-unit queue, largest units first (tmp: queue-sized scratch).
+void launch_unit_order(hipStream_t s, const SolverScratch& S, const uint32_t* step_ctr,
+                       uint32_t* tmp);
 void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                         DevManifold* ml, const SolverScratch& S, uint32_t* step_ctr, uint32_t nw,
                         uint32_t n_cu, size_t lds_limit);
