@@ -422,15 +422,19 @@ int vxp_server_info_get(vxp_server* srv, vxp_server_info* out);
  * speed differs.  An online tuner times the candidates on live frames (a
  * window of 15 frames at creation, again every 600 frames) and keeps the
  * fastest.  pair_color_max: colours of at most this many manifolds are solved
- * on lane pairs (one body per lane) instead of one lane per manifold. */
+ * on lane pairs (one body per lane) instead of one lane per manifold.
+ * unit_threads: threads per solve-unit workgroup, 256 or 512 (default: 256
+ * for systems of 64 worlds or more, where many units share the device; 512
+ * otherwise, where one large unit bounds the solve). */
 typedef struct vxp_tuning {
   uint32_t pair_color_max;
-  uint32_t windows;  /* tuning windows completed */
-  uint32_t fixed;    /* set by vxp_tuning_set: the tuner is off */
-  uint32_t pad;
+  uint32_t windows;       /* tuning windows completed */
+  uint32_t fixed;         /* set by vxp_tuning_set: the tuner is off */
+  uint32_t unit_threads;  /* get: in use; set: 256 or 512, 0 = default */
 } vxp_tuning;
 int vxp_tuning_get(vxp_system* sys, vxp_tuning* out);
-/* Fix the tuning (the tuner stops); pair_color_max = 0xFFFFFFFF restarts it. */
+/* Fix the tuning (the tuner stops); pair_color_max = 0xFFFFFFFF restarts it.
+ * unit_threads is applied either way (VXP_ERR_ARGUMENT unless 0, 256 or 512). */
 int vxp_tuning_set(vxp_system* sys, const vxp_tuning* tuning);
 
 /* Last error message of this thread (never NULL). */

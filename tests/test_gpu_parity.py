@@ -147,20 +147,22 @@ def test_empty_collider_body_in_scene(product, oracle, order):
 
 
 def test_tuning_choices_are_bit_identical(product):
-    # The lane-pair colour limit changes only the launch shape of the solver:
-    # forced extremes (every colour on single lanes / every colour of up to
-    # 1024 manifolds on lane pairs) and the online tuner give the same bits.
+    # The lane-pair colour limit and the solver's workgroup size change only
+    # the launch shape: forced extremes (every colour on single lanes / every
+    # colour of up to 1024 manifolds on lane pairs), both workgroup sizes and
+    # the online tuner give the same bits.
     sc = scenes.batched_worlds(n_worlds=6, lo=8, hi=1024, blob_pool=8)
     runs = []
-    for forced in (0, 1024, None):
+    for forced, threads in ((0, 512), (1024, 256), (0, 256), (None, 0)):
         s = vxp.from_scene(sc, lib=product)
         if forced is not None:
-            s.setTuning(forced)
-        print(f"tuning forced={forced}", flush=True)
+            s.setTuning(forced, threads)
+        print(f"tuning forced={forced} unit_threads={threads}", flush=True)
         for _ in range(40):
             s.update(sc.dt)
         t = s.tuning()
         assert t["fixed"] == (forced is not None)
+        assert t["unit_threads"] == (threads or 512)  # 6 worlds: the default is 512
         if forced is not None:
             assert t["pair_color_max"] == forced and t["windows"] == 0
         else:

@@ -335,6 +335,14 @@ int ensureIslands(vxp_system* s) {
   return VXP_OK;
 }
 
+// Threads per solve-unit workgroup: forced, or 256 when many worlds' units
+// share the device (two workgroups fit a CU) and 512 when a few large units
+// bound the solve (more lane pairs per colour).
+uint32_t unitThreads(const vxp_system* s) {
+  if (s->unit_threads) return s->unit_threads;
+  return s->nw >= 64 ? 256u : 512u;
+}
+
 IslandArrays islandArrays(vxp_system* s) {
   return IslandArrays{s->d_isl_parent.as<uint32_t>(), s->d_isl_end.as<uint32_t>(),
                       s->d_isl_size.as<uint32_t>(), s->d_isl_timer.as<float>(),
@@ -799,11 +807,11 @@ int stepAll(vxp_system* s) {
   profEnd(s, VXP_K_SOLVE, pe);
   pe = profBegin(s);
   launch_unit_order(st, S, ctr, s->s_uqueue2.as<uint32_t>());
-  launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit);
+  launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit, unitThreads(s));
   if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
     (void)hipGetLastError();
     s->lds_limit = 0;
-    launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, 0);
+    launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, 0, unitThreads(s));
   }
   profEnd(s, VXP_K_SOLVE_UNITS, pe);
   pe = profBegin(s);
@@ -882,10 +890,16 @@ int vxp_tuning_get(vxp_system* s, vxp_tuning* out) {
   out->pair_color_max = s->tune.pick;
   out->windows = s->tune.windows;
   out->fixed = s->tune.fixed;
+  out->unit_threads = unitThreads(s);
   return VXP_OK;
 }
 int vxp_tuning_set(vxp_system* s, const vxp_tuning* t) {
   if (!s || !t) return VXP_ERR_ARGUMENT;
+  if (t->unit_threads != 0 && t->unit_threads != 256 && t->unit_threads != 512) {
+    g_err = "unit_threads must be 0, 256 or 512";
+    return VXP_ERR_ARGUMENT;
+  }
+  s->unit_threads = t->unit_threads;
   if (t->pair_color_max == 0xFFFFFFFFu) {
     s->tune.fixed = false;
     s->tune.next_window = s->tune.frame;

@@ -163,6 +163,7 @@ struct vxp_system {
     uint32_t pending_idx = 0;
     hipEvent_t a = nullptr, b = nullptr;
   } tune;
+  uint32_t unit_threads = 0;  // solve-unit workgroup size forced by vxp_tuning_set (0: default)
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   uint32_t n_cu = 256;   // compute units of the device
   // collider build scratch (classification, metadata, mass properties)

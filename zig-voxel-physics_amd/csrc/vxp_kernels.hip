@@ -2628,17 +2628,18 @@ __device__ void unitPositionCombine(WorldState& x, uint32_t slot, float minsep, 
   __syncthreads();
 }
 
-constexpr uint32_t kUnitThreads = 512;  // 256 VGPRs: the passes keep a manifold's records in registers
-template <bool kLds>
-__global__ __launch_bounds__(kUnitThreads) void k_solve_units(DevCfg cfg, DevBodies B,
-                                                              WorldState* ws, DevManifold* ml,
-                                                              SolverScratch S,
-                                                              uint32_t* step_ctr) {
+// Unit blocks of kT threads: 512 lets one large unit run its colours on more
+// lane pairs; 256 fits two blocks per CU (the passes keep a manifold's records
+// in registers, about 210 VGPRs), for launches with many small units.
+template <bool kLds, uint32_t kT>
+__global__ __launch_bounds__(kT) void k_solve_units(DevCfg cfg, DevBodies B, WorldState* ws,
+                                                    DevManifold* ml, SolverScratch S,
+                                                    uint32_t* step_ctr) {
   extern __shared__ float4 dyn_vel[];
-  __shared__ uint32_t tmp[kUnitThreads];
+  __shared__ uint32_t tmp[kT];
   __shared__ uint32_t ccount[kMaxColors + 1];
   __shared__ uint32_t ccursor[kMaxColors];
-  __shared__ float s_red[kUnitThreads];
+  __shared__ float s_red[kT];
   __shared__ uint32_t s_item, s_ncol;
   const uint32_t tid = threadIdx.x, bs = blockDim.x;
   const uint32_t total = step_ctr[kCtrSolveUnits];
@@ -3616,13 +3617,21 @@ void launch_unit_order(hipStream_t s, const SolverScratch& S, const uint32_t* st
 
 void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                         DevManifold* ml, const SolverScratch& S, uint32_t* step_ctr, uint32_t nw,
-                        uint32_t n_cu, size_t lds_limit) {
+                        uint32_t n_cu, size_t lds_limit, uint32_t threads) {
   size_t vel_bytes = (size_t)cfg.max_bodies * 2 * sizeof(float4);
   uint32_t grid = std::min<uint32_t>(nw * kUnitsMax, 2 * std::max(n_cu, 1u));
-  if (vel_bytes <= lds_limit)
-    k_solve_units<true><<<grid, kUnitThreads, vel_bytes, s>>>(cfg, B, ws, ml, S, step_ctr);
-  else
-    k_solve_units<false><<<grid, kUnitThreads, 0, s>>>(cfg, B, ws, ml, S, step_ctr);
+  const bool lds = vel_bytes <= lds_limit;
+  if (threads == 256) {
+    if (lds)
+      k_solve_units<true, 256><<<grid, 256, vel_bytes, s>>>(cfg, B, ws, ml, S, step_ctr);
+    else
+      k_solve_units<false, 256><<<grid, 256, 0, s>>>(cfg, B, ws, ml, S, step_ctr);
+  } else {
+    if (lds)
+      k_solve_units<true, 512><<<grid, 512, vel_bytes, s>>>(cfg, B, ws, ml, S, step_ctr);
+    else
+      k_solve_units<false, 512><<<grid, 512, 0, s>>>(cfg, B, ws, ml, S, step_ctr);
+  }
 }
 void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, const SolverScratch& S, const uint32_t* step_ctr,

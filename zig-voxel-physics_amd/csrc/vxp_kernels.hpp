@@ -94,7 +94,7 @@ void launch_unit_order(hipStream_t s, const SolverScratch& S, const uint32_t* st
                        uint32_t* tmp);
 void launch_solve_units(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                         DevManifold* ml, const SolverScratch& S, uint32_t* step_ctr, uint32_t nw,
-                        uint32_t n_cu, size_t lds_limit);
+                        uint32_t n_cu, size_t lds_limit, uint32_t threads);
 void launch_solve_tail(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, const SolverScratch& S, const uint32_t* step_ctr,
                        uint32_t nw, uint32_t n_cu);

@@ -206,7 +206,7 @@ class ServerInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("pair_color_max", C.c_uint32), ("windows", C.c_uint32), ("fixed", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("unit_threads", C.c_uint32)]
 
 
 TUNE_AUTO = 0xFFFFFFFF

@@ -61,11 +61,14 @@ class VoxelPhysicsSystem:
     def tuning(self):
         t = _abi.Tuning()
         self.lib.check(self.lib.tuning_get(self.ptr, C.byref(t)), "tuning_get")
-        return {"pair_color_max": t.pair_color_max, "windows": t.windows, "fixed": bool(t.fixed)}
+        return {"pair_color_max": t.pair_color_max, "windows": t.windows, "fixed": bool(t.fixed),
+                "unit_threads": t.unit_threads}
 
-    def setTuning(self, pair_color_max=None):
-        """Fix the solver's lane-pair colour limit; None restarts the online tuner."""
-        t = _abi.Tuning(pair_color_max=_abi.TUNE_AUTO if pair_color_max is None else pair_color_max)
+    def setTuning(self, pair_color_max=None, unit_threads=0):
+        """Fix the solver's lane-pair colour limit (None restarts the online tuner)
+        and its workgroup size (256 or 512; 0 for the default)."""
+        t = _abi.Tuning(pair_color_max=_abi.TUNE_AUTO if pair_color_max is None else pair_color_max,
+                        unit_threads=unit_threads)
         self.lib.check(self.lib.tuning_set(self.ptr, C.byref(t)), "tuning_set")
 
     # ---- colliders / bodies (src/system.zig:294-433) -------------------------
