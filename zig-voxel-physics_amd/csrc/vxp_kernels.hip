@@ -1297,10 +1297,19 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
     uint32_t swi0 = 0, swi1 = 0, swi2 = 0, swi3 = 0;  // work item of each slot's candidate
     float spen0 = 0.f, spen1 = 0.f, spen2 = 0.f, spen3 = 0.f;
     uint64_t sfid0 = 0, sfid1 = 0, sfid2 = 0, sfid3 = 0;
-    uint32_t ns = 0;
-    for (uint32_t wi = w0; wi < w1; wi++) {
-      const uint32_t cnt = wi_ccnt[wi], base = wi_cbase[wi];
-      ncand += cnt;
+    uint32_t ns = 0, ncl = 0;
+    // work items 64 at a time: lanes read the counts, and only the non-empty
+    // items are visited, in ascending order
+    for (uint32_t wb = w0; wb < w1; wb += kWave) {
+      const uint32_t wl = wb + lane;
+      const uint32_t cl = wl < w1 ? wi_ccnt[wl] : 0u, bl = wl < w1 ? wi_cbase[wl] : 0u;
+      ncl += cl;
+      uint64_t nz = __ballot(cl != 0u);
+      while (nz) {
+      const int kw = __ffsll((long long)nz) - 1;
+      nz &= nz - 1;
+      const uint32_t wi = wb + (uint32_t)kw;
+      const uint32_t cnt = rdlane(cl, kw), base = rdlane(bl, kw);
       for (uint32_t off = 0; off < cnt; off += kWave) {
         const bool valid = off + lane < cnt;
         float pen = 0.0f;
@@ -1351,7 +1360,9 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
           else if (mi == 3) { sidx3 = ik; swi3 = wi; spen3 = pk; sfid3 = fk; }
         }
       }
+      }
     }
+    ncand = waveSum(ncl);
     ncont = ns;
     if (lane < ncont) {
       // rebuild the winner's contact from its A voxel (lane of its work item)
