@@ -400,13 +400,39 @@ int streamWait(vxp_system* s) {
   }
 }
 
-// Copies device words into the pinned read area and waits for them.
+// Reads up to two device words: a one-thread kernel, queued behind the work
+// that produces them, writes them and then a new sequence number into pinned
+// host memory, and the host polls that number (the stream is checked now and
+// then so that a failed launch ends the wait).
 int readSizes(vxp_system* s, std::initializer_list<const uint32_t*> src, uint32_t* out) {
-  uint32_t i = 0;
-  for (const uint32_t* p : src)
-    HIP_TRY(hipMemcpyAsync(s->h_read + i++, p, 4, hipMemcpyDeviceToHost, s->stream));
-  RC(streamWait(s));
-  for (uint32_t k = 0; k < i; k++) out[k] = s->h_read[k];
+  const uint32_t* p[2] = {nullptr, nullptr};
+  uint32_t n = 0;
+  for (const uint32_t* q : src) {
+    if (n == 2) {
+      g_err = "readSizes: at most two words";
+      return VXP_ERR_ARGUMENT;
+    }
+    p[n++] = q;
+  }
+  const uint32_t seq = ++s->read_seq;
+  launch_publish(s->stream, s->d_read, p[0], p[1], seq);
+  HIP_TRY(hipGetLastError());
+  for (uint32_t spin = 1;; spin++) {
+    if (__atomic_load_n(&s->h_read[0], __ATOMIC_ACQUIRE) == seq) break;
+    if ((spin & 1023u) == 0) {
+      hipError_t e = hipStreamQuery(s->stream);
+      if (e != hipSuccess && e != hipErrorNotReady) {
+        g_err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+        return VXP_ERR_DEVICE;
+      }
+      if (e == hipSuccess && __atomic_load_n(&s->h_read[0], __ATOMIC_ACQUIRE) != seq) {
+        g_err = "size read: the stream drained without the published words";
+        return VXP_ERR_DEVICE;
+      }
+    }
+    __builtin_ia32_pause();
+  }
+  for (uint32_t k = 0; k < n; k++) out[k] = __atomic_load_n(&s->h_read[1 + k], __ATOMIC_RELAXED);
   return VXP_OK;
 }
 
@@ -1041,10 +1067,14 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
     g_err = "hipStreamCreate failed";
     return fail(VXP_ERR_DEVICE);
   }
-  if (hipHostMalloc((void**)&s->h_read, 256, hipHostMallocDefault) != hipSuccess) {
+  // coherent (fine-grained) so that a kernel's writes reach the host while it polls
+  if (hipHostMalloc((void**)&s->h_read, 256, hipHostMallocCoherent | hipHostMallocMapped) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void**)&s->d_read, s->h_read, 0) != hipSuccess) {
     g_err = "pinned host allocation failed";
     return fail(VXP_ERR_OOM);
   }
+  std::memset(s->h_read, 0, 256);
   std::vector<WorldState> ws(n_worlds);
   std::memset(ws.data(), 0, ws.size() * sizeof(WorldState));
   for (WorldState& x : ws) x.isl_need = 1;  // IslandManager.init (src/islands.zig:248-270)

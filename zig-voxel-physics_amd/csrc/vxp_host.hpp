@@ -150,7 +150,9 @@ struct vxp_system {
       s_pts, s_prio, s_tstamp, s_upar, s_ucnt, s_uoff, s_ubody, s_uorder, s_units, s_uqueue, s_uqueue2,
       s_pcnt, s_fq, s_cend;
   uint32_t n_pairs = 0, n_manifolds = 0;
-  uint32_t* h_read = nullptr;  // pinned host words for the per-step size reads
+  uint32_t* h_read = nullptr;  // pinned host words for the per-step size reads: [seq, a, b]
+  uint32_t* d_read = nullptr;  // their device address
+  uint32_t read_seq = 0;
   // Online tuning of launch shapes that do not change results: during a window
   // of frames the candidates take turns (one per frame, timed with events
   // around the solver); the fastest on average is kept until the next window.

@@ -3426,6 +3426,19 @@ __global__ void k_invert(const uint32_t* perm, uint32_t n, uint32_t* inv) {
   if (p < n) inv[perm[p]] = p;
 }
 
+// Writes *a and *b (when given), then seq, into host memory that the host
+// polls; the system-scope release orders the words before the number.
+__global__ void k_publish(uint32_t* host, const uint32_t* a, const uint32_t* b, uint32_t seq) {
+  if (threadIdx.x != 0) return;
+  host[1] = a ? *a : 0u;
+  host[2] = b ? *b : 0u;
+  __hip_atomic_store(&host[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+void launch_publish(hipStream_t s, uint32_t* host, const uint32_t* a, const uint32_t* b,
+                    uint32_t seq) {
+  k_publish<<<1, 64, 0, s>>>(host, a, b, seq);
+}
+
 void launch_tree_update(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
                         const TreeSet& T, const uint8_t* moved, uint32_t nw, size_t lds_limit) {
   const size_t need = (size_t)T.nc * sizeof(TreeNode);

@@ -37,6 +37,9 @@ void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_s
                          uint32_t n_pairs, uint32_t nw);
 // Broad-phase trees (pair order VXP_PAIR_ORDER_TREE): per world, re-insert the
 // leaves of the dynamic tree whose fat box moved, in slot order.
+// host[1..2] = *a, *b (0 when null), then host[0] = seq (host-visible memory).
+void launch_publish(hipStream_t s, uint32_t* host, const uint32_t* a, const uint32_t* b,
+                    uint32_t seq);
 void launch_tree_update(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
                         const TreeSet& T, const uint8_t* moved, uint32_t nw, size_t lds_limit);
 // Pairs in tree order: count (out == nullptr) or write packed keys
