@@ -1544,6 +1544,21 @@ __device__ __forceinline__ uint32_t findRoot(uint32_t* P, uint32_t r) {
   }
 }
 
+// Joins the sets of a and b, the larger root hooked under the smaller, so a
+// root is always its set's smallest element and one pass over the edges
+// leaves the same roots as hooking rounds run to a fixed point.  Lock-free: a
+// failed compare-and-swap means that root was hooked meanwhile; retry from
+// the new roots (every failure is some other thread's progress).
+__device__ __forceinline__ void unite(uint32_t* P, uint32_t a, uint32_t b) {
+  while (true) {
+    a = findRoot(P, a);
+    b = findRoot(P, b);
+    if (a == b) return;
+    const uint32_t hi = max(a, b), lo = min(a, b);
+    if (atomicCAS(&P[hi], hi, lo) == hi) return;
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, WorldState* ws,
                                                   vxp_stats* st, const DevManifold* ml,
                                                   const DevManifold* prev, IslandArrays I,
@@ -1582,37 +1597,24 @@ __global__ __launch_bounds__(1024) void k_islands(DevCfg cfg, DevBodies B, World
     // the key sequence against the previous step's (same length only)
     if (i < ws[w].prev_count && d.key != prev[ws[w].prev_start + i].key) atomicOr(&differ, 1u);
   }
-  __shared__ uint32_t changed;
-  for (int round = 0; round < 100000; round++) {
-    __syncthreads();
-    if (tid == 0) changed = 0;
-    __syncthreads();
-    for (uint32_t i = tid; i < M; i += bs) {
-      uint32_t ra, rb;
-      if (L) {
-        if (eb[i] == kNone) continue;
-        ra = ea[i];
-        rb = eb[i];
-      } else {
-        const DevManifold& d = ml[m0 + i];
-        uint32_t ga = w * cfg.max_bodies + d.sa, gb = w * cfg.max_bodies + d.sb;
-        if (B.type[ga] == VXP_BODY_STATIC || B.type[gb] == VXP_BODY_STATIC) continue;
-        ra = d.m.body_a_id;
-        rb = d.m.body_b_id;
-      }
-      ra = findRoot(P, ra);
-      rb = findRoot(P, rb);
-      if (ra != rb) {
-        atomicMin(&P[max(ra, rb)], min(ra, rb));
-        changed = 1;
-      }
+  __syncthreads();
+  for (uint32_t i = tid; i < M; i += bs) {
+    uint32_t ra, rb;
+    if (L) {
+      if (eb[i] == kNone) continue;
+      ra = ea[i];
+      rb = eb[i];
+    } else {
+      const DevManifold& d = ml[m0 + i];
+      uint32_t ga = w * cfg.max_bodies + d.sa, gb = w * cfg.max_bodies + d.sb;
+      if (B.type[ga] == VXP_BODY_STATIC || B.type[gb] == VXP_BODY_STATIC) continue;
+      ra = d.m.body_a_id;
+      rb = d.m.body_b_id;
     }
-    __syncthreads();
-    // pointer jumping
-    for (uint32_t i = tid; i < cap; i += bs) P[i] = findRoot(P, i);
-    __syncthreads();
-    if (!changed) break;
+    unite(P, ra, rb);
   }
+  __syncthreads();
+  for (uint32_t i = tid; i < cap; i += bs) P[i] = findRoot(P, i);  // labels = roots
   __syncthreads();
   __shared__ uint32_t n_isl, largest, n_end, n_sleep, wsum[16], rbase;
   if (tid == 0) {
@@ -2292,7 +2294,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
   const uint32_t MB = cfg.max_bodies, g0 = w * MB;
   const uint32_t m0 = ws[w].man_start, M = ws[w].man_count;
   __shared__ uint32_t tmp[kSolveThreads];
-  __shared__ uint32_t s_left, s_ncol, s_nsolve, s_skipped, s_points, s_nunits, s_changed,
+  __shared__ uint32_t s_left, s_ncol, s_nsolve, s_skipped, s_points, s_nunits,
       s_points_tot;
   extern __shared__ uint4 kdyn[];
   __shared__ uint32_t cnt_m[kUnitsMax], cnt_b[kUnitsMax], off_m[kUnitsMax], off_b[kUnitsMax];
@@ -2399,27 +2401,15 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
     UE[i] = make_uint4(d.sa, d.sb, bits, 0u);
   }
   __syncthreads();
-  // components over solved manifolds between two mobile bodies: hooking to
-  // the smaller root, then pointer jumping, until nothing changes
-  for (int round = 0; round < 100000; round++) {
-    if (tid == 0) s_changed = 0;
-    __syncthreads();
-    for (uint32_t i = tid; i < M; i += bs) {
-      const uint4 e = UE[i];
-      if (e.z != 7u) continue;
-      uint32_t ra = findRoot(P, e.x), rb = findRoot(P, e.y);
-      if (ra != rb) {
-        atomicMin(&P[max(ra, rb)], min(ra, rb));
-        s_changed = 1;
-      }
-    }
-    __syncthreads();
-    for (uint32_t s = tid; s < MB; s += bs) P[s] = findRoot(P, s);
-    __syncthreads();
-    const uint32_t more = s_changed;
-    __syncthreads();
-    if (!more) break;
+  // components over solved manifolds between two mobile bodies, each labelled
+  // by its smallest slot
+  for (uint32_t i = tid; i < M; i += bs) {
+    const uint4 e = UE[i];
+    if (e.z == 7u) unite(P, e.x, e.y);
   }
+  __syncthreads();
+  for (uint32_t s = tid; s < MB; s += bs) P[s] = findRoot(P, s);
+  __syncthreads();
   // solved manifolds per component (indexed by the root slot), then units:
   // components in root order, cut every T manifolds (a component goes to the
   // unit where it starts)
