@@ -577,13 +577,16 @@ int stepAll(vxp_system* s) {
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wcnt.as<uint32_t>(),
                                s->d_woff.as<uint32_t>(), ns + 1, st));
   }
-  // two passes over the same work-item partition: count per wave, scan, write
+  // two passes over the same work-item partition: count per wave, scan, write;
+  // packed keys (world, a, b) are zero above bit 2 * id_bits + log2(worlds)
+  // and written as 32-bit words when that fits
+  const bool narrow = 2 * id_bits + wbits <= 32;
   const uint32_t swv = sweep_waves();
   HIP_TRY(s->d_swc.ensure((size_t)(swv + 1) * 4));
   HIP_TRY(s->d_swo.ensure((size_t)(swv + 1) * 4));
   launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
                      s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), s->d_swc.as<uint32_t>(),
-                     nullptr, nullptr, true, id_bits);
+                     nullptr, nullptr, true, id_bits, narrow);
   {
     size_t cb = exclusive_scan_u32_bytes(swv + 1);
     HIP_TRY(s->d_sort_tmp.ensure(cb));
@@ -595,7 +598,8 @@ int stepAll(vxp_system* s) {
   if (npairs)
     launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
                        s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), nullptr,
-                       s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false, id_bits);
+                       s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false, id_bits,
+                       narrow);
   profEnd(s, VXP_K_PAIRS, pe);
   }
   s->n_pairs = npairs;
@@ -617,13 +621,20 @@ int stepAll(vxp_system* s) {
                                  s->d_pairs.as<uint64_t>(), s->d_t_pos.as<uint32_t>(),
                                  s->d_perm.as<uint32_t>(), npairs, kbits, st));
       perm = s->d_perm.as<uint32_t>();
+    } else if (kbits <= 32) {
+      // 32-bit keys: sorted from the first half of the raw buffer into its second
+      uint32_t* raw = s->d_pairs_raw.as<uint32_t>();
+      size_t kb = sort_keys_u32_bytes(npairs, kbits);
+      HIP_TRY(s->d_sort_tmp.ensure(kb));
+      HIP_TRY(sort_keys_u32(s->d_sort_tmp.p, kb, raw, raw + npairs, npairs, kbits, st));
+      launch_expand_keys32(st, raw + npairs, s->d_pairs.as<uint64_t>(), npairs, id_bits);
     } else {
       size_t kb = sort_keys_u64_bytes(npairs, kbits);
       HIP_TRY(s->d_sort_tmp.ensure(kb));
       HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(),
                             s->d_pairs.as<uint64_t>(), npairs, kbits, st));
     }
-    launch_expand_keys(st, s->d_pairs.as<uint64_t>(), npairs, id_bits);
+    if (tree || kbits > 32) launch_expand_keys(st, s->d_pairs.as<uint64_t>(), npairs, id_bits);
   }
   profEnd(s, VXP_K_PAIRS, pe);
   const uint64_t* keys = s->d_pairs.as<uint64_t>();

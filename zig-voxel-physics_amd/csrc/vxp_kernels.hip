@@ -470,7 +470,9 @@ __global__ void k_sweep_count(DevCfg cfg, const WorldState* ws, const float4* sl
 // owns the contiguous items [gw * q, (gw + 1) * q), q = ceil(total / waves).
 // kCount: wcount[gw] = its overlapping pairs (wcount[waves] = 0).  Otherwise
 // the pairs (inclusive overlap, src/broad_phase.zig:488-526; static-static
-// excluded) are written from woff[gw] on, in work-item order.
+// excluded) are written from woff[gw] on, in work-item order, as packed keys
+// of 32 bits when narrow (the packed width 2 * id_bits + world bits fits) and
+// of 64 otherwise.
 constexpr uint32_t kSweepWaves = 8192;
 
 template <bool kCount>
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
                                                      const float4* shi, uint32_t n_slots,
                                                      const uint32_t* send, const uint32_t* woff_i,
                                                      uint32_t* wcount, const uint32_t* woff_w,
-                                                     uint64_t* out, uint32_t id_bits) {
+                                                     uint64_t* out, uint32_t id_bits, bool narrow) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const uint32_t total = woff_i[n_slots];
@@ -520,9 +522,15 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
         n += (uint32_t)__popcll(em);
       } else {
         // packed key (world, a, b) with ids of id_bits bits, expanded after sorting
-        if (emit)
-          out[o + (uint32_t)__popcll(em & ((1ull << lane) - 1ull))] =
+        if (emit) {
+          const uint64_t k =
               ((uint64_t)w << (2 * id_bits)) | ((uint64_t)min(idi, idj) << id_bits) | max(idi, idj);
+          const uint32_t at = o + (uint32_t)__popcll(em & ((1ull << lane) - 1ull));
+          if (narrow)
+            reinterpret_cast<uint32_t*>(out)[at] = (uint32_t)k;
+          else
+            out[at] = k;
+        }
         o += (uint32_t)__popcll(em);
       }
     }
@@ -3499,17 +3507,28 @@ __global__ void k_expand_keys(uint64_t* keys, uint32_t n, uint32_t id_bits) {
 void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits) {
   if (n) k_expand_keys<<<cdiv(n, 256), 256, 0, s>>>(keys, n, id_bits);
 }
+// The same from packed keys of 32 bits into a separate array.
+__global__ void k_expand_keys32(const uint32_t* in, uint64_t* keys, uint32_t n, uint32_t id_bits) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = in[i], m = (1u << id_bits) - 1u;
+  keys[i] = makeKey(k >> (2 * id_bits), (k >> id_bits) & m, k & m);
+}
+void launch_expand_keys32(hipStream_t s, const uint32_t* in, uint64_t* keys, uint32_t n,
+                          uint32_t id_bits) {
+  if (n) k_expand_keys32<<<cdiv(n, 256), 256, 0, s>>>(in, keys, n, id_bits);
+}
 void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
-                        uint32_t id_bits) {
+                        uint32_t id_bits, bool narrow) {
   constexpr uint32_t blocks = kSweepWaves / 4;
   if (count)
     k_sweep_pairs<true><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount, woff_w,
-                                               out, id_bits);
+                                               out, id_bits, narrow);
   else
     k_sweep_pairs<false><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount,
-                                                woff_w, out, id_bits);
+                                                woff_w, out, id_bits, narrow);
 }
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,

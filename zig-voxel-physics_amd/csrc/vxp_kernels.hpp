@@ -24,8 +24,10 @@ uint32_t sweep_waves();
 void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
-                        uint32_t id_bits);
+                        uint32_t id_bits, bool narrow);
 void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits);
+void launch_expand_keys32(hipStream_t s, const uint32_t* in, uint64_t* keys, uint32_t n,
+                          uint32_t id_bits);
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
@@ -120,6 +122,9 @@ hipError_t sort_pairs_u64_u32(void* tmp, size_t bytes, const uint64_t* kin, uint
                               hipStream_t s);
 size_t sort_keys_u64_bytes(uint32_t n, uint32_t end_bit);
 hipError_t sort_keys_u64(void* tmp, size_t bytes, const uint64_t* kin, uint64_t* kout, uint32_t n,
+                         uint32_t end_bit, hipStream_t s);
+size_t sort_keys_u32_bytes(uint32_t n, uint32_t end_bit);
+hipError_t sort_keys_u32(void* tmp, size_t bytes, const uint32_t* kin, uint32_t* kout, uint32_t n,
                          uint32_t end_bit, hipStream_t s);
 size_t exclusive_scan_u32_bytes(uint32_t n);
 hipError_t exclusive_scan_u32(void* tmp, size_t bytes, const uint32_t* in, uint32_t* out,

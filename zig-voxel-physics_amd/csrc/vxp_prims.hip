@@ -32,6 +32,16 @@ hipError_t sort_keys_u64(void* tmp, size_t bytes, const uint64_t* kin, uint64_t*
                          uint32_t end_bit, hipStream_t s) {
   return rocprim::radix_sort_keys(tmp, bytes, kin, kout, n, 0, end_bit, s);
 }
+size_t sort_keys_u32_bytes(uint32_t n, uint32_t end_bit) {
+  size_t b = 0;
+  (void)rocprim::radix_sort_keys(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0,
+                                 end_bit);
+  return b;
+}
+hipError_t sort_keys_u32(void* tmp, size_t bytes, const uint32_t* kin, uint32_t* kout, uint32_t n,
+                         uint32_t end_bit, hipStream_t s) {
+  return rocprim::radix_sort_keys(tmp, bytes, kin, kout, n, 0, end_bit, s);
+}
 size_t exclusive_scan_u32_bytes(uint32_t n) {
   size_t b = 0;
   (void)rocprim::exclusive_scan(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr, 0u, n,
