@@ -339,7 +339,7 @@ __device__ __forceinline__ float comp(const float4& v, uint32_t a) {
 __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n_worlds) {
   uint32_t w = blockIdx.x;
   if (w >= n_worlds) return;
-  __shared__ float lo[3][256], hi[3][256], ext[3][256];
+  __shared__ float lo[3][1024], hi[3][1024], ext[3][1024];
   float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
   float e[3] = {0.f, 0.f, 0.f};
   for (uint32_t s = threadIdx.x; s < cfg.max_bodies; s += blockDim.x) {
@@ -3283,13 +3283,18 @@ __global__ void k_update_begin(vxp_stats* st, uint32_t n_worlds, float acc) {
 // launchers
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+// Threads of the block-per-world scans over body slots (step begin and end,
+// sweep axis): up to four slots per thread, power of two in 256..1024.
+static inline uint32_t slotThreads(uint32_t max_bodies) {
+  return max_bodies > 2048 ? 1024u : max_bodies > 1024 ? 512u : 256u;
+}
 
 void launch_update_begin(hipStream_t s, vxp_stats* st, uint32_t nw, float acc) {
   k_update_begin<<<cdiv(nw, 256), 256, 0, s>>>(st, nw, acc);
 }
 void launch_step_begin(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, uint32_t nw, uint32_t* step_ctr) {
-  k_step_begin<<<nw, 256, 0, s>>>(cfg, B, ws, st, nw, step_ctr);
+  k_step_begin<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, ws, st, nw, step_ctr);
 }
 // ---------------------------------------------------------------------------
 // Broad-phase trees (pair order VXP_PAIR_ORDER_TREE)
@@ -3475,17 +3480,17 @@ void launch_fat_aabb(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const
 }
 void launch_sweep_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw) {
-  k_world_axis<<<nw, 256, 0, s>>>(cfg, B, ws, nw);
+  k_world_axis<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, ws, nw);
   k_sweep_keys<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, ws, keys, vals, n_slots);
 }
 bool launch_sweep_sort(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        uint32_t* order, uint32_t nw) {
   const uint32_t MB = cfg.max_bodies;
   if (MB > 8192) return false;
-  k_world_axis<<<nw, 256, 0, s>>>(cfg, B, ws, nw);
+  k_world_axis<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, ws, nw);
   if (MB <= 1024) k_sweep_sort<256, 4><<<nw, 256, 0, s>>>(cfg, B, ws, order);
   else if (MB <= 2048) k_sweep_sort<256, 8><<<nw, 256, 0, s>>>(cfg, B, ws, order);
-  else if (MB <= 4096) k_sweep_sort<512, 8><<<nw, 512, 0, s>>>(cfg, B, ws, order);
+  else if (MB <= 4096) k_sweep_sort<1024, 4><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
   else k_sweep_sort<1024, 8><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
   return true;
 }
@@ -3691,7 +3696,7 @@ void launch_collider_mass(hipStream_t s, const int32_t* pos, const uint8_t* vox,
 }
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps, float acc) {
-  k_update_end<<<nw, 256, 0, s>>>(cfg, B, st, nw, substeps, acc);
+  k_update_end<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, st, nw, substeps, acc);
 }
 
 void launch_copy_many(hipStream_t s, const CopyDesc* d, uint32_t n) {
