@@ -813,134 +813,147 @@ __global__ void k_cache_scatter(const uint32_t* cflag, const uint32_t* cidx, con
   if (i < n_pairs && cflag[i]) clist[cidx[i]] = order ? order[i] : i;
 }
 
-// One wave per world replays the gets/puts of its cache-eligible pairs
-// (clist[cidx[pair_lo] .. cidx[pair_hi])) strictly in pair order, with the
-// key table staged in LDS: lanes 0..15 probe the 16-entry window of one
-// get/put in parallel and a ballot picks the first stopping entry.
-__global__ __launch_bounds__(64) void k_cache_replay(DevCfg cfg, WorldState* ws, vxp_stats* st,
-                                                     PairRec* pr, CacheKeyRec* keys,
-                                                     const uint32_t* cidx, const uint32_t* clist,
-                                                     uint32_t n_worlds) {
+// One block per world: its kCacheThreads threads stage the key table in LDS
+// and write it back; wave 0 replays the gets/puts of the world's
+// cache-eligible pairs (clist[cidx[pair_lo] .. cidx[pair_hi])) strictly in
+// pair order: lanes 0..15 probe the 16-entry window of one get/put in
+// parallel and a ballot picks the first stopping entry.  A world without
+// gets/puts this step leaves its table untouched.
+constexpr uint32_t kCacheThreads = 256;
+__device__ __forceinline__ void copyWords(uint2* dst, const uint2* src, uint32_t n) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t i = tid;
+  for (; i + 3 * kCacheThreads < n; i += 4 * kCacheThreads) {
+    const uint2 a = src[i], b = src[i + kCacheThreads], c = src[i + 2 * kCacheThreads],
+                d = src[i + 3 * kCacheThreads];
+    dst[i] = a;
+    dst[i + kCacheThreads] = b;
+    dst[i + 2 * kCacheThreads] = c;
+    dst[i + 3 * kCacheThreads] = d;
+  }
+  for (; i < n; i += kCacheThreads) dst[i] = src[i];
+}
+__global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, WorldState* ws,
+                                                                vxp_stats* st, PairRec* pr,
+                                                                CacheKeyRec* keys,
+                                                                const uint32_t* cidx,
+                                                                const uint32_t* clist,
+                                                                uint32_t n_worlds) {
   extern __shared__ CacheKeyRec E[];
-  const uint32_t w = blockIdx.x, lane = threadIdx.x;
+  const uint32_t w = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1);
   if (w >= n_worlds || ws[w].skip) return;
   const uint32_t frame = ws[w].frame + 1;
-  waveSync();
-  if (lane == 0) ws[w].frame = frame;
+  __syncthreads();
+  if (tid == 0) ws[w].frame = frame;
   if (!cfg.use_cache) return;
-  const uint32_t cap = cfg.cache_cap, maxp = cap < 16 ? cap : 16;
-  CacheKeyRec* G = keys + (size_t)w * cap;
-  // stage the key table as 16-byte words (an entry is 40 bytes)
-  static_assert(sizeof(CacheKeyRec) % 8 == 0, "cache key record");
-  const uint32_t nq = cap * (uint32_t)sizeof(CacheKeyRec) / 8;
-  {
-    const uint2* g8 = reinterpret_cast<const uint2*>(G);
-    uint2* e8 = reinterpret_cast<uint2*>(E);
-    for (uint32_t i = lane; i < nq; i += kWave) e8[i] = g8[i];
-  }
   const uint32_t lo = ws[w].pair_lo, hi = ws[w].pair_hi;
   const uint32_t c0 = lo < hi ? cidx[lo] : 0, total = lo < hi ? cidx[hi] - c0 : 0;
-  waveSync();
+  if (!total) return;
+  const uint32_t cap = cfg.cache_cap, maxp = cap < 16 ? cap : 16;
+  CacheKeyRec* G = keys + (size_t)w * cap;
+  // stage the key table as 8-byte words (an entry is 40 bytes)
+  static_assert(sizeof(CacheKeyRec) % 8 == 0, "cache key record");
+  const uint32_t nq = cap * (uint32_t)sizeof(CacheKeyRec) / 8;
+  copyWords(reinterpret_cast<uint2*>(E), reinterpret_cast<const uint2*>(G), nq);
+  __syncthreads();
   uint32_t hits = 0, misses = 0;
-  const bool probe = lane < maxp;
-  // pair data of a batch of 64 pairs; the next batch's is loaded (and each
-  // lane hashes its own pair) while the current batch is walked
-  auto fetch = [&](uint32_t base, uint32_t& p, uint32_t& ida, uint32_t& idb, uint64_t& th) {
-    const bool inb = base + lane < total;
-    p = inb ? clist[c0 + base + lane] : 0;
-    ida = inb ? pr[p].ida : 0;
-    idb = inb ? pr[p].idb : 0;
-    th = inb ? pr[p].th : 0;
-  };
-  uint32_t np = 0, nida = 0, nidb = 0;
-  uint64_t nth = 0;
-  if (total) fetch(0, np, nida, nidb, nth);
-  for (uint32_t base = 0; base < total; base += kWave) {
-    const bool inb = base + lane < total;
-    const uint32_t p = np, ida = nida, idb = nidb;
-    const uint64_t th = nth;
-    if (base + kWave < total) fetch(base + kWave, np, nida, nidb, nth);
-    const uint32_t my_start = (uint32_t)(cacheHash(ida, idb, th) % cap);
-    uint32_t my_flag = PF_CACHE, my_aux = kNone;
-    const uint32_t nb = min(total - base, (uint32_t)kWave);
-    // The window of the pair being resolved is held in registers (lane l <
-    // maxp: entry start + l).  The next pair's window is loaded before this
-    // pair's update is stored; LDS operations of a wave complete in order, so
-    // that load sees every earlier update and only this pair's is patched in.
-    uint32_t idx = cacheSlot(rdlane(my_start, 0), lane, cap);
-    CacheWin cur = probe ? cacheWin(E[idx]) : CacheWin{};
-    for (uint32_t j = 0; j < nb; j++) {
-      const uint32_t a = rdlane(ida, (int)j), b = rdlane(idb, (int)j);
-      const uint64_t t = ((uint64_t)rdlane((uint32_t)(th >> 32), (int)j) << 32) |
-                         rdlane((uint32_t)th, (int)j);
-      const bool more = j + 1 < nb;
-      uint32_t nidx = 0;
-      CacheWin nxt{};
-      if (more) {
-        nidx = cacheSlot(rdlane(my_start, (int)j + 1), lane, cap);
-        if (probe) nxt = cacheWin(E[nidx]);
-      }
-      const bool eq = cur.valid && cur.a == a && cur.b == b && cur.th == t;
-      const uint64_t stop = __ballot(probe && (!cur.valid || eq));
-      uint32_t rflag, raux;
-      if (stop) {
-        const int f = __ffsll((long long)stop) - 1;
-        raux = rdlane(idx, f);
-        rflag = rdlane((uint32_t)eq, f) ? PF_HIT : PF_MISS;
-      } else {
-        // window full: evict the first entry with the smallest last_used
-        // (last_used only ever holds a 32-bit frame number; the probe lanes
-        // are lanes 0..maxp-1, all in the first row of 16)
-        const uint32_t lu = probe ? (uint32_t)cur.lu : 0xFFFFFFFFu;
-        const uint32_t lo = rdlane(rowMin16(lu), 0);
-        const uint64_t at = __ballot(probe && lu == lo);
-        raux = rdlane(idx, __ffsll((long long)at) - 1);
-        rflag = PF_MISS;
-      }
-      if (lane == 0) {
-        if (rflag == PF_HIT) {
-          E[raux].last_used = frame;
-          hits++;
+  if (tid < kWave) {
+    const bool probe = lane < maxp;
+    // pair data of a batch of 64 pairs; the next batch's is loaded (and each
+    // lane hashes its own pair) while the current batch is walked
+    auto fetch = [&](uint32_t base, uint32_t& p, uint32_t& ida, uint32_t& idb, uint64_t& th) {
+      const bool inb = base + lane < total;
+      p = inb ? clist[c0 + base + lane] : 0;
+      ida = inb ? pr[p].ida : 0;
+      idb = inb ? pr[p].idb : 0;
+      th = inb ? pr[p].th : 0;
+    };
+    uint32_t np = 0, nida = 0, nidb = 0;
+    uint64_t nth = 0;
+    if (total) fetch(0, np, nida, nidb, nth);
+    for (uint32_t base = 0; base < total; base += kWave) {
+      const bool inb = base + lane < total;
+      const uint32_t p = np, ida = nida, idb = nidb;
+      const uint64_t th = nth;
+      if (base + kWave < total) fetch(base + kWave, np, nida, nidb, nth);
+      const uint32_t my_start = (uint32_t)(cacheHash(ida, idb, th) % cap);
+      uint32_t my_flag = PF_CACHE, my_aux = kNone;
+      const uint32_t nb = min(total - base, (uint32_t)kWave);
+      // The window of the pair being resolved is held in registers (lane l <
+      // maxp: entry start + l).  The next pair's window is loaded before this
+      // pair's update is stored; LDS operations of a wave complete in order, so
+      // that load sees every earlier update and only this pair's is patched in.
+      uint32_t idx = cacheSlot(rdlane(my_start, 0), lane, cap);
+      CacheWin cur = probe ? cacheWin(E[idx]) : CacheWin{};
+      for (uint32_t j = 0; j < nb; j++) {
+        const uint32_t a = rdlane(ida, (int)j), b = rdlane(idb, (int)j);
+        const uint64_t t = ((uint64_t)rdlane((uint32_t)(th >> 32), (int)j) << 32) |
+                           rdlane((uint32_t)th, (int)j);
+        const bool more = j + 1 < nb;
+        uint32_t nidx = 0;
+        CacheWin nxt{};
+        if (more) {
+          nidx = cacheSlot(rdlane(my_start, (int)j + 1), lane, cap);
+          if (probe) nxt = cacheWin(E[nidx]);
+        }
+        const bool eq = cur.valid && cur.a == a && cur.b == b && cur.th == t;
+        const uint64_t stop = __ballot(probe && (!cur.valid || eq));
+        uint32_t rflag, raux;
+        if (stop) {
+          const int f = __ffsll((long long)stop) - 1;
+          raux = rdlane(idx, f);
+          rflag = rdlane((uint32_t)eq, f) ? PF_HIT : PF_MISS;
         } else {
-          CacheKeyRec& e = E[raux];
-          e.a = a;
-          e.b = b;
-          e.th = t;
-          e.last_used = frame;
-          e.valid = 1;
-          e.owner_frame = frame;
-          e.owner_pair = rdlane(p, (int)j);
-          misses++;
+          // window full: evict the first entry with the smallest last_used
+          // (last_used only ever holds a 32-bit frame number; the probe lanes
+          // are lanes 0..maxp-1, all in the first row of 16)
+          const uint32_t lu = probe ? (uint32_t)cur.lu : 0xFFFFFFFFu;
+          const uint32_t lo = rdlane(rowMin16(lu), 0);
+          const uint64_t at = __ballot(probe && lu == lo);
+          raux = rdlane(idx, __ffsll((long long)at) - 1);
+          rflag = PF_MISS;
         }
-      }
-      if (more && probe && nidx == raux) {
-        nxt.lu = frame;
-        if (rflag == PF_MISS) {
-          nxt.a = a;
-          nxt.b = b;
-          nxt.th = t;
-          nxt.valid = 1;
+        if (lane == 0) {
+          if (rflag == PF_HIT) {
+            E[raux].last_used = frame;
+            hits++;
+          } else {
+            CacheKeyRec& e = E[raux];
+            e.a = a;
+            e.b = b;
+            e.th = t;
+            e.last_used = frame;
+            e.valid = 1;
+            e.owner_frame = frame;
+            e.owner_pair = rdlane(p, (int)j);
+            misses++;
+          }
         }
+        if (more && probe && nidx == raux) {
+          nxt.lu = frame;
+          if (rflag == PF_MISS) {
+            nxt.a = a;
+            nxt.b = b;
+            nxt.th = t;
+            nxt.valid = 1;
+          }
+        }
+        if (lane == j) {
+          my_flag = rflag;
+          my_aux = raux;
+        }
+        cur = nxt;
+        idx = nidx;
       }
-      if (lane == j) {
-        my_flag = rflag;
-        my_aux = raux;
+      if (inb) {
+        pr[p].flag = my_flag;
+        pr[p].aux = my_aux;
       }
-      cur = nxt;
-      idx = nidx;
-    }
-    if (inb) {
-      pr[p].flag = my_flag;
-      pr[p].aux = my_aux;
     }
   }
-  waveSync();
-  {
-    const uint2* e8 = reinterpret_cast<const uint2*>(E);
-    uint2* g8 = reinterpret_cast<uint2*>(G);
-    for (uint32_t i = lane; i < nq; i += kWave) g8[i] = e8[i];
-  }
-  if (lane == 0) {
+  __syncthreads();
+  copyWords(reinterpret_cast<uint2*>(G), reinterpret_cast<const uint2*>(E), nq);
+  if (tid == 0) {
     st[w].cache_hits += hits;
     st[w].cache_misses += misses;
   }
@@ -3550,7 +3563,7 @@ void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_s
                          uint32_t n_pairs, uint32_t nw) {
   if (n_pairs)
     k_cache_scatter<<<cdiv(n_pairs, 256), 256, 0, s>>>(cflag, cidx, order, n_pairs, clist);
-  k_cache_replay<<<nw, 64, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(cfg, ws, st, pr, keys, cidx,
+  k_cache_replay<<<nw, kCacheThreads, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(cfg, ws, st, pr, keys, cidx,
                                                                       clist, nw);
 }
 void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts) {
