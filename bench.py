@@ -57,14 +57,16 @@ def solver_bytes(st, substeps, iters):
             + min(J, 1) * (K * 40 + M * 192) + K * 20)
 
 
-def run(sys_, sc, warmup, steps, profile=False):
-    """Step `steps` frames after `warmup`; per-frame wall time of update()."""
+def run(sys_, sc, warmup, steps, profile=0):
+    """Step `steps` frames after `warmup`; per-frame wall time of update().
+    `profile`: the library's timing mode over those frames (0 off, 1 every
+    phase, 2 only the solve units: two HIP events per step)."""
     lib = sys_.lib
     cfg = sc.config
     for _ in range(warmup):
         sys_.update(sc.dt)
     if profile:
-        lib.profile_enable(sys_.ptr, 1)
+        lib.profile_enable(sys_.ptr, profile)
     frame_ms, body_substeps, alg, cands = [], 0, 0, 0
     for _ in range(steps):
         t = time.perf_counter()
@@ -173,6 +175,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=590)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--breakdown-steps", type=int, default=100,
+                    help="frames after the timed ones with every phase timed (kernel_ms)")
     ap.add_argument("--batched-steps", type=int, default=290)
     ap.add_argument("--looped-worlds", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -198,11 +202,17 @@ def main():
     log("4096-body scene")
     sc = scenes.c1_large()
     s1 = vxp.from_scene(sc, lib=lib)
-    plain = run(s1, sc, args.warmup, args.steps, profile=True)
+    # the timed frames time only the solve units (the roofline's kernel), with
+    # HIP events on their stream; timing every phase would add two events per
+    # phase and keep the island labels off their side stream
+    plain = run(s1, sc, args.warmup, args.steps, profile=vxp._abi.PROFILE_SOLVE_UNITS)
     final = s1.stats()
+    brk = (run(s1, sc, 0, args.breakdown_steps, profile=vxp._abi.PROFILE_ALL)
+           if args.breakdown_steps > 0 else None)
     s1.close()
     kern = plain["kern"]
-    dom = max(kern, key=lambda k: kern[k][0])
+    bk = brk["kern"] if brk else {}
+    dom = max(bk, key=lambda k: bk[k][0]) if bk else "solve_units"
     solve_ms, solve_n = kern["solve_units"]
     avg_solve_s = solve_ms / max(solve_n, 1) / 1e3
     alg_per_launch = plain["alg_solver_bytes"] / max(solve_n, 1)
@@ -339,7 +349,9 @@ def main():
                      "traffic": traffic,
                      "avg_launch_ms": avg_solve_s * 1e3, "launches": solve_n,
                      "alg_bytes_per_launch": alg_per_launch, "dominant_kernel": dom},
-        "kernel_ms": {k: round(v[0], 3) for k, v in kern.items()},
+        "kernel_ms": {k: round(v[0], 3) for k, v in bk.items()},
+        "kernel_ms_frames": (f"{args.warmup + args.steps}..{args.warmup + args.steps + args.breakdown_steps - 1}"
+                             " (every phase timed; not in the timed region)") if bk else None,
         "tuning": plain["tuning"],
         "final_stats": {k: final[k] for k in ("manifolds", "colors", "sleeping_bodies",
                                               "voxels_checked", "broad_phase_pairs")},

@@ -350,7 +350,15 @@ int vxp_coloring_get_w(vxp_system* sys, uint32_t world, uint8_t* colors, uint32_
 #define VXP_K_SOLVE_UNITS 10 /* the solve units: substeps, first position pass */
 #define VXP_K_SOLVE_TAIL 11  /* further position passes, writeback, sleep */
 #define VXP_K_COUNT 12
-/* Enable (1) or disable (0) timing; enabling clears the accumulators. */
+/* Timing modes: every phase with HIP events on the system's stream, plus the
+ * solver's device timestamps (and the island labels kept on that stream);
+ * or only VXP_K_SOLVE_UNITS, two events per step, which leaves the step's
+ * schedule as it is unprofiled. */
+#define VXP_PROFILE_OFF 0
+#define VXP_PROFILE_ALL 1
+#define VXP_PROFILE_SOLVE_UNITS 2
+/* Set the timing mode (VXP_PROFILE_*; VXP_ERR_ARGUMENT otherwise); a mode
+ * other than OFF clears the accumulators. */
 int vxp_profile_enable(vxp_system* sys, int on);
 /* Accumulated device time (ms) and launch count of kernel `id`. */
 int vxp_profile_get(vxp_system* sys, uint32_t id, double* total_ms, uint64_t* launches);

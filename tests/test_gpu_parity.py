@@ -228,3 +228,44 @@ def test_body_mutators_between_frames(product, oracle):
         assert a.getWorldAABB(ha) == b.getWorldAABB(hb)
     a.close()
     b.close()
+
+
+def test_profile_modes_change_no_result(product):
+    # Timing every phase keeps the island labels on the main stream; timing
+    # only the solve units (or nothing) runs them beside the solver on a second
+    # stream.  Every mode gives the same bits, including island labels and
+    # island statistics, and the solve-units mode times nothing else.
+    import ctypes as C
+    sc = scenes.c0_small(n_side=4, frames=150)
+    frames = 150
+    runs = []
+    for mode in (_abi.PROFILE_OFF, _abi.PROFILE_ALL, _abi.PROFILE_SOLVE_UNITS):
+        print(f"profile mode {mode}", flush=True)
+        s = vxp.from_scene(sc, lib=product)
+        assert s.lib.profile_enable(s.ptr, 7) == _abi.VXP_ERR_ARGUMENT
+        assert s.lib.profile_enable(s.ptr, mode) == _abi.VXP_OK
+        labels = []
+        for _ in range(frames):
+            s.update(sc.dt)
+            labels.append(s.islands())
+        launches = []
+        for i in range(len(_abi.KERNEL_NAMES)):
+            ms, n = C.c_double(), C.c_uint64()
+            s.lib.profile_get(s.ptr, i, C.byref(ms), C.byref(n))
+            launches.append(n.value)
+        su = _abi.KERNEL_NAMES.index("solve_units")
+        if mode == _abi.PROFILE_SOLVE_UNITS:
+            assert launches[su] > 0 and sum(launches) == launches[su]
+        elif mode == _abi.PROFILE_ALL:
+            assert all(launches[i] > 0 for i in (su, _abi.KERNEL_NAMES.index("islands")))
+        else:
+            assert sum(launches) == 0
+        runs.append((s.bodies(), s.manifolds(), s.stats(), s.getIslandStats(), labels))
+        s.close()
+    for other in runs[1:]:
+        assert compare.diff_bodies(other[0], runs[0][0]) is None
+        assert compare.diff_manifolds(other[1], runs[0][1]) is None
+        assert compare.diff_stats(other[2], runs[0][2]) is None
+        assert other[3] == runs[0][3]
+        for a, b in zip(other[4], runs[0][4]):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))

@@ -360,14 +360,19 @@ hipEvent_t takeEvent(vxp_system* s) {
   return e;
 }
 // Brackets a group of launches of kernel `id` with events (when profiling).
-hipEvent_t profBegin(vxp_system* s) {
-  if (!s->prof) return nullptr;
+// Phase `id` is timed in mode VXP_PROFILE_ALL, and the solve units also in
+// VXP_PROFILE_SOLVE_UNITS.
+bool profTimed(const vxp_system* s, int id) {
+  return s->prof == VXP_PROFILE_ALL || (s->prof == VXP_PROFILE_SOLVE_UNITS && id == VXP_K_SOLVE_UNITS);
+}
+hipEvent_t profBegin(vxp_system* s, int id) {
+  if (!profTimed(s, id)) return nullptr;
   hipEvent_t a = takeEvent(s);
   (void)hipEventRecord(a, s->stream);
   return a;
 }
 void profEnd(vxp_system* s, int id, hipEvent_t a) {
-  if (!s->prof || !a) return;
+  if (!a) return;
   hipEvent_t b = takeEvent(s);
   (void)hipEventRecord(b, s->stream);
   s->pending.push_back({id, a, b});
@@ -489,7 +494,23 @@ int tuneEnd(vxp_system* s) {
   return VXP_OK;
 }
 
+// Orders the main stream after the side stream's last work.
+int joinSide(vxp_system* s) {
+  if (s->side_pending) {
+    HIP_TRY(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    s->side_pending = false;
+  }
+  return VXP_OK;
+}
+
+static int stepAllImpl(vxp_system* s);
 int stepAll(vxp_system* s) {
+  const int rc = stepAllImpl(s);
+  const int jr = joinSide(s);  // a step that ended early leaves the join undone
+  return rc != VXP_OK ? rc : jr;
+}
+
+static int stepAllImpl(vxp_system* s) {
   hipStream_t st = s->stream;
   RC(ensureDevice(s));
   DevCfg cfg = devCfg(s);
@@ -503,7 +524,7 @@ int stepAll(vxp_system* s) {
   // ContactManager.beginFrame: the current manifolds become the previous ones
   s->cur ^= 1;
   DevManifold* prev = s->d_ml[s->cur ^ 1].as<DevManifold>();
-  hipEvent_t pe = profBegin(s);
+  hipEvent_t pe = profBegin(s, VXP_K_STEP_BEGIN);
   HIP_TRY(s->d_step_ctr.ensure(kStepCounters * 4));
   launch_step_begin(st, cfg, B, ws, stats, nw, s->d_step_ctr.as<uint32_t>());
   profEnd(s, VXP_K_STEP_BEGIN, pe);
@@ -513,7 +534,7 @@ int stepAll(vxp_system* s) {
     HIP_TRY(s->d_t_moved.ensure(ns));
     moved = s->d_t_moved.as<uint8_t>();
   }
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_FAT_AABB);
   launch_fat_aabb(st, cfg, B, C, ws, ns, moved);
   profEnd(s, VXP_K_FAT_AABB, pe);
 
@@ -529,7 +550,7 @@ int stepAll(vxp_system* s) {
     const uint32_t nq = nw * 2 * s->t_nc;
     HIP_TRY(s->d_t_cnt.ensure((size_t)(nq + 1) * 4));
     HIP_TRY(s->d_t_off.ensure((size_t)(nq + 1) * 4));
-    pe = profBegin(s);
+    pe = profBegin(s, VXP_K_PAIRS);
     launch_tree_update(st, cfg, B, ws, T, moved, nw, s->lds_limit);
     launch_tree_pairs(st, ws, T, nw, s->d_t_cnt.as<uint32_t>(), nullptr, nullptr, nullptr, id_bits);
     HIP_TRY(hipMemsetAsync(s->d_t_cnt.as<uint32_t>() + nq, 0, 4, st));
@@ -555,7 +576,7 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_svals2.ensure((size_t)ns * 4));
   HIP_TRY(s->d_slo.ensure((size_t)ns * 16));
   HIP_TRY(s->d_shi.ensure((size_t)ns * 16));
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_PAIRS);
   if (!launch_sweep_sort(st, cfg, B, ws, s->d_svals2.as<uint32_t>(), nw)) {
     // worlds above the block sort's size: device-wide sort of (world << 32 | coordinate)
     // keys, which are zero above bit 32 + log2(worlds)
@@ -605,7 +626,7 @@ int stepAll(vxp_system* s) {
   s->n_pairs = npairs;
   HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
   const uint32_t* perm = nullptr;  // key position -> tree position
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_PAIRS);
   if (npairs) {
     // packed pair keys (world, a, b) are zero above bit 2 * id_bits + log2(worlds)
     const uint32_t kbits = 2 * id_bits + wbits;
@@ -654,7 +675,7 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_staging.ensure((size_t)np1 * sizeof(DevManifold)));
   HIP_TRY(s->d_cstage.ensure((size_t)np1 * sizeof(CacheVal)));
   PairRec* pr = s->d_pr.as<PairRec>();
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_PAIR_PROCESS);
   // the filtered-pair cap (src/system.zig:594-616), counted in the narrow
   // phase's order over the pairs that are not both asleep
   const uint8_t* drop = nullptr;
@@ -675,7 +696,7 @@ int stepAll(vxp_system* s) {
                       s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(),
                       s->d_wlist.as<uint32_t>(), s->d_step_ctr.as<uint32_t>());
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_CACHE_REPLAY);
   // cache-eligible flags in the narrow phase's order, and their positions
   uint32_t* cflag_r = s->d_cflag.as<uint32_t>();
   uint32_t* cidx_r = s->d_cidx.as<uint32_t>();
@@ -717,7 +738,7 @@ int stepAll(vxp_system* s) {
   HIP_TRY(s->d_lane_n.ensure((size_t)std::max(total_wi, 1u) * 64));
   HIP_TRY(s->d_wi_pair.ensure((size_t)std::max(total_wi, 1u) * 4));
   // pass 1 counts, the scan places every work item's candidates, pass 2 writes
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_PROBE);
   uint32_t ncand = 0;
   if (total_wi) {
     launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), npairs,
@@ -738,7 +759,7 @@ int stepAll(vxp_system* s) {
                  s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(), s->d_cand.as<Cand>(),
                  false);
   profEnd(s, VXP_K_PROBE, pe);
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_REDUCE);
   launch_reduce(st, cfg, B, C, ws, stats, pr, keys, npairs, s->d_wlist.as<uint32_t>(),
                 s->d_step_ctr.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
@@ -747,7 +768,7 @@ int stepAll(vxp_system* s) {
                 s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>(),
                 s->d_rstat.as<RStat>());
   profEnd(s, VXP_K_REDUCE, pe);
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_COMPACT);
   if (npairs) {
     size_t sb = exclusive_scan_u32_bytes(npairs);
     HIP_TRY(s->d_sort_tmp.ensure(sb));
@@ -764,7 +785,7 @@ int stepAll(vxp_system* s) {
   uint32_t nm1 = std::max(nman, 1u);
   HIP_TRY(s->d_ml[s->cur].ensure((size_t)nm1 * sizeof(DevManifold)));
   DevManifold* ml = s->d_ml[s->cur].as<DevManifold>();
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_COMPACT);
   launch_scatter(st, ws, pr, npairs, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(),
                  s->d_staging.as<DevManifold>(), ml);
   launch_copy_skipped(st, ws, prev, ml, nw);
@@ -773,9 +794,22 @@ int stepAll(vxp_system* s) {
   // ---- islands / colouring artefacts ------------------------------------------
   if (s->cfg.enable_islands) {
     RC(ensureIslands(s));
-    pe = profBegin(s);
-    launch_islands(st, cfg, B, ws, stats, ml, prev, islandArrays(s), nw, max_man, s->lds_limit);
+    // the labels read the manifolds and the pre-solve body state, and the
+    // solver reads nothing they write: on the side stream they overlap the
+    // solver's prep and units (kept in line when phases are being timed)
+    hipStream_t is = st;
+    if (s->prof != VXP_PROFILE_ALL) {
+      HIP_TRY(hipEventRecord(s->ev_fork, st));
+      HIP_TRY(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+      is = s->side;
+    }
+    pe = profBegin(s, VXP_K_ISLANDS);
+    launch_islands(is, cfg, B, ws, stats, ml, prev, islandArrays(s), nw, max_man, s->lds_limit);
     profEnd(s, VXP_K_ISLANDS, pe);
+    if (is != st) {
+      HIP_TRY(hipEventRecord(s->ev_join, s->side));
+      s->side_pending = true;
+    }
   }
   if (s->cfg.enable_coloring) {
     HIP_TRY(s->d_ref_masks.ensure((size_t)nw * s->id_cap * 32));
@@ -822,7 +856,7 @@ int stepAll(vxp_system* s) {
   S.order = s->s_order.as<uint32_t>();
   S.rec = s->s_rec.as<float4>();
   S.pts = s->s_pts.as<float4>();
-  S.tstamp = s->prof ? s->s_tstamp.as<uint64_t>() : nullptr;
+  S.tstamp = s->prof == VXP_PROFILE_ALL ? s->s_tstamp.as<uint64_t>() : nullptr;
   S.upar = s->s_upar.as<uint32_t>();
   S.ucnt = s->s_ucnt.as<uint32_t>();
   S.uoff = s->s_uoff.as<uint32_t>();
@@ -838,11 +872,11 @@ int stepAll(vxp_system* s) {
   cfg.unit_min = std::max(64u, nman / (2 * std::max(s->n_cu, 1u)));
   uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
   RC(tuneBegin(s, cfg));
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_SOLVE);
   launch_solve_prep(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), ctr, nw, max_man,
                     s->lds_limit);
   profEnd(s, VXP_K_SOLVE, pe);
-  pe = profBegin(s);
+  pe = profBegin(s, VXP_K_SOLVE_UNITS);
   launch_unit_order(st, S, ctr, s->s_uqueue2.as<uint32_t>());
   launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit, unitThreads(s));
   if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
@@ -851,7 +885,8 @@ int stepAll(vxp_system* s) {
     launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, 0, unitThreads(s));
   }
   profEnd(s, VXP_K_SOLVE_UNITS, pe);
-  pe = profBegin(s);
+  RC(joinSide(s));  // before the writeback overwrites the velocities the labels read
+  pe = profBegin(s, VXP_K_SOLVE_TAIL);
   launch_solve_tail(st, cfg, B, ws, stats, S, ctr, nw, s->n_cu);
   profEnd(s, VXP_K_SOLVE_TAIL, pe);
   RC(tuneEnd(s));
@@ -905,7 +940,9 @@ int vxp_profile_enable(vxp_system* s, int on) {
   if (!s) return VXP_ERR_ARGUMENT;
   HIP_TRY(hipStreamSynchronize(s->stream));
   profFlush(s);
-  s->prof = on != 0;
+  if (on != VXP_PROFILE_OFF && on != VXP_PROFILE_ALL && on != VXP_PROFILE_SOLVE_UNITS)
+    return VXP_ERR_ARGUMENT;
+  s->prof = on;
   if (s->prof)
     for (int i = 0; i < VXP_K_COUNT; i++) {
       s->prof_ms[i] = 0.0;
@@ -1060,6 +1097,9 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
   int rc = VXP_OK;
   auto fail = [&](int r) {
     if (s->h_read) (void)hipHostFree(s->h_read);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return r;
@@ -1074,7 +1114,10 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0)
       s->n_cu = (uint32_t)cu;
   }
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
     g_err = "hipStreamCreate failed";
     return fail(VXP_ERR_DEVICE);
   }
@@ -1133,6 +1176,10 @@ int vxp_system_destroy(vxp_system* s) {
   if (s->h_read) (void)hipHostFree(s->h_read);
   if (s->tune.a) (void)hipEventDestroy(s->tune.a);
   if (s->tune.b) (void)hipEventDestroy(s->tune.b);
+  (void)hipStreamSynchronize(s->side);
+  (void)hipEventDestroy(s->ev_fork);
+  (void)hipEventDestroy(s->ev_join);
+  (void)hipStreamDestroy(s->side);
   (void)hipStreamDestroy(s->stream);
   delete s;
   return VXP_OK;

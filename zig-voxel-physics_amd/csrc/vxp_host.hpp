@@ -104,6 +104,12 @@ struct vxp_system {
   vxp_config cfg;
   uint32_t nw = 1, MB = 0, id_cap = 0;
   hipStream_t stream = nullptr;
+  // Second stream for the island labels, which run beside the solver's prep
+  // and units: ev_fork orders them after the manifolds, ev_join orders the
+  // solver's writeback (and everything after it) after them.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool side_pending = false;  // ev_join recorded, the main stream not yet joined
   float acc = 0.0f;
 
   std::vector<HostCollider> colliders;
@@ -171,7 +177,7 @@ struct vxp_system {
   // collider build scratch (classification, metadata, mass properties)
   DevBuf d_cb_pos, d_cb_vox, d_cb_cnt, d_cb_bounds, d_cb_box, d_cb_out, d_cb_solid, d_cb_nbr;
   // kernel timing (HIP events on `stream`)
-  bool prof = false;
+  int prof = VXP_PROFILE_OFF;
   double prof_ms[VXP_K_COUNT] = {};
   uint64_t prof_n[VXP_K_COUNT] = {};
   std::vector<hipEvent_t> ev_pool;

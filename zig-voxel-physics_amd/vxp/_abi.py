@@ -19,6 +19,7 @@ STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
 EMPTY, CORNER, EDGE, FACE, INSIDE = 0, 1, 2, 3, 4
 SOLVE_ORDER_COLOR, SOLVE_ORDER_CANONICAL = 0, 1
 PAIR_ORDER_KEY, PAIR_ORDER_TREE = 0, 1
+PROFILE_OFF, PROFILE_ALL, PROFILE_SOLVE_UNITS = 0, 1, 2
 NO_COLLIDER = 0xFFFFFFFF
 
 
