@@ -792,25 +792,6 @@ static int stepAllImpl(vxp_system* s) {
   profEnd(s, VXP_K_COMPACT, pe);
 
   // ---- islands / colouring artefacts ------------------------------------------
-  if (s->cfg.enable_islands) {
-    RC(ensureIslands(s));
-    // the labels read the manifolds and the pre-solve body state, and the
-    // solver reads nothing they write: on the side stream they overlap the
-    // solver's prep and units (kept in line when phases are being timed)
-    hipStream_t is = st;
-    if (s->prof != VXP_PROFILE_ALL) {
-      HIP_TRY(hipEventRecord(s->ev_fork, st));
-      HIP_TRY(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-      is = s->side;
-    }
-    pe = profBegin(s, VXP_K_ISLANDS);
-    launch_islands(is, cfg, B, ws, stats, ml, prev, islandArrays(s), nw, max_man, s->lds_limit);
-    profEnd(s, VXP_K_ISLANDS, pe);
-    if (is != st) {
-      HIP_TRY(hipEventRecord(s->ev_join, s->side));
-      s->side_pending = true;
-    }
-  }
   if (s->cfg.enable_coloring) {
     HIP_TRY(s->d_ref_masks.ensure((size_t)nw * s->id_cap * 32));
     HIP_TRY(s->d_ref_colors.ensure((size_t)nm1));
@@ -867,6 +848,27 @@ static int stepAllImpl(vxp_system* s) {
   S.pcnt = s->s_pcnt.as<uint32_t>();
   S.fq = s->s_fq.as<uint32_t>();
   S.cend = s->s_cend.as<uint4>();
+  // The island labels (manifolds and pre-solve body state in, island arrays
+  // out) and the body working state (body state in, working state out) run on
+  // the side stream beside the solver's prep, which reads neither output; the
+  // solve units join them.  Kept in line when every phase is being timed.
+  if (s->cfg.enable_islands) RC(ensureIslands(s));
+  hipStream_t side = st;
+  if (s->prof != VXP_PROFILE_ALL) {
+    HIP_TRY(hipEventRecord(s->ev_fork, st));
+    HIP_TRY(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+    side = s->side;
+  }
+  if (s->cfg.enable_islands) {
+    pe = profBegin(s, VXP_K_ISLANDS);
+    launch_islands(side, cfg, B, ws, stats, ml, prev, islandArrays(s), nw, max_man, s->lds_limit);
+    profEnd(s, VXP_K_ISLANDS, pe);
+  }
+  launch_body_state(side, cfg, B, ws, S, ns);
+  if (side != st) {
+    HIP_TRY(hipEventRecord(s->ev_join, s->side));
+    s->side_pending = true;
+  }
   // solve units of at least this many manifolds: about two per CU over the
   // whole batch (results do not depend on it)
   cfg.unit_min = std::max(64u, nman / (2 * std::max(s->n_cu, 1u)));
@@ -876,6 +878,7 @@ static int stepAllImpl(vxp_system* s) {
   launch_solve_prep(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), ctr, nw, max_man,
                     s->lds_limit);
   profEnd(s, VXP_K_SOLVE, pe);
+  RC(joinSide(s));  // the units read the working state; the writeback overwrites what the labels read
   pe = profBegin(s, VXP_K_SOLVE_UNITS);
   launch_unit_order(st, S, ctr, s->s_uqueue2.as<uint32_t>());
   launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit, unitThreads(s));
@@ -885,7 +888,6 @@ static int stepAllImpl(vxp_system* s) {
     launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, 0, unitThreads(s));
   }
   profEnd(s, VXP_K_SOLVE_UNITS, pe);
-  RC(joinSide(s));  // before the writeback overwrites the velocities the labels read
   pe = profBegin(s, VXP_K_SOLVE_TAIL);
   launch_solve_tail(st, cfg, B, ws, stats, S, ctr, nw, s->n_cu);
   profEnd(s, VXP_K_SOLVE_TAIL, pe);

@@ -2303,6 +2303,32 @@ __device__ void colourManifolds(uint32_t M, uint32_t nadj, uint32_t m0, uint32_t
   __syncthreads();
 }
 
+// Body working state (BodyState.fromBody, src/solver.zig:231-243) of every
+// slot of the worlds that step: pose, velocities and world inverse inertia,
+// which only the solve units and later passes read.
+__global__ void k_body_state(DevCfg cfg, DevBodies B, const WorldState* ws, SolverScratch S,
+                             uint32_t n_slots) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_slots || ws[g / cfg.max_bodies].skip) return;
+  if (!B.active[g]) {
+    S.ws_lv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    S.ws_av[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const float4 rot = B.rot[g];
+  S.ws_pos[g] = B.pos[g];
+  S.ws_rot[g] = rot;
+  S.ws_lv[g] = B.lv[g];
+  S.ws_av[g] = B.av[g];
+  M3 ii;
+  if (B.inv_mass[g] == 0.0f) {
+    for (int i = 0; i < 9; i++) ii.m[i] = 0.0f;
+  } else {
+    ii = worldInvInertia(f4q(rot), loadM3(B.iil + (size_t)g * 9));
+  }
+  storeM3(S.ws_ii + (size_t)g * 9, ii);
+}
+
 __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBodies B,
                                                               WorldState* ws, vxp_stats* st,
                                                               DevManifold* ml, SolverScratch S,
@@ -2322,28 +2348,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
   __shared__ uint32_t uid[kUnitsMax];
   stamp(S.tstamp, 0);
 
-  // body working state (BodyState.fromBody, src/solver.zig:231-243)
-  for (uint32_t s = tid; s < MB; s += bs) {
-    uint32_t g = g0 + s;
-    S.adj_cnt[g] = 0;
-    if (!B.active[g]) {
-      S.ws_lv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-      S.ws_av[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-      continue;
-    }
-    float4 rot = B.rot[g];
-    S.ws_pos[g] = B.pos[g];
-    S.ws_rot[g] = rot;
-    S.ws_lv[g] = B.lv[g];
-    S.ws_av[g] = B.av[g];
-    M3 ii;
-    if (B.inv_mass[g] == 0.0f) {
-      for (int i = 0; i < 9; i++) ii.m[i] = 0.0f;
-    } else {
-      ii = worldInvInertia(f4q(rot), loadM3(B.iil + (size_t)g * 9));
-    }
-    storeM3(S.ws_ii + (size_t)g * 9, ii);
-  }
+  // (the body working state is k_body_state's, beside this kernel)
+  for (uint32_t s = tid; s < MB; s += bs) S.adj_cnt[g0 + s] = 0;
   if (tid == 0) {
     s_nsolve = 0;
     s_skipped = 0;
@@ -3620,6 +3626,10 @@ void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldS
 void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
                          const DevManifold* ml, uint64_t* masks, uint8_t* colors, uint32_t nw) {
   k_ref_coloring<<<nw, 64, 0, s>>>(cfg, ws, ml, masks, colors);
+}
+void launch_body_state(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                       const WorldState* ws, const SolverScratch& S, uint32_t n_slots) {
+  if (n_slots) k_body_state<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, ws, S, n_slots);
 }
 void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,

@@ -90,6 +90,8 @@ void launch_ref_coloring(hipStream_t s, const DevCfg& cfg, const WorldState* ws,
 // The solver in three steps: prep (block / world), units (persistent grid over
 // the solve units, body velocities in LDS when they fit in lds_limit), tail
 // (position passes 2.., then writeback / sleep / stats per world).
+void launch_body_state(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                       const WorldState* ws, const SolverScratch& S, uint32_t n_slots);
 void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
                        uint32_t* step_ctr, uint32_t nw, uint32_t max_manifolds,
