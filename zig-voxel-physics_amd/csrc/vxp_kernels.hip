@@ -148,26 +148,36 @@ __device__ __forceinline__ uint32_t upperBound(const uint32_t* a, uint32_t n, ui
 // entries); *total receives the sum when non-null.
 __device__ void blockScanExclusive(uint32_t* data, uint32_t n, uint32_t* tmp /*blockDim*/,
                                    uint32_t* total) {
-  __shared__ uint32_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < n; base += blockDim.x) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t v = i < n ? data[i] : 0;
-    tmp[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
-      uint32_t t = threadIdx.x >= d ? tmp[threadIdx.x - d] : 0;
-      __syncthreads();
-      tmp[threadIdx.x] += t;
-      __syncthreads();
+  // per block-sized chunk: inclusive scans within each wave by shuffles, the
+  // wave totals through tmp, and a running carry kept in registers (the same
+  // in every thread)
+  constexpr uint32_t W = kWave;
+  const uint32_t tid = threadIdx.x, bs = blockDim.x, lane = tid & (W - 1), wave = tid / W;
+  const uint32_t nwv = (bs + W - 1) / W, last = min(W - 1, bs - 1 - wave * W);
+  uint32_t carry = 0;
+  __syncthreads();  // the data may have just been written by other threads
+  for (uint32_t base = 0; base < n; base += bs) {
+    const uint32_t i = base + tid;
+    const uint32_t v = i < n ? data[i] : 0;
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t d = 1; d < W; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)incl, d, kWave);
+      if (lane >= d) incl += t;
     }
-    if (i < n) data[i] = carry + tmp[threadIdx.x] - v;
+    if (lane == last) tmp[wave] = incl;
     __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) carry += tmp[threadIdx.x];
+    uint32_t before = 0, chunk = 0;
+    for (uint32_t k = 0; k < nwv; k++) {
+      const uint32_t x = tmp[k];
+      before += k < wave ? x : 0u;
+      chunk += x;
+    }
+    if (i < n) data[i] = carry + before + incl - v;
+    carry += chunk;
     __syncthreads();
   }
-  if (total && threadIdx.x == 0) *total = carry;
+  if (total && tid == 0) *total = carry;
   __syncthreads();
 }
 
@@ -2349,7 +2359,12 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
   stamp(S.tstamp, 0);
 
   // (the body working state is k_body_state's, beside this kernel)
-  for (uint32_t s = tid; s < MB; s += bs) S.adj_cnt[g0 + s] = 0;
+  // Awake-dynamic incidence counts AC and CSR offsets AO per body slot, in LDS
+  // (the colouring's space, unused until then) when they fit.
+  const bool csr_l = (size_t)MB * 8 <= lds_bytes;
+  uint32_t* AC = csr_l ? reinterpret_cast<uint32_t*>(kdyn) : S.adj_cnt + g0;
+  uint32_t* AO = csr_l ? AC + MB : S.adj_off + g0;
+  for (uint32_t s = tid; s < MB; s += bs) AC[s] = 0;
   if (tid == 0) {
     s_nsolve = 0;
     s_skipped = 0;
@@ -2373,16 +2388,16 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
     atomicAdd(&s_points, d.m.contact_count);
     S.color[m0 + i] = -1;
     prio_buf[m0 + i] = colorPrio(exportKey(d.key));
-    if (B.type[ga] == VXP_BODY_DYNAMIC && !asl) atomicAdd(&S.adj_cnt[ga], 1u);
-    if (B.type[gb] == VXP_BODY_DYNAMIC && !bsl) atomicAdd(&S.adj_cnt[gb], 1u);
+    if (B.type[ga] == VXP_BODY_DYNAMIC && !asl) atomicAdd(&AC[d.sa], 1u);
+    if (B.type[gb] == VXP_BODY_DYNAMIC && !bsl) atomicAdd(&AC[d.sb], 1u);
   }
   __syncthreads();
   // CSR offsets (relative to 2 * m0)
-  for (uint32_t s = tid; s < MB; s += bs) S.adj_off[g0 + s] = S.adj_cnt[g0 + s];
+  for (uint32_t s = tid; s < MB; s += bs) AO[s] = AC[s];
   __syncthreads();
-  blockScanExclusive(S.adj_off + g0, MB, tmp, &s_points_tot);
+  blockScanExclusive(AO, MB, tmp, &s_points_tot);
   const uint32_t nadj = s_points_tot;
-  for (uint32_t s = tid; s < MB; s += bs) S.adj_cnt[g0 + s] = 0;
+  for (uint32_t s = tid; s < MB; s += bs) AC[s] = 0;
   __syncthreads();
   uint32_t* adj = S.adj + 2 * (size_t)m0;
   for (uint32_t i = tid; i < M; i += bs) {
@@ -2390,11 +2405,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
     const DevManifold& d = ml[m0 + i];
     uint32_t ga = g0 + d.sa, gb = g0 + d.sb;
     if (B.type[ga] == VXP_BODY_DYNAMIC && !B.sleeping[ga])
-      adj[S.adj_off[ga] + atomicAdd(&S.adj_cnt[ga], 1u)] = i;
+      adj[AO[d.sa] + atomicAdd(&AC[d.sa], 1u)] = i;
     if (B.type[gb] == VXP_BODY_DYNAMIC && !B.sleeping[gb])
-      adj[S.adj_off[gb] + atomicAdd(&S.adj_cnt[gb], 1u)] = i;
+      adj[AO[d.sb] + atomicAdd(&AC[d.sb], 1u)] = i;
   }
   __syncthreads();
+  if (csr_l) {  // the colouring reads them from global memory
+    for (uint32_t s = tid; s < MB; s += bs) {
+      S.adj_off[g0 + s] = AO[s];
+      S.adj_cnt[g0 + s] = AC[s];
+    }
+    __syncthreads();
+  }
   stamp(S.tstamp, 1);
   if ((size_t)M * kColourLdsPerManifold <= lds_bytes)
     colourManifolds<true>(M, nadj, m0, g0, B, ml, S, prio_buf, adj, kdyn, ws[w], s_left, s_ncol);
@@ -3635,7 +3657,8 @@ void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
                        vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
                        uint32_t* step_ctr, uint32_t nw, uint32_t max_manifolds,
                        size_t lds_limit) {
-  size_t lds = std::min(lds_limit, (size_t)max_manifolds * kColourLdsPerManifold);
+  size_t lds = std::min(lds_limit, std::max((size_t)max_manifolds * kColourLdsPerManifold,
+                                            (size_t)cfg.max_bodies * 8));
   k_solve_prep<<<nw, kSolveThreads, lds, s>>>(cfg, B, ws, st, ml, S, prio, step_ctr, lds);
 }
 // Longest units first: the queue is reordered by descending power-of-two
