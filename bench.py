@@ -45,16 +45,20 @@ def solver_bytes(st, substeps, iters):
     M manifolds in the solve, S substeps, I velocity iterations, J executed
     position iterations (FP32 SoA, each datum counted once per mandated pass).
     The rest of the solver's bytes, (J - 1)*(K*40 + M*192) + N*144, belong to
-    k_solve_pos and k_solve_final."""
-    if not st.steps_executed:
+    k_solve_pos and k_solve_final.  A frame of several executed steps launches
+    the kernel once per step: each counts with the frame's mean N and the last
+    step's K, M and J (the stats keep the last step's), so the bytes stay
+    matched to the launch count."""
+    n_steps = st.steps_executed
+    if not n_steps:
         return 0
-    N = st.awake_bodies_stepped // st.steps_executed
+    N = st.awake_bodies_stepped // n_steps
     K, M = st.solver_points, st.solver_manifolds
     S, I, J = substeps, iters, st.position_iterations_run
     if M == 0:
-        return S * N * 208
-    return (S * (N * 208 + K * 120 + M * 184 + (I + 1) * (K * 112 + M * 176)) + (K * 72 + M * 176)
-            + min(J, 1) * (K * 40 + M * 192) + K * 20)
+        return n_steps * S * N * 208
+    return n_steps * (S * (N * 208 + K * 120 + M * 184 + (I + 1) * (K * 112 + M * 176))
+                      + (K * 72 + M * 176) + min(J, 1) * (K * 40 + M * 192) + K * 20)
 
 
 def run(sys_, sc, warmup, steps, profile=0):

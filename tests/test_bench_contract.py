@@ -26,14 +26,15 @@ def test_more_than_one_gpu_is_refused():
 
 def test_solver_bytes_terms():
     b = _bench()
-    # 2 steps executed with 10 awake bodies each, 20 contact points, 5 manifolds,
+    # 2 steps executed with 10 awake bodies each (one solve-units launch per step), 20 contact points, 5 manifolds,
     # 4 substeps, 8 velocity iterations, 1 executed position iteration
     st = SimpleNamespace(steps_executed=2, awake_bodies_stepped=20, solver_points=20,
                          solver_manifolds=5, position_iterations_run=1)
     N, K, M, S, I = 10, 20, 5, 4, 8
     per_substep = N * 208 + K * 120 + M * 184 + (I + 1) * (K * 112 + M * 176)
-    want = S * per_substep + (K * 72 + M * 176) + (K * 40 + M * 192) + K * 20
-    assert b.solver_bytes(st, S, I) == want
+    one_step = S * per_substep + (K * 72 + M * 176) + (K * 40 + M * 192) + K * 20
+    # two launches of the kernel in that frame: two steps' bytes
+    assert b.solver_bytes(st, S, I) == 2 * one_step
     # no manifolds: only the bodies' substeps; nothing executed: nothing
     st0 = SimpleNamespace(steps_executed=1, awake_bodies_stepped=7, solver_points=0,
                           solver_manifolds=0, position_iterations_run=0)

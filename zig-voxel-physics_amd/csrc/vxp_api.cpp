@@ -881,9 +881,11 @@ static int stepAllImpl(vxp_system* s) {
   RC(joinSide(s));  // the units read the working state; the writeback overwrites what the labels read
   pe = profBegin(s, VXP_K_SOLVE_UNITS);
   launch_unit_order(st, S, ctr, s->s_uqueue2.as<uint32_t>());
+  // an earlier launch's failure is reported here, so that only the solve
+  // units' own refusal below selects the variant without LDS staging
+  HIP_TRY(hipGetLastError());
   launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, s->lds_limit, unitThreads(s));
-  if (hipPeekAtLastError() != hipSuccess) {  // LDS request refused: global variant
-    (void)hipGetLastError();
+  if (hipGetLastError() != hipSuccess) {  // LDS request refused: global variant
     s->lds_limit = 0;
     launch_solve_units(st, cfg, B, ws, ml, S, ctr, nw, s->n_cu, 0, unitThreads(s));
   }
@@ -927,7 +929,7 @@ int updateOnce(vxp_system* s, float dt) {
     n++;
   }
   if (s->acc > c.fixed_timestep * (float)c.max_substeps) s->acc = c.fixed_timestep;
-  launch_update_end(s->stream, devCfg(s), devBodies(s), s->d_stats.as<vxp_stats>(), s->nw, n, s->acc);
+  launch_update_end(s->stream, devCfg(s), devBodies(s), s->d_stats.as<vxp_stats>(), s->nw, n);
   HIP_TRY(hipGetLastError());
   return VXP_OK;
 }

@@ -3282,8 +3282,11 @@ __global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const 
   }
 }
 
+// End of update(): substeps_this_frame and updateBodyCounts (src/system.zig:
+// 527-533).  accumulated_time keeps the value k_update_begin recorded right
+// after `accumulated_time += dt`, before any step (src/system.zig:512-513).
 __global__ void k_update_end(DevCfg cfg, DevBodies B, vxp_stats* st, uint32_t n_worlds,
-                             uint32_t substeps_this_frame, float acc) {
+                             uint32_t substeps_this_frame) {
   uint32_t w = blockIdx.x;
   if (w >= n_worlds) return;
   __shared__ uint32_t sl, ac;
@@ -3302,7 +3305,6 @@ __global__ void k_update_end(DevCfg cfg, DevBodies B, vxp_stats* st, uint32_t n_
     st[w].sleeping_bodies = sl;
     st[w].active_bodies = ac;
     st[w].substeps_this_frame = substeps_this_frame;
-    st[w].accumulated_time = acc;
   }
 }
 
@@ -3741,8 +3743,8 @@ void launch_collider_mass(hipStream_t s, const int32_t* pos, const uint8_t* vox,
   k_collider_mass<<<1, 64, 0, s>>>(pos, vox, nb, solid, vs, density, out);
 }
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
-                       uint32_t nw, uint32_t substeps, float acc) {
-  k_update_end<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, st, nw, substeps, acc);
+                       uint32_t nw, uint32_t substeps) {
+  k_update_end<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, st, nw, substeps);
 }
 
 void launch_copy_many(hipStream_t s, const CopyDesc* d, uint32_t n) {

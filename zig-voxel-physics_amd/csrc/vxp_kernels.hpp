@@ -115,7 +115,7 @@ void launch_collider_meta(hipStream_t s, const int32_t* pos, const uint8_t* vox,
 void launch_collider_mass(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
                           uint32_t solid, float vs, float density, float* out);
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
-                       uint32_t nw, uint32_t substeps, float acc);
+                       uint32_t nw, uint32_t substeps);
 
 // sort / scan primitives (vxp_prims.hip)
 size_t sort_pairs_u64_u32_bytes(uint32_t n, uint32_t end_bit);

@@ -99,20 +99,24 @@ def diff_stats(sa, sb):
     for k in STAT_KEYS:
         if sa[k] != sb[k]:
             return f"stat {k}: {sa[k]} vs {sb[k]}"
-    if sa["max_penetration"] != sb["max_penetration"]:
-        return f"stat max_penetration: {sa['max_penetration']} vs {sb['max_penetration']}"
+    for k in ("max_penetration", "accumulated_time"):
+        if sa[k] != sb[k]:
+            return f"stat {k}: {sa[k]} vs {sb[k]}"
     return None
 
 
 def lockstep(scene, lib_a, lib_b, frames, cfg_over=None, worlds=None, check_every=1,
-             rtol=0.0, atol=0.0, log=None):
-    """Step both systems `frames` times; return (first_frame_with_diff, message) or (None, None)."""
+             rtol=0.0, atol=0.0, log=None, dts=None):
+    """Step both systems `frames` times; return (first_frame_with_diff, message) or (None, None).
+    `dts`: the update() argument of frame f is dts[f % len(dts)] (default: the
+    scene's dt, one fixed step per frame)."""
     sa = from_scene(scene, lib=lib_a, cfg_over=cfg_over, worlds=worlds)
     sb = from_scene(scene, lib=lib_b, cfg_over=cfg_over, worlds=worlds)
     nwl = sa.n_worlds
     for f in range(frames):
-        sa.update(scene.dt)
-        sb.update(scene.dt)
+        dt = dts[f % len(dts)] if dts else scene.dt
+        sa.update(dt)
+        sb.update(dt)
         if (f + 1) % check_every and f != frames - 1:
             continue
         for w in range(nwl):
