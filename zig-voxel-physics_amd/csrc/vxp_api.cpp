@@ -84,26 +84,104 @@ int colliderMetadata(vxp_system* s, HostCollider& c) {
   return VXP_OK;
 }
 
-// computeMassProperties (src/body.zig:462-519) on the device (k_collider_mass:
-// voxels summed brick by brick in ascending (z, y, x) brick order).  com may be
-// null.
-int massProperties(vxp_system* s, const HostCollider& c, float density, MassProps* out, V3* com) {
-  if (c.solid == 0) {
-    *out = MassProps{0.0f, M3{{0, 0, 0, 0, 0, 0, 0, 0, 0}}};
-    if (com) *com = V3{0.0f, 0.0f, 0.0f};
-    return VXP_OK;
+// computeMassProperties (src/body.zig:462-519) of a batch of (collider,
+// density) jobs on the device: one k_collider_mass workgroup per job, every
+// job's bricks staged once, one launch and one readback for the whole batch.
+// out[j] receives kMassOut floats (inv_mass, inverse inertia, centre of mass).
+// Colliders without solid voxels are not launched: zero mass properties.
+static int massBatch(vxp_system* s, const std::vector<std::pair<const HostCollider*, float>>& jobs,
+                     std::vector<float>& out) {
+  out.assign(jobs.size() * kMassOut, 0.0f);
+  std::vector<MassJob> dj;
+  std::vector<uint32_t> slot;  // out index of each launched job
+  std::unordered_map<const HostCollider*, uint32_t> off;
+  std::vector<int32_t> pos;
+  std::vector<uint8_t> vox;
+  for (size_t j = 0; j < jobs.size(); j++) {
+    const HostCollider* c = jobs[j].first;
+    if (c->solid == 0) continue;
+    auto it = off.find(c);
+    if (it == off.end()) {
+      it = off.emplace(c, (uint32_t)(pos.size() / 3)).first;
+      pos.insert(pos.end(), c->pos.begin(), c->pos.end());
+      vox.insert(vox.end(), c->vox.begin(), c->vox.end());
+    }
+    MassJob m{};
+    m.brick_off = it->second;
+    m.nb = (uint32_t)(c->pos.size() / 3);
+    m.solid = c->solid;
+    m.vs = c->vs;
+    m.density = jobs[j].second;
+    dj.push_back(m);
+    slot.push_back((uint32_t)j);
   }
-  RC(stageCollider(s, c));
-  HIP_TRY(s->d_cb_out.ensure(16 * 4));
-  launch_collider_mass(s->stream, s->d_cb_pos.as<int32_t>(), s->d_cb_vox.as<uint8_t>(),
-                       (uint32_t)(c.pos.size() / 3), c.solid, c.vs, density, s->d_cb_out.as<float>());
+  if (dj.empty()) return VXP_OK;
+  const size_t nb = pos.size() / 3;
+  HIP_TRY(s->d_cb_pos.ensure(std::max<size_t>(nb * 12, 16)));
+  HIP_TRY(s->d_cb_vox.ensure(std::max<size_t>(nb * 512, 512)));
+  HIP_TRY(s->d_cb_jobs.ensure(dj.size() * sizeof(MassJob)));
+  HIP_TRY(s->d_cb_out.ensure(dj.size() * kMassOut * 4));
+  HIP_TRY(hipMemcpyAsync(s->d_cb_pos.p, pos.data(), nb * 12, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_cb_vox.p, vox.data(), nb * 512, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_cb_jobs.p, dj.data(), dj.size() * sizeof(MassJob), hipMemcpyHostToDevice,
+                         s->stream));
+  launch_collider_mass(s->stream, s->d_cb_jobs.as<MassJob>(), (uint32_t)dj.size(),
+                       s->d_cb_pos.as<int32_t>(), s->d_cb_vox.as<uint8_t>(), s->d_cb_out.as<float>());
   HIP_TRY(hipGetLastError());
-  float r[13];
-  HIP_TRY(hipMemcpyAsync(r, s->d_cb_out.p, sizeof(r), hipMemcpyDeviceToHost, s->stream));
+  std::vector<float> r(dj.size() * kMassOut);
+  HIP_TRY(hipMemcpyAsync(r.data(), s->d_cb_out.p, r.size() * 4, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
-  out->inv_mass = r[0];
-  for (int i = 0; i < 9; i++) out->inv_inertia.m[i] = r[1 + i];
+  for (size_t k = 0; k < dj.size(); k++)
+    std::memcpy(&out[(size_t)slot[k] * kMassOut], &r[k * kMassOut], kMassOut * 4);
+  return VXP_OK;
+}
+
+static MassProps massFromOut(const float* r) {
+  MassProps m;
+  m.inv_mass = r[0];
+  for (int i = 0; i < 9; i++) m.inv_inertia.m[i] = r[1 + i];
+  return m;
+}
+
+// One collider at one density.  com may be null.
+int massProperties(vxp_system* s, const HostCollider& c, float density, MassProps* out, V3* com) {
+  std::vector<float> r;
+  RC(massBatch(s, {{&c, density}}, r));
+  *out = massFromOut(r.data());
   if (com) *com = V3{r[10], r[11], r[12]};
+  return VXP_OK;
+}
+
+// The mass properties of every dynamic body created since the last call
+// (createDynamicBody, src/system.zig:294-322): the (collider, density) pairs
+// not yet known are computed in one batch, then the bodies' inverse mass and
+// local inverse inertia are filled in on the host mirror.
+int resolveMasses(vxp_system* s) {
+  if (s->mass_pending.empty()) return VXP_OK;
+  std::vector<std::pair<const HostCollider*, float>> jobs;
+  std::vector<std::pair<uint32_t, uint32_t>> keys;  // (collider, density bits) of each job
+  std::unordered_set<uint64_t> seen;
+  for (const auto& p : s->mass_pending) {
+    uint32_t key;
+    std::memcpy(&key, &p.density, 4);
+    const HostCollider& hc = s->colliders[p.collider];
+    if (hc.mass_cache.count(key) || !seen.insert((uint64_t)p.collider << 32 | key).second) continue;
+    keys.push_back({p.collider, key});
+    jobs.push_back({&hc, p.density});
+  }
+  std::vector<float> r;
+  RC(massBatch(s, jobs, r));
+  for (size_t j = 0; j < jobs.size(); j++)
+    s->colliders[keys[j].first].mass_cache.emplace(keys[j].second, massFromOut(&r[j * kMassOut]));
+  for (const auto& p : s->mass_pending) {
+    uint32_t key;
+    std::memcpy(&key, &p.density, 4);
+    const MassProps& mp = s->colliders[p.collider].mass_cache.at(key);
+    s->inv_mass[p.g] = mp.inv_mass;
+    for (int i = 0; i < 9; i++) s->iil[p.g * 9 + i] = mp.inv_inertia.m[i];
+  }
+  s->mass_pending.clear();
+  s->host_dirty = true;
   return VXP_OK;
 }
 
@@ -170,6 +248,7 @@ int down(const DevBuf& b, std::vector<T>& v) {
 int ensureDevice(vxp_system* s) {
   int rc = uploadColliders(s);
   if (rc) return rc;
+  if ((rc = resolveMasses(s))) return rc;
   if (s->t_host_dirty) {
     if ((rc = up(s->d_t_nodes, s->t_nodes)) || (rc = up(s->d_t_heads, s->t_heads)) ||
         (rc = up(s->d_t_leaf, s->t_leaf)))
@@ -1362,12 +1441,10 @@ static int createBody(vxp_system* s, uint32_t w, uint32_t type, const float pos[
       uint32_t key;
       std::memcpy(&key, &density, 4);
       auto it = hc.mass_cache.find(key);
-      if (it == hc.mass_cache.end()) {
-        MassProps m;
-        RC(massProperties(s, hc, density, &m, nullptr));
-        it = hc.mass_cache.emplace(key, m).first;
-      }
-      mp = it->second;
+      if (it != hc.mass_cache.end())
+        mp = it->second;
+      else  // computed with every other new body's before anything reads it
+        s->mass_pending.push_back({g, col, density});
     }
     s->inv_mass[g] = mp.inv_mass;
     for (int i = 0; i < 9; i++) s->iil[g * 9 + i] = mp.inv_inertia.m[i];
@@ -1465,6 +1542,7 @@ static bool validHandle(vxp_system* s, uint32_t w, vxp_handle h) {
 int vxp_body_destroy_w(vxp_system* s, uint32_t w, vxp_handle h) {
   if (!s) return VXP_ERR_ARGUMENT;
   RC(ensureHost(s));
+  RC(resolveMasses(s));  // no pending entry may outlive its body's slot
   if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
   size_t g = (size_t)w * s->MB + h.index;
   s->n_total[w]--;
@@ -1561,6 +1639,7 @@ int vxp_body_op_w(vxp_system* s, uint32_t w, vxp_handle h, uint32_t op, const fl
   if ((op == VXP_OP_APPLY_FORCE_AT_POINT || op == VXP_OP_APPLY_IMPULSE_AT_POINT) && !point)
     return VXP_ERR_ARGUMENT;
   RC(ensureHost(s));
+  RC(resolveMasses(s));  // the mutators read the inverse mass and inertia
   if (!validHandle(s, w, h)) return VXP_ERR_INVALID_HANDLE;
   const size_t g = (size_t)w * s->MB + h.index;
   const uint8_t type = s->type[g];

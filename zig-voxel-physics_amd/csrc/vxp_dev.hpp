@@ -34,6 +34,17 @@ struct DevCollider {
   uint32_t solid;
 };
 
+// One computeMassProperties job: a collider's bricks at brick_off..brick_off+nb
+// of the staged brick arrays, its solid count and voxel size, and a density.
+// Results go to 16 floats per job (inv_mass, 3x3 inverse inertia, centre of
+// mass, unused).
+struct MassJob {
+  uint32_t brick_off, nb, solid;
+  float vs, density;
+  uint32_t pad[3];
+};
+constexpr uint32_t kMassOut = 16;
+
 // Configuration values the kernels read (vxp_config subset, by value).
 struct DevCfg {
   float gravity[3];

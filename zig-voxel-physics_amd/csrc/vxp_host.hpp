@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <utility>
 #include <vector>
 #include <initializer_list>
@@ -95,6 +96,7 @@ struct HostCollider {
 int stageCollider(vxp_system* s, const HostCollider& c);
 int colliderMetadata(vxp_system* s, HostCollider& c);
 int massProperties(vxp_system* s, const HostCollider& c, float density, MassProps* out, V3* com);
+int resolveMasses(vxp_system* s);
 
 }  // namespace vxh
 
@@ -175,7 +177,17 @@ struct vxp_system {
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   uint32_t n_cu = 256;   // compute units of the device
   // collider build scratch (classification, metadata, mass properties)
-  DevBuf d_cb_pos, d_cb_vox, d_cb_cnt, d_cb_bounds, d_cb_box, d_cb_out, d_cb_solid, d_cb_nbr;
+  DevBuf d_cb_pos, d_cb_vox, d_cb_cnt, d_cb_bounds, d_cb_box, d_cb_out, d_cb_solid, d_cb_nbr,
+      d_cb_jobs;
+  // dynamic bodies created since the last batch of mass-property jobs: their
+  // inverse mass and inertia are filled in by resolveMasses (one launch for
+  // every (collider, density) not yet computed)
+  struct PendingMass {
+    size_t g;
+    uint32_t collider;
+    float density;
+  };
+  std::vector<PendingMass> mass_pending;
   // kernel timing (HIP events on `stream`)
   int prof = VXP_PROFILE_OFF;
   double prof_ms[VXP_K_COUNT] = {};

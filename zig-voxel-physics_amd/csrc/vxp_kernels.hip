@@ -3183,17 +3183,23 @@ __global__ __launch_bounds__(512) void k_collider_meta(const int32_t* pos, const
 // order given (ascending (z, y, x) position), so the float sums follow that
 // order exactly; one lane accumulates while the block stages each brick's
 // voxel bytes in LDS.  out: inv_mass, inverse inertia (9), centre of mass (3).
-// computeMassProperties (src/body.zig:462-519) of one collider.  Every float
-// sum runs in the canonical order (bricks as given, voxels by index inside a
+// computeMassProperties (src/body.zig:462-519), one workgroup per job (a
+// collider at a density), all jobs of a batch in one launch.  Every float sum
+// runs in the canonical order (bricks as given, voxels by index inside a
 // brick), one accumulator per lane: the 64 lanes compute the per-voxel terms of
 // a brick into LDS, then lane k adds term k of each solid voxel in order.  The
 // terms are the same expressions as a sequential loop's, so the sums are too.
-__global__ __launch_bounds__(64) void k_collider_mass(const int32_t* pos, const uint8_t* vox,
-                                                      uint32_t nb, uint32_t solid, float vs,
-                                                      float density, float* out) {
+__global__ __launch_bounds__(64) void k_collider_mass(const MassJob* jobs, const int32_t* pos_all,
+                                                      const uint8_t* vox_all, float* out_all) {
   __shared__ float term[6][512];
   __shared__ uint64_t occ[8];  // solid voxels of the staged brick, bit i % 64 of word i / 64
   __shared__ float s_com[3];
+  const MassJob job = jobs[blockIdx.x];
+  const int32_t* pos = pos_all + (size_t)job.brick_off * 3;
+  const uint8_t* vox = vox_all + (size_t)job.brick_off * 512;
+  float* out = out_all + (size_t)blockIdx.x * kMassOut;
+  const uint32_t nb = job.nb, solid = job.solid;
+  const float vs = job.vs, density = job.density;
   const uint32_t lane = threadIdx.x;
   const float vv = vs * vs * vs;
   const float vm = vv * density;
@@ -3738,9 +3744,9 @@ void launch_collider_meta(hipStream_t s, const int32_t* pos, const uint8_t* vox,
                           float vs, uint32_t* cnt, int32_t* bounds, float* box) {
   if (nb) k_collider_meta<<<nb, 512, 0, s>>>(pos, vox, vs, cnt, bounds, box);
 }
-void launch_collider_mass(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
-                          uint32_t solid, float vs, float density, float* out) {
-  k_collider_mass<<<1, 64, 0, s>>>(pos, vox, nb, solid, vs, density, out);
+void launch_collider_mass(hipStream_t s, const MassJob* jobs, uint32_t n_jobs, const int32_t* pos,
+                          const uint8_t* vox, float* out) {
+  if (n_jobs) k_collider_mass<<<n_jobs, 64, 0, s>>>(jobs, pos, vox, out);
 }
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps) {

@@ -112,8 +112,10 @@ void launch_collider_classify(hipStream_t s, const uint64_t* solid, const int32_
                               uint32_t nb, uint8_t* out);
 void launch_collider_meta(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
                           float vs, uint32_t* cnt, int32_t* bounds, float* box);
-void launch_collider_mass(hipStream_t s, const int32_t* pos, const uint8_t* vox, uint32_t nb,
-                          uint32_t solid, float vs, float density, float* out);
+// computeMassProperties of n_jobs (collider, density) jobs, one workgroup each:
+// bricks staged in pos (3 ints) / vox (512 bytes), kMassOut floats out per job
+void launch_collider_mass(hipStream_t s, const MassJob* jobs, uint32_t n_jobs, const int32_t* pos,
+                          const uint8_t* vox, float* out);
 void launch_update_end(hipStream_t s, const DevCfg& cfg, const DevBodies& B, vxp_stats* st,
                        uint32_t nw, uint32_t substeps);
 
