@@ -203,6 +203,19 @@ int vxp_collider_info(vxp_system* sys, uint32_t collider, int32_t bounds[6], flo
  * (src/collider.zig:376-403). */
 int vxp_collider_mass_properties(vxp_system* sys, uint32_t collider, float density,
                                  float out[13]);
+/* Allocation-free variant of vxp_collider_mass_properties, with the same
+ * results bit for bit.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops
+ * that admit worlds between ticks, harnesses that forbid hipMalloc).  It calls
+ * no hipMalloc/hipFree: the collider's bricks, the job and the results are
+ * staged in one 4 MiB device array per GPU, held until the process ends.
+ * Colliders of more than vxp_noalloc_max_bricks() stored bricks are rejected
+ * with VXP_ERR_CAPACITY before anything is launched.  Calls share that array:
+ * calls that overlap on one GPU (two threads, two systems) race. */
+int vxp_collider_mass_properties_noalloc(vxp_system* sys, uint32_t collider, float density,
+                                         float out[13]);
+/* The largest collider (stored bricks) vxp_collider_mass_properties_noalloc accepts. */
+uint32_t vxp_noalloc_max_bricks(void);
 /* The collider's stored bricks (after hash merging, src/collider.zig:232-239),
  * ascending (z, y, x) position: 3 ints and 512 voxel bytes each; `count`
  * receives the brick count (call with capacity 0 to size the arrays). */

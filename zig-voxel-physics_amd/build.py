@@ -15,7 +15,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 OUT = os.path.join(PKG, "libvxp.so")
-SOURCES = ["vxp_kernels.hip", "vxp_prims.hip", "vxp_api.cpp", "vxp_serve.cpp"]
+SOURCES = ["vxp_kernels.hip", "vxp_prims.hip", "vxp_api.cpp", "vxp_serve.cpp", "vxp_noalloc.hip"]
 HEADERS = ["vxp_math.hpp", "vxp_dev.hpp", "vxp_kernels.hpp", "vxp_tree.hpp", "../../include/vxp.h"]
 
 ARCH = os.environ.get("VXP_OFFLOAD_ARCH", "gfx950")

@@ -1190,6 +1190,9 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
   {
     int dev = 0, lds = 0;
     (void)hipGetDevice(&dev);
+    s->device = dev;
+    // the allocation-free entry point's scratch address on this device
+    if ((rc = noallocInit(dev))) return fail(rc);
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
         lds > 24 * 1024)
       s->lds_limit = (size_t)lds - 20 * 1024;  // the solver's static LDS

@@ -97,6 +97,8 @@ int stageCollider(vxp_system* s, const HostCollider& c);
 int colliderMetadata(vxp_system* s, HostCollider& c);
 int massProperties(vxp_system* s, const HostCollider& c, float density, MassProps* out, V3* com);
 int resolveMasses(vxp_system* s);
+// defined in vxp_noalloc.hip
+int noallocInit(int device);
 
 }  // namespace vxh
 
@@ -176,6 +178,7 @@ struct vxp_system {
   uint32_t unit_threads = 0;  // solve-unit workgroup size forced by vxp_tuning_set (0: default)
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   uint32_t n_cu = 256;   // compute units of the device
+  int device = 0;        // the device current at creation (the streams' device)
   // collider build scratch (classification, metadata, mass properties)
   DevBuf d_cb_pos, d_cb_vox, d_cb_cnt, d_cb_bounds, d_cb_box, d_cb_out, d_cb_solid, d_cb_nbr,
       d_cb_jobs;
