@@ -10,8 +10,13 @@ compares them with the plain single-world path looped over the same worlds;
 "rocks" reports voxel-pair tests/s and contacts/s on the 256-rock scene;
 "serving" drives the continuous-batching loop over 2048 resident worlds with
 Poisson request arrivals and reports p50/p99 request latency per offered rate;
-"cpu_baseline" times the CPU restatement on the 4096-body scene.  Prints one
-JSON line.
+"c0" times the 64-cube scene; "cpu_baseline" times the CPU restatement on the
+last frames of the 4096-body window, and each section carries the CPU
+restatement on its own workload ("cpu").  Prints one JSON line.
+
+The 4096-body scene's trajectory is 600 frames; the timed frames are its last
+--steps frames (frames 600 - steps .. 599), after the earlier ones run untimed,
+so a short run times the settled pile, not the start of the drop.
 
   python bench.py [--gpus 1] [--steps 590] [--warmup 10] [--batched-steps 290]
                   [--c2-steps 390] [--serve-rates 2000,5000,10000,20000]
@@ -62,24 +67,27 @@ def solver_bytes(st, substeps, iters):
 
 
 def run(sys_, sc, warmup, steps, profile=0):
-    """Step `steps` frames after `warmup`; per-frame wall time of update().
-    `profile`: the library's timing mode over those frames (0 off, 1 every
-    phase, 2 only the solve units: two HIP events per step)."""
+    """Step `steps` frames after `warmup` untimed ones; per-frame wall time of
+    update() and per-frame counts.  `profile`: the library's timing mode over
+    the timed frames (0 off, 1 every phase, 2 only the solve units: two HIP
+    events per step)."""
     lib = sys_.lib
     cfg = sc.config
     for _ in range(warmup):
         sys_.update(sc.dt)
     if profile:
         lib.profile_enable(sys_.ptr, profile)
-    frame_ms, body_substeps, alg, cands = [], 0, 0, 0
+    frame_ms, frame_bs, alg, cands = [], [], 0, 0
     for _ in range(steps):
         t = time.perf_counter()
         sys_.update(sc.dt)  # synchronous: returns after the frame's kernels finish
         frame_ms.append((time.perf_counter() - t) * 1e3)
+        bs = 0
         for st in sys_.stats_all():
-            body_substeps += st.awake_bodies_stepped * cfg["substeps"]
+            bs += st.awake_bodies_stepped * cfg["substeps"]
             cands += st.contact_candidates
             alg += solver_bytes(st, cfg["substeps"], cfg["velocity_iterations"])
+        frame_bs.append(bs)
     kern = {}
     if profile:
         import vxp
@@ -89,9 +97,42 @@ def run(sys_, sc, warmup, steps, profile=0):
             kern[name] = (ms.value, n.value)
         lib.profile_enable(sys_.ptr, 0)
     wall = sum(frame_ms) / 1e3
-    return dict(value=body_substeps / wall, wall=wall, frame_ms=frame_ms, kern=kern,
-                alg_solver_bytes=alg, body_substeps=body_substeps, candidates=cands,
+    return dict(value=sum(frame_bs) / wall, wall=wall, frame_ms=frame_ms, frame_bs=frame_bs,
+                kern=kern, alg_solver_bytes=alg, body_substeps=sum(frame_bs), candidates=cands,
                 tuning=sys_.tuning())
+
+
+def cpu_run(sys_, sc, pre, frames, seconds, extra=()):
+    """The CPU restatement: `pre` untimed frames, then up to `frames` timed ones
+    (fewer if `seconds` of CPU time run out).  Returns per-frame times and
+    body-substeps plus the sums of the `extra` stats."""
+    cfg = sc.config
+    t0 = time.perf_counter()
+    for _ in range(pre):
+        sys_.update(sc.dt)
+    pre_s = time.perf_counter() - t0
+    ms, bs, sums = [], [], {k: 0 for k in extra}
+    while len(ms) < frames and sum(ms) / 1e3 < seconds:
+        t = time.perf_counter()
+        sys_.update(sc.dt)
+        ms.append((time.perf_counter() - t) * 1e3)
+        st = sys_.stats_all()
+        bs.append(sum(x.awake_bodies_stepped for x in st) * cfg["substeps"])
+        for k in extra:
+            sums[k] += sum(getattr(x, k) for x in st)
+    return dict(frame_ms=ms, frame_bs=bs, wall=sum(ms) / 1e3, sums=sums, pre_s=pre_s)
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
 
 
 def pct(xs, q):
@@ -177,16 +218,22 @@ def run_serving(lib, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=590)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=590,
+                    help="timed frames: the last STEPS frames of the 600-frame 4096-body trajectory")
+    ap.add_argument("--warmup", type=int, default=10,
+                    help="untimed frames before the timed ones (at least 600 - STEPS)")
     ap.add_argument("--no-breakdown", action="store_true",
                     help="skip the second, untimed pass of the scene with every phase timed (kernel_ms)")
     ap.add_argument("--batched-steps", type=int, default=290)
     ap.add_argument("--looped-worlds", type=int, default=48)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU time of each timed CPU sample")
+    ap.add_argument("--cpu-frames", type=int, default=30,
+                    help="frames of the 4096-body CPU sample (the last ones of the timed window)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-c0", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=390)
     ap.add_argument("--serve-worlds", type=int, default=2048)
     ap.add_argument("--serve-slots", type=int, default=256)
@@ -202,20 +249,24 @@ def main():
     from vxp import scenes
 
     lib = vxp.product_lib()
-    # ---- plain single scene: 4096 bodies ----------------------------------------
-    log("4096-body scene")
+    threads = min(16, os.cpu_count() or 1)
+    cpu_model, nproc = host_cpu()
+    # ---- plain single scene: 4096 bodies, the last `steps` of its 600 frames ----
     sc = scenes.c1_large()
+    pre = max(args.warmup, sc.frames - args.steps)
+    first, last = pre, pre + args.steps - 1
+    log(f"4096-body scene: frames {first}..{last} timed after {pre} untimed")
     s1 = vxp.from_scene(sc, lib=lib)
     # the timed frames time only the solve units (the roofline's kernel), with
     # HIP events on their stream; timing every phase would add two events per
     # phase and keep the island labels off their side stream
-    plain = run(s1, sc, args.warmup, args.steps, profile=vxp._abi.PROFILE_SOLVE_UNITS)
+    plain = run(s1, sc, pre, args.steps, profile=vxp._abi.PROFILE_SOLVE_UNITS)
     final = s1.stats()
     s1.close()
     brk = None
     if not args.no_breakdown:  # the same frames again on a fresh copy, every phase timed
         s1 = vxp.from_scene(sc, lib=lib)
-        brk = run(s1, sc, args.warmup, args.steps, profile=vxp._abi.PROFILE_ALL)
+        brk = run(s1, sc, pre, args.steps, profile=vxp._abi.PROFILE_ALL)
         s1.close()
     kern = plain["kern"]
     bk = brk["kern"] if brk else {}
@@ -228,6 +279,29 @@ def main():
     tf = os.path.join(REPO, "profiles", "k_solve_units_c1_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+
+    # ---- the 64-cube scene (the small configuration), frames warmup..299 -------
+    c0 = None
+    if not args.no_c0:
+        log("64-cube scene")
+        s0c = scenes.c0_small()
+        s0 = vxp.from_scene(s0c, lib=lib)
+        r0 = run(s0, s0c, args.warmup, s0c.frames - args.warmup)
+        s0.close()
+        c0 = {"workload": f"{s0c.name}: 64 cubes + ground, frames {args.warmup}..{s0c.frames - 1}",
+              "value": r0["value"], "unit": "body-substeps/s",
+              "ms_per_step": r0["wall"] * 1e3 / len(r0["frame_ms"]),
+              "p50_step_ms": statistics.median(r0["frame_ms"]),
+              "p99_step_ms": pct(r0["frame_ms"], 0.99)}
+        if not args.no_cpu:
+            o0 = vxp.from_scene(s0c, lib=vxp.oracle_lib(), cfg_over=dict(cpu_threads=threads))
+            rc0 = cpu_run(o0, s0c, args.warmup, s0c.frames - args.warmup, args.cpu_seconds)
+            o0.close()
+            n = len(rc0["frame_ms"])
+            c0["cpu"] = {"value": sum(rc0["frame_bs"]) / rc0["wall"], "cores": threads,
+                         "frames": f"{args.warmup}..{args.warmup + n - 1}",
+                         "p50_step_ms": statistics.median(rc0["frame_ms"]),
+                         "p99_step_ms": pct(rc0["frame_ms"], 0.99)}
 
     # ---- batched: 512 worlds ------------------------------------------------------
     batched = None
@@ -270,6 +344,28 @@ def main():
             "plain_looped_sample": f"{nl} of the {len(sb.worlds)} worlds stepped one system at a "
                                    f"time, frames {args.warmup}..{args.warmup + lf - 1}",
         }
+        if not args.no_cpu:
+            # the reference has no world batching: one system per world, looped
+            # over the same sample of worlds, frame by frame
+            ors = [vxp.from_scene(sb, lib=vxp.oracle_lib(), worlds=[w],
+                                  cfg_over=dict(cpu_threads=threads)) for w in idx]
+            for o in ors:
+                for _ in range(args.warmup):
+                    o.update(sb.dt)
+            ct, cbs, cf = 0.0, 0, 0
+            while cf < lf and ct < args.cpu_seconds:
+                for o in ors:
+                    t = time.perf_counter()
+                    o.update(sb.dt)
+                    ct += time.perf_counter() - t
+                    cbs += o.stats_all()[0].awake_bodies_stepped * sb.config["substeps"]
+                cf += 1
+            for o in ors:
+                o.close()
+            batched["cpu"] = {"value": cbs / ct, "cores": threads,
+                              "sample": f"CPU restatement looped over the same {nl} worlds "
+                                        f"(one system each), frames {args.warmup}.."
+                                        f"{args.warmup + cf - 1}"}
 
     # ---- narrow-phase-heavy scene: 256 rocks -------------------------------------
     rocks = None
@@ -279,24 +375,39 @@ def main():
         s2 = vxp.from_scene(sr, lib=lib)
         for _ in range(args.warmup):
             s2.update(sr.dt)
-        vchk = cont = bsub = cand = 0
-        t_tot = 0.0
+        per = []  # per frame: (ms, voxels_checked, contacts, candidates, body-substeps)
         for _ in range(args.c2_steps):
             t = time.perf_counter()
             s2.update(sr.dt)
-            t_tot += time.perf_counter() - t
+            ms = (time.perf_counter() - t) * 1e3
             st = s2.stats()
-            vchk += st["voxels_checked"]
-            cont += st["contacts_generated"]
-            cand += st["contact_candidates"]
-            bsub += st["awake_bodies_stepped"] * sr.config["substeps"]
+            per.append((ms, st["voxels_checked"], st["contacts_generated"], st["contact_candidates"],
+                        st["awake_bodies_stepped"] * sr.config["substeps"]))
         s2.close()
+        t_tot = sum(p[0] for p in per) / 1e3
         rocks = {"workload": f"{sr.name}: 256 rocks of 32^3 voxels (0.1 m), dt 1/120, "
                              f"{sr.config['substeps']} substeps, {sr.config['velocity_iterations']} "
                              f"iterations, frames {args.warmup}..{args.warmup + args.c2_steps - 1}",
-                 "voxel_pair_tests_per_s": vchk / t_tot, "contacts_per_s": cont / t_tot,
-                 "contact_candidates_per_s": cand / t_tot,
-                 "body_substeps_per_s": bsub / t_tot, "ms_per_step": t_tot * 1e3 / args.c2_steps}
+                 "voxel_pair_tests_per_s": sum(p[1] for p in per) / t_tot,
+                 "contacts_per_s": sum(p[2] for p in per) / t_tot,
+                 "contact_candidates_per_s": sum(p[3] for p in per) / t_tot,
+                 "body_substeps_per_s": sum(p[4] for p in per) / t_tot,
+                 "ms_per_step": t_tot * 1e3 / args.c2_steps}
+        if not args.no_cpu:
+            o2 = vxp.from_scene(sr, lib=vxp.oracle_lib(), cfg_over=dict(cpu_threads=threads))
+            rc2 = cpu_run(o2, sr, args.warmup, args.c2_steps, args.cpu_seconds,
+                          extra=("voxels_checked", "contacts_generated"))
+            o2.close()
+            n = len(rc2["frame_ms"])
+            g_t = sum(p[0] for p in per[:n]) / 1e3
+            rocks["cpu"] = {
+                "voxel_pair_tests_per_s": rc2["sums"]["voxels_checked"] / rc2["wall"],
+                "contacts_per_s": rc2["sums"]["contacts_generated"] / rc2["wall"],
+                "body_substeps_per_s": sum(rc2["frame_bs"]) / rc2["wall"], "cores": threads,
+                "frames": f"{args.warmup}..{args.warmup + n - 1}",
+                "gpu_on_same_frames": {
+                    "voxel_pair_tests_per_s": sum(p[1] for p in per[:n]) / g_t,
+                    "contacts_per_s": sum(p[2] for p in per[:n]) / g_t}}
 
     # ---- serving: Poisson requests over 2048 resident worlds -------------------
     serving = None
@@ -304,30 +415,26 @@ def main():
         log("serving")
         serving = run_serving(lib, args)
 
-    # ---- CPU baseline: the CPU restatement on the same scene, bounded sample ----
+    # ---- CPU baseline: the CPU restatement on the last frames of the timed window
     cpu = None
     if not args.no_cpu:
-        log("CPU baseline")
-        threads = min(16, os.cpu_count() or 1)
+        k = min(args.cpu_frames, args.steps)
+        c_first = last - k + 1
+        log(f"CPU baseline: frames {c_first}..{last} of the 4096-body scene after {c_first} untimed")
         ora = vxp.from_scene(sc, lib=vxp.oracle_lib(), cfg_over=dict(cpu_threads=threads))
-        gpu2 = vxp.from_scene(sc, lib=lib)
-        cb, gb, frames, tc, tg = 0, 0, 0, 0.0, 0.0
-        while tc < args.cpu_seconds and frames < args.warmup + args.steps:
-            t = time.perf_counter()
-            ora.update(sc.dt)
-            tc += time.perf_counter() - t
-            t = time.perf_counter()
-            gpu2.update(sc.dt)
-            tg += time.perf_counter() - t
-            cb += ora.stats()["awake_bodies_stepped"] * sc.config["substeps"]
-            gb += gpu2.stats()["awake_bodies_stepped"] * sc.config["substeps"]
-            frames += 1
-        cpu = {"value": cb / tc, "unit": "body-substeps/s", "cores": threads, "kind": "port",
-               "sample": f"CPU restatement (oracle/, {threads} narrow-phase threads) on frames "
-                         f"0..{frames - 1} of the 4096-body scene ({tc:.1f} s of CPU time); "
-                         f"the GPU on the same frames: {gb / tg:.0f} body-substeps/s"}
+        rc = cpu_run(ora, sc, c_first, k, 1e9)
         ora.close()
-        gpu2.close()
+        g_ms = sum(plain["frame_ms"][-k:]) / 1e3
+        g_bs = sum(plain["frame_bs"][-k:])
+        cpu = {"value": sum(rc["frame_bs"]) / rc["wall"], "unit": "body-substeps/s",
+               "cores": threads, "kind": "port",
+               "sample": f"CPU restatement (oracle/, {threads} narrow-phase threads, "
+                         f"{cpu_model}, {nproc} logical CPUs) on frames {c_first}..{last} of the "
+                         f"4096-body scene ({rc['wall']:.1f} s of CPU time after an untimed "
+                         f"{rc['pre_s']:.0f} s run of frames 0..{c_first - 1}); the GPU on the "
+                         f"same frames: {g_bs / g_ms:.0f} body-substeps/s",
+               "p50_step_ms": statistics.median(rc["frame_ms"]),
+               "p99_step_ms": pct(rc["frame_ms"], 0.99)}
 
     out = {
         "metric": "body-substeps/s (4096-body scene; 512 batched worlds); p50/p99 step latency",
@@ -336,6 +443,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "untimed_frames": pre,
         "ms_per_step": plain["wall"] * 1e3 / args.steps,
         # every dynamic body counted every frame, asleep or not
         "nominal_value": sc.n_dynamic * sc.config["substeps"] * args.steps / plain["wall"],
@@ -349,19 +457,23 @@ def main():
         "config": {"workload": f"{sc.name}: {sc.n_dynamic} dynamic bodies + static ground, "
                                f"dt {sc.dt:.5f}, {sc.config['substeps']} substeps, "
                                f"{sc.config['velocity_iterations']} iterations, frames "
-                               f"{args.warmup}..{args.warmup + args.steps - 1}",
+                               f"{first}..{last} of {sc.frames} timed (frames 0..{pre - 1} run "
+                               f"untimed first)",
                    "worlds": 1},
         "roofline": {"bound": "hbm", "kernel": "k_solve_units", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "avg_launch_ms": avg_solve_s * 1e3, "launches": solve_n,
-                     "alg_bytes_per_launch": alg_per_launch, "dominant_kernel": dom},
+                     "alg_bytes_per_launch": alg_per_launch, "dominant_kernel": dom,
+                     "frames": f"{first}..{last}"},
         "kernel_ms": {k: round(v[0], 3) for k, v in bk.items()},
         "kernel_ms_note": ("the same frames on a second copy of the scene with every phase "
                            "timed (HIP events per phase; not the timed run)") if bk else None,
         "tuning": plain["tuning"],
         "final_stats": {k: final[k] for k in ("manifolds", "colors", "sleeping_bodies",
                                               "voxels_checked", "broad_phase_pairs")},
+        "host_cpu": {"model": cpu_model, "logical_cpus": nproc},
+        "c0": c0,
         "batched": batched,
         "rocks": rocks,
         "serving": serving,

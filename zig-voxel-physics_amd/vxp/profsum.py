@@ -2,6 +2,9 @@
 
   python3 -m vxp.profsum trace  <results.db> [top]
   python3 -m vxp.profsum pmc    <results.db> <COUNTER> [name-filter]
+  python3 -m vxp.profsum last   <results.db> <kernel-substring> <N> [COUNTER]
+                               (the last N dispatches of one kernel: average
+                                duration, or average counter value per dispatch)
   python3 vxp/profsum.py copycal      (the calibration program: five 256 MiB
                                        device copies, profiled like the bench)
 """
@@ -36,6 +39,28 @@ def pmc(db, counter, filt=""):
     return "\n".join(out)
 
 
+def last(db, sub, n, counter=None):
+    """Average over the last `n` dispatches (by dispatch id) of the kernels whose
+    name contains `sub`: duration in us (trace run) or `counter` (PMC run)."""
+    con = sqlite3.connect(db)
+    if counter is None:
+        rows = con.execute("select dispatch_id, duration from kernels where name like ? "
+                           "order by dispatch_id desc limit ?", (f"%{sub}%", n)).fetchall()
+        vals = [r[1] / 1e3 for r in rows]
+        what = "duration (us)"
+    else:
+        rows = con.execute("select dispatch_id, sum(value) from counters_collection "
+                           "where kernel_name like ? and counter_name = ? group by dispatch_id "
+                           "order by dispatch_id desc limit ?", (f"%{sub}%", counter, n)).fetchall()
+        vals = [r[1] for r in rows]
+        what = f"{counter} (KiB)"
+    if not vals:
+        return f"no dispatches of {sub}"
+    ids = sorted(r[0] for r in rows)
+    return (f"{sub}: last {len(vals)} dispatches (ids {ids[0]}..{ids[-1]}): average {what} = "
+            f"{sum(vals) / len(vals):.1f}, min {min(vals):.1f}, max {max(vals):.1f}")
+
+
 def copycal():
     import torch
     n = 64 * 1024 * 1024  # floats: 256 MiB
@@ -50,6 +75,9 @@ def copycal():
 if __name__ == "__main__":
     if sys.argv[1] == "copycal":
         copycal()
+    elif sys.argv[1] == "last":
+        print(last(sys.argv[2], sys.argv[3], int(sys.argv[4]),
+                   sys.argv[5] if len(sys.argv) > 5 else None))
     elif sys.argv[1] == "trace":
         print(trace(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 15))
     else:
