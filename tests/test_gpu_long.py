@@ -146,3 +146,11 @@ def test_max_bodies_reached_and_slot_reuse(product, oracle):
     assert compare.diff_bodies(ba, bb) is None
     assert compare.diff_manifolds(ma, mb) is None
     assert compare.diff_stats(sa, sb) is None
+
+
+def test_wide_id_space_uses_64_bit_pair_keys(product, oracle):
+    # 70 000 body slots: pair keys need 2 x 17 id bits, beyond 32, so the pair
+    # sort runs on 64-bit keys; the sweep order is sorted device-wide (over
+    # 8192 slots per world)
+    sc = scenes.c0_small(frames=60)
+    _lockstep(product, oracle, sc, 60, check_every=10, cfg_over=dict(max_bodies=70000))

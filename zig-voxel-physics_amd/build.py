@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 OUT = os.path.join(PKG, "libvxp.so")
 SOURCES = ["vxp_kernels.hip", "vxp_prims.hip", "vxp_api.cpp", "vxp_serve.cpp", "vxp_noalloc.hip"]
-HEADERS = ["vxp_math.hpp", "vxp_dev.hpp", "vxp_kernels.hpp", "vxp_tree.hpp", "../../include/vxp.h"]
+HEADERS = ["vxp_math.hpp", "vxp_dev.hpp", "vxp_kernels.hpp", "vxp_tree.hpp", "vxp_sort.hpp", "../../include/vxp.h"]
 
 ARCH = os.environ.get("VXP_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",

@@ -17,7 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
-#include <rocprim/block/block_radix_sort.hpp>
+#include "vxp_sort.hpp"
 
 #include "vxp_dev.hpp"
 #include "vxp_kernels.hpp"
@@ -399,22 +399,23 @@ __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n
 template <uint32_t kBS, uint32_t kIPT>
 __global__ __launch_bounds__(kBS) void k_sweep_sort(DevCfg cfg, DevBodies B, const WorldState* ws,
                                                     uint32_t* order) {
-  using Sort = rocprim::block_radix_sort<uint32_t, kBS, kIPT, uint32_t>;
-  __shared__ typename Sort::storage_type storage;
+  __shared__ uint32_t s_k[kBS * kIPT], s_v[kBS * kIPT];
+  __shared__ uint32_t s_wcnt[2 * (kBS / 64) * kRadixBins], s_run[2 * kRadixBins];
   const uint32_t w = blockIdx.x, MB = cfg.max_bodies, ax = ws[w].axis;
+  for (uint32_t d = threadIdx.x; d < 2 * (kBS / 64) * kRadixBins; d += kBS) s_wcnt[d] = 0;
   uint32_t keys[kIPT], vals[kIPT];
 #pragma unroll
-  for (uint32_t k = 0; k < kIPT; k++) {
-    const uint32_t sl = threadIdx.x * kIPT + k;
+  for (uint32_t k = 0; k < kIPT; k++) {  // striped: slot k * kBS + tid
+    const uint32_t sl = k * kBS + threadIdx.x;
     const uint32_t g = w * MB + sl;
     keys[k] = 0xFFFFFFFFu;
     vals[k] = g;
     if (sl < MB && B.active[g]) keys[k] = orderedFloat(comp(B.fat_lo[g], ax));
   }
-  Sort().sort(keys, vals, storage);
+  blockRadixSort<kBS, kIPT, true>(keys, vals, MB, 32, s_k, s_v, s_wcnt, s_run);
 #pragma unroll
   for (uint32_t k = 0; k < kIPT; k++) {
-    const uint32_t r = threadIdx.x * kIPT + k;
+    const uint32_t r = k * kBS + threadIdx.x;
     if (r < MB) order[w * MB + r] = vals[k];
   }
 }
@@ -3540,6 +3541,7 @@ bool launch_sweep_sort(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wor
   if (MB <= 1024) k_sweep_sort<256, 4><<<nw, 256, 0, s>>>(cfg, B, ws, order);
   else if (MB <= 2048) k_sweep_sort<256, 8><<<nw, 256, 0, s>>>(cfg, B, ws, order);
   else if (MB <= 4096) k_sweep_sort<1024, 4><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
+  else if (MB <= 5120) k_sweep_sort<1024, 5><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
   else k_sweep_sort<1024, 8><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
   return true;
 }

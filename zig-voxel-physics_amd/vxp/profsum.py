@@ -19,9 +19,6 @@ def trace(db, top=15):
     out = ["| kernel | calls | total ms | avg us | % |", "|---|---|---|---|---|"]
     for name, n, tot, avg, pct in rows:
         short = name.split("(")[0].replace("void ", "")
-        if "rocprim" in short:
-            short = "rocprim::" + short.split("wrapped_")[-1].split("<")[0] if "wrapped_" in short \
-                else "rocprim (" + short.split("::")[-1][:40] + ")"
         out.append(f"| {short[:70]} | {n} | {tot / 1e3:.3f} | {avg:.1f} | {pct:.1f} |")
     return "\n".join(out)
 
