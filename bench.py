@@ -275,10 +275,14 @@ def main():
     avg_solve_s = solve_ms / max(solve_n, 1) / 1e3
     alg_per_launch = plain["alg_solver_bytes"] / max(solve_n, 1)
     achieved = alg_per_launch / avg_solve_s / 1e9 if solve_n else 0.0
+    # HBM bytes per launch from the PMC runs of the same frame window (null when
+    # no run covers this window)
     traffic = None
     tf = os.path.join(REPO, "profiles", "k_solve_units_c1_traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        for wnd in json.load(open(tf)).get("windows", []):
+            if wnd["frames"] == f"{first}..{last}":
+                traffic = wnd["hbm_bytes_per_launch"]
 
     # ---- the 64-cube scene (the small configuration), frames warmup..299 -------
     c0 = None

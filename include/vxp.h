@@ -440,22 +440,38 @@ int vxp_server_manifolds_get(vxp_server* srv, uint32_t world, vxp_manifold* out,
 int vxp_server_info_get(vxp_server* srv, vxp_server_info* out);
 
 /* Launch-shape tuning.  Every choice computes bit-identical results; only the
- * speed differs.  An online tuner times the candidates on live frames (a
- * window of 15 frames at creation, again every 600 frames) and keeps the
- * fastest.  pair_color_max: colours of at most this many manifolds are solved
+ * speed differs.  An online tuner times the candidates of one knob at a time
+ * on live frames (3 frames per candidate, timed over the phase the knob
+ * shapes: solve, broad phase or voxel probe), keeps the fastest, and goes on
+ * to the next knob; a round over every knob that is not forced starts at
+ * creation and again 600 frames after the last one ended.
+ * pair_color_max: colours of at most this many manifolds are solved
  * on lane pairs (one body per lane) instead of one lane per manifold.
  * unit_threads: threads per solve-unit workgroup, 256 or 512 (default: 256
  * for systems of 64 worlds or more, where many units share the device; 512
- * otherwise, where one large unit bounds the solve). */
+ * otherwise, where one large unit bounds the solve).  broad_phase: how the
+ * pair set of the untraced pair order is found (VXP_PAIR_ORDER_TREE keeps the
+ * reference's trees): a sort-and-sweep along each world's best axis, or a
+ * Morton-sorted LBVH per world; both find the pairs of
+ * BroadPhase.findOverlappingPairs (src/broad_phase.zig:488-526). */
+#define VXP_BROAD_AUTO 0u
+#define VXP_BROAD_SWEEP 1u
+#define VXP_BROAD_LBVH 2u
 typedef struct vxp_tuning {
   uint32_t pair_color_max;
   uint32_t windows;       /* tuning windows completed */
   uint32_t fixed;         /* set by vxp_tuning_set: the tuner is off */
   uint32_t unit_threads;  /* get: in use; set: 256 or 512, 0 = default */
+  uint32_t broad_phase;   /* get: in use; set: VXP_BROAD_*, AUTO = tuned */
+  uint32_t probe_threads; /* voxel probe workgroup: 64, 128 or 256 threads; 0 = tuned */
+  uint32_t probe_xcd_run; /* probe workgroups kept on one XCD per run (power of two
+                             <= 4096; consecutive work items of a pair share its
+                             L2); 0 = tuned */
 } vxp_tuning;
 int vxp_tuning_get(vxp_system* sys, vxp_tuning* out);
-/* Fix the tuning (the tuner stops); pair_color_max = 0xFFFFFFFF restarts it.
- * unit_threads is applied either way (VXP_ERR_ARGUMENT unless 0, 256 or 512). */
+/* Fix the tuning (the tuner stops); pair_color_max = 0xFFFFFFFF restarts it
+ * instead, over the knobs left at 0.  The other knobs are forced either way
+ * when not 0 (VXP_ERR_ARGUMENT for a value outside those listed above). */
 int vxp_tuning_set(vxp_system* sys, const vxp_tuning* tuning);
 
 /* Last error message of this thread (never NULL). */

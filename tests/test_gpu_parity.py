@@ -147,26 +147,34 @@ def test_empty_collider_body_in_scene(product, oracle, order):
 
 
 def test_tuning_choices_are_bit_identical(product):
-    # The lane-pair colour limit and the solver's workgroup size change only
-    # the launch shape: forced extremes (every colour on single lanes / every
-    # colour of up to 1024 manifolds on lane pairs), both workgroup sizes and
-    # the online tuner give the same bits.
+    # Every tuning knob changes only a launch shape: forced extremes of the
+    # lane-pair colour limit (every colour on single lanes / every colour of up
+    # to 1024 manifolds on lane pairs), both solver workgroup sizes, both broad
+    # phases, every probe workgroup size and XCD runs of 1, 8 and 64, and the
+    # online tuner going through a whole round, give the same bits.
     sc = scenes.batched_worlds(n_worlds=6, lo=8, hi=1024, blob_pool=8)
     runs = []
-    for forced, threads in ((0, 512), (1024, 256), (0, 256), (None, 0)):
+    shapes = ((0, 512, "sweep", 256, 64), (1024, 256, "lbvh", 64, 8), (0, 256, "sweep", 128, 1),
+              (None, 0, "auto", 0, 0))
+    for forced, threads, bp, pt, run in shapes:
         s = vxp.from_scene(sc, lib=product)
         if forced is not None:
-            s.setTuning(forced, threads)
-        print(f"tuning forced={forced} unit_threads={threads}", flush=True)
-        for _ in range(40):
+            s.setTuning(forced, threads, bp, pt, run)
+        print(f"tuning forced={forced} unit_threads={threads} broad={bp} probe={pt}/{run}",
+              flush=True)
+        for _ in range(60):
             s.update(sc.dt)
         t = s.tuning()
         assert t["fixed"] == (forced is not None)
-        assert t["unit_threads"] == (threads or 512)  # 6 worlds: the default is 512
         if forced is not None:
             assert t["pair_color_max"] == forced and t["windows"] == 0
+            assert (t["unit_threads"], t["broad_phase"], t["probe_threads"],
+                    t["probe_xcd_run"]) == (threads, bp, pt, run)
         else:
-            assert t["windows"] >= 1
+            # one window per knob: 15 + 6 + 6 + 9 + 9 frames
+            assert t["windows"] >= 5
+            assert t["unit_threads"] in (256, 512) and t["broad_phase"] in ("sweep", "lbvh")
+            assert t["probe_threads"] in (64, 128, 256) and t["probe_xcd_run"] in (8, 64, 512)
         runs.append(([s.bodies(w) for w in range(len(sc.worlds))],
                      [s.manifolds(w) for w in range(len(sc.worlds))],
                      [s.stats(w) for w in range(len(sc.worlds))]))
@@ -176,6 +184,18 @@ def test_tuning_choices_are_bit_identical(product):
             assert compare.diff_bodies(other[0][w], runs[0][0][w]) is None, w
             assert compare.diff_manifolds(other[1][w], runs[0][1][w]) is None, w
             assert compare.diff_stats(other[2][w], runs[0][2][w]) is None, w
+
+
+def test_tuning_rejects_values_outside_the_knobs(product):
+    s = vxp.from_scene(scenes.c0_small(frames=1), lib=product)
+    for kw in (dict(unit_threads=384), dict(probe_threads=96), dict(probe_xcd_run=48),
+               dict(probe_xcd_run=8192)):
+        with pytest.raises(vxp.VxpError):
+            s.setTuning(None, **kw)
+    s.setTuning(None, probe_threads=128, probe_xcd_run=16)
+    t = s.tuning()
+    assert (t["probe_threads"], t["probe_xcd_run"], t["fixed"]) == (128, 16, False)
+    s.close()
 
 
 def test_body_mutators_between_frames(product, oracle):

@@ -417,10 +417,9 @@ int ensureIslands(vxp_system* s) {
 // Threads per solve-unit workgroup: forced, or 256 when many worlds' units
 // share the device (two workgroups fit a CU) and 512 when a few large units
 // bound the solve (more lane pairs per colour).
-uint32_t unitThreads(const vxp_system* s) {
-  if (s->unit_threads) return s->unit_threads;
-  return s->nw >= 64 ? 256u : 512u;
-}
+// This step's launch shapes (chosen by tuneStep).
+uint32_t unitThreads(const vxp_system* s) { return s->tune.use[vxp_system::kUnitThreads]; }
+uint32_t broadPhase(const vxp_system* s) { return s->tune.use[vxp_system::kBroadPhase]; }
 
 IslandArrays islandArrays(vxp_system* s) {
   return IslandArrays{s->d_isl_parent.as<uint32_t>(), s->d_isl_end.as<uint32_t>(),
@@ -520,56 +519,88 @@ int readSizes(vxp_system* s, std::initializer_list<const uint32_t*> src, uint32_
   return VXP_OK;
 }
 
-// Folds in the solver time of the previous tuned frame (its events completed
-// before this step's first size read), then chooses this frame's candidate.
-int tuneBegin(vxp_system* s, DevCfg& cfg) {
+static vxp_system::Phase knobPhase(uint32_t k) {
+  return k == vxp_system::kBroadPhase ? vxp_system::kPhPairs
+         : k >= vxp_system::kProbeThreads ? vxp_system::kPhProbe
+                                          : vxp_system::kPhSolve;
+}
+
+// At the start of a step: folds in the previous tuned step's time (its events
+// completed before this step begins), ends a window when its samples are in
+// (keeping the fastest candidate), starts the next window of the round when
+// one is due, and sets this step's value of every knob.
+int tuneStep(vxp_system* s) {
   auto& t = s->tune;
-  if (t.ms.size() != t.cand.size()) {
-    t.ms.assign(t.cand.size(), 0.0);
-    t.n.assign(t.cand.size(), 0);
-  }
+  constexpr uint32_t K = vxp_system::kKnobs;
+  if (!t.pick[vxp_system::kUnitThreads])  // two 256-thread units fit a CU: many worlds
+    t.pick[vxp_system::kUnitThreads] = s->nw >= 64 ? 256u : 512u;
   if (t.pending) {
-    float ms = 0.0f;
-    HIP_TRY(hipEventSynchronize(t.b));
-    HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
-    t.ms[t.pending_idx] += ms;
-    t.n[t.pending_idx]++;
+    if (t.marked_a && t.marked_b) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventSynchronize(t.b));
+      HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+      t.ms[t.pending_idx] += ms;
+      t.n[t.pending_idx]++;
+    }
     t.pending = false;
-    if (t.window_left == 0) {  // window over: keep the fastest
+    if (t.window_left == 0) {  // window over: keep the fastest candidate
       double best = 1e300;
-      for (size_t i = 0; i < t.cand.size(); i++)
+      for (size_t i = 0; i < t.cand[t.knob].size(); i++)
         if (t.n[i] && t.ms[i] / t.n[i] < best) {
           best = t.ms[i] / t.n[i];
-          t.pick = t.cand[i];
+          t.pick[t.knob] = t.cand[t.knob][i];
         }
       t.windows++;
-      t.next_window = t.frame + 600;
+      // the next knob of the round at once; after the last, a new round later
+      uint32_t k = t.knob + 1;
+      while (k < K && t.forced[k]) k++;
+      if (k < K) {
+        t.knob = k;
+        t.next_window = t.frame;
+      } else {
+        t.knob = 0;
+        t.next_window = t.frame + 600;
+      }
     }
   }
-  cfg.pair_color_max = t.pick;
+  for (uint32_t k = 0; k < K; k++) t.use[k] = t.forced[k] ? t.forced[k] : t.pick[k];
+  t.frame++;
   if (t.fixed) return VXP_OK;
-  if (t.window_left == 0 && t.frame >= t.next_window) {
-    t.window_left = 3 * (uint32_t)t.cand.size();
-    std::fill(t.ms.begin(), t.ms.end(), 0.0);
-    std::fill(t.n.begin(), t.n.end(), 0u);
+  if (t.window_left == 0 && t.frame - 1 >= t.next_window) {
+    while (t.knob < K && t.forced[t.knob]) t.knob++;
+    if (t.knob == K) {  // every knob forced
+      t.knob = 0;
+      return VXP_OK;
+    }
+    t.window_left = 3 * (uint32_t)t.cand[t.knob].size();
+    t.ms.assign(t.cand[t.knob].size(), 0.0);
+    t.n.assign(t.cand[t.knob].size(), 0u);
   }
   if (t.window_left) {
     if (!t.a) {
       HIP_TRY(hipEventCreate(&t.a));
       HIP_TRY(hipEventCreate(&t.b));
     }
-    t.pending_idx = t.window_left % (uint32_t)t.cand.size();
-    cfg.pair_color_max = t.cand[t.pending_idx];
-    HIP_TRY(hipEventRecord(t.a, s->stream));
+    t.pending_idx = t.window_left % (uint32_t)t.cand[t.knob].size();
+    t.use[t.knob] = t.cand[t.knob][t.pending_idx];
     t.pending = true;
+    t.marked_a = t.marked_b = false;
     t.window_left--;
   }
   return VXP_OK;
 }
-int tuneEnd(vxp_system* s) {
+
+// Brackets the phase that the window's knob shapes (events on the main stream).
+int tuneMark(vxp_system* s, vxp_system::Phase ph, bool begin) {
   auto& t = s->tune;
-  if (t.pending) HIP_TRY(hipEventRecord(t.b, s->stream));
-  t.frame++;
+  if (!t.pending || knobPhase(t.knob) != ph) return VXP_OK;
+  if (begin) {
+    HIP_TRY(hipEventRecord(t.a, s->stream));
+    t.marked_a = true;
+  } else if (t.marked_a) {
+    HIP_TRY(hipEventRecord(t.b, s->stream));
+    t.marked_b = true;
+  }
   return VXP_OK;
 }
 
@@ -592,7 +623,9 @@ int stepAll(vxp_system* s) {
 static int stepAllImpl(vxp_system* s) {
   hipStream_t st = s->stream;
   RC(ensureDevice(s));
+  RC(tuneStep(s));
   DevCfg cfg = devCfg(s);
+  cfg.pair_color_max = s->tune.use[vxp_system::kPairColorMax];
   DevBodies B = devBodies(s);
   DevColliders C = devColliders(s);
   const uint32_t nw = s->nw;
@@ -622,6 +655,7 @@ static int stepAllImpl(vxp_system* s) {
   uint32_t id_bits = 1;  // body ids are below id_cap
   while ((1u << id_bits) < s->id_cap) id_bits++;
   uint32_t npairs = 0;
+  RC(tuneMark(s, vxp_system::kPhPairs, true));
   if (tree) {
     // ---- pair set in tree order: re-insert the moved leaves, then every
     // dynamic leaf queries both trees (count, scan, write) --------------------
@@ -647,6 +681,54 @@ static int stepAllImpl(vxp_system* s) {
                         s->d_pairs_raw.as<uint64_t>(), s->d_t_pos.as<uint32_t>(), id_bits);
     profEnd(s, VXP_K_PAIRS, pe);
     s->t_dev_newer = true;
+  } else if (broadPhase(s) == VXP_BROAD_LBVH) {
+    // ---- pair set: LBVH, one stackless query per leaf (count, scan, write) --
+    HIP_TRY(s->d_skeys.ensure((size_t)ns * 8));
+    HIP_TRY(s->d_svals.ensure((size_t)ns * 4));
+    HIP_TRY(s->d_skeys2.ensure((size_t)ns * 8));
+    HIP_TRY(s->d_svals2.ensure((size_t)ns * 4));
+    HIP_TRY(s->d_slo.ensure((size_t)ns * 16));
+    HIP_TRY(s->d_shi.ensure((size_t)ns * 16));
+    HIP_TRY(s->d_lb_ilo.ensure((size_t)ns * 16));
+    HIP_TRY(s->d_lb_ihi.ensure((size_t)ns * 16));
+    HIP_TRY(s->d_lb_code.ensure((size_t)ns * 4));
+    HIP_TRY(s->d_lb_child.ensure((size_t)ns * 8));
+    HIP_TRY(s->d_lb_range.ensure((size_t)ns * 8));
+    HIP_TRY(s->d_lb_iparent.ensure((size_t)ns * 4));
+    HIP_TRY(s->d_lb_lparent.ensure((size_t)ns * 4));
+    HIP_TRY(s->d_lb_flag.ensure((size_t)ns * 4));
+    HIP_TRY(s->d_wcnt.ensure((size_t)(ns + 1) * 4));
+    HIP_TRY(s->d_woff.ensure((size_t)(ns + 1) * 4));
+    const LbvhBufs L{s->d_slo.as<float4>(),        s->d_shi.as<float4>(),
+                     s->d_lb_ilo.as<float4>(),     s->d_lb_ihi.as<float4>(),
+                     s->d_lb_code.as<uint32_t>(),  s->d_lb_child.as<uint2>(),
+                     s->d_lb_range.as<uint2>(),    s->d_lb_iparent.as<uint32_t>(),
+                     s->d_lb_lparent.as<uint32_t>(), s->d_lb_flag.as<uint32_t>()};
+    pe = profBegin(s, VXP_K_PAIRS);
+    if (!launch_lbvh_sort(st, cfg, B, ws, s->d_svals2.as<uint32_t>(), nw)) {
+      launch_lbvh_keys(st, cfg, B, ws, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(), ns, nw);
+      size_t tb = sort_pairs_u64_u32_bytes(ns, 32 + wbits);
+      HIP_TRY(s->d_sort_tmp.ensure(tb));
+      HIP_TRY(sort_pairs_u64_u32(s->d_sort_tmp.p, tb, s->d_skeys.as<uint64_t>(),
+                                 s->d_skeys2.as<uint64_t>(), s->d_svals.as<uint32_t>(),
+                                 s->d_svals2.as<uint32_t>(), ns, 32 + wbits, st));
+    }
+    launch_lbvh_build(st, cfg, B, ws, s->d_svals2.as<uint32_t>(), ns, L);
+    const bool narrow = 2 * id_bits + wbits <= 32;
+    launch_lbvh_pairs(st, cfg, ws, ns, L, s->d_wcnt.as<uint32_t>(), nullptr, nullptr, id_bits,
+                      narrow);
+    {
+      size_t cb = exclusive_scan_u32_bytes(ns + 1);
+      HIP_TRY(s->d_sort_tmp.ensure(cb));
+      HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wcnt.as<uint32_t>(),
+                                 s->d_woff.as<uint32_t>(), ns + 1, st));
+    }
+    RC(readSizes(s, {s->d_woff.as<uint32_t>() + ns}, &npairs));
+    HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
+    if (npairs)
+      launch_lbvh_pairs(st, cfg, ws, ns, L, nullptr, s->d_woff.as<uint32_t>(),
+                        s->d_pairs_raw.as<uint64_t>(), id_bits, narrow);
+    profEnd(s, VXP_K_PAIRS, pe);
   } else {
   // ---- pair set: sort-and-sweep --------------------------------------------
   HIP_TRY(s->d_skeys.ensure((size_t)ns * 8));
@@ -737,6 +819,7 @@ static int stepAllImpl(vxp_system* s) {
     if (tree || kbits > 32) launch_expand_keys(st, s->d_pairs.as<uint64_t>(), npairs, id_bits);
   }
   profEnd(s, VXP_K_PAIRS, pe);
+  RC(tuneMark(s, vxp_system::kPhPairs, false));
   const uint64_t* keys = s->d_pairs.as<uint64_t>();
 
   // ---- narrow phase ----------------------------------------------------------
@@ -818,13 +901,16 @@ static int stepAllImpl(vxp_system* s) {
   HIP_TRY(s->d_wi_pair.ensure((size_t)std::max(total_wi, 1u) * 4));
   // pass 1 counts, the scan places every work item's candidates, pass 2 writes
   pe = profBegin(s, VXP_K_PROBE);
+  RC(tuneMark(s, vxp_system::kPhProbe, true));
+  const ProbeShape shape{s->tune.use[vxp_system::kProbeThreads],
+                         s->tune.use[vxp_system::kProbeXcdRun]};
   uint32_t ncand = 0;
   if (total_wi) {
     launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), npairs,
                    s->d_wi_pair.as<uint32_t>());
     launch_probe(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), total_wi, nullptr,
                  s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
-                 nullptr, true);
+                 nullptr, true, shape);
     size_t cb = exclusive_scan_u32_bytes(total_wi + 1);
     HIP_TRY(s->d_sort_tmp.ensure(cb));
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wi_ccnt.as<uint32_t>(),
@@ -836,7 +922,8 @@ static int stepAllImpl(vxp_system* s) {
     launch_probe(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), total_wi,
                  s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
                  s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(), s->d_cand.as<Cand>(),
-                 false);
+                 false, shape);
+  RC(tuneMark(s, vxp_system::kPhProbe, false));
   profEnd(s, VXP_K_PROBE, pe);
   pe = profBegin(s, VXP_K_REDUCE);
   launch_reduce(st, cfg, B, C, ws, stats, pr, keys, npairs, s->d_wlist.as<uint32_t>(),
@@ -952,7 +1039,7 @@ static int stepAllImpl(vxp_system* s) {
   // whole batch (results do not depend on it)
   cfg.unit_min = std::max(64u, nman / (2 * std::max(s->n_cu, 1u)));
   uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
-  RC(tuneBegin(s, cfg));
+  RC(tuneMark(s, vxp_system::kPhSolve, true));
   pe = profBegin(s, VXP_K_SOLVE);
   launch_solve_prep(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), ctr, nw, max_man,
                     s->lds_limit);
@@ -972,7 +1059,7 @@ static int stepAllImpl(vxp_system* s) {
   pe = profBegin(s, VXP_K_SOLVE_TAIL);
   launch_solve_tail(st, cfg, B, ws, stats, S, ctr, nw, s->n_cu);
   profEnd(s, VXP_K_SOLVE_TAIL, pe);
-  RC(tuneEnd(s));
+  RC(tuneMark(s, vxp_system::kPhSolve, false));
   HIP_TRY(hipGetLastError());
   s->dev_newer = true;
   return VXP_OK;
@@ -991,6 +1078,10 @@ int checkErrors(vxp_system* s) {
     if (w.error & kErrTreeStack) {
       g_err = "a broad-phase tree query exceeded its stack (tree deeper than 63 levels)";
       return VXP_ERR_CAPACITY;
+    }
+    if (w.error & kErrLbvh) {
+      g_err = "an LBVH broad-phase query walked past the size of its tree";
+      return VXP_ERR_DEVICE;
     }
   }
   return VXP_OK;
@@ -1044,10 +1135,19 @@ int vxp_profile_solver_phases(vxp_system* s, uint32_t w, uint64_t* ts) {
 int vxp_tuning_get(vxp_system* s, vxp_tuning* out) {
   if (!s || !out) return VXP_ERR_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
-  out->pair_color_max = s->tune.pick;
-  out->windows = s->tune.windows;
-  out->fixed = s->tune.fixed;
-  out->unit_threads = unitThreads(s);
+  const auto& t = s->tune;
+  auto val = [&](uint32_t k) {
+    if (t.forced[k]) return t.forced[k];
+    if (k == vxp_system::kUnitThreads && !t.pick[k]) return s->nw >= 64 ? 256u : 512u;
+    return t.pick[k];
+  };
+  out->pair_color_max = t.pick[vxp_system::kPairColorMax];
+  out->windows = t.windows;
+  out->fixed = t.fixed;
+  out->unit_threads = val(vxp_system::kUnitThreads);
+  out->broad_phase = val(vxp_system::kBroadPhase);
+  out->probe_threads = val(vxp_system::kProbeThreads);
+  out->probe_xcd_run = val(vxp_system::kProbeXcdRun);
   return VXP_OK;
 }
 int vxp_tuning_set(vxp_system* s, const vxp_tuning* t) {
@@ -1056,14 +1156,33 @@ int vxp_tuning_set(vxp_system* s, const vxp_tuning* t) {
     g_err = "unit_threads must be 0, 256 or 512";
     return VXP_ERR_ARGUMENT;
   }
-  s->unit_threads = t->unit_threads;
-  if (t->pair_color_max == 0xFFFFFFFFu) {
-    s->tune.fixed = false;
-    s->tune.next_window = s->tune.frame;
+  if (t->broad_phase > VXP_BROAD_LBVH) {
+    g_err = "broad_phase must be VXP_BROAD_AUTO, VXP_BROAD_SWEEP or VXP_BROAD_LBVH";
+    return VXP_ERR_ARGUMENT;
+  }
+  if (t->probe_threads != 0 && t->probe_threads != 64 && t->probe_threads != 128 &&
+      t->probe_threads != 256) {
+    g_err = "probe_threads must be 0, 64, 128 or 256";
+    return VXP_ERR_ARGUMENT;
+  }
+  if (t->probe_xcd_run > 4096 || (t->probe_xcd_run & (t->probe_xcd_run - 1))) {
+    g_err = "probe_xcd_run must be 0 or a power of two up to 4096";
+    return VXP_ERR_ARGUMENT;
+  }
+  auto& u = s->tune;
+  u.forced[vxp_system::kUnitThreads] = t->unit_threads;
+  u.forced[vxp_system::kBroadPhase] = t->broad_phase;
+  u.forced[vxp_system::kProbeThreads] = t->probe_threads;
+  u.forced[vxp_system::kProbeXcdRun] = t->probe_xcd_run;
+  u.window_left = 0;
+  u.pending = false;
+  u.knob = 0;
+  if (t->pair_color_max == 0xFFFFFFFFu) {  // a new round from the next step
+    u.fixed = false;
+    u.next_window = u.frame;
   } else {
-    s->tune.fixed = true;
-    s->tune.pick = t->pair_color_max;
-    s->tune.window_left = 0;
+    u.fixed = true;
+    u.pick[vxp_system::kPairColorMax] = t->pair_color_max;
   }
   return VXP_OK;
 }
@@ -1255,7 +1374,9 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
                     &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off,
                     &s->d_t_pos, &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake,
-                    &s->d_arank, &s->d_drop, &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair};
+                    &s->d_arank, &s->d_drop, &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair,
+                    &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code, &s->d_lb_child, &s->d_lb_range,
+                    &s->d_lb_iparent, &s->d_lb_lparent, &s->d_lb_flag};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);

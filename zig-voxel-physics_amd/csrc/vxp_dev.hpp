@@ -94,6 +94,10 @@ struct WorldState {
   uint32_t frame;                     // BrickPairCache.current_frame
   uint32_t error;
   uint32_t axis;                      // sort-and-sweep axis (longest extent)
+  // LBVH broad phase: active slots (tree leaves) and the Morton grid of the
+  // doubled fat-box centres, q = (c - c_lo) * c_scale clamped to [0, 1023]
+  uint32_t n_leaves;
+  float c_lo[3], c_scale[3];
   // solver (written by its prep each step)
   uint32_t n_units, nsolve;           // solve units, solved manifolds
   uint32_t max_unit_m;                // manifolds of the largest unit
@@ -121,6 +125,7 @@ struct TreeSet {
 // WorldState.error bits
 constexpr uint32_t kErrColours = 1;     // solver colouring past its colour limit
 constexpr uint32_t kErrTreeStack = 2;   // a tree query deeper than its stack
+constexpr uint32_t kErrLbvh = 4;        // an LBVH walk longer than its tree (malformed tree)
 
 // Id-indexed island arrays, [world * id_cap + body id]: union-find parents
 // (labels after a step), endpoint flags, component sizes, and the sleep timer

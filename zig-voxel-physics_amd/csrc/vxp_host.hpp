@@ -141,6 +141,8 @@ struct vxp_system {
   DevBuf d_slo, d_shi, d_send, d_wcnt, d_woff, d_swc, d_swo, d_pairs_raw, d_pairs, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_wi_vchk, d_lane_n,
       d_cand,
       d_produce, d_pidx, d_staging, d_cstage, d_total, d_rstat, d_clist, d_cflag, d_cidx, d_wlist, d_step_ctr;
+  DevBuf d_lb_ilo, d_lb_ihi, d_lb_code, d_lb_child, d_lb_range, d_lb_iparent, d_lb_lparent,
+      d_lb_flag;  // LBVH broad phase
   DevBuf d_ml[2];
   int cur = 0;
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;
@@ -166,16 +168,29 @@ struct vxp_system {
   // Online tuning of launch shapes that do not change results: during a window
   // of frames the candidates take turns (one per frame, timed with events
   // around the solver); the fastest on average is kept until the next window.
+  // Online tuner of the launch shapes (every choice gives the same bits).  A
+  // window times the candidates of one knob in turn, round-robin over 3
+  // passes, on the phase the knob shapes, and keeps the fastest; a round
+  // tunes every knob that is not forced, one window after the other.
+  enum Knob { kPairColorMax, kUnitThreads, kBroadPhase, kProbeThreads, kProbeXcdRun, kKnobs };
+  enum Phase { kPhSolve, kPhPairs, kPhProbe };
   struct Tuner {
-    std::vector<uint32_t> cand{0, 64, 128, 256, 512};  // lane-pair colour limits
+    std::vector<uint32_t> cand[kKnobs] = {
+        {0, 64, 128, 256, 512},  // lane-pair colour limit
+        {256, 512},              // threads per solve-unit workgroup
+        {1, 2},                  // VXP_BROAD_SWEEP, VXP_BROAD_LBVH
+        {64, 128, 256},          // threads per probe workgroup
+        {8, 64, 512}};           // probe workgroups per XCD run
+    uint32_t pick[kKnobs] = {128, 0, 1, 256, 64};  // pick[kUnitThreads] 0: by world count
+    uint32_t forced[kKnobs] = {0, 0, 0, 0, 0};     // 0: tuned
+    uint32_t use[kKnobs] = {128, 512, 1, 256, 64};  // this step's values
     std::vector<double> ms;
     std::vector<uint32_t> n;
-    uint32_t pick = 128, frame = 0, window_left = 0, next_window = 0, windows = 0;
-    bool fixed = false, pending = false;
+    uint32_t frame = 0, window_left = 0, next_window = 0, windows = 0, knob = 0;
+    bool fixed = false, pending = false, marked_a = false, marked_b = false;
     uint32_t pending_idx = 0;
     hipEvent_t a = nullptr, b = nullptr;
   } tune;
-  uint32_t unit_threads = 0;  // solve-unit workgroup size forced by vxp_tuning_set (0: default)
   size_t lds_limit = 0;  // dynamic LDS available to the solver block
   uint32_t n_cu = 256;   // compute units of the device
   int device = 0;        // the device current at creation (the streams' device)

@@ -345,16 +345,22 @@ __device__ __forceinline__ float comp(const float4& v, uint32_t a) {
 
 // Sweep axis of each world: the axis with the fewest expected overlaps, i.e.
 // the smallest mean fat-box extent relative to the spread of the box centres
-// (the pair set does not depend on the axis).
+// (the pair set does not depend on the axis).  Also the world's active slots
+// and the bounds of its doubled box centres (the LBVH's Morton grid).
 __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n_worlds) {
   uint32_t w = blockIdx.x;
   if (w >= n_worlds) return;
   __shared__ float lo[3][1024], hi[3][1024], ext[3][1024];
+  __shared__ uint32_t n_act;
+  if (threadIdx.x == 0) n_act = 0;
+  __syncthreads();
   float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
   float e[3] = {0.f, 0.f, 0.f};
+  uint32_t na = 0;
   for (uint32_t s = threadIdx.x; s < cfg.max_bodies; s += blockDim.x) {
     uint32_t g = w * cfg.max_bodies + s;
     if (!B.active[g]) continue;
+    na++;
     float4 a = B.fat_lo[g], b = B.fat_hi[g];
     float c[3] = {a.x + b.x, a.y + b.y, a.z + b.z};
     float x[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
@@ -364,6 +370,7 @@ __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n
       e[k] += fminf(x[k], 1e30f);
     }
   }
+  if (na) atomicAdd(&n_act, na);
   for (int k = 0; k < 3; k++) {
     lo[k][threadIdx.x] = l[k];
     hi[k][threadIdx.x] = h[k];
@@ -390,6 +397,13 @@ __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n
     if (r[1] < r[ax]) ax = 1;
     if (r[2] < r[ax]) ax = 2;
     ws[w].axis = ax;
+    ws[w].n_leaves = n_act;
+    for (int k = 0; k < 3; k++) {
+      // a spread that is zero or not finite puts every centre in cell 0
+      const float spread = hi[k][0] - lo[k][0];
+      ws[w].c_lo[k] = lo[k][0];
+      ws[w].c_scale[k] = spread > 0.0f && spread < INFINITY ? 1023.0f / spread : 0.0f;
+    }
   }
 }
 
@@ -550,6 +564,287 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
     wcount[gw] = n;
     if (gw == kSweepWaves - 1) wcount[kSweepWaves] = 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// broad phase, LBVH: a Karras radix tree per world over the Morton codes of
+// the fat-box centres, refit bottom-up, queried once per leaf.  It finds the
+// same pair set as the sweep (every pair of active boxes that overlap,
+// inclusive, and are not both static; src/broad_phase.zig:488-526) and emits
+// it in the same packed form, so the pair sort after it is shared.
+//
+// Per world w the leaves are its active slots in Morton order, at
+// [w * max_bodies, + n_leaves) of the leaf arrays, and the n_leaves - 1
+// internal nodes at the same offsets of the node arrays (node 0 the root).
+// A node reference is a leaf index | kLbLeaf, an internal index, or kLbEnd.
+// Every node carries its escape ("rope"): the first node after its subtree in
+// depth-first order, so a query needs no stack, and a query that starts at
+// leaf i's escape meets exactly the leaves after i, each pair once.
+//   leaf k:     llo = lo.xyz, id     lhi = hi.xyz, escape | kLbStatic if static
+//   internal i: ilo = lo.xyz, left   ihi = hi.xyz, escape
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLbLeaf = 0x80000000u, kLbEnd = 0x40000000u, kLbStatic = 0x20000000u;
+constexpr uint32_t kLbIdx = 0x1FFFFFFFu;
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+// 30-bit Morton code of a fat box's doubled centre on its world's 1024^3 grid.
+__device__ __forceinline__ uint32_t mortonOf(float4 lo, float4 hi, const WorldState& w) {
+  const float c[3] = {lo.x + hi.x, lo.y + hi.y, lo.z + hi.z};
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float t = fminf(fmaxf((c[k] - w.c_lo[k]) * w.c_scale[k], 0.0f), 1023.0f);  // NaN -> 0
+    q[k] = (uint32_t)t;
+  }
+  return (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+}
+
+// Morton order of each world in one block (inactive slots last, ties in slot
+// order, as the device-wide sort of (world, code) keys).
+template <uint32_t kBS, uint32_t kIPT>
+__global__ __launch_bounds__(kBS) void k_lbvh_sort(DevCfg cfg, DevBodies B, const WorldState* ws,
+                                                   uint32_t* order) {
+  __shared__ uint32_t s_k[kBS * kIPT], s_v[kBS * kIPT];
+  __shared__ uint32_t s_wcnt[2 * (kBS / 64) * kRadixBins], s_run[2 * kRadixBins];
+  const uint32_t w = blockIdx.x, MB = cfg.max_bodies;
+  const WorldState W = ws[w];
+  for (uint32_t d = threadIdx.x; d < 2 * (kBS / 64) * kRadixBins; d += kBS) s_wcnt[d] = 0;
+  uint32_t keys[kIPT], vals[kIPT];
+#pragma unroll
+  for (uint32_t k = 0; k < kIPT; k++) {  // striped: slot k * kBS + tid
+    const uint32_t sl = k * kBS + threadIdx.x;
+    const uint32_t g = w * MB + sl;
+    keys[k] = 0xFFFFFFFFu;
+    vals[k] = g;
+    if (sl < MB && B.active[g]) keys[k] = mortonOf(B.fat_lo[g], B.fat_hi[g], W);
+  }
+  blockRadixSort<kBS, kIPT, true>(keys, vals, MB, 32, s_k, s_v, s_wcnt, s_run);
+#pragma unroll
+  for (uint32_t k = 0; k < kIPT; k++) {
+    const uint32_t r = k * kBS + threadIdx.x;
+    if (r < MB) order[w * MB + r] = vals[k];
+  }
+}
+
+__global__ void k_lbvh_keys(DevCfg cfg, DevBodies B, const WorldState* ws, uint64_t* keys,
+                            uint32_t* vals, uint32_t n_slots) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_slots) return;
+  uint32_t w = g / cfg.max_bodies;
+  uint32_t k = B.active[g] ? mortonOf(B.fat_lo[g], B.fat_hi[g], ws[w]) : 0xFFFFFFFFu;
+  keys[g] = ((uint64_t)w << 32) | k;
+  vals[g] = g;
+}
+
+// Leaves in Morton order: boxes, ids, static flags and codes.
+__global__ void k_lbvh_gather(DevCfg cfg, DevBodies B, const WorldState* ws,
+                              const uint32_t* order, uint32_t n_slots, float4* llo, float4* lhi,
+                              uint32_t* code) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_slots) return;
+  const uint32_t w = i / cfg.max_bodies;
+  if (i - w * cfg.max_bodies >= ws[w].n_leaves) return;
+  const uint32_t g = order[i];
+  float4 lo = B.fat_lo[g], hi = B.fat_hi[g];
+  code[i] = mortonOf(lo, hi, ws[w]);
+  lo.w = __uint_as_float(B.id[g]);
+  hi.w = __uint_as_float(B.type[g] == VXP_BODY_STATIC ? kLbStatic : 0u);
+  llo[i] = lo;
+  lhi[i] = hi;
+}
+
+// Common-prefix length of leaves a and b of a world (codes extended by the
+// leaf index so that equal codes still split); -1 outside [0, n).
+__device__ __forceinline__ int lbDelta(const uint32_t* code, int n, int a, int b) {
+  if (b < 0 || b >= n) return -1;
+  const uint32_t ca = code[a], cb = code[b];
+  return ca != cb ? __clz(ca ^ cb) : 32 + __clz((uint32_t)a ^ (uint32_t)b);
+}
+
+// Internal node i of each world (Karras 2012, "Maximizing parallelism in the
+// construction of BVHs, octrees, and k-d trees", section 4): its key range
+// [first, last], split and children; parents of both children; the refit
+// counter cleared.
+__global__ void k_lbvh_build(DevCfg cfg, const WorldState* ws, uint32_t n_slots,
+                             const uint32_t* code_all, uint2* child, uint2* range,
+                             uint32_t* iparent, uint32_t* lparent, uint32_t* flag) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_slots) return;
+  const uint32_t w = t / cfg.max_bodies, base = w * cfg.max_bodies;
+  const int n = (int)ws[w].n_leaves, i = (int)(t - base);
+  if (i + 1 >= n) return;
+  const uint32_t* code = code_all + base;
+  const int d = lbDelta(code, n, i, i + 1) > lbDelta(code, n, i, i - 1) ? 1 : -1;
+  const int dmin = lbDelta(code, n, i, i - d);
+  int lmax = 2;
+  while (lbDelta(code, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int s = lmax >> 1; s >= 1; s >>= 1)
+    if (lbDelta(code, n, i, i + (l + s) * d) > dmin) l += s;
+  const int j = i + l * d;
+  const int dnode = lbDelta(code, n, i, j);
+  int s = 0;
+  for (int div = 2;; div <<= 1) {
+    const int st = (l + div - 1) / div;
+    if (lbDelta(code, n, i, i + (s + st) * d) > dnode) s += st;
+    if (st == 1) break;
+  }
+  const int gamma = i + s * d + min(d, 0);
+  const int lo = min(i, j), hi = max(i, j);
+  const uint32_t left = lo == gamma ? (kLbLeaf | (uint32_t)gamma) : (uint32_t)gamma;
+  const uint32_t right = hi == gamma + 1 ? (kLbLeaf | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+  child[t] = make_uint2(left, right);
+  range[t] = make_uint2((uint32_t)lo, (uint32_t)hi);
+  flag[t] = 0;
+  if (left & kLbLeaf) lparent[base + gamma] = (uint32_t)i;
+  else iparent[base + gamma] = (uint32_t)i;
+  if (right & kLbLeaf) lparent[base + gamma + 1] = (uint32_t)i;
+  else iparent[base + gamma + 1] = (uint32_t)i;
+}
+
+// Escape of a node whose key range ends at leaf `last`: the subtree that
+// starts at leaf last + 1 (internal node last + 1 when its range starts
+// there, else that leaf), or kLbEnd after the world's last leaf.
+__device__ __forceinline__ uint32_t lbEscape(const uint2* range, int n, int last) {
+  if (last + 1 >= n) return kLbEnd;
+  if (last + 1 <= n - 2 && (int)range[last + 1].x == last + 1) return (uint32_t)(last + 1);
+  return kLbLeaf | (uint32_t)(last + 1);
+}
+
+// Leaf k's escape, then node boxes bottom-up from it: the second of a node's
+// two children to finish computes the node (the counter orders the writes of
+// the first before the reads of the second), up to the root.  kWg: the
+// world's leaves are all handled by one workgroup, counters in LDS and
+// workgroup-scope ordering; otherwise device-scope counters and fences.
+template <bool kWg>
+__device__ __forceinline__ void lbRefitLeaf(uint32_t base, int n, int k, float4* llo, float4* lhi,
+                                            const uint2* child, const uint2* range,
+                                            const uint32_t* iparent, const uint32_t* lparent,
+                                            uint32_t* flag, float4* ilo, float4* ihi) {
+  {
+    float4 h = lhi[base + k];
+    h.w = __uint_as_float((__float_as_uint(h.w) & kLbStatic) | lbEscape(range + base, n, k));
+    lhi[base + k] = h;
+  }
+  if (n < 2) return;
+  uint32_t p = lparent[base + k];
+  for (int up = 0; up < n; up++) {  // at most n - 1 levels
+    if (kWg) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (atomicAdd(&flag[p], 1u) == 0u) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+      __threadfence();
+      if (atomicAdd(&flag[base + p], 1u) == 0u) break;
+      __threadfence();
+    }
+    const uint2 ch = child[base + p];
+    const uint32_t a = ch.x & kLbIdx, b = ch.y & kLbIdx;
+    const float4 alo = (ch.x & kLbLeaf) ? llo[base + a] : ilo[base + a];
+    const float4 ahi = (ch.x & kLbLeaf) ? lhi[base + a] : ihi[base + a];
+    const float4 blo = (ch.y & kLbLeaf) ? llo[base + b] : ilo[base + b];
+    const float4 bhi = (ch.y & kLbLeaf) ? lhi[base + b] : ihi[base + b];
+    const float4 lo = make_float4(fminf(alo.x, blo.x), fminf(alo.y, blo.y), fminf(alo.z, blo.z),
+                                  __uint_as_float(ch.x));
+    const float4 hi = make_float4(fmaxf(ahi.x, bhi.x), fmaxf(ahi.y, bhi.y), fmaxf(ahi.z, bhi.z),
+                                  __uint_as_float(lbEscape(range + base, n, (int)range[base + p].y)));
+    ilo[base + p] = lo;
+    ihi[base + p] = hi;
+    if (p == 0) break;
+    p = iparent[base + p];
+  }
+}
+
+// One thread per leaf slot over every world (counters cleared by the build).
+__global__ void k_lbvh_refit(DevCfg cfg, const WorldState* ws, uint32_t n_slots, float4* llo,
+                             float4* lhi, const uint2* child, const uint2* range,
+                             const uint32_t* iparent, const uint32_t* lparent, uint32_t* flag,
+                             float4* ilo, float4* ihi) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_slots) return;
+  const uint32_t w = t / cfg.max_bodies, base = w * cfg.max_bodies;
+  const int n = (int)ws[w].n_leaves, k = (int)(t - base);
+  if (k >= n) return;
+  lbRefitLeaf<false>(base, n, k, llo, lhi, child, range, iparent, lparent, flag, ilo, ihi);
+}
+
+// One workgroup per world, counters in LDS (max_bodies words): the fences
+// stay inside the workgroup's compute unit instead of reaching every XCD's L2.
+__global__ void k_lbvh_refit_wg(DevCfg cfg, const WorldState* ws, float4* llo, float4* lhi,
+                                const uint2* child, const uint2* range, const uint32_t* iparent,
+                                const uint32_t* lparent, float4* ilo, float4* ihi) {
+  extern __shared__ uint32_t s_flag[];
+  const uint32_t w = blockIdx.x, base = w * cfg.max_bodies;
+  const int n = (int)ws[w].n_leaves;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s_flag[i] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x)
+    lbRefitLeaf<true>(base, n, k, llo, lhi, child, range, iparent, lparent, s_flag, ilo, ihi);
+}
+
+__device__ __forceinline__ bool lbOverlap(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+  return alo.x <= bhi.x && ahi.x >= blo.x && alo.y <= bhi.y && ahi.y >= blo.y &&
+         alo.z <= bhi.z && ahi.z >= blo.z;
+}
+
+// One query per leaf from its escape: count its pairs (cnt[i]; cnt[n] = 0),
+// or write them from off[i] as packed (world, min id, max id) keys, 32-bit
+// when `narrow`.
+template <bool kCount>
+__global__ __launch_bounds__(256) void k_lbvh_pairs(DevCfg cfg, WorldState* ws,
+                                                    uint32_t n_slots, const float4* llo,
+                                                    const float4* lhi, const float4* ilo,
+                                                    const float4* ihi, uint32_t* cnt,
+                                                    const uint32_t* off, uint64_t* out,
+                                                    uint32_t id_bits, bool narrow) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_slots) return;
+  if (kCount && t == n_slots - 1) cnt[n_slots] = 0;
+  const uint32_t w = t / cfg.max_bodies, base = w * cfg.max_bodies;
+  const uint32_t n = ws[w].n_leaves;
+  uint32_t c = 0;
+  if (t - base < n) {
+    const float4 qlo = llo[t], qhi = lhi[t];
+    const uint32_t idi = __float_as_uint(qlo.w), qm = __float_as_uint(qhi.w);
+    const bool si = qm & kLbStatic;
+    uint32_t node = qm & ~kLbStatic, o = kCount ? 0u : off[t];
+    // a walk visits each of the 2n - 1 nodes at most once
+    for (uint32_t steps = 0; !(node & kLbEnd); steps++) {
+      if (steps >= 2 * n) {
+        atomicOr(&ws[w].error, kErrLbvh);
+        break;
+      }
+      const uint32_t x = base + (node & kLbIdx);
+      if (node & kLbLeaf) {
+        const float4 lo = llo[x], hi = lhi[x];
+        const uint32_t m = __float_as_uint(hi.w);
+        if (lbOverlap(qlo, qhi, lo, hi) && !(si && (m & kLbStatic))) {
+          if (kCount) {
+            c++;
+          } else {
+            const uint32_t idj = __float_as_uint(lo.w);
+            const uint64_t key = ((uint64_t)w << (2 * id_bits)) |
+                                 ((uint64_t)min(idi, idj) << id_bits) | max(idi, idj);
+            if (narrow) reinterpret_cast<uint32_t*>(out)[o] = (uint32_t)key;
+            else out[o] = key;
+            o++;
+          }
+        }
+        node = m & ~kLbStatic;
+      } else {
+        const float4 lo = ilo[x], hi = ihi[x];
+        node = lbOverlap(qlo, qhi, lo, hi) ? __float_as_uint(lo.w) : __float_as_uint(hi.w);
+      }
+    }
+  }
+  if (kCount) cnt[t] = c;
 }
 
 // ---------------------------------------------------------------------------
@@ -1198,16 +1493,16 @@ __device__ __forceinline__ void regionVoxel(const PairRec& r, uint32_t lin, int3
 // voxels_checked sums.  Otherwise: candidates written from wi_cbase[wid] (the
 // exclusive scan of wi_ccnt) in (lane, probe) order.
 // Workgroups go to the 8 XCDs round-robin; this maps them so that runs of
-// kXcdRun consecutive logical blocks execute on one XCD (consecutive work
-// items of a pair then share that XCD's L2) while the runs still rotate over
-// the XCDs.  A bijection on [0, gridDim.x): blocks past the last whole round
-// of runs keep their index.
-constexpr uint32_t kXcds = 8, kXcdRun = 64;
-__device__ __forceinline__ uint32_t xcdBlock() {
-  const uint32_t b = blockIdx.x, full = gridDim.x / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+// `run` consecutive logical blocks execute on one XCD (consecutive work items
+// of a pair then share that XCD's L2) while the runs still rotate over the
+// XCDs.  A bijection on [0, gridDim.x) for any run >= 1: blocks past the last
+// whole round of runs keep their index.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcdBlock(uint32_t run) {
+  const uint32_t b = blockIdx.x, full = gridDim.x / (kXcds * run) * (kXcds * run);
   if (b >= full) return b;
   const uint32_t x = b % kXcds, r = b / kXcds;
-  return (r / kXcdRun) * (kXcds * kXcdRun) + x * kXcdRun + r % kXcdRun;
+  return (r / run) * (kXcds * run) + x * run + r % run;
 }
 
 template <bool kCount>
@@ -1215,9 +1510,10 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
                                                const PairRec* pr, const uint32_t* wi_pair,
                                                const uint32_t* wi_off, uint32_t total_wi,
                                                const uint32_t* wi_cbase, uint32_t* wi_ccnt,
-                                               uint32_t* wi_vchk, uint8_t* lane_n, Cand* cand) {
+                                               uint32_t* wi_vchk, uint8_t* lane_n, Cand* cand,
+                                               uint32_t xcd_run) {
   uint32_t lane = threadIdx.x & (kWave - 1);
-  uint32_t wid = xcdBlock() * (blockDim.x / kWave) + threadIdx.x / kWave;
+  uint32_t wid = xcdBlock(xcd_run) * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (wid >= total_wi) return;  // whole wave
   if (!kCount && wi_ccnt[wid] == 0) return;
   uint32_t p = wi_pair[wid];
@@ -3553,6 +3849,49 @@ void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, co
 }
 uint32_t sweep_waves() { return kSweepWaves; }
 
+bool launch_lbvh_sort(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                      uint32_t* order, uint32_t nw) {
+  const uint32_t MB = cfg.max_bodies;
+  if (MB > 8192) return false;
+  k_world_axis<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, ws, nw);
+  if (MB <= 1024) k_lbvh_sort<256, 4><<<nw, 256, 0, s>>>(cfg, B, ws, order);
+  else if (MB <= 2048) k_lbvh_sort<256, 8><<<nw, 256, 0, s>>>(cfg, B, ws, order);
+  else if (MB <= 4096) k_lbvh_sort<1024, 4><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
+  else if (MB <= 5120) k_lbvh_sort<1024, 5><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
+  else k_lbvh_sort<1024, 8><<<nw, 1024, 0, s>>>(cfg, B, ws, order);
+  return true;
+}
+void launch_lbvh_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                      uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw) {
+  k_world_axis<<<nw, slotThreads(cfg.max_bodies), 0, s>>>(cfg, B, ws, nw);
+  k_lbvh_keys<<<cdiv(n_slots, 256), 256, 0, s>>>(cfg, B, ws, keys, vals, n_slots);
+}
+void launch_lbvh_build(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                       const uint32_t* order, uint32_t n_slots, const LbvhBufs& L) {
+  const uint32_t blocks = cdiv(n_slots, 256);
+  k_lbvh_gather<<<blocks, 256, 0, s>>>(cfg, B, ws, order, n_slots, L.llo, L.lhi, L.code);
+  k_lbvh_build<<<blocks, 256, 0, s>>>(cfg, ws, n_slots, L.code, L.child, L.range, L.iparent,
+                                      L.lparent, L.flag);
+  if (cfg.max_bodies <= 8192)
+    k_lbvh_refit_wg<<<n_slots / cfg.max_bodies, slotThreads(cfg.max_bodies),
+                      (size_t)cfg.max_bodies * 4, s>>>(cfg, ws, L.llo, L.lhi, L.child, L.range,
+                                                        L.iparent, L.lparent, L.ilo, L.ihi);
+  else
+    k_lbvh_refit<<<blocks, 256, 0, s>>>(cfg, ws, n_slots, L.llo, L.lhi, L.child, L.range,
+                                        L.iparent, L.lparent, L.flag, L.ilo, L.ihi);
+}
+void launch_lbvh_pairs(hipStream_t s, const DevCfg& cfg, WorldState* ws, uint32_t n_slots,
+                       const LbvhBufs& L, uint32_t* cnt, const uint32_t* off, uint64_t* out,
+                       uint32_t id_bits, bool narrow) {
+  const uint32_t blocks = cdiv(n_slots, 256);
+  if (out)
+    k_lbvh_pairs<false><<<blocks, 256, 0, s>>>(cfg, ws, n_slots, L.llo, L.lhi, L.ilo, L.ihi, cnt,
+                                               off, out, id_bits, narrow);
+  else
+    k_lbvh_pairs<true><<<blocks, 256, 0, s>>>(cfg, ws, n_slots, L.llo, L.lhi, L.ilo, L.ihi, cnt,
+                                              off, out, id_bits, narrow);
+}
+
 // Packed (world, a, b) keys -> the canonical (world << 44 | a << 22 | b).
 __global__ void k_expand_keys(uint64_t* keys, uint32_t n, uint32_t id_bits) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3614,14 +3953,18 @@ void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_of
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                   const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
                   uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                  uint8_t* lane_n, Cand* cand, bool count) {
+                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape) {
   if (!total_wi) return;
+  const uint32_t threads = shape.threads, waves = threads / kWave;
+  const uint32_t run = shape.xcd_run ? shape.xcd_run : 1u;
   if (count)
-    k_probe<true><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi,
-                                                    wi_cbase, wi_ccnt, wi_vchk, lane_n, cand);
+    k_probe<true><<<cdiv(total_wi, waves), threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
+                                                            total_wi, wi_cbase, wi_ccnt, wi_vchk,
+                                                            lane_n, cand, run);
   else
-    k_probe<false><<<cdiv(total_wi, 4), 256, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi,
-                                                     wi_cbase, wi_ccnt, wi_vchk, lane_n, cand);
+    k_probe<false><<<cdiv(total_wi, waves), threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
+                                                             total_wi, wi_cbase, wi_ccnt, wi_vchk,
+                                                             lane_n, cand, run);
 }
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,

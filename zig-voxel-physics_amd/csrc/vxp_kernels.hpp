@@ -25,6 +25,27 @@ void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, 
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
                         uint32_t id_bits, bool narrow);
+// LBVH broad phase (per-world trees at each world's slot offsets).
+struct LbvhBufs {
+  float4 *llo, *lhi, *ilo, *ihi;  // leaves in Morton order; internal nodes
+  uint32_t* code;                 // leaf Morton codes
+  uint2 *child, *range;           // internal nodes: children, leaf range
+  uint32_t *iparent, *lparent, *flag;
+};
+// Per-world block sort of the Morton order (k_world_axis included); false
+// when max_bodies is above the largest block sort.
+bool launch_lbvh_sort(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                      uint32_t* order, uint32_t nw);
+// (world << 32 | code) keys for the device-wide sort (k_world_axis included).
+void launch_lbvh_keys(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                      uint64_t* keys, uint32_t* vals, uint32_t n_slots, uint32_t nw);
+// Leaves, internal nodes and their boxes from the Morton order.
+void launch_lbvh_build(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
+                       const uint32_t* order, uint32_t n_slots, const LbvhBufs& L);
+// Count (cnt[0..n_slots], cnt[n_slots] = 0) or write (from off) each leaf's pairs.
+void launch_lbvh_pairs(hipStream_t s, const DevCfg& cfg, WorldState* ws, uint32_t n_slots,
+                       const LbvhBufs& L, uint32_t* cnt, const uint32_t* off, uint64_t* out,
+                       uint32_t id_bits, bool narrow);
 void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits);
 void launch_expand_keys32(hipStream_t s, const uint32_t* in, uint64_t* keys, uint32_t n,
                           uint32_t id_bits);
@@ -64,10 +85,15 @@ void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t*
 // wi_pair[w] = the pair of work item w (after the scan of the chunk counts)
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
                     uint32_t* wi_pair);
+// Launch shape of the probe: threads per workgroup (64, 128 or 256; one work
+// item per wave) and the run of consecutive workgroups kept on one XCD.
+struct ProbeShape {
+  uint32_t threads, xcd_run;
+};
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                   const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
                   uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                  uint8_t* lane_n, Cand* cand, bool count);
+                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape);
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,

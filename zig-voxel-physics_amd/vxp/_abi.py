@@ -207,7 +207,8 @@ class ServerInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("pair_color_max", C.c_uint32), ("windows", C.c_uint32), ("fixed", C.c_uint32),
-                ("unit_threads", C.c_uint32)]
+                ("unit_threads", C.c_uint32), ("broad_phase", C.c_uint32),
+                ("probe_threads", C.c_uint32), ("probe_xcd_run", C.c_uint32)]
 
 
 TUNE_AUTO = 0xFFFFFFFF

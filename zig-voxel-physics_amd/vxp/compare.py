@@ -106,12 +106,15 @@ def diff_stats(sa, sb):
 
 
 def lockstep(scene, lib_a, lib_b, frames, cfg_over=None, worlds=None, check_every=1,
-             rtol=0.0, atol=0.0, log=None, dts=None):
+             rtol=0.0, atol=0.0, log=None, dts=None, tune_a=None):
     """Step both systems `frames` times; return (first_frame_with_diff, message) or (None, None).
     `dts`: the update() argument of frame f is dts[f % len(dts)] (default: the
-    scene's dt, one fixed step per frame)."""
+    scene's dt, one fixed step per frame).  `tune_a`: setTuning arguments of
+    the first system (the product)."""
     sa = from_scene(scene, lib=lib_a, cfg_over=cfg_over, worlds=worlds)
     sb = from_scene(scene, lib=lib_b, cfg_over=cfg_over, worlds=worlds)
+    if tune_a:
+        sa.setTuning(**tune_a)
     nwl = sa.n_worlds
     for f in range(frames):
         dt = dts[f % len(dts)] if dts else scene.dt

@@ -12,6 +12,9 @@ import numpy as np
 
 from . import _abi
 
+# vxp_tuning.broad_phase values (VXP_BROAD_AUTO, _SWEEP, _LBVH)
+BROAD_NAMES = ["auto", "sweep", "lbvh"]
+
 
 class VoxelCollider:
     """VoxelCollider built on the host (src/collider.zig:144-409), uploaded on
@@ -62,13 +65,19 @@ class VoxelPhysicsSystem:
         t = _abi.Tuning()
         self.lib.check(self.lib.tuning_get(self.ptr, C.byref(t)), "tuning_get")
         return {"pair_color_max": t.pair_color_max, "windows": t.windows, "fixed": bool(t.fixed),
-                "unit_threads": t.unit_threads}
+                "unit_threads": t.unit_threads, "broad_phase": BROAD_NAMES[t.broad_phase],
+                "probe_threads": t.probe_threads, "probe_xcd_run": t.probe_xcd_run}
 
-    def setTuning(self, pair_color_max=None, unit_threads=0):
-        """Fix the solver's lane-pair colour limit (None restarts the online tuner)
-        and its workgroup size (256 or 512; 0 for the default)."""
+    def setTuning(self, pair_color_max=None, unit_threads=0, broad_phase="auto", probe_threads=0,
+                  probe_xcd_run=0):
+        """Fix the solver's lane-pair colour limit (None restarts the online tuner
+        over the knobs left at their default), and force the solver's workgroup
+        size (256 or 512), the broad phase ("sweep" or "lbvh"), the probe's
+        workgroup size (64, 128 or 256) and its XCD run (a power of two); 0 /
+        "auto" leaves a knob to the tuner."""
         t = _abi.Tuning(pair_color_max=_abi.TUNE_AUTO if pair_color_max is None else pair_color_max,
-                        unit_threads=unit_threads)
+                        unit_threads=unit_threads, broad_phase=BROAD_NAMES.index(broad_phase),
+                        probe_threads=probe_threads, probe_xcd_run=probe_xcd_run)
         self.lib.check(self.lib.tuning_set(self.ptr, C.byref(t)), "tuning_set")
 
     # ---- colliders / bodies (src/system.zig:294-433) -------------------------
