@@ -91,7 +91,11 @@ typedef struct vxp_config {
   /* NarrowPhaseConfig */
   uint8_t use_brick_sat;       /* 1 */
   uint8_t use_brick_pair_cache;/* 1 */
-  uint8_t pad0[2];
+  /* PhysicsConfig.enable_threading (0): the thread-pool narrow phase of
+     src/system.zig:618-638 (first 256 active bodies only, read-only cache
+     for 8+ pairs; see thread_count) */
+  uint8_t enable_threading;
+  uint8_t pad0;
   uint32_t brick_pair_cache_size; /* 1024 */
   /* deviation knobs */
   uint32_t pair_cap;           /* 0 = unlimited; 512 = the reference's
@@ -99,6 +103,9 @@ typedef struct vxp_config {
   uint32_t solve_order;        /* VXP_SOLVE_ORDER_* */
   uint32_t cpu_threads;        /* CPU restatement only: narrow-phase workers */
   uint32_t pair_order;         /* VXP_PAIR_ORDER_* */
+  uint32_t thread_count;       /* PhysicsConfig.thread_count (0 = the host's CPU
+                                  count); with enable_threading only 1 versus 2+
+                                  changes results */
 } vxp_config;
 
 /* Per-step statistics: PhysicsStats (src/system.zig:91-155) counters, for

@@ -31,6 +31,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../include/vxp.h"
@@ -1435,8 +1436,8 @@ struct World {
     DetectResult det;
   };
 
-  void prepPair(PairWork& w) {
-    stats.narrow_phase_tested++;
+  void prepPair(PairWork& w, bool count_tested = true) {
+    if (count_tested) stats.narrow_phase_tested++;
     const Collider* ca = coll(*w.a);
     const Collider* cb = coll(*w.b);
     if (!ca || !cb) return;
@@ -1523,6 +1524,10 @@ struct World {
       }
     }
     stats.pairs_skipped_sleeping = skipped;
+    if (cfg.enable_threading) {
+      threadPoolNarrowPhase(work);
+      return;
+    }
     if (cfg.cpu_threads <= 1) {
       // literal sequential processPair order
       for (PairWork& w : work) {
@@ -1593,6 +1598,47 @@ struct World {
         if (cache.e[e].valid && cache.e[e].key.eql(w.key)) cache.e[e].value = toCacheValue(w.det);
       }
       finishDetect(w);
+    }
+  }
+
+  // The narrow phase of a system with a thread pool (src/system.zig:618-638):
+  // it sees only the first 256 active bodies in slot order; with fewer than
+  // MIN_PARALLEL_PAIRS (8) pairs or one thread, processPair runs on the pairs
+  // whose bodies are both listed (src/narrow_phase.zig:1546-1561); otherwise
+  // each pair goes through processPairIntoBuffer (src/narrow_phase.zig:1187-1290):
+  // pairs_tested counts every pair before the body lookup, the brick-pair
+  // cache is read but never written, and the per-thread buffers (256
+  // manifolds, src/narrow_phase.zig:35,1105-1110) merge in pair order.  No
+  // buffer can fill: at most 512 pairs reach the narrow phase and two or more
+  // threads split them into ranges of ceil(n / threads) <= 256 pairs.  Only
+  // threads <= 1 versus >= 2 changes results; thread_count 0 (the host's CPU
+  // count) counts as >= 2.
+  void threadPoolNarrowPhase(std::vector<PairWork>& work) {
+    std::unordered_set<const Body*> listed;
+    for (Slot& sl : slots)
+      if (sl.active && listed.size() < 256) listed.insert(&sl.body);
+    const uint32_t threads = cfg.thread_count ? cfg.thread_count : 2u;
+    const bool parallel = work.size() >= 8 && threads > 1;
+    for (PairWork& w : work) {
+      if (parallel) stats.narrow_phase_tested++;
+      if (!listed.count(w.a) || !listed.count(w.b)) continue;
+      prepPair(w, !parallel);
+      if (!w.sat_pass) continue;
+      if (w.use_cache) {
+        long e = cache.get(w.key);
+        if (e >= 0) {
+          w.hit_value = cache.e[e].value;
+          finishHit(w);
+          continue;
+        }
+        stats.cache_misses++;
+        w.det = detectVoxelCollisions(*w.a, *coll(*w.a), *w.b, *coll(*w.b));
+        if (!parallel) cache.put(w.key, toCacheValue(w.det));
+        finishDetect(w);
+      } else {
+        w.det = detectVoxelCollisions(*w.a, *coll(*w.a), *w.b, *coll(*w.b));
+        finishDetect(w);
+      }
     }
   }
 
@@ -1901,7 +1947,7 @@ static bool configChangeAllowed(const vxp_config& a, const vxp_config& b) {
     c.hertz = c.damping_ratio = c.position_correction_rate = c.max_position_correction = 0.0f;
     c.slop = c.warm_start_scale = 0.0f;
     c.disable_rotation = c.warm_starting = c.enable_friction = c.enable_restitution = 0;
-    c.pad0[0] = c.pad0[1] = 0;
+    c.pad0 = 0;
     c.cpu_threads = 0;
   };
   clear(x);

@@ -369,6 +369,8 @@ DevCfg devCfg(vxp_system* s) {
   d.id_cap = s->id_cap;
   d.cache_cap = c.brick_pair_cache_size ? c.brick_pair_cache_size : 1024;
   d.enable_sleeping = c.enable_sleeping;
+  d.threaded = c.enable_threading ? 1 : 0;
+  d.thread_multi = c.thread_count != 1 ? 1 : 0;
   d.warm_starting = c.warm_starting;
   d.enable_friction = c.enable_friction;
   d.enable_restitution = c.enable_restitution;
@@ -841,22 +843,29 @@ static int stepAllImpl(vxp_system* s) {
   // the filtered-pair cap (src/system.zig:594-616), counted in the narrow
   // phase's order over the pairs that are not both asleep
   const uint8_t* drop = nullptr;
-  if (s->cfg.pair_cap && npairs) {
+  const bool threaded = s->cfg.enable_threading;
+  if ((s->cfg.pair_cap || threaded) && npairs) {
     HIP_TRY(s->d_awake.ensure((size_t)(npairs + 1) * 4));
     HIP_TRY(s->d_arank.ensure((size_t)(npairs + 1) * 4));
-    HIP_TRY(s->d_drop.ensure((size_t)npairs));
     launch_pair_awake(st, cfg, B, keys, perm, npairs, s->d_awake.as<uint32_t>());
     size_t cb = exclusive_scan_u32_bytes(npairs + 1);
     HIP_TRY(s->d_sort_tmp.ensure(cb));
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_awake.as<uint32_t>(),
                                s->d_arank.as<uint32_t>(), npairs + 1, st));
-    launch_pair_drop(st, keys, perm, npairs, s->d_awake.as<uint32_t>(), s->d_arank.as<uint32_t>(),
-                     s->cfg.pair_cap, s->d_drop.as<uint8_t>());
-    drop = s->d_drop.as<uint8_t>();
+    if (s->cfg.pair_cap) {
+      HIP_TRY(s->d_drop.ensure((size_t)npairs));
+      launch_pair_drop(st, keys, perm, npairs, s->d_awake.as<uint32_t>(),
+                       s->d_arank.as<uint32_t>(), s->cfg.pair_cap, s->d_drop.as<uint8_t>());
+      drop = s->d_drop.as<uint8_t>();
+    }
   }
+  if (threaded)  // the thread-pool narrow phase's body list and branch, per world
+    launch_thread_mode(st, cfg, B, ws, keys, npairs, npairs ? s->d_arank.as<uint32_t>() : nullptr,
+                       s->cfg.pair_cap, nw);
   launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, drop,
                       s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(),
-                      s->d_wlist.as<uint32_t>(), s->d_step_ctr.as<uint32_t>());
+                      s->d_wlist.as<uint32_t>(), s->d_step_ctr.as<uint32_t>(),
+                      s->d_counts.as<uint32_t>());
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
   pe = profBegin(s, VXP_K_CACHE_REPLAY);
   // cache-eligible flags in the narrow phase's order, and their positions
@@ -880,11 +889,10 @@ static int stepAllImpl(vxp_system* s) {
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, cflag_r, cidx_r, npairs + 1, st));
   }
   launch_cache_replay(st, cfg, ws, stats, pr, s->d_cache_keys.as<CacheKeyRec>(), cflag_r, cidx_r,
-                      order, s->d_clist.as<uint32_t>(), npairs, nw);
+                      order, s->d_clist.as<uint32_t>(), npairs, nw, s->d_counts.as<uint32_t>());
   profEnd(s, VXP_K_CACHE_REPLAY, pe);
   uint32_t total_wi = 0;
   if (npairs) {
-    launch_chunk_counts(st, pr, npairs, s->d_counts.as<uint32_t>());
     size_t sb = exclusive_scan_u32_bytes(npairs);
     HIP_TRY(s->d_sort_tmp.ensure(sb));
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, sb, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
@@ -1638,7 +1646,7 @@ static bool configChangeAllowed(const vxp_config& a, const vxp_config& b) {
     c.hertz = c.damping_ratio = c.position_correction_rate = c.max_position_correction = 0.0f;
     c.slop = c.warm_start_scale = 0.0f;
     c.disable_rotation = c.warm_starting = c.enable_friction = c.enable_restitution = 0;
-    c.pad0[0] = c.pad0[1] = 0;
+    c.pad0 = 0;
     c.cpu_threads = 0;
   };
   clear(x);

@@ -58,7 +58,10 @@ struct DevCfg {
   uint32_t cache_cap;
   uint8_t enable_sleeping, warm_starting, enable_friction, enable_restitution;
   uint8_t disable_rotation, use_sat, use_cache, enable_islands;
-  uint8_t store_impulses, pad[3];
+  uint8_t store_impulses;
+  uint8_t threaded;      // PhysicsConfig.enable_threading: the thread-pool narrow phase
+  uint8_t thread_multi;  // ... with 2+ threads (thread_count 0 or >= 2)
+  uint8_t pad;
   uint32_t pair_color_max;  // colours of at most this many manifolds run on lane pairs
   uint32_t unit_min;        // solve units hold at least this many manifolds (when split)
 };
@@ -98,6 +101,9 @@ struct WorldState {
   // doubled fat-box centres, q = (c - c_lo) * c_scale clamped to [0, 1023]
   uint32_t n_leaves;
   float c_lo[3], c_scale[3];
+  // thread-pool narrow phase (enable_threading): slots below thr_cut hold the
+  // first 256 active bodies; thr_par: the parallel branch runs this step
+  uint32_t thr_cut, thr_par;
   // solver (written by its prep each step)
   uint32_t n_units, nsolve;           // solve units, solved manifolds
   uint32_t max_unit_m;                // manifolds of the largest unit

@@ -963,6 +963,17 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
     pr[p] = r;
     return cnt;
   }
+  if (cfg.threaded) {
+    // the thread-pool narrow phase sees the first 256 active bodies only; its
+    // parallel branch counts every pair as tested before looking the bodies
+    // up (src/system.zig:618-629, src/narrow_phase.zig:1188-1199)
+    const WorldState& W = ws[w];
+    if ((uint32_t)sa >= W.thr_cut || (uint32_t)sb >= W.thr_cut) {
+      cnt.tested = W.thr_par;
+      pr[p] = r;
+      return cnt;
+    }
+  }
   cnt.tested = 1;
   int32_t ca = B.collider[ga], cb = B.collider[gb];
   if (ca < 0 || cb < 0) {
@@ -1035,7 +1046,7 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
                                const DevManifold* prev, const uint8_t* drop, uint32_t* cflag,
                                uint32_t* produce, RStat* rstat, uint32_t* wlist,
-                               uint32_t* step_ctr) {
+                               uint32_t* step_ctr, uint32_t* counts) {
   __shared__ uint32_t s_n, s_base;
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   PairCounts c{0, 0, 0};
@@ -1049,6 +1060,9 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
     w = keyWorld(keys[p]);
     const uint32_t f = pr[p].flag;
     cflag[p] = f == PF_CACHE;
+    // work items of the voxel probe: the region's chunks when the pair runs the
+    // detection (a cache pair's count is set by the replay when it misses)
+    counts[p] = f == PF_DETECT ? pr[p].nchunks : 0u;
     if (p == n_pairs - 1) cflag[n_pairs] = 0;
     produce[p] = 0;
     rstat[p] = RStat{0u, 0u, 0u, 0u, 0u, 0u};
@@ -1144,6 +1158,7 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
                                                                 CacheKeyRec* keys,
                                                                 const uint32_t* cidx,
                                                                 const uint32_t* clist,
+                                                                uint32_t* counts,
                                                                 uint32_t n_worlds) {
   extern __shared__ CacheKeyRec E[];
   const uint32_t w = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1);
@@ -1163,6 +1178,9 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
   copyWords(reinterpret_cast<uint2*>(E), reinterpret_cast<const uint2*>(G), nq);
   __syncthreads();
   uint32_t hits = 0, misses = 0;
+  // the thread-pool narrow phase's parallel branch only reads the cache: a
+  // miss stores nothing (src/narrow_phase.zig:1228-1260)
+  const bool store = !(cfg.threaded && ws[w].thr_par);
   if (tid < kWave) {
     const bool probe = lane < maxp;
     // pair data of a batch of 64 pairs; the next batch's is loaded (and each
@@ -1223,7 +1241,7 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
           if (rflag == PF_HIT) {
             E[raux].last_used = frame;
             hits++;
-          } else {
+          } else if (store) {
             CacheKeyRec& e = E[raux];
             e.a = a;
             e.b = b;
@@ -1233,9 +1251,11 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
             e.owner_frame = frame;
             e.owner_pair = rdlane(p, (int)j);
             misses++;
+          } else {
+            misses++;
           }
         }
-        if (more && probe && nidx == raux) {
+        if (more && probe && nidx == raux && (store || rflag == PF_HIT)) {
           nxt.lu = frame;
           if (rflag == PF_MISS) {
             nxt.a = a;
@@ -1254,6 +1274,7 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
       if (inb) {
         pr[p].flag = my_flag;
         pr[p].aux = my_aux;
+        if (my_flag == PF_MISS) counts[p] = pr[p].nchunks;  // a miss runs the detection
       }
     }
   }
@@ -1265,12 +1286,6 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
   }
 }
 
-__global__ void k_chunk_counts(const PairRec* pr, uint32_t n_pairs, uint32_t* counts) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
-  uint32_t f = pr[p].flag;
-  counts[p] = (f == PF_DETECT || f == PF_MISS) ? pr[p].nchunks : 0u;
-}
 
 // The pair of every work item: wi_pair[wi_off[p] + c] = p for c < counts[p].
 __global__ void k_wi_fill(const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
@@ -3764,6 +3779,52 @@ __global__ void k_pair_drop(const uint64_t* keys, const uint32_t* perm, uint32_t
   drop[p] = awake[t] && rank[t] - rank[lo] >= cap;
 }
 
+// Per world, the choices of the thread-pool narrow phase (src/system.zig:618-638,
+// src/narrow_phase.zig:1546): thr_cut = one past the slot of the 256th active
+// body (the narrow phase sees the slots below it; all of them when fewer are
+// active), thr_par = the parallel branch: at least 8 pairs reach the narrow
+// phase (not both asleep, within the cap) and the pool has 2+ threads.  One
+// wave per world; awake/rank: the not-both-asleep flags and their exclusive
+// scan (n + 1 entries) in the narrow phase's order.
+__global__ void k_thread_mode(DevCfg cfg, DevBodies B, WorldState* ws, const uint64_t* keys,
+                              uint32_t n, const uint32_t* rank, uint32_t cap) {
+  const uint32_t w = blockIdx.x, lane = threadIdx.x, MB = cfg.max_bodies;
+  uint32_t run = 0, cut = MB;
+  for (uint32_t base = 0; base < MB; base += kWave) {
+    const uint32_t sl = base + lane;
+    const bool a = sl < MB && B.active[(size_t)w * MB + sl];
+    const uint64_t m = __ballot(a);
+    const uint32_t c = (uint32_t)__popcll(m);
+    if (run + c >= 256u) {
+      const uint32_t below = (uint32_t)__popcll(m & laneMaskLt());
+      const uint64_t hit = __ballot(a && below == 255u - run);
+      cut = base + (uint32_t)(__ffsll((long long)hit) - 1) + 1u;
+      break;
+    }
+    run += c;
+  }
+  uint32_t np = 0;
+  if (n) {
+    // the world's pairs: the same index range in either pair order
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (keyWorld(keys[mid]) < w) lo = mid + 1; else hi = mid;
+    }
+    uint32_t e = lo, top = n;
+    while (e < top) {
+      const uint32_t mid = (e + top) >> 1;
+      if (keyWorld(keys[mid]) <= w) e = mid + 1; else top = mid;
+    }
+    np = rank[e] - rank[lo];
+    if (cap) np = min(np, cap);
+  }
+  if (lane == 0) {
+    ws[w].thr_cut = cut;
+    ws[w].thr_par = np >= 8u && cfg.thread_multi;
+  }
+}
+
 __global__ void k_gather_flags(const uint32_t* in, const uint32_t* idx, uint32_t n, uint32_t* out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[idx[i]];
@@ -3811,6 +3872,11 @@ void launch_pair_awake(hipStream_t s, const DevCfg& cfg, const DevBodies& B, con
 void launch_pair_drop(hipStream_t s, const uint64_t* keys, const uint32_t* perm, uint32_t n,
                       const uint32_t* awake, const uint32_t* rank, uint32_t cap, uint8_t* drop) {
   if (n) k_pair_drop<<<cdiv(n, 256), 256, 0, s>>>(keys, perm, n, awake, rank, cap, drop);
+}
+void launch_thread_mode(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                        const uint64_t* keys, uint32_t n, const uint32_t* rank, uint32_t cap,
+                        uint32_t nw) {
+  k_thread_mode<<<nw, kWave, 0, s>>>(cfg, B, ws, keys, n, rank, cap);
 }
 void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t n,
                          uint32_t* out) {
@@ -3929,22 +3995,19 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
-                         uint32_t* wlist, uint32_t* step_ctr) {
+                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts) {
   if (n)
     k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag,
-                                                produce, rstat, wlist, step_ctr);
+                                                produce, rstat, wlist, step_ctr, counts);
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
                          const uint32_t* cidx, const uint32_t* order, uint32_t* clist,
-                         uint32_t n_pairs, uint32_t nw) {
+                         uint32_t n_pairs, uint32_t nw, uint32_t* counts) {
   if (n_pairs)
     k_cache_scatter<<<cdiv(n_pairs, 256), 256, 0, s>>>(cflag, cidx, order, n_pairs, clist);
-  k_cache_replay<<<nw, kCacheThreads, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(cfg, ws, st, pr, keys, cidx,
-                                                                      clist, nw);
-}
-void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts) {
-  if (n) k_chunk_counts<<<cdiv(n, 256), 256, 0, s>>>(pr, n, counts);
+  k_cache_replay<<<nw, kCacheThreads, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(
+      cfg, ws, st, pr, keys, cidx, clist, counts, nw);
 }
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
                     uint32_t* wi_pair) {

@@ -53,11 +53,12 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
-                         uint32_t* wlist, uint32_t* step_ctr);
+                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts);
+// counts[p] (the probe's work items of pair p) is set for the misses.
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
                          const uint32_t* cidx, const uint32_t* order, uint32_t* clist,
-                         uint32_t n_pairs, uint32_t nw);
+                         uint32_t n_pairs, uint32_t nw, uint32_t* counts);
 // Broad-phase trees (pair order VXP_PAIR_ORDER_TREE): per world, re-insert the
 // leaves of the dynamic tree whose fat box moved, in slot order.
 // host[1..2] = *a, *b (0 when null), then host[0] = seq (host-visible memory).
@@ -78,10 +79,15 @@ void launch_pair_awake(hipStream_t s, const DevCfg& cfg, const DevBodies& B, con
 void launch_pair_drop(hipStream_t s, const uint64_t* keys, const uint32_t* perm, uint32_t n,
                       const uint32_t* awake, const uint32_t* rank, uint32_t cap, uint8_t* drop);
 // out[i] = in[idx[i]] for i < n, out[n] = 0; inv[perm[p]] = p
+// enable_threading: per world, the slots the thread-pool narrow phase sees
+// and whether its parallel branch runs (rank: exclusive scan of the
+// not-both-asleep flags, n + 1 entries; cap 0 = none).
+void launch_thread_mode(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
+                        const uint64_t* keys, uint32_t n, const uint32_t* rank, uint32_t cap,
+                        uint32_t nw);
 void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t n,
                          uint32_t* out);
 void launch_invert(hipStream_t s, const uint32_t* perm, uint32_t n, uint32_t* inv);
-void launch_chunk_counts(hipStream_t s, const PairRec* pr, uint32_t n, uint32_t* counts);
 // wi_pair[w] = the pair of work item w (after the scan of the chunk counts)
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
                     uint32_t* wi_pair);

@@ -56,12 +56,14 @@ class Config(C.Structure):
         ("enable_restitution", C.c_uint8),
         ("use_brick_sat", C.c_uint8),
         ("use_brick_pair_cache", C.c_uint8),
-        ("pad0", C.c_uint8 * 2),
+        ("enable_threading", C.c_uint8),
+        ("pad0", C.c_uint8),
         ("brick_pair_cache_size", C.c_uint32),
         ("pair_cap", C.c_uint32),
         ("solve_order", C.c_uint32),
         ("cpu_threads", C.c_uint32),
         ("pair_order", C.c_uint32),
+        ("thread_count", C.c_uint32),
     ]
 
 
@@ -218,7 +220,7 @@ OP_APPLY_FORCE, OP_APPLY_FORCE_AT_POINT, OP_APPLY_IMPULSE, OP_APPLY_IMPULSE_AT_P
     OP_APPLY_TORQUE, OP_APPLY_ANGULAR_IMPULSE, OP_SET_LINEAR_VELOCITY, OP_SET_ANGULAR_VELOCITY, \
     OP_SET_POSITION, OP_SET_ROTATION, OP_WAKE_UP, OP_SLEEP = range(12)
 
-assert C.sizeof(Config) == 104, C.sizeof(Config)
+assert C.sizeof(Config) == 108, C.sizeof(Config)
 assert C.sizeof(Contact) == 88, C.sizeof(Contact)
 assert C.sizeof(Manifold) == 376, C.sizeof(Manifold)
 assert C.sizeof(BodyState) == 64, C.sizeof(BodyState)

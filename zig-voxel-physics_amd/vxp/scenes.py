@@ -566,3 +566,55 @@ def build_system(lib, scene, cfg_over=None, worlds=None):
             hs.append(h)
         handles.append(hs)
     return sysp, handles
+
+
+# Cubes that rest on the ground from the first frame, so that every
+# cube-ground pair reaches the narrow phase in every frame (the thread-pool
+# narrow phase of src/system.zig:618-638 takes its branch on the pair count).
+def _stored_bricks(brick_pos):
+    """Brick positions a collider keeps: its brick map is keyed by
+    hashBrickPos (src/collider.zig:187-192), so of the positions that share a
+    hash only the first one stores a brick."""
+    seen, kept = set(), set()
+    m = (1 << 64) - 1
+    for x, y, z in brick_pos.tolist():
+        h = (x & m) ^ (((y & m) * 0x517cc1b727220a95) & m) ^ (((z & m) * 0x9e3779b97f4a7c15) & m)
+        if h not in seen:
+            seen.add(h)
+            kept.add((x, y, z))
+    return kept
+
+
+def resting_cubes(n_cubes, spacing=1.0, y=0.33, frames=60, name="resting"):
+    """`n_cubes` README cubes on a square grid (row-major from the -x/-z
+    corner), each over ground that the collider stores, with their lower faces
+    0.03 m above it; the ground is body 1 (slot 0).  A cube's fat box overlaps
+    the ground's from the start."""
+    vs = 0.2
+    cells = 64
+    while True:
+        vox, org = hand_ground(cells, cells)
+        bp, _ = grid_to_bricks(vox, org)
+        kept = _stored_bricks(bp)
+        spots = []
+        half = cells * vs / 2 - 1.0
+        k = int(half / spacing)
+        for i in range(-k, k + 1):
+            for j in range(-k, k + 1):
+                x, z = i * spacing, j * spacing
+                # the cube's footprint (0.6 m) in voxel columns, all in kept bricks
+                ok = all((int(math.floor((x + dx) / vs)) // 8, 0, int(math.floor((z + dz) / vs)) // 8)
+                         in kept for dx in (-0.3, 0.29) for dz in (-0.3, 0.29))
+                if ok:
+                    spots.append((x, z))
+        if len(spots) >= n_cubes:
+            break
+        cells *= 2
+    sc = Scene(name, scene_config("c0", 1 / 60, 4, 8, n_cubes + 1), 1 / 60, frames)
+    g = _add_collider(sc, vox, org, vs, f"ground{cells}")
+    cube = _add_collider(sc, *readme_cube(), vs, "cube3")
+    bodies = [BodySpec(_abi.STATIC, (0.0, -vs * 0.5, 0.0), (0.0, 0.0, 0.0, 1.0), g)]
+    for x, z in spots[:n_cubes]:
+        bodies.append(BodySpec(_abi.DYNAMIC, (x, y, z), (0.0, 0.0, 0.0, 1.0), cube, 1.0))
+    sc.worlds.append(bodies)
+    return sc
