@@ -449,9 +449,10 @@ int vxp_server_info_get(vxp_server* srv, vxp_server_info* out);
 /* Launch-shape tuning.  Every choice computes bit-identical results; only the
  * speed differs.  An online tuner times the candidates of one knob at a time
  * on live frames (3 frames per candidate, timed over the phase the knob
- * shapes: solve, broad phase or voxel probe), keeps the fastest, and goes on
- * to the next knob; a round over every knob that is not forced starts at
- * creation and again 600 frames after the last one ended.
+ * shapes: solve, broad phase or voxel probe), switches to the fastest when it
+ * beats the value in use by 10 %, and goes on to the next knob; a round over
+ * every knob that is not forced starts at creation and again 120 frames after
+ * the last one ended (the best shapes change as piles form).
  * pair_color_max: colours of at most this many manifolds are solved
  * on lane pairs (one body per lane) instead of one lane per manifold.
  * unit_threads: threads per solve-unit workgroup, 256 or 512 (default: 256

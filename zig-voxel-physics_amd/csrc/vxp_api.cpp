@@ -527,6 +527,8 @@ static vxp_system::Phase knobPhase(uint32_t k) {
                                           : vxp_system::kPhSolve;
 }
 
+constexpr uint32_t kTuneRoundGap = 120;  // frames between tuning rounds
+
 // At the start of a step: folds in the previous tuned step's time (its events
 // completed before this step begins), ends a window when its samples are in
 // (keeping the fastest candidate), starts the next window of the round when
@@ -545,15 +547,25 @@ int tuneStep(vxp_system* s) {
       t.n[t.pending_idx]++;
     }
     t.pending = false;
-    if (t.window_left == 0) {  // window over: keep the fastest candidate
-      double best = 1e300;
-      for (size_t i = 0; i < t.cand[t.knob].size(); i++)
-        if (t.n[i] && t.ms[i] / t.n[i] < best) {
-          best = t.ms[i] / t.n[i];
-          t.pick[t.knob] = t.cand[t.knob][i];
+    if (t.window_left == 0) {
+      // window over: the fastest candidate replaces the value in use only if
+      // it is at least 10 % faster (timings of a few live frames are noisy)
+      const auto& cand = t.cand[t.knob];
+      double best = 1e300, inc = 1e300;
+      size_t bi = cand.size();
+      for (size_t i = 0; i < cand.size(); i++) {
+        if (!t.n[i]) continue;
+        const double avg = t.ms[i] / t.n[i];
+        if (avg < best) {
+          best = avg;
+          bi = i;
         }
+        if (cand[i] == t.pick[t.knob]) inc = avg;
+      }
+      if (bi < cand.size() && best < 0.9 * inc) t.pick[t.knob] = cand[bi];
       t.windows++;
       // the next knob of the round at once; after the last, a new round later
+      // (the best shapes change as a scene's piles form and its islands grow)
       uint32_t k = t.knob + 1;
       while (k < K && t.forced[k]) k++;
       if (k < K) {
@@ -561,7 +573,7 @@ int tuneStep(vxp_system* s) {
         t.next_window = t.frame;
       } else {
         t.knob = 0;
-        t.next_window = t.frame + 600;
+        t.next_window = t.frame + kTuneRoundGap;
       }
     }
   }
@@ -657,6 +669,8 @@ static int stepAllImpl(vxp_system* s) {
   uint32_t id_bits = 1;  // body ids are below id_cap
   while ((1u << id_bits) < s->id_cap) id_bits++;
   uint32_t npairs = 0;
+  uint32_t max_world_pairs = 0xFFFFFFFFu;  // the most pairs one world emitted (untraced orders)
+  HIP_TRY(s->d_maxw.ensure(4));
   RC(tuneMark(s, vxp_system::kPhPairs, true));
   if (tree) {
     // ---- pair set in tree order: re-insert the moved leaves, then every
@@ -725,7 +739,13 @@ static int stepAllImpl(vxp_system* s) {
       HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wcnt.as<uint32_t>(),
                                  s->d_woff.as<uint32_t>(), ns + 1, st));
     }
-    RC(readSizes(s, {s->d_woff.as<uint32_t>() + ns}, &npairs));
+    launch_world_pairs(st, ws, nw, s->d_woff.as<uint32_t>(), s->MB, s->d_maxw.as<uint32_t>());
+    {
+      uint32_t sz[2];
+      RC(readSizes(s, {s->d_woff.as<uint32_t>() + ns, s->d_maxw.as<uint32_t>()}, sz));
+      npairs = sz[0];
+      max_world_pairs = sz[1];
+    }
     HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
     if (npairs)
       launch_lbvh_pairs(st, cfg, ws, ns, L, nullptr, s->d_woff.as<uint32_t>(),
@@ -777,7 +797,13 @@ static int stepAllImpl(vxp_system* s) {
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_swc.as<uint32_t>(),
                                s->d_swo.as<uint32_t>(), swv + 1, st));
   }
-  RC(readSizes(s, {s->d_swo.as<uint32_t>() + swv}, &npairs));
+  launch_world_pairs(st, ws, nw, nullptr, s->MB, s->d_maxw.as<uint32_t>());
+  {
+    uint32_t sz[2];
+    RC(readSizes(s, {s->d_swo.as<uint32_t>() + swv, s->d_maxw.as<uint32_t>()}, sz));
+    npairs = sz[0];
+    max_world_pairs = sz[1];
+  }
   HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
   if (npairs)
     launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
@@ -790,6 +816,7 @@ static int stepAllImpl(vxp_system* s) {
   HIP_TRY(s->d_pairs.ensure((size_t)std::max(npairs, 1u) * 8));
   const uint32_t* perm = nullptr;  // key position -> tree position
   pe = profBegin(s, VXP_K_PAIRS);
+  bool seg = false;
   if (npairs) {
     // packed pair keys (world, a, b) are zero above bit 2 * id_bits + log2(worlds)
     const uint32_t kbits = 2 * id_bits + wbits;
@@ -805,6 +832,9 @@ static int stepAllImpl(vxp_system* s) {
                                  s->d_pairs.as<uint64_t>(), s->d_t_pos.as<uint32_t>(),
                                  s->d_perm.as<uint32_t>(), npairs, kbits, st));
       perm = s->d_perm.as<uint32_t>();
+    } else if (launch_segment_sort(st, s->d_pairs_raw.p, kbits <= 32, npairs, nw, id_bits,
+                                   max_world_pairs, s->d_pairs.as<uint64_t>())) {
+      seg = true;  // every world's keys sorted in one workgroup, written canonical
     } else if (kbits <= 32) {
       // 32-bit keys: sorted from the first half of the raw buffer into its second
       uint32_t* raw = s->d_pairs_raw.as<uint32_t>();
@@ -818,7 +848,8 @@ static int stepAllImpl(vxp_system* s) {
       HIP_TRY(sort_keys_u64(s->d_sort_tmp.p, kb, s->d_pairs_raw.as<uint64_t>(),
                             s->d_pairs.as<uint64_t>(), npairs, kbits, st));
     }
-    if (tree || kbits > 32) launch_expand_keys(st, s->d_pairs.as<uint64_t>(), npairs, id_bits);
+    if (tree || (!seg && kbits > 32))
+      launch_expand_keys(st, s->d_pairs.as<uint64_t>(), npairs, id_bits);
   }
   profEnd(s, VXP_K_PAIRS, pe);
   RC(tuneMark(s, vxp_system::kPhPairs, false));
@@ -1384,7 +1415,7 @@ int vxp_system_destroy(vxp_system* s) {
                     &s->d_t_pos, &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake,
                     &s->d_arank, &s->d_drop, &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair,
                     &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code, &s->d_lb_child, &s->d_lb_range,
-                    &s->d_lb_iparent, &s->d_lb_lparent, &s->d_lb_flag};
+                    &s->d_lb_iparent, &s->d_lb_lparent, &s->d_lb_flag, &s->d_maxw};
   for (DevBuf* b : bufs) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);

@@ -104,6 +104,7 @@ struct WorldState {
   // thread-pool narrow phase (enable_threading): slots below thr_cut hold the
   // first 256 active bodies; thr_par: the parallel branch runs this step
   uint32_t thr_cut, thr_par;
+  uint32_t np_emit;  // pairs the broad phase emits for this world this step
   // solver (written by its prep each step)
   uint32_t n_units, nsolve;           // solve units, solved manifolds
   uint32_t max_unit_m;                // manifolds of the largest unit

@@ -143,6 +143,7 @@ struct vxp_system {
       d_produce, d_pidx, d_staging, d_cstage, d_total, d_rstat, d_clist, d_cflag, d_cidx, d_wlist, d_step_ctr;
   DevBuf d_lb_ilo, d_lb_ihi, d_lb_code, d_lb_child, d_lb_range, d_lb_iparent, d_lb_lparent,
       d_lb_flag;  // LBVH broad phase
+  DevBuf d_maxw;  // the most pairs one world emitted this step
   DevBuf d_ml[2];
   int cur = 0;
   DevBuf d_isl_parent, d_isl_end, d_isl_size, d_ref_masks, d_ref_colors;

@@ -397,6 +397,7 @@ __global__ void k_world_axis(DevCfg cfg, DevBodies B, WorldState* ws, uint32_t n
     if (r[1] < r[ax]) ax = 1;
     if (r[2] < r[ax]) ax = 2;
     ws[w].axis = ax;
+    ws[w].np_emit = 0;
     ws[w].n_leaves = n_act;
     for (int k = 0; k < 3; k++) {
       // a spread that is zero or not finite puts every centre in cell 0
@@ -501,7 +502,7 @@ __global__ void k_sweep_count(DevCfg cfg, const WorldState* ws, const float4* sl
 constexpr uint32_t kSweepWaves = 8192;
 
 template <bool kCount>
-__global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const float4* slo,
+__global__ __launch_bounds__(256) void k_sweep_pairs(WorldState* ws, const float4* slo,
                                                      const float4* shi, uint32_t n_slots,
                                                      const uint32_t* send, const uint32_t* woff_i,
                                                      uint32_t* wcount, const uint32_t* woff_w,
@@ -511,7 +512,7 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
   const uint32_t total = woff_i[n_slots];
   const uint32_t q = (total + kSweepWaves - 1) / kSweepWaves;
   const uint32_t g0 = min(total, gw * q), g1 = min(total, g0 + q);
-  uint32_t n = 0;
+  uint32_t n = 0, wn = 0, wcur = kNone;  // wn: pairs of world wcur not yet added to it
   uint32_t o = kCount ? 0 : woff_w[gw];
   if (g0 < g1) {
     uint32_t i = upperBound(woff_i, n_slots, g0) - 1, cur = kNone, end = 0, w = 0, idi = 0;
@@ -545,6 +546,12 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
       const uint64_t em = __ballot(emit);
       if (kCount) {
         n += (uint32_t)__popcll(em);
+        if (w != wcur) {
+          if (wn && lane == 0) atomicAdd(&ws[wcur].np_emit, wn);
+          wcur = w;
+          wn = 0;
+        }
+        wn += (uint32_t)__popcll(em);
       } else {
         // packed key (world, a, b) with ids of id_bits bits, expanded after sorting
         if (emit) {
@@ -563,6 +570,7 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(const WorldState* ws, const
   if (kCount && lane == 0) {
     wcount[gw] = n;
     if (gw == kSweepWaves - 1) wcount[kSweepWaves] = 0;
+    if (wn) atomicAdd(&ws[wcur].np_emit, wn);
   }
 }
 
@@ -3958,6 +3966,63 @@ void launch_lbvh_pairs(hipStream_t s, const DevCfg& cfg, WorldState* ws, uint32_
                                               off, out, id_bits, narrow);
 }
 
+// Pairs each world emitted (from the scan of per-leaf counts, LBVH) and the
+// largest world's count: one block.
+__global__ void k_world_pairs(WorldState* ws, uint32_t nw, const uint32_t* leaf_off, uint32_t MB,
+                              uint32_t* max_out) {
+  __shared__ uint32_t s_m;
+  if (threadIdx.x == 0) s_m = 0;
+  __syncthreads();
+  uint32_t m = 0;
+  for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
+    if (leaf_off) ws[w].np_emit = leaf_off[(size_t)(w + 1) * MB] - leaf_off[(size_t)w * MB];
+    m = max(m, ws[w].np_emit);
+  }
+  atomicMax(&s_m, m);
+  __syncthreads();
+  if (threadIdx.x == 0) *max_out = s_m;
+}
+
+// The pair keys of each world sorted in one block (the broad phase emits each
+// world's packed keys (world, a, b) contiguously, worlds in ascending order):
+// the world's segment found by binary search on the world field, its
+// 2 * id_bits key bits radix-sorted in LDS, written as canonical keys at the
+// same positions.  Requires every world's segment to fit kBS * kIPT.
+template <uint32_t kBS, uint32_t kIPT, class R>
+__global__ __launch_bounds__(kBS) void k_segment_sort(const R* raw, uint32_t n, uint32_t id_bits,
+                                                      uint64_t* keys) {
+  __shared__ uint32_t s_k[kBS * kIPT];
+  __shared__ uint32_t s_wcnt[2 * (kBS / 64) * kRadixBins], s_run[2 * kRadixBins];
+  const uint32_t w = blockIdx.x, sh = 2 * id_bits;
+  auto lower = [&](uint32_t x) {  // first position whose world is >= x
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if ((uint32_t)((uint64_t)raw[mid] >> sh) < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  const uint32_t b = lower(w), e = lower(w + 1);
+  const uint32_t cnt = min(e - b, kBS * kIPT);  // the host sized the block for the largest world
+  if (!cnt) return;
+  for (uint32_t d = threadIdx.x; d < 2 * (kBS / 64) * kRadixBins; d += kBS) s_wcnt[d] = 0;
+  const uint32_t m = sh >= 32 ? 0xFFFFFFFFu : (1u << sh) - 1u;
+  uint32_t k[kIPT], v[kIPT];
+#pragma unroll
+  for (uint32_t i = 0; i < kIPT; i++) {
+    const uint32_t j = i * kBS + threadIdx.x;
+    k[i] = j < cnt ? (uint32_t)raw[b + j] & m : 0u;
+    v[i] = 0;
+  }
+  blockRadixSort<kBS, kIPT, false>(k, v, cnt, sh, s_k, nullptr, s_wcnt, s_run);
+  const uint32_t im = (1u << id_bits) - 1u;
+#pragma unroll
+  for (uint32_t i = 0; i < kIPT; i++) {
+    const uint32_t j = i * kBS + threadIdx.x;
+    if (j < cnt) keys[b + j] = makeKey(w, k[i] >> id_bits, k[i] & im);
+  }
+}
+
 // Packed (world, a, b) keys -> the canonical (world << 44 | a << 22 | b).
 __global__ void k_expand_keys(uint64_t* keys, uint32_t n, uint32_t id_bits) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3979,7 +4044,30 @@ void launch_expand_keys32(hipStream_t s, const uint32_t* in, uint64_t* keys, uin
                           uint32_t id_bits) {
   if (n) k_expand_keys32<<<cdiv(n, 256), 256, 0, s>>>(in, keys, n, id_bits);
 }
-void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
+void launch_world_pairs(hipStream_t s, WorldState* ws, uint32_t nw, const uint32_t* leaf_off,
+                        uint32_t MB, uint32_t* max_out) {
+  k_world_pairs<<<1, 1024, 0, s>>>(ws, nw, leaf_off, MB, max_out);
+}
+bool launch_segment_sort(hipStream_t s, const void* raw, bool narrow, uint32_t n, uint32_t nw,
+                         uint32_t id_bits, uint32_t max_world, uint64_t* keys) {
+  if (2 * id_bits > 32 || max_world > 16384) return false;
+  if (!n) return true;
+#define VXP_SEGSORT(BS, IPT)                                                                   \
+  do {                                                                                         \
+    if (narrow)                                                                                \
+      k_segment_sort<BS, IPT, uint32_t><<<nw, BS, 0, s>>>((const uint32_t*)raw, n, id_bits, keys); \
+    else                                                                                       \
+      k_segment_sort<BS, IPT, uint64_t><<<nw, BS, 0, s>>>((const uint64_t*)raw, n, id_bits, keys); \
+  } while (0)
+  if (max_world <= 1024) VXP_SEGSORT(256, 4);
+  else if (max_world <= 2048) VXP_SEGSORT(256, 8);
+  else if (max_world <= 4096) VXP_SEGSORT(512, 8);
+  else if (max_world <= 8192) VXP_SEGSORT(1024, 8);
+  else VXP_SEGSORT(1024, 16);
+#undef VXP_SEGSORT
+  return true;
+}
+void launch_sweep_pairs(hipStream_t s, WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
                         uint32_t id_bits, bool narrow) {

@@ -21,7 +21,16 @@ void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, co
                         const uint32_t* order, uint32_t n_slots, float4* slo, float4* shi,
                         uint32_t* send, uint32_t* wcnt);
 uint32_t sweep_waves();
-void launch_sweep_pairs(hipStream_t s, const WorldState* ws, const float4* slo, const float4* shi,
+// The largest count of pairs one world emitted this step into *max_out (from
+// the per-leaf offsets when given, LBVH; else the counts the sweep added).
+void launch_world_pairs(hipStream_t s, WorldState* ws, uint32_t nw, const uint32_t* leaf_off,
+                        uint32_t MB, uint32_t* max_out);
+// Each world's packed pair keys (raw: 32-bit when narrow) sorted in one
+// workgroup into canonical keys; false (nothing launched) when a world holds
+// more than 16384 pairs or the ids need more than 32 key bits.
+bool launch_segment_sort(hipStream_t s, const void* raw, bool narrow, uint32_t n, uint32_t nw,
+                         uint32_t id_bits, uint32_t max_world, uint64_t* keys);
+void launch_sweep_pairs(hipStream_t s, WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
                         uint32_t id_bits, bool narrow);

@@ -183,8 +183,25 @@ __global__ __launch_bounds__(kSortT) void k_radix_scatter(const K* __restrict__ 
     if (valid) {
       kout[o] = key[k];
       if (kVals) vout[o] = val[k];
-      if (next)
-        atomicAdd(&next[(size_t)digitOf(key[k], next_shift, nmask) * n_tiles + o / kSortTile], 1u);
+    }
+    if (next) {
+      // the next digit's count for the destination tile; lanes of a wave
+      // that share the counter add once (when a tile's keys concentrate on a
+      // few counters, per-lane device atomics on one address serialize)
+      const uint32_t v = valid ? digitOf(key[k], next_shift, nmask) * n_tiles + o / kSortTile
+                               : 0xFFFFFFFFu;
+      bool todo = valid;
+      for (int round = 0; round < 2; round++) {
+        const uint64_t act = __ballot(todo);
+        if (!act) break;
+        const uint32_t lead = (uint32_t)__ffsll((long long)act) - 1u;
+        const uint32_t lv = (uint32_t)__shfl((int)v, (int)lead, 64);
+        const uint64_t same = __ballot(todo && v == lv);
+        if (__popcll(same) < 2) break;  // no sharing here: per-lane adds below
+        if ((threadIdx.x & 63u) == lead) atomicAdd(&next[lv], (uint32_t)__popcll(same));
+        if (v == lv) todo = false;
+      }
+      if (todo) atomicAdd(&next[v], 1u);
     }
   }
 }
