@@ -788,28 +788,34 @@ static int stepAllImpl(vxp_system* s) {
   const uint32_t swv = sweep_waves();
   HIP_TRY(s->d_swc.ensure((size_t)(swv + 1) * 4));
   HIP_TRY(s->d_swo.ensure((size_t)(swv + 1) * 4));
+  // per-world pair counts (for the per-world key sort) only with several
+  // worlds: one world's count is the total, and 8192 waves adding to one
+  // counter would serialize
   launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
                      s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), s->d_swc.as<uint32_t>(),
-                     nullptr, nullptr, true, id_bits, narrow);
+                     nullptr, nullptr, true, id_bits, narrow, nw > 1);
   {
     size_t cb = exclusive_scan_u32_bytes(swv + 1);
     HIP_TRY(s->d_sort_tmp.ensure(cb));
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_swc.as<uint32_t>(),
                                s->d_swo.as<uint32_t>(), swv + 1, st));
   }
-  launch_world_pairs(st, ws, nw, nullptr, s->MB, s->d_maxw.as<uint32_t>());
-  {
+  if (nw > 1) {
+    launch_world_pairs(st, ws, nw, nullptr, s->MB, s->d_maxw.as<uint32_t>());
     uint32_t sz[2];
     RC(readSizes(s, {s->d_swo.as<uint32_t>() + swv, s->d_maxw.as<uint32_t>()}, sz));
     npairs = sz[0];
     max_world_pairs = sz[1];
+  } else {
+    RC(readSizes(s, {s->d_swo.as<uint32_t>() + swv}, &npairs));
+    max_world_pairs = npairs;
   }
   HIP_TRY(s->d_pairs_raw.ensure((size_t)std::max(npairs, 1u) * 8));
   if (npairs)
     launch_sweep_pairs(st, ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), ns,
                        s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), nullptr,
                        s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false, id_bits,
-                       narrow);
+                       narrow, false);
   profEnd(s, VXP_K_PAIRS, pe);
   }
   s->n_pairs = npairs;

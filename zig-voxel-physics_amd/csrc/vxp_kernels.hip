@@ -506,7 +506,8 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(WorldState* ws, const float
                                                      const float4* shi, uint32_t n_slots,
                                                      const uint32_t* send, const uint32_t* woff_i,
                                                      uint32_t* wcount, const uint32_t* woff_w,
-                                                     uint64_t* out, uint32_t id_bits, bool narrow) {
+                                                     uint64_t* out, uint32_t id_bits, bool narrow,
+                                                     bool per_world) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const uint32_t total = woff_i[n_slots];
@@ -546,12 +547,14 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(WorldState* ws, const float
       const uint64_t em = __ballot(emit);
       if (kCount) {
         n += (uint32_t)__popcll(em);
-        if (w != wcur) {
-          if (wn && lane == 0) atomicAdd(&ws[wcur].np_emit, wn);
-          wcur = w;
-          wn = 0;
+        if (per_world) {
+          if (w != wcur) {
+            if (wn && lane == 0) atomicAdd(&ws[wcur].np_emit, wn);
+            wcur = w;
+            wn = 0;
+          }
+          wn += (uint32_t)__popcll(em);
         }
-        wn += (uint32_t)__popcll(em);
       } else {
         // packed key (world, a, b) with ids of id_bits bits, expanded after sorting
         if (emit) {
@@ -4070,14 +4073,14 @@ bool launch_segment_sort(hipStream_t s, const void* raw, bool narrow, uint32_t n
 void launch_sweep_pairs(hipStream_t s, WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
-                        uint32_t id_bits, bool narrow) {
+                        uint32_t id_bits, bool narrow, bool per_world) {
   constexpr uint32_t blocks = kSweepWaves / 4;
   if (count)
     k_sweep_pairs<true><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount, woff_w,
-                                               out, id_bits, narrow);
+                                               out, id_bits, narrow, per_world);
   else
     k_sweep_pairs<false><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount,
-                                                woff_w, out, id_bits, narrow);
+                                                woff_w, out, id_bits, narrow, false);
 }
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,

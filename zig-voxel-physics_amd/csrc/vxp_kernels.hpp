@@ -22,7 +22,8 @@ void launch_sweep_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B, co
                         uint32_t* send, uint32_t* wcnt);
 uint32_t sweep_waves();
 // The largest count of pairs one world emitted this step into *max_out (from
-// the per-leaf offsets when given, LBVH; else the counts the sweep added).
+// the per-leaf offsets when given, LBVH; else the counts the sweep's count
+// pass added when per_world: several worlds; one world's count is the total).
 void launch_world_pairs(hipStream_t s, WorldState* ws, uint32_t nw, const uint32_t* leaf_off,
                         uint32_t MB, uint32_t* max_out);
 // Each world's packed pair keys (raw: 32-bit when narrow) sorted in one
@@ -33,7 +34,8 @@ bool launch_segment_sort(hipStream_t s, const void* raw, bool narrow, uint32_t n
 void launch_sweep_pairs(hipStream_t s, WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
-                        uint32_t id_bits, bool narrow);
+                        uint32_t id_bits, bool narrow,
+                        bool per_world);
 // LBVH broad phase (per-world trees at each world's slot offsets).
 struct LbvhBufs {
   float4 *llo, *lhi, *ilo, *ihi;  // leaves in Morton order; internal nodes

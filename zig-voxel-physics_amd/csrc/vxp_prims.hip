@@ -122,13 +122,13 @@ __device__ __forceinline__ uint32_t digitOf(K key, uint32_t shift, uint32_t mask
   return (uint32_t)(key >> shift) & mask;
 }
 
-// Digit counts of each tile for the first pass, digit-major:
-// hist[d * n_tiles + tile]; also clears the `clear` words after the first
-// table (the tables of the later passes, which the scatters fill).
+// Digit counts of each tile for one pass (digit key bits [shift, shift +
+// bits)), digit-major: hist[d * n_tiles + tile].
 template <class K>
 __global__ __launch_bounds__(kSortT) void k_radix_hist(const K* __restrict__ keys, uint32_t n,
-                                                       uint32_t bits, uint32_t n_tiles,
-                                                       uint32_t* __restrict__ hist, uint32_t clear) {
+                                                       uint32_t shift, uint32_t bits,
+                                                       uint32_t n_tiles,
+                                                       uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[kDevBins];
   const uint32_t tid = threadIdx.x, mask = (1u << bits) - 1u;
   for (uint32_t d = tid; d < kDevBins; d += kSortT) s_h[d] = 0;
@@ -137,30 +137,26 @@ __global__ __launch_bounds__(kSortT) void k_radix_hist(const K* __restrict__ key
 #pragma unroll
   for (uint32_t k = 0; k < kSortIPT; k++) {
     const size_t i = base + k * kSortT + tid;
-    if (i < n) atomicAdd(&s_h[digitOf(keys[i], 0, mask)], 1u);
+    if (i < n) atomicAdd(&s_h[digitOf(keys[i], shift, mask)], 1u);
   }
   __syncthreads();
   for (uint32_t d = tid; d < kDevBins; d += kSortT) hist[(size_t)d * n_tiles + blockIdx.x] = s_h[d];
-  uint32_t* later = hist + (size_t)kDevBins * n_tiles;
-  for (size_t i = (size_t)blockIdx.x * kSortT + tid; i < clear; i += (size_t)gridDim.x * kSortT)
-    later[i] = 0;
 }
 
 // Every item of the tile to off[d * n_tiles + tile] + its stable rank among
-// the tile's items of digit d; with a next pass, counts the item's next digit
-// for its destination tile in next[d' * n_tiles + o / tile].
+// the tile's items of digit d.  (The next pass's per-tile digit counts are a
+// pass of their own over the output: counting them here needs device-scope
+// atomics that serialize whenever many tiles feed one destination tile.)
 template <class K, bool kVals>
 __global__ __launch_bounds__(kSortT) void k_radix_scatter(const K* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin,
                                                           K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                           uint32_t n, uint32_t shift, uint32_t bits,
                                                           uint32_t n_tiles,
-                                                          const uint32_t* __restrict__ off,
-                                                          uint32_t next_shift, uint32_t next_bits,
-                                                          uint32_t* __restrict__ next) {
+                                                          const uint32_t* __restrict__ off) {
   __shared__ uint32_t s_wcnt[2 * (kSortT / 64) * kDevBins];
   __shared__ uint32_t s_run[2 * kDevBins];
-  const uint32_t tid = threadIdx.x, mask = (1u << bits) - 1u, nmask = (1u << next_bits) - 1u;
+  const uint32_t tid = threadIdx.x, mask = (1u << bits) - 1u;
   for (uint32_t d = tid; d < 2 * (kSortT / 64) * kDevBins; d += kSortT) s_wcnt[d] = 0;
   for (uint32_t d = tid; d < kDevBins; d += kSortT)
     s_run[d] = off[(size_t)d * n_tiles + blockIdx.x];  // digit d's first output slot
@@ -183,25 +179,6 @@ __global__ __launch_bounds__(kSortT) void k_radix_scatter(const K* __restrict__ 
     if (valid) {
       kout[o] = key[k];
       if (kVals) vout[o] = val[k];
-    }
-    if (next) {
-      // the next digit's count for the destination tile; lanes of a wave
-      // that share the counter add once (when a tile's keys concentrate on a
-      // few counters, per-lane device atomics on one address serialize)
-      const uint32_t v = valid ? digitOf(key[k], next_shift, nmask) * n_tiles + o / kSortTile
-                               : 0xFFFFFFFFu;
-      bool todo = valid;
-      for (int round = 0; round < 2; round++) {
-        const uint64_t act = __ballot(todo);
-        if (!act) break;
-        const uint32_t lead = (uint32_t)__ffsll((long long)act) - 1u;
-        const uint32_t lv = (uint32_t)__shfl((int)v, (int)lead, 64);
-        const uint64_t same = __ballot(todo && v == lv);
-        if (__popcll(same) < 2) break;  // no sharing here: per-lane adds below
-        if ((threadIdx.x & 63u) == lead) atomicAdd(&next[lv], (uint32_t)__popcll(same));
-        if (v == lv) todo = false;
-      }
-      if (todo) atomicAdd(&next[v], 1u);
     }
   }
 }
@@ -227,13 +204,14 @@ static uint32_t sortPasses(uint32_t end_bit) {
   return std::max(1u, (end_bit + kDevMaxBits - 1) / kDevMaxBits);
 }
 
-// Temporary storage of a sort: one digit table per pass and the tables'
-// scan storage, then the ping-pong key and value arrays.
+// Temporary storage of a sort: the digit table and its scan storage, then
+// the ping-pong key and value arrays.
 template <class K>
 static size_t sortTmpBytes(uint32_t n, uint32_t end_bit, bool vals) {
+  (void)end_bit;
   const uint32_t tiles = std::max(1u, cdivU(n, kSortTile));
   const size_t table = (size_t)kDevBins * tiles;
-  return align256(table * 4 * sortPasses(end_bit)) + scanTmpBytes((uint32_t)table) +
+  return align256(table * 4) + scanTmpBytes((uint32_t)table) +
          align256((size_t)n * sizeof(K)) + (vals ? align256((size_t)n * 4) : 0);
 }
 
@@ -247,8 +225,8 @@ static hipError_t sortImpl(void* tmp, size_t bytes, const K* kin, K* kout, const
   const uint32_t passes = sortPasses(end_bit);
   const uint32_t width = std::max(1u, (end_bit + passes - 1) / passes);
   char* p = (char*)tmp;
-  uint32_t* hist = (uint32_t*)p;  // `passes` tables of `table` words
-  p += align256((size_t)table * 4 * passes);
+  uint32_t* hist = (uint32_t*)p;  // the digit table (`table` words), reused by every pass
+  p += align256((size_t)table * 4);
   void* scan_tmp = p;
   p += scanTmpBytes(table);
   K* kalt = (K*)p;
@@ -266,17 +244,14 @@ static hipError_t sortImpl(void* tmp, size_t bytes, const K* kin, K* kout, const
     const uint32_t sh = pass * width;
     return sh >= end_bit ? 1u : std::min(width, end_bit - sh);
   };
-  k_radix_hist<K><<<tiles, kSortT, 0, s>>>(src, n, bitsOf(0), tiles, hist, table * (passes - 1));
   for (uint32_t pass = 0; pass < passes; pass++) {
-    uint32_t* h = hist + (size_t)table * pass;
-    hipError_t e = scanImpl(scan_tmp, h, h, table, s);
+    k_radix_hist<K><<<tiles, kSortT, 0, s>>>(src, n, pass * width, bitsOf(pass), tiles, hist);
+    hipError_t e = scanImpl(scan_tmp, hist, hist, table, s);
     if (e != hipSuccess) return e;
     K* dst = (pass & 1) ? kb : ka;
     uint32_t* vdst = (pass & 1) ? vb : va;
-    const bool more = pass + 1 < passes;
-    k_radix_scatter<K, kVals><<<tiles, kSortT, 0, s>>>(
-        src, vsrc, dst, vdst, n, pass * width, bitsOf(pass), tiles, h, (pass + 1) * width,
-        more ? bitsOf(pass + 1) : 1u, more ? h + table : nullptr);
+    k_radix_scatter<K, kVals><<<tiles, kSortT, 0, s>>>(src, vsrc, dst, vdst, n, pass * width,
+                                                       bitsOf(pass), tiles, hist);
     src = dst;
     vsrc = vdst;
   }

@@ -5,6 +5,9 @@
   python3 -m vxp.profsum last   <results.db> <kernel-substring> <N> [COUNTER]
                                (the last N dispatches of one kernel: average
                                 duration, or average counter value per dispatch)
+  python3 -m vxp.profsum section <results.db> <marker-kernel> <skip> [top]
+                               (kernels dispatched from the (skip+1)-th dispatch
+                                of the marker kernel on: one section of a run)
   python3 vxp/profsum.py copycal      (the calibration program: five 256 MiB
                                        device copies, profiled like the bench)
 """
@@ -33,6 +36,41 @@ def pmc(db, counter, filt=""):
         if filt and filt not in name:
             continue
         out.append(f"| {name.split('(')[0].replace('void ', '')[:70]} | {n} | {avg:.1f} |")
+    return "\n".join(out)
+
+
+def section(db, marker, skip, top=30):
+    """Per-kernel totals over the dispatches from the (skip+1)-th dispatch of the
+    kernel whose name contains `marker` on (trace run), with the count of marker
+    dispatches in the section (steps, for k_step_begin)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, start, duration from kernels order by start").fetchall()
+    seen, t0 = 0, None
+    for name, start, _ in rows:
+        if marker in name:
+            seen += 1
+            if seen == skip + 1:
+                t0 = start
+                break
+    if t0 is None:
+        return f"fewer than {skip + 1} dispatches of {marker}"
+    agg = {}
+    steps = 0
+    for name, start, dur in rows:
+        if start < t0:
+            continue
+        short = name.split("(")[0].replace("void ", "")
+        a = agg.setdefault(short, [0, 0.0])
+        a[0] += 1
+        a[1] += dur
+        if marker in name:
+            steps += 1
+    total = sum(v[1] for v in agg.values())
+    out = [f"{steps} dispatches of {marker} in the section", "",
+           "| kernel | calls | total ms | avg us | per step us | % |", "|---|---|---|---|---|---|"]
+    for k, (n, d) in sorted(agg.items(), key=lambda x: -x[1][1])[:top]:
+        out.append(f"| {k[:70]} | {n} | {d / 1e6:.3f} | {d / n / 1e3:.1f} | "
+                   f"{d / max(steps, 1) / 1e3:.1f} | {100 * d / total:.1f} |")
     return "\n".join(out)
 
 
@@ -75,6 +113,9 @@ if __name__ == "__main__":
     elif sys.argv[1] == "last":
         print(last(sys.argv[2], sys.argv[3], int(sys.argv[4]),
                    sys.argv[5] if len(sys.argv) > 5 else None))
+    elif sys.argv[1] == "section":
+        print(section(sys.argv[2], sys.argv[3], int(sys.argv[4]),
+                      int(sys.argv[5]) if len(sys.argv) > 5 else 30))
     elif sys.argv[1] == "trace":
         print(trace(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 15))
     else:
