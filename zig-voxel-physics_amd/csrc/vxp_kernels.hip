@@ -1396,36 +1396,45 @@ __device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
     *vchk = 0;
     return 0;
   }
-  const int32_t bx0 = lo[0] >> 3, bx1 = hi[0] >> 3;
-  for (int32_t bz = lo[2]; bz <= hi[2]; bz++)
-    for (int32_t by = lo[1]; by <= hi[1]; by++)
-      for (int32_t bxb = bx0; bxb <= bx1; bxb++) {
-        // one brick's part of the row
-        const int32_t gx = bxb - mb.org[0], gy = (by >> 3) - mb.org[1], gz = (bz >> 3) - mb.org[2];
+  // Brick by brick (the window spans at most two bricks per axis): one
+  // directory read per brick, one read per mask plane per z slice, which
+  // covers the slice's rows of the window (bit x + 8y of the slice's word).
+  // Counts are sums, so the visiting order does not matter.
+  for (int32_t zb = lo[2] >> 3; zb <= hi[2] >> 3; zb++)
+    for (int32_t yb = lo[1] >> 3; yb <= hi[1] >> 3; yb++)
+      for (int32_t xb8 = lo[0] >> 3; xb8 <= hi[0] >> 3; xb8++) {
+        const int32_t gx = xb8 - mb.org[0], gy = yb - mb.org[1], gz = zb - mb.org[2];
         if ((uint32_t)gx >= (uint32_t)mb.dim[0] || (uint32_t)gy >= (uint32_t)mb.dim[1] ||
             (uint32_t)gz >= (uint32_t)mb.dim[2])
           continue;
         const int32_t bi = dir[mb.dir_off + ((uint32_t)gz * mb.dim[1] + gy) * mb.dim[0] + gx];
         if (bi < 0) continue;
-        const int32_t xa = max(lo[0], bxb * 8), xb = min(hi[0], bxb * 8 + 7);
-        const uint32_t sh = 8u * (uint32_t)(by & 7) + (uint32_t)(xa & 7);
-        const uint32_t wmask = (1u << (xb - xa + 1)) - 1u;
-        const uint64_t* m = mask + (size_t)bi * 24 + (bz & 7);
-        const uint32_t occ = (uint32_t)(m[0] >> sh) & wmask;
-        vc += (uint32_t)__builtin_popcount(occ);
-        uint32_t cand = occ;
-        if (ta != VXP_VOXEL_CORNER) {
-          uint32_t t = (uint32_t)(m[8] >> sh);
-          if (ta == VXP_VOXEL_EDGE) t |= (uint32_t)(m[16] >> sh);
-          cand &= t;
-        }
-        while (cand) {
-          const int32_t bx = xa + __builtin_ctz(cand);
-          cand &= cand - 1;
-          V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
-          V3 diff = sub(wb, wa);
-          float dsq = dot(diff, diff);
-          if (dsq < c.spec) n++;
+        const int32_t xa = max(lo[0], xb8 * 8), xe = min(hi[0], xb8 * 8 + 7);
+        const int32_t ya = max(lo[1], yb * 8), ye = min(hi[1], yb * 8 + 7);
+        const int32_t za = max(lo[2], zb * 8), ze = min(hi[2], zb * 8 + 7);
+        const uint32_t wmask = (1u << (xe - xa + 1)) - 1u;
+        const uint64_t* mz = mask + (size_t)bi * 24;
+        for (int32_t bz = za; bz <= ze; bz++) {
+          const uint64_t w0 = mz[bz & 7];
+          uint64_t wt = ~0ull;
+          if (ta != VXP_VOXEL_CORNER) {
+            wt = mz[8 + (bz & 7)];
+            if (ta == VXP_VOXEL_EDGE) wt |= mz[16 + (bz & 7)];
+          }
+          for (int32_t by = ya; by <= ye; by++) {
+            const uint32_t sh = 8u * (uint32_t)(by & 7) + (uint32_t)(xa & 7);
+            const uint32_t occ = (uint32_t)(w0 >> sh) & wmask;
+            vc += (uint32_t)__builtin_popcount(occ);
+            uint32_t cand = occ & (uint32_t)(wt >> sh);
+            while (cand) {
+              const int32_t bx = xa + __builtin_ctz(cand);
+              cand &= cand - 1;
+              V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
+              V3 diff = sub(wb, wa);
+              float dsq = dot(diff, diff);
+              if (dsq < c.spec) n++;
+            }
+          }
         }
       }
   *vchk = vc;
@@ -2168,13 +2177,20 @@ __device__ __forceinline__ float4 f4(float x, float y, float z, float w) {
 }
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
+// A field's base (uniform: scalar registers) plus the manifold's byte offset as
+// a 32-bit word, so every record load is a scalar-base + 32-bit vector-offset
+// access with no 64-bit address arithmetic per load (a world's records stay
+// far below 4 GiB per field).
 struct SoaRec {
   float4* rec;
   float4* pts;
   size_t M;
-  __device__ __forceinline__ float4& r(uint32_t j, int q) const { return rec[q * M + j]; }
+  __device__ __forceinline__ static float4& at(float4* base, uint32_t j) {
+    return *reinterpret_cast<float4*>(reinterpret_cast<char*>(base) + j * 16u);
+  }
+  __device__ __forceinline__ float4& r(uint32_t j, int q) const { return at(rec + q * M, j); }
   __device__ __forceinline__ float4& p(uint32_t j, uint32_t k, int q) const {
-    return pts[q * 4 * M + k * M + j];
+    return at(pts + (q * 4 + k) * M, j);
   }
 };
 
@@ -2292,21 +2308,23 @@ __device__ __forceinline__ void solveManifoldVel(const DevCfg& cfg, VelPtr VL, V
   }
 }
 
-// impulseTo without a branch: the update is computed and kept only for an
-// awake body (the same arithmetic, so the same bits).
-__device__ __forceinline__ void impulseSel(bool awake, V3& lv, V3& av, float im, const M3& ii,
-                                           V3 imp, V3 r) {
-  V3 nl = add(lv, scale(imp, im));
-  V3 na = add(av, mulVec(ii, cross(r, imp)));
-  lv = V3{awake ? nl.x : lv.x, awake ? nl.y : lv.y, awake ? nl.z : lv.z};
-  av = V3{awake ? na.x : av.x, awake ? na.y : av.y, awake ? na.z : av.z};
-}
-
 // The partner lane of a lane pair (2k, 2k + 1): DPP quad_perm(1, 0, 3, 2).
 __device__ __forceinline__ float pairSwap(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 __device__ __forceinline__ V3 pairSwap(V3 v) { return V3{pairSwap(v.x), pairSwap(v.y), pairSwap(v.z)}; }
+
+// One row's velocity change of the lane's own body: BodyState.applyImpulse
+// AtPoint (src/solver.zig:316-322) with the impulse this body receives (already
+// negated for body A).  Lanes whose body is not awake skip it under the exec
+// mask instead of computing and discarding it.
+__device__ __forceinline__ void impulseMine(bool awake, V3& lv, V3& av, float im, const M3& ii,
+                                            V3 imp, V3 r) {
+  if (awake) {
+    lv = add(lv, scale(imp, im));
+    av = add(av, mulVec(ii, cross(r, imp)));
+  }
+}
 
 // solveManifoldVel on a lane pair: lane `side` 0 holds body A, side 1 body B.
 // Each lane evaluates the contact velocity of its own body and applies its own
@@ -2314,6 +2332,15 @@ __device__ __forceinline__ V3 pairSwap(V3 v) { return V3{pairSwap(v.x), pairSwap
 // and every shared scalar (vn, impulse, clamp) is computed identically in both
 // lanes, so the result equals the one-lane pass bit for bit.  Both lanes of a
 // pair follow the same control flow (same manifold, count and friction).
+//
+// Signs without branches: each lane forms d = mine - partner, which is
+// vb - va in lane B and its exact negation in lane A, so dot(d, axis) times
+// sgn (+1 for B, -1 for A) is the reference's dot(vb - va, axis) (negating
+// both operands of a rounded subtraction, or the terms of a rounded sum,
+// negates the result exactly; only an exact zero can come out with the other
+// sign, which changes no later non-zero value); the impulse a lane's body
+// receives is the row impulse times sgn, which is the reference's neg(P) for
+// body A bit for bit (-(x * y) == x * (-y)).
 template <bool kBias, class VelPtr>
 __device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, VelPtr VA,
                                                   const SoaRec& S, uint32_t j, uint32_t side) {
@@ -2345,44 +2372,37 @@ __device__ __forceinline__ void solveManifoldVel2(const DevCfg& cfg, VelPtr VL, 
     float ni = q4.w, ti0 = q5.w, ti1 = q6.w;
     const float sgn = side ? 1.0f : -1.0f;  // A receives -P, B receives +P (exact negation)
     V3 mine = velAt(lv, av, pos, pt);
-    V3 other = pairSwap(mine);
-    V3 rel = side ? sub(mine, other) : sub(other, mine);
-    float vn = dot(rel, nrm);
+    V3 d = sub(mine, pairSwap(mine));
+    float vn = dot(d, nrm) * sgn;
     float imp;
     if (kBias) imp = -cfg.mass_coeff * nm * (vn + bias) - cfg.impulse_coeff * ni;
     else imp = -nm * vn;
     float nw = fmaxf(ni + imp, 0.0f);
     imp = nw - ni;
     ni = nw;
-    V3 Pn = scale(nrm, imp);
-    impulseSel(aw, lv, av, im, ii, side ? Pn : neg(Pn), r);
+    impulseMine(aw, lv, av, im, ii, scale(nrm, imp * sgn), r);
     if (cfg.enable_friction && friction > 0.0f) {
       mine = velAt(lv, av, pos, pt);
-      other = pairSwap(mine);
-      V3 rel1 = side ? sub(mine, other) : sub(other, mine);
+      d = sub(mine, pairSwap(mine));
       float maxf = friction * ni;
-      float vt = dot(rel1, t1);
+      float vt = dot(d, t1) * sgn;
       float fi = -tm0 * vt;
       float old = ti0;
       ti0 = clampz(old + fi, -maxf, maxf);
       fi = ti0 - old;
-      V3 i1 = scale(t1, fi);
-      impulseSel(aw, lv, av, im, ii, side ? i1 : neg(i1), r);
+      impulseMine(aw, lv, av, im, ii, scale(t1, fi * sgn), r);
       mine = velAt(lv, av, pos, pt);
-      other = pairSwap(mine);
-      V3 rel2 = side ? sub(mine, other) : sub(other, mine);
-      float vt2 = dot(rel2, t2);
+      d = sub(mine, pairSwap(mine));
+      float vt2 = dot(d, t2) * sgn;
       float fi2 = -tm1 * vt2;
       float old2 = ti1;
       ti1 = clampz(old2 + fi2, -maxf, maxf);
       fi2 = ti1 - old2;
-      V3 i2 = scale(t2, fi2);
-      impulseSel(aw, lv, av, im, ii, side ? i2 : neg(i2), r);
+      impulseMine(aw, lv, av, im, ii, scale(t2, fi2 * sgn), r);
       S.p(j, k, 5).w = ti0;  // both lanes hold the same value
       S.p(j, k, 6).w = ti1;
     }
     S.p(j, k, 4).w = ni;
-    (void)sgn;
   }
   if (aw) {
     VL[slot] = v4(lv);
@@ -3004,6 +3024,26 @@ __device__ void unitPositionCombine(WorldState& x, uint32_t slot, float minsep, 
 // Unit blocks of kT threads: 512 lets one large unit run its colours on more
 // lane pairs; 256 fits two blocks per CU (the passes keep a manifold's records
 // in registers, about 210 VGPRs), for launches with many small units.
+// A run of colour passes (src/solver.zig:526-548,848-856): colour c of cn
+// manifolds goes to lane pairs (two lanes per manifold, bs / 2 manifolds at a
+// time) when cn <= pair_color_max, to single lanes otherwise.
+template <bool kBias, class VelPtr>
+__device__ __forceinline__ void colourPasses(const DevCfg& cfg, VelPtr VL, VelPtr VA,
+                                             const SoaRec& SR, const uint32_t* ccount, uint32_t mb,
+                                             uint32_t ncol, uint32_t tid, uint32_t bs) {
+  for (uint32_t c = 0; c < ncol; c++) {
+    const uint32_t c0 = mb + ccount[c], cn = ccount[c + 1] - ccount[c];
+    if (cn <= cfg.pair_color_max) {
+      for (uint32_t t = tid >> 1; t < cn; t += bs >> 1)
+        solveManifoldVel2<kBias>(cfg, VL, VA, SR, c0 + t, tid & 1u);
+    } else {
+      for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
+        solveManifoldVel<kBias>(cfg, VL, VA, SR, j);
+    }
+    __syncthreads();
+  }
+}
+
 template <bool kLds, uint32_t kT>
 __global__ __launch_bounds__(kT) void k_solve_units(DevCfg cfg, DevBodies B, WorldState* ws,
                                                     DevManifold* ml, SolverScratch S,
@@ -3188,19 +3228,8 @@ __global__ __launch_bounds__(kT) void k_solve_units(DevCfg cfg, DevBodies B, Wor
           }
         }
         // solveVelocityConstraints with bias
-        for (uint32_t it = 0; it < cfg.vel_iters; it++) {
-          for (uint32_t c = 0; c < ncol; c++) {
-            const uint32_t c0 = mb + ccount[c], cn = ccount[c + 1] - ccount[c];
-            if (cn <= cfg.pair_color_max) {  // lane pairs, bs / 2 manifolds at a time
-              for (uint32_t t = tid >> 1; t < cn; t += bs >> 1)
-                solveManifoldVel2<true>(cfg, VL, VA, SR, c0 + t, tid & 1u);
-            } else {
-              for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
-                solveManifoldVel<true>(cfg, VL, VA, SR, j);
-            }
-            __syncthreads();
-          }
-        }
+        for (uint32_t it = 0; it < cfg.vel_iters; it++)
+          colourPasses<true>(cfg, VL, VA, SR, ccount, mb, ncol, tid, bs);
       }
       if (sub_i == 0) stampW(ts, w, 11);
       // integratePositions (src/solver.zig:341-356,841-845): the unit's
@@ -3234,17 +3263,7 @@ __global__ __launch_bounds__(kT) void k_solve_units(DevCfg cfg, DevBodies B, Wor
           SR.r(j, 8) = f4(iib.m[7], iib.m[8], 0.0f, 0.0f);
         }
         __syncthreads();
-        for (uint32_t c = 0; c < ncol; c++) {
-          const uint32_t c0 = mb + ccount[c], cn = ccount[c + 1] - ccount[c];
-          if (cn <= cfg.pair_color_max) {
-            for (uint32_t t = tid >> 1; t < cn; t += bs >> 1)
-              solveManifoldVel2<false>(cfg, VL, VA, SR, c0 + t, tid & 1u);
-          } else {
-            for (uint32_t j = c0 + tid; j < c0 + cn; j += bs)
-              solveManifoldVel<false>(cfg, VL, VA, SR, j);
-          }
-          __syncthreads();
-        }
+        colourPasses<false>(cfg, VL, VA, SR, ccount, mb, ncol, tid, bs);
       }
       if (sub_i == 0) stampW(ts, w, 12);
     }
