@@ -12,7 +12,10 @@ compares them with the plain single-world path looped over the same worlds;
 Poisson request arrivals and reports p50/p99 request latency per offered rate;
 "c0" times the 64-cube scene; "cpu_baseline" times the CPU restatement on the
 last frames of the 4096-body window, and each section carries the CPU
-restatement on its own workload ("cpu").  Prints one JSON line.
+restatement on its own workload ("cpu").  "recorded" lines repeat the 4096-body
+window, the batch and the serving loop with the steps replayed from recorded
+HIP graphs (vxp_graph_enable; same results, fewer host round trips).  Prints
+one JSON line.
 
 The 4096-body scene's trajectory is 600 frames; the timed frames are its last
 --steps frames (frames 600 - steps .. 599), after the earlier ones run untimed,
@@ -140,7 +143,14 @@ def pct(xs, q):
     return s[min(len(s) - 1, int(q * len(s)))]
 
 
-def run_serving(lib, args):
+def graph_line(r, sys_, steps):
+    """Timing of a run whose steps replay recorded graphs, with its graph counters."""
+    return {"value": r["value"], "ms_per_step": r["wall"] * 1e3 / steps,
+            "p50_step_ms": statistics.median(r["frame_ms"]), "p99_step_ms": pct(r["frame_ms"], 0.99),
+            "graph": sys_.graphInfo()}
+
+
+def run_serving(lib, args, recorded=False):
     """Requests "advance world W by k frames" arrive as a Poisson process at each
     offered rate for --serve-seconds (W uniform over the resident worlds, k
     uniform in {1, 2, 4, 8}); the loop submits every request whose arrival time
@@ -152,6 +162,7 @@ def run_serving(lib, args):
     sc = scenes.serving_pool(n_worlds=args.serve_worlds)
     t_gen = time.perf_counter() - t0
     srv = serve.from_scene(sc, slots=args.serve_slots, budget_bytes=200 * 10**9, lib=lib)
+    srv.set_recorded(recorded)
     t_build = time.perf_counter() - t0 - t_gen
     nw = len(sc.worlds)
     rng = random.Random(46)
@@ -204,8 +215,10 @@ def run_serving(lib, args):
             srv.tick()
         srv.poll()
     info = srv.info()
+    ginfo = srv.graph_info()
     srv.close()
     return {
+        "steps_recorded": recorded, "graph": ginfo,
         "workload": f"{sc.name}: {nw} resident worlds, {sc.n_dynamic} dynamic bodies (log-uniform "
                     f"8..2048 per world), {args.serve_slots} batch slots, Poisson arrivals for "
                     f"{args.serve_seconds:g} s per rate, k uniform in {{1,2,4,8}}",
@@ -234,6 +247,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-c0", action="store_true")
+    ap.add_argument("--no-recorded", action="store_true",
+                    help="skip the recorded-step (HIP graph) runs beside the eager ones")
     ap.add_argument("--c2-steps", type=int, default=390)
     ap.add_argument("--serve-worlds", type=int, default=2048)
     ap.add_argument("--serve-slots", type=int, default=256)
@@ -267,6 +282,13 @@ def main():
     if not args.no_breakdown:  # the same frames again on a fresh copy, every phase timed
         s1 = vxp.from_scene(sc, lib=lib)
         brk = run(s1, sc, pre, args.steps, profile=vxp._abi.PROFILE_ALL)
+        s1.close()
+    rec1 = None
+    if not args.no_recorded:  # the same frames replayed from recorded graphs (no kernel timing)
+        s1 = vxp.from_scene(sc, lib=lib)
+        s1.setRecorded(True)
+        r = run(s1, sc, pre, args.steps)
+        rec1 = graph_line(r, s1, args.steps)
         s1.close()
     kern = plain["kern"]
     bk = brk["kern"] if brk else {}
@@ -315,6 +337,13 @@ def main():
         sysb = vxp.from_scene(sb, lib=lib)
         rb = run(sysb, sb, args.warmup, args.batched_steps)
         sysb.close()
+        recb = None
+        if not args.no_recorded:
+            sysb = vxp.from_scene(sb, lib=lib)
+            sysb.setRecorded(True)
+            r = run(sysb, sb, args.warmup, args.batched_steps)
+            recb = graph_line(r, sysb, args.batched_steps)
+            sysb.close()
         # the plain single-world path looped over a sample of the same worlds,
         # over the same frames
         nl = min(args.looped_worlds, len(sb.worlds))
@@ -344,6 +373,7 @@ def main():
             "p50_step_ms": statistics.median(rb["frame_ms"]),
             "p99_step_ms": pct(rb["frame_ms"], 0.99),
             "tuning": rb["tuning"],
+            "recorded": recb,
             "plain_looped_value": loop_bs / loop_t if loop_t else None,
             "plain_looped_sample": f"{nl} of the {len(sb.worlds)} worlds stepped one system at a "
                                    f"time, frames {args.warmup}..{args.warmup + lf - 1}",
@@ -418,6 +448,9 @@ def main():
     if not args.no_serving:
         log("serving")
         serving = run_serving(lib, args)
+        if not args.no_recorded:
+            log("serving, recorded steps")
+            serving["recorded"] = run_serving(lib, args, recorded=True)
 
     # ---- CPU baseline: the CPU restatement on the last frames of the timed window
     cpu = None
@@ -474,6 +507,7 @@ def main():
         "kernel_ms_note": ("the same frames on a second copy of the scene with every phase "
                            "timed (HIP events per phase; not the timed run)") if bk else None,
         "tuning": plain["tuning"],
+        "recorded": rec1,
         "final_stats": {k: final[k] for k in ("manifolds", "colors", "sleeping_bodies",
                                               "voxels_checked", "broad_phase_pairs")},
         "host_cpu": {"model": cpu_model, "logical_cpus": nproc},

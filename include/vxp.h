@@ -482,6 +482,30 @@ int vxp_tuning_get(vxp_system* sys, vxp_tuning* out);
  * when not 0 (VXP_ERR_ARGUMENT for a value outside those listed above). */
 int vxp_tuning_set(vxp_system* sys, const vxp_tuning* tuning);
 
+/* ---- recorded step (HIP graphs) ---------------------------------------------
+ * No counterpart in the reference, whose update() steps on the CPU; this is
+ * for batched systems and the serving loop, where the step's ~45 launches and
+ * its per-step size reads leave the GPU idle between kernels.  With recording
+ * on, a step of the default pipeline (sort-and-sweep pairs in key order, no
+ * pair cap, no thread-pool narrow phase, timing off, no tuning window in
+ * progress) runs as three recorded HIP graphs whose launches are sized for
+ * capacities and read their counts on the device; the host reads the counts
+ * twice per step (instead of four times) and a step past a capacity is rolled
+ * back and replayed with larger graphs.  Other steps run as before.  Results
+ * are bit-identical either way. */
+typedef struct vxp_graph_info {
+  uint64_t recorded_steps; /* steps run from recorded graphs */
+  uint64_t redos;          /* steps rolled back and replayed past a capacity */
+  uint64_t recordings;     /* graph sets recorded (shapes, buffers or capacities changed) */
+  double recording_ms;     /* host time spent recording them */
+  uint32_t cap_pairs, cap_world_pairs, cap_work_items, cap_candidates;
+} vxp_graph_info;
+int vxp_graph_enable(vxp_system* sys, int on);
+int vxp_graph_info_get(vxp_system* sys, vxp_graph_info* out);
+/* The server's batch steps recorded (as vxp_graph_enable on its batch system). */
+int vxp_server_graph_enable(vxp_server* srv, int on);
+int vxp_server_graph_info_get(vxp_server* srv, vxp_graph_info* out);
+
 /* Last error message of this thread (never NULL). */
 const char* vxp_last_error(void);
 

@@ -36,10 +36,11 @@ def test_every_declared_symbol_is_exported(lib):
 
 
 def test_oracle_exports_the_same_api(oracle):
-    # timing, launch tuning, the serving loop and the allocation-free entry
-    # point exist only in the product
+    # timing, launch tuning, recorded steps, the serving loop and the
+    # allocation-free entry point exist only in the product
     names = [n for n in declared_functions()
-             if not n.startswith(("vxp_profile", "vxp_server", "vxp_tuning", "vxp_noalloc"))
+             if not n.startswith(("vxp_profile", "vxp_server", "vxp_tuning", "vxp_noalloc",
+                                  "vxp_graph"))
              and n not in ("vxp_config_default", "vxp_collider_mass_properties_noalloc")]
     missing = [n for n in names if not hasattr(oracle.dll, "vxo_" + n[4:])]
     assert not missing, missing

@@ -5,6 +5,8 @@
 // per-step work runs in the gfx950 kernels of vxp_kernels.hip on one HIP
 // stream.  Body state lives on the device between steps; getters copy it out
 // and setters copy it in before the next step.
+#include <chrono>
+
 #include "vxp_host.hpp"
 
 namespace vxh {
@@ -379,6 +381,7 @@ DevCfg devCfg(vxp_system* s) {
   d.use_cache = c.use_brick_pair_cache;
   d.enable_islands = c.enable_islands;
   d.store_impulses = c.enable_warm_starting;
+  d.n_cu = s->n_cu;
   return d;
 }
 
@@ -628,16 +631,18 @@ int joinSide(vxp_system* s) {
 }
 
 static int stepAllImpl(vxp_system* s);
+static bool recordable(const vxp_system* s);
+static int stepRecorded(vxp_system* s);
 int stepAll(vxp_system* s) {
-  const int rc = stepAllImpl(s);
+  RC(ensureDevice(s));
+  RC(tuneStep(s));
+  const int rc = recordable(s) ? stepRecorded(s) : stepAllImpl(s);
   const int jr = joinSide(s);  // a step that ended early leaves the join undone
   return rc != VXP_OK ? rc : jr;
 }
 
 static int stepAllImpl(vxp_system* s) {
   hipStream_t st = s->stream;
-  RC(ensureDevice(s));
-  RC(tuneStep(s));
   DevCfg cfg = devCfg(s);
   cfg.pair_color_max = s->tune.use[vxp_system::kPairColorMax];
   DevBodies B = devBodies(s);
@@ -970,6 +975,8 @@ static int stepAllImpl(vxp_system* s) {
                  false, shape);
   RC(tuneMark(s, vxp_system::kPhProbe, false));
   profEnd(s, VXP_K_PROBE, pe);
+  s->last_wi = total_wi;
+  s->last_cand = ncand;
   pe = profBegin(s, VXP_K_REDUCE);
   launch_reduce(st, cfg, B, C, ws, stats, pr, keys, npairs, s->d_wlist.as<uint32_t>(),
                 s->d_step_ctr.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
@@ -993,6 +1000,7 @@ static int stepAllImpl(vxp_system* s) {
   RC(readSizes(s, {s->d_total.as<uint32_t>(), s->d_total.as<uint32_t>() + 1}, sizes));
   const uint32_t nman = sizes[0], max_man = sizes[1];
   s->n_manifolds = nman;
+  s->last_max_man = max_man;
   uint32_t nm1 = std::max(nman, 1u);
   HIP_TRY(s->d_ml[s->cur].ensure((size_t)nm1 * sizeof(DevManifold)));
   DevManifold* ml = s->d_ml[s->cur].as<DevManifold>();
@@ -1080,14 +1088,11 @@ static int stepAllImpl(vxp_system* s) {
     HIP_TRY(hipEventRecord(s->ev_join, s->side));
     s->side_pending = true;
   }
-  // solve units of at least this many manifolds: about two per CU over the
-  // whole batch (results do not depend on it)
-  cfg.unit_min = std::max(64u, nman / (2 * std::max(s->n_cu, 1u)));
   uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
   RC(tuneMark(s, vxp_system::kPhSolve, true));
   pe = profBegin(s, VXP_K_SOLVE);
   launch_solve_prep(st, cfg, B, ws, stats, ml, S, s->s_prio.as<uint32_t>(), ctr, nw, max_man,
-                    s->lds_limit);
+                    s->lds_limit, s->d_total.as<uint32_t>());
   profEnd(s, VXP_K_SOLVE, pe);
   RC(joinSide(s));  // the units read the working state; the writeback overwrites what the labels read
   pe = profBegin(s, VXP_K_SOLVE_UNITS);
@@ -1110,8 +1115,544 @@ static int stepAllImpl(vxp_system* s) {
   return VXP_OK;
 }
 
+// ---- recorded step ------------------------------------------------------------
+// vxp_graph_enable: the step of the default pipeline (sort-and-sweep pairs in
+// key order, no filtered-pair cap, no thread-pool narrow phase, no per-phase
+// timing, no tuning window in progress) is recorded as three HIP graphs per
+// manifold-buffer parity and replayed.  Every launch is sized for a capacity
+// and reads its item count on the device, so a replay needs no size read
+// between launches; the host reads the counts twice per step, after segment 1
+// (pairs, pairs of one world, the probe work items if every cache-eligible
+// pair missed: the three counts the capacities bound) and after segment 2
+// (contact candidates).  Segment 1 changes no state that a redo could not
+// restore (world states and stats are copied aside first; the fat boxes are a
+// function of the poses), so a step past a capacity is rolled back and
+// replayed with larger graphs; past the candidate capacity, only segment 3 is
+// recorded again.  The kernels and their order are the eager step's, so the
+// results are the same bits.
+
+// Every device buffer of a system (released together; their addresses are
+// part of what a recorded step bakes in).
+static std::vector<DevBuf*> allBufs(vxp_system* s) {
+  return {&s->d_hdr, &s->d_dir, &s->d_vox, &s->d_mask, &s->b_pos, &s->b_rot, &s->b_lv, &s->b_av,
+          &s->b_fat_lo, &s->b_fat_hi, &s->b_inv_mass, &s->b_iil, &s->b_fric, &s->b_rest,
+          &s->b_ldamp, &s->b_adamp, &s->b_sleep_timer, &s->b_type, &s->b_sleeping,
+          &s->b_active, &s->b_collider, &s->b_id, &s->b_idmap, &s->d_ws, &s->d_stats,
+          &s->d_cache_keys, &s->d_cache_vals, &s->d_skeys, &s->d_svals, &s->d_skeys2,
+          &s->d_svals2, &s->d_sort_tmp, &s->d_slo, &s->d_shi, &s->d_send, &s->d_wcnt, &s->d_woff,
+          &s->d_swc, &s->d_swo, &s->d_pairs_raw, &s->d_pairs, &s->d_pr, &s->d_counts, &s->d_wi_off,
+          &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_wi_vchk, &s->d_lane_n, &s->d_cand, &s->d_produce,
+          &s->d_pidx, &s->d_staging, &s->d_cstage, &s->d_rstat, &s->d_clist, &s->d_cflag,
+          &s->d_cidx, &s->d_wlist, &s->d_step_ctr, &s->d_total, &s->d_ml[0], &s->d_ml[1],
+          &s->d_isl_parent, &s->d_isl_end, &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors,
+          &s->s_pos, &s->s_rot, &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off,
+          &s->s_adj, &s->s_color, &s->s_order, &s->s_rec, &s->s_pts, &s->s_prio, &s->s_tstamp,
+          &s->s_upar, &s->s_ucnt, &s->s_uoff, &s->s_ubody, &s->s_uorder, &s->s_units,
+          &s->s_uqueue, &s->s_uqueue2, &s->s_pcnt, &s->s_fq, &s->s_cend, &s->d_cb_pos,
+          &s->d_cb_vox, &s->d_cb_cnt, &s->d_cb_bounds, &s->d_cb_box, &s->d_cb_out,
+          &s->d_cb_solid, &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
+          &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off, &s->d_t_pos,
+          &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake, &s->d_arank, &s->d_drop,
+          &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair, &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code,
+          &s->d_lb_child, &s->d_lb_range, &s->d_lb_iparent, &s->d_lb_lparent, &s->d_lb_flag,
+          &s->d_maxw, &s->d_rsnap, &s->d_rdesc, &s->d_rsize};
+}
+
+static bool recordable(const vxp_system* s) {
+  return s->rec_on && !treeMode(s) && broadPhase(s) == VXP_BROAD_SWEEP && !s->cfg.pair_cap &&
+         !s->cfg.enable_threading && s->prof == VXP_PROFILE_OFF && !s->tune.pending &&
+         s->lds_limit > 0 && s->n_cu > 0 && s->read_seq > 0;  // after an eager step
+}
+
+// Publication slots in the pinned words (h_read[0..2] serve readSizes).
+constexpr uint32_t kPub1 = 16, kPub2 = 32, kPub3 = 48;
+
+// What a recorded step bakes in: buffer addresses, shapes, knobs, capacities.
+static uint64_t recSignature(vxp_system* s, int par) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  for (DevBuf* b : allBufs(s)) mix((uint64_t)(uintptr_t)b->p);
+  mix(s->nw);
+  mix(s->MB);
+  mix(s->id_cap);
+  mix((uint64_t)par);
+  mix(s->lds_limit);
+  mix(s->n_cu);
+  for (uint32_t k = 0; k < vxp_system::kKnobs; k++) mix(s->tune.use[k]);
+  mix(s->rc_pairs);
+  mix(s->rc_wp);
+  mix(s->rc_wi);
+  mix(s->rc_cand);
+  mix(s->rc_carry);
+  mix(s->rc_mm);
+  const unsigned char* c = (const unsigned char*)&s->cfg;
+  for (size_t i = 0; i < sizeof(vxp_config); i++) mix(c[i]);
+  return h;
+}
+
+static uint32_t idBits(const vxp_system* s) {
+  uint32_t b = 1;
+  while ((1u << b) < s->id_cap) b++;
+  return b;
+}
+static uint32_t worldBits(const vxp_system* s) {
+  uint32_t b = 0;
+  while ((1u << b) < s->nw) b++;
+  return b;
+}
+
+// Every buffer a recorded step touches, at its capacity (no allocation may
+// happen while a step is being recorded).
+static int recEnsure(vxp_system* s) {
+  const size_t ns = slots(s), nw = s->nw;
+  const uint32_t swv = sweep_waves(), wbits = worldBits(s), kbits = 2 * idBits(s) + wbits;
+  const size_t P = s->rc_pairs, WI = s->rc_wi, CA = s->rc_cand;
+  const size_t M = P + s->rc_carry;  // manifolds: produced (<= pairs) + carried over
+  HIP_TRY(s->d_step_ctr.ensure(kStepCounters * 4));
+  HIP_TRY(s->d_maxw.ensure(4));
+  HIP_TRY(s->d_skeys.ensure(ns * 8));
+  HIP_TRY(s->d_svals.ensure(ns * 4));
+  HIP_TRY(s->d_skeys2.ensure(ns * 8));
+  HIP_TRY(s->d_svals2.ensure(ns * 4));
+  HIP_TRY(s->d_slo.ensure(ns * 16));
+  HIP_TRY(s->d_shi.ensure(ns * 16));
+  HIP_TRY(s->d_send.ensure(ns * 4));
+  HIP_TRY(s->d_wcnt.ensure((ns + 1) * 4));
+  HIP_TRY(s->d_woff.ensure((ns + 1) * 4));
+  HIP_TRY(s->d_swc.ensure((size_t)(swv + 1) * 4));
+  HIP_TRY(s->d_swo.ensure((size_t)(swv + 1) * 4));
+  HIP_TRY(s->d_pairs_raw.ensure(P * 8));
+  HIP_TRY(s->d_pairs.ensure(P * 8));
+  HIP_TRY(s->d_pr.ensure(P * sizeof(PairRec)));
+  HIP_TRY(s->d_counts.ensure(P * 4));
+  HIP_TRY(s->d_wi_off.ensure(P * 4));
+  HIP_TRY(s->d_produce.ensure(P * 4));
+  HIP_TRY(s->d_rstat.ensure(P * sizeof(RStat)));
+  HIP_TRY(s->d_clist.ensure(P * 4));
+  HIP_TRY(s->d_wlist.ensure(P * 4));
+  HIP_TRY(s->d_cflag.ensure((P + 1) * 4));
+  HIP_TRY(s->d_cidx.ensure((P + 1) * 4));
+  HIP_TRY(s->d_pidx.ensure(P * 4));
+  HIP_TRY(s->d_staging.ensure(P * sizeof(DevManifold)));
+  HIP_TRY(s->d_cstage.ensure(P * sizeof(CacheVal)));
+  HIP_TRY(s->d_wi_cbase.ensure((WI + 1) * 4));
+  HIP_TRY(s->d_wi_ccnt.ensure((WI + 1) * 4));
+  HIP_TRY(s->d_wi_vchk.ensure(WI * 4));
+  HIP_TRY(s->d_lane_n.ensure(WI * 64));
+  HIP_TRY(s->d_wi_pair.ensure(WI * 4));
+  HIP_TRY(s->d_cand.ensure(CA * sizeof(Cand)));
+  HIP_TRY(s->d_total.ensure(64));
+  HIP_TRY(s->d_ml[0].ensure(M * sizeof(DevManifold), true, s->stream));
+  HIP_TRY(s->d_ml[1].ensure(M * sizeof(DevManifold), true, s->stream));
+  if (s->cfg.enable_coloring) {
+    HIP_TRY(s->d_ref_masks.ensure(nw * s->id_cap * 32));
+    HIP_TRY(s->d_ref_colors.ensure(M));
+  }
+  if (s->cfg.enable_islands) RC(ensureIslands(s));
+  HIP_TRY(s->s_pos.ensure(ns * 16));
+  HIP_TRY(s->s_rot.ensure(ns * 16));
+  HIP_TRY(s->s_lv.ensure(ns * 16));
+  HIP_TRY(s->s_av.ensure(ns * 16));
+  HIP_TRY(s->s_ii.ensure(ns * 36));
+  HIP_TRY(s->s_adj_cnt.ensure(ns * 4));
+  HIP_TRY(s->s_adj_off.ensure(ns * 4));
+  HIP_TRY(s->s_adj.ensure(M * 8));
+  HIP_TRY(s->s_color.ensure(M * 4));
+  HIP_TRY(s->s_order.ensure(M * 4));
+  HIP_TRY(s->s_prio.ensure(M * 4));
+  HIP_TRY(s->s_rec.ensure(M * 9 * sizeof(float4)));
+  HIP_TRY(s->s_pts.ensure(M * 4 * 8 * sizeof(float4)));
+  HIP_TRY(s->s_tstamp.ensure(nw * 16 * 8));
+  HIP_TRY(s->s_upar.ensure(ns * 4));
+  HIP_TRY(s->s_ucnt.ensure(ns * 4));
+  HIP_TRY(s->s_uoff.ensure(ns * 4));
+  HIP_TRY(s->s_ubody.ensure(ns * 4));
+  HIP_TRY(s->s_uorder.ensure(M * 4));
+  HIP_TRY(s->s_units.ensure(nw * kUnitsPerWorld * sizeof(uint4)));
+  HIP_TRY(s->s_uqueue.ensure(nw * kUnitsPerWorld * 4));
+  HIP_TRY(s->s_uqueue2.ensure(nw * kUnitsPerWorld * 4));
+  HIP_TRY(s->s_pcnt.ensure(M * 4));
+  HIP_TRY(s->s_fq.ensure(M * 4));
+  HIP_TRY(s->s_cend.ensure(M * 16));
+  size_t tmp = std::max({exclusive_scan_u32_bytes((uint32_t)ns + 1),
+                         exclusive_scan_u32_bytes(swv + 1),
+                         exclusive_scan_u32_bytes((uint32_t)P + 1),
+                         exclusive_scan_u32_bytes((uint32_t)WI + 1),
+                         sort_pairs_u64_u32_bytes((uint32_t)ns, 32 + wbits),
+                         kbits <= 32 ? sort_keys_u32_bytes((uint32_t)P, kbits)
+                                     : sort_keys_u64_bytes((uint32_t)P, kbits)});
+  HIP_TRY(s->d_sort_tmp.ensure(tmp));
+  // snapshot of the world states and stats, and the copies to and from it
+  const size_t wsb = nw * sizeof(WorldState), stb = nw * sizeof(vxp_stats);
+  HIP_TRY(s->d_rsnap.ensure(wsb + stb));
+  HIP_TRY(s->d_rsize.ensure(64));
+  HIP_TRY(s->d_rdesc.ensure(4 * sizeof(CopyDesc)));
+  char* snap = s->d_rsnap.as<char>();
+  const CopyDesc d[4] = {{s->d_ws.p, snap, wsb}, {s->d_stats.p, snap + wsb, stb},
+                         {snap, s->d_ws.p, wsb}, {snap + wsb, s->d_stats.p, stb}};
+  if (std::memcmp(d, s->rec_desc, sizeof(d)) != 0) {  // only when an address changed
+    HIP_TRY(hipMemcpyAsync(s->d_rdesc.p, d, sizeof(d), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::memcpy(s->rec_desc, d, sizeof(d));
+  }
+  return VXP_OK;
+}
+
+struct RecCtx {
+  DevCfg cfg;
+  DevBodies B;
+  DevColliders C;
+  WorldState* ws;
+  vxp_stats* st;
+  uint32_t nw, ns, wbits, id_bits, swv, P, WP, WI, CA;
+  const uint32_t* npairs;  // device: the step's pairs
+  const uint32_t* maxw;    // device: the most pairs of one world
+  uint32_t* rsize;         // device: [0] probe work items, [1] contact candidates
+  DevManifold *prev, *ml;
+};
+
+static RecCtx recCtx(vxp_system* s, int par) {
+  RecCtx c;
+  c.cfg = devCfg(s);
+  c.cfg.pair_color_max = s->tune.use[vxp_system::kPairColorMax];
+  c.B = devBodies(s);
+  c.C = devColliders(s);
+  c.ws = s->d_ws.as<WorldState>();
+  c.st = s->d_stats.as<vxp_stats>();
+  c.nw = s->nw;
+  c.ns = (uint32_t)slots(s);
+  c.wbits = worldBits(s);
+  c.id_bits = idBits(s);
+  c.swv = sweep_waves();
+  c.P = s->rc_pairs;
+  c.WP = s->rc_wp;
+  c.WI = s->rc_wi;
+  c.CA = s->rc_cand;
+  c.npairs = s->d_swo.as<uint32_t>() + c.swv;
+  c.maxw = c.nw > 1 ? s->d_maxw.as<uint32_t>() : c.npairs;
+  c.rsize = s->d_rsize.as<uint32_t>();
+  c.ml = s->d_ml[par].as<DevManifold>();
+  c.prev = s->d_ml[par ^ 1].as<DevManifold>();
+  return c;
+}
+
+// Segment 1: snapshot, step begin, fat boxes, sort-and-sweep pairs, pair keys,
+// pair processing, the cache flags' scan; publishes the bounded counts.
+static int recSeg1(vxp_system* s, const RecCtx& c) {
+  hipStream_t st = s->stream;
+  void* tmp = s->d_sort_tmp.p;
+  const size_t tb = s->d_sort_tmp.bytes;
+  uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
+  launch_copy_many(st, s->d_rdesc.as<CopyDesc>(), 2);
+  launch_step_begin(st, c.cfg, c.B, c.ws, c.st, c.nw, ctr);
+  launch_rec_carried(st, c.ws, c.nw, c.rsize);
+  launch_fat_aabb(st, c.cfg, c.B, c.C, c.ws, c.ns, nullptr);
+  if (!launch_sweep_sort(st, c.cfg, c.B, c.ws, s->d_svals2.as<uint32_t>(), c.nw)) {
+    launch_sweep_keys(st, c.cfg, c.B, c.ws, s->d_skeys.as<uint64_t>(), s->d_svals.as<uint32_t>(),
+                      c.ns, c.nw);
+    HIP_TRY(sort_pairs_u64_u32(tmp, tb, s->d_skeys.as<uint64_t>(), s->d_skeys2.as<uint64_t>(),
+                               s->d_svals.as<uint32_t>(), s->d_svals2.as<uint32_t>(), c.ns,
+                               32 + c.wbits, st));
+  }
+  launch_sweep_count(st, c.cfg, c.B, c.ws, s->d_svals2.as<uint32_t>(), c.ns, s->d_slo.as<float4>(),
+                     s->d_shi.as<float4>(), s->d_send.as<uint32_t>(), s->d_wcnt.as<uint32_t>());
+  HIP_TRY(exclusive_scan_u32(tmp, tb, s->d_wcnt.as<uint32_t>(), s->d_woff.as<uint32_t>(),
+                             c.ns + 1, st));
+  const bool narrow = 2 * c.id_bits + c.wbits <= 32;
+  launch_sweep_pairs(st, c.ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), c.ns,
+                     s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), s->d_swc.as<uint32_t>(),
+                     nullptr, nullptr, true, c.id_bits, narrow, c.nw > 1);
+  HIP_TRY(exclusive_scan_u32(tmp, tb, s->d_swc.as<uint32_t>(), s->d_swo.as<uint32_t>(),
+                             c.swv + 1, st));
+  if (c.nw > 1) launch_world_pairs(st, c.ws, c.nw, nullptr, s->MB, s->d_maxw.as<uint32_t>());
+  launch_sweep_pairs(st, c.ws, s->d_slo.as<float4>(), s->d_shi.as<float4>(), c.ns,
+                     s->d_send.as<uint32_t>(), s->d_woff.as<uint32_t>(), nullptr,
+                     s->d_swo.as<uint32_t>(), s->d_pairs_raw.as<uint64_t>(), false, c.id_bits,
+                     narrow, false, c.P);
+  // the pair keys in canonical order (as the eager step sorts them): per world
+  // in one workgroup when every world holds at most 16384 pairs, device-wide
+  // otherwise; both are recorded, each acting on a count gated on the device
+  const uint32_t kbits = 2 * c.id_bits + c.wbits;
+  uint64_t* keys = s->d_pairs.as<uint64_t>();
+  const uint32_t* radix_n = c.npairs;
+  if (2 * c.id_bits <= 32) {
+    launch_rec_gate(st, c.npairs, c.maxw, 16384, c.rsize);
+    launch_segment_sort(st, s->d_pairs_raw.p, kbits <= 32, c.P, c.nw, c.id_bits, 16384, keys,
+                        c.rsize + 4);
+    radix_n = c.rsize + 3;
+  }
+  if (kbits <= 32) {
+    uint32_t* raw = s->d_pairs_raw.as<uint32_t>();
+    HIP_TRY(sort_keys_u32_n(tmp, tb, raw, raw + c.P, c.P, radix_n, kbits, st));
+    launch_expand_keys32(st, raw + c.P, keys, c.P, c.id_bits, radix_n);
+  } else {
+    HIP_TRY(sort_keys_u64_n(tmp, tb, s->d_pairs_raw.as<uint64_t>(), keys, c.P, radix_n, kbits,
+                            st));
+    launch_expand_keys(st, keys, c.P, c.id_bits, radix_n);
+  }
+  launch_pair_process(st, c.cfg, c.B, c.C, c.ws, c.st, keys, c.P, s->d_pr.as<PairRec>(), c.prev,
+                      nullptr, s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(),
+                      s->d_rstat.as<RStat>(), s->d_wlist.as<uint32_t>(), ctr,
+                      s->d_counts.as<uint32_t>(), c.npairs);
+  HIP_TRY(exclusive_scan_u32_n(tmp, s->d_cflag.as<uint32_t>(), s->d_cidx.as<uint32_t>(), c.P + 1,
+                               c.npairs, 1, st));
+  launch_publish4(st, s->d_read + kPub1, c.npairs, c.maxw, ctr + kCtrChunkBound, c.rsize + 2, 1);
+  HIP_TRY(hipGetLastError());
+  return VXP_OK;
+}
+
+// Segment 2: the brick-pair cache replay, the probe's work items and its count
+// pass; publishes the work items and the candidates.
+static int recSeg2(vxp_system* s, const RecCtx& c) {
+  hipStream_t st = s->stream;
+  void* tmp = s->d_sort_tmp.p;
+  PairRec* pr = s->d_pr.as<PairRec>();
+  launch_cache_replay(st, c.cfg, c.ws, c.st, pr, s->d_cache_keys.as<CacheKeyRec>(),
+                      s->d_cflag.as<uint32_t>(), s->d_cidx.as<uint32_t>(), nullptr,
+                      s->d_clist.as<uint32_t>(), c.P, c.nw, s->d_counts.as<uint32_t>(), c.npairs);
+  HIP_TRY(exclusive_scan_u32_n(tmp, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), c.P,
+                               c.npairs, 0, st));
+  launch_rec_total_wi(st, s->d_wi_off.as<uint32_t>(), s->d_counts.as<uint32_t>(), c.npairs,
+                      c.rsize);
+  const ProbeShape shape{s->tune.use[vxp_system::kProbeThreads],
+                         s->tune.use[vxp_system::kProbeXcdRun]};
+  launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), c.P,
+                 s->d_wi_pair.as<uint32_t>(), c.npairs);
+  launch_probe(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
+               c.WI, nullptr, s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
+               s->d_lane_n.as<uint8_t>(), nullptr, true, shape, c.rsize);
+  HIP_TRY(exclusive_scan_u32_n(tmp, s->d_wi_ccnt.as<uint32_t>(), s->d_wi_cbase.as<uint32_t>(),
+                               c.WI + 1, c.rsize, 1, st));
+  launch_rec_total_cand(st, s->d_wi_cbase.as<uint32_t>(), c.rsize);
+  launch_publish4(st, s->d_read + kPub2, c.rsize, c.rsize + 1, nullptr, nullptr, 2);
+  HIP_TRY(hipGetLastError());
+  return VXP_OK;
+}
+
+// Segment 3: the probe's write pass, the manifold reduction and compaction,
+// islands and the solver; publishes the manifold count.
+static int recSeg3(vxp_system* s, const RecCtx& c) {
+  hipStream_t st = s->stream;
+  void* tmp = s->d_sort_tmp.p;
+  PairRec* pr = s->d_pr.as<PairRec>();
+  uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
+  const ProbeShape shape{s->tune.use[vxp_system::kProbeThreads],
+                         s->tune.use[vxp_system::kProbeXcdRun]};
+  launch_probe(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
+               c.WI, s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
+               s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(), s->d_cand.as<Cand>(), false,
+               shape, c.rsize);
+  launch_reduce(st, c.cfg, c.B, c.C, c.ws, c.st, pr, s->d_pairs.as<uint64_t>(), c.P,
+                s->d_wlist.as<uint32_t>(), ctr, s->d_wi_off.as<uint32_t>(),
+                s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
+                s->d_wi_vchk.as<uint32_t>(), s->d_cand.as<Cand>(), c.prev,
+                s->d_cache_keys.as<CacheKeyRec>(), s->d_cache_vals.as<CacheVal>(),
+                s->d_cstage.as<CacheVal>(), s->d_staging.as<DevManifold>(),
+                s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(), c.npairs);
+  HIP_TRY(exclusive_scan_u32_n(tmp, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(), c.P,
+                               c.npairs, 0, st));
+  launch_world_segments(st, c.ws, c.nw, s->d_total.as<uint32_t>());
+  launch_scatter(st, c.ws, pr, c.P, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(),
+                 s->d_staging.as<DevManifold>(), c.ml, c.npairs);
+  launch_copy_skipped(st, c.ws, c.prev, c.ml, c.nw);
+  if (s->cfg.enable_coloring)
+    launch_ref_coloring(st, c.cfg, c.ws, c.ml, s->d_ref_masks.as<uint64_t>(),
+                        s->d_ref_colors.as<uint8_t>(), c.nw);
+  SolverScratch S;
+  S.ws_pos = s->s_pos.as<float4>();
+  S.ws_rot = s->s_rot.as<float4>();
+  S.ws_lv = s->s_lv.as<float4>();
+  S.ws_av = s->s_av.as<float4>();
+  S.ws_ii = s->s_ii.as<float>();
+  S.adj_cnt = s->s_adj_cnt.as<uint32_t>();
+  S.adj_off = s->s_adj_off.as<uint32_t>();
+  S.adj = s->s_adj.as<uint32_t>();
+  S.color = s->s_color.as<int32_t>();
+  S.order = s->s_order.as<uint32_t>();
+  S.rec = s->s_rec.as<float4>();
+  S.pts = s->s_pts.as<float4>();
+  S.tstamp = nullptr;
+  S.upar = s->s_upar.as<uint32_t>();
+  S.ucnt = s->s_ucnt.as<uint32_t>();
+  S.uoff = s->s_uoff.as<uint32_t>();
+  S.ubody = s->s_ubody.as<uint32_t>();
+  S.uorder = s->s_uorder.as<uint32_t>();
+  S.units = s->s_units.as<uint4>();
+  S.uqueue = s->s_uqueue.as<uint32_t>();
+  S.pcnt = s->s_pcnt.as<uint32_t>();
+  S.fq = s->s_fq.as<uint32_t>();
+  S.cend = s->s_cend.as<uint4>();
+  // island labels and the body working state beside the solver's prep
+  HIP_TRY(hipEventRecord(s->ev_fork, st));
+  HIP_TRY(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+  if (s->cfg.enable_islands)
+    launch_islands(s->side, c.cfg, c.B, c.ws, c.st, c.ml, c.prev, islandArrays(s), c.nw, s->rc_mm,
+                   s->lds_limit);
+  launch_body_state(s->side, c.cfg, c.B, c.ws, S, c.ns);
+  HIP_TRY(hipEventRecord(s->ev_join, s->side));
+  launch_solve_prep(st, c.cfg, c.B, c.ws, c.st, c.ml, S, s->s_prio.as<uint32_t>(), ctr, c.nw,
+                    s->rc_mm, s->lds_limit, s->d_total.as<uint32_t>());
+  HIP_TRY(hipStreamWaitEvent(st, s->ev_join, 0));
+  launch_unit_order(st, S, ctr, s->s_uqueue2.as<uint32_t>());
+  launch_solve_units(st, c.cfg, c.B, c.ws, c.ml, S, ctr, c.nw, s->n_cu, s->lds_limit,
+                     unitThreads(s));
+  launch_solve_tail(st, c.cfg, c.B, c.ws, c.st, S, ctr, c.nw, s->n_cu);
+  launch_publish4(st, s->d_read + kPub3, s->d_total.as<uint32_t>(), s->d_total.as<uint32_t>() + 1,
+                  nullptr, nullptr, 3);
+  HIP_TRY(hipGetLastError());
+  return VXP_OK;
+}
+
+static int recRecord(vxp_system* s, int par) {
+  auto& R = s->rec[par];
+  const auto t0 = std::chrono::steady_clock::now();
+  const RecCtx c = recCtx(s, par);
+  for (int k = 0; k < 3; k++) {
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = k == 0 ? recSeg1(s, c) : k == 1 ? recSeg2(s, c) : recSeg3(s, c);
+    const hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc != VXP_OK) {
+      if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIP_TRY(e);
+    if (R.g[k]) {
+      (void)hipGraphExecDestroy(R.g[k]);
+      R.g[k] = nullptr;
+    }
+    const hipError_t ie = hipGraphInstantiate(&R.g[k], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(ie);
+  }
+  R.sig = recSignature(s, par);
+  s->rec_records++;
+  s->rec_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return VXP_OK;
+}
+
+// Launches one recorded segment and, when asked, waits for its publication
+// (polling the pinned words, as readSizes does).
+static int recRun(vxp_system* s, int par, int k, uint32_t slot, uint32_t seq, uint32_t* out,
+                  int nout) {
+  __atomic_store_n(&s->h_read[slot], 0u, __ATOMIC_RELEASE);
+  HIP_TRY(hipGraphLaunch(s->rec[par].g[k], s->stream));
+  if (!out) return VXP_OK;
+  for (uint32_t spin = 1;; spin++) {
+    if (__atomic_load_n(&s->h_read[slot], __ATOMIC_ACQUIRE) == seq) break;
+    if ((spin & 1023u) == 0) {
+      hipError_t e = hipStreamQuery(s->stream);
+      if (e != hipSuccess && e != hipErrorNotReady) {
+        g_err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+        return VXP_ERR_DEVICE;
+      }
+      if (e == hipSuccess && __atomic_load_n(&s->h_read[slot], __ATOMIC_ACQUIRE) != seq) {
+        g_err = "recorded step: the stream drained without the published counts";
+        return VXP_ERR_DEVICE;
+      }
+    }
+    __builtin_ia32_pause();
+  }
+  for (int i = 0; i < nout; i++) out[i] = __atomic_load_n(&s->h_read[slot + 1 + i], __ATOMIC_RELAXED);
+  return VXP_OK;
+}
+
+// A capacity that holds `need`: twice it (the recorded launches loop over their
+// items, so a larger capacity costs memory, not time, and fewer recordings).
+static uint32_t grown(uint32_t need, uint32_t have) {
+  const uint64_t g = std::max<uint64_t>(2 * (uint64_t)need + 1024, (uint64_t)have * 2);
+  return (uint32_t)std::min<uint64_t>(g, 0xFFFFFFF0ull);
+}
+
+static int stepRecorded(vxp_system* s) {
+  // first capacities: the last eager step's counts with headroom
+  if (!s->rc_pairs) {
+    s->rc_mm = s->last_max_man + s->last_max_man / 4 + 64;
+    s->rc_carry = grown(s->n_manifolds, 256);
+    s->rc_pairs = grown(s->n_pairs, 1024);
+    s->rc_wp = s->nw > 1 ? std::min(s->rc_pairs, grown(s->n_pairs / s->nw, 1024)) : s->rc_pairs;
+    s->rc_wi = grown(s->last_wi, 1024);
+    s->rc_cand = grown(s->last_cand, 4096);
+  }
+  s->cur ^= 1;  // ContactManager.beginFrame: the current manifolds become the previous ones
+  const int par = s->cur;
+  for (int attempt = 0;; attempt++) {
+    RC(recEnsure(s));
+    if (!s->rec[par].g[0] || s->rec[par].sig != recSignature(s, par)) RC(recRecord(s, par));
+    uint32_t c1[4];
+    RC(recRun(s, par, 0, kPub1, 1, c1, 4));
+    // the previous recorded step's most manifolds of one world (its segment 3
+    // ran before this segment 1): the islands' and solver prep's LDS request
+    // of the next recording
+    if (__atomic_load_n(&s->h_read[kPub3], __ATOMIC_ACQUIRE) == 3) {
+      s->last_max_man = __atomic_load_n(&s->h_read[kPub3 + 2], __ATOMIC_RELAXED);
+      if (s->last_max_man > s->rc_mm) s->rc_mm = s->last_max_man + s->last_max_man / 4 + 64;
+    }
+    const uint32_t npairs = c1[0], maxw = c1[1], bound = c1[2], carried = c1[3];
+    // the most pairs of one world only sizes the islands' and the solver
+    // prep's LDS request (their kernels check the fit per world): a larger
+    // value is taken for the next recording, without a redo
+    if (maxw > s->rc_wp) s->rc_wp = std::min(s->rc_pairs, grown(maxw, s->rc_wp));
+    if (npairs <= s->rc_pairs && bound <= s->rc_wi && carried <= s->rc_carry) {
+      s->n_pairs = npairs;
+      break;
+    }
+    // past a capacity: roll the world states and stats back, grow, record again
+    launch_copy_many(s->stream, s->d_rdesc.as<CopyDesc>() + 2, 2);
+    HIP_TRY(hipGetLastError());
+    s->rec_redos++;
+    if (npairs > s->rc_pairs) s->rc_pairs = grown(npairs, s->rc_pairs);
+    if (bound > s->rc_wi) s->rc_wi = grown(bound, s->rc_wi);
+    if (carried > s->rc_carry) s->rc_carry = grown(carried, s->rc_carry);
+    if (attempt == 4) {
+      g_err = "recorded step: the capacities did not settle";
+      return VXP_ERR_DEVICE;
+    }
+  }
+  uint32_t c2[2];
+  RC(recRun(s, par, 1, kPub2, 2, c2, 2));
+  s->last_wi = c2[0];
+  s->last_cand = c2[1];
+  if (c2[1] > s->rc_cand) {  // segment 3 only: nothing it reads has been written yet
+    s->rc_cand = grown(c2[1], s->rc_cand);
+    RC(recEnsure(s));
+    RC(recRecord(s, par));
+  }
+  RC(recRun(s, par, 2, kPub3, 3, nullptr, 0));
+  s->rec_nman_pending = true;
+  s->rec_steps++;
+  s->dev_newer = true;
+  return VXP_OK;
+}
+
+// The manifold count of the last recorded step (published at its end).
+int settleManifolds(vxp_system* s) {
+  if (!s->rec_nman_pending) return VXP_OK;
+  RC(streamWait(s));
+  if (__atomic_load_n(&s->h_read[kPub3], __ATOMIC_ACQUIRE) != 3) {
+    g_err = "recorded step: the manifold count was not published";
+    return VXP_ERR_DEVICE;
+  }
+  s->n_manifolds = __atomic_load_n(&s->h_read[kPub3 + 1], __ATOMIC_RELAXED);
+  s->rec_nman_pending = false;
+  return VXP_OK;
+}
+
+static void recRelease(vxp_system* s) {
+  for (auto& R : s->rec)
+    for (auto& g : R.g)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+}
+
 int checkErrors(vxp_system* s) {
   HIP_TRY(hipStreamSynchronize(s->stream));
+  RC(settleManifolds(s));
   profFlush(s);
   std::vector<WorldState> ws(s->nw);
   HIP_TRY(hipMemcpy(ws.data(), s->d_ws.p, ws.size() * sizeof(WorldState), hipMemcpyDeviceToHost));
@@ -1167,6 +1708,27 @@ int vxp_profile_enable(vxp_system* s, int on) {
       s->prof_ms[i] = 0.0;
       s->prof_n[i] = 0;
     }
+  return VXP_OK;
+}
+
+int vxp_graph_enable(vxp_system* s, int on) {
+  if (!s) return VXP_ERR_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  RC(settleManifolds(s));
+  s->rec_on = on != 0;
+  return VXP_OK;
+}
+
+int vxp_graph_info_get(vxp_system* s, vxp_graph_info* out) {
+  if (!s || !out) return VXP_ERR_ARGUMENT;
+  out->recorded_steps = s->rec_steps;
+  out->redos = s->rec_redos;
+  out->recordings = s->rec_records;
+  out->recording_ms = s->rec_ms;
+  out->cap_pairs = s->rc_pairs;
+  out->cap_world_pairs = s->rc_wp;
+  out->cap_work_items = s->rc_wi;
+  out->cap_candidates = s->rc_cand;
   return VXP_OK;
 }
 
@@ -1401,28 +1963,8 @@ int vxp_system_create(const vxp_config* cfg, uint32_t n_worlds, vxp_system** out
 int vxp_system_destroy(vxp_system* s) {
   if (!s) return VXP_ERR_ARGUMENT;
   (void)hipStreamSynchronize(s->stream);
-  DevBuf* bufs[] = {&s->d_hdr, &s->d_dir, &s->d_vox, &s->d_mask, &s->b_pos, &s->b_rot, &s->b_lv, &s->b_av,
-                    &s->b_fat_lo, &s->b_fat_hi, &s->b_inv_mass, &s->b_iil, &s->b_fric, &s->b_rest,
-                    &s->b_ldamp, &s->b_adamp, &s->b_sleep_timer, &s->b_type, &s->b_sleeping,
-                    &s->b_active, &s->b_collider, &s->b_id, &s->b_idmap, &s->d_ws, &s->d_stats,
-                    &s->d_cache_keys, &s->d_cache_vals, &s->d_skeys, &s->d_svals, &s->d_skeys2,
-                    &s->d_svals2, &s->d_sort_tmp, &s->d_slo, &s->d_shi, &s->d_send, &s->d_wcnt, &s->d_woff, &s->d_swc, &s->d_swo, &s->d_pairs_raw, &s->d_pairs,
-                    &s->d_pr, &s->d_counts, &s->d_wi_off, &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_wi_vchk, &s->d_lane_n,
-                    &s->d_cand, &s->d_produce, &s->d_pidx, &s->d_staging, &s->d_cstage, &s->d_rstat, &s->d_clist, &s->d_cflag, &s->d_cidx, &s->d_wlist, &s->d_step_ctr,
-                    &s->d_total, &s->d_ml[0], &s->d_ml[1], &s->d_isl_parent, &s->d_isl_end,
-                    &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors, &s->s_pos, &s->s_rot,
-                    &s->s_lv, &s->s_av, &s->s_ii, &s->s_adj_cnt, &s->s_adj_off, &s->s_adj,
-                    &s->s_color, &s->s_order, &s->s_rec, &s->s_pts, &s->s_prio, &s->s_tstamp,
-                    &s->s_upar, &s->s_ucnt, &s->s_uoff, &s->s_ubody, &s->s_uorder, &s->s_units,
-                    &s->s_uqueue, &s->s_uqueue2, &s->s_pcnt, &s->s_fq, &s->s_cend, &s->d_cb_pos, &s->d_cb_vox,
-                    &s->d_cb_cnt, &s->d_cb_bounds, &s->d_cb_box, &s->d_cb_out, &s->d_cb_solid,
-                    &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
-                    &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off,
-                    &s->d_t_pos, &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake,
-                    &s->d_arank, &s->d_drop, &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair,
-                    &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code, &s->d_lb_child, &s->d_lb_range,
-                    &s->d_lb_iparent, &s->d_lb_lparent, &s->d_lb_flag, &s->d_maxw};
-  for (DevBuf* b : bufs) b->release();
+  recRelease(s);
+  for (DevBuf* b : allBufs(s)) b->release();
   profFlush(s);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
   if (s->h_read) (void)hipHostFree(s->h_read);

@@ -18,6 +18,7 @@ constexpr uint32_t kCtrReduceWork = 0;  // pairs on the reduction work list
 constexpr uint32_t kUnitsPerWorld = 64;  // solve units per world, at most
 constexpr uint32_t kCtrSolveUnits = 1;  // solve units queued by the solver's prep
 constexpr uint32_t kCtrSolveUnitPull = 2;  // solve units taken by workgroups
+constexpr uint32_t kCtrChunkBound = 3;  // probe work items if every cache-eligible pair missed
 constexpr uint32_t kStepCounters = 4;
 
 // Collider header.  The brick directory is a dense grid over brick positions
@@ -63,7 +64,7 @@ struct DevCfg {
   uint8_t thread_multi;  // ... with 2+ threads (thread_count 0 or >= 2)
   uint8_t pad;
   uint32_t pair_color_max;  // colours of at most this many manifolds run on lane pairs
-  uint32_t unit_min;        // solve units hold at least this many manifolds (when split)
+  uint32_t n_cu;            // compute units (solve units: about two per CU over the batch)
 };
 
 // Body state, global slot g = world * max_bodies + slot.

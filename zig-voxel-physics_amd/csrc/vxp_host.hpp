@@ -163,6 +163,28 @@ struct vxp_system {
       s_pts, s_prio, s_tstamp, s_upar, s_ucnt, s_uoff, s_ubody, s_uorder, s_units, s_uqueue, s_uqueue2,
       s_pcnt, s_fq, s_cend;
   uint32_t n_pairs = 0, n_manifolds = 0;
+  uint32_t last_wi = 0, last_cand = 0;  // probe work items / candidates of the last step
+  // Recorded step (vxp_graph_enable): the step as three HIP graphs per
+  // manifold-buffer parity, sized for capacities (pairs, pairs of one world,
+  // probe work items, contact candidates) with the counts read on the device.
+  // Segment 1 ends with the counts that the capacities bound, read by the host
+  // before segment 2 (a step past a capacity is redone from the world-state
+  // and stats snapshot that segment 1 takes first); segment 2 ends with the
+  // candidate count (past its capacity: segment 3 is recorded again, larger).
+  struct RecGraph {
+    hipGraphExec_t g[3] = {nullptr, nullptr, nullptr};
+    uint64_t sig = 0;  // the buffers, shapes and capacities it was recorded with
+  };
+  bool rec_on = false;
+  RecGraph rec[2];
+  uint32_t rc_pairs = 0, rc_wp = 0, rc_wi = 0, rc_cand = 0, rc_carry = 0;
+  uint32_t rc_mm = 0;         // most manifolds of one world the recorded LDS requests assume
+  uint32_t last_max_man = 0;  // most manifolds of one world in the last step
+  uint64_t rec_steps = 0, rec_redos = 0, rec_records = 0;
+  double rec_ms = 0.0;  // host time spent recording
+  CopyDesc rec_desc[4] = {};  // the snapshot copies as last uploaded to d_rdesc
+  bool rec_nman_pending = false;  // n_manifolds awaits the last recorded step's count
+  DevBuf d_rsnap, d_rdesc, d_rsize;
   uint32_t* h_read = nullptr;  // pinned host words for the per-step size reads: [seq, a, b]
   uint32_t* d_read = nullptr;  // their device address
   uint32_t read_seq = 0;
@@ -234,6 +256,7 @@ int checkErrors(vxp_system* s);
 int streamWait(vxp_system* s);
 int readSizes(vxp_system* s, std::initializer_list<const uint32_t*> src, uint32_t* out);
 int updateOnce(vxp_system* s, float dt);
+int settleManifolds(vxp_system* s);  // n_manifolds after a recorded step
 }  // namespace vxh
 
 #define RC(x)            \

@@ -28,6 +28,12 @@ namespace vxd {
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
+// The item count of a launch: `n` (the grid's size), or min(*n_dev, n) when a
+// device word is given (launches recorded before the count exists: the grid
+// is sized for a capacity n).
+__device__ __forceinline__ uint32_t devCount(uint32_t n, const uint32_t* n_dev) {
+  return n_dev ? min(*n_dev, n) : n;
+}
 __device__ __forceinline__ V3 f4v(float4 a) { return V3{a.x, a.y, a.z}; }
 __device__ __forceinline__ Q4 f4q(float4 a) { return Q4{a.x, a.y, a.z, a.w}; }
 __device__ __forceinline__ float4 v4(V3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
@@ -507,7 +513,7 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(WorldState* ws, const float
                                                      const uint32_t* send, const uint32_t* woff_i,
                                                      uint32_t* wcount, const uint32_t* woff_w,
                                                      uint64_t* out, uint32_t id_bits, bool narrow,
-                                                     bool per_world) {
+                                                     bool per_world, uint32_t cap) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   const uint32_t total = woff_i[n_slots];
@@ -561,7 +567,10 @@ __global__ __launch_bounds__(256) void k_sweep_pairs(WorldState* ws, const float
           const uint64_t k =
               ((uint64_t)w << (2 * id_bits)) | ((uint64_t)min(idi, idj) << id_bits) | max(idi, idj);
           const uint32_t at = o + (uint32_t)__popcll(em & ((1ull << lane) - 1ull));
-          if (narrow)
+          if (at >= cap) {
+            // past the output's capacity (a recorded step sized from an earlier
+            // count): the overflow is detected from the count and the step redone
+          } else if (narrow)
             reinterpret_cast<uint32_t*>(out)[at] = (uint32_t)k;
           else
             out[at] = k;
@@ -1057,13 +1066,16 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
                                const DevManifold* prev, const uint8_t* drop, uint32_t* cflag,
                                uint32_t* produce, RStat* rstat, uint32_t* wlist,
-                               uint32_t* step_ctr, uint32_t* counts) {
-  __shared__ uint32_t s_n, s_base;
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+                               uint32_t* step_ctr, uint32_t* counts, const uint32_t* n_dev) {
+  __shared__ uint32_t s_n, s_base, s_chunks;
+  n_pairs = devCount(n_pairs, n_dev);
+  // block-strided (a grid smaller than the pairs, as in a recorded step, loops)
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n_pairs; base += gridDim.x * blockDim.x) {
+  const uint32_t p = base + threadIdx.x;
   PairCounts c{0, 0, 0};
   uint32_t w = 0;
   bool work = false;
-  if (threadIdx.x == 0) s_n = 0;
+  if (threadIdx.x == 0) s_n = s_chunks = 0;
   __syncthreads();
   uint32_t pos = 0;
   if (p < n_pairs) {
@@ -1079,15 +1091,23 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
     rstat[p] = RStat{0u, 0u, 0u, 0u, 0u, 0u};
     work = f == PF_CARRY || f == PF_DETECT || f == PF_CACHE;
     if (work) pos = atomicAdd(&s_n, 1u);
+    // work items of this pair if it runs the detection (a cache pair only on a
+    // miss): the step's bound on the probe's work items
+    if ((f == PF_DETECT || f == PF_CACHE) && pr[p].nchunks) atomicAdd(&s_chunks, pr[p].nchunks);
   }
   __syncthreads();
-  if (threadIdx.x == 0) s_base = s_n ? atomicAdd(&step_ctr[kCtrReduceWork], s_n) : 0;
+  if (threadIdx.x == 0) {
+    s_base = s_n ? atomicAdd(&step_ctr[kCtrReduceWork], s_n) : 0;
+    if (s_chunks) atomicAdd(&step_ctr[kCtrChunkBound], s_chunks);
+  }
   __syncthreads();
   if (work) wlist[s_base + pos] = p;
   const uint32_t v[3] = {c.bp, c.skipped, c.tested};
   uint32_t* const ctr[3] = {&st->broad_phase_pairs, &st->pairs_skipped_sleeping,
                             &st->narrow_phase_tested};
   blockAggAdd<3>(w, v, ctr, sizeof(vxp_stats) / 4);
+  __syncthreads();  // the next round reuses the block's LDS counters
+  }
 }
 
 
@@ -1139,9 +1159,10 @@ __device__ __forceinline__ uint32_t cacheSlot(uint32_t start, uint32_t lane, uin
 // holds pair order[i] (i itself in key order, order == nullptr); cflag and
 // cidx are indexed by position: clist[cidx[i]] = pair.
 __global__ void k_cache_scatter(const uint32_t* cflag, const uint32_t* cidx, const uint32_t* order,
-                                uint32_t n_pairs, uint32_t* clist) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_pairs && cflag[i]) clist[cidx[i]] = order ? order[i] : i;
+                                uint32_t n_pairs, uint32_t* clist, const uint32_t* n_dev) {
+  n_pairs = devCount(n_pairs, n_dev);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs; i += gridDim.x * blockDim.x)
+    if (cflag[i]) clist[cidx[i]] = order ? order[i] : i;
 }
 
 // One block per world: its kCacheThreads threads stage the key table in LDS
@@ -1300,11 +1321,12 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
 
 // The pair of every work item: wi_pair[wi_off[p] + c] = p for c < counts[p].
 __global__ void k_wi_fill(const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                          uint32_t* wi_pair) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
-  const uint32_t c = counts[p], b = wi_off[p];
-  for (uint32_t k = 0; k < c; k++) wi_pair[b + k] = p;
+                          uint32_t* wi_pair, const uint32_t* n_dev) {
+  n_pairs = devCount(n_pairs, n_dev);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+    const uint32_t c = counts[p], b = wi_off[p];
+    for (uint32_t k = 0; k < c; k++) wi_pair[b + k] = p;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1546,11 +1568,15 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
                                                const uint32_t* wi_off, uint32_t total_wi,
                                                const uint32_t* wi_cbase, uint32_t* wi_ccnt,
                                                uint32_t* wi_vchk, uint8_t* lane_n, Cand* cand,
-                                               uint32_t xcd_run) {
+                                               uint32_t xcd_run, const uint32_t* n_dev) {
   uint32_t lane = threadIdx.x & (kWave - 1);
-  uint32_t wid = xcdBlock(xcd_run) * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (wid >= total_wi) return;  // whole wave
-  if (!kCount && wi_ccnt[wid] == 0) return;
+  total_wi = devCount(total_wi, n_dev);
+  // one work item per wave; a grid smaller than the work items (a recorded
+  // step's persistent grid) loops, keeping runs of blocks on one XCD
+  const uint32_t wpb = blockDim.x / kWave;
+  for (uint32_t wid = xcdBlock(xcd_run) * wpb + threadIdx.x / kWave; wid < total_wi;
+       wid += gridDim.x * wpb) {
+  if (!kCount && wi_ccnt[wid] == 0) continue;
   uint32_t p = wi_pair[wid];
   const PairRec& r = pr[p];
   uint32_t chunk = wid - wi_off[p];
@@ -1583,6 +1609,7 @@ __global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColli
     if (n > 0)
       probeVoxel<true>(C.dir, C.vox, voxMap(HB), c, x, y, z, va, lane, nullptr,
                        cand + wi_cbase[wid] + (uint32_t)incl - n, n);
+  }
   }
 }
 
@@ -1818,8 +1845,10 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, DevColl
 // reserved it during the replay), and the per-world narrow-phase counters.
 __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const PairRec* pr,
                               uint32_t n_pairs, const CacheKeyRec* keys, CacheVal* vals,
-                              const CacheVal* cstage, const RStat* rstat) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+                              const CacheVal* cstage, const RStat* rstat, const uint32_t* n_dev) {
+  n_pairs = devCount(n_pairs, n_dev);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n_pairs; base += gridDim.x * blockDim.x) {
+  const uint32_t p = base + threadIdx.x;
   RStat c{0u, 0u, 0u, 0u, 0u, 0u};
   uint32_t w = 0;
   if (p < n_pairs) {
@@ -1839,6 +1868,8 @@ __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const P
   uint32_t* const c1[4] = {&st->voxels_checked, &st->narrow_phase_colliding,
                            &st->contacts_generated, &st->contact_candidates};
   blockAggAdd<4>(w, v1, c1, sizeof(vxp_stats) / 4);
+  __syncthreads();
+  }
 }
 
 // New manifold segments: skipped worlds keep their previous segment.
@@ -1877,12 +1908,14 @@ __global__ void k_world_segments(WorldState* ws, uint32_t n_worlds, uint32_t* to
 
 __global__ void k_scatter(const WorldState* ws, const PairRec* pr, uint32_t n_pairs,
                           const uint32_t* produce, const uint32_t* pidx,
-                          const DevManifold* staging, DevManifold* out) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pairs || !produce[p]) return;
-  uint32_t w = pr[p].world;
-  uint32_t dst = ws[w].man_start + pidx[p] - pidx[ws[w].pair_lo];
-  out[dst] = staging[p];
+                          const DevManifold* staging, DevManifold* out, const uint32_t* n_dev) {
+  n_pairs = devCount(n_pairs, n_dev);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+    if (!produce[p]) continue;
+    uint32_t w = pr[p].world;
+    uint32_t dst = ws[w].man_start + pidx[p] - pidx[ws[w].pair_lo];
+    out[dst] = staging[p];
+  }
 }
 
 __global__ void k_copy_skipped(const WorldState* ws, const DevManifold* prev, DevManifold* out) {
@@ -2687,7 +2720,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
                                                               DevManifold* ml, SolverScratch S,
                                                               uint32_t* prio_buf,
                                                               uint32_t* step_ctr,
-                                                              size_t lds_bytes) {
+                                                              size_t lds_bytes,
+                                                              const uint32_t* man_total) {
   const uint32_t w = blockIdx.x;
   if (ws[w].skip) return;
   const uint32_t tid = threadIdx.x, bs = blockDim.x;
@@ -2813,7 +2847,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_prep(DevCfg cfg, DevBod
   for (uint32_t s = tid; s < MB; s += bs) UO[s] = UC[s];
   __syncthreads();
   blockScanExclusive(UO, MB, tmp, nullptr);
-  const uint32_t T = max(max(cfg.unit_min, 1u), (nsolve + kUnitsMax - 1) / kUnitsMax);
+  // solve units of at least max(64, all manifolds / (2 CUs)) manifolds: about
+  // two per CU over the whole batch (results do not depend on it)
+  const uint32_t unit_min = max(64u, man_total[0] / (2u * max(cfg.n_cu, 1u)));
+  const uint32_t T = max(unit_min, (nsolve + kUnitsMax - 1) / kUnitsMax);
   for (uint32_t u = tid; u < kUnitsMax; u += bs) cnt_m[u] = cnt_b[u] = uid[u] = 0;
   __syncthreads();
   for (uint32_t s = tid; s < MB; s += bs)
@@ -3878,6 +3915,71 @@ void launch_publish(hipStream_t s, uint32_t* host, const uint32_t* a, const uint
                     uint32_t seq) {
   k_publish<<<1, 64, 0, s>>>(host, a, b, seq);
 }
+// host[1..4] = *w[0..3] (0 for a null word), then host[0] = seq.
+__global__ void k_publish4(uint32_t* host, const uint32_t* w0, const uint32_t* w1,
+                           const uint32_t* w2, const uint32_t* w3, uint32_t seq) {
+  if (threadIdx.x != 0) return;
+  host[1] = w0 ? *w0 : 0u;
+  host[2] = w1 ? *w1 : 0u;
+  host[3] = w2 ? *w2 : 0u;
+  host[4] = w3 ? *w3 : 0u;
+  __hip_atomic_store(&host[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+void launch_publish4(hipStream_t s, uint32_t* host, const uint32_t* w0, const uint32_t* w1,
+                     const uint32_t* w2, const uint32_t* w3, uint32_t seq) {
+  k_publish4<<<1, 64, 0, s>>>(host, w0, w1, w2, w3, seq);
+}
+// Device-side sizes of a recorded step: out[0] = the probe's work items
+// (wi_off[n - 1] + counts[n - 1], 0 without pairs), read after the chunk scan;
+// out[1] = the candidates (wi_cbase[out[0]]), read after the candidate scan.
+__global__ void k_rec_total_wi(const uint32_t* wi_off, const uint32_t* counts, const uint32_t* npairs,
+                               uint32_t* out) {
+  if (threadIdx.x != 0) return;
+  const uint32_t n = *npairs;
+  out[0] = n ? wi_off[n - 1] + counts[n - 1] : 0u;
+}
+// out[2] = the manifolds a step carries over unchanged: every manifold of the
+// worlds whose step is skipped (all bodies asleep), read after the step begins.
+__global__ __launch_bounds__(1024) void k_rec_carried(const WorldState* ws, uint32_t nw,
+                                                      uint32_t* out) {
+  __shared__ uint32_t s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  uint32_t c = 0;
+  for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x)
+    if (ws[w].skip) c += ws[w].prev_count;
+  c = waveSum(c);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(&s_sum, c);
+  __syncthreads();
+  if (threadIdx.x == 0) out[2] = s_sum;
+}
+void launch_rec_carried(hipStream_t s, const WorldState* ws, uint32_t nw, uint32_t* out) {
+  k_rec_carried<<<1, 1024, 0, s>>>(ws, nw, out);
+}
+// The pair-key sort of a recorded step runs both ways, each on a gated count:
+// out[3] = pairs for the device-wide sort (a world holds more than `limit`
+// pairs), out[4] = pairs for the per-world sort (otherwise), the other 0.
+__global__ void k_rec_gate(const uint32_t* npairs, const uint32_t* maxw, uint32_t limit,
+                           uint32_t* out) {
+  if (threadIdx.x != 0) return;
+  const bool big = *maxw > limit;
+  out[3] = big ? *npairs : 0u;
+  out[4] = big ? 0u : *npairs;
+}
+void launch_rec_gate(hipStream_t s, const uint32_t* npairs, const uint32_t* maxw, uint32_t limit,
+                     uint32_t* out) {
+  k_rec_gate<<<1, 64, 0, s>>>(npairs, maxw, limit, out);
+}
+__global__ void k_rec_total_cand(const uint32_t* wi_cbase, uint32_t* out) {
+  if (threadIdx.x == 0) out[1] = wi_cbase[out[0]];
+}
+void launch_rec_total_wi(hipStream_t s, const uint32_t* wi_off, const uint32_t* counts,
+                         const uint32_t* npairs, uint32_t* out) {
+  k_rec_total_wi<<<1, 64, 0, s>>>(wi_off, counts, npairs, out);
+}
+void launch_rec_total_cand(hipStream_t s, const uint32_t* wi_cbase, uint32_t* out) {
+  k_rec_total_cand<<<1, 64, 0, s>>>(wi_cbase, out);
+}
 
 void launch_tree_update(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
                         const TreeSet& T, const uint8_t* moved, uint32_t nw, size_t lds_limit) {
@@ -4012,7 +4114,8 @@ __global__ void k_world_pairs(WorldState* ws, uint32_t nw, const uint32_t* leaf_
 // same positions.  Requires every world's segment to fit kBS * kIPT.
 template <uint32_t kBS, uint32_t kIPT, class R>
 __global__ __launch_bounds__(kBS) void k_segment_sort(const R* raw, uint32_t n, uint32_t id_bits,
-                                                      uint64_t* keys) {
+                                                      uint64_t* keys, const uint32_t* n_dev) {
+  n = devCount(n, n_dev);
   __shared__ uint32_t s_k[kBS * kIPT];
   __shared__ uint32_t s_wcnt[2 * (kBS / 64) * kRadixBins], s_run[2 * kRadixBins];
   const uint32_t w = blockIdx.x, sh = 2 * id_bits;
@@ -4046,40 +4149,56 @@ __global__ __launch_bounds__(kBS) void k_segment_sort(const R* raw, uint32_t n, 
 }
 
 // Packed (world, a, b) keys -> the canonical (world << 44 | a << 22 | b).
-__global__ void k_expand_keys(uint64_t* keys, uint32_t n, uint32_t id_bits) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t k = keys[i], m = (1ull << id_bits) - 1ull;
-  keys[i] = makeKey((uint32_t)(k >> (2 * id_bits)), (uint32_t)((k >> id_bits) & m), (uint32_t)(k & m));
+__global__ void k_expand_keys(uint64_t* keys, uint32_t n, uint32_t id_bits, const uint32_t* n_dev) {
+  n = devCount(n, n_dev);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i], m = (1ull << id_bits) - 1ull;
+    keys[i] = makeKey((uint32_t)(k >> (2 * id_bits)), (uint32_t)((k >> id_bits) & m), (uint32_t)(k & m));
+  }
 }
-void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits) {
-  if (n) k_expand_keys<<<cdiv(n, 256), 256, 0, s>>>(keys, n, id_bits);
+// Blocks of a launch over n items (per items per block): all of them, or, for
+// a launch whose count is on the device (sized for a capacity n, as in a
+// recorded step), at most `most` blocks that loop over the items, so a large
+// capacity costs no idle blocks.
+static inline uint32_t gridFor(uint32_t n, uint32_t per, const uint32_t* n_dev, uint32_t most) {
+  const uint32_t all = cdiv(n, per);
+  return n_dev ? std::min(all, most) : all;
+}
+constexpr uint32_t kLoopBlocks = 16384;  // 64 blocks per CU
+
+void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits,
+                        const uint32_t* n_dev) {
+  if (n) k_expand_keys<<<gridFor(n, 256, n_dev, kLoopBlocks), 256, 0, s>>>(keys, n, id_bits, n_dev);
 }
 // The same from packed keys of 32 bits into a separate array.
-__global__ void k_expand_keys32(const uint32_t* in, uint64_t* keys, uint32_t n, uint32_t id_bits) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = in[i], m = (1u << id_bits) - 1u;
-  keys[i] = makeKey(k >> (2 * id_bits), (k >> id_bits) & m, k & m);
+__global__ void k_expand_keys32(const uint32_t* in, uint64_t* keys, uint32_t n, uint32_t id_bits,
+                                const uint32_t* n_dev) {
+  n = devCount(n, n_dev);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t k = in[i], m = (1u << id_bits) - 1u;
+    keys[i] = makeKey(k >> (2 * id_bits), (k >> id_bits) & m, k & m);
+  }
 }
 void launch_expand_keys32(hipStream_t s, const uint32_t* in, uint64_t* keys, uint32_t n,
-                          uint32_t id_bits) {
-  if (n) k_expand_keys32<<<cdiv(n, 256), 256, 0, s>>>(in, keys, n, id_bits);
+                          uint32_t id_bits, const uint32_t* n_dev) {
+  if (n)
+    k_expand_keys32<<<gridFor(n, 256, n_dev, kLoopBlocks), 256, 0, s>>>(in, keys, n, id_bits, n_dev);
 }
 void launch_world_pairs(hipStream_t s, WorldState* ws, uint32_t nw, const uint32_t* leaf_off,
                         uint32_t MB, uint32_t* max_out) {
   k_world_pairs<<<1, 1024, 0, s>>>(ws, nw, leaf_off, MB, max_out);
 }
 bool launch_segment_sort(hipStream_t s, const void* raw, bool narrow, uint32_t n, uint32_t nw,
-                         uint32_t id_bits, uint32_t max_world, uint64_t* keys) {
+                         uint32_t id_bits, uint32_t max_world, uint64_t* keys,
+                         const uint32_t* n_dev) {
   if (2 * id_bits > 32 || max_world > 16384) return false;
   if (!n) return true;
 #define VXP_SEGSORT(BS, IPT)                                                                   \
   do {                                                                                         \
     if (narrow)                                                                                \
-      k_segment_sort<BS, IPT, uint32_t><<<nw, BS, 0, s>>>((const uint32_t*)raw, n, id_bits, keys); \
+      k_segment_sort<BS, IPT, uint32_t><<<nw, BS, 0, s>>>((const uint32_t*)raw, n, id_bits, keys, n_dev); \
     else                                                                                       \
-      k_segment_sort<BS, IPT, uint64_t><<<nw, BS, 0, s>>>((const uint64_t*)raw, n, id_bits, keys); \
+      k_segment_sort<BS, IPT, uint64_t><<<nw, BS, 0, s>>>((const uint64_t*)raw, n, id_bits, keys, n_dev); \
   } while (0)
   if (max_world <= 1024) VXP_SEGSORT(256, 4);
   else if (max_world <= 2048) VXP_SEGSORT(256, 8);
@@ -4092,52 +4211,62 @@ bool launch_segment_sort(hipStream_t s, const void* raw, bool narrow, uint32_t n
 void launch_sweep_pairs(hipStream_t s, WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
-                        uint32_t id_bits, bool narrow, bool per_world) {
+                        uint32_t id_bits, bool narrow, bool per_world, uint32_t cap) {
   constexpr uint32_t blocks = kSweepWaves / 4;
   if (count)
     k_sweep_pairs<true><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount, woff_w,
-                                               out, id_bits, narrow, per_world);
+                                               out, id_bits, narrow, per_world, cap);
   else
     k_sweep_pairs<false><<<blocks, 256, 0, s>>>(ws, slo, shi, n_slots, send, woff_i, wcount,
-                                                woff_w, out, id_bits, narrow, false);
+                                                woff_w, out, id_bits, narrow, false, cap);
 }
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
-                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts) {
+                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
+                         const uint32_t* n_dev) {
   if (n)
-    k_pair_process<<<cdiv(n, 128), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag,
-                                                produce, rstat, wlist, step_ctr, counts);
+    k_pair_process<<<gridFor(n, 128, n_dev, kLoopBlocks), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag,
+                                                produce, rstat, wlist, step_ctr, counts, n_dev);
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
                          const uint32_t* cidx, const uint32_t* order, uint32_t* clist,
-                         uint32_t n_pairs, uint32_t nw, uint32_t* counts) {
+                         uint32_t n_pairs, uint32_t nw, uint32_t* counts, const uint32_t* n_dev) {
   if (n_pairs)
-    k_cache_scatter<<<cdiv(n_pairs, 256), 256, 0, s>>>(cflag, cidx, order, n_pairs, clist);
+    k_cache_scatter<<<gridFor(n_pairs, 256, n_dev, kLoopBlocks), 256, 0, s>>>(cflag, cidx, order,
+                                                                             n_pairs, clist, n_dev);
   k_cache_replay<<<nw, kCacheThreads, cfg.cache_cap * sizeof(CacheKeyRec), s>>>(
       cfg, ws, st, pr, keys, cidx, clist, counts, nw);
 }
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                    uint32_t* wi_pair) {
-  if (n_pairs) k_wi_fill<<<cdiv(n_pairs, 256), 256, 0, s>>>(counts, wi_off, n_pairs, wi_pair);
+                    uint32_t* wi_pair, const uint32_t* n_dev) {
+  if (n_pairs)
+    k_wi_fill<<<gridFor(n_pairs, 256, n_dev, kLoopBlocks), 256, 0, s>>>(counts, wi_off, n_pairs,
+                                                                        wi_pair, n_dev);
 }
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                   const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
                   uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape) {
+                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape,
+                  const uint32_t* n_dev) {
   if (!total_wi) return;
   const uint32_t threads = shape.threads, waves = threads / kWave;
   const uint32_t run = shape.xcd_run ? shape.xcd_run : 1u;
+  // a recorded step's grid: four rounds of 28 waves per CU (7 per SIMD, the
+  // probe's occupancy), a whole number of XCD rounds of runs when it can be
+  uint32_t most = 4 * 256 * 28 / waves;
+  if (most >= kXcds * run) most -= most % (kXcds * run);
+  const uint32_t blocks = gridFor(total_wi, waves, n_dev, most);
   if (count)
-    k_probe<true><<<cdiv(total_wi, waves), threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
+    k_probe<true><<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
                                                             total_wi, wi_cbase, wi_ccnt, wi_vchk,
-                                                            lane_n, cand, run);
+                                                            lane_n, cand, run, n_dev);
   else
-    k_probe<false><<<cdiv(total_wi, waves), threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
+    k_probe<false><<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
                                                              total_wi, wi_cbase, wi_ccnt, wi_vchk,
-                                                             lane_n, cand, run);
+                                                             lane_n, cand, run, n_dev);
 }
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
@@ -4146,20 +4275,24 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const D
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, RStat* rstat) {
+                   DevManifold* staging, uint32_t* produce, RStat* rstat,
+                   const uint32_t* n_dev) {
   if (!n) return;
   uint32_t blocks = std::min<uint32_t>(2048, cdiv(n, 4));
   k_reduce<<<blocks, 256, 0, s>>>(cfg, B, C, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase, wi_ccnt,
                                   wi_vchk, cand, prev, cvals, cstage, staging, produce, rstat);
-  k_reduce_tail<<<cdiv(n, 256), 256, 0, s>>>(cfg, ws, st, pr, n, ckeys, cvals, cstage, rstat);
+  k_reduce_tail<<<gridFor(n, 256, n_dev, kLoopBlocks), 256, 0, s>>>(cfg, ws, st, pr, n, ckeys, cvals,
+                                                                     cstage, rstat, n_dev);
 }
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total) {
   k_world_segments<<<1, 1024, 0, s>>>(ws, nw, total);
 }
 void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
                     const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
-                    DevManifold* out) {
-  if (n) k_scatter<<<cdiv(n, 256), 256, 0, s>>>(ws, pr, n, produce, pidx, staging, out);
+                    DevManifold* out, const uint32_t* n_dev) {
+  if (n)
+    k_scatter<<<gridFor(n, 256, n_dev, kLoopBlocks), 256, 0, s>>>(ws, pr, n, produce, pidx, staging,
+                                                                  out, n_dev);
 }
 void launch_copy_skipped(hipStream_t s, const WorldState* ws, const DevManifold* prev,
                          DevManifold* out, uint32_t nw) {
@@ -4182,10 +4315,10 @@ void launch_body_state(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
 void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
                        uint32_t* step_ctr, uint32_t nw, uint32_t max_manifolds,
-                       size_t lds_limit) {
+                       size_t lds_limit, const uint32_t* man_total) {
   size_t lds = std::min(lds_limit, std::max((size_t)max_manifolds * kColourLdsPerManifold,
                                             (size_t)cfg.max_bodies * 8));
-  k_solve_prep<<<nw, kSolveThreads, lds, s>>>(cfg, B, ws, st, ml, S, prio, step_ctr, lds);
+  k_solve_prep<<<nw, kSolveThreads, lds, s>>>(cfg, B, ws, st, ml, S, prio, step_ctr, lds, man_total);
 }
 // Longest units first: the queue is reordered by descending power-of-two
 // bucket of manifold count (order inside a bucket is free), so that the

@@ -29,13 +29,18 @@ void launch_world_pairs(hipStream_t s, WorldState* ws, uint32_t nw, const uint32
 // Each world's packed pair keys (raw: 32-bit when narrow) sorted in one
 // workgroup into canonical keys; false (nothing launched) when a world holds
 // more than 16384 pairs or the ids need more than 32 key bits.
+//
+// Launchers that take `const uint32_t* n_dev` size their grid for n items and
+// act on min(*n_dev, n) when n_dev is given (a step recorded for replay, whose
+// counts exist only on the device); nullptr: exactly n.
 bool launch_segment_sort(hipStream_t s, const void* raw, bool narrow, uint32_t n, uint32_t nw,
-                         uint32_t id_bits, uint32_t max_world, uint64_t* keys);
+                         uint32_t id_bits, uint32_t max_world, uint64_t* keys,
+                         const uint32_t* n_dev = nullptr);
 void launch_sweep_pairs(hipStream_t s, WorldState* ws, const float4* slo, const float4* shi,
                         uint32_t n_slots, const uint32_t* send, const uint32_t* woff_i,
                         uint32_t* wcount, const uint32_t* woff_w, uint64_t* out, bool count,
                         uint32_t id_bits, bool narrow,
-                        bool per_world);
+                        bool per_world, uint32_t cap = 0xFFFFFFFFu);  // cap: output slots
 // LBVH broad phase (per-world trees at each world's slot offsets).
 struct LbvhBufs {
   float4 *llo, *lhi, *ilo, *ihi;  // leaves in Morton order; internal nodes
@@ -57,24 +62,39 @@ void launch_lbvh_build(hipStream_t s, const DevCfg& cfg, const DevBodies& B, con
 void launch_lbvh_pairs(hipStream_t s, const DevCfg& cfg, WorldState* ws, uint32_t n_slots,
                        const LbvhBufs& L, uint32_t* cnt, const uint32_t* off, uint64_t* out,
                        uint32_t id_bits, bool narrow);
-void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits);
+void launch_expand_keys(hipStream_t s, uint64_t* keys, uint32_t n, uint32_t id_bits,
+                        const uint32_t* n_dev = nullptr);
 void launch_expand_keys32(hipStream_t s, const uint32_t* in, uint64_t* keys, uint32_t n,
-                          uint32_t id_bits);
+                          uint32_t id_bits, const uint32_t* n_dev = nullptr);
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
-                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts);
+                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
+                         const uint32_t* n_dev = nullptr);
 // counts[p] (the probe's work items of pair p) is set for the misses.
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
                          const uint32_t* cidx, const uint32_t* order, uint32_t* clist,
-                         uint32_t n_pairs, uint32_t nw, uint32_t* counts);
+                         uint32_t n_pairs, uint32_t nw, uint32_t* counts,
+                         const uint32_t* n_dev = nullptr);
 // Broad-phase trees (pair order VXP_PAIR_ORDER_TREE): per world, re-insert the
 // leaves of the dynamic tree whose fat box moved, in slot order.
 // host[1..2] = *a, *b (0 when null), then host[0] = seq (host-visible memory).
 void launch_publish(hipStream_t s, uint32_t* host, const uint32_t* a, const uint32_t* b,
                     uint32_t seq);
+void launch_publish4(hipStream_t s, uint32_t* host, const uint32_t* w0, const uint32_t* w1,
+                     const uint32_t* w2, const uint32_t* w3, uint32_t seq);
+// Recorded step sizes: out[0] = probe work items (after the chunk scan),
+// out[1] = contact candidates (after the candidate scan).
+void launch_rec_total_wi(hipStream_t s, const uint32_t* wi_off, const uint32_t* counts,
+                         const uint32_t* npairs, uint32_t* out);
+void launch_rec_total_cand(hipStream_t s, const uint32_t* wi_cbase, uint32_t* out);
+// out[3] = npairs if a world holds more than `limit` pairs (else 0), out[4] the reverse.
+void launch_rec_gate(hipStream_t s, const uint32_t* npairs, const uint32_t* maxw, uint32_t limit,
+                     uint32_t* out);
+// out[2] = manifolds carried over by the skipped worlds (after the step begins).
+void launch_rec_carried(hipStream_t s, const WorldState* ws, uint32_t nw, uint32_t* out);
 void launch_tree_update(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const WorldState* ws,
                         const TreeSet& T, const uint8_t* moved, uint32_t nw, size_t lds_limit);
 // Pairs in tree order: count (out == nullptr) or write packed keys
@@ -101,7 +121,7 @@ void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx,
 void launch_invert(hipStream_t s, const uint32_t* perm, uint32_t n, uint32_t* inv);
 // wi_pair[w] = the pair of work item w (after the scan of the chunk counts)
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                    uint32_t* wi_pair);
+                    uint32_t* wi_pair, const uint32_t* n_dev = nullptr);
 // Launch shape of the probe: threads per workgroup (64, 128 or 256; one work
 // item per wave) and the run of consecutive workgroups kept on one XCD.
 struct ProbeShape {
@@ -110,7 +130,8 @@ struct ProbeShape {
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                   const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
                   uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape);
+                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape,
+                  const uint32_t* n_dev = nullptr);
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
@@ -118,11 +139,12 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const D
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, RStat* rstat);
+                   DevManifold* staging, uint32_t* produce, RStat* rstat,
+                   const uint32_t* n_dev = nullptr);
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total);
 void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
                     const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
-                    DevManifold* out);
+                    DevManifold* out, const uint32_t* n_dev = nullptr);
 void launch_copy_skipped(hipStream_t s, const WorldState* ws, const DevManifold* prev,
                          DevManifold* out, uint32_t nw);
 void launch_islands(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
@@ -138,7 +160,7 @@ void launch_body_state(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
 void launch_solve_prep(hipStream_t s, const DevCfg& cfg, const DevBodies& B, WorldState* ws,
                        vxp_stats* st, DevManifold* ml, const SolverScratch& S, uint32_t* prio,
                        uint32_t* step_ctr, uint32_t nw, uint32_t max_manifolds,
-                       size_t lds_limit);
+                       size_t lds_limit, const uint32_t* man_total);  // man_total: all manifolds
 // Reorders the solve-unit queue, largest units first (tmp: queue-sized scratch).
 void launch_unit_order(hipStream_t s, const SolverScratch& S, const uint32_t* step_ctr,
                        uint32_t* tmp);
@@ -174,6 +196,13 @@ size_t sort_keys_u32_bytes(uint32_t n, uint32_t end_bit);
 hipError_t sort_keys_u32(void* tmp, size_t bytes, const uint32_t* kin, uint32_t* kout, uint32_t n,
                          uint32_t end_bit, hipStream_t s);
 size_t exclusive_scan_u32_bytes(uint32_t n);
+// Device-count forms (sized for `cap` items, acting on min(*n_dev + n_add, cap)).
+hipError_t exclusive_scan_u32_n(void* tmp, const uint32_t* in, uint32_t* out, uint32_t cap,
+                                const uint32_t* n_dev, uint32_t n_add, hipStream_t s);
+hipError_t sort_keys_u64_n(void* tmp, size_t bytes, const uint64_t* kin, uint64_t* kout,
+                           uint32_t cap, const uint32_t* n_dev, uint32_t end_bit, hipStream_t s);
+hipError_t sort_keys_u32_n(void* tmp, size_t bytes, const uint32_t* kin, uint32_t* kout,
+                           uint32_t cap, const uint32_t* n_dev, uint32_t end_bit, hipStream_t s);
 hipError_t exclusive_scan_u32(void* tmp, size_t bytes, const uint32_t* in, uint32_t* out,
                               uint32_t n, hipStream_t s);
 

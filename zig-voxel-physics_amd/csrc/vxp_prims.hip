@@ -34,6 +34,18 @@ constexpr uint32_t kSortT = 256, kSortIPT = 8, kSortTile = kSortT * kSortIPT;
 constexpr uint32_t kDevMaxBits = 9, kDevBins = 1u << kDevMaxBits;  // digits of the device-wide sort
 
 static inline uint32_t cdivU(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+// The item count of a launch: `cap` (the grid's size), or, when a device word
+// is given (launches whose count is known only on the device, as in a
+// captured step), min(*n_dev + n_add, cap).
+__device__ __forceinline__ uint32_t itemCount(uint32_t cap, const uint32_t* n_dev, uint32_t n_add) {
+  return n_dev ? min(*n_dev + n_add, cap) : cap;
+}
+__device__ __forceinline__ uint32_t cdivD(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+// The words of a sort's digit table for a device count: bins x tiles of the
+// actual items (the table scan's count).
+__global__ void k_table_count(const uint32_t* n_dev, uint32_t cap, uint32_t* out) {
+  if (threadIdx.x == 0) out[0] = kDevBins * cdivD(min(*n_dev, cap), kSortTile);
+}
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Exclusive scan of one value per thread over the block; *total = the sum.
@@ -64,8 +76,10 @@ __device__ __forceinline__ uint32_t blockExclusive(uint32_t v, uint32_t* s_wave,
 // place is allowed: a tile is read whole before it is written.
 // Sum of each tile of `in` (zero past n).
 __global__ __launch_bounds__(kScanT) void k_scan_sums(const uint32_t* __restrict__ in, uint32_t n,
-                                                      uint32_t* __restrict__ sums) {
+                                                      uint32_t* __restrict__ sums,
+                                                      const uint32_t* n_dev, uint32_t n_add) {
   __shared__ uint32_t s_wave[kScanT / 64];
+  n = itemCount(n, n_dev, n_add);
   const size_t base = (size_t)blockIdx.x * kScanTile;
   uint32_t v = 0;
 #pragma unroll
@@ -79,9 +93,11 @@ __global__ __launch_bounds__(kScanT) void k_scan_sums(const uint32_t* __restrict
 }
 
 __global__ __launch_bounds__(kScanT) void k_scan_tile(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                      const uint32_t* __restrict__ sums) {
+                                                      const uint32_t* __restrict__ sums,
+                                                      const uint32_t* n_dev, uint32_t n_add) {
   __shared__ uint32_t s_d[kScanTile + kScanTile / 32];
   __shared__ uint32_t s_wave[kScanT / 64];
+  n = itemCount(n, n_dev, n_add);
   const uint32_t tid = threadIdx.x;
   const size_t base = (size_t)blockIdx.x * kScanTile;
   uint32_t offset = 0;
@@ -128,8 +144,16 @@ template <class K>
 __global__ __launch_bounds__(kSortT) void k_radix_hist(const K* __restrict__ keys, uint32_t n,
                                                        uint32_t shift, uint32_t bits,
                                                        uint32_t n_tiles,
-                                                       uint32_t* __restrict__ hist) {
+                                                       uint32_t* __restrict__ hist,
+                                                       const uint32_t* n_dev) {
   __shared__ uint32_t s_h[kDevBins];
+  n = itemCount(n, n_dev, 0);
+  // with a device count the table holds the tiles of the actual items only
+  // (digit-major over those tiles); the grid's further tiles have no column
+  if (n_dev) {
+    n_tiles = cdivD(n, kSortTile);
+    if (blockIdx.x >= n_tiles) return;
+  }
   const uint32_t tid = threadIdx.x, mask = (1u << bits) - 1u;
   for (uint32_t d = tid; d < kDevBins; d += kSortT) s_h[d] = 0;
   __syncthreads();
@@ -153,10 +177,16 @@ __global__ __launch_bounds__(kSortT) void k_radix_scatter(const K* __restrict__ 
                                                           K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                           uint32_t n, uint32_t shift, uint32_t bits,
                                                           uint32_t n_tiles,
-                                                          const uint32_t* __restrict__ off) {
+                                                          const uint32_t* __restrict__ off,
+                                                          const uint32_t* n_dev) {
   __shared__ uint32_t s_wcnt[2 * (kSortT / 64) * kDevBins];
   __shared__ uint32_t s_run[2 * kDevBins];
   const uint32_t tid = threadIdx.x, mask = (1u << bits) - 1u;
+  n = itemCount(n, n_dev, 0);
+  if (n_dev) {
+    n_tiles = cdivD(n, kSortTile);
+    if (blockIdx.x >= n_tiles) return;
+  }
   for (uint32_t d = tid; d < 2 * (kSortT / 64) * kDevBins; d += kSortT) s_wcnt[d] = 0;
   for (uint32_t d = tid; d < kDevBins; d += kSortT)
     s_run[d] = off[(size_t)d * n_tiles + blockIdx.x];  // digit d's first output slot
@@ -185,16 +215,18 @@ __global__ __launch_bounds__(kSortT) void k_radix_scatter(const K* __restrict__ 
 
 static size_t scanTmpBytes(uint32_t n) { return align256((size_t)cdivU(n, kScanTile) * 4); }
 
-static hipError_t scanImpl(void* tmp, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t s) {
+// n: the item count, or with n_dev the capacity the launch is sized for.
+static hipError_t scanImpl(void* tmp, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t s,
+                           const uint32_t* n_dev = nullptr, uint32_t n_add = 0) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = cdivU(n, kScanTile);
   if (nb == 1) {
-    k_scan_tile<<<1, kScanT, 0, s>>>(in, out, n, nullptr);
+    k_scan_tile<<<1, kScanT, 0, s>>>(in, out, n, nullptr, n_dev, n_add);
     return hipGetLastError();
   }
   uint32_t* sums = (uint32_t*)tmp;
-  k_scan_sums<<<nb, kScanT, 0, s>>>(in, n, sums);
-  k_scan_tile<<<nb, kScanT, 0, s>>>(in, out, n, sums);
+  k_scan_sums<<<nb, kScanT, 0, s>>>(in, n, sums, n_dev, n_add);
+  k_scan_tile<<<nb, kScanT, 0, s>>>(in, out, n, sums, n_dev, n_add);
   return hipGetLastError();
 }
 
@@ -212,12 +244,13 @@ static size_t sortTmpBytes(uint32_t n, uint32_t end_bit, bool vals) {
   const uint32_t tiles = std::max(1u, cdivU(n, kSortTile));
   const size_t table = (size_t)kDevBins * tiles;
   return align256(table * 4) + scanTmpBytes((uint32_t)table) +
-         align256((size_t)n * sizeof(K)) + (vals ? align256((size_t)n * 4) : 0);
+         align256((size_t)n * sizeof(K)) + (vals ? align256((size_t)n * 4) : 0) + 256;
 }
 
 template <class K, bool kVals>
 static hipError_t sortImpl(void* tmp, size_t bytes, const K* kin, K* kout, const uint32_t* vin,
-                           uint32_t* vout, uint32_t n, uint32_t end_bit, hipStream_t s) {
+                           uint32_t* vout, uint32_t n, uint32_t end_bit, hipStream_t s,
+                           const uint32_t* n_dev = nullptr) {
   if (bytes < sortTmpBytes<K>(n, end_bit, kVals)) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
   const uint32_t tiles = cdivU(n, kSortTile);
@@ -232,6 +265,12 @@ static hipError_t sortImpl(void* tmp, size_t bytes, const K* kin, K* kout, const
   K* kalt = (K*)p;
   p += align256((size_t)n * sizeof(K));
   uint32_t* valt = kVals ? (uint32_t*)p : nullptr;
+  p += kVals ? align256((size_t)n * 4) : 0;
+  uint32_t* table_n = nullptr;  // the digit table's words for a device count
+  if (n_dev) {
+    table_n = (uint32_t*)p;
+    k_table_count<<<1, 64, 0, s>>>(n_dev, n, table_n);
+  }
   // the last pass must write kout: start from kout or the spare array so that
   // the passes alternate into it
   K* ka = (passes & 1) ? kout : kalt;
@@ -245,13 +284,14 @@ static hipError_t sortImpl(void* tmp, size_t bytes, const K* kin, K* kout, const
     return sh >= end_bit ? 1u : std::min(width, end_bit - sh);
   };
   for (uint32_t pass = 0; pass < passes; pass++) {
-    k_radix_hist<K><<<tiles, kSortT, 0, s>>>(src, n, pass * width, bitsOf(pass), tiles, hist);
-    hipError_t e = scanImpl(scan_tmp, hist, hist, table, s);
+    k_radix_hist<K><<<tiles, kSortT, 0, s>>>(src, n, pass * width, bitsOf(pass), tiles, hist,
+                                             n_dev);
+    hipError_t e = scanImpl(scan_tmp, hist, hist, table, s, table_n, 0);
     if (e != hipSuccess) return e;
     K* dst = (pass & 1) ? kb : ka;
     uint32_t* vdst = (pass & 1) ? vb : va;
     k_radix_scatter<K, kVals><<<tiles, kSortT, 0, s>>>(src, vsrc, dst, vdst, n, pass * width,
-                                                       bitsOf(pass), tiles, hist);
+                                                       bitsOf(pass), tiles, hist, n_dev);
     src = dst;
     vsrc = vdst;
   }
@@ -282,6 +322,21 @@ hipError_t sort_keys_u32(void* tmp, size_t bytes, const uint32_t* kin, uint32_t*
   return sortImpl<uint32_t, false>(tmp, bytes, kin, kout, nullptr, nullptr, n, end_bit, s);
 }
 size_t exclusive_scan_u32_bytes(uint32_t n) { return scanTmpBytes(n); }
+// The device-count forms: sized for `cap` items, they act on min(*n_dev
+// (+ n_add), cap) items, so a launch sequence can be recorded before the
+// counts exist.
+hipError_t exclusive_scan_u32_n(void* tmp, const uint32_t* in, uint32_t* out, uint32_t cap,
+                                const uint32_t* n_dev, uint32_t n_add, hipStream_t s) {
+  return scanImpl(tmp, in, out, cap, s, n_dev, n_add);
+}
+hipError_t sort_keys_u64_n(void* tmp, size_t bytes, const uint64_t* kin, uint64_t* kout,
+                           uint32_t cap, const uint32_t* n_dev, uint32_t end_bit, hipStream_t s) {
+  return sortImpl<uint64_t, false>(tmp, bytes, kin, kout, nullptr, nullptr, cap, end_bit, s, n_dev);
+}
+hipError_t sort_keys_u32_n(void* tmp, size_t bytes, const uint32_t* kin, uint32_t* kout,
+                           uint32_t cap, const uint32_t* n_dev, uint32_t end_bit, hipStream_t s) {
+  return sortImpl<uint32_t, false>(tmp, bytes, kin, kout, nullptr, nullptr, cap, end_bit, s, n_dev);
+}
 hipError_t exclusive_scan_u32(void* tmp, size_t /*bytes*/, const uint32_t* in, uint32_t* out,
                               uint32_t n, hipStream_t s) {
   return scanImpl(tmp, in, out, n, s);

@@ -156,6 +156,7 @@ int admit(vxp_server* v, const std::vector<std::pair<uint32_t, uint32_t>>& moves
   std::vector<WorldState> wsrec;
   std::vector<WorldState*> wsdst;
   std::vector<uint32_t> rekey;  // (begin, count, slot) triples
+  RC(settleManifolds(b));
   uint32_t total = b->n_manifolds;
   size_t add = 0;
   for (auto& m : moves) add += v->res[m.first].mcount;
@@ -340,6 +341,16 @@ int vxp_server_submit(vxp_server* v, uint32_t w, uint32_t frames, uint64_t tag, 
   Req r{tag, w, frames, frames, submit_s < 0.0 ? nowS() : submit_s, 0.0, 0};
   v->queue.push_back(r);
   return VXP_OK;
+}
+
+int vxp_server_graph_enable(vxp_server* v, int on) {
+  if (!v) return VXP_ERR_ARGUMENT;
+  return vxp_graph_enable(v->batch, on);
+}
+
+int vxp_server_graph_info_get(vxp_server* v, vxp_graph_info* out) {
+  if (!v) return VXP_ERR_ARGUMENT;
+  return vxp_graph_info_get(v->batch, out);
 }
 
 int vxp_server_tick(vxp_server* v, uint32_t* n_done) {

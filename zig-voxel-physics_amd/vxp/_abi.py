@@ -278,8 +278,19 @@ SIGNATURES = {
 }
 
 # product-only exports (declared in include/vxp.h)
+class GraphInfo(C.Structure):
+    _fields_ = [("recorded_steps", C.c_uint64), ("redos", C.c_uint64), ("recordings", C.c_uint64),
+                ("recording_ms", C.c_double), ("cap_pairs", C.c_uint32), ("cap_world_pairs", C.c_uint32),
+                ("cap_work_items", C.c_uint32), ("cap_candidates", C.c_uint32)]
+
+
 PRODUCT_ONLY = {
     "config_default": (C.c_int, [C.POINTER(Config)]),
+    # recorded step (HIP graphs)
+    "graph_enable": (C.c_int, [_P, C.c_int]),
+    "graph_info_get": (C.c_int, [_P, C.POINTER(GraphInfo)]),
+    "server_graph_enable": (C.c_int, [_P, C.c_int]),
+    "server_graph_info_get": (C.c_int, [_P, C.POINTER(GraphInfo)]),
     "profile_enable": (C.c_int, [_P, C.c_int]),
     "profile_get": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "profile_solver_phases": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),

@@ -106,15 +106,18 @@ def diff_stats(sa, sb):
 
 
 def lockstep(scene, lib_a, lib_b, frames, cfg_over=None, worlds=None, check_every=1,
-             rtol=0.0, atol=0.0, log=None, dts=None, tune_a=None):
+             rtol=0.0, atol=0.0, log=None, dts=None, tune_a=None, prep_a=None):
     """Step both systems `frames` times; return (first_frame_with_diff, message) or (None, None).
     `dts`: the update() argument of frame f is dts[f % len(dts)] (default: the
     scene's dt, one fixed step per frame).  `tune_a`: setTuning arguments of
-    the first system (the product)."""
+    the first system (the product); `prep_a`: called with the first system
+    before the first frame."""
     sa = from_scene(scene, lib=lib_a, cfg_over=cfg_over, worlds=worlds)
     sb = from_scene(scene, lib=lib_b, cfg_over=cfg_over, worlds=worlds)
     if tune_a:
         sa.setTuning(**tune_a)
+    if prep_a:
+        prep_a(sa)
     nwl = sa.n_worlds
     for f in range(frames):
         dt = dts[f % len(dts)] if dts else scene.dt

@@ -61,6 +61,15 @@ class Server:
     def submit(self, world, frames, tag, submit_s=-1.0):
         self.lib.check(self.lib.server_submit(self.ptr, world, frames, tag, submit_s), "server_submit")
 
+    def set_recorded(self, on=True):
+        """Step the batch from recorded HIP graphs (vxp_server_graph_enable)."""
+        self.lib.check(self.lib.server_graph_enable(self.ptr, 1 if on else 0), "server_graph_enable")
+
+    def graph_info(self):
+        g = _abi.GraphInfo()
+        self.lib.check(self.lib.server_graph_info_get(self.ptr, C.byref(g)), "server_graph_info_get")
+        return {f: getattr(g, f) for f, _ in g._fields_}
+
     def tick(self):
         n = C.c_uint32()
         self.lib.check(self.lib.server_tick(self.ptr, C.byref(n)), "server_tick")

@@ -60,6 +60,17 @@ class VoxelPhysicsSystem:
         except Exception:
             pass
 
+    # ---- recorded steps (product only; results never depend on it) ------------
+    def setRecorded(self, on=True):
+        """Run the default pipeline's steps from recorded HIP graphs (three per
+        step, sized for capacities; the host reads the counts twice per step)."""
+        self.lib.check(self.lib.graph_enable(self.ptr, 1 if on else 0), "graph_enable")
+
+    def graphInfo(self):
+        g = _abi.GraphInfo()
+        self.lib.check(self.lib.graph_info_get(self.ptr, C.byref(g)), "graph_info_get")
+        return {f: getattr(g, f) for f, _ in g._fields_}
+
     # ---- launch tuning (product only; results never depend on it) -------------
     def tuning(self):
         t = _abi.Tuning()
