@@ -957,7 +957,8 @@ static int stepAllImpl(vxp_system* s) {
   uint32_t ncand = 0;
   if (total_wi) {
     launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), npairs,
-                   s->d_wi_pair.as<uint32_t>());
+                   s->d_wi_pair.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
+                   s->d_wi_vchk.as<uint32_t>());
     launch_probe(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), total_wi, nullptr,
                  s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
                  nullptr, true, shape);
@@ -1420,7 +1421,8 @@ static int recSeg2(vxp_system* s, const RecCtx& c) {
   const ProbeShape shape{s->tune.use[vxp_system::kProbeThreads],
                          s->tune.use[vxp_system::kProbeXcdRun]};
   launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), c.P,
-                 s->d_wi_pair.as<uint32_t>(), c.npairs);
+                 s->d_wi_pair.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
+                 s->d_wi_vchk.as<uint32_t>(), c.npairs);
   launch_probe(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
                c.WI, nullptr, s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
                s->d_lane_n.as<uint8_t>(), nullptr, true, shape, c.rsize);

@@ -1321,11 +1321,16 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
 
 // The pair of every work item: wi_pair[wi_off[p] + c] = p for c < counts[p].
 __global__ void k_wi_fill(const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                          uint32_t* wi_pair, const uint32_t* n_dev) {
+                          uint32_t* wi_pair, uint32_t* wi_ccnt, uint32_t* wi_vchk,
+                          const uint32_t* n_dev) {
   n_pairs = devCount(n_pairs, n_dev);
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
     const uint32_t c = counts[p], b = wi_off[p];
-    for (uint32_t k = 0; k < c; k++) wi_pair[b + k] = p;
+    for (uint32_t k = 0; k < c; k++) {
+      wi_pair[b + k] = p;
+      wi_ccnt[b + k] = 0;
+      wi_vchk[b + k] = 0;
+    }
   }
 }
 
@@ -1336,12 +1341,47 @@ __global__ void k_wi_fill(const uint32_t* counts, const uint32_t* wi_off, uint32
 // Candidates of a chunk are written contiguously in (lane, probe) order, so
 // chunk order + in-chunk order is the reference's candidate sequence.
 // ---------------------------------------------------------------------------
+// A rotation as qrotate applies it (src/math.zig:267-275): the quaternion
+// after qrotate's norm check, and 2w.  Derived once per pair, xformBy and
+// invXformBy give the bits of xformPoint / invXformPoint (src/math.zig) with
+// the body's quaternion, without the per-point normalisation (a square root
+// and a division of uniform values).
+struct Rot {
+  V3 v;
+  float w2;
+};
+__device__ __forceinline__ Rot rotOf(Q4 q) {
+  float ls = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+  if (ls < 0.99f || ls > 1.01f) q = qnormalize(q);
+  return Rot{V3{q.x, q.y, q.z}, 2.0f * q.w};
+}
+__device__ __forceinline__ V3 rotApply(const Rot& r, V3 v) {
+  V3 uv = cross(r.v, v);
+  V3 uuv = cross(r.v, uv);
+  return add(add(v, scale(uv, r.w2)), scale(uuv, 2.0f));
+}
+// xformPoint(pos, rot, p) = qrotate(qnormalize(rot), p) + pos
+__device__ __forceinline__ V3 xformBy(const Rot& r, V3 pos, V3 p) {
+  return add(rotApply(r, p), pos);
+}
+// invXformPoint(pos, rot, p) = qrotate(qconj(rot), p - pos)
+__device__ __forceinline__ V3 invXformBy(const Rot& r, V3 pos, V3 p) {
+  return rotApply(r, sub(p, pos));
+}
+
 struct ProbeCtx {
   V3 pa, pb;
   Q4 qa, qb;
+  Rot xa, xb, ia, ib;  // rotations of xformPoint and invXformPoint for A and B
   float vsa, vsb, spec;
   int32_t bminb[3], bmaxb[3];
 };
+__device__ __forceinline__ void ctxRots(ProbeCtx& c) {
+  c.xa = rotOf(qnormalize(c.qa));
+  c.xb = rotOf(qnormalize(c.qb));
+  c.ia = rotOf(qconj(c.qa));
+  c.ib = rotOf(qconj(c.qb));
+}
 
 // Probe order of a neighbour (dz, dy, dx in -1..1): (dz+1)*9 + (dy+1)*3 + (dx+1).
 // kWrite records each candidate as (penetration, feature id, lane << 5 | probe);
@@ -1352,13 +1392,15 @@ __device__ __forceinline__ uint32_t probeVoxel(const int32_t* __restrict__ dir,
                                                const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
                                                uint8_t va, uint32_t lane, uint32_t* vchk,
                                                Cand* __restrict__ out, uint32_t limit = 0) {
-  V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
-  V3 lb = invXformPoint(c.pb, c.qb, wa);
+  V3 wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
+  V3 lb = invXformBy(c.ib, c.pb, wa);
   int32_t cb[3];
   localToVoxel(c.vsb, lb, cb);
   uint32_t n = 0, vc = 0;
   uint32_t ta = va & 7u;
+#pragma unroll 1
   for (int dz = -1; dz <= 1; dz++)
+#pragma unroll 1
     for (int dy = -1; dy <= 1; dy++)
       for (int dx = -1; dx <= 1; dx++) {
         int32_t bx = cb[0] + dx, by = cb[1] + dy, bz = cb[2] + dz;
@@ -1370,7 +1412,7 @@ __device__ __forceinline__ uint32_t probeVoxel(const int32_t* __restrict__ dir,
         if (tb == 0) continue;
         vc++;
         if (!shouldCheck(ta, tb)) continue;
-        V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
+        V3 wb = xformBy(c.xb, c.pb, voxelToLocal(c.vsb, bx, by, bz));
         V3 diff = sub(wb, wa);
         float dsq = dot(diff, diff);
         if (!(dsq < c.spec)) continue;
@@ -1402,8 +1444,8 @@ __device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
                                                const uint64_t* __restrict__ mask,
                                                const VoxMap& mb, const ProbeCtx& c, int32_t x,
                                                int32_t y, int32_t z, uint8_t va, uint32_t* vchk) {
-  V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
-  V3 lb = invXformPoint(c.pb, c.qb, wa);
+  V3 wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
+  V3 lb = invXformBy(c.ib, c.pb, wa);
   int32_t cb[3];
   localToVoxel(c.vsb, lb, cb);
   const uint32_t ta = va & 7u;
@@ -1451,7 +1493,7 @@ __device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
             while (cand) {
               const int32_t bx = xa + __builtin_ctz(cand);
               cand &= cand - 1;
-              V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
+              V3 wb = xformBy(c.xb, c.pb, voxelToLocal(c.vsb, bx, by, bz));
               V3 diff = sub(wb, wa);
               float dsq = dot(diff, diff);
               if (dsq < c.spec) n++;
@@ -1469,8 +1511,8 @@ __device__ vxp_contact contactAt(const DevColliders& C, const DevCollider& HA,
                                  const DevCollider& HB, const ProbeCtx& c, int32_t x, int32_t y,
                                  int32_t z, uint32_t probe) {
   const uint8_t va = voxelAt(C, HA, x, y, z);
-  V3 wa = xformPoint(c.pa, c.qa, voxelToLocal(c.vsa, x, y, z));
-  V3 lb = invXformPoint(c.pb, c.qb, wa);
+  V3 wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
+  V3 lb = invXformBy(c.ib, c.pb, wa);
   int32_t cb[3];
   localToVoxel(c.vsb, lb, cb);
   const int32_t dz = (int32_t)(probe / 9) - 1, dy = (int32_t)(probe / 3 % 3) - 1,
@@ -1478,7 +1520,7 @@ __device__ vxp_contact contactAt(const DevColliders& C, const DevCollider& HA,
   const int32_t bx = cb[0] + dx, by = cb[1] + dy, bz = cb[2] + dz;
   const uint8_t vb = voxelAt(C, HB, bx, by, bz);
   const uint32_t ta = va & 7u, tb = vb & 7u;
-  V3 wb = xformPoint(c.pb, c.qb, voxelToLocal(c.vsb, bx, by, bz));
+  V3 wb = xformBy(c.xb, c.pb, voxelToLocal(c.vsb, bx, by, bz));
   V3 diff = sub(wb, wa);
   float dsq = dot(diff, diff);
   float dist = sqrtf(dsq);
@@ -1500,8 +1542,8 @@ __device__ vxp_contact contactAt(const DevColliders& C, const DevCollider& HA,
     nrm = na;
   }
   if (ta == VXP_VOXEL_INSIDE || tb == VXP_VOXEL_INSIDE) pen = fmaxf(pen, c.vsa);
-  V3 la = invXformPoint(c.pa, c.qa, cp);
-  V3 lbb = invXformPoint(c.pb, c.qb, cp);
+  V3 la = invXformBy(c.ia, c.pa, cp);
+  V3 lbb = invXformBy(c.ib, c.pb, cp);
   vxp_contact o;
   o.point[0] = cp.x; o.point[1] = cp.y; o.point[2] = cp.z;
   o.normal[0] = nrm.x; o.normal[1] = nrm.y; o.normal[2] = nrm.z;
@@ -1533,6 +1575,7 @@ __device__ __forceinline__ ProbeCtx probeCtx(const DevBodies& B, const DevCollid
     c.bminb[k] = HB.minb[k];
     c.bmaxb[k] = HB.maxb[k];
   }
+  ctxRots(c);
   return c;
 }
 
@@ -1562,54 +1605,219 @@ __device__ __forceinline__ uint32_t xcdBlock(uint32_t run) {
   return (r / run) * (kXcds * run) + x * run + r % run;
 }
 
-template <bool kCount>
-__global__ __launch_bounds__(256) void k_probe(DevCfg cfg, DevBodies B, DevColliders C,
-                                               const PairRec* pr, const uint32_t* wi_pair,
-                                               const uint32_t* wi_off, uint32_t total_wi,
-                                               const uint32_t* wi_cbase, uint32_t* wi_ccnt,
-                                               uint32_t* wi_vchk, uint8_t* lane_n, Cand* cand,
-                                               uint32_t xcd_run, const uint32_t* n_dev) {
-  uint32_t lane = threadIdx.x & (kWave - 1);
+// A load of launch-invariant data through the scalar cache: the address is
+// wave-uniform and nothing in the launch writes it.
+template <class T>
+__device__ __forceinline__ T sload(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  typedef const __attribute__((address_space(4))) uint32_t* CW;
+  const CW w = (CW)(p);
+  uint32_t v[sizeof(T) / 4];
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; i++) v[i] = w[i];
+  T t;
+  __builtin_memcpy(&t, v, sizeof(T));
+  return t;
+}
+
+// A pair's probe context, loaded through the scalar cache by the waves that
+// take its work items.
+struct ProbePair {
+  uint32_t woff, nx, ny, n;  // first work item, region extent, region voxels
+  uint32_t nxy, mx, mxy;     // nx * ny; division multipliers of nx and nxy (0: divide)
+  int32_t mn[3];
+  DevCollider HA, HB;
+  ProbeCtx c;
+};
+// Multiplier of the exact division by d of every n < count: floor(n / d) =
+// umulhi(n, m) with m = ceil(2^32 / d) when (count - 1) * d <= 2^32 (with
+// e = m d - 2^32 < d, n m / 2^32 = n / d + n e / (d 2^32) and n e < 2^32
+// keeps the quotient's floor).  0 where that does not hold or d = 1.
+__device__ __forceinline__ uint32_t divMagic(uint32_t d, uint32_t count) {
+  if (d <= 1u || (uint64_t)(count ? count - 1u : 0u) * d > (1ull << 32)) return 0u;
+  return (uint32_t)(((1ull << 32) + d - 1u) / d);
+}
+__device__ __forceinline__ uint32_t divBy(uint32_t n, uint32_t d, uint32_t m) {
+  return m ? __umulhi(n, m) : n / d;
+}
+__device__ __forceinline__ void loadProbePair(ProbePair& q, const DevCfg& cfg, const DevBodies& B,
+                                              const DevColliders& C, const PairRec* pr,
+                                              const uint32_t* wi_off, uint32_t p) {
+  const PairRec r = sload(pr + p);
+  q.woff = sload(wi_off + p);
+  q.nx = r.nx;
+  q.ny = r.ny;
+  q.n = r.nx * r.ny * r.nz;
+  q.nxy = r.nx * r.ny;
+  q.mx = divMagic(q.nx, q.n);
+  q.mxy = divMagic(q.nxy, q.n);
+  for (int k = 0; k < 3; k++) q.mn[k] = r.mn[k];
+  const uint32_t ga = r.world * cfg.max_bodies + r.sa, gb = r.world * cfg.max_bodies + r.sb;
+  q.HA = sload(C.hdr + sload(B.collider + ga));
+  q.HB = sload(C.hdr + sload(B.collider + gb));
+  ProbeCtx& c = q.c;
+  c.pa = f4v(sload(B.pos + ga));
+  c.pb = f4v(sload(B.pos + gb));
+  c.qa = f4q(sload(B.rot + ga));
+  c.qb = f4q(sload(B.rot + gb));
+  c.vsa = q.HA.vs;
+  c.vsb = q.HB.vs;
+  float cds = c.vsa * c.vsa * 1.0f * 1.0f;
+  c.spec = cds + 0.02f * 0.02f + 2.0f * sqrtf(cds) * 0.02f;
+  for (int k = 0; k < 3; k++) {
+    c.bminb[k] = q.HB.minb[k];
+    c.bmaxb[k] = q.HB.maxb[k];
+  }
+  ctxRots(c);
+}
+// Region voxel of lane `lane` of work item `wid` (z, y, x order); false past
+// the region's end.
+__device__ __forceinline__ bool probeVoxelXYZ(const ProbePair& q, uint32_t wid, uint32_t lane,
+                                              int32_t& x, int32_t& y, int32_t& z) {
+  const uint32_t lin = (wid - q.woff) * kWave + lane;
+  if (lin >= q.n) return false;
+  const uint32_t r = divBy(lin, q.nx, q.mx), zz = divBy(lin, q.nxy, q.mxy);
+  x = q.mn[0] + (int32_t)(lin - r * q.nx);
+  y = q.mn[1] + (int32_t)(r - zz * q.ny);
+  z = q.mn[2] + (int32_t)zz;
+  return true;
+}
+
+// The waves' share of the work items: contiguous ranges, so a pair's
+// consecutive chunks reuse its context; runs of `xcd_run` blocks stay on one
+// XCD, so neighbouring ranges share that XCD's L2.
+__device__ __forceinline__ void probeRange(uint32_t total_wi, uint32_t xcd_run, uint32_t& w_lo,
+                                           uint32_t& w_hi) {
+  const uint32_t wpb = blockDim.x / kWave, nwaves = gridDim.x * wpb;
+  const uint32_t g =
+      __builtin_amdgcn_readfirstlane(xcdBlock(xcd_run) * wpb + threadIdx.x / kWave);
+  const uint32_t per = (total_wi + nwaves - 1) / nwaves;
+  w_lo = min(g * per, total_wi);
+  w_hi = min(w_lo + per, total_wi);
+}
+
+// Count pass: per-lane candidate counts (lane_n), per-work-item totals
+// wi_ccnt and voxels_checked sums wi_vchk (both zeroed by k_wi_fill), and
+// wi_ccnt[total_wi] = 0.  Most region voxels of A are empty, so the lanes
+// are compacted before the neighbourhood test: each wave queues its
+// non-empty A voxels (work item, lane, type) in LDS and runs probeCount on
+// 64 of them at a time (fewer at a pair's end), adding each work item's sums
+// with one atomic per work item and batch.
+__global__ __launch_bounds__(256) void k_probe_count(DevCfg cfg, DevBodies B, DevColliders C,
+                                                     const PairRec* pr, const uint32_t* wi_pair,
+                                                     const uint32_t* wi_off, uint32_t total_wi,
+                                                     uint32_t* wi_ccnt, uint32_t* wi_vchk,
+                                                     uint8_t* lane_n, uint32_t xcd_run,
+                                                     const uint32_t* n_dev) {
+  __shared__ uint32_t q_wid[4][kWave], q_lv[4][kWave];
+  const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  uint32_t* const qw = q_wid[wv];
+  uint32_t* const ql = q_lv[wv];
   total_wi = devCount(total_wi, n_dev);
-  // one work item per wave; a grid smaller than the work items (a recorded
-  // step's persistent grid) loops, keeping runs of blocks on one XCD
-  const uint32_t wpb = blockDim.x / kWave;
-  for (uint32_t wid = xcdBlock(xcd_run) * wpb + threadIdx.x / kWave; wid < total_wi;
-       wid += gridDim.x * wpb) {
-  if (!kCount && wi_ccnt[wid] == 0) continue;
-  uint32_t p = wi_pair[wid];
-  const PairRec& r = pr[p];
-  uint32_t chunk = wid - wi_off[p];
-  uint32_t w = r.world;
-  uint32_t ga = w * cfg.max_bodies + r.sa, gb = w * cfg.max_bodies + r.sb;
-  const DevCollider& HA = C.hdr[B.collider[ga]];
-  const DevCollider& HB = C.hdr[B.collider[gb]];
-  const ProbeCtx c = probeCtx(B, HA, HB, ga, gb);
-  uint32_t lin = chunk * kWave + lane;
-  bool valid = lin < r.nx * r.ny * r.nz;
-  int32_t x = 0, y = 0, z = 0;
-  uint8_t va = 0;
-  if (valid) {
-    regionVoxel(r, lin, x, y, z);
-    va = voxelAt(C, HA, x, y, z);
-  }
-  if (kCount) {
-    uint32_t vchk = 0, n = 0;
-    if ((va & 7u) != 0) n = probeCount(C.dir, C.mask, voxMap(HB), c, x, y, z, va, &vchk);
-    lane_n[(size_t)wid * kWave + lane] = (uint8_t)n;
-    uint32_t total = waveSum(n), vsum = waveSum(vchk);
-    if (lane == 0) {
-      wi_ccnt[wid] = total;
-      wi_vchk[wid] = vsum;
-      if (wid == total_wi - 1) wi_ccnt[total_wi] = 0;
+  uint32_t w_lo, w_hi;
+  probeRange(total_wi, xcd_run, w_lo, w_hi);
+  if (lane == 0 && w_lo < w_hi && w_hi == total_wi) wi_ccnt[total_wi] = 0;
+  ProbePair q{};
+  uint32_t pcur = 0xFFFFFFFFu, qn = 0;
+  // probeCount on queue entries [0, cnt)
+  auto flush = [&](uint32_t cnt) {
+    const bool act = lane < cnt;
+    const uint32_t ew = act ? qw[lane] : 0u, el = act ? ql[lane] : 0u;
+    uint32_t n = 0, vc = 0;
+    if (act) {
+      int32_t x, y, z;
+      probeVoxelXYZ(q, ew, el & (kWave - 1), x, y, z);
+      n = probeCount(C.dir, C.mask, voxMap(q.HB), q.c, x, y, z, (uint8_t)(el >> 8), &vc);
+      lane_n[(size_t)ew * kWave + (el & (kWave - 1))] = (uint8_t)n;
     }
-  } else {
-    uint32_t n = lane_n[(size_t)wid * kWave + lane];
-    int incl = waveInclusiveScan((int)n, (int)lane);
-    if (n > 0)
-      probeVoxel<true>(C.dir, C.vox, voxMap(HB), c, x, y, z, va, lane, nullptr,
-                       cand + wi_cbase[wid] + (uint32_t)incl - n, n);
+    // entries are in work-item order: one sum per work item of the batch
+    uint64_t rem = __ballot(act);
+    while (rem) {
+      const int f = __ffsll((long long)rem) - 1;
+      const uint32_t w0 = rdlane(ew, f);
+      const bool mine = act && ew == w0;
+      const uint32_t sn = waveSum(mine ? n : 0u), sv = waveSum(mine ? vc : 0u);
+      if ((int)lane == f) {
+        if (sn) atomicAdd(wi_ccnt + w0, sn);
+        if (sv) atomicAdd(wi_vchk + w0, sv);
+      }
+      rem &= ~__ballot(mine);
+    }
+  };
+  for (uint32_t wid = w_lo; wid < w_hi; wid++) {
+    const uint32_t p = sload(wi_pair + wid);
+    if (p != pcur) {
+      if (qn) flush(qn);
+      qn = 0;
+      pcur = p;
+      loadProbePair(q, cfg, B, C, pr, wi_off, p);
+    }
+    int32_t x, y, z;
+    uint8_t va = 0;
+    if (probeVoxelXYZ(q, wid, lane, x, y, z)) va = voxelAt(C, q.HA, x, y, z);
+    const bool ne = (va & 7u) != 0;
+    if (!ne) lane_n[(size_t)wid * kWave + lane] = 0;
+    const uint64_t m = __ballot(ne);
+    if (!m) continue;
+    const uint32_t k = (uint32_t)__popcll(m);
+    const uint32_t slot = qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const uint32_t ent = lane | ((uint32_t)va << 8);
+    if (ne && slot < kWave) {
+      qw[slot] = wid;
+      ql[slot] = ent;
+    }
+    if (qn + k < kWave) {
+      qn += k;
+      continue;
+    }
+    flush(kWave);
+    if (ne && slot >= kWave) {
+      qw[slot - kWave] = wid;
+      ql[slot - kWave] = ent;
+    }
+    qn = qn + k - kWave;
   }
+  if (qn) flush(qn);
+}
+
+// Write pass: candidates of every work item with a non-zero count written
+// from wi_cbase[wid] (the exclusive scan of wi_ccnt) in (lane, probe) order.
+// The counts of 64 work items are read at once and only the non-zero ones
+// are visited.
+__global__ __launch_bounds__(256) void k_probe_write(DevCfg cfg, DevBodies B, DevColliders C,
+                                                     const PairRec* pr, const uint32_t* wi_pair,
+                                                     const uint32_t* wi_off, uint32_t total_wi,
+                                                     const uint32_t* wi_cbase,
+                                                     const uint32_t* wi_ccnt,
+                                                     const uint8_t* lane_n, Cand* cand,
+                                                     uint32_t xcd_run, const uint32_t* n_dev) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  total_wi = devCount(total_wi, n_dev);
+  uint32_t w_lo, w_hi;
+  probeRange(total_wi, xcd_run, w_lo, w_hi);
+  ProbePair q{};
+  uint32_t pcur = 0xFFFFFFFFu;
+  for (uint32_t wb = w_lo; wb < w_hi; wb += kWave) {
+    const uint32_t wl = wb + lane;
+    uint64_t nz = __ballot(wl < w_hi && wi_ccnt[wl] != 0u);
+    while (nz) {
+      const uint32_t wid = wb + (uint32_t)(__ffsll((long long)nz) - 1);
+      nz &= nz - 1;
+      const uint32_t p = sload(wi_pair + wid);
+      if (p != pcur) {
+        pcur = p;
+        loadProbePair(q, cfg, B, C, pr, wi_off, p);
+      }
+      const uint32_t n = lane_n[(size_t)wid * kWave + lane];
+      const int incl = waveInclusiveScan((int)n, (int)lane);
+      if (n > 0) {
+        int32_t x, y, z;
+        probeVoxelXYZ(q, wid, lane, x, y, z);
+        const uint8_t va = voxelAt(C, q.HA, x, y, z);
+        probeVoxel<true>(C.dir, C.vox, voxMap(q.HB), q.c, x, y, z, va, lane, nullptr,
+                         cand + sload(wi_cbase + wid) + (uint32_t)incl - n, n);
+      }
+    }
   }
 }
 
@@ -4241,10 +4449,11 @@ void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_s
       cfg, ws, st, pr, keys, cidx, clist, counts, nw);
 }
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                    uint32_t* wi_pair, const uint32_t* n_dev) {
+                    uint32_t* wi_pair, uint32_t* wi_ccnt, uint32_t* wi_vchk,
+                    const uint32_t* n_dev) {
   if (n_pairs)
-    k_wi_fill<<<gridFor(n_pairs, 256, n_dev, kLoopBlocks), 256, 0, s>>>(counts, wi_off, n_pairs,
-                                                                        wi_pair, n_dev);
+    k_wi_fill<<<gridFor(n_pairs, 256, n_dev, kLoopBlocks), 256, 0, s>>>(
+        counts, wi_off, n_pairs, wi_pair, wi_ccnt, wi_vchk, n_dev);
 }
 void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                   const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
@@ -4254,19 +4463,17 @@ void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const De
   if (!total_wi) return;
   const uint32_t threads = shape.threads, waves = threads / kWave;
   const uint32_t run = shape.xcd_run ? shape.xcd_run : 1u;
-  // a recorded step's grid: four rounds of 28 waves per CU (7 per SIMD, the
-  // probe's occupancy), a whole number of XCD rounds of runs when it can be
+  // at most four rounds of 28 waves per CU, a whole number of XCD rounds of
+  // runs when it can be; each wave takes a contiguous range of work items
   uint32_t most = 4 * 256 * 28 / waves;
   if (most >= kXcds * run) most -= most % (kXcds * run);
-  const uint32_t blocks = gridFor(total_wi, waves, n_dev, most);
+  const uint32_t blocks = std::min(cdiv(total_wi, waves), most);
   if (count)
-    k_probe<true><<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
-                                                            total_wi, wi_cbase, wi_ccnt, wi_vchk,
-                                                            lane_n, cand, run, n_dev);
+    k_probe_count<<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt,
+                                              wi_vchk, lane_n, run, n_dev);
   else
-    k_probe<false><<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off,
-                                                             total_wi, wi_cbase, wi_ccnt, wi_vchk,
-                                                             lane_n, cand, run, n_dev);
+    k_probe_write<<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_cbase,
+                                              wi_ccnt, lane_n, cand, run, n_dev);
 }
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,

@@ -119,11 +119,14 @@ void launch_thread_mode(hipStream_t s, const DevCfg& cfg, const DevBodies& B, Wo
 void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t n,
                          uint32_t* out);
 void launch_invert(hipStream_t s, const uint32_t* perm, uint32_t n, uint32_t* inv);
-// wi_pair[w] = the pair of work item w (after the scan of the chunk counts)
+// wi_pair[w] = the pair of work item w (after the scan of the chunk counts);
+// wi_ccnt[w] = wi_vchk[w] = 0 for the probe's count pass to add to
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                    uint32_t* wi_pair, const uint32_t* n_dev = nullptr);
-// Launch shape of the probe: threads per workgroup (64, 128 or 256; one work
-// item per wave) and the run of consecutive workgroups kept on one XCD.
+                    uint32_t* wi_pair, uint32_t* wi_ccnt, uint32_t* wi_vchk,
+                    const uint32_t* n_dev = nullptr);
+// Launch shape of the probe: threads per workgroup (64, 128 or 256; each wave
+// takes a contiguous range of work items) and the run of consecutive
+// workgroups kept on one XCD.
 struct ProbeShape {
   uint32_t threads, xcd_run;
 };
