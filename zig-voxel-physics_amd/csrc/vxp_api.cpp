@@ -949,31 +949,39 @@ static int stepAllImpl(vxp_system* s) {
   HIP_TRY(s->d_wi_vchk.ensure((size_t)std::max(total_wi, 1u) * 4));
   HIP_TRY(s->d_lane_n.ensure((size_t)std::max(total_wi, 1u) * 64));
   HIP_TRY(s->d_wi_pair.ensure((size_t)std::max(total_wi, 1u) * 4));
-  // pass 1 counts, the scan places every work item's candidates, pass 2 writes
+  HIP_TRY(s->d_wi_list.ensure((size_t)std::max(total_wi, 1u) * 4));
+  // pass 1 counts and lists the work items with candidates, the scan places
+  // every work item's candidates, pass 2 writes those of the listed ones
   pe = profBegin(s, VXP_K_PROBE);
   RC(tuneMark(s, vxp_system::kPhProbe, true));
   const ProbeShape shape{s->tune.use[vxp_system::kProbeThreads],
                          s->tune.use[vxp_system::kProbeXcdRun]};
-  uint32_t ncand = 0;
+  uint32_t ncand = 0, nlist = 0;
   if (total_wi) {
     launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), npairs,
                    s->d_wi_pair.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
                    s->d_wi_vchk.as<uint32_t>());
-    launch_probe(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), total_wi, nullptr,
-                 s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
-                 nullptr, true, shape);
+    launch_probe_count(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(),
+                       s->d_wi_off.as<uint32_t>(), total_wi, s->d_wi_ccnt.as<uint32_t>(),
+                       s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
+                       s->d_wi_list.as<uint32_t>(), s->d_step_ctr.as<uint32_t>() + kCtrProbeItems,
+                       shape);
     size_t cb = exclusive_scan_u32_bytes(total_wi + 1);
     HIP_TRY(s->d_sort_tmp.ensure(cb));
     HIP_TRY(exclusive_scan_u32(s->d_sort_tmp.p, cb, s->d_wi_ccnt.as<uint32_t>(),
                                s->d_wi_cbase.as<uint32_t>(), total_wi + 1, st));
-    RC(readSizes(s, {s->d_wi_cbase.as<uint32_t>() + total_wi}, &ncand));
+    uint32_t sz[2];
+    RC(readSizes(s, {s->d_wi_cbase.as<uint32_t>() + total_wi,
+                     s->d_step_ctr.as<uint32_t>() + kCtrProbeItems}, sz));
+    ncand = sz[0];
+    nlist = sz[1];
   }
   HIP_TRY(s->d_cand.ensure((size_t)std::max(ncand, 1u) * sizeof(Cand)));
   if (ncand)
-    launch_probe(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), total_wi,
-                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
-                 s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(), s->d_cand.as<Cand>(),
-                 false, shape);
+    launch_probe_write(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(),
+                       s->d_wi_off.as<uint32_t>(), s->d_wi_list.as<uint32_t>(), nlist,
+                       s->d_wi_cbase.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
+                       s->d_cand.as<Cand>(), shape);
   RC(tuneMark(s, vxp_system::kPhProbe, false));
   profEnd(s, VXP_K_PROBE, pe);
   s->last_wi = total_wi;
@@ -1154,7 +1162,7 @@ static std::vector<DevBuf*> allBufs(vxp_system* s) {
           &s->d_cb_solid, &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
           &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off, &s->d_t_pos,
           &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake, &s->d_arank, &s->d_drop,
-          &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair, &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code,
+          &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair, &s->d_wi_list, &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code,
           &s->d_lb_child, &s->d_lb_range, &s->d_lb_iparent, &s->d_lb_lparent, &s->d_lb_flag,
           &s->d_maxw, &s->d_rsnap, &s->d_rdesc, &s->d_rsize};
 }
@@ -1244,6 +1252,7 @@ static int recEnsure(vxp_system* s) {
   HIP_TRY(s->d_wi_vchk.ensure(WI * 4));
   HIP_TRY(s->d_lane_n.ensure(WI * 64));
   HIP_TRY(s->d_wi_pair.ensure(WI * 4));
+  HIP_TRY(s->d_wi_list.ensure(WI * 4));
   HIP_TRY(s->d_cand.ensure(CA * sizeof(Cand)));
   HIP_TRY(s->d_total.ensure(64));
   HIP_TRY(s->d_ml[0].ensure(M * sizeof(DevManifold), true, s->stream));
@@ -1423,9 +1432,11 @@ static int recSeg2(vxp_system* s, const RecCtx& c) {
   launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), c.P,
                  s->d_wi_pair.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
                  s->d_wi_vchk.as<uint32_t>(), c.npairs);
-  launch_probe(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
-               c.WI, nullptr, s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
-               s->d_lane_n.as<uint8_t>(), nullptr, true, shape, c.rsize);
+  launch_probe_count(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(),
+                     s->d_wi_off.as<uint32_t>(), c.WI, s->d_wi_ccnt.as<uint32_t>(),
+                     s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
+                     s->d_wi_list.as<uint32_t>(), s->d_step_ctr.as<uint32_t>() + kCtrProbeItems,
+                     shape, c.rsize);
   HIP_TRY(exclusive_scan_u32_n(tmp, s->d_wi_ccnt.as<uint32_t>(), s->d_wi_cbase.as<uint32_t>(),
                                c.WI + 1, c.rsize, 1, st));
   launch_rec_total_cand(st, s->d_wi_cbase.as<uint32_t>(), c.rsize);
@@ -1443,10 +1454,10 @@ static int recSeg3(vxp_system* s, const RecCtx& c) {
   uint32_t* ctr = s->d_step_ctr.as<uint32_t>();
   const ProbeShape shape{s->tune.use[vxp_system::kProbeThreads],
                          s->tune.use[vxp_system::kProbeXcdRun]};
-  launch_probe(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
-               c.WI, s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
-               s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(), s->d_cand.as<Cand>(), false,
-               shape, c.rsize);
+  launch_probe_write(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(),
+                     s->d_wi_off.as<uint32_t>(), s->d_wi_list.as<uint32_t>(), c.WI,
+                     s->d_wi_cbase.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
+                     s->d_cand.as<Cand>(), shape, ctr + kCtrProbeItems);
   launch_reduce(st, c.cfg, c.B, c.C, c.ws, c.st, pr, s->d_pairs.as<uint64_t>(), c.P,
                 s->d_wlist.as<uint32_t>(), ctr, s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),

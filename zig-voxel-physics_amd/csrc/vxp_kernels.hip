@@ -1383,87 +1383,38 @@ __device__ __forceinline__ void ctxRots(ProbeCtx& c) {
   c.ib = rotOf(qconj(c.qb));
 }
 
-// Probe order of a neighbour (dz, dy, dx in -1..1): (dz+1)*9 + (dy+1)*3 + (dx+1).
-// kWrite records each candidate as (penetration, feature id, lane << 5 | probe);
-// the full contact of the few that survive the top-4 is rebuilt by contactAt.
-template <bool kWrite>
-__device__ __forceinline__ uint32_t probeVoxel(const int32_t* __restrict__ dir,
-                                               const uint8_t* __restrict__ vox, const VoxMap& mb,
-                                               const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
-                                               uint8_t va, uint32_t lane, uint32_t* vchk,
-                                               Cand* __restrict__ out, uint32_t limit = 0) {
-  V3 wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
-  V3 lb = invXformBy(c.ib, c.pb, wa);
-  int32_t cb[3];
-  localToVoxel(c.vsb, lb, cb);
-  uint32_t n = 0, vc = 0;
-  uint32_t ta = va & 7u;
-#pragma unroll 1
-  for (int dz = -1; dz <= 1; dz++)
-#pragma unroll 1
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        int32_t bx = cb[0] + dx, by = cb[1] + dy, bz = cb[2] + dz;
-        if (bx < c.bminb[0] || bx >= c.bmaxb[0] || by < c.bminb[1] || by >= c.bmaxb[1] ||
-            bz < c.bminb[2] || bz >= c.bmaxb[2])
-          continue;
-        uint8_t vb = voxelAt(dir, vox, mb, bx, by, bz);
-        uint32_t tb = vb & 7u;
-        if (tb == 0) continue;
-        vc++;
-        if (!shouldCheck(ta, tb)) continue;
-        V3 wb = xformBy(c.xb, c.pb, voxelToLocal(c.vsb, bx, by, bz));
-        V3 diff = sub(wb, wa);
-        float dsq = dot(diff, diff);
-        if (!(dsq < c.spec)) continue;
-        if (kWrite && n < limit) {  // never beyond the slots the count pass sized
-          // the penetration of generateContact (src/narrow_phase.zig:879-906)
-          float dist = sqrtf(dsq);
-          float pen = c.vsa - dist;
-          if (ta == VXP_VOXEL_INSIDE || tb == VXP_VOXEL_INSIDE) pen = fmaxf(pen, c.vsa);
-          uint64_t fid = featureId(x, y, z, bx, by, bz);
-          Cand& o = out[n];
-          o.pen = pen;
-          o.fid_lo = (uint32_t)fid;
-          o.fid_hi = (uint32_t)(fid >> 32);
-          o.code = (lane << 5) | (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
-        }
-        n++;
-      }
-  if (vchk) *vchk = vc;
-  return n;
-}
+// Probe order of a neighbour (dz, dy, dx in -1..1): (dz+1)*9 + (dy+1)*3 + (dx+1)
+// (checkNeighborhood, src/narrow_phase.zig:796-857).  The write pass records
+// each candidate as (penetration, feature id, lane << 5 | probe); the full
+// contact of the few that survive the top-4 is rebuilt by contactAt.
 
-// The counts of probeVoxel<false> (candidates n, non-empty probed B voxels
-// vchk) from the brick masks: per (dz, dy) row of the 3x3x3 neighbourhood,
-// the in-bounds x window of the non-empty mask gives vchk by popcount, and
-// only the B voxels whose type passes shouldCheck with the A voxel's (any
-// non-empty one for an A corner, corner or edge for an A edge, corner
-// otherwise) take the distance test, with the same arithmetic.
-__device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
-                                               const uint64_t* __restrict__ mask,
-                                               const VoxMap& mb, const ProbeCtx& c, int32_t x,
-                                               int32_t y, int32_t z, uint8_t va, uint32_t* vchk) {
-  V3 wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
+// checkNeighborhood's probes of A voxel (x, y, z) from the brick masks: the
+// probes whose B voxel is in bounds, non-empty and passes shouldCheck with
+// the A voxel's type (any non-empty one for an A corner, corner or edge for
+// an A edge, corner otherwise), as bit (dz+1)*9 + (dy+1)*3 + (dx+1) of the
+// result, so that ascending bits are the probe order; *vchk = the non-empty
+// in-bounds B voxels.  The window spans at most two bricks per axis: one
+// directory read per brick and one read per mask plane per z slice, which
+// covers the slice's rows (bit x + 8y of the slice's word).
+__device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
+                                              const uint64_t* __restrict__ mask, const VoxMap& mb,
+                                              const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
+                                              uint32_t ta, V3& wa, int32_t cb[3],
+                                              uint32_t* vchk) {
+  wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
   V3 lb = invXformBy(c.ib, c.pb, wa);
-  int32_t cb[3];
   localToVoxel(c.vsb, lb, cb);
-  const uint32_t ta = va & 7u;
   // in-bounds window of the neighbourhood per axis
   int32_t lo[3], hi[3];
   for (int k = 0; k < 3; k++) {
     lo[k] = max(cb[k] - 1, c.bminb[k]);
     hi[k] = min(cb[k] + 1, c.bmaxb[k] - 1);
   }
-  uint32_t n = 0, vc = 0;
+  uint32_t pm = 0, vc = 0;
   if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) {
     *vchk = 0;
     return 0;
   }
-  // Brick by brick (the window spans at most two bricks per axis): one
-  // directory read per brick, one read per mask plane per z slice, which
-  // covers the slice's rows of the window (bit x + 8y of the slice's word).
-  // Counts are sums, so the visiting order does not matter.
   for (int32_t zb = lo[2] >> 3; zb <= hi[2] >> 3; zb++)
     for (int32_t yb = lo[1] >> 3; yb <= hi[1] >> 3; yb++)
       for (int32_t xb8 = lo[0] >> 3; xb8 <= hi[0] >> 3; xb8++) {
@@ -1489,20 +1440,85 @@ __device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
             const uint32_t sh = 8u * (uint32_t)(by & 7) + (uint32_t)(xa & 7);
             const uint32_t occ = (uint32_t)(w0 >> sh) & wmask;
             vc += (uint32_t)__builtin_popcount(occ);
-            uint32_t cand = occ & (uint32_t)(wt >> sh);
-            while (cand) {
-              const int32_t bx = xa + __builtin_ctz(cand);
-              cand &= cand - 1;
-              V3 wb = xformBy(c.xb, c.pb, voxelToLocal(c.vsb, bx, by, bz));
-              V3 diff = sub(wb, wa);
-              float dsq = dot(diff, diff);
-              if (dsq < c.spec) n++;
-            }
+            pm |= (occ & (uint32_t)(wt >> sh))
+                  << ((bz - cb[2] + 1) * 9 + (by - cb[1] + 1) * 3 + (xa - cb[0] + 1));
           }
         }
       }
   *vchk = vc;
+  return pm;
+}
+// B voxel of probe k around the B voxel cb.
+__device__ __forceinline__ void probeB(const int32_t cb[3], uint32_t k, int32_t& bx, int32_t& by,
+                                       int32_t& bz) {
+  bz = cb[2] + (int32_t)(k / 9u) - 1;
+  by = cb[1] + (int32_t)(k / 3u % 3u) - 1;
+  bx = cb[0] + (int32_t)(k % 3u) - 1;
+}
+
+// The counts of checkNeighborhood (candidates n, non-empty probed B voxels
+// vchk): the probes of probeMask that pass the distance test of
+// checkNeighborhood (src/narrow_phase.zig:831-840).
+__device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
+                                               const uint64_t* __restrict__ mask,
+                                               const VoxMap& mb, const ProbeCtx& c, int32_t x,
+                                               int32_t y, int32_t z, uint8_t va, uint32_t* vchk) {
+  V3 wa;
+  int32_t cb[3];
+  uint32_t pm = probeMask(dir, mask, mb, c, x, y, z, va & 7u, wa, cb, vchk);
+  uint32_t n = 0;
+  while (pm) {
+    const uint32_t k = (uint32_t)__builtin_ctz(pm);
+    pm &= pm - 1;
+    int32_t bx, by, bz;
+    probeB(cb, k, bx, by, bz);
+    V3 wb = xformBy(c.xb, c.pb, voxelToLocal(c.vsb, bx, by, bz));
+    V3 diff = sub(wb, wa);
+    float dsq = dot(diff, diff);
+    if (dsq < c.spec) n++;
+  }
   return n;
+}
+
+// The candidates of checkNeighborhood: the first `limit` of A voxel
+// (x, y, z) in probe order, as (penetration, feature id, lane << 5 | probe).
+// The B voxel's byte is read only for a candidate whose penetration rule
+// needs its type (an A voxel other than INSIDE).
+__device__ __forceinline__ void probeWrite(const int32_t* __restrict__ dir,
+                                           const uint64_t* __restrict__ mask,
+                                           const uint8_t* __restrict__ vox, const VoxMap& mb,
+                                           const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
+                                           uint8_t va, uint32_t lane, Cand* __restrict__ out,
+                                           uint32_t limit) {
+  V3 wa;
+  int32_t cb[3];
+  uint32_t vc;
+  const uint32_t ta = va & 7u;
+  uint32_t pm = probeMask(dir, mask, mb, c, x, y, z, ta, wa, cb, &vc);
+  uint32_t n = 0;
+  while (pm && n < limit) {
+    const uint32_t k = (uint32_t)__builtin_ctz(pm);
+    pm &= pm - 1;
+    int32_t bx, by, bz;
+    probeB(cb, k, bx, by, bz);
+    V3 wb = xformBy(c.xb, c.pb, voxelToLocal(c.vsb, bx, by, bz));
+    V3 diff = sub(wb, wa);
+    float dsq = dot(diff, diff);
+    if (!(dsq < c.spec)) continue;
+    // the penetration of generateContact (src/narrow_phase.zig:879-906)
+    float dist = sqrtf(dsq);
+    float pen = c.vsa - dist;
+    bool inside = ta == VXP_VOXEL_INSIDE;
+    if (!inside) inside = (voxelAt(dir, vox, mb, bx, by, bz) & 7u) == VXP_VOXEL_INSIDE;
+    if (inside) pen = fmaxf(pen, c.vsa);
+    uint64_t fid = featureId(x, y, z, bx, by, bz);
+    Cand& o = out[n];
+    o.pen = pen;
+    o.fid_lo = (uint32_t)fid;
+    o.fid_hi = (uint32_t)(fid >> 32);
+    o.code = (lane << 5) | k;
+    n++;
+  }
 }
 
 // generateContact (src/narrow_phase.zig:860-922) for A voxel (x, y, z) against
@@ -1697,22 +1713,34 @@ __device__ __forceinline__ void probeRange(uint32_t total_wi, uint32_t xcd_run, 
 }
 
 // Count pass: per-lane candidate counts (lane_n), per-work-item totals
-// wi_ccnt and voxels_checked sums wi_vchk (both zeroed by k_wi_fill), and
-// wi_ccnt[total_wi] = 0.  Most region voxels of A are empty, so the lanes
+// wi_ccnt and voxels_checked sums wi_vchk (both zeroed by k_wi_fill),
+// wi_ccnt[total_wi] = 0, and the work items with candidates on wi_list.  Most region voxels of A are empty, so the lanes
 // are compacted before the neighbourhood test: each wave queues its
 // non-empty A voxels (work item, lane, type) in LDS and runs probeCount on
 // 64 of them at a time (fewer at a pair's end), adding each work item's sums
 // with one atomic per work item and batch.
-__global__ __launch_bounds__(256) void k_probe_count(DevCfg cfg, DevBodies B, DevColliders C,
+__global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B, DevColliders C,
                                                      const PairRec* pr, const uint32_t* wi_pair,
                                                      const uint32_t* wi_off, uint32_t total_wi,
                                                      uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                                                     uint8_t* lane_n, uint32_t xcd_run,
+                                                     uint8_t* lane_n, uint32_t* wi_list,
+                                                     uint32_t* list_n, uint32_t xcd_run,
                                                      const uint32_t* n_dev) {
-  __shared__ uint32_t q_wid[4][kWave], q_lv[4][kWave];
+  __shared__ uint32_t q_wid[4][kWave], q_lv[4][kWave], l_wid[4][kWave];
   const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   uint32_t* const qw = q_wid[wv];
   uint32_t* const ql = q_lv[wv];
+  uint32_t* const lw = l_wid[wv];
+  // the work items with candidates gather in lw and go to wi_list 64 at a
+  // time (or at the end), with one returning atomic per flush
+  uint32_t ln = 0, last_listed = 0xFFFFFFFFu;
+  auto listFlush = [&]() {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(list_n, ln);
+    base = rdlane(base, 0);
+    if (lane < ln) wi_list[base + lane] = lw[lane];
+    ln = 0;
+  };
   total_wi = devCount(total_wi, n_dev);
   uint32_t w_lo, w_hi;
   probeRange(total_wi, xcd_run, w_lo, w_hi);
@@ -1742,6 +1770,13 @@ __global__ __launch_bounds__(256) void k_probe_count(DevCfg cfg, DevBodies B, De
         if (sv) atomicAdd(wi_vchk + w0, sv);
       }
       rem &= ~__ballot(mine);
+      // a work item's entries are consecutive (at most two batches), so its
+      // first non-zero sum lists it
+      if (sn && w0 != last_listed) {
+        if ((int)lane == f) lw[ln] = w0;
+        last_listed = w0;
+        if (++ln == kWave) listFlush();
+      }
     }
   };
   for (uint32_t wid = w_lo; wid < w_hi; wid++) {
@@ -1778,45 +1813,40 @@ __global__ __launch_bounds__(256) void k_probe_count(DevCfg cfg, DevBodies B, De
     qn = qn + k - kWave;
   }
   if (qn) flush(qn);
+  if (ln) listFlush();
 }
 
-// Write pass: candidates of every work item with a non-zero count written
-// from wi_cbase[wid] (the exclusive scan of wi_ccnt) in (lane, probe) order.
-// The counts of 64 work items are read at once and only the non-zero ones
-// are visited.
+// Write pass: the candidates of every work item on wi_list (those with a
+// non-zero count) written from wi_cbase[wid] (the exclusive scan of wi_ccnt)
+// in (lane, probe) order.  One list entry per wave at a time.
 __global__ __launch_bounds__(256) void k_probe_write(DevCfg cfg, DevBodies B, DevColliders C,
                                                      const PairRec* pr, const uint32_t* wi_pair,
-                                                     const uint32_t* wi_off, uint32_t total_wi,
+                                                     const uint32_t* wi_off,
+                                                     const uint32_t* wi_list, uint32_t n_items,
                                                      const uint32_t* wi_cbase,
-                                                     const uint32_t* wi_ccnt,
                                                      const uint8_t* lane_n, Cand* cand,
-                                                     uint32_t xcd_run, const uint32_t* n_dev) {
+                                                     const uint32_t* n_dev) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  total_wi = devCount(total_wi, n_dev);
-  uint32_t w_lo, w_hi;
-  probeRange(total_wi, xcd_run, w_lo, w_hi);
+  n_items = devCount(n_items, n_dev);
+  const uint32_t wpb = blockDim.x / kWave, nwaves = gridDim.x * wpb;
+  const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + threadIdx.x / kWave);
   ProbePair q{};
   uint32_t pcur = 0xFFFFFFFFu;
-  for (uint32_t wb = w_lo; wb < w_hi; wb += kWave) {
-    const uint32_t wl = wb + lane;
-    uint64_t nz = __ballot(wl < w_hi && wi_ccnt[wl] != 0u);
-    while (nz) {
-      const uint32_t wid = wb + (uint32_t)(__ffsll((long long)nz) - 1);
-      nz &= nz - 1;
-      const uint32_t p = sload(wi_pair + wid);
-      if (p != pcur) {
-        pcur = p;
-        loadProbePair(q, cfg, B, C, pr, wi_off, p);
-      }
-      const uint32_t n = lane_n[(size_t)wid * kWave + lane];
-      const int incl = waveInclusiveScan((int)n, (int)lane);
-      if (n > 0) {
-        int32_t x, y, z;
-        probeVoxelXYZ(q, wid, lane, x, y, z);
-        const uint8_t va = voxelAt(C, q.HA, x, y, z);
-        probeVoxel<true>(C.dir, C.vox, voxMap(q.HB), q.c, x, y, z, va, lane, nullptr,
-                         cand + sload(wi_cbase + wid) + (uint32_t)incl - n, n);
-      }
+  for (uint32_t i = g; i < n_items; i += nwaves) {
+    const uint32_t wid = sload(wi_list + i);
+    const uint32_t p = sload(wi_pair + wid);
+    if (p != pcur) {
+      pcur = p;
+      loadProbePair(q, cfg, B, C, pr, wi_off, p);
+    }
+    const uint32_t n = lane_n[(size_t)wid * kWave + lane];
+    const int incl = waveInclusiveScan((int)n, (int)lane);
+    if (n > 0) {
+      int32_t x, y, z;
+      probeVoxelXYZ(q, wid, lane, x, y, z);
+      const uint8_t va = voxelAt(C, q.HA, x, y, z);
+      probeWrite(C.dir, C.mask, C.vox, voxMap(q.HB), q.c, x, y, z, va, lane,
+                 cand + sload(wi_cbase + wid) + (uint32_t)incl - n, n);
     }
   }
 }
@@ -4455,25 +4485,33 @@ void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_of
     k_wi_fill<<<gridFor(n_pairs, 256, n_dev, kLoopBlocks), 256, 0, s>>>(
         counts, wi_off, n_pairs, wi_pair, wi_ccnt, wi_vchk, n_dev);
 }
-void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                  const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
-                  uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape,
-                  const uint32_t* n_dev) {
-  if (!total_wi) return;
-  const uint32_t threads = shape.threads, waves = threads / kWave;
-  const uint32_t run = shape.xcd_run ? shape.xcd_run : 1u;
-  // at most four rounds of 28 waves per CU, a whole number of XCD rounds of
-  // runs when it can be; each wave takes a contiguous range of work items
+// At most four rounds of 28 waves per CU, a whole number of XCD rounds of
+// runs of `run` blocks when it can be.
+static uint32_t probeBlocks(uint32_t items, uint32_t waves, uint32_t run) {
   uint32_t most = 4 * 256 * 28 / waves;
   if (most >= kXcds * run) most -= most % (kXcds * run);
-  const uint32_t blocks = std::min(cdiv(total_wi, waves), most);
-  if (count)
-    k_probe_count<<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt,
-                                              wi_vchk, lane_n, run, n_dev);
-  else
-    k_probe_write<<<blocks, threads, 0, s>>>(cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_cbase,
-                                              wi_ccnt, lane_n, cand, run, n_dev);
+  return std::min(cdiv(items, waves), most);
+}
+void launch_probe_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                        const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
+                        const uint32_t* wi_off, uint32_t total_wi, uint32_t* wi_ccnt,
+                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* wi_list, uint32_t* list_n,
+                        ProbeShape shape, const uint32_t* n_dev) {
+  if (!total_wi) return;
+  const uint32_t threads = shape.threads, run = shape.xcd_run ? shape.xcd_run : 1u;
+  k_probe_count<<<probeBlocks(total_wi, threads / kWave, run), threads, 0, s>>>(
+      cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt, wi_vchk, lane_n, wi_list, list_n, run,
+      n_dev);
+}
+void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                        const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
+                        const uint32_t* wi_off, const uint32_t* wi_list, uint32_t n_items,
+                        const uint32_t* wi_cbase, const uint8_t* lane_n, Cand* cand,
+                        ProbeShape shape, const uint32_t* n_dev) {
+  if (!n_items) return;
+  const uint32_t threads = shape.threads;
+  k_probe_write<<<probeBlocks(n_items, threads / kWave, 1), threads, 0, s>>>(
+      cfg, B, C, pr, wi_pair, wi_off, wi_list, n_items, wi_cbase, lane_n, cand, n_dev);
 }
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,

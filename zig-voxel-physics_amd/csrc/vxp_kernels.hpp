@@ -130,11 +130,20 @@ void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_of
 struct ProbeShape {
   uint32_t threads, xcd_run;
 };
-void launch_probe(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
-                  const PairRec* pr, const uint32_t* wi_pair, const uint32_t* wi_off,
-                  uint32_t total_wi, const uint32_t* wi_cbase, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                  uint8_t* lane_n, Cand* cand, bool count, ProbeShape shape,
-                  const uint32_t* n_dev = nullptr);
+// The probe's count pass over total_wi work items (lane_n, wi_ccnt, wi_vchk;
+// the work items with candidates appended to wi_list, their number at
+// *list_n, which starts at 0) and its write pass over the n_items work items
+// of wi_list (*n_dev when given, at most n_items).
+void launch_probe_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                        const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
+                        const uint32_t* wi_off, uint32_t total_wi, uint32_t* wi_ccnt,
+                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* wi_list, uint32_t* list_n,
+                        ProbeShape shape, const uint32_t* n_dev = nullptr);
+void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
+                        const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
+                        const uint32_t* wi_off, const uint32_t* wi_list, uint32_t n_items,
+                        const uint32_t* wi_cbase, const uint8_t* lane_n, Cand* cand,
+                        ProbeShape shape, const uint32_t* n_dev = nullptr);
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
