@@ -905,7 +905,7 @@ static int stepAllImpl(vxp_system* s) {
     launch_thread_mode(st, cfg, B, ws, keys, npairs, npairs ? s->d_arank.as<uint32_t>() : nullptr,
                        s->cfg.pair_cap, nw);
   launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, drop,
-                      s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(),
+                      s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(),
                       s->d_wlist.as<uint32_t>(), s->d_step_ctr.as<uint32_t>(),
                       s->d_counts.as<uint32_t>());
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
@@ -1405,8 +1405,7 @@ static int recSeg1(vxp_system* s, const RecCtx& c) {
   }
   launch_pair_process(st, c.cfg, c.B, c.C, c.ws, c.st, keys, c.P, s->d_pr.as<PairRec>(), c.prev,
                       nullptr, s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(),
-                      s->d_rstat.as<RStat>(), s->d_wlist.as<uint32_t>(), ctr,
-                      s->d_counts.as<uint32_t>(), c.npairs);
+                      s->d_wlist.as<uint32_t>(), ctr, s->d_counts.as<uint32_t>(), c.npairs);
   HIP_TRY(exclusive_scan_u32_n(tmp, s->d_cflag.as<uint32_t>(), s->d_cidx.as<uint32_t>(), c.P + 1,
                                c.npairs, 1, st));
   launch_publish4(st, s->d_read + kPub1, c.npairs, c.maxw, ctr + kCtrChunkBound, c.rsize + 2, 1);
@@ -1464,7 +1463,7 @@ static int recSeg3(vxp_system* s, const RecCtx& c) {
                 s->d_wi_vchk.as<uint32_t>(), s->d_cand.as<Cand>(), c.prev,
                 s->d_cache_keys.as<CacheKeyRec>(), s->d_cache_vals.as<CacheVal>(),
                 s->d_cstage.as<CacheVal>(), s->d_staging.as<DevManifold>(),
-                s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(), c.npairs);
+                s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>());
   HIP_TRY(exclusive_scan_u32_n(tmp, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(), c.P,
                                c.npairs, 0, st));
   launch_world_segments(st, c.ws, c.nw, s->d_total.as<uint32_t>());

@@ -940,14 +940,13 @@ struct PairCounts {
 
 __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                                   WorldState* ws, const uint64_t* keys, uint32_t n_pairs,
-                                  uint32_t p, PairRec* pr, const DevManifold* prev,
+                                  uint32_t p, PairRec& r, const DevManifold* prev,
                                   const uint8_t* drop) {
   PairCounts cnt{0, 0, 0};
   uint64_t key = keys[p];
   uint32_t w = keyWorld(key), ida = keyA(key), idb = keyB(key);
   if (p == 0 || keyWorld(keys[p - 1]) != w) ws[w].pair_lo = p;
   if (p == n_pairs - 1 || keyWorld(keys[p + 1]) != w) ws[w].pair_hi = p + 1;
-  PairRec r;
   r.world = w;
   r.ida = ida;
   r.idb = idb;
@@ -963,7 +962,6 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
   r.sa = (uint32_t)sa;
   r.sb = (uint32_t)sb;
   if (ws[w].skip || sa < 0 || sb < 0) {
-    pr[p] = r;
     return cnt;
   }
   cnt.bp = 1;
@@ -976,11 +974,9 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
       r.flag = PF_CARRY;
       r.aux = (uint32_t)j;
     }
-    pr[p] = r;
     return cnt;
   }
   if (drop && drop[p]) {  // past the filtered-pair array (src/system.zig:594,611-614)
-    pr[p] = r;
     return cnt;
   }
   if (cfg.threaded) {
@@ -990,14 +986,12 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
     const WorldState& W = ws[w];
     if ((uint32_t)sa >= W.thr_cut || (uint32_t)sb >= W.thr_cut) {
       cnt.tested = W.thr_par;
-      pr[p] = r;
       return cnt;
     }
   }
   cnt.tested = 1;
   int32_t ca = B.collider[ga], cb = B.collider[gb];
   if (ca < 0 || cb < 0) {
-    pr[p] = r;
     return cnt;
   }
   const DevCollider& HA = C.hdr[ca];
@@ -1008,7 +1002,6 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
     Obb oa = obbFromBox(colliderBox(HA), pa, qa);
     Obb ob = obbFromBox(colliderBox(HB), pb, qb);
     if (satSeparated(oa, ob)) {
-      pr[p] = r;
       return cnt;
     }
   }
@@ -1055,7 +1048,6 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
       r.nchunks = (uint32_t)((nv + kWave - 1) / kWave);
     }
   }
-  pr[p] = r;
   return cnt;
 }
 
@@ -1065,8 +1057,8 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
 __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
                                const DevManifold* prev, const uint8_t* drop, uint32_t* cflag,
-                               uint32_t* produce, RStat* rstat, uint32_t* wlist,
-                               uint32_t* step_ctr, uint32_t* counts, const uint32_t* n_dev) {
+                               uint32_t* produce, uint32_t* wlist, uint32_t* step_ctr,
+                               uint32_t* counts, const uint32_t* n_dev) {
   __shared__ uint32_t s_n, s_base, s_chunks;
   n_pairs = devCount(n_pairs, n_dev);
   // block-strided (a grid smaller than the pairs, as in a recorded step, loops)
@@ -1079,21 +1071,24 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
   __syncthreads();
   uint32_t pos = 0;
   if (p < n_pairs) {
-    c = pairProcess(cfg, B, C, ws, keys, n_pairs, p, pr, prev, drop);
-    w = keyWorld(keys[p]);
-    const uint32_t f = pr[p].flag;
+    PairRec r;
+    c = pairProcess(cfg, B, C, ws, keys, n_pairs, p, r, prev, drop);
+    w = r.world;
+    const uint32_t f = r.flag;
+    // only the pairs with work have readers of their record (the reduction's
+    // work list, the cache replay, the probe)
+    if (f != PF_NONE) pr[p] = r;
     cflag[p] = f == PF_CACHE;
     // work items of the voxel probe: the region's chunks when the pair runs the
     // detection (a cache pair's count is set by the replay when it misses)
-    counts[p] = f == PF_DETECT ? pr[p].nchunks : 0u;
+    counts[p] = f == PF_DETECT ? r.nchunks : 0u;
     if (p == n_pairs - 1) cflag[n_pairs] = 0;
     produce[p] = 0;
-    rstat[p] = RStat{0u, 0u, 0u, 0u, 0u, 0u};
     work = f == PF_CARRY || f == PF_DETECT || f == PF_CACHE;
     if (work) pos = atomicAdd(&s_n, 1u);
     // work items of this pair if it runs the detection (a cache pair only on a
     // miss): the step's bound on the probe's work items
-    if ((f == PF_DETECT || f == PF_CACHE) && pr[p].nchunks) atomicAdd(&s_chunks, pr[p].nchunks);
+    if ((f == PF_DETECT || f == PF_CACHE) && r.nchunks) atomicAdd(&s_chunks, r.nchunks);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -2082,14 +2077,17 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, DevColl
 // BrickPairCache.put of this frame's misses (entry owner = the pair that
 // reserved it during the replay), and the per-world narrow-phase counters.
 __global__ void k_reduce_tail(DevCfg cfg, WorldState* ws, vxp_stats* st, const PairRec* pr,
-                              uint32_t n_pairs, const CacheKeyRec* keys, CacheVal* vals,
-                              const CacheVal* cstage, const RStat* rstat, const uint32_t* n_dev) {
-  n_pairs = devCount(n_pairs, n_dev);
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n_pairs; base += gridDim.x * blockDim.x) {
-  const uint32_t p = base + threadIdx.x;
+                              uint32_t n_pairs, const uint32_t* wlist, const uint32_t* step_ctr,
+                              const CacheKeyRec* keys, CacheVal* vals, const CacheVal* cstage,
+                              const RStat* rstat) {
+  // the reduction's work list (the pairs with a record and counters)
+  const uint32_t n = min(step_ctr[kCtrReduceWork], n_pairs);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+  const uint32_t i = base + threadIdx.x;
   RStat c{0u, 0u, 0u, 0u, 0u, 0u};
   uint32_t w = 0;
-  if (p < n_pairs) {
+  if (i < n) {
+    const uint32_t p = wlist[i];
     const PairRec& r = pr[p];
     w = r.world;
     c = rstat[p];
@@ -4461,12 +4459,12 @@ void launch_sweep_pairs(hipStream_t s, WorldState* ws, const float4* slo, const 
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
-                         const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
+                         const uint8_t* drop, uint32_t* cflag, uint32_t* produce,
                          uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
                          const uint32_t* n_dev) {
   if (n)
-    k_pair_process<<<gridFor(n, 128, n_dev, kLoopBlocks), 128, 0, s>>>(cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag,
-                                                produce, rstat, wlist, step_ctr, counts, n_dev);
+    k_pair_process<<<gridFor(n, 128, n_dev, kLoopBlocks), 128, 0, s>>>(
+        cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag, produce, wlist, step_ctr, counts, n_dev);
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
@@ -4520,14 +4518,13 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const D
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, RStat* rstat,
-                   const uint32_t* n_dev) {
+                   DevManifold* staging, uint32_t* produce, RStat* rstat) {
   if (!n) return;
   uint32_t blocks = std::min<uint32_t>(2048, cdiv(n, 4));
   k_reduce<<<blocks, 256, 0, s>>>(cfg, B, C, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase, wi_ccnt,
                                   wi_vchk, cand, prev, cvals, cstage, staging, produce, rstat);
-  k_reduce_tail<<<gridFor(n, 256, n_dev, kLoopBlocks), 256, 0, s>>>(cfg, ws, st, pr, n, ckeys, cvals,
-                                                                     cstage, rstat, n_dev);
+  k_reduce_tail<<<gridFor(n, 256, step_ctr, kLoopBlocks), 256, 0, s>>>(
+      cfg, ws, st, pr, n, wlist, step_ctr, ckeys, cvals, cstage, rstat);
 }
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total) {
   k_world_segments<<<1, 1024, 0, s>>>(ws, nw, total);

@@ -69,7 +69,7 @@ void launch_expand_keys32(hipStream_t s, const uint32_t* in, uint64_t* keys, uin
 void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
-                         const uint8_t* drop, uint32_t* cflag, uint32_t* produce, RStat* rstat,
+                         const uint8_t* drop, uint32_t* cflag, uint32_t* produce,
                          uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
                          const uint32_t* n_dev = nullptr);
 // counts[p] (the probe's work items of pair p) is set for the misses.
@@ -151,8 +151,7 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const D
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, RStat* rstat,
-                   const uint32_t* n_dev = nullptr);
+                   DevManifold* staging, uint32_t* produce, RStat* rstat);
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total);
 void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
                     const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
