@@ -227,7 +227,7 @@ int uploadColliders(vxp_system* s) {
   HIP_TRY(hipMemcpy(s->d_dir.p, dir.data(), dir.size() * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d_vox.p, vox.data(), vox.size(), hipMemcpyHostToDevice));
   const uint32_t nb = (uint32_t)(vox.size() / 512);
-  HIP_TRY(s->d_mask.ensure((size_t)nb * 24 * 8));
+  HIP_TRY(s->d_mask.ensure((size_t)nb * kMaskWords * 8));
   launch_brick_masks(s->stream, s->d_vox.as<uint8_t>(), nb, s->d_mask.as<uint64_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s->stream));

@@ -81,11 +81,13 @@ struct DevBodies {
   int32_t* idmap;  // [world * id_cap + id] -> slot or -1
 };
 
+constexpr uint32_t kMaskWords = 32;  // brick mask words: 4 planes of 8 z slices
 struct DevColliders {
   const DevCollider* hdr;
   const int32_t* dir;
   const uint8_t* vox;     // 512 bytes per brick
-  const uint64_t* mask;   // 24 words per brick: non-empty, corner, edge; bit x + 8y of word z
+  const uint64_t* mask;   // kMaskWords per brick: non-empty, corner, edge, inside planes
+                          // of 8 words; bit x + 8y of word z
 };
 
 // Per-world step bookkeeping.

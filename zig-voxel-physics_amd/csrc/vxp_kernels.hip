@@ -1391,11 +1391,12 @@ __device__ __forceinline__ void ctxRots(ProbeCtx& c) {
 // in-bounds B voxels.  The window spans at most two bricks per axis: one
 // directory read per brick and one read per mask plane per z slice, which
 // covers the slice's rows (bit x + 8y of the slice's word).
+template <bool kInside>
 __device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
                                               const uint64_t* __restrict__ mask, const VoxMap& mb,
                                               const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
                                               uint32_t ta, V3& wa, int32_t cb[3],
-                                              uint32_t* vchk) {
+                                              uint32_t* vchk, uint32_t* inside = nullptr) {
   wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
   V3 lb = invXformBy(c.ib, c.pb, wa);
   localToVoxel(c.vsb, lb, cb);
@@ -1405,7 +1406,8 @@ __device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
     lo[k] = max(cb[k] - 1, c.bminb[k]);
     hi[k] = min(cb[k] + 1, c.bmaxb[k] - 1);
   }
-  uint32_t pm = 0, vc = 0;
+  uint32_t pm = 0, vc = 0, im = 0;
+  if (kInside) *inside = 0;
   if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) {
     *vchk = 0;
     return 0;
@@ -1423,24 +1425,28 @@ __device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
         const int32_t ya = max(lo[1], yb * 8), ye = min(hi[1], yb * 8 + 7);
         const int32_t za = max(lo[2], zb * 8), ze = min(hi[2], zb * 8 + 7);
         const uint32_t wmask = (1u << (xe - xa + 1)) - 1u;
-        const uint64_t* mz = mask + (size_t)bi * 24;
+        const uint64_t* mz = mask + (size_t)bi * kMaskWords;
         for (int32_t bz = za; bz <= ze; bz++) {
           const uint64_t w0 = mz[bz & 7];
-          uint64_t wt = ~0ull;
+          uint64_t wt = ~0ull, wi = 0;
           if (ta != VXP_VOXEL_CORNER) {
             wt = mz[8 + (bz & 7)];
             if (ta == VXP_VOXEL_EDGE) wt |= mz[16 + (bz & 7)];
+          } else if (kInside) {
+            wi = mz[24 + (bz & 7)];  // only an A corner takes inside B voxels
           }
           for (int32_t by = ya; by <= ye; by++) {
             const uint32_t sh = 8u * (uint32_t)(by & 7) + (uint32_t)(xa & 7);
             const uint32_t occ = (uint32_t)(w0 >> sh) & wmask;
+            const uint32_t at = (bz - cb[2] + 1) * 9 + (by - cb[1] + 1) * 3 + (xa - cb[0] + 1);
             vc += (uint32_t)__builtin_popcount(occ);
-            pm |= (occ & (uint32_t)(wt >> sh))
-                  << ((bz - cb[2] + 1) * 9 + (by - cb[1] + 1) * 3 + (xa - cb[0] + 1));
+            pm |= (occ & (uint32_t)(wt >> sh)) << at;
+            if (kInside) im |= (occ & (uint32_t)(wi >> sh)) << at;
           }
         }
       }
   *vchk = vc;
+  if (kInside) *inside = im;
   return pm;
 }
 // B voxel of probe k around the B voxel cb.
@@ -1460,7 +1466,7 @@ __device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
                                                int32_t y, int32_t z, uint8_t va, uint32_t* vchk) {
   V3 wa;
   int32_t cb[3];
-  uint32_t pm = probeMask(dir, mask, mb, c, x, y, z, va & 7u, wa, cb, vchk);
+  uint32_t pm = probeMask<false>(dir, mask, mb, c, x, y, z, va & 7u, wa, cb, vchk);
   uint32_t n = 0;
   while (pm) {
     const uint32_t k = (uint32_t)__builtin_ctz(pm);
@@ -1477,19 +1483,17 @@ __device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
 
 // The candidates of checkNeighborhood: the first `limit` of A voxel
 // (x, y, z) in probe order, as (penetration, feature id, lane << 5 | probe).
-// The B voxel's byte is read only for a candidate whose penetration rule
-// needs its type (an A voxel other than INSIDE).
+// The penetration rule's INSIDE test of the B voxel reads the inside plane.
 __device__ __forceinline__ void probeWrite(const int32_t* __restrict__ dir,
-                                           const uint64_t* __restrict__ mask,
-                                           const uint8_t* __restrict__ vox, const VoxMap& mb,
+                                           const uint64_t* __restrict__ mask, const VoxMap& mb,
                                            const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
                                            uint8_t va, uint32_t lane, Cand* __restrict__ out,
                                            uint32_t limit) {
   V3 wa;
   int32_t cb[3];
-  uint32_t vc;
+  uint32_t vc, im;
   const uint32_t ta = va & 7u;
-  uint32_t pm = probeMask(dir, mask, mb, c, x, y, z, ta, wa, cb, &vc);
+  uint32_t pm = probeMask<true>(dir, mask, mb, c, x, y, z, ta, wa, cb, &vc, &im);
   uint32_t n = 0;
   while (pm && n < limit) {
     const uint32_t k = (uint32_t)__builtin_ctz(pm);
@@ -1503,9 +1507,7 @@ __device__ __forceinline__ void probeWrite(const int32_t* __restrict__ dir,
     // the penetration of generateContact (src/narrow_phase.zig:879-906)
     float dist = sqrtf(dsq);
     float pen = c.vsa - dist;
-    bool inside = ta == VXP_VOXEL_INSIDE;
-    if (!inside) inside = (voxelAt(dir, vox, mb, bx, by, bz) & 7u) == VXP_VOXEL_INSIDE;
-    if (inside) pen = fmaxf(pen, c.vsa);
+    if (ta == VXP_VOXEL_INSIDE || ((im >> k) & 1u)) pen = fmaxf(pen, c.vsa);
     uint64_t fid = featureId(x, y, z, bx, by, bz);
     Cand& o = out[n];
     o.pen = pen;
@@ -1759,7 +1761,9 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
       const int f = __ffsll((long long)rem) - 1;
       const uint32_t w0 = rdlane(ew, f);
       const bool mine = act && ew == w0;
-      const uint32_t sn = waveSum(mine ? n : 0u), sv = waveSum(mine ? vc : 0u);
+      // a work item's sums are below 64 * 27: both in one reduction
+      const uint32_t s2 = waveSum(mine ? (n << 16) | vc : 0u);
+      const uint32_t sn = s2 >> 16, sv = s2 & 0xFFFFu;
       if ((int)lane == f) {
         if (sn) atomicAdd(wi_ccnt + w0, sn);
         if (sv) atomicAdd(wi_vchk + w0, sv);
@@ -1840,7 +1844,7 @@ __global__ __launch_bounds__(256) void k_probe_write(DevCfg cfg, DevBodies B, De
       int32_t x, y, z;
       probeVoxelXYZ(q, wid, lane, x, y, z);
       const uint8_t va = voxelAt(C, q.HA, x, y, z);
-      probeWrite(C.dir, C.mask, C.vox, voxMap(q.HB), q.c, x, y, z, va, lane,
+      probeWrite(C.dir, C.mask, voxMap(q.HB), q.c, x, y, z, va, lane,
                  cand + sload(wi_cbase + wid) + (uint32_t)incl - n, n);
     }
   }
@@ -3693,28 +3697,30 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_final(DevCfg cfg, DevBo
 // ---------------------------------------------------------------------------
 
 // Per-brick voxel masks of the brick pool: words 0..7 non-empty, 8..15 corner,
-// 16..23 edge voxels; bit x + 8y of word z.  Block per brick, thread per
-// (y, z) row of 8 voxels.
+// 16..23 edge, 24..31 inside voxels; bit x + 8y of word z.  Block per brick,
+// thread per (y, z) row of 8 voxels.
 __global__ __launch_bounds__(64) void k_brick_masks(const uint8_t* vox, uint64_t* mask) {
-  __shared__ uint8_t rows[3][64];
+  __shared__ uint8_t rows[4][64];
   const uint32_t b = blockIdx.x, t = threadIdx.x;  // t = y + 8z
   const uint64_t w = reinterpret_cast<const uint64_t*>(vox + (size_t)b * 512)[t];
-  uint8_t occ = 0, cor = 0, edg = 0;
+  uint8_t occ = 0, cor = 0, edg = 0, ins = 0;
   for (int x = 0; x < 8; x++) {
     const uint32_t ty = (uint32_t)(w >> (8 * x)) & 7u;
     occ |= ty != 0 ? 1u << x : 0u;
     cor |= ty == VXP_VOXEL_CORNER ? 1u << x : 0u;
     edg |= ty == VXP_VOXEL_EDGE ? 1u << x : 0u;
+    ins |= ty == VXP_VOXEL_INSIDE ? 1u << x : 0u;
   }
   rows[0][t] = occ;
   rows[1][t] = cor;
   rows[2][t] = edg;
+  rows[3][t] = ins;
   __syncthreads();
-  if (t < 24) {
+  if (t < kMaskWords) {
     const uint32_t k = t / 8, z = t % 8;
     uint64_t word = 0;
     for (int y = 0; y < 8; y++) word |= (uint64_t)rows[k][y + 8 * z] << (8 * y);
-    mask[(size_t)b * 24 + t] = word;
+    mask[(size_t)b * kMaskWords + t] = word;
   }
 }
 
