@@ -10,6 +10,11 @@
                                 of the marker kernel on: one section of a run)
   python3 vxp/profsum.py copycal      (the calibration program: five 256 MiB
                                        device copies, profiled like the bench)
+  python3 vxp/profsum.py scene <c1|c3> <frames>
+                               (a program to profile: the 4096-body scene or the
+                                512-world batch for the first `frames` frames,
+                                printing the per-frame means of the pair,
+                                manifold, candidate and checked-voxel counts)
 """
 import sqlite3
 import sys
@@ -107,9 +112,31 @@ def copycal():
     print("copied", n * 4, "bytes x5", flush=True)
 
 
+def scene(name, frames):
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import vxp
+    from vxp import scenes
+    sc = {"c1": scenes.c1_large, "c3": scenes.batched_worlds}[name]()
+    s = vxp.from_scene(sc, lib=vxp.product_lib())
+    keys = ("broad_phase_pairs", "narrow_phase_tested", "manifolds", "contact_candidates",
+            "voxels_checked")
+    acc = dict.fromkeys(keys, 0)
+    for _ in range(frames):
+        s.update(sc.dt)
+        for st in s.stats_all():
+            for k in keys:
+                acc[k] += getattr(st, k)
+    print(f"{sc.name}: frames 0..{frames - 1}, per-frame means:",
+          {k: round(v / frames, 1) for k, v in acc.items()}, flush=True)
+    s.close()
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "copycal":
         copycal()
+    elif sys.argv[1] == "scene":
+        scene(sys.argv[2], int(sys.argv[3]))
     elif sys.argv[1] == "last":
         print(last(sys.argv[2], sys.argv[3], int(sys.argv[4]),
                    sys.argv[5] if len(sys.argv) > 5 else None))
