@@ -319,6 +319,12 @@ def main():
               "ms_per_step": r0["wall"] * 1e3 / len(r0["frame_ms"]),
               "p50_step_ms": statistics.median(r0["frame_ms"]),
               "p99_step_ms": pct(r0["frame_ms"], 0.99)}
+        if not args.no_recorded:  # the same frames replayed from recorded graphs
+            s0 = vxp.from_scene(s0c, lib=lib)
+            s0.setRecorded(True)
+            r = run(s0, s0c, args.warmup, s0c.frames - args.warmup)
+            c0["recorded"] = graph_line(r, s0, s0c.frames - args.warmup)
+            s0.close()
         if not args.no_cpu:
             o0 = vxp.from_scene(s0c, lib=vxp.oracle_lib(), cfg_over=dict(cpu_threads=threads))
             rc0 = cpu_run(o0, s0c, args.warmup, s0c.frames - args.warmup, args.cpu_seconds)
