@@ -949,7 +949,11 @@ static int stepAllImpl(vxp_system* s) {
   HIP_TRY(s->d_wi_vchk.ensure((size_t)std::max(total_wi, 1u) * 4));
   HIP_TRY(s->d_lane_n.ensure((size_t)std::max(total_wi, 1u) * 64));
   HIP_TRY(s->d_wi_pair.ensure((size_t)std::max(total_wi, 1u) * 4));
-  HIP_TRY(s->d_wi_list.ensure((size_t)std::max(total_wi, 1u) * 4));
+  if (total_wi >= (1u << 26)) {  // lane-list entries are work item * 64 + lane in 32 bits
+    g_err = "probe: more than 2^26 work items in one step";
+    return VXP_ERR_CAPACITY;
+  }
+  HIP_TRY(s->d_wi_list.ensure((size_t)std::max(total_wi, 1u) * 64 * 4));
   // pass 1 counts and lists the work items with candidates, the scan places
   // every work item's candidates, pass 2 writes those of the listed ones
   pe = profBegin(s, VXP_K_PROBE);
@@ -981,7 +985,7 @@ static int stepAllImpl(vxp_system* s) {
     launch_probe_write(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(),
                        s->d_wi_off.as<uint32_t>(), s->d_wi_list.as<uint32_t>(), nlist,
                        s->d_wi_cbase.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
-                       s->d_cand.as<Cand>(), shape);
+                       s->d_cand.as<Cand>());
   RC(tuneMark(s, vxp_system::kPhProbe, false));
   profEnd(s, VXP_K_PROBE, pe);
   s->last_wi = total_wi;
@@ -1252,7 +1256,11 @@ static int recEnsure(vxp_system* s) {
   HIP_TRY(s->d_wi_vchk.ensure(WI * 4));
   HIP_TRY(s->d_lane_n.ensure(WI * 64));
   HIP_TRY(s->d_wi_pair.ensure(WI * 4));
-  HIP_TRY(s->d_wi_list.ensure(WI * 4));
+  if (WI >= (1u << 26)) {  // lane-list entries are work item * 64 + lane in 32 bits
+    g_err = "recorded step: more than 2^26 probe work items";
+    return VXP_ERR_CAPACITY;
+  }
+  HIP_TRY(s->d_wi_list.ensure(WI * 64 * 4));
   HIP_TRY(s->d_cand.ensure(CA * sizeof(Cand)));
   HIP_TRY(s->d_total.ensure(64));
   HIP_TRY(s->d_ml[0].ensure(M * sizeof(DevManifold), true, s->stream));
@@ -1454,9 +1462,9 @@ static int recSeg3(vxp_system* s, const RecCtx& c) {
   const ProbeShape shape{s->tune.use[vxp_system::kProbeThreads],
                          s->tune.use[vxp_system::kProbeXcdRun]};
   launch_probe_write(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(),
-                     s->d_wi_off.as<uint32_t>(), s->d_wi_list.as<uint32_t>(), c.WI,
+                     s->d_wi_off.as<uint32_t>(), s->d_wi_list.as<uint32_t>(), c.WI * 64,
                      s->d_wi_cbase.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
-                     s->d_cand.as<Cand>(), shape, ctr + kCtrProbeItems);
+                     s->d_cand.as<Cand>(), ctr + kCtrProbeItems);
   launch_reduce(st, c.cfg, c.B, c.C, c.ws, c.st, pr, s->d_pairs.as<uint64_t>(), c.P,
                 s->d_wlist.as<uint32_t>(), ctr, s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),

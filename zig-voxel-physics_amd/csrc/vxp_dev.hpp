@@ -19,7 +19,7 @@ constexpr uint32_t kUnitsPerWorld = 64;  // solve units per world, at most
 constexpr uint32_t kCtrSolveUnits = 1;  // solve units queued by the solver's prep
 constexpr uint32_t kCtrSolveUnitPull = 2;  // solve units taken by workgroups
 constexpr uint32_t kCtrChunkBound = 3;  // probe work items if every cache-eligible pair missed
-constexpr uint32_t kCtrProbeItems = 4;  // probe work items with candidates (the write pass's list)
+constexpr uint32_t kCtrProbeItems = 4;  // probe lanes with candidates (the write pass's list)
 constexpr uint32_t kStepCounters = 5;
 
 // Collider header.  The brick directory is a dense grid over brick positions

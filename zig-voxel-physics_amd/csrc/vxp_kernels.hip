@@ -1711,31 +1711,33 @@ __device__ __forceinline__ void probeRange(uint32_t total_wi, uint32_t xcd_run, 
 
 // Count pass: per-lane candidate counts (lane_n), per-work-item totals
 // wi_ccnt and voxels_checked sums wi_vchk (both zeroed by k_wi_fill),
-// wi_ccnt[total_wi] = 0, and the work items with candidates on wi_list.  Most region voxels of A are empty, so the lanes
-// are compacted before the neighbourhood test: each wave queues its
-// non-empty A voxels (work item, lane, type) in LDS and runs probeCount on
-// 64 of them at a time (fewer at a pair's end), adding each work item's sums
-// with one atomic per work item and batch.
+// wi_ccnt[total_wi] = 0, and the lanes with candidates (work item * 64 +
+// lane) on lane_list.  Most region voxels of A are solid but a work item's
+// lanes past the region or on empty voxels would idle, so the lanes are
+// compacted before the neighbourhood test: each wave queues its non-empty A
+// voxels (work item, lane, type) in LDS and runs probeCount on 64 of them at a
+// time (fewer at a pair's end), adding each work item's sums with one atomic
+// per work item and batch.
 __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B, DevColliders C,
                                                      const PairRec* pr, const uint32_t* wi_pair,
                                                      const uint32_t* wi_off, uint32_t total_wi,
                                                      uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                                                     uint8_t* lane_n, uint32_t* wi_list,
+                                                     uint8_t* lane_n, uint32_t* lane_list,
                                                      uint32_t* list_n, uint32_t xcd_run,
                                                      const uint32_t* n_dev) {
-  __shared__ uint32_t q_wid[4][kWave], q_lv[4][kWave], l_wid[4][kWave];
+  __shared__ uint32_t q_wid[4][kWave], q_lv[4][kWave], l_ent[4][2 * kWave];
   const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   uint32_t* const qw = q_wid[wv];
   uint32_t* const ql = q_lv[wv];
-  uint32_t* const lw = l_wid[wv];
-  // the work items with candidates gather in lw and go to wi_list 64 at a
-  // time (or at the end), with one returning atomic per flush
-  uint32_t ln = 0, last_listed = 0xFFFFFFFFu;
+  uint32_t* const le = l_ent[wv];
+  // the lanes with candidates gather in le and go to lane_list once 64 are
+  // there (or at the end), with one returning atomic per flush
+  uint32_t ln = 0;
   auto listFlush = [&]() {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(list_n, ln);
     base = rdlane(base, 0);
-    if (lane < ln) wi_list[base + lane] = lw[lane];
+    for (uint32_t i = lane; i < ln; i += kWave) lane_list[base + i] = le[i];
     ln = 0;
   };
   total_wi = devCount(total_wi, n_dev);
@@ -1755,6 +1757,12 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
       n = probeCount(C.dir, C.mask, voxMap(q.HB), q.c, x, y, z, (uint8_t)(el >> 8), &vc);
       lane_n[(size_t)ew * kWave + (el & (kWave - 1))] = (uint8_t)n;
     }
+    const uint64_t hm = __ballot(n != 0u);
+    if (hm) {
+      if (n) le[ln + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull))] = ew * kWave + (el & (kWave - 1));
+      ln += (uint32_t)__popcll(hm);
+      if (ln >= kWave) listFlush();
+    }
     // entries are in work-item order: one sum per work item of the batch
     uint64_t rem = __ballot(act);
     while (rem) {
@@ -1769,13 +1777,6 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
         if (sv) atomicAdd(wi_vchk + w0, sv);
       }
       rem &= ~__ballot(mine);
-      // a work item's entries are consecutive (at most two batches), so its
-      // first non-zero sum lists it
-      if (sn && w0 != last_listed) {
-        if ((int)lane == f) lw[ln] = w0;
-        last_listed = w0;
-        if (++ln == kWave) listFlush();
-      }
     }
   };
   for (uint32_t wid = w_lo; wid < w_hi; wid++) {
@@ -1815,38 +1816,44 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
   if (ln) listFlush();
 }
 
-// Write pass: the candidates of every work item on wi_list (those with a
-// non-zero count) written from wi_cbase[wid] (the exclusive scan of wi_ccnt)
-// in (lane, probe) order.  One list entry per wave at a time.
+// Write pass: one thread per listed lane (work item * 64 + lane, a lane with
+// candidates): its pair's context, its offset wi_cbase[wid] plus the counts
+// of the work item's lower lanes, and its candidates in probe order.
 __global__ __launch_bounds__(256) void k_probe_write(DevCfg cfg, DevBodies B, DevColliders C,
                                                      const PairRec* pr, const uint32_t* wi_pair,
                                                      const uint32_t* wi_off,
-                                                     const uint32_t* wi_list, uint32_t n_items,
+                                                     const uint32_t* lane_list, uint32_t n_items,
                                                      const uint32_t* wi_cbase,
                                                      const uint8_t* lane_n, Cand* cand,
                                                      const uint32_t* n_dev) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
   n_items = devCount(n_items, n_dev);
-  const uint32_t wpb = blockDim.x / kWave, nwaves = gridDim.x * wpb;
-  const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + threadIdx.x / kWave);
-  ProbePair q{};
-  uint32_t pcur = 0xFFFFFFFFu;
-  for (uint32_t i = g; i < n_items; i += nwaves) {
-    const uint32_t wid = sload(wi_list + i);
-    const uint32_t p = sload(wi_pair + wid);
-    if (p != pcur) {
-      pcur = p;
-      loadProbePair(q, cfg, B, C, pr, wi_off, p);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t e = lane_list[i], wid = e / kWave, lane = e % kWave;
+    // counts of the lanes below this one: bytes of the work item's 64 count
+    // bytes, four per word (sums stay below 2^8 per word: 4 * 27)
+    const uint4* row = reinterpret_cast<const uint4*>(lane_n + (size_t)wid * kWave);
+    uint32_t below = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      const uint4 v = row[k];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) {
+        const uint32_t first = 16 * k + 4 * j;  // lane of the word's low byte
+        const uint32_t keep = lane <= first ? 0u : lane - first >= 4 ? 4u : lane - first;
+        const uint32_t m = keep == 4u ? 0xFFFFFFFFu : (1u << (8 * keep)) - 1u;
+        below += ((w[j] & m) * 0x01010101u) >> 24;
+      }
     }
-    const uint32_t n = lane_n[(size_t)wid * kWave + lane];
-    const int incl = waveInclusiveScan((int)n, (int)lane);
-    if (n > 0) {
-      int32_t x, y, z;
-      probeVoxelXYZ(q, wid, lane, x, y, z);
-      const uint8_t va = voxelAt(C, q.HA, x, y, z);
-      probeWrite(C.dir, C.mask, voxMap(q.HB), q.c, x, y, z, va, lane,
-                 cand + sload(wi_cbase + wid) + (uint32_t)incl - n, n);
-    }
+    const uint32_t n = lane_n[e];
+    ProbePair q;
+    loadProbePair(q, cfg, B, C, pr, wi_off, wi_pair[wid]);
+    int32_t x, y, z;
+    probeVoxelXYZ(q, wid, lane, x, y, z);
+    const uint8_t va = voxelAt(C, q.HA, x, y, z);
+    probeWrite(C.dir, C.mask, voxMap(q.HB), q.c, x, y, z, va, lane,
+               cand + wi_cbase[wid] + below, n);
   }
 }
 
@@ -4499,23 +4506,22 @@ static uint32_t probeBlocks(uint32_t items, uint32_t waves, uint32_t run) {
 void launch_probe_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
                         const uint32_t* wi_off, uint32_t total_wi, uint32_t* wi_ccnt,
-                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* wi_list, uint32_t* list_n,
-                        ProbeShape shape, const uint32_t* n_dev) {
+                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* lane_list,
+                        uint32_t* list_n, ProbeShape shape, const uint32_t* n_dev) {
   if (!total_wi) return;
   const uint32_t threads = shape.threads, run = shape.xcd_run ? shape.xcd_run : 1u;
   k_probe_count<<<probeBlocks(total_wi, threads / kWave, run), threads, 0, s>>>(
-      cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt, wi_vchk, lane_n, wi_list, list_n, run,
+      cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt, wi_vchk, lane_n, lane_list, list_n, run,
       n_dev);
 }
 void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
-                        const uint32_t* wi_off, const uint32_t* wi_list, uint32_t n_items,
+                        const uint32_t* wi_off, const uint32_t* lane_list, uint32_t n_items,
                         const uint32_t* wi_cbase, const uint8_t* lane_n, Cand* cand,
-                        ProbeShape shape, const uint32_t* n_dev) {
+                        const uint32_t* n_dev) {
   if (!n_items) return;
-  const uint32_t threads = shape.threads;
-  k_probe_write<<<probeBlocks(n_items, threads / kWave, 1), threads, 0, s>>>(
-      cfg, B, C, pr, wi_pair, wi_off, wi_list, n_items, wi_cbase, lane_n, cand, n_dev);
+  k_probe_write<<<gridFor(n_items, 256, n_dev, kLoopBlocks), 256, 0, s>>>(
+      cfg, B, C, pr, wi_pair, wi_off, lane_list, n_items, wi_cbase, lane_n, cand, n_dev);
 }
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,

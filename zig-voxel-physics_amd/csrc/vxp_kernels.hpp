@@ -131,19 +131,20 @@ struct ProbeShape {
   uint32_t threads, xcd_run;
 };
 // The probe's count pass over total_wi work items (lane_n, wi_ccnt, wi_vchk;
-// the work items with candidates appended to wi_list, their number at
-// *list_n, which starts at 0) and its write pass over the n_items work items
-// of wi_list (*n_dev when given, at most n_items).
+// the lanes with candidates appended to lane_list as work item * 64 + lane,
+// their number at *list_n, which starts at 0; total_wi below 2^26) and its
+// write pass over the n_items entries of lane_list (*n_dev when given, at
+// most n_items).
 void launch_probe_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
                         const uint32_t* wi_off, uint32_t total_wi, uint32_t* wi_ccnt,
-                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* wi_list, uint32_t* list_n,
-                        ProbeShape shape, const uint32_t* n_dev = nullptr);
+                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* lane_list,
+                        uint32_t* list_n, ProbeShape shape, const uint32_t* n_dev = nullptr);
 void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
-                        const uint32_t* wi_off, const uint32_t* wi_list, uint32_t n_items,
+                        const uint32_t* wi_off, const uint32_t* lane_list, uint32_t n_items,
                         const uint32_t* wi_cbase, const uint8_t* lane_n, Cand* cand,
-                        ProbeShape shape, const uint32_t* n_dev = nullptr);
+                        const uint32_t* n_dev = nullptr);
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
