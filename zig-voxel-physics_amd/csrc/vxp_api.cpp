@@ -872,6 +872,7 @@ static int stepAllImpl(vxp_system* s) {
   HIP_TRY(s->d_counts.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_wi_off.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_produce.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_pvchk.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_rstat.ensure((size_t)np1 * sizeof(RStat)));
   HIP_TRY(s->d_clist.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_wlist.ensure((size_t)np1 * 4));
@@ -906,7 +907,8 @@ static int stepAllImpl(vxp_system* s) {
                        s->cfg.pair_cap, nw);
   launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, drop,
                       s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(),
-                      s->d_wlist.as<uint32_t>(), s->d_step_ctr.as<uint32_t>(),
+                      s->d_pvchk.as<uint32_t>(), s->d_wlist.as<uint32_t>(),
+                      s->d_step_ctr.as<uint32_t>(),
                       s->d_counts.as<uint32_t>());
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
   pe = profBegin(s, VXP_K_CACHE_REPLAY);
@@ -946,7 +948,6 @@ static int stepAllImpl(vxp_system* s) {
   }
   HIP_TRY(s->d_wi_cbase.ensure((size_t)(total_wi + 1) * 4));
   HIP_TRY(s->d_wi_ccnt.ensure((size_t)(total_wi + 1) * 4));
-  HIP_TRY(s->d_wi_vchk.ensure((size_t)std::max(total_wi, 1u) * 4));
   HIP_TRY(s->d_lane_n.ensure((size_t)std::max(total_wi, 1u) * 64));
   HIP_TRY(s->d_wi_pair.ensure((size_t)std::max(total_wi, 1u) * 4));
   if (total_wi >= (1u << 26)) {  // lane-list entries are work item * 64 + lane in 32 bits
@@ -963,11 +964,10 @@ static int stepAllImpl(vxp_system* s) {
   uint32_t ncand = 0, nlist = 0;
   if (total_wi) {
     launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), npairs,
-                   s->d_wi_pair.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
-                   s->d_wi_vchk.as<uint32_t>());
+                   s->d_wi_pair.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>());
     launch_probe_count(st, cfg, B, C, pr, s->d_wi_pair.as<uint32_t>(),
                        s->d_wi_off.as<uint32_t>(), total_wi, s->d_wi_ccnt.as<uint32_t>(),
-                       s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
+                       s->d_pvchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
                        s->d_wi_list.as<uint32_t>(), s->d_step_ctr.as<uint32_t>() + kCtrProbeItems,
                        shape);
     size_t cb = exclusive_scan_u32_bytes(total_wi + 1);
@@ -993,7 +993,7 @@ static int stepAllImpl(vxp_system* s) {
   pe = profBegin(s, VXP_K_REDUCE);
   launch_reduce(st, cfg, B, C, ws, stats, pr, keys, npairs, s->d_wlist.as<uint32_t>(),
                 s->d_step_ctr.as<uint32_t>(), s->d_wi_off.as<uint32_t>(),
-                s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_wi_vchk.as<uint32_t>(),
+                s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(), s->d_pvchk.as<uint32_t>(),
                 s->d_cand.as<Cand>(), prev, s->d_cache_keys.as<CacheKeyRec>(),
                 s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>(),
                 s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>(),
@@ -1154,7 +1154,7 @@ static std::vector<DevBuf*> allBufs(vxp_system* s) {
           &s->d_cache_keys, &s->d_cache_vals, &s->d_skeys, &s->d_svals, &s->d_skeys2,
           &s->d_svals2, &s->d_sort_tmp, &s->d_slo, &s->d_shi, &s->d_send, &s->d_wcnt, &s->d_woff,
           &s->d_swc, &s->d_swo, &s->d_pairs_raw, &s->d_pairs, &s->d_pr, &s->d_counts, &s->d_wi_off,
-          &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_wi_vchk, &s->d_lane_n, &s->d_cand, &s->d_produce,
+          &s->d_wi_cbase, &s->d_wi_ccnt, &s->d_pvchk, &s->d_lane_n, &s->d_cand, &s->d_produce,
           &s->d_pidx, &s->d_staging, &s->d_cstage, &s->d_rstat, &s->d_clist, &s->d_cflag,
           &s->d_cidx, &s->d_wlist, &s->d_step_ctr, &s->d_total, &s->d_ml[0], &s->d_ml[1],
           &s->d_isl_parent, &s->d_isl_end, &s->d_isl_size, &s->d_ref_masks, &s->d_ref_colors,
@@ -1243,6 +1243,7 @@ static int recEnsure(vxp_system* s) {
   HIP_TRY(s->d_counts.ensure(P * 4));
   HIP_TRY(s->d_wi_off.ensure(P * 4));
   HIP_TRY(s->d_produce.ensure(P * 4));
+  HIP_TRY(s->d_pvchk.ensure(P * 4));
   HIP_TRY(s->d_rstat.ensure(P * sizeof(RStat)));
   HIP_TRY(s->d_clist.ensure(P * 4));
   HIP_TRY(s->d_wlist.ensure(P * 4));
@@ -1253,7 +1254,6 @@ static int recEnsure(vxp_system* s) {
   HIP_TRY(s->d_cstage.ensure(P * sizeof(CacheVal)));
   HIP_TRY(s->d_wi_cbase.ensure((WI + 1) * 4));
   HIP_TRY(s->d_wi_ccnt.ensure((WI + 1) * 4));
-  HIP_TRY(s->d_wi_vchk.ensure(WI * 4));
   HIP_TRY(s->d_lane_n.ensure(WI * 64));
   HIP_TRY(s->d_wi_pair.ensure(WI * 4));
   if (WI >= (1u << 26)) {  // lane-list entries are work item * 64 + lane in 32 bits
@@ -1413,7 +1413,8 @@ static int recSeg1(vxp_system* s, const RecCtx& c) {
   }
   launch_pair_process(st, c.cfg, c.B, c.C, c.ws, c.st, keys, c.P, s->d_pr.as<PairRec>(), c.prev,
                       nullptr, s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(),
-                      s->d_wlist.as<uint32_t>(), ctr, s->d_counts.as<uint32_t>(), c.npairs);
+                      s->d_pvchk.as<uint32_t>(), s->d_wlist.as<uint32_t>(), ctr,
+                      s->d_counts.as<uint32_t>(), c.npairs);
   HIP_TRY(exclusive_scan_u32_n(tmp, s->d_cflag.as<uint32_t>(), s->d_cidx.as<uint32_t>(), c.P + 1,
                                c.npairs, 1, st));
   launch_publish4(st, s->d_read + kPub1, c.npairs, c.maxw, ctr + kCtrChunkBound, c.rsize + 2, 1);
@@ -1438,10 +1439,10 @@ static int recSeg2(vxp_system* s, const RecCtx& c) {
                          s->tune.use[vxp_system::kProbeXcdRun]};
   launch_wi_fill(st, s->d_counts.as<uint32_t>(), s->d_wi_off.as<uint32_t>(), c.P,
                  s->d_wi_pair.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
-                 s->d_wi_vchk.as<uint32_t>(), c.npairs);
+                 c.npairs);
   launch_probe_count(st, c.cfg, c.B, c.C, pr, s->d_wi_pair.as<uint32_t>(),
                      s->d_wi_off.as<uint32_t>(), c.WI, s->d_wi_ccnt.as<uint32_t>(),
-                     s->d_wi_vchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
+                     s->d_pvchk.as<uint32_t>(), s->d_lane_n.as<uint8_t>(),
                      s->d_wi_list.as<uint32_t>(), s->d_step_ctr.as<uint32_t>() + kCtrProbeItems,
                      shape, c.rsize);
   HIP_TRY(exclusive_scan_u32_n(tmp, s->d_wi_ccnt.as<uint32_t>(), s->d_wi_cbase.as<uint32_t>(),
@@ -1468,7 +1469,7 @@ static int recSeg3(vxp_system* s, const RecCtx& c) {
   launch_reduce(st, c.cfg, c.B, c.C, c.ws, c.st, pr, s->d_pairs.as<uint64_t>(), c.P,
                 s->d_wlist.as<uint32_t>(), ctr, s->d_wi_off.as<uint32_t>(),
                 s->d_wi_cbase.as<uint32_t>(), s->d_wi_ccnt.as<uint32_t>(),
-                s->d_wi_vchk.as<uint32_t>(), s->d_cand.as<Cand>(), c.prev,
+                s->d_pvchk.as<uint32_t>(), s->d_cand.as<Cand>(), c.prev,
                 s->d_cache_keys.as<CacheKeyRec>(), s->d_cache_vals.as<CacheVal>(),
                 s->d_cstage.as<CacheVal>(), s->d_staging.as<DevManifold>(),
                 s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>());

@@ -138,7 +138,7 @@ struct vxp_system {
   // per world / per step device state
   DevBuf d_ws, d_stats, d_cache_keys, d_cache_vals;
   DevBuf d_skeys, d_svals, d_skeys2, d_svals2, d_sort_tmp;
-  DevBuf d_slo, d_shi, d_send, d_wcnt, d_woff, d_swc, d_swo, d_pairs_raw, d_pairs, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_wi_vchk, d_lane_n,
+  DevBuf d_slo, d_shi, d_send, d_wcnt, d_woff, d_swc, d_swo, d_pairs_raw, d_pairs, d_pr, d_counts, d_wi_off, d_wi_cbase, d_wi_ccnt, d_pvchk, d_lane_n,
       d_cand,
       d_produce, d_pidx, d_staging, d_cstage, d_total, d_rstat, d_clist, d_cflag, d_cidx, d_wlist, d_step_ctr;
   DevBuf d_lb_ilo, d_lb_ihi, d_lb_code, d_lb_child, d_lb_range, d_lb_iparent, d_lb_lparent,

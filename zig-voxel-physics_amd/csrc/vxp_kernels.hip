@@ -1057,8 +1057,8 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
 __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
                                const DevManifold* prev, const uint8_t* drop, uint32_t* cflag,
-                               uint32_t* produce, uint32_t* wlist, uint32_t* step_ctr,
-                               uint32_t* counts, const uint32_t* n_dev) {
+                               uint32_t* produce, uint32_t* p_vchk, uint32_t* wlist,
+                               uint32_t* step_ctr, uint32_t* counts, const uint32_t* n_dev) {
   __shared__ uint32_t s_n, s_base, s_chunks;
   n_pairs = devCount(n_pairs, n_dev);
   // block-strided (a grid smaller than the pairs, as in a recorded step, loops)
@@ -1084,6 +1084,7 @@ __global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldSta
     counts[p] = f == PF_DETECT ? r.nchunks : 0u;
     if (p == n_pairs - 1) cflag[n_pairs] = 0;
     produce[p] = 0;
+    p_vchk[p] = 0;  // the probe's count pass adds the pair's checked voxels
     work = f == PF_CARRY || f == PF_DETECT || f == PF_CACHE;
     if (work) pos = atomicAdd(&s_n, 1u);
     // work items of this pair if it runs the detection (a cache pair only on a
@@ -1316,15 +1317,13 @@ __global__ __launch_bounds__(kCacheThreads) void k_cache_replay(DevCfg cfg, Worl
 
 // The pair of every work item: wi_pair[wi_off[p] + c] = p for c < counts[p].
 __global__ void k_wi_fill(const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                          uint32_t* wi_pair, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                          const uint32_t* n_dev) {
+                          uint32_t* wi_pair, uint32_t* wi_ccnt, const uint32_t* n_dev) {
   n_pairs = devCount(n_pairs, n_dev);
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
     const uint32_t c = counts[p], b = wi_off[p];
     for (uint32_t k = 0; k < c; k++) {
       wi_pair[b + k] = p;
       wi_ccnt[b + k] = 0;
-      wi_vchk[b + k] = 0;
     }
   }
 }
@@ -1710,7 +1709,8 @@ __device__ __forceinline__ void probeRange(uint32_t total_wi, uint32_t xcd_run, 
 }
 
 // Count pass: per-lane candidate counts (lane_n), per-work-item totals
-// wi_ccnt and voxels_checked sums wi_vchk (both zeroed by k_wi_fill),
+// wi_ccnt (zeroed by k_wi_fill), each pair's voxels_checked added to p_vchk
+// (zeroed by k_pair_process),
 // wi_ccnt[total_wi] = 0, and the lanes with candidates (work item * 64 +
 // lane) on lane_list.  Most region voxels of A are solid but a work item's
 // lanes past the region or on empty voxels would idle, so the lanes are
@@ -1721,7 +1721,7 @@ __device__ __forceinline__ void probeRange(uint32_t total_wi, uint32_t xcd_run, 
 __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B, DevColliders C,
                                                      const PairRec* pr, const uint32_t* wi_pair,
                                                      const uint32_t* wi_off, uint32_t total_wi,
-                                                     uint32_t* wi_ccnt, uint32_t* wi_vchk,
+                                                     uint32_t* wi_ccnt, uint32_t* p_vchk,
                                                      uint8_t* lane_n, uint32_t* lane_list,
                                                      uint32_t* list_n, uint32_t xcd_run,
                                                      const uint32_t* n_dev) {
@@ -1763,21 +1763,22 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
       ln += (uint32_t)__popcll(hm);
       if (ln >= kWave) listFlush();
     }
-    // entries are in work-item order: one sum per work item of the batch
+    // entries are in work-item order: one candidate sum per work item of the
+    // batch; the batch is one pair's, so its checked voxels are one sum
     uint64_t rem = __ballot(act);
+    uint32_t bv = 0;
     while (rem) {
       const int f = __ffsll((long long)rem) - 1;
       const uint32_t w0 = rdlane(ew, f);
       const bool mine = act && ew == w0;
       // a work item's sums are below 64 * 27: both in one reduction
       const uint32_t s2 = waveSum(mine ? (n << 16) | vc : 0u);
-      const uint32_t sn = s2 >> 16, sv = s2 & 0xFFFFu;
-      if ((int)lane == f) {
-        if (sn) atomicAdd(wi_ccnt + w0, sn);
-        if (sv) atomicAdd(wi_vchk + w0, sv);
-      }
+      const uint32_t sn = s2 >> 16;
+      bv += s2 & 0xFFFFu;
+      if ((int)lane == f && sn) atomicAdd(wi_ccnt + w0, sn);
       rem &= ~__ballot(mine);
     }
+    if (lane == 0 && bv) atomicAdd(p_vchk + pcur, bv);
   };
   for (uint32_t wid = w_lo; wid < w_hi; wid++) {
     const uint32_t p = sload(wi_pair + wid);
@@ -1889,7 +1890,7 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
                                                const WorldState* ws, const PairRec* pr,
                                                const uint64_t* keys, uint32_t p, uint32_t f,
                                                const uint32_t* wi_off, const uint32_t* wi_cbase,
-                                               const uint32_t* wi_ccnt, const uint32_t* wi_vchk,
+                                               const uint32_t* wi_ccnt, const uint32_t* p_vchk,
                                                const Cand* cand, const DevManifold* prev,
                                                const CacheVal* cvals, CacheVal* cstage,
                                                DevManifold* staging, uint32_t* produce,
@@ -1924,9 +1925,10 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
     if (lane < ncont) mine = v.c[lane];
   } else {
     const uint32_t w0 = wi_off[p], w1 = w0 + pr[p].nchunks;
-    uint32_t vs = 0;
-    for (uint32_t wi = w0 + lane; wi < w1; wi += kWave) vs += wi_vchk[wi];
-    vchk = waveSum(vs);
+    vchk = p_vchk[p];
+    // no candidates (the work items' counts sum to the difference of their
+    // scanned offsets): no contact, nothing to reduce
+    const bool none = wi_cbase[w1] == wi_cbase[w0];
     uint32_t sidx0 = 0, sidx1 = 0, sidx2 = 0, sidx3 = 0;
     uint32_t swi0 = 0, swi1 = 0, swi2 = 0, swi3 = 0;  // work item of each slot's candidate
     float spen0 = 0.f, spen1 = 0.f, spen2 = 0.f, spen3 = 0.f;
@@ -1934,7 +1936,7 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
     uint32_t ns = 0, ncl = 0;
     // work items 64 at a time: lanes read the counts, and only the non-empty
     // items are visited, in ascending order
-    for (uint32_t wb = w0; wb < w1; wb += kWave) {
+    for (uint32_t wb = w0; !none && wb < w1; wb += kWave) {
       const uint32_t wl = wb + lane;
       const uint32_t cl = wl < w1 ? wi_ccnt[wl] : 0u, bl = wl < w1 ? wi_cbase[wl] : 0u;
       ncl += cl;
@@ -2071,7 +2073,7 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, DevColl
                                                 const uint32_t* wlist, const uint32_t* step_ctr,
                                                 const uint32_t* wi_off,
                                                 const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
-                                                const uint32_t* wi_vchk, const Cand* cand,
+                                                const uint32_t* p_vchk, const Cand* cand,
                                                 const DevManifold* prev, const CacheVal* cvals,
                                                 CacheVal* cstage, DevManifold* staging,
                                                 uint32_t* produce, RStat* rstat) {
@@ -2080,7 +2082,7 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, DevColl
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < total;
        i += nwaves) {
     const uint32_t p = wlist[i];
-    reducePairWave(cfg, B, C, ws, pr, keys, p, pr[p].flag, wi_off, wi_cbase, wi_ccnt, wi_vchk, cand,
+    reducePairWave(cfg, B, C, ws, pr, keys, p, pr[p].flag, wi_off, wi_cbase, wi_ccnt, p_vchk, cand,
                    prev, cvals, cstage, staging, produce, rstat);
   }
 }
@@ -4473,11 +4475,12 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce,
-                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
+                         uint32_t* p_vchk, uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
                          const uint32_t* n_dev) {
   if (n)
     k_pair_process<<<gridFor(n, 128, n_dev, kLoopBlocks), 128, 0, s>>>(
-        cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag, produce, wlist, step_ctr, counts, n_dev);
+        cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag, produce, p_vchk, wlist, step_ctr, counts,
+        n_dev);
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
@@ -4490,11 +4493,10 @@ void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_s
       cfg, ws, st, pr, keys, cidx, clist, counts, nw);
 }
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                    uint32_t* wi_pair, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                    const uint32_t* n_dev) {
+                    uint32_t* wi_pair, uint32_t* wi_ccnt, const uint32_t* n_dev) {
   if (n_pairs)
     k_wi_fill<<<gridFor(n_pairs, 256, n_dev, kLoopBlocks), 256, 0, s>>>(
-        counts, wi_off, n_pairs, wi_pair, wi_ccnt, wi_vchk, n_dev);
+        counts, wi_off, n_pairs, wi_pair, wi_ccnt, n_dev);
 }
 // At most four rounds of 28 waves per CU, a whole number of XCD rounds of
 // runs of `run` blocks when it can be.
@@ -4506,12 +4508,12 @@ static uint32_t probeBlocks(uint32_t items, uint32_t waves, uint32_t run) {
 void launch_probe_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
                         const uint32_t* wi_off, uint32_t total_wi, uint32_t* wi_ccnt,
-                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* lane_list,
+                        uint32_t* p_vchk, uint8_t* lane_n, uint32_t* lane_list,
                         uint32_t* list_n, ProbeShape shape, const uint32_t* n_dev) {
   if (!total_wi) return;
   const uint32_t threads = shape.threads, run = shape.xcd_run ? shape.xcd_run : 1u;
   k_probe_count<<<probeBlocks(total_wi, threads / kWave, run), threads, 0, s>>>(
-      cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt, wi_vchk, lane_n, lane_list, list_n, run,
+      cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt, p_vchk, lane_n, lane_list, list_n, run,
       n_dev);
 }
 void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
@@ -4528,13 +4530,14 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const D
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
                    const uint32_t* wlist, const uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
-                   const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
+                   const uint32_t* p_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
                    DevManifold* staging, uint32_t* produce, RStat* rstat) {
   if (!n) return;
-  uint32_t blocks = std::min<uint32_t>(2048, cdiv(n, 4));
-  k_reduce<<<blocks, 256, 0, s>>>(cfg, B, C, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase, wi_ccnt,
-                                  wi_vchk, cand, prev, cvals, cstage, staging, produce, rstat);
+  const uint32_t blocks = std::min<uint32_t>(2048, cdiv(n, 4));
+  k_reduce<<<blocks, 256, 0, s>>>(cfg, B, C, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase,
+                                  wi_ccnt, p_vchk, cand, prev, cvals, cstage, staging, produce,
+                                  rstat);
   k_reduce_tail<<<gridFor(n, 256, step_ctr, kLoopBlocks), 256, 0, s>>>(
       cfg, ws, st, pr, n, wlist, step_ctr, ckeys, cvals, cstage, rstat);
 }

@@ -70,7 +70,7 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const DevColliders& C, WorldState* ws, vxp_stats* st,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce,
-                         uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
+                         uint32_t* p_vchk, uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
                          const uint32_t* n_dev = nullptr);
 // counts[p] (the probe's work items of pair p) is set for the misses.
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
@@ -120,17 +120,17 @@ void launch_gather_flags(hipStream_t s, const uint32_t* in, const uint32_t* idx,
                          uint32_t* out);
 void launch_invert(hipStream_t s, const uint32_t* perm, uint32_t n, uint32_t* inv);
 // wi_pair[w] = the pair of work item w (after the scan of the chunk counts);
-// wi_ccnt[w] = wi_vchk[w] = 0 for the probe's count pass to add to
+// wi_ccnt[w] = 0 for the probe's count pass to add to
 void launch_wi_fill(hipStream_t s, const uint32_t* counts, const uint32_t* wi_off, uint32_t n_pairs,
-                    uint32_t* wi_pair, uint32_t* wi_ccnt, uint32_t* wi_vchk,
-                    const uint32_t* n_dev = nullptr);
+                    uint32_t* wi_pair, uint32_t* wi_ccnt, const uint32_t* n_dev = nullptr);
 // Launch shape of the probe: threads per workgroup (64, 128 or 256; each wave
 // takes a contiguous range of work items) and the run of consecutive
 // workgroups kept on one XCD.
 struct ProbeShape {
   uint32_t threads, xcd_run;
 };
-// The probe's count pass over total_wi work items (lane_n, wi_ccnt, wi_vchk;
+// The probe's count pass over total_wi work items (lane_n, wi_ccnt, and each
+// pair's voxels_checked added to p_vchk;
 // the lanes with candidates appended to lane_list as work item * 64 + lane,
 // their number at *list_n, which starts at 0; total_wi below 2^26) and its
 // write pass over the n_items entries of lane_list (*n_dev when given, at
@@ -138,7 +138,7 @@ struct ProbeShape {
 void launch_probe_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
                         const uint32_t* wi_off, uint32_t total_wi, uint32_t* wi_ccnt,
-                        uint32_t* wi_vchk, uint8_t* lane_n, uint32_t* lane_list,
+                        uint32_t* p_vchk, uint8_t* lane_n, uint32_t* lane_list,
                         uint32_t* list_n, ProbeShape shape, const uint32_t* n_dev = nullptr);
 void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         const DevColliders& C, const PairRec* pr, const uint32_t* wi_pair,
@@ -150,7 +150,7 @@ void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const D
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
                    const uint32_t* wlist, const uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
-                   const uint32_t* wi_vchk, const Cand* cand, const DevManifold* prev,
+                   const uint32_t* p_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
                    DevManifold* staging, uint32_t* produce, RStat* rstat);
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total);
