@@ -1054,7 +1054,7 @@ __device__ PairCounts pairProcess(const DevCfg& cfg, const DevBodies& B, const D
 // Also flags the cache-eligible pairs (cflag[0..n_pairs], last entry 0) for
 // their ordered compaction, clears the pair's reduction outputs and appends
 // the pairs that the reduction must visit to wlist (any order).
-__global__ void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
+__global__ __launch_bounds__(512) void k_pair_process(DevCfg cfg, DevBodies B, DevColliders C, WorldState* ws,
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
                                const DevManifold* prev, const uint8_t* drop, uint32_t* cflag,
                                uint32_t* produce, uint32_t* p_vchk, uint32_t* wlist,
@@ -4478,7 +4478,11 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          uint32_t* p_vchk, uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
                          const uint32_t* n_dev) {
   if (n)
-    k_pair_process<<<gridFor(n, 128, n_dev, kLoopBlocks), 128, 0, s>>>(
+    // 512 threads per block: each block appends its work pairs and adds its
+    // chunk bound with one device atomic per counter, and these single-address
+    // atomics serialize (128-thread blocks: 365 us per frame late in the
+    // 512-world batch, 512: 154 us)
+    k_pair_process<<<gridFor(n, 512, n_dev, kLoopBlocks), 512, 0, s>>>(
         cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag, produce, p_vchk, wlist, step_ctr, counts,
         n_dev);
 }
