@@ -876,6 +876,7 @@ static int stepAllImpl(vxp_system* s) {
   HIP_TRY(s->d_rstat.ensure((size_t)np1 * sizeof(RStat)));
   HIP_TRY(s->d_clist.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_wlist.ensure((size_t)np1 * 4));
+  HIP_TRY(s->d_coop.ensure((size_t)np1 * 4));
   HIP_TRY(s->d_cflag.ensure((size_t)(npairs + 1) * 4));
   HIP_TRY(s->d_cidx.ensure((size_t)(npairs + 1) * 4));
   HIP_TRY(s->d_pidx.ensure((size_t)np1 * 4));
@@ -997,7 +998,7 @@ static int stepAllImpl(vxp_system* s) {
                 s->d_cand.as<Cand>(), prev, s->d_cache_keys.as<CacheKeyRec>(),
                 s->d_cache_vals.as<CacheVal>(), s->d_cstage.as<CacheVal>(),
                 s->d_staging.as<DevManifold>(), s->d_produce.as<uint32_t>(),
-                s->d_rstat.as<RStat>());
+                s->d_rstat.as<RStat>(), s->d_coop.as<uint32_t>());
   profEnd(s, VXP_K_REDUCE, pe);
   pe = profBegin(s, VXP_K_COMPACT);
   if (npairs) {
@@ -1166,7 +1167,7 @@ static std::vector<DevBuf*> allBufs(vxp_system* s) {
           &s->d_cb_solid, &s->d_cb_nbr, &s->d_isl_timer, &s->d_isl_sleep, &s->d_t_nodes,
           &s->d_t_heads, &s->d_t_leaf, &s->d_t_moved, &s->d_t_cnt, &s->d_t_off, &s->d_t_pos,
           &s->d_perm, &s->d_inv, &s->d_pairs_tree, &s->d_awake, &s->d_arank, &s->d_drop,
-          &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair, &s->d_wi_list, &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code,
+          &s->d_cflag_r, &s->d_cidx_r, &s->d_wi_pair, &s->d_wi_list, &s->d_coop, &s->d_lb_ilo, &s->d_lb_ihi, &s->d_lb_code,
           &s->d_lb_child, &s->d_lb_range, &s->d_lb_iparent, &s->d_lb_lparent, &s->d_lb_flag,
           &s->d_maxw, &s->d_rsnap, &s->d_rdesc, &s->d_rsize};
 }
@@ -1247,6 +1248,7 @@ static int recEnsure(vxp_system* s) {
   HIP_TRY(s->d_rstat.ensure(P * sizeof(RStat)));
   HIP_TRY(s->d_clist.ensure(P * 4));
   HIP_TRY(s->d_wlist.ensure(P * 4));
+  HIP_TRY(s->d_coop.ensure(P * 4));
   HIP_TRY(s->d_cflag.ensure((P + 1) * 4));
   HIP_TRY(s->d_cidx.ensure((P + 1) * 4));
   HIP_TRY(s->d_pidx.ensure(P * 4));
@@ -1472,7 +1474,7 @@ static int recSeg3(vxp_system* s, const RecCtx& c) {
                 s->d_pvchk.as<uint32_t>(), s->d_cand.as<Cand>(), c.prev,
                 s->d_cache_keys.as<CacheKeyRec>(), s->d_cache_vals.as<CacheVal>(),
                 s->d_cstage.as<CacheVal>(), s->d_staging.as<DevManifold>(),
-                s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>());
+                s->d_produce.as<uint32_t>(), s->d_rstat.as<RStat>(), s->d_coop.as<uint32_t>());
   HIP_TRY(exclusive_scan_u32_n(tmp, s->d_produce.as<uint32_t>(), s->d_pidx.as<uint32_t>(), c.P,
                                c.npairs, 0, st));
   launch_world_segments(st, c.ws, c.nw, s->d_total.as<uint32_t>());

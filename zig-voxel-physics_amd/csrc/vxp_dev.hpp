@@ -20,7 +20,8 @@ constexpr uint32_t kCtrSolveUnits = 1;  // solve units queued by the solver's pr
 constexpr uint32_t kCtrSolveUnitPull = 2;  // solve units taken by workgroups
 constexpr uint32_t kCtrChunkBound = 3;  // probe work items if every cache-eligible pair missed
 constexpr uint32_t kCtrProbeItems = 4;  // probe lanes with candidates (the write pass's list)
-constexpr uint32_t kStepCounters = 5;
+constexpr uint32_t kCtrReduceCoop = 5;  // work-list pairs the reduction's waves take
+constexpr uint32_t kStepCounters = 6;
 
 // Collider header.  The brick directory is a dense grid over brick positions
 // floor(min_bound/8) .. floor(max_bound/8) (inclusive: the overlap region loop

@@ -158,6 +158,7 @@ struct vxp_system {
   bool t_host_dirty = false, t_dev_newer = false;
   DevBuf d_t_nodes, d_t_heads, d_t_leaf, d_t_moved, d_t_cnt, d_t_off, d_t_pos, d_perm, d_inv,
       d_pairs_tree, d_awake, d_arank, d_drop, d_cflag_r, d_cidx_r;
+  DevBuf d_coop;     // work-list pairs left to the reduction's waves (any order)
   DevBuf d_wi_pair;  // the pair of each probe work item
   DevBuf d_wi_list;  // probe lanes with candidates (work item * 64 + lane), in no particular order
   DevBuf s_pos, s_rot, s_lv, s_av, s_ii, s_adj_cnt, s_adj_off, s_adj, s_color, s_order, s_rec,

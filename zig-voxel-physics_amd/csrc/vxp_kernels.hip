@@ -2065,12 +2065,73 @@ __device__ __forceinline__ void reducePairWave(const DevCfg& cfg, const DevBodie
   }
 }
 
-// Waves take the pairs of the work list (built by k_pair_process; pairs not
-// on it were cleared there) one at a time in a grid-stride loop.
+// The work list (built by k_pair_process; pairs not on it were cleared
+// there), one thread per pair: a pair whose outcome needs no reduction is
+// settled here — a detection or cache miss without candidates (the sum of
+// its work items' counts, a difference of their scanned offsets, is zero),
+// a cache hit without collision — and the others are listed for k_reduce's
+// waves (one append per block).
+__global__ __launch_bounds__(256) void k_reduce_settle(DevCfg cfg, const PairRec* pr,
+                                                       const uint32_t* wlist, uint32_t* step_ctr,
+                                                       const uint32_t* wi_off,
+                                                       const uint32_t* wi_cbase,
+                                                       const uint32_t* p_vchk,
+                                                       const CacheVal* cvals, CacheVal* cstage,
+                                                       uint32_t* produce, RStat* rstat,
+                                                       uint32_t* coop, uint32_t cap) {
+  __shared__ uint32_t s_n, s_base;
+  const uint32_t total = min(step_ctr[kCtrReduceWork], cap);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    uint32_t p = 0, pos = 0;
+    bool keep = false;
+    if (i < total) {
+      p = wlist[i];
+      const PairRec& r = pr[p];
+      const uint32_t f = r.flag;
+      if (f == PF_DETECT || f == PF_MISS) {
+        const uint32_t w0 = wi_off[p], w1 = w0 + r.nchunks;
+        if (wi_cbase[w1] == wi_cbase[w0]) {
+          const uint32_t vchk = p_vchk[p];
+          produce[p] = 0;
+          rstat[p] = RStat{0u, vchk, 0u, 0u, 0u, 0u};
+          if (f == PF_MISS) {  // the cached value of a miss without contacts
+            CacheVal& cv = cstage[p];
+            cv.count = 0;
+            cv.has_collision = 0;
+            cv.vchk = vchk;
+            cv.pad = 0;
+          }
+        } else {
+          keep = true;
+        }
+      } else if (f == PF_HIT) {
+        if (!cvals[(size_t)r.world * cfg.cache_cap + r.aux].has_collision) {
+          produce[p] = 0;
+          rstat[p] = RStat{0u, 0u, 0u, 0u, 0u, 0u};
+        } else {
+          keep = true;
+        }
+      } else {
+        keep = true;
+      }
+      if (keep) pos = atomicAdd(&s_n, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_n ? atomicAdd(&step_ctr[kCtrReduceCoop], s_n) : 0u;
+    __syncthreads();
+    if (keep) coop[s_base + pos] = p;
+    __syncthreads();  // s_n and s_base are reused by the next round
+  }
+}
+
+// Waves take the listed pairs one at a time in a grid-stride loop.
 __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, DevColliders C,
                                                 const WorldState* ws,
                                                 const PairRec* pr, const uint64_t* keys,
-                                                const uint32_t* wlist, const uint32_t* step_ctr,
+                                                const uint32_t* coop, const uint32_t* step_ctr,
                                                 const uint32_t* wi_off,
                                                 const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                                                 const uint32_t* p_vchk, const Cand* cand,
@@ -2078,10 +2139,10 @@ __global__ __launch_bounds__(256) void k_reduce(DevCfg cfg, DevBodies B, DevColl
                                                 CacheVal* cstage, DevManifold* staging,
                                                 uint32_t* produce, RStat* rstat) {
   const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
-  const uint32_t total = step_ctr[kCtrReduceWork];
+  const uint32_t total = step_ctr[kCtrReduceCoop];
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < total;
        i += nwaves) {
-    const uint32_t p = wlist[i];
+    const uint32_t p = coop[i];
     reducePairWave(cfg, B, C, ws, pr, keys, p, pr[p].flag, wi_off, wi_cbase, wi_ccnt, p_vchk, cand,
                    prev, cvals, cstage, staging, produce, rstat);
   }
@@ -4532,14 +4593,17 @@ void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
-                   const uint32_t* wlist, const uint32_t* step_ctr,
+                   const uint32_t* wlist, uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* p_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, RStat* rstat) {
+                   DevManifold* staging, uint32_t* produce, RStat* rstat, uint32_t* coop) {
   if (!n) return;
+  k_reduce_settle<<<gridFor(n, 256, step_ctr, kLoopBlocks), 256, 0, s>>>(
+      cfg, pr, wlist, step_ctr, wi_off, wi_cbase, p_vchk, cvals, cstage,
+      produce, rstat, coop, n);
   const uint32_t blocks = std::min<uint32_t>(2048, cdiv(n, 4));
-  k_reduce<<<blocks, 256, 0, s>>>(cfg, B, C, ws, pr, keys, wlist, step_ctr, wi_off, wi_cbase,
+  k_reduce<<<blocks, 256, 0, s>>>(cfg, B, C, ws, pr, keys, coop, step_ctr, wi_off, wi_cbase,
                                   wi_ccnt, p_vchk, cand, prev, cvals, cstage, staging, produce,
                                   rstat);
   k_reduce_tail<<<gridFor(n, 256, step_ctr, kLoopBlocks), 256, 0, s>>>(

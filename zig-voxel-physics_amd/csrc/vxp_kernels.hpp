@@ -148,11 +148,11 @@ void launch_probe_write(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
 void launch_reduce(hipStream_t s, const DevCfg& cfg, const DevBodies& B, const DevColliders& C,
                    WorldState* ws,
                    vxp_stats* st, PairRec* pr, const uint64_t* keys, uint32_t n,
-                   const uint32_t* wlist, const uint32_t* step_ctr,
+                   const uint32_t* wlist, uint32_t* step_ctr,
                    const uint32_t* wi_off, const uint32_t* wi_cbase, const uint32_t* wi_ccnt,
                    const uint32_t* p_vchk, const Cand* cand, const DevManifold* prev,
                    const CacheKeyRec* ckeys, CacheVal* cvals, CacheVal* cstage,
-                   DevManifold* staging, uint32_t* produce, RStat* rstat);
+                   DevManifold* staging, uint32_t* produce, RStat* rstat, uint32_t* coop);
 void launch_world_segments(hipStream_t s, WorldState* ws, uint32_t nw, uint32_t* total);
 void launch_scatter(hipStream_t s, const WorldState* ws, const PairRec* pr, uint32_t n,
                     const uint32_t* produce, const uint32_t* pidx, const DevManifold* staging,
