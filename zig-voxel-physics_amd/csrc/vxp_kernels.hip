@@ -1390,15 +1390,25 @@ __device__ __forceinline__ void ctxRots(ProbeCtx& c) {
 // in-bounds B voxels.  The window spans at most two bricks per axis: one
 // directory read per brick and one read per mask plane per z slice, which
 // covers the slice's rows (bit x + 8y of the slice's word).
-template <bool kInside>
-__device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
-                                              const uint64_t* __restrict__ mask, const VoxMap& mb,
-                                              const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
-                                              uint32_t ta, V3& wa, int32_t cb[3],
-                                              uint32_t* vchk, uint32_t* inside = nullptr) {
+// A voxel (x, y, z) in world space (wa) and the B voxel around it (cb);
+// false when no probe of its neighbourhood is in B's voxel bounds.
+__device__ __forceinline__ bool probeWindow(const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
+                                            V3& wa, int32_t cb[3]) {
   wa = xformBy(c.xa, c.pa, voxelToLocal(c.vsa, x, y, z));
   V3 lb = invXformBy(c.ib, c.pb, wa);
   localToVoxel(c.vsb, lb, cb);
+  bool inb = true;
+  for (int k = 0; k < 3; k++)
+    inb = inb && max(cb[k] - 1, c.bminb[k]) <= min(cb[k] + 1, c.bmaxb[k] - 1);
+  return inb;
+}
+// The probe set of probeMask from the window around cb (in bounds).
+template <bool kInside>
+__device__ __forceinline__ uint32_t probeMaskAt(const int32_t* __restrict__ dir,
+                                                const uint64_t* __restrict__ mask,
+                                                const VoxMap& mb, const ProbeCtx& c,
+                                                const int32_t cb[3], uint32_t ta, uint32_t* vchk,
+                                                uint32_t* inside) {
   // in-bounds window of the neighbourhood per axis
   int32_t lo[3], hi[3];
   for (int k = 0; k < 3; k++) {
@@ -1406,11 +1416,6 @@ __device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
     hi[k] = min(cb[k] + 1, c.bmaxb[k] - 1);
   }
   uint32_t pm = 0, vc = 0, im = 0;
-  if (kInside) *inside = 0;
-  if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) {
-    *vchk = 0;
-    return 0;
-  }
   for (int32_t zb = lo[2] >> 3; zb <= hi[2] >> 3; zb++)
     for (int32_t yb = lo[1] >> 3; yb <= hi[1] >> 3; yb++)
       for (int32_t xb8 = lo[0] >> 3; xb8 <= hi[0] >> 3; xb8++) {
@@ -1448,6 +1453,19 @@ __device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
   if (kInside) *inside = im;
   return pm;
 }
+template <bool kInside>
+__device__ __forceinline__ uint32_t probeMask(const int32_t* __restrict__ dir,
+                                              const uint64_t* __restrict__ mask, const VoxMap& mb,
+                                              const ProbeCtx& c, int32_t x, int32_t y, int32_t z,
+                                              uint32_t ta, V3& wa, int32_t cb[3],
+                                              uint32_t* vchk, uint32_t* inside = nullptr) {
+  if (!probeWindow(c, x, y, z, wa, cb)) {
+    *vchk = 0;
+    if (kInside) *inside = 0;
+    return 0;
+  }
+  return probeMaskAt<kInside>(dir, mask, mb, c, cb, ta, vchk, inside);
+}
 // B voxel of probe k around the B voxel cb.
 __device__ __forceinline__ void probeB(const int32_t cb[3], uint32_t k, int32_t& bx, int32_t& by,
                                        int32_t& bz) {
@@ -1459,13 +1477,12 @@ __device__ __forceinline__ void probeB(const int32_t cb[3], uint32_t k, int32_t&
 // The counts of checkNeighborhood (candidates n, non-empty probed B voxels
 // vchk): the probes of probeMask that pass the distance test of
 // checkNeighborhood (src/narrow_phase.zig:831-840).
-__device__ __forceinline__ uint32_t probeCount(const int32_t* __restrict__ dir,
-                                               const uint64_t* __restrict__ mask,
-                                               const VoxMap& mb, const ProbeCtx& c, int32_t x,
-                                               int32_t y, int32_t z, uint8_t va, uint32_t* vchk) {
-  V3 wa;
-  int32_t cb[3];
-  uint32_t pm = probeMask<false>(dir, mask, mb, c, x, y, z, va & 7u, wa, cb, vchk);
+__device__ __forceinline__ uint32_t probeCountAt(const int32_t* __restrict__ dir,
+                                                 const uint64_t* __restrict__ mask,
+                                                 const VoxMap& mb, const ProbeCtx& c, V3 wa,
+                                                 const int32_t cb[3], uint8_t va,
+                                                 uint32_t* vchk) {
+  uint32_t pm = probeMaskAt<false>(dir, mask, mb, c, cb, va & 7u, vchk, nullptr);
   uint32_t n = 0;
   while (pm) {
     const uint32_t k = (uint32_t)__builtin_ctz(pm);
@@ -1708,6 +1725,8 @@ __device__ __forceinline__ void probeRange(uint32_t total_wi, uint32_t xcd_run, 
   w_hi = min(w_lo + per, total_wi);
 }
 
+constexpr uint32_t kProbeLdsWords = 12 * kWave;  // count pass LDS per wave (3 KiB)
+
 // Count pass: per-lane candidate counts (lane_n), per-work-item totals
 // wi_ccnt (zeroed by k_wi_fill), each pair's voxels_checked added to p_vchk
 // (zeroed by k_pair_process),
@@ -1725,11 +1744,19 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
                                                      uint8_t* lane_n, uint32_t* lane_list,
                                                      uint32_t* list_n, uint32_t xcd_run,
                                                      const uint32_t* n_dev) {
-  __shared__ uint32_t q_wid[4][kWave], q_lv[4][kWave], l_ent[4][2 * kWave];
+  // per wave in dynamic LDS (kProbeLdsWords words): the queue of non-empty A
+  // voxels (work item, lane | type << 8), the lanes with candidates, and the
+  // queue of voxels whose window lies in B's bounds (work item, lane | type,
+  // B voxel cb, world position wa)
+  extern __shared__ uint32_t probe_lds[];
   const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  uint32_t* const qw = q_wid[wv];
-  uint32_t* const ql = q_lv[wv];
-  uint32_t* const le = l_ent[wv];
+  uint32_t* const qw = probe_lds + wv * kProbeLdsWords;
+  uint32_t* const ql = qw + kWave;
+  uint32_t* const le = ql + kWave;            // 2 * kWave
+  uint32_t* const rw = le + 2 * kWave;
+  uint32_t* const rl = rw + kWave;
+  uint32_t* const rcb = rl + kWave;           // 3 * kWave
+  float* const rwa = reinterpret_cast<float*>(rcb + 3 * kWave);  // 3 * kWave
   // the lanes with candidates gather in le and go to lane_list once 64 are
   // there (or at the end), with one returning atomic per flush
   uint32_t ln = 0;
@@ -1745,16 +1772,18 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
   probeRange(total_wi, xcd_run, w_lo, w_hi);
   if (lane == 0 && w_lo < w_hi && w_hi == total_wi) wi_ccnt[total_wi] = 0;
   ProbePair q{};
-  uint32_t pcur = 0xFFFFFFFFu, qn = 0;
-  // probeCount on queue entries [0, cnt)
-  auto flush = [&](uint32_t cnt) {
+  uint32_t pcur = 0xFFFFFFFFu, qn = 0, rn = 0;
+  // stage 2: the neighbourhood test of the in-window entries [0, cnt)
+  auto flush2 = [&](uint32_t cnt) {
     const bool act = lane < cnt;
-    const uint32_t ew = act ? qw[lane] : 0u, el = act ? ql[lane] : 0u;
-    uint32_t n = 0, vc = 0;
+    uint32_t ew = 0, el = 0, n = 0, vc = 0;
     if (act) {
-      int32_t x, y, z;
-      probeVoxelXYZ(q, ew, el & (kWave - 1), x, y, z);
-      n = probeCount(C.dir, C.mask, voxMap(q.HB), q.c, x, y, z, (uint8_t)(el >> 8), &vc);
+      ew = rw[lane];
+      el = rl[lane];
+      const int32_t cb[3] = {(int32_t)rcb[lane], (int32_t)rcb[kWave + lane],
+                             (int32_t)rcb[2 * kWave + lane]};
+      const V3 wa{rwa[lane], rwa[kWave + lane], rwa[2 * kWave + lane]};
+      n = probeCountAt(C.dir, C.mask, voxMap(q.HB), q.c, wa, cb, (uint8_t)(el >> 8), &vc);
       lane_n[(size_t)ew * kWave + (el & (kWave - 1))] = (uint8_t)n;
     }
     const uint64_t hm = __ballot(n != 0u);
@@ -1780,11 +1809,47 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
     }
     if (lane == 0 && bv) atomicAdd(p_vchk + pcur, bv);
   };
+  // stage 1: the window of the queued non-empty voxels [0, cnt); those with
+  // no probe in B's bounds count nothing, the others move to stage 2
+  auto flush1 = [&](uint32_t cnt) {
+    const bool act = lane < cnt;
+    const uint32_t ew = act ? qw[lane] : 0u, el = act ? ql[lane] : 0u;
+    bool inb = false;
+    V3 wa{0.f, 0.f, 0.f};
+    int32_t cb[3] = {0, 0, 0};
+    if (act) {
+      int32_t x, y, z;
+      probeVoxelXYZ(q, ew, el & (kWave - 1), x, y, z);
+      inb = probeWindow(q.c, x, y, z, wa, cb);
+      if (!inb) lane_n[(size_t)ew * kWave + (el & (kWave - 1))] = 0;
+    }
+    const uint64_t m = __ballot(inb);
+    if (!m) return;
+    const uint32_t k = (uint32_t)__popcll(m);
+    const uint32_t slot = rn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    auto put = [&](uint32_t at) {
+      rw[at] = ew;
+      rl[at] = el;
+      for (int j = 0; j < 3; j++) rcb[j * kWave + at] = (uint32_t)cb[j];
+      rwa[at] = wa.x;
+      rwa[kWave + at] = wa.y;
+      rwa[2 * kWave + at] = wa.z;
+    };
+    if (inb && slot < kWave) put(slot);
+    if (rn + k < kWave) {
+      rn += k;
+      return;
+    }
+    flush2(kWave);
+    if (inb && slot >= kWave) put(slot - kWave);
+    rn = rn + k - kWave;
+  };
   for (uint32_t wid = w_lo; wid < w_hi; wid++) {
     const uint32_t p = sload(wi_pair + wid);
     if (p != pcur) {
-      if (qn) flush(qn);
-      qn = 0;
+      if (qn) flush1(qn);
+      if (rn) flush2(rn);
+      qn = rn = 0;
       pcur = p;
       loadProbePair(q, cfg, B, C, pr, wi_off, p);
     }
@@ -1806,14 +1871,15 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
       qn += k;
       continue;
     }
-    flush(kWave);
+    flush1(kWave);
     if (ne && slot >= kWave) {
       qw[slot - kWave] = wid;
       ql[slot - kWave] = ent;
     }
     qn = qn + k - kWave;
   }
-  if (qn) flush(qn);
+  if (qn) flush1(qn);
+  if (rn) flush2(rn);
   if (ln) listFlush();
 }
 
@@ -4577,7 +4643,8 @@ void launch_probe_count(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                         uint32_t* list_n, ProbeShape shape, const uint32_t* n_dev) {
   if (!total_wi) return;
   const uint32_t threads = shape.threads, run = shape.xcd_run ? shape.xcd_run : 1u;
-  k_probe_count<<<probeBlocks(total_wi, threads / kWave, run), threads, 0, s>>>(
+  k_probe_count<<<probeBlocks(total_wi, threads / kWave, run), threads,
+                  threads / kWave * kProbeLdsWords * 4, s>>>(
       cfg, B, C, pr, wi_pair, wi_off, total_wi, wi_ccnt, p_vchk, lane_n, lane_list, list_n, run,
       n_dev);
 }
