@@ -257,7 +257,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              "(worlds shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+              "(worlds are independent and would shard across GPUs without communication, "
+              "one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import vxp

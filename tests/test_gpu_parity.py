@@ -7,8 +7,8 @@ colours, warm-start impulses and effective masses), every body's pose,
 velocities and sleep state, island labels and the step statistics.  Floats are
 compared with IEEE equality (NaN == NaN), i.e. exact up to the sign of zero,
 which the reference itself leaves unspecified (Zig @min/@max lower to
-minnum/maxnum).  The north-star tolerances (1e-5 contacts, 1e-4 state) are
-therefore met with margin 0.
+minnum/maxnum).  Tolerances of 1e-5 relative on contacts and 1e-4 on body state
+are therefore met with margin 0.
 """
 import os
 
