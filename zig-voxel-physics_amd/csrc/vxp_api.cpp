@@ -909,8 +909,8 @@ static int stepAllImpl(vxp_system* s) {
   launch_pair_process(st, cfg, B, C, ws, stats, keys, npairs, pr, prev, drop,
                       s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(),
                       s->d_pvchk.as<uint32_t>(), s->d_wlist.as<uint32_t>(),
-                      s->d_step_ctr.as<uint32_t>(),
-                      s->d_counts.as<uint32_t>());
+                      s->d_step_ctr.as<uint32_t>(), s->d_counts.as<uint32_t>(),
+                      false);  // the eager step reads the work items after the scan
   profEnd(s, VXP_K_PAIR_PROCESS, pe);
   pe = profBegin(s, VXP_K_CACHE_REPLAY);
   // cache-eligible flags in the narrow phase's order, and their positions
@@ -1416,7 +1416,7 @@ static int recSeg1(vxp_system* s, const RecCtx& c) {
   launch_pair_process(st, c.cfg, c.B, c.C, c.ws, c.st, keys, c.P, s->d_pr.as<PairRec>(), c.prev,
                       nullptr, s->d_cflag.as<uint32_t>(), s->d_produce.as<uint32_t>(),
                       s->d_pvchk.as<uint32_t>(), s->d_wlist.as<uint32_t>(), ctr,
-                      s->d_counts.as<uint32_t>(), c.npairs);
+                      s->d_counts.as<uint32_t>(), true, c.npairs);
   HIP_TRY(exclusive_scan_u32_n(tmp, s->d_cflag.as<uint32_t>(), s->d_cidx.as<uint32_t>(), c.P + 1,
                                c.npairs, 1, st));
   launch_publish4(st, s->d_read + kPub1, c.npairs, c.maxw, ctr + kCtrChunkBound, c.rsize + 2, 1);

@@ -1058,7 +1058,8 @@ __global__ __launch_bounds__(512) void k_pair_process(DevCfg cfg, DevBodies B, D
                                vxp_stats* st, const uint64_t* keys, uint32_t n_pairs, PairRec* pr,
                                const DevManifold* prev, const uint8_t* drop, uint32_t* cflag,
                                uint32_t* produce, uint32_t* p_vchk, uint32_t* wlist,
-                               uint32_t* step_ctr, uint32_t* counts, const uint32_t* n_dev) {
+                               uint32_t* step_ctr, uint32_t* counts, bool chunk_bound,
+                               const uint32_t* n_dev) {
   __shared__ uint32_t s_n, s_base, s_chunks;
   n_pairs = devCount(n_pairs, n_dev);
   // block-strided (a grid smaller than the pairs, as in a recorded step, loops)
@@ -1089,7 +1090,8 @@ __global__ __launch_bounds__(512) void k_pair_process(DevCfg cfg, DevBodies B, D
     if (work) pos = atomicAdd(&s_n, 1u);
     // work items of this pair if it runs the detection (a cache pair only on a
     // miss): the step's bound on the probe's work items
-    if ((f == PF_DETECT || f == PF_CACHE) && r.nchunks) atomicAdd(&s_chunks, r.nchunks);
+    if (chunk_bound && (f == PF_DETECT || f == PF_CACHE) && r.nchunks)
+      atomicAdd(&s_chunks, r.nchunks);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -4603,7 +4605,7 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce,
                          uint32_t* p_vchk, uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
-                         const uint32_t* n_dev) {
+                         bool chunk_bound, const uint32_t* n_dev) {
   if (n)
     // 512 threads per block: each block appends its work pairs and adds its
     // chunk bound with one device atomic per counter, and these single-address
@@ -4611,7 +4613,7 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
     // 512-world batch, 512: 154 us)
     k_pair_process<<<gridFor(n, 512, n_dev, kLoopBlocks), 512, 0, s>>>(
         cfg, B, C, ws, st, keys, n, pr, prev, drop, cflag, produce, p_vchk, wlist, step_ctr, counts,
-        n_dev);
+        chunk_bound, n_dev);
 }
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,

@@ -71,7 +71,7 @@ void launch_pair_process(hipStream_t s, const DevCfg& cfg, const DevBodies& B,
                          const uint64_t* keys, uint32_t n, PairRec* pr, const DevManifold* prev,
                          const uint8_t* drop, uint32_t* cflag, uint32_t* produce,
                          uint32_t* p_vchk, uint32_t* wlist, uint32_t* step_ctr, uint32_t* counts,
-                         const uint32_t* n_dev = nullptr);
+                         bool chunk_bound, const uint32_t* n_dev = nullptr);
 // counts[p] (the probe's work items of pair p) is set for the misses.
 void launch_cache_replay(hipStream_t s, const DevCfg& cfg, WorldState* ws, vxp_stats* st,
                          PairRec* pr, CacheKeyRec* keys, const uint32_t* cflag,
