@@ -1727,7 +1727,7 @@ __device__ __forceinline__ void probeRange(uint32_t total_wi, uint32_t xcd_run, 
   w_hi = min(w_lo + per, total_wi);
 }
 
-constexpr uint32_t kProbeLdsWords = 12 * kWave;  // count pass LDS per wave (3 KiB)
+constexpr uint32_t kProbeLdsWords = 15 * kWave;  // count pass LDS per wave (3.75 KiB)
 
 // Count pass: per-lane candidate counts (lane_n), per-work-item totals
 // wi_ccnt (zeroed by k_wi_fill), each pair's voxels_checked added to p_vchk
@@ -1747,9 +1747,9 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
                                                      uint32_t* list_n, uint32_t xcd_run,
                                                      const uint32_t* n_dev) {
   // per wave in dynamic LDS (kProbeLdsWords words): the queue of non-empty A
-  // voxels (work item, lane | type << 8), the lanes with candidates, and the
-  // queue of voxels whose window lies in B's bounds (work item, lane | type,
-  // B voxel cb, world position wa)
+  // voxels (work item, lane | type << 8, and x, y, z), the lanes with
+  // candidates, and the queue of voxels whose window lies in B's bounds (work
+  // item, lane | type, B voxel cb, world position wa)
   extern __shared__ uint32_t probe_lds[];
   const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   uint32_t* const qw = probe_lds + wv * kProbeLdsWords;
@@ -1759,6 +1759,7 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
   uint32_t* const rl = rw + kWave;
   uint32_t* const rcb = rl + kWave;           // 3 * kWave
   float* const rwa = reinterpret_cast<float*>(rcb + 3 * kWave);  // 3 * kWave
+  uint32_t* const qxyz = reinterpret_cast<uint32_t*>(rwa + 3 * kWave);  // 3 * kWave: x, y, z
   // the lanes with candidates gather in le and go to lane_list once 64 are
   // there (or at the end), with one returning atomic per flush
   uint32_t ln = 0;
@@ -1820,8 +1821,8 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
     V3 wa{0.f, 0.f, 0.f};
     int32_t cb[3] = {0, 0, 0};
     if (act) {
-      int32_t x, y, z;
-      probeVoxelXYZ(q, ew, el & (kWave - 1), x, y, z);
+      const int32_t x = (int32_t)qxyz[lane], y = (int32_t)qxyz[kWave + lane],
+                    z = (int32_t)qxyz[2 * kWave + lane];
       inb = probeWindow(q.c, x, y, z, wa, cb);
       if (!inb) lane_n[(size_t)ew * kWave + (el & (kWave - 1))] = 0;
     }
@@ -1865,19 +1866,20 @@ __global__ __launch_bounds__(256, 5) void k_probe_count(DevCfg cfg, DevBodies B,
     const uint32_t k = (uint32_t)__popcll(m);
     const uint32_t slot = qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     const uint32_t ent = lane | ((uint32_t)va << 8);
-    if (ne && slot < kWave) {
-      qw[slot] = wid;
-      ql[slot] = ent;
-    }
+    auto queue = [&](uint32_t at) {
+      qw[at] = wid;
+      ql[at] = ent;
+      qxyz[at] = (uint32_t)x;
+      qxyz[kWave + at] = (uint32_t)y;
+      qxyz[2 * kWave + at] = (uint32_t)z;
+    };
+    if (ne && slot < kWave) queue(slot);
     if (qn + k < kWave) {
       qn += k;
       continue;
     }
     flush1(kWave);
-    if (ne && slot >= kWave) {
-      qw[slot - kWave] = wid;
-      ql[slot - kWave] = ent;
-    }
+    if (ne && slot >= kWave) queue(slot - kWave);
     qn = qn + k - kWave;
   }
   if (qn) flush1(qn);
